@@ -1,0 +1,166 @@
+/*
+ * twx.h -- C ABI of the MI355X (gfx950) frame-synthesis backend of the retime engine.
+ *
+ * The entry points mirror the engine's device backend trait
+ * (plugins/retime-engine/src/device/exec/backend.rs:27-46, `trait Backend`) and the per-frame
+ * render loop that drives it (crates/fx-cli/src/temporal.rs:183-250). Plain C types only: host
+ * or device pointers, sizes, POD structs. No torch, no C++ types.
+ *
+ * Conventions shared by every entry point
+ *  - Engine images are interleaved A,R,G,B, premultiplied, rows top first
+ *    (plugins/retime-engine/src/image.rs:1-14); `fmt` is the format code of image.rs:42-48
+ *    (0 = 8 bpc u8, 1 = 16 bpc u16 in 0..65535, 2 = 32 bpc f32).
+ *  - Wire frames are what `vfx` reads and writes (crates/fx-core/src/frame.rs:79-141): packed
+ *    RGBA u8, packed RGBA u16 little endian 0..65535, or planar float planes G,B,R,A.
+ *  - Flows are interleaved (dx, dy) f32 at the solver's working size (flow.rs:37-45).
+ *  - Every call blocks until its results are in place (the reference wrappers block,
+ *    device/cl.rs:20-24). One in-order HIP stream per handle; the caller serialises calls on a
+ *    handle. A nested call from a stage callback is refused with TWX_E_UNSUPPORTED
+ *    (exec/backend.rs:151-159).
+ *  - Status codes map onto StageError (device/image.rs:14-23): any non-zero status makes the
+ *    engine rerun the stage on the CPU. `err`/`errlen` receive a message (may be NULL/0).
+ */
+#ifndef TWX_H
+#define TWX_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TWX_ABI_VERSION 1
+
+enum twx_status {
+    TWX_OK = 0,
+    TWX_E_DEVICE = 1,      /* StageError::Cl (driver / runtime failure) */
+    TWX_E_NAN_OUTPUT = 2,  /* StageError::NanOutput (a NaN in a float flow) */
+    TWX_E_UNSUPPORTED = 3, /* StageError::Unsupported (settings or sizes this backend refuses) */
+    TWX_E_MISMATCH = 4     /* FxError::Mismatch (shapes or arguments that do not fit) */
+};
+
+enum twx_fmt { TWX_FMT_U8 = 0, TWX_FMT_U16 = 1, TWX_FMT_F32 = 2 };
+
+/* FlowSettings (plugins/retime-engine/src/flow.rs:83-110). */
+typedef struct twx_flow_settings {
+    int32_t quality;     /* vectors quality 1..5 (4 = Best, half resolution) */
+    int32_t hiq;         /* high quality render */
+    float sensitivity;   /* slider / 100 (0.7 by default) */
+    int32_t prep_contrast_edge, prep_delinearize, track_alpha, video_range_luma, skip_refine, wrap;
+} twx_flow_settings;
+
+/* One keyframe of a float parameter (crates/fx-core/src/curves.rs `Keyframe`): `time` in ticks. */
+typedef struct twx_key {
+    int64_t time;
+    double value;
+    int32_t hold;
+    int32_t ease;
+} twx_key;
+
+/* The parameters of one `vfx apply retime` run that the frame-synthesis path reads
+ * (plugins/retime/src/lib.rs:59-95 for ids and defaults; frontend.rs:184-243 for the derivation).
+ * Parameters not listed keep their defaults (display Output, no 360, no prep, Speed mode, no
+ * source offset, no blur compensation, no segment marks). */
+typedef struct twx_retime_params {
+    int32_t width, height, fmt;  /* frame size and wire depth code */
+    int32_t time_step;           /* ticks per output frame (also the local time step) */
+    int32_t time_scale;          /* ticks per second */
+    double fps;                  /* composition rate (cx.fps as f64) */
+    int32_t vectors;             /* "Motion Vectors" popup 1..5, default 5 (Best) */
+    int32_t frame_interp;        /* "Frame Interp" popup 1..3, default 2 (Blend) */
+    int32_t smart_blend;         /* checkbox, default 0 */
+    int32_t quality_high;        /* 1 = --quality high (default), 0 = draft */
+    double sensitivity;          /* "Motion Sensitivity" percent, default 70 */
+    int32_t in_fps_is_out_fps;   /* checkbox, default 1 */
+    double input_frame_rate;     /* slider, used when in_fps_is_out_fps == 0 */
+    double speed;                /* "Speed %" constant, used when n_speed_keys == 0 */
+    int32_t n_speed_keys;        /* keyframes of "Speed %", sorted by time, or 0 */
+    const twx_key* speed_keys;
+} twx_retime_params;
+
+/* The frame selection of one output frame (plugins/retime-engine/src/timeline.rs:359-380). */
+typedef struct twx_selection {
+    float s0, s1;
+    int32_t a, b;
+    float wa;
+    int32_t exact;
+    int32_t vectors_quality; /* popup - 1, after the searchable test and the segment override */
+    int32_t interp;          /* popup - 1 */
+} twx_selection;
+
+typedef struct twx_handle twx_handle;
+
+/* ---- device open / close / gate ---------------------------------------------------------- */
+
+/* Opens device `index` (-1 = the first device) and runs the self-check gate
+ * (device/exec/open.rs:45-78: self-check, then the program build). */
+int twx_open(int index, twx_handle** out, char* err, size_t errlen);
+void twx_close(twx_handle* h);
+/* The adversarial arithmetic self-check (device/selfcheck.rs): float ops, conversions,
+ * rounding, denormals and the order rules the kernels rely on, compared with host results. */
+int twx_selfcheck(twx_handle* h, char* err, size_t errlen);
+/* Bytes of device memory the handle may use (75 % of global memory, docs/engine.md:124). */
+uint64_t twx_memory_budget(twx_handle* h);
+
+/* ---- Backend::solve / solve_pair (exec/backend.rs:27-46, flow.rs:1328-1353) --------------- */
+
+/* Working geometry of a solve for a frame of width x height (flow.rs:118-137, prep.rs:63-146).
+ * Returns TWX_E_UNSUPPORTED for working sizes below 2x2. */
+int twx_flow_geometry(const twx_flow_settings* fs, int32_t width, int32_t height,
+                      int32_t* flow_w, int32_t* flow_h, int32_t* levels);
+
+/* flow::solve(X, Y): X(p) ~ Y(p + out(p)). Host engine images in, host flow out
+ * (flow_w * flow_h * 2 floats, from twx_flow_geometry). */
+int twx_solve(twx_handle* h, const void* x, const void* y, int32_t width, int32_t height,
+              int32_t fmt, const twx_flow_settings* fs, float* flow_out, char* err, size_t errlen);
+
+/* flow::solve_pair(A, B): F_AB = solve(A, B), F_BA = solve(B, A), pyramids built once. */
+int twx_solve_pair(twx_handle* h, const void* a, const void* b, int32_t width, int32_t height,
+                   int32_t fmt, const twx_flow_settings* fs, float* f_ab_out, float* f_ba_out,
+                   char* err, size_t errlen);
+
+/* ---- Backend::compose_vectors (compose/vector.rs:209-284) -------------------------------- */
+
+/* Warping code 0 (inverse warps with hold edges, mode 1, no 360) for interp 0 (Nearest) and
+ * 1 (Blend); wa is the weight of A (Selection::wa). For Nearest only the flow that
+ * nearest_uses_f_ab names is read (the other may be NULL). Host engine images. */
+int twx_compose_vectors(twx_handle* h, int32_t interp, float wa, const void* a, const void* b,
+                        int32_t width, int32_t height, int32_t fmt, const float* f_ab,
+                        const float* f_ba, int32_t flow_w, int32_t flow_h, void* out,
+                        char* err, size_t errlen);
+
+/* ---- the per-frame render loop of `vfx apply retime` ------------------------------------- */
+
+/* Frame selection of output frame `n` (compose.rs:76-99 and timeline.rs:447-464). */
+int twx_select(const twx_retime_params* p, int32_t n, twx_selection* out, char* err, size_t errlen);
+
+/* Renders output frames [first, first + count) of a retime run over a clip of `n_in` wire
+ * frames, as `vfx apply retime` writes them (temporal.rs:183-250: select, fetch and
+ * premultiply, rows reversed, solve_pair, compose, finish). `clip` and `out` are DEVICE
+ * pointers (the clip resident in HBM, count wire frames out). Identical frame pairs are solved
+ * once. */
+int twx_retime_frames_dev(twx_handle* h, const twx_retime_params* p, const void* clip,
+                          int32_t n_in, int32_t first, int32_t count, void* out,
+                          char* err, size_t errlen);
+
+/* The same with host pointers (uploads the clip, downloads the frames). */
+int twx_retime_frames(twx_handle* h, const twx_retime_params* p, const void* clip, int32_t n_in,
+                      int32_t first, int32_t count, void* out, char* err, size_t errlen);
+
+/* ---- counters (Exec::summary, device/exec/mod.rs:292-322) -------------------------------- */
+
+typedef struct twx_stats {
+    uint64_t solve_calls, pair_solves, pairs_reused, composes, fast_frames;
+    uint64_t pyramid_builds, pyramid_hits;
+    double device_ms;
+} twx_stats;
+int twx_get_stats(twx_handle* h, twx_stats* out);
+
+/* Device pointer of the handle's HIP stream (hipStream_t), so a caller can order its own work. */
+void* twx_stream(twx_handle* h);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* TWX_H */

@@ -1,0 +1,1501 @@
+// cpuref -- CPU restatement of the reference retime engine's CPU path (the ORACLE).
+//
+// TEST INFRASTRUCTURE ONLY. This file is the checker the HIP backend is compared with; only the
+// tests, the smoke check and the CPU-baseline leg of the benchmark load it. The product library
+// (twixtor_amd/libtwx.so) never links, loads or calls it.
+//
+// It restates, operation for operation, the Rust CPU path of baairon/twixtor:
+//   plugins/retime-engine/src/flow/conv.rs     separable convolution, border renormalisation
+//   plugins/retime-engine/src/flow/sample.rs   cvtt, cell, bilinear samplers, resample
+//   plugins/retime-engine/src/prep.rs          geometry, luma, reduce
+//   plugins/retime-engine/src/flow.rs          pyramid, solver kernels, block matching, driver
+//   plugins/retime-engine/src/warp.rs          inverse warp (hold edges), store rules
+//   plugins/retime-engine/src/blend.rs         blend, lerp, copy
+//   plugins/retime-engine/src/image.rs         premultiply / unpremultiply, wire conversions
+//   crates/fx-core/src/frame.rs                wire byte layouts, 16 bpc range mapping
+//   plugins/retime-engine/src/timeline.rs      rates, Speed time map, selection
+//   crates/fx-core/src/curves.rs, fixed.rs     keyframe sampling, 16.16 fixed point
+//   plugins/retime-engine/src/compose.rs, compose/vector.rs   fast path and vector path order
+//
+// Numerics: single precision in the order written, every operation rounded on its own
+// (compiled with -ffp-contract=off, no fast-math), the double precision steps where the
+// reference widens, and x86 truncation semantics (NaN and out-of-range give INT_MIN).
+// Rows run in parallel with OpenMP; no float sum crosses rows, so the bytes do not depend on
+// the thread count.
+//
+// Parity status per function is recorded in the test files that pin it (tests/test_oracle_*.py).
+
+#include <algorithm>
+#include <climits>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../include/twx.h"
+
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+namespace cref {
+
+// ------------------------------------------------------------------------------------------
+// Truncations (flow/sample.rs:18-25, warp.rs:104-121, image.rs:249-262, timeline.rs:43-58)
+
+static inline int32_t cvtt(float x) {
+    if (std::isnan(x) || x >= 2147483648.0f || x < -2147483648.0f) return INT32_MIN;
+    return (int32_t)x;
+}
+static inline int64_t cvtt64f(float x) {  // blend.rs:18-25
+    const float two63 = 9223372036854775808.0f;
+    if (std::isnan(x) || x >= two63 || x < -two63) return INT64_MIN;
+    return (int64_t)x;
+}
+static inline int64_t cvtt64d(double x) {  // image.rs:249-255, warp.rs:114-121
+    if (std::isnan(x) || x >= 9.223372036854775808e18 || x < -9.223372036854775808e18) return INT64_MIN;
+    return (int64_t)x;
+}
+static inline int32_t cvtt_i32d(double x) {  // timeline.rs:43-49
+    if (x > -2147483649.0 && x < 2147483648.0) return (int32_t)x;
+    return INT32_MIN;
+}
+static inline int32_t cvtt_i32f(float x) {  // timeline.rs:52-58
+    if (x > -2147483904.0f && x < 2147483648.0f) return (int32_t)x;
+    return INT32_MIN;
+}
+
+// ------------------------------------------------------------------------------------------
+// Convolution (flow/conv.rs)
+
+static const float BINOMIAL5[3] = {0.375f, 0.25f, 0.0625f};  // conv.rs:49
+static const float PAD_MULTI = -1.0f;                         // conv.rs:89
+
+// conv.rs:60-86: taps exp2(-(i*i)/d) as double pow narrowed, normalised by sum*2 - k0.
+static std::vector<float> gaussian_kernel(float r) {
+    int n = std::max((int32_t)r + 1, 0);
+    float s3 = r / 3.0f;
+    float d = s3 * s3;
+    std::vector<float> k;
+    float sum = 0.0f;
+    for (int i = 0; i < n; i++) {
+        float fi = (float)i;
+        float t = fi * fi;
+        float e = -(t / d);
+        float v = (float)std::pow(2.0, (double)e);
+        k.push_back(v);
+        sum += v;
+    }
+    if (n == 0) return k;
+    float den = sum * 2.0f - k[0];
+    for (auto& v : k) v /= den;
+    return k;
+}
+
+// conv.rs:99-141: border element with renormalisation. `get(t, &v)` returns false outside.
+// LEFT_FIRST takes the tap at i-j before the tap at i+j.
+template <int C, bool LEFT_FIRST, class Get>
+static inline float border_value(const Get& get, long i, const float* k, int n) {
+    float centre;
+    if (!get(i, centre)) centre = 0.0f;
+    float num = k[0] * centre;
+    float den = k[0];
+    for (int j = n - 1; j >= 1; j--) {
+        float kj = k[j];
+        for (int side = 0; side < 2; side++) {
+            bool right = (side == 0) != LEFT_FIRST;
+            long t = right ? i + j : i - j;
+            float xv, w;
+            float v;
+            if (C == 1) {
+                if (get(t, v)) { xv = v; w = kj; } else { xv = 0.0f; w = 0.0f; }
+            } else {
+                if (!get(t, v)) v = PAD_MULTI;
+                float mask = (v != PAD_MULTI) ? 1.0f : 0.0f;
+                xv = v;
+                w = mask * kj;
+            }
+            den += w;
+            num = num + xv * w;
+        }
+    }
+    return num / den;
+}
+
+// conv.rs:144-199: one row of the horizontal pass.
+template <int C>
+static void hrow(const float* x, float* out, long len, const float* k, int n) {
+    long head = std::min((long)n, len);
+    long interior = len > 2L * n ? len - 2L * n : 0;
+    auto border = [&](long i) {
+        for (int c = 0; c < C; c++) {
+            auto get = [&](long t, float& v) -> bool {
+                if (t < 0 || t >= len) return false;
+                v = x[t * C + c];
+                return true;
+            };
+            out[i * C + c] = border_value<C, false>(get, i, k, n);
+        }
+    };
+    for (long i = 0; i < head; i++) border(i);
+    if (interior > 0) {
+        long a = head * C, b = (head + interior) * C;
+        float k0 = k[0];
+        for (long t = a; t < b; t++) out[t] = x[t] * k0;
+        for (int j = n - 1; j >= 1; j--) {
+            long off = (long)j * C;
+            float kj = k[j];
+            for (long t = a; t < b; t++) out[t] = out[t] + (x[t + off] + x[t - off]) * kj;
+        }
+    }
+    for (long i = head + interior; i < len; i++) border(i);
+}
+
+// conv.rs:272-311: vertical pass output row y; ABOVE_FIRST takes row y-j before y+j in borders.
+template <int C, bool ABOVE_FIRST>
+static void vrow(const float* src, float* out, long y, long w, long h, const float* k, int n) {
+    long row = w * C;
+    long head = std::min((long)n, h);
+    long interior = h > 2L * n ? h - 2L * n : 0;
+    if (y >= head && y < head + interior) {
+        float k0 = k[0];
+        const float* s0 = src + y * row;
+        for (long t = 0; t < row; t++) out[t] = s0[t] * k0;
+        for (int j = n - 1; j >= 1; j--) {
+            const float* below = src + (y + j) * row;
+            const float* above = src + (y - j) * row;
+            float kj = k[j];
+            for (long t = 0; t < row; t++) out[t] = out[t] + (below[t] + above[t]) * kj;
+        }
+    } else {
+        for (long xc = 0; xc < w; xc++) {
+            for (int c = 0; c < C; c++) {
+                auto get = [&](long t, float& v) -> bool {
+                    if (t < 0 || t >= h) return false;
+                    v = src[(t * w + xc) * C + c];
+                    return true;
+                };
+                out[xc * C + c] = border_value<C, ABOVE_FIRST>(get, y, k, n);
+            }
+        }
+    }
+}
+
+template <int C, bool ABOVE_FIRST>
+static void convolve_c(float* data, long w, long h, const float* k, int n) {
+    long len = w * h * C;
+    if (len == 0 || n == 0) return;
+    std::vector<float> tmp(len);
+    long row = w * C;
+#pragma omp parallel for schedule(static)
+    for (long y = 0; y < h; y++) hrow<C>(data + y * row, tmp.data() + y * row, w, k, n);
+#pragma omp parallel for schedule(static)
+    for (long y = 0; y < h; y++) vrow<C, ABOVE_FIRST>(tmp.data(), data + y * row, y, w, h, k, n);
+}
+
+// conv.rs:584-606 convolve_above_first (the solver's and the pyramid's), and conv.rs:417-428
+// convolve (below first) for completeness.
+static void convolve(float* data, long w, long h, int c, const float* k, int n, bool above_first) {
+    if (above_first) {
+        if (c == 1) convolve_c<1, true>(data, w, h, k, n);
+        else if (c == 2) convolve_c<2, true>(data, w, h, k, n);
+        else convolve_c<3, true>(data, w, h, k, n);
+    } else {
+        if (c == 1) convolve_c<1, false>(data, w, h, k, n);
+        else if (c == 2) convolve_c<2, false>(data, w, h, k, n);
+        else convolve_c<3, false>(data, w, h, k, n);
+    }
+}
+
+// conv.rs:610-616
+static void blur_gaussian_above_first(float* data, long w, long h, int c, float r) {
+    if ((double)r < 0.001) return;
+    std::vector<float> k = gaussian_kernel(r);
+    convolve(data, w, h, c, k.data(), (int)k.size(), true);
+}
+
+// ------------------------------------------------------------------------------------------
+// Samplers (flow/sample.rs)
+
+// sample.rs:31-41
+static inline void cell(float x, long len, long& io, float& fo) {
+    int32_t i = cvtt(x);
+    float f = x - (float)i;
+    int32_t last = (int32_t)len - 1;
+    if (i >= last) { i = last - 1; f = 1.0f; }
+    long ii = i < 0 ? 0 : (long)i;
+    long cap = len >= 2 ? len - 2 : 0;
+    io = std::min(ii, cap);
+    fo = f;
+}
+
+// sample.rs:46-64; lerp_g of nanrule.rs:227: (b - a) * f + a
+static inline float bilinear1(const float* img, long w, long h, float x, float y) {
+    long ix, iy;
+    float fx, fy;
+    cell(x, w, ix, fx);
+    cell(y, h, iy, fy);
+    long i = iy * w + ix;
+    float p00 = img[i], p10 = img[i + 1], p01 = img[i + w], p11 = img[i + w + 1];
+    float top = (p10 - p00) * fx + p00;
+    float bot = (p11 - p01) * fx + p01;
+    return (bot - top) * fy + top;
+}
+
+// sample.rs:69-87
+static inline void bilinear2(const float* img, long w, long h, float x, float y, float out[2]) {
+    long ix, iy;
+    float fx, fy;
+    cell(x, w, ix, fx);
+    cell(y, h, iy, fy);
+    long i = (iy * w + ix) * 2;
+    long r = w * 2;
+    for (int c = 0; c < 2; c++) {
+        float p00 = img[i + c], p10 = img[i + 2 + c], p01 = img[i + r + c], p11 = img[i + r + 2 + c];
+        float top = (p10 - p00) * fx + p00;
+        float bot = (p11 - p01) * fx + p01;
+        out[c] = (bot - top) * fy + top;
+    }
+}
+
+// sample.rs:93-112
+static std::vector<float> resample1(const float* src, long sw, long sh, long dw, long dh) {
+    float sx = ((float)sw - 1.0f) / (float)((int32_t)dw - 1);
+    float sy = ((float)sh - 1.0f) / (float)((int32_t)dh - 1);
+    std::vector<float> dst(dw * dh, 0.0f);
+    if (dw == 0 || dh == 0) return dst;
+#pragma omp parallel for schedule(static)
+    for (long y = 0; y < dh; y++) {
+        float fy = (float)(int32_t)y * sy;
+        for (long x = 0; x < dw; x++) dst[y * dw + x] = bilinear1(src, sw, sh, (float)(int32_t)x * sx, fy);
+    }
+    return dst;
+}
+
+// ------------------------------------------------------------------------------------------
+// Geometry and luma (prep.rs)
+
+// prep.rs:63-89
+static void flow_resolution(int q, bool hiq, long w, long h, int& r, long& w0, long& h0) {
+    r = (q <= 2) ? 3 : (q == 3 ? 2 : (q == 4 ? 1 : 0));
+    if (!hiq) {
+        if (r == 0) r = 1;
+        else if (r == 1) r = 2;
+        else if (r == 2) r = 3;
+    }
+    while (r > 0 && ((w >> r) < 2 || (h >> r) < 2)) r--;
+    w0 = w;
+    h0 = h;
+    for (int i = 0; i < r; i++) { w0 /= 2; h0 /= 2; }
+}
+
+// prep.rs:116-134
+static int level_count(long w0, long h0, long min_size) {
+    int levels = 1;
+    long w = w0, h = h0;
+    for (;;) {
+        w /= 2;
+        if (w < min_size) break;
+        h /= 2;
+        if (h < min_size) break;
+        if (w > 2 && h > 2 && !(w == 2 && h == 2)) levels++;
+        else break;
+    }
+    return levels;
+}
+
+// prep.rs:137-146
+static std::vector<std::pair<long, long>> level_sizes(long w0, long h0, int levels) {
+    std::vector<std::pair<long, long>> v;
+    long w = w0, h = h0;
+    for (int i = 0; i < levels; i++) {
+        v.push_back({w, h});
+        w = std::max(w / 2, 2L);
+        h = std::max(h / 2, 2L);
+    }
+    return v;
+}
+
+// prep.rs:168-235 (track_alpha and video_range off: the retime path)
+static std::vector<float> luma(const void* img, long w, long h, int fmt) {
+    std::vector<float> out(w * h);
+    if (fmt == TWX_FMT_U8) {
+        const uint8_t* s = (const uint8_t*)img;
+#pragma omp parallel for schedule(static)
+        for (long i = 0; i < w * h; i++) {
+            const uint8_t* px = s + i * 4;
+            float r = (float)px[1], g = (float)px[2], b = (float)px[3];
+            out[i] = (r * 0.3f + g * 0.59f) + b * 0.11f;
+        }
+    } else if (fmt == TWX_FMT_U16) {
+        const uint16_t* s = (const uint16_t*)img;
+#pragma omp parallel for schedule(static)
+        for (long i = 0; i < w * h; i++) {
+            const uint16_t* px = s + i * 4;
+            double r = px[1], g = px[2], b = px[3];
+            double sum = (g * 0.59 + r * 0.3) + b * 0.11;
+            out[i] = (float)(sum / 257.0);
+        }
+    } else {
+        const float* s = (const float*)img;
+#pragma omp parallel for schedule(static)
+        for (long i = 0; i < w * h; i++) {
+            const float* px = s + i * 4;
+            double r = px[1], g = px[2], b = px[3];
+            double sum = (g * 0.59 + r * 0.3) + b * 0.11;
+            out[i] = (float)(sum * 255.0);
+        }
+    }
+    return out;
+}
+
+// ------------------------------------------------------------------------------------------
+// Flow geometry, start level (flow.rs:118-197)
+
+struct Geometry {
+    int reduction;
+    long width, height;
+    int levels;
+    std::vector<std::pair<long, long>> sizes;
+};
+
+// flow.rs:118-137. Returns a status.
+static int geometry(const twx_flow_settings& fs, long out_w, long out_h, Geometry& g, std::string& err) {
+    if (fs.quality == 0 || fs.quality > 5) { err = "flow quality outside 1..=5"; return TWX_E_MISMATCH; }
+    if (!(fs.sensitivity >= 0.0f) || !std::isfinite(fs.sensitivity)) { err = "bad sensitivity"; return TWX_E_MISMATCH; }
+    flow_resolution(fs.quality, fs.hiq != 0, out_w, out_h, g.reduction, g.width, g.height);
+    if (g.width < 2 || g.height < 2) { err = "optical flow: working size below 2x2"; return TWX_E_UNSUPPORTED; }
+    g.levels = level_count(g.width, g.height, 2);
+    g.sizes = level_sizes(g.width, g.height, g.levels);
+    return TWX_OK;
+}
+
+// flow.rs:179-197
+static void start_level(float sensitivity, int levels, int& start, float& damp) {
+    float p = (float)std::pow((double)sensitivity, 2.0);
+    float l1 = (float)std::log((double)p);
+    float l2 = (float)std::log(2.0);
+    float lv = (float)levels;
+    float x = l1 / l2 + lv;
+    if (0.0f <= x || std::isnan(x)) {
+        if (x < lv || std::isnan(x)) {
+            float f = std::floor(x);
+            int32_t s = cvtt(f);
+            long su = s < 0 ? 0 : (long)s;
+            start = (int)std::min(su, (long)levels - 1);
+            damp = x - (float)su;
+        } else {
+            start = levels - 1;
+            damp = 1.0f;
+        }
+    } else {
+        start = 0;
+        damp = 0.0f;
+    }
+}
+
+// ------------------------------------------------------------------------------------------
+// Pyramid (prep.rs:246-259, flow.rs:242-275)
+
+struct Pyramid {
+    Geometry geo;
+    std::vector<std::vector<float>> levels;
+};
+
+static void blur5(std::vector<float>& p, long w, long h) { convolve(p.data(), w, h, 1, BINOMIAL5, 3, true); }
+
+static int build_pyramid(const void* img, long w, long h, int fmt, const twx_flow_settings& fs, Pyramid& out,
+                         std::string& err) {
+    int st = geometry(fs, w, h, out.geo, err);
+    if (st) return st;
+    if (fs.prep_contrast_edge || fs.prep_delinearize || fs.wrap || fs.track_alpha || fs.video_range_luma) {
+        err = "oracle: prep, wrap, track alpha and video range luma are not restated";
+        return TWX_E_UNSUPPORTED;
+    }
+    std::vector<float> t = luma(img, w, h, fmt);
+    long tw = w, th = h;
+    for (int i = 0; i < out.geo.reduction; i++) {  // prep.rs:252-259 reduce
+        blur5(t, tw, th);
+        long nw = tw / 2, nh = th / 2;
+        t = resample1(t.data(), tw, th, nw, nh);
+        tw = nw;
+        th = nh;
+    }
+    int n = out.geo.levels;
+    std::vector<float> orig = t;
+    out.levels.clear();
+    for (int i = 0; i < n; i++) {
+        long lw = out.geo.sizes[i].first, lh = out.geo.sizes[i].second;
+        std::vector<float> tmp = orig;
+        blur5(tmp, lw, lh);
+        std::vector<float> next;
+        bool has_next = i + 1 < n;
+        if (has_next) next = resample1(tmp.data(), lw, lh, out.geo.sizes[i + 1].first, out.geo.sizes[i + 1].second);
+        std::vector<float> band = orig;
+        for (size_t q = 0; q < band.size(); q++) band[q] = band[q] - tmp[q];
+        out.levels.push_back(std::move(band));
+        if (!has_next) break;
+        orig = std::move(next);
+    }
+    return TWX_OK;
+}
+
+// ------------------------------------------------------------------------------------------
+// Solver kernels (flow.rs:281-442)
+
+static std::vector<float> calc_grad(const float* img, long w, long h) {
+    std::vector<float> g(w * h * 2, 0.0f);
+#pragma omp parallel for schedule(static)
+    for (long y = 0; y < h; y++) {
+        for (long x = 0; x < w; x++) {
+            float c = img[y * w + x];
+            float nx = 0.0f, dx = 0.0f, ny = 0.0f, dy = 0.0f;
+            if (x >= 1) { nx = (c - img[y * w + x - 1]) + 0.0f; dx = 1.0f; }
+            if (y >= 1) { ny = (c - img[(y - 1) * w + x]) + 0.0f; dy = 1.0f; }
+            if (x + 1 <= w - 1) { nx = nx + (img[y * w + x + 1] - c); dx = dx + 1.0f; }
+            if (y + 1 <= h - 1) { ny = ny + (img[(y + 1) * w + x] - c); dy = dy + 1.0f; }
+            g[(y * w + x) * 2] = nx / dx;
+            g[(y * w + x) * 2 + 1] = ny / dy;
+        }
+    }
+    return g;
+}
+
+static std::vector<float> calc_a(const std::vector<float>& grad, long w, long h) {
+    std::vector<float> a(w * h * 3);
+#pragma omp parallel for schedule(static)
+    for (long i = 0; i < w * h; i++) {
+        float gx = grad[i * 2], gy = grad[i * 2 + 1];
+        a[i * 3] = gx * gx;
+        a[i * 3 + 1] = gx * gy;
+        a[i * 3 + 2] = gy * gy;
+    }
+    return a;
+}
+
+static std::vector<float> calc_b(const float* i1, const float* i0, const float* grad, const float* flow, long w,
+                                 long h) {
+    std::vector<float> b(w * h * 2, 0.0f);
+    float fw = (float)w, fh = (float)h;
+#pragma omp parallel for schedule(static)
+    for (long y = 0; y < h; y++) {
+        float fy = (float)y;
+        for (long x = 0; x < w; x++) {
+            float fx = (float)x;
+            long i = y * w + x;
+            float sx = fx - flow[i * 2];
+            float sy = fy - flow[i * 2 + 1];
+            if (sx < 0.0f || fw - 1.0f < sx || sy < 0.0f || fh - 1.0f < sy) {
+                b[i * 2] = 0.0f;
+                b[i * 2 + 1] = 0.0f;
+            } else {
+                float s = bilinear1(i0, w, h, sx, sy);
+                float r = s - i1[i];
+                b[i * 2] = r * grad[i * 2];
+                b[i * 2 + 1] = r * grad[i * 2 + 1];
+            }
+        }
+    }
+    return b;
+}
+
+static void new_disp(float* flow, const float* a, const float* b, long n) {
+#pragma omp parallel for schedule(static)
+    for (long i = 0; i < n; i++) {
+        float aa = a[i * 3], ab = a[i * 3 + 1], ad = a[i * 3 + 2];
+        float bx = b[i * 2], by = b[i * 2 + 1];
+        float t = ab * ab - ad * aa;
+        if (std::fabs(t) > 40.0f) {
+            flow[i * 2] = (ab * by - ad * bx) / t + flow[i * 2];
+            flow[i * 2 + 1] = (ab * bx - aa * by) / t + flow[i * 2 + 1];
+        }
+    }
+}
+
+static std::vector<float> upsample_flow(const float* src, long sw, long sh, long dw, long dh) {
+    float sx = ((float)dw - 1.0f) / (float)((int32_t)sw - 1);
+    float sy = ((float)dh - 1.0f) / (float)((int32_t)sh - 1);
+    float rx = (float)(1.0 / (double)sx);
+    float ry = (float)(1.0 / (double)sy);
+    std::vector<float> dst(dw * dh * 2, 0.0f);
+#pragma omp parallel for schedule(static)
+    for (long y = 0; y < dh; y++) {
+        float fy = (float)(int32_t)y * ry;
+        for (long x = 0; x < dw; x++) {
+            float v[2];
+            bilinear2(src, sw, sh, (float)(int32_t)x * rx, fy, v);
+            dst[(y * dw + x) * 2] = v[0] * sx;
+            dst[(y * dw + x) * 2 + 1] = v[1] * sy;
+        }
+    }
+    return dst;
+}
+
+static void scale_values(std::vector<float>& d, float f) {
+    if (f == 1.0f) return;
+    for (auto& v : d) v = f * v;
+}
+
+static bool all_flow_out_of_range(const float* flow, long w, long h) {
+    float fw = (float)(w - 1), fh = (float)(h - 1);
+    for (long y = 0; y < h; y++) {
+        float fy = (float)y;
+        for (long x = 0; x < w; x++) {
+            float px = (float)(int32_t)x + flow[(y * w + x) * 2];
+            float py = fy + flow[(y * w + x) * 2 + 1];
+            if (0.0f <= px && px <= fw && 0.0f <= py && py <= fh) return false;
+        }
+    }
+    return true;
+}
+
+// ------------------------------------------------------------------------------------------
+// Block matching (flow.rs:446-622)
+
+struct Block {
+    const float* a;
+    const float* b;
+    long w, h;
+
+    inline void patch(long x, long y, float p[9]) const {
+        long i = (y - 1) * w + x - 1;
+        p[0] = a[i];
+        p[1] = a[i + 1];
+        p[2] = a[i + 2];
+        p[3] = a[i + w];
+        p[4] = a[i + w + 1];
+        p[5] = a[i + w + 2];
+        p[6] = a[i + 2 * w];
+        p[7] = a[i + 2 * w + 1];
+        p[8] = a[i + 2 * w + 2];
+    }
+
+    // flow.rs:472-524; returns false for "None".
+    inline bool cost(const float p[9], float x2, float y2, float& out) const {
+        float wm2 = (float)((int32_t)w - 2);
+        float hm2 = (float)((int32_t)h - 2);
+        if (x2 > wm2 || y2 > hm2) return false;
+        int32_t ix, iy;
+        float fx, fy;
+        if (x2 == wm2) { ix = cvtt(wm2 - 1.0f); fx = 1.0f; }
+        else { ix = cvtt(x2); fx = x2 - (float)ix; }
+        if (y2 == hm2) { iy = cvtt(hm2 - 1.0f); fy = 1.0f; }
+        else { iy = cvtt(y2); fy = y2 - (float)iy; }
+        if (ix < 1 || iy < 1) return false;
+        float w11 = fy * fx;
+        float w10 = fx - w11;
+        float w01 = fy - w11;
+        float w00 = ((1.0f - fx) - fy) + w11;
+        long base = ((long)iy - 1) * w + (long)ix - 1;
+        float sq[9];
+        int n = 0;
+        for (int bb = 0; bb < 3; bb++) {
+            for (int aa = 0; aa < 3; aa++) {
+                const float* t0 = b + base + (long)bb * w + aa;
+                const float* t1 = t0 + w;
+                float q = ((w00 * t0[0] + w10 * t0[1]) + w01 * t1[0]) + w11 * t1[1];
+                float d = p[n] - q;
+                sq[n] = d * d;
+                n++;
+            }
+        }
+        float s = sq[0] + sq[1];
+        for (int k = 2; k < 8; k++) s = s + sq[k];
+        out = sq[8] + s;
+        return true;
+    }
+
+    // flow.rs:528-544; returns true when the neighbour's vector replaces the pixel's.
+    inline bool step(long x, long y, const float cur[2], const float nb[2]) const {
+        float tx = (float)x - cur[0];
+        float ty = (float)y - cur[1];
+        float w1 = (float)(w - 1), h1 = (float)(h - 1);
+        if (0.0f <= tx && tx <= w1 && 0.0f <= ty && ty <= h1) {
+            float p[9];
+            patch(x, y, p);
+            float c0, c1;
+            if (cost(p, tx, ty, c0)) {
+                if (cost(p, (float)x - nb[0], (float)y - nb[1], c1)) {
+                    if (c1 < c0) return true;
+                }
+            }
+        }
+        return false;
+    }
+};
+
+static void block_match(const float* a, const float* b, float* flow, long w, long h) {
+    if (w < 3 || h < 3) return;
+    Block bm{a, b, w, h};
+    // sweep 1: left to right
+#pragma omp parallel for schedule(static)
+    for (long y = 1; y <= h - 2; y++) {
+        float* row = flow + y * w * 2;
+        for (long x = 1; x <= w - 2; x++) {
+            float cur[2] = {row[x * 2], row[x * 2 + 1]};
+            float nb[2] = {row[(x - 1) * 2], row[(x - 1) * 2 + 1]};
+            if (bm.step(x, y, cur, nb)) { row[x * 2] = nb[0]; row[x * 2 + 1] = nb[1]; }
+        }
+    }
+    // sweep 2: right to left
+#pragma omp parallel for schedule(static)
+    for (long y = 1; y <= h - 2; y++) {
+        float* row = flow + y * w * 2;
+        for (long x = w - 2; x >= 1; x--) {
+            float cur[2] = {row[x * 2], row[x * 2 + 1]};
+            float nb[2] = {row[(x + 1) * 2], row[(x + 1) * 2 + 1]};
+            if (bm.step(x, y, cur, nb)) { row[x * 2] = nb[0]; row[x * 2 + 1] = nb[1]; }
+        }
+    }
+    // sweeps 3 and 4: columns (the reference transposes for contiguity; the values are the same)
+#pragma omp parallel for schedule(static)
+    for (long x = 1; x <= w - 2; x++) {
+        for (long y = 1; y <= h - 2; y++) {
+            float* c = flow + (y * w + x) * 2;
+            float* n = flow + ((y - 1) * w + x) * 2;
+            float cur[2] = {c[0], c[1]};
+            float nb[2] = {n[0], n[1]};
+            if (bm.step(x, y, cur, nb)) { c[0] = nb[0]; c[1] = nb[1]; }
+        }
+    }
+#pragma omp parallel for schedule(static)
+    for (long x = 1; x <= w - 2; x++) {
+        for (long y = h - 2; y >= 1; y--) {
+            float* c = flow + (y * w + x) * 2;
+            float* n = flow + ((y + 1) * w + x) * 2;
+            float cur[2] = {c[0], c[1]};
+            float nb[2] = {n[0], n[1]};
+            if (bm.step(x, y, cur, nb)) { c[0] = nb[0]; c[1] = nb[1]; }
+        }
+    }
+}
+
+// ------------------------------------------------------------------------------------------
+// Driver (flow.rs:1258-1353)
+
+static int solve_pyramids(const Pyramid& X, const Pyramid& Y, const twx_flow_settings& fs, std::vector<float>& out,
+                          std::string& err) {
+    if (X.geo.sizes != Y.geo.sizes) { err = "pyramids of different geometry"; return TWX_E_MISMATCH; }
+    const Geometry& geo = X.geo;
+    int start;
+    float damp;
+    start_level(fs.sensitivity, geo.levels, start, damp);
+    std::vector<float> flow(geo.sizes[start].first * geo.sizes[start].second * 2, 0.0f);
+    bool check = true;
+    for (int l = start; l >= 0; l--) {
+        long w = geo.sizes[l].first, h = geo.sizes[l].second;
+        const float* img_a = X.levels[l].data();
+        const float* img_b = Y.levels[l].data();
+        std::vector<float> grad = calc_grad(img_a, w, h);
+        std::vector<float> a = calc_a(grad, w, h);
+        blur_gaussian_above_first(a.data(), w, h, 3, 7.0f);
+        for (int it = 0; it < 3; it++) {
+            std::vector<float> b = calc_b(img_a, img_b, grad.data(), flow.data(), w, h);
+            blur_gaussian_above_first(b.data(), w, h, 2, 7.0f);
+            new_disp(flow.data(), a.data(), b.data(), w * h);
+            blur_gaussian_above_first(flow.data(), w, h, 2, 5.0f);
+            if (it == 1 && l == 0 && !fs.skip_refine) block_match(img_a, img_b, flow.data(), w, h);
+        }
+        if (l != 0 && !fs.skip_refine) block_match(img_a, img_b, flow.data(), w, h);
+        if (check) {
+            if (all_flow_out_of_range(flow.data(), w, h)) std::fill(flow.begin(), flow.end(), 0.0f);
+            else check = false;
+        }
+        if (l == start && std::fabs((double)damp - 1.0) >= 1e-6) scale_values(flow, damp);
+        if (l > 0) flow = upsample_flow(flow.data(), w, h, geo.sizes[l - 1].first, geo.sizes[l - 1].second);
+    }
+    scale_values(flow, -1.0f);
+    out = std::move(flow);
+    return TWX_OK;
+}
+
+// ------------------------------------------------------------------------------------------
+// Engine images (image.rs:249-351, frame.rs:79-141)
+
+static inline int64_t q(float x) { return cvtt64d((double)x + 0.5); }  // image.rs:259-261
+static const float INV_255 = [] { uint32_t b = 0x3b808081u; float f; memcpy(&f, &b, 4); return f; }();
+static const float PREMUL_16_COLOR = [] { uint32_t b = 0x387fff00u; float f; memcpy(&f, &b, 4); return f; }();
+static const float PREMUL_16_ALPHA = [] { uint32_t b = 0x3fffff00u; float f; memcpy(&f, &b, 4); return f; }();
+static const float UNPREMUL_16_ALPHA = [] { uint32_t b = 0x3f000080u; float f; memcpy(&f, &b, 4); return f; }();
+
+static inline uint16_t u16_to_q15(uint16_t v) { return (uint16_t)(((uint32_t)v * 32768u + 32767u) / 65535u); }
+static inline uint16_t q15_to_u16(uint16_t v) {
+    uint32_t m = std::min<uint32_t>(v, 32768u);
+    return (uint16_t)((m * 65535u + 16384u) / 32768u);
+}
+
+static inline void premultiply_u8(const uint8_t p[4], uint8_t o[4]) {
+    float a = (float)p[0] * INV_255;
+    o[0] = p[0];
+    for (int c = 1; c < 4; c++) o[c] = (uint8_t)q((float)p[c] * a);
+}
+static inline void premultiply_u16(const uint16_t p[4], uint16_t o[4]) {
+    float a = (float)p[0] * PREMUL_16_COLOR;
+    o[0] = (uint16_t)q((float)p[0] * PREMUL_16_ALPHA);
+    for (int c = 1; c < 4; c++) o[c] = (uint16_t)q((float)p[c] * a);
+}
+static inline void premultiply_f32(const float p[4], float o[4]) {
+    o[0] = p[0];
+    for (int c = 1; c < 4; c++) o[c] = p[0] * p[c];
+}
+static inline void unpremultiply_u8(const uint8_t p[4], uint8_t o[4]) {
+    if (p[0] == 0) { memcpy(o, p, 4); return; }
+    float s = 255.0f / (float)p[0];
+    o[0] = p[0];
+    for (int c = 1; c < 4; c++) {
+        float v = (float)p[c] * s;
+        if (v > 255.0f) v = 255.0f;
+        o[c] = (uint8_t)q(v);
+    }
+}
+static inline void unpremultiply_u16(const uint16_t p[4], uint16_t o[4]) {
+    if (p[0] == 0) { memcpy(o, p, 8); return; }
+    float a = (float)p[0];
+    float s = 32768.0f / a;
+    o[0] = (uint16_t)q(a * UNPREMUL_16_ALPHA);
+    for (int c = 1; c < 4; c++) o[c] = (uint16_t)q((float)p[c] * s);
+}
+static inline void unpremultiply_f32(const float p[4], float o[4]) {
+    if ((double)p[0] < 1e-12) { o[0] = o[1] = o[2] = o[3] = 0.0f; return; }
+    float inv = 1.0f / p[0];
+    o[0] = p[0];
+    for (int c = 1; c < 4; c++) o[c] = p[c] * inv;
+}
+
+static inline size_t sample_size(int fmt) { return fmt == TWX_FMT_U8 ? 1 : (fmt == TWX_FMT_U16 ? 2 : 4); }
+static inline size_t image_bytes(long w, long h, int fmt) { return (size_t)w * h * 4 * sample_size(fmt); }
+
+// Image::from_frame of one wire frame (image.rs:173-203, frame.rs:79-111).
+static void from_frame(const uint8_t* wire, long w, long h, int fmt, void* engine) {
+    long n = w * h;
+    if (fmt == TWX_FMT_U8) {
+        uint8_t* o = (uint8_t*)engine;
+#pragma omp parallel for schedule(static)
+        for (long i = 0; i < n; i++) {
+            const uint8_t* c = wire + i * 4;
+            uint8_t p[4] = {c[3], c[0], c[1], c[2]};
+            premultiply_u8(p, o + i * 4);
+        }
+    } else if (fmt == TWX_FMT_U16) {
+        uint16_t* o = (uint16_t*)engine;
+#pragma omp parallel for schedule(static)
+        for (long i = 0; i < n; i++) {
+            const uint8_t* c = wire + i * 8;
+            uint16_t v[4];
+            for (int k = 0; k < 4; k++) v[k] = u16_to_q15((uint16_t)(c[2 * k] | (c[2 * k + 1] << 8)));
+            uint16_t p[4] = {v[3], v[0], v[1], v[2]};
+            premultiply_u16(p, o + i * 4);
+        }
+    } else {
+        float* o = (float*)engine;
+        const float* pl = (const float*)wire;  // planes G, B, R, A
+#pragma omp parallel for schedule(static)
+        for (long i = 0; i < n; i++) {
+            float p[4] = {pl[3 * n + i], pl[2 * n + i], pl[0 * n + i], pl[1 * n + i]};
+            premultiply_f32(p, o + i * 4);
+        }
+    }
+}
+
+// Image::to_frame (image.rs:205-245, frame.rs:113-132)
+static void to_frame(const void* engine, long w, long h, int fmt, uint8_t* wire) {
+    long n = w * h;
+    if (fmt == TWX_FMT_U8) {
+        const uint8_t* s = (const uint8_t*)engine;
+#pragma omp parallel for schedule(static)
+        for (long i = 0; i < n; i++) {
+            uint8_t o[4];
+            unpremultiply_u8(s + i * 4, o);
+            uint8_t* c = wire + i * 4;
+            c[0] = o[1]; c[1] = o[2]; c[2] = o[3]; c[3] = o[0];
+        }
+    } else if (fmt == TWX_FMT_U16) {
+        const uint16_t* s = (const uint16_t*)engine;
+#pragma omp parallel for schedule(static)
+        for (long i = 0; i < n; i++) {
+            uint16_t o[4];
+            unpremultiply_u16(s + i * 4, o);
+            uint16_t v[4] = {q15_to_u16(o[1]), q15_to_u16(o[2]), q15_to_u16(o[3]), q15_to_u16(o[0])};
+            uint8_t* c = wire + i * 8;
+            for (int k = 0; k < 4; k++) { c[2 * k] = (uint8_t)(v[k] & 0xff); c[2 * k + 1] = (uint8_t)(v[k] >> 8); }
+        }
+    } else {
+        const float* s = (const float*)engine;
+        float* pl = (float*)wire;
+#pragma omp parallel for schedule(static)
+        for (long i = 0; i < n; i++) {
+            float o[4];
+            unpremultiply_f32(s + i * 4, o);
+            // o = [A, R, G, B]; planes G, B, R, A
+            pl[0 * n + i] = o[2];
+            pl[1 * n + i] = o[3];
+            pl[2 * n + i] = o[1];
+            pl[3 * n + i] = o[0];
+        }
+    }
+}
+
+// Image::rows_reversed (image.rs:96-114)
+static void rows_reversed(const void* src, long w, long h, int fmt, void* dst) {
+    size_t row = (size_t)w * 4 * sample_size(fmt);
+    for (long y = 0; y < h; y++) memcpy((uint8_t*)dst + y * row, (const uint8_t*)src + (h - 1 - y) * row, row);
+}
+
+// ------------------------------------------------------------------------------------------
+// Inverse warp, hold edges (warp.rs:99-786)
+
+static inline uint8_t sat_u8(float v) {
+    int32_t i = cvtt(v + 0.5f);
+    int32_t t = (int32_t)((uint32_t)i + (uint32_t)-128);
+    int32_t w = std::min(std::max(t, -32768), 32767);
+    int32_t b = std::min(std::max(w, -128), 127);
+    return (uint8_t)(b + 128);
+}
+static inline uint16_t sat_u16(float v) {
+    int32_t i = cvtt(v + 0.5f);
+    int32_t t = (int32_t)((uint32_t)i + (uint32_t)-32768);
+    int32_t w = std::min(std::max(t, -32768), 32767);
+    return (uint16_t)(w + 32768);
+}
+
+// warp.rs:178-215
+static bool clip_to_rect(float x, float y, float& ix, float& iy, float w1, float h1) {
+    for (uint32_t n = 0; n < 4096; n++) {
+        if (ix < 0.0f) {
+            float dy = y - iy, dxs = x - ix, num = x - 0.0f;
+            ix = 0.0f;
+            iy = y - (num / dxs) * dy;
+            continue;
+        }
+        if (ix > w1) {
+            float dy = y - iy, dxs = x - ix, num = x - w1;
+            ix = w1;
+            iy = y - (num / dxs) * dy;
+            continue;
+        }
+        if (iy < 0.0f) {
+            float dyy = y - iy, num = y - 0.0f, dx = x - ix;
+            ix = x - (num / dyy) * dx;
+            iy = 0.0f;
+            continue;
+        }
+        if (iy > h1) {
+            float dyy = y - iy, num = y - h1, dx = x - ix;
+            ix = x - (num / dyy) * dx;
+            iy = h1;
+            continue;
+        }
+        return true;
+    }
+    return false;
+}
+
+template <class P> static inline float to_f(P v) { return (float)v; }
+template <class P> static inline P store(float v);
+template <> inline uint8_t store<uint8_t>(float v) { return sat_u8(v); }
+template <> inline uint16_t store<uint16_t>(float v) { return sat_u16(v); }
+template <> inline float store<float>(float v) { return v; }
+
+// warp.rs:306-360 bilerp4 + mix4 (lerp_base_first: a + (b - a) * f, the same value as (b - a) * f + a)
+template <class P>
+static inline void bilerp4(const P* src, long w, int32_t w1, int32_t h1, float x, float y, float out[4]) {
+    int32_t ix = cvtt(x), iy = cvtt(y);
+    float fx = x - (float)ix, fy = y - (float)iy;
+    if (ix >= w1) { ix = w1 - 1; fx = 1.0f; }
+    if (iy >= h1) { iy = h1 - 1; fy = 1.0f; }
+    long i00 = ((long)iy * w + ix) * 4;
+    long i01 = i00 + w * 4;
+    for (int k = 0; k < 4; k++) {
+        float p00 = to_f(src[i00 + k]), p10 = to_f(src[i00 + 4 + k]);
+        float p01 = to_f(src[i01 + k]), p11 = to_f(src[i01 + 4 + k]);
+        float top = p00 + (p10 - p00) * fx;
+        float bot = p01 + (p11 - p01) * fx;
+        out[k] = top + (bot - top) * fy;
+    }
+}
+
+// warp.rs:415-442 Flow::bilerp
+static inline void flow_bilerp(const float* d, long fw, int32_t fw1, int32_t fh1, float x, float y, float out[2]) {
+    int32_t ix = cvtt(x), iy = cvtt(y);
+    float fx = x - (float)ix, fy = y - (float)iy;
+    if (ix >= fw1) { ix = fw1 - 1; fx = 1.0f; }
+    if (iy >= fh1) { iy = fh1 - 1; fy = 1.0f; }
+    long i00 = ((long)iy * fw + ix) * 2;
+    long i01 = i00 + fw * 2;
+    for (int k = 0; k < 2; k++) {
+        float p00 = d[i00 + k], p10 = d[i00 + 2 + k], p01 = d[i01 + k], p11 = d[i01 + 2 + k];
+        float top = (p10 - p00) * fx + p00;
+        float bot = (p11 - p01) * fx + p01;
+        out[k] = (bot - top) * fy + top;
+    }
+}
+
+template <class P>
+static bool warp_run(P* out, const P* src, const float* flow, long fw, long fh, float amount, long w, long h) {
+    int32_t w1i = (int32_t)w - 1, h1i = (int32_t)h - 1;
+    float rx = (float)((int32_t)fw - 1) / (float)w1i;
+    float ry = (float)((int32_t)fh - 1) / (float)h1i;
+    float sx = amount / rx, sy = amount / ry;
+    float w1 = (float)w1i, h1 = (float)h1i;
+    int32_t fw1 = (int32_t)fw - 1, fh1 = (int32_t)fh - 1;
+    bool bad = false;
+#pragma omp parallel for schedule(static) reduction(|| : bad)
+    for (long y = 0; y < h; y++) {
+        float yf = (float)y;
+        float fy = yf * ry;
+        for (long x = 0; x < w; x++) {
+            float xf = (float)x;
+            float d[2];
+            flow_bilerp(flow, fw, fw1, fh1, xf * rx, fy, d);
+            float ix = d[0] * sx + xf;
+            float iy = d[1] * sy + yf;
+            P* px = out + (y * w + x) * 4;
+            if (!clip_to_rect(xf, yf, ix, iy, w1, h1) || std::isnan(ix) || std::isnan(iy)) {
+                bad = true;
+                for (int k = 0; k < 4; k++) px[k] = (P)0;
+                continue;
+            }
+            float s[4];
+            bilerp4(src, w, w1i, h1i, ix, iy, s);
+            for (int k = 0; k < 4; k++) px[k] = store<P>(s[k]);
+        }
+    }
+    return !bad;
+}
+
+// warp.rs:693-786 (hold = true, mode 1, is360 = false)
+static int inv_warp(void* out, const void* src, long w, long h, int fmt, const float* flow, long fw, long fh,
+                    float amount, std::string& err) {
+    if (w < 2 || h < 2 || fw < 2 || fh < 2) { err = "inverse warp needs images and a flow of at least 2x2"; return TWX_E_MISMATCH; }
+    float a = amount < 0.0f ? -amount : amount;
+    bool copy_only = 0.005 > (double)a;
+    if (!copy_only) {
+        float r = (float)((int32_t)fw - 1) / (float)((int32_t)w - 1);
+        float qv = (float)((double)amount / (double)r);
+        if (qv < 0.0f) qv = -qv;
+        copy_only = 0.0001 > (double)qv;
+    }
+    if (copy_only) { memcpy(out, src, image_bytes(w, h, fmt)); return TWX_OK; }
+    bool ok;
+    if (fmt == TWX_FMT_U8) ok = warp_run<uint8_t>((uint8_t*)out, (const uint8_t*)src, flow, fw, fh, amount, w, h);
+    else if (fmt == TWX_FMT_U16) ok = warp_run<uint16_t>((uint16_t*)out, (const uint16_t*)src, flow, fw, fh, amount, w, h);
+    else ok = warp_run<float>((float*)out, (const float*)src, flow, fw, fh, amount, w, h);
+    if (!ok) { err = "inverse warp: a sample coordinate is NaN or never converges, which is not supported"; return TWX_E_UNSUPPORTED; }
+    return TWX_OK;
+}
+
+// ------------------------------------------------------------------------------------------
+// Blend, lerp, copy (blend.rs:55-257)
+
+template <class T>
+static void lerp_int(T* f, const T* s, float w, float scale) {
+    auto diff = [&](int i) { return (float)((int32_t)f[i] - (int32_t)s[i]); };
+    auto sf = [&](int i) { return (float)(int32_t)s[i]; };
+    float a = diff(0) * w + sf(0);
+    float n = a / scale;
+    if (!(n > 0.0f)) { for (int k = 0; k < 4; k++) f[k] = (T)cvtt64f(0.0f); return; }
+    float r = diff(1) * w + sf(1);
+    float g = diff(2) * w + sf(2);
+    float b = diff(3) * w + sf(3);
+    if (n > 1.0f) { r = r / n; g = g / n; b = b / n; a = scale; }
+    auto clamp = [&](float c) { return c > a ? a : (0.0f > c ? 0.0f : c); };
+    r = clamp(r); g = clamp(g); b = clamp(b);
+    f[1] = (T)cvtt64f(r + 0.5f);
+    f[2] = (T)cvtt64f(g + 0.5f);
+    f[3] = (T)cvtt64f(b + 0.5f);
+    f[0] = (T)cvtt64f(a + 0.5f);
+}
+static void lerp_f32(float* f, const float* s, float w) {
+    float a = (f[0] - s[0]) * w + s[0];
+    if (!(a > 0.0f)) { f[0] = f[1] = f[2] = f[3] = 0.0f; return; }
+    float r = (f[1] - s[1]) * w + s[1];
+    float g = (f[2] - s[2]) * w + s[2];
+    float b = (f[3] - s[3]) * w + s[3];
+    if (a > 1.0f) { r = r / a; g = g / a; b = b / a; a = 1.0f; }
+    f[1] = r + 0.0f; f[2] = g + 0.0f; f[3] = b + 0.0f; f[0] = a + 0.0f;
+}
+
+// blend.rs:145-178
+static void lerp(void* first, const void* second, long w, long h, int fmt, float wt) {
+    if (wt == 1.0f) return;
+    if (wt == 0.0f) { memcpy(first, second, image_bytes(w, h, fmt)); return; }
+    long n = w * h;
+#pragma omp parallel for schedule(static)
+    for (long i = 0; i < n; i++) {
+        if (fmt == TWX_FMT_U8) lerp_int<uint8_t>((uint8_t*)first + i * 4, (const uint8_t*)second + i * 4, wt, 255.0f);
+        else if (fmt == TWX_FMT_U16) lerp_int<uint16_t>((uint16_t*)first + i * 4, (const uint16_t*)second + i * 4, wt, 65535.0f);
+        else lerp_f32((float*)first + i * 4, (const float*)second + i * 4, wt);
+    }
+}
+
+// blend.rs:55-114
+static void blend(void* out, const void* a, const void* b, long w, long h, int fmt, float wt) {
+    size_t bytes = image_bytes(w, h, fmt);
+    if (0.005 > (double)std::fabs(wt)) { memcpy(out, a, bytes); return; }
+    if (0.005 > std::fabs((double)wt - 1.0)) { memcpy(out, b, bytes); return; }
+    if (wt < 0.0f || wt > 1.0f) {
+        memcpy(out, a, bytes);
+        lerp(out, b, w, h, fmt, (float)(1.0 - (double)wt));
+        return;
+    }
+    long n = w * h * 4;
+    if (fmt == TWX_FMT_U8) {
+        uint8_t* o = (uint8_t*)out;
+        const uint8_t *pa = (const uint8_t*)a, *pb = (const uint8_t*)b;
+#pragma omp parallel for schedule(static)
+        for (long i = 0; i < n; i++) {
+            float af = pa[i], bf = pb[i];
+            o[i] = (uint8_t)cvtt64f(((bf - af) * wt + af) + 0.5f);
+        }
+    } else if (fmt == TWX_FMT_U16) {
+        uint16_t* o = (uint16_t*)out;
+        const uint16_t *pa = (const uint16_t*)a, *pb = (const uint16_t*)b;
+#pragma omp parallel for schedule(static)
+        for (long i = 0; i < n; i++) {
+            float af = pa[i], bf = pb[i];
+            o[i] = (uint16_t)cvtt64f(((bf - af) * wt + af) + 0.5f);
+        }
+    } else {
+        float* o = (float*)out;
+        const float *pa = (const float*)a, *pb = (const float*)b;
+#pragma omp parallel for schedule(static)
+        for (long i = 0; i < n; i++) o[i] = ((pb[i] - pa[i]) * wt + pa[i]) + 0.0f;
+    }
+}
+
+// ------------------------------------------------------------------------------------------
+// Time model (timeline.rs, curves.rs, fixed.rs, frontend.rs)
+
+static inline int32_t float_to_fixed(double f) {  // fixed.rs:8-10 (Rust `as i32` saturates, NaN 0)
+    double v = f * 65536.0 + (f < 0.0 ? -0.5 : 0.5);
+    if (std::isnan(v)) return 0;
+    if (v >= 2147483647.0) return INT32_MAX;
+    if (v <= -2147483648.0) return INT32_MIN;
+    return (int32_t)v;
+}
+static inline float fixed_reader(int32_t fixed, float scale) {  // frontend.rs:130-132
+    return (float)((double)fixed * (1.0 / 65536.0)) * scale;
+}
+
+// curves.rs:537-544
+static inline double eased_progress(double u, bool es, bool ee) {
+    if (!es && !ee) return u;
+    if (es && ee) return u * u * (3.0 - 2.0 * u);
+    if (es) return u * u * (2.0 - u);
+    return u * (1.0 + u * (1.0 - u));
+}
+
+// curves.rs:344-370 for a float parameter (no Bezier handles)
+static double speed_at(const twx_retime_params& p, int64_t time) {
+    if (p.n_speed_keys <= 0) return p.speed;
+    const twx_key* k = p.speed_keys;
+    int last = p.n_speed_keys - 1;
+    if (time <= k[0].time) return k[0].value;
+    if (time >= k[last].time) return k[last].value;
+    int i = 0;
+    while (i + 1 <= last && k[i + 1].time <= time) i++;
+    const twx_key& a = k[i];
+    const twx_key& b = k[i + 1];
+    if (a.hold) return a.value;
+    double f = eased_progress((double)(time - a.time) / (double)(b.time - a.time), a.ease != 0, b.ease != 0);
+    return a.value + (b.value - a.value) * f;
+}
+
+static inline int32_t speed_fixed(const twx_retime_params& p, int32_t time) { return float_to_fixed(speed_at(p, time)); }
+
+static double snap(double rate) {  // timeline.rs:69-76
+    const double t[3] = {29.97, 59.94, 23.976};
+    for (double c : t)
+        if ((double)std::fabs((float)(rate - c)) < 2e-5) rate = c;
+    return rate;
+}
+
+struct Rates {
+    double in_fps, out_fps;
+    float dur;
+    int32_t lts;
+};
+
+static int make_rates(const twx_retime_params& p, Rates& r, std::string& err) {  // timeline.rs:108-133
+    double in_fps = (double)((float)p.input_frame_rate * 1.0f);
+    double out_fps = p.fps;
+    if (p.in_fps_is_out_fps) { in_fps = 30.0; out_fps = 30.0; }
+    in_fps = snap(in_fps);
+    out_fps = snap(out_fps);
+    if (0.01 > out_fps || 0.01 >= in_fps) { err = "error 5 (bad parameter): frame rates"; return TWX_E_MISMATCH; }
+    r.in_fps = in_fps;
+    r.out_fps = out_fps;
+    r.lts = p.time_step;
+    r.dur = (float)(((double)p.time_step * out_fps) / in_fps);
+    return TWX_OK;
+}
+
+// timeline.rs:280-331 (Speed mode, src_time_offset 0 so base stays 0)
+static int src_pos(const twx_retime_params& p, const Rates& r, float t, float& out_pos, std::string& err) {
+    float lts_f = (float)r.lts;
+    float out = 0.0f;
+    int32_t n2 = cvtt_i32f(t * 2.0f);
+    if (n2 >= (1 << 24)) { err = "output position beyond 2^23 frames is not supported"; return TWX_E_UNSUPPORTED; }
+    float k = 1.0f;
+    while ((float)n2 >= k) {
+        int32_t time = cvtt_i32d((double)((lts_f * k) / 2.0f) + 0.05);
+        float sp = (float)((double)speed_fixed(p, time) / 6553600.0);
+        out = (sp / 2.0f) + out;
+        k += 1.0f;
+    }
+    float tt = t * 2.0f;
+    float rem = tt - (k + -1.0f);
+    if ((double)rem > 0.0001) {
+        int32_t time = cvtt_i32d((double)((tt * lts_f) / 2.0f) + 0.05);
+        float sp = (float)((double)speed_fixed(p, time) / 6553600.0);
+        out = ((rem * sp) / 2.0f) + out;
+    }
+    out_pos = (float)((r.in_fps / r.out_fps) * (double)out);
+    return TWX_OK;
+}
+
+// timeline.rs:342-350 frame_time with base 0
+static int32_t frame_time(const Rates& r, float f) {
+    float d = f - 0.0f;
+    if (std::fabs(r.in_fps - r.out_fps) < 0.05) return cvtt_i32d((double)d * (double)r.dur + 0.5);
+    return cvtt_i32d((double)(d * r.dur) + (double)r.dur * 0.05);
+}
+
+// timeline.rs:397-409
+static bool exact_position(float s0, float s1, float blur) {
+    bool exact = true;
+    float frac = s0 - std::floor(s0);
+    if ((double)std::fabs(frac) > 0.005 && (double)std::fabs((float)((double)frac - 1.0)) > 0.005) exact = false;
+    if ((double)std::fabs(s0 - s1) > 1e-6 && (double)blur > 0.005) exact = false;
+    return exact;
+}
+
+static int select_frame(const twx_retime_params& p, int32_t n, twx_selection& sel, Rates& rates, std::string& err) {
+    int st = make_rates(p, rates, err);
+    if (st) return st;
+    int32_t cur = n * p.time_step;
+    // timeline.rs:137-142 frame_positions
+    float curf = (float)cur, ltsf = (float)p.time_step;
+    float t0 = curf / ltsf;
+    float t1 = ((float)p.time_step + curf) / ltsf;
+    float s0, s1;
+    if ((st = src_pos(p, rates, t0, s0, err))) return st;
+    if ((st = src_pos(p, rates, t1, s1, err))) return st;
+    // frontend.rs:215-218 and prep.rs:106-112: popups minus one, the tiny frames rule
+    int vq = p.vectors - 1;
+    if (!(p.width >= 2 && p.height >= 2)) vq = 0;
+    // the segment marks are at their default (popup 1 = Cut A): marker 0 everywhere, no override
+    int32_t a = cvtt_i32f(s0);
+    float frac = s0 - (float)a;
+    sel.s0 = s0;
+    sel.s1 = s1;
+    sel.a = a;
+    sel.b = (int32_t)((uint32_t)a + 1u);
+    sel.wa = 1.0f - frac;
+    sel.exact = exact_position(s0, s1, 0.0f) ? 1 : 0;
+    sel.vectors_quality = vq;
+    sel.interp = p.frame_interp - 1;
+    return TWX_OK;
+}
+
+// ------------------------------------------------------------------------------------------
+// The per-frame render (compose.rs:93-246, compose/vector.rs:163-321, frontend.rs:290-312)
+
+struct Clip {
+    const uint8_t* wire;
+    int32_t n;
+    long w, h;
+    int fmt;
+};
+
+// host.rs main_frame: the clip frame at frame_time(position), transparent outside the clip
+// (clip.rs:170-172 frame_at gives None; host.rs:123-127 transparent image).
+static std::vector<uint8_t> main_frame(const Clip& c, const Rates& r, int32_t lts, int32_t pos) {
+    std::vector<uint8_t> img(image_bytes(c.w, c.h, c.fmt), 0);
+    int32_t time = frame_time(r, (float)pos);
+    int64_t idx = (int64_t)std::floor((double)time / (double)lts);  // TimeBase::frame_of, div_euclid
+    if (idx >= 0 && idx < c.n) {
+        size_t fb = image_bytes(c.w, c.h, c.fmt);
+        from_frame(c.wire + (size_t)idx * fb, c.w, c.h, c.fmt, img.data());
+    }
+    return img;
+}
+
+static int render_frame(const twx_retime_params& p, const Clip& c, int32_t n, uint8_t* out_wire, std::string& err) {
+    if (p.smart_blend) { err = "oracle: Smart Blend is not restated"; return TWX_E_UNSUPPORTED; }
+    twx_selection sel;
+    Rates rates;
+    int st = select_frame(p, n, sel, rates, err);
+    if (st) return st;
+    long w = c.w, h = c.h;
+    int fmt = c.fmt;
+    size_t bytes = image_bytes(w, h, fmt);
+    std::vector<uint8_t> out(bytes, 0);
+    if (sel.vectors_quality != 0 && !sel.exact) {
+        // vector path (compose/vector.rs:289-314)
+        if (!(sel.interp >= 0 && sel.interp <= 2)) { err = "frame_interp outside 1..3"; return TWX_E_MISMATCH; }
+        if (sel.interp == 2) { err = "oracle: Weighted Blend on the vector path is not restated"; return TWX_E_UNSUPPORTED; }
+        std::vector<uint8_t> a0 = main_frame(c, rates, p.time_step, sel.a);
+        std::vector<uint8_t> b0 = main_frame(c, rates, p.time_step, sel.b);
+        std::vector<uint8_t> a(bytes), b(bytes);
+        rows_reversed(a0.data(), w, h, fmt, a.data());
+        rows_reversed(b0.data(), w, h, fmt, b.data());
+        twx_flow_settings fs{};
+        fs.quality = sel.vectors_quality;
+        fs.hiq = p.quality_high;
+        fs.sensitivity = fixed_reader(float_to_fixed(p.sensitivity), 1.0f) / 100.0f;
+        Pyramid pa, pb;
+        if ((st = build_pyramid(a.data(), w, h, fmt, fs, pa, err))) return st;
+        if ((st = build_pyramid(b.data(), w, h, fmt, fs, pb, err))) return st;
+        long fw = pa.geo.width, fh = pa.geo.height;
+        float wa = sel.wa;
+        float frac = 1.0f - wa;
+        std::vector<uint8_t> composed(bytes, 0);
+        if (sel.interp == 0) {
+            bool use_ab = wa < 0.5f;  // nearest_uses_f_ab with warping code 0
+            std::vector<float> flow;
+            if (use_ab) { if ((st = solve_pyramids(pa, pb, fs, flow, err))) return st; }
+            else { if ((st = solve_pyramids(pb, pa, fs, flow, err))) return st; }
+            if (0.5f <= wa) st = inv_warp(composed.data(), a.data(), w, h, fmt, flow.data(), fw, fh, frac, err);
+            else st = inv_warp(composed.data(), b.data(), w, h, fmt, flow.data(), fw, fh, wa, err);
+            if (st) return st;
+        } else {
+            std::vector<float> f_ba, f_ab;
+            if ((st = solve_pyramids(pb, pa, fs, f_ba, err))) return st;
+            if ((st = solve_pyramids(pa, pb, fs, f_ab, err))) return st;
+            std::vector<uint8_t> ta(bytes, 0), tb(bytes, 0);
+            if ((st = inv_warp(ta.data(), a.data(), w, h, fmt, f_ba.data(), fw, fh, frac, err))) return st;
+            if ((st = inv_warp(tb.data(), b.data(), w, h, fmt, f_ab.data(), fw, fh, wa, err))) return st;
+            blend(composed.data(), ta.data(), tb.data(), w, h, fmt, frac);
+        }
+        rows_reversed(composed.data(), w, h, fmt, out.data());
+    } else {
+        // fast path (compose.rs:214-246)
+        std::vector<uint8_t> a = main_frame(c, rates, p.time_step, sel.a);
+        std::vector<uint8_t> b = main_frame(c, rates, p.time_step, sel.b);
+        if (sel.interp == 0) {
+            memcpy(out.data(), sel.wa >= 0.5f ? a.data() : b.data(), bytes);
+        } else if (sel.interp == 1) {
+            blend(out.data(), a.data(), b.data(), w, h, fmt, (float)(1.0 - (double)sel.wa));
+        } else if (sel.interp == 2) {
+            // weighted_blend_noflow (blend.rs:280-300): the plain Blend with 1 - wa in f32, and for
+            // float images the inputs plus 0.0 (finite inputs of magnitude up to 1e30)
+            float t = 1.0f - sel.wa;
+            if (fmt == TWX_FMT_F32) {
+                float* fa = (float*)a.data();
+                float* fb = (float*)b.data();
+                for (long i = 0; i < w * h * 4; i++) {
+                    if (!std::isfinite(fa[i]) || !std::isfinite(fb[i]) || std::fabs(fa[i]) > 1e30f || std::fabs(fb[i]) > 1e30f) {
+                        err = "oracle: non-finite float input to the weighted blend is not restated";
+                        return TWX_E_UNSUPPORTED;
+                    }
+                    fa[i] = fa[i] + 0.0f;
+                    fb[i] = fb[i] + 0.0f;
+                }
+            }
+            blend(out.data(), a.data(), b.data(), w, h, fmt, t);
+        } else {
+            err = "frame_interp outside 1..3";
+            return TWX_E_MISMATCH;
+        }
+    }
+    to_frame(out.data(), w, h, fmt, out_wire);
+    return TWX_OK;
+}
+
+}  // namespace cref
+
+// ==========================================================================================
+// C ABI for the tests and the CPU-baseline leg of the benchmark (ctypes)
+
+using namespace cref;
+
+static void set_err(char* err, size_t errlen, const std::string& m) {
+    if (err && errlen) {
+        size_t n = std::min(errlen - 1, m.size());
+        memcpy(err, m.data(), n);
+        err[n] = 0;
+    }
+}
+
+extern "C" {
+
+int cref_set_threads(int n) {
+#ifdef _OPENMP
+    if (n > 0) omp_set_num_threads(n);
+    return omp_get_max_threads();
+#else
+    (void)n;
+    return 1;
+#endif
+}
+
+int cref_gaussian_kernel(float r, float* out, int cap) {
+    std::vector<float> k = gaussian_kernel(r);
+    for (int i = 0; i < (int)k.size() && i < cap; i++) out[i] = k[i];
+    return (int)k.size();
+}
+
+void cref_convolve(float* data, int w, int h, int c, const float* k, int n, int above_first) {
+    convolve(data, w, h, c, k, n, above_first != 0);
+}
+
+void cref_resample1(const float* src, int sw, int sh, int dw, int dh, float* dst) {
+    std::vector<float> d = resample1(src, sw, sh, dw, dh);
+    memcpy(dst, d.data(), d.size() * 4);
+}
+
+float cref_bilinear1(const float* img, int w, int h, float x, float y) { return bilinear1(img, w, h, x, y); }
+
+void cref_luma(const void* img, int w, int h, int fmt, float* out) {
+    std::vector<float> l = luma(img, w, h, fmt);
+    memcpy(out, l.data(), l.size() * 4);
+}
+
+int cref_geometry(const twx_flow_settings* fs, int w, int h, int* fw, int* fh, int* levels, int* sizes_wh) {
+    Geometry g;
+    std::string e;
+    int st = geometry(*fs, w, h, g, e);
+    if (st) return st;
+    *fw = (int)g.width;
+    *fh = (int)g.height;
+    *levels = g.levels;
+    if (sizes_wh)
+        for (int i = 0; i < g.levels; i++) { sizes_wh[2 * i] = (int)g.sizes[i].first; sizes_wh[2 * i + 1] = (int)g.sizes[i].second; }
+    return TWX_OK;
+}
+
+void cref_start_level(float sensitivity, int levels, int* start, float* damp) { start_level(sensitivity, levels, *start, *damp); }
+
+// The band-pass levels of one engine image, concatenated finest first.
+int cref_build_pyramid(const void* img, int w, int h, int fmt, const twx_flow_settings* fs, float* out, char* err, size_t errlen) {
+    Pyramid p;
+    std::string e;
+    int st = build_pyramid(img, w, h, fmt, *fs, p, e);
+    if (st) { set_err(err, errlen, e); return st; }
+    size_t off = 0;
+    for (auto& l : p.levels) { memcpy(out + off, l.data(), l.size() * 4); off += l.size(); }
+    return TWX_OK;
+}
+
+void cref_calc_grad(const float* img, int w, int h, float* out) {
+    std::vector<float> g = calc_grad(img, w, h);
+    memcpy(out, g.data(), g.size() * 4);
+}
+void cref_calc_b(const float* i1, const float* i0, const float* grad, const float* flow, int w, int h, float* out) {
+    std::vector<float> b = calc_b(i1, i0, grad, flow, w, h);
+    memcpy(out, b.data(), b.size() * 4);
+}
+void cref_new_disp(float* flow, const float* a, const float* b, int n) { new_disp(flow, a, b, n); }
+void cref_upsample_flow(const float* src, int sw, int sh, int dw, int dh, float* dst) {
+    std::vector<float> d = upsample_flow(src, sw, sh, dw, dh);
+    memcpy(dst, d.data(), d.size() * 4);
+}
+void cref_block_match(const float* a, const float* b, float* flow, int w, int h) { block_match(a, b, flow, w, h); }
+int cref_block_cost(const float* a, const float* b, int w, int h, int x, int y, float x2, float y2, float* out) {
+    Block bm{a, b, w, h};
+    float p[9];
+    bm.patch(x, y, p);
+    return bm.cost(p, x2, y2, *out) ? 1 : 0;
+}
+int cref_all_flow_out_of_range(const float* flow, int w, int h) { return all_flow_out_of_range(flow, w, h) ? 1 : 0; }
+
+// flow::solve(X, Y) and flow::solve_pair(A, B) on engine images.
+int cref_solve(const void* x, const void* y, int w, int h, int fmt, const twx_flow_settings* fs, float* out, char* err, size_t errlen) {
+    Pyramid px, py;
+    std::string e;
+    int st = build_pyramid(x, w, h, fmt, *fs, px, e);
+    if (!st) st = build_pyramid(y, w, h, fmt, *fs, py, e);
+    std::vector<float> f;
+    if (!st) st = solve_pyramids(px, py, *fs, f, e);
+    if (st) { set_err(err, errlen, e); return st; }
+    memcpy(out, f.data(), f.size() * 4);
+    return TWX_OK;
+}
+
+int cref_solve_pair(const void* a, const void* b, int w, int h, int fmt, const twx_flow_settings* fs, float* f_ab, float* f_ba,
+                    char* err, size_t errlen) {
+    Pyramid pa, pb;
+    std::string e;
+    int st = build_pyramid(a, w, h, fmt, *fs, pa, e);
+    if (!st) st = build_pyramid(b, w, h, fmt, *fs, pb, e);
+    std::vector<float> ba, ab;
+    if (!st) st = solve_pyramids(pb, pa, *fs, ba, e);
+    if (!st) st = solve_pyramids(pa, pb, *fs, ab, e);
+    if (st) { set_err(err, errlen, e); return st; }
+    memcpy(f_ab, ab.data(), ab.size() * 4);
+    memcpy(f_ba, ba.data(), ba.size() * 4);
+    return TWX_OK;
+}
+
+int cref_inv_warp(void* out, const void* src, int w, int h, int fmt, const float* flow, int fw, int fh, float amount, char* err,
+                  size_t errlen) {
+    std::string e;
+    int st = inv_warp(out, src, w, h, fmt, flow, fw, fh, amount, e);
+    if (st) set_err(err, errlen, e);
+    return st;
+}
+
+void cref_blend(void* out, const void* a, const void* b, int w, int h, int fmt, float wt) { blend(out, a, b, w, h, fmt, wt); }
+void cref_lerp(void* first, const void* second, int w, int h, int fmt, float wt) { lerp(first, second, w, h, fmt, wt); }
+void cref_from_frame(const uint8_t* wire, int w, int h, int fmt, void* engine) { from_frame(wire, w, h, fmt, engine); }
+void cref_to_frame(const void* engine, int w, int h, int fmt, uint8_t* wire) { to_frame(engine, w, h, fmt, wire); }
+uint32_t cref_sat_u8(float v) { return sat_u8(v); }
+uint32_t cref_sat_u16(float v) { return sat_u16(v); }
+
+// compose_vectors, warping code 0 (compose/vector.rs:249-276), on engine images in motion rows.
+int cref_compose_vectors(int interp, float wa, const void* a, const void* b, int w, int h, int fmt, const float* f_ab,
+                         const float* f_ba, int fw, int fh, void* out, char* err, size_t errlen) {
+    std::string e;
+    float frac = 1.0f - wa;
+    size_t bytes = image_bytes(w, h, fmt);
+    int st;
+    if (interp == 0) {
+        if (0.5f <= wa) st = inv_warp(out, a, w, h, fmt, f_ba, fw, fh, frac, e);
+        else st = inv_warp(out, b, w, h, fmt, f_ab, fw, fh, wa, e);
+    } else if (interp == 1) {
+        std::vector<uint8_t> ta(bytes), tb(bytes);
+        st = inv_warp(ta.data(), a, w, h, fmt, f_ba, fw, fh, frac, e);
+        if (!st) st = inv_warp(tb.data(), b, w, h, fmt, f_ab, fw, fh, wa, e);
+        if (!st) blend(out, ta.data(), tb.data(), w, h, fmt, frac);
+    } else {
+        e = "oracle: only Nearest and Blend compositions are restated";
+        st = TWX_E_UNSUPPORTED;
+    }
+    if (st) set_err(err, errlen, e);
+    return st;
+}
+
+int cref_select(const twx_retime_params* p, int n, twx_selection* out, char* err, size_t errlen) {
+    std::string e;
+    Rates r;
+    int st = select_frame(*p, n, *out, r, e);
+    if (st) set_err(err, errlen, e);
+    return st;
+}
+
+// Source clip index of frame position `pos` (main_frame), or -1 outside the clip.
+int cref_frame_index(const twx_retime_params* p, int pos, int n_in) {
+    Rates r;
+    std::string e;
+    if (make_rates(*p, r, e)) return -2;
+    int32_t time = frame_time(r, (float)pos);
+    int64_t idx = (int64_t)std::floor((double)time / (double)p->time_step);
+    return (idx >= 0 && idx < n_in) ? (int)idx : -1;
+}
+
+// Output frames [first, first + count) of `vfx apply retime` over a wire clip of n_in frames.
+int cref_retime_frames(const twx_retime_params* p, const uint8_t* clip, int n_in, int first, int count, uint8_t* out, char* err,
+                       size_t errlen) {
+    Clip c{clip, n_in, p->width, p->height, p->fmt};
+    size_t fb = image_bytes(p->width, p->height, p->fmt);
+    for (int i = 0; i < count; i++) {
+        std::string e;
+        int st = render_frame(*p, c, first + i, out + (size_t)i * fb, e);
+        if (st) { set_err(err, errlen, "frame " + std::to_string(first + i) + ": " + e); return st; }
+    }
+    return TWX_OK;
+}
+
+}  // extern "C"
