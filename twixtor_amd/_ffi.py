@@ -1,0 +1,98 @@
+"""ctypes binding of libtwx.so (include/twx.h). Fails loudly when the library is missing: there is
+no CPU fallback in the product path."""
+from __future__ import annotations
+
+import ctypes
+import os
+
+from .params import RetimeParams, Selection
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libtwx.so")
+
+TWX_OK, TWX_E_DEVICE, TWX_E_NAN_OUTPUT, TWX_E_UNSUPPORTED, TWX_E_MISMATCH = 0, 1, 2, 3, 4
+
+# every entry point of include/twx.h
+EXPORTS = [
+    "twx_open", "twx_close", "twx_selfcheck", "twx_memory_budget", "twx_flow_geometry", "twx_solve",
+    "twx_solve_pair", "twx_compose_vectors", "twx_select", "twx_retime_frames_dev", "twx_retime_frames",
+    "twx_get_stats", "twx_stream",
+]
+
+
+class FlowSettings(ctypes.Structure):
+    _fields_ = [
+        ("quality", ctypes.c_int32),
+        ("hiq", ctypes.c_int32),
+        ("sensitivity", ctypes.c_float),
+        ("prep_contrast_edge", ctypes.c_int32),
+        ("prep_delinearize", ctypes.c_int32),
+        ("track_alpha", ctypes.c_int32),
+        ("video_range_luma", ctypes.c_int32),
+        ("skip_refine", ctypes.c_int32),
+        ("wrap", ctypes.c_int32),
+    ]
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [
+        ("solve_calls", ctypes.c_uint64),
+        ("pair_solves", ctypes.c_uint64),
+        ("pairs_reused", ctypes.c_uint64),
+        ("composes", ctypes.c_uint64),
+        ("fast_frames", ctypes.c_uint64),
+        ("pyramid_builds", ctypes.c_uint64),
+        ("pyramid_hits", ctypes.c_uint64),
+        ("device_ms", ctypes.c_double),
+    ]
+
+
+class StageError(RuntimeError):
+    """A non-zero twx status (the reference's StageError, device/image.rs:14-23)."""
+
+    def __init__(self, status: int, message: str):
+        super().__init__(f"status {status}: {message}")
+        self.status = status
+
+
+_lib = None
+
+P = ctypes.c_void_p
+I32 = ctypes.c_int32
+ERR = [ctypes.c_char_p, ctypes.c_size_t]
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is not built: run `python -m twixtor_amd.build` (no CPU fallback exists)")
+        L = ctypes.CDLL(LIB_PATH)
+        L.twx_open.argtypes = [ctypes.c_int, ctypes.POINTER(P)] + ERR
+        L.twx_close.argtypes = [P]
+        L.twx_close.restype = None
+        L.twx_selfcheck.argtypes = [P] + ERR
+        L.twx_memory_budget.argtypes = [P]
+        L.twx_memory_budget.restype = ctypes.c_uint64
+        L.twx_flow_geometry.argtypes = [ctypes.POINTER(FlowSettings), I32, I32, ctypes.POINTER(I32), ctypes.POINTER(I32),
+                                        ctypes.POINTER(I32)]
+        L.twx_solve.argtypes = [P, P, P, I32, I32, I32, ctypes.POINTER(FlowSettings), P] + ERR
+        L.twx_solve_pair.argtypes = [P, P, P, I32, I32, I32, ctypes.POINTER(FlowSettings), P, P] + ERR
+        L.twx_compose_vectors.argtypes = [P, I32, ctypes.c_float, P, P, I32, I32, I32, P, P, I32, I32, P] + ERR
+        L.twx_select.argtypes = [ctypes.POINTER(RetimeParams), I32, ctypes.POINTER(Selection)] + ERR
+        L.twx_retime_frames_dev.argtypes = [P, ctypes.POINTER(RetimeParams), P, I32, I32, I32, P] + ERR
+        L.twx_retime_frames.argtypes = [P, ctypes.POINTER(RetimeParams), P, I32, I32, I32, P] + ERR
+        L.twx_get_stats.argtypes = [P, ctypes.POINTER(Stats)]
+        L.twx_stream.argtypes = [P]
+        L.twx_stream.restype = P
+        _lib = L
+    return _lib
+
+
+def check(status: int, err) -> None:
+    if status != TWX_OK:
+        raise StageError(status, err.value.decode(errors="replace"))
+
+
+def errbuf():
+    return ctypes.create_string_buffer(512)

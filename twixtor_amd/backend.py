@@ -1,0 +1,130 @@
+"""HipBackend: the reference's device Backend trait over libtwx.so.
+
+Mirrors `trait Backend` (plugins/retime-engine/src/device/exec/backend.rs:27-46): `solve`,
+`solve_pair` and `compose_vectors` take engine images (A,R,G,B premultiplied, numpy arrays) and
+return flows or images, raising StageError on any non-zero status so the caller can rerun the stage
+on the CPU (device/exec/mod.rs:345-367). `retime_frames` is the whole per-frame render loop of
+`vfx apply retime` (crates/fx-cli/src/temporal.rs:183-250) batched over output frames, on host
+bytes or on device tensors.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import _ffi
+from .params import Retime, RetimeParams, Selection
+
+DTYPES = {0: np.uint8, 1: np.uint16, 2: np.float32}
+
+
+def _p(a: np.ndarray):
+    return ctypes.c_void_p(a.ctypes.data)
+
+
+def flow_settings(quality=4, hiq=True, sensitivity=None) -> _ffi.FlowSettings:
+    fs = _ffi.FlowSettings()
+    fs.quality = quality
+    fs.hiq = 1 if hiq else 0
+    fs.sensitivity = float(np.float32(70.0) / np.float32(100.0)) if sensitivity is None else float(sensitivity)
+    return fs
+
+
+def flow_geometry(fs: _ffi.FlowSettings, width: int, height: int):
+    fw, fh, lv = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
+    st = _ffi.lib().twx_flow_geometry(ctypes.byref(fs), width, height, ctypes.byref(fw), ctypes.byref(fh), ctypes.byref(lv))
+    if st:
+        raise _ffi.StageError(st, "flow geometry")
+    return fw.value, fh.value, lv.value
+
+
+def select(params: RetimeParams, n: int) -> Selection:
+    s = Selection()
+    err = _ffi.errbuf()
+    _ffi.check(_ffi.lib().twx_select(ctypes.byref(params), n, ctypes.byref(s), err, 512), err)
+    return s
+
+
+class HipBackend:
+    """One device handle (one in-order stream). Calls block; the handle serialises them."""
+
+    def __init__(self, device: int = -1):
+        L = _ffi.lib()
+        h = ctypes.c_void_p()
+        err = _ffi.errbuf()
+        _ffi.check(L.twx_open(device, ctypes.byref(h), err, 512), err)
+        self._h = h
+
+    def close(self):
+        if self._h:
+            _ffi.lib().twx_close(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def selfcheck(self):
+        err = _ffi.errbuf()
+        _ffi.check(_ffi.lib().twx_selfcheck(self._h, err, 512), err)
+
+    @property
+    def stream(self) -> int:
+        return _ffi.lib().twx_stream(self._h)
+
+    def stats(self) -> dict:
+        s = _ffi.Stats()
+        _ffi.lib().twx_get_stats(self._h, ctypes.byref(s))
+        return {k: getattr(s, k) for k, _ in s._fields_}
+
+    # ---- Backend::solve / solve_pair ----
+    def solve(self, x: np.ndarray, y: np.ndarray, width: int, height: int, fmt: int, fs: _ffi.FlowSettings) -> np.ndarray:
+        fw, fh, _ = flow_geometry(fs, width, height)
+        out = np.zeros(fw * fh * 2, np.float32)
+        x, y = np.ascontiguousarray(x), np.ascontiguousarray(y)
+        err = _ffi.errbuf()
+        _ffi.check(_ffi.lib().twx_solve(self._h, _p(x), _p(y), width, height, fmt, ctypes.byref(fs), _p(out), err, 512), err)
+        return out
+
+    def solve_pair(self, a, b, width, height, fmt, fs):
+        fw, fh, _ = flow_geometry(fs, width, height)
+        f_ab = np.zeros(fw * fh * 2, np.float32)
+        f_ba = np.zeros(fw * fh * 2, np.float32)
+        a, b = np.ascontiguousarray(a), np.ascontiguousarray(b)
+        err = _ffi.errbuf()
+        _ffi.check(_ffi.lib().twx_solve_pair(self._h, _p(a), _p(b), width, height, fmt, ctypes.byref(fs), _p(f_ab), _p(f_ba),
+                                             err, 512), err)
+        return f_ab, f_ba
+
+    # ---- Backend::compose_vectors (warping code 0) ----
+    def compose_vectors(self, interp, wa, a, b, width, height, fmt, f_ab, f_ba, fw, fh) -> np.ndarray:
+        out = np.zeros_like(a)
+        a, b = np.ascontiguousarray(a), np.ascontiguousarray(b)
+        fab = None if f_ab is None else np.ascontiguousarray(f_ab, np.float32)
+        fba = None if f_ba is None else np.ascontiguousarray(f_ba, np.float32)
+        err = _ffi.errbuf()
+        _ffi.check(_ffi.lib().twx_compose_vectors(self._h, interp, ctypes.c_float(wa), _p(a), _p(b), width, height, fmt,
+                                                  None if fab is None else _p(fab), None if fba is None else _p(fba),
+                                                  fw, fh, _p(out), err, 512), err)
+        return out
+
+    # ---- the per-frame render loop, batched ----
+    def retime_frames(self, cfg: Retime, clip: bytes, n_in: int, first: int, count: int) -> bytes:
+        p = cfg.struct()
+        src = np.frombuffer(clip, np.uint8)
+        out = np.zeros(cfg.frame_bytes() * count, np.uint8)
+        err = _ffi.errbuf()
+        _ffi.check(_ffi.lib().twx_retime_frames(self._h, ctypes.byref(p), _p(src), n_in, first, count, _p(out), err, 512), err)
+        return out.tobytes()
+
+    def retime_frames_dev(self, cfg: Retime, clip_ptr: int, n_in: int, first: int, count: int, out_ptr: int,
+                          params: RetimeParams | None = None) -> None:
+        """Device-resident form: `clip_ptr` and `out_ptr` are device addresses (e.g. torch
+        tensor.data_ptr() on the handle's device). Blocks until the frames are written."""
+        p = params if params is not None else cfg.struct()
+        err = _ffi.errbuf()
+        _ffi.check(_ffi.lib().twx_retime_frames_dev(self._h, ctypes.byref(p), ctypes.c_void_p(clip_ptr), n_in, first, count,
+                                                    ctypes.c_void_p(out_ptr), err, 512), err)

@@ -1,0 +1,911 @@
+// The C ABI of include/twx.h: device open and gate, the Backend-trait entry points
+// (plugins/retime-engine/src/device/exec/backend.rs:27-46) and the batched per-clip render of
+// `vfx apply retime` (crates/fx-cli/src/temporal.rs:183-250).
+//
+// A call plans on the host (frame selection, unique source frames, unique frame pairs), then
+// issues batched launches on the handle's stream: ingest of every needed source frame, one
+// pyramid build over all of them, one solve over every flow job, one compose-and-finish over
+// every output frame. Identical pairs are solved once (the reference solves every in-between
+// frame's pair again, compose/vector.rs:163-206; the flows are the same, so are the bytes).
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <map>
+#include <mutex>
+#include <thread>
+#include <vector>
+
+#include "twx_common.hpp"
+#include "twx_timeline.hpp"
+
+using namespace twx;
+
+struct twx_handle {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    uint64_t budget = 0;
+    std::mutex mu;
+    std::thread::id owner;
+    bool busy = false;
+    twx_stats stats{};
+};
+
+namespace {
+
+void set_err(char* err, size_t errlen, const std::string& m) {
+    if (err && errlen) {
+        size_t n = std::min(errlen - 1, m.size());
+        memcpy(err, m.data(), n);
+        err[n] = 0;
+    }
+}
+
+struct HipError {
+    std::string msg;
+};
+
+#define HIPCHECK(expr)                                                                        \
+    do {                                                                                      \
+        hipError_t e_ = (expr);                                                               \
+        if (e_ != hipSuccess) throw HipError{std::string(#expr) + ": " + hipGetErrorString(e_)}; \
+    } while (0)
+
+// Stream-ordered device buffers, returned to the pool when the call ends.
+class DevPool {
+  public:
+    explicit DevPool(hipStream_t s) : s_(s) {}
+    ~DevPool() {
+        for (void* p : ptrs_) (void)hipFreeAsync(p, s_);
+    }
+    template <class T>
+    T* alloc(size_t count) {
+        void* p = nullptr;
+        size_t bytes = std::max<size_t>(count * sizeof(T), 256);
+        HIPCHECK(hipMallocAsync(&p, bytes, s_));
+        ptrs_.push_back(p);
+        return (T*)p;
+    }
+    template <class T>
+    T* upload(const std::vector<T>& v) {
+        T* d = alloc<T>(v.size());
+        if (!v.empty()) HIPCHECK(hipMemcpyAsync(d, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s_));
+        return d;
+    }
+
+  private:
+    hipStream_t s_;
+    std::vector<void*> ptrs_;
+};
+
+// Serialises the calls on a handle and refuses a nested call from the same thread
+// (exec/backend.rs:151-159).
+class Guard {
+  public:
+    Guard(twx_handle* h, int& st, std::string& err) : h_(h) {
+        {
+            std::lock_guard<std::mutex> g(meta_mu());
+            if (h->busy && h->owner == std::this_thread::get_id()) {
+                st = TWX_E_UNSUPPORTED;
+                err = "nested device call from the same thread";
+                return;
+            }
+        }
+        h->mu.lock();
+        {
+            std::lock_guard<std::mutex> g(meta_mu());
+            h->busy = true;
+            h->owner = std::this_thread::get_id();
+        }
+        locked_ = true;
+        (void)hipSetDevice(h->device);
+    }
+    ~Guard() {
+        if (!locked_) return;
+        {
+            std::lock_guard<std::mutex> g(meta_mu());
+            h_->busy = false;
+        }
+        h_->mu.unlock();
+    }
+    bool ok() const { return locked_; }
+
+  private:
+    static std::mutex& meta_mu() {
+        static std::mutex m;
+        return m;
+    }
+    twx_handle* h_;
+    bool locked_ = false;
+};
+
+// ---- geometry (prep.rs:63-146, flow.rs:118-197) ------------------------------------------------
+
+struct Geo {
+    int reduction = 0;
+    int w0 = 0, h0 = 0;
+    int levels = 0;
+    int w[MAXL], h[MAXL];
+};
+
+int geometry(const twx_flow_settings& fs, int W, int H, Geo& g, std::string& err) {
+    if (fs.quality == 0 || fs.quality > 5) { err = "flow quality is outside 1..=5"; return TWX_E_MISMATCH; }
+    if (!(fs.sensitivity >= 0.0f) || !std::isfinite(fs.sensitivity)) { err = "sensitivity must be finite and not negative"; return TWX_E_MISMATCH; }
+    if (fs.prep_contrast_edge || fs.prep_delinearize || fs.track_alpha || fs.video_range_luma || fs.wrap) {
+        err = "prep, track alpha, video range luma and wrap-around solves are not served by this backend";
+        return TWX_E_UNSUPPORTED;
+    }
+    int q = fs.quality;
+    int r = (q <= 2) ? 3 : (q == 3 ? 2 : (q == 4 ? 1 : 0));
+    if (!fs.hiq) r = (r == 0) ? 1 : (r == 1 ? 2 : (r == 2 ? 3 : r));
+    while (r > 0 && ((W >> r) < 2 || (H >> r) < 2)) r--;
+    int w0 = W, h0 = H;
+    for (int i = 0; i < r; i++) { w0 /= 2; h0 /= 2; }
+    if (w0 < 2 || h0 < 2) { err = "optical flow: working size below 2x2 is not supported"; return TWX_E_UNSUPPORTED; }
+    int levels = 1;
+    {
+        int w = w0, h = h0;
+        for (;;) {
+            w /= 2;
+            if (w < 2) break;
+            h /= 2;
+            if (h < 2) break;
+            if (w > 2 && h > 2 && !(w == 2 && h == 2)) levels++;
+            else break;
+        }
+    }
+    if (levels > MAXL) { err = "pyramid deeper than this backend's level table"; return TWX_E_UNSUPPORTED; }
+    if (w0 > 16384 || h0 > 16384) { err = "working size above 16384"; return TWX_E_UNSUPPORTED; }
+    g.reduction = r;
+    g.w0 = w0;
+    g.h0 = h0;
+    g.levels = levels;
+    int w = w0, h = h0;
+    for (int i = 0; i < levels; i++) {
+        g.w[i] = w;
+        g.h[i] = h;
+        w = std::max(w / 2, 2);
+        h = std::max(h / 2, 2);
+    }
+    return TWX_OK;
+}
+
+void start_level(float sensitivity, int levels, int& start, float& damp) {
+    float p = (float)std::pow((double)sensitivity, 2.0);
+    float l1 = (float)std::log((double)p);
+    float l2 = (float)std::log(2.0);
+    float lv = (float)levels;
+    float x = l1 / l2 + lv;
+    if (0.0f <= x || std::isnan(x)) {
+        if (x < lv || std::isnan(x)) {
+            float f = std::floor(x);
+            int32_t s = (std::isnan(f) || f >= 2147483648.0f || f < -2147483648.0f) ? INT32_MIN : (int32_t)f;
+            long su = s < 0 ? 0 : (long)s;
+            start = (int)std::min(su, (long)levels - 1);
+            damp = x - (float)su;
+        } else {
+            start = levels - 1;
+            damp = 1.0f;
+        }
+    } else {
+        start = 0;
+        damp = 0.0f;
+    }
+}
+
+size_t pyramid_floats(const Geo& g) {
+    size_t n = 0;
+    for (int i = 0; i < g.levels; i++) n += (size_t)g.w[i] * g.h[i];
+    return n;
+}
+
+// ---- batched pyramid build (flow.rs:242-275, prep.rs:246-259) ----------------------------------
+
+struct PyrIn {
+    const void* img;  // engine image on the device
+    int W, H, fmt;
+    Geo geo;
+    float* levels;    // pyramid_floats(geo) floats
+};
+
+void build_pyramids(hipStream_t s, DevPool& pool, const std::vector<PyrIn>& in) {
+    if (in.empty()) return;
+    const Taps bin = binomial_taps();
+    const size_t chunk = 64;
+    for (size_t c0 = 0; c0 < in.size(); c0 += chunk) {
+        size_t c1 = std::min(in.size(), c0 + chunk);
+        int n = (int)(c1 - c0);
+        // per frame scratch: lum (W*H), red (W/2*H/2), tmp (W*H), blur (w0*h0)
+        std::vector<float*> lum(n), red(n), tmp(n), blur(n);
+        int maxW = 0, maxH = 0, maxR = 0, maxL = 0;
+        for (int i = 0; i < n; i++) {
+            const PyrIn& p = in[c0 + i];
+            lum[i] = pool.alloc<float>((size_t)p.W * p.H);
+            red[i] = pool.alloc<float>((size_t)(p.W / 2 + 1) * (p.H / 2 + 1));
+            tmp[i] = pool.alloc<float>((size_t)p.W * p.H);
+            blur[i] = pool.alloc<float>((size_t)p.geo.w0 * p.geo.h0);
+            maxW = std::max(maxW, p.W);
+            maxH = std::max(maxH, p.H);
+            maxR = std::max(maxR, p.geo.reduction);
+            maxL = std::max(maxL, p.geo.levels);
+        }
+        // every step is one launch over the chunk; a frame without the step gets an empty op
+        std::vector<std::vector<PlaneOp>> steps;
+        std::vector<int> kinds;  // 0 luma, 1 hpass, 2 vpass, 3 resample, 4 subtract
+        auto level_ptr = [&](int i, int l) {
+            const PyrIn& p = in[c0 + i];
+            float* base = p.levels;
+            for (int k = 0; k < l; k++) base += (size_t)p.geo.w[k] * p.geo.h[k];
+            return base;
+        };
+        auto empty = PlaneOp{nullptr, nullptr, 0, 0, 0, 0, 0, 0};
+        // luma into lum (or straight into level 0 without a reduction)
+        {
+            std::vector<PlaneOp> ops(n, empty);
+            for (int i = 0; i < n; i++) {
+                const PyrIn& p = in[c0 + i];
+                ops[i] = PlaneOp{p.img, p.geo.reduction ? lum[i] : level_ptr(i, 0), p.W, p.H, p.W, p.H, p.fmt, 0};
+            }
+            steps.push_back(ops);
+            kinds.push_back(0);
+        }
+        // reduction: R x (blur5, resample to the halved size)
+        for (int r = 0; r < maxR; r++) {
+            std::vector<PlaneOp> hp(n, empty), vp(n, empty), rs(n, empty);
+            for (int i = 0; i < n; i++) {
+                const PyrIn& p = in[c0 + i];
+                if (r >= p.geo.reduction) continue;
+                int cw = p.W, ch = p.H;
+                for (int k = 0; k < r; k++) { cw /= 2; ch /= 2; }
+                float* cur = (r % 2 == 0) ? lum[i] : red[i];
+                float* nxt = (r + 1 == p.geo.reduction) ? level_ptr(i, 0) : ((r % 2 == 0) ? red[i] : lum[i]);
+                hp[i] = PlaneOp{cur, tmp[i], cw, ch, cw, ch, 0, 0};
+                vp[i] = PlaneOp{tmp[i], cur, cw, ch, cw, ch, 0, 0};
+                rs[i] = PlaneOp{cur, nxt, cw, ch, cw / 2, ch / 2, 0, 0};
+            }
+            steps.push_back(hp); kinds.push_back(1);
+            steps.push_back(vp); kinds.push_back(2);
+            steps.push_back(rs); kinds.push_back(3);
+        }
+        // levels: blur = blur5(orig); next = resample(blur); band = orig - blur
+        for (int l = 0; l < maxL; l++) {
+            std::vector<PlaneOp> hp(n, empty), vp(n, empty), rs(n, empty), sb(n, empty);
+            for (int i = 0; i < n; i++) {
+                const PyrIn& p = in[c0 + i];
+                if (l >= p.geo.levels) continue;
+                int w = p.geo.w[l], h = p.geo.h[l];
+                float* orig = level_ptr(i, l);
+                hp[i] = PlaneOp{orig, tmp[i], w, h, w, h, 0, 0};
+                vp[i] = PlaneOp{tmp[i], blur[i], w, h, w, h, 0, 0};
+                if (l + 1 < p.geo.levels) rs[i] = PlaneOp{blur[i], level_ptr(i, l + 1), w, h, p.geo.w[l + 1], p.geo.h[l + 1], 0, 0};
+                sb[i] = PlaneOp{blur[i], orig, w, h, w, h, 0, 0};
+            }
+            steps.push_back(hp); kinds.push_back(1);
+            steps.push_back(vp); kinds.push_back(2);
+            steps.push_back(rs); kinds.push_back(3);
+            steps.push_back(sb); kinds.push_back(4);
+        }
+        std::vector<PlaneOp> flat;
+        for (auto& st : steps) flat.insert(flat.end(), st.begin(), st.end());
+        PlaneOp* d = pool.upload(flat);
+        for (size_t k = 0; k < steps.size(); k++) {
+            const PlaneOp* dk = d + k * n;
+            int mw = 0, mh = 0;
+            for (auto& op : steps[k]) {
+                mw = std::max({mw, op.sw, op.dw});
+                mh = std::max({mh, op.sh, op.dh});
+            }
+            if (mw == 0 || mh == 0) continue;
+            switch (kinds[k]) {
+                case 0: launch_luma(s, dk, n, mw, mh); break;
+                case 1: launch_hpass1(s, dk, n, mw, mh, bin); break;
+                case 2: launch_vpass1(s, dk, n, mw, mh, bin); break;
+                case 3: launch_resample(s, dk, n, mw, mh); break;
+                case 4: launch_subtract(s, dk, n, mw, mh); break;
+            }
+        }
+    }
+}
+
+// ---- batched solves ---------------------------------------------------------------------------
+
+struct SolveIn {
+    const float* X;  // pyramid levels (concatenated) of X and Y
+    const float* Y;
+    Geo geo;
+    float* out;      // w0 * h0 * 2 floats, the flow X(p) ~ Y(p + out(p))
+};
+
+// Runs every solve, in chunks that fit the scratch budget. Returns a status.
+int run_solves(hipStream_t s, DevPool& pool, const std::vector<SolveIn>& in, const twx_flow_settings& fs,
+               uint64_t scratch_budget, std::string& err) {
+    if (in.empty()) return TWX_OK;
+    size_t i0 = 0;
+    while (i0 < in.size()) {
+        // chunk by the scratch bytes of each job: flow[1] 2 + grad 2 + ta 3 + tb 2 + tmp 3 floats/pixel
+        size_t i1 = i0, bytes = 0;
+        while (i1 < in.size()) {
+            size_t px = (size_t)in[i1].geo.w0 * in[i1].geo.h0;
+            size_t b = px * 12 * sizeof(float) + 4096;
+            if (i1 > i0 && bytes + b > scratch_budget) break;
+            bytes += b;
+            i1++;
+        }
+        int n = (int)(i1 - i0);
+        std::vector<SolveJob> jobs(n);
+        int* state = pool.alloc<int>((size_t)n * 4);
+        std::vector<int> state_init((size_t)n * 4, 0);
+        for (int k = 0; k < n; k++) {
+            const SolveIn& si = in[i0 + k];
+            SolveJob& J = jobs[k];
+            memset(&J, 0, sizeof(J));
+            const float* xl = si.X;
+            const float* yl = si.Y;
+            for (int l = 0; l < si.geo.levels; l++) {
+                J.X[l] = xl;
+                J.Y[l] = yl;
+                J.w[l] = si.geo.w[l];
+                J.h[l] = si.geo.h[l];
+                xl += (size_t)si.geo.w[l] * si.geo.h[l];
+                yl += (size_t)si.geo.w[l] * si.geo.h[l];
+            }
+            J.levels = si.geo.levels;
+            start_level(fs.sensitivity, si.geo.levels, J.start, J.damp);
+            J.scale_damp = (std::fabs((double)J.damp - 1.0) >= 1e-6 && J.damp != 1.0f) ? 1 : 0;
+            size_t px = (size_t)si.geo.w0 * si.geo.h0;
+            J.flow[0] = si.out;
+            J.flow[1] = pool.alloc<float>(px * 2);
+            J.grad = pool.alloc<float>(px * 2);
+            J.ta = pool.alloc<float>(px * 3);
+            J.tb = pool.alloc<float>(px * 2);
+            J.tmp = pool.alloc<float>(px * 3);
+            J.state = state + k * 4;
+            J.max_w = si.geo.w0;
+            J.max_h = si.geo.h0;
+            state_init[k * 4] = 1;
+            // the flow starts at zero at the start level, in the buffer of that level's parity
+            size_t spx = (size_t)si.geo.w[J.start] * si.geo.h[J.start];
+            HIPCHECK(hipMemsetAsync(J.flow[J.start & 1], 0, spx * 2 * sizeof(float), s));
+        }
+        HIPCHECK(hipMemcpyAsync(state, state_init.data(), state_init.size() * sizeof(int), hipMemcpyHostToDevice, s));
+        SolveJob* d = pool.upload(jobs);
+        launch_solve(s, d, jobs, fs.skip_refine != 0);
+        std::vector<int> st((size_t)n * 4);
+        HIPCHECK(hipMemcpyAsync(st.data(), state, st.size() * sizeof(int), hipMemcpyDeviceToHost, s));
+        HIPCHECK(hipStreamSynchronize(s));
+        for (int k = 0; k < n; k++)
+            if (st[k * 4 + 2]) { err = "the solved flow holds a NaN"; return TWX_E_NAN_OUTPUT; }
+        i0 = i1;
+    }
+    return TWX_OK;
+}
+
+// ---- frame composition tables -----------------------------------------------------------------
+
+// The host-side branch decisions of blend (blend.rs:55-72) for weight w.
+void blend_branch(float w, FrameJob& F) {
+    F.weight = w;
+    if (0.005 > (double)std::fabs(w)) F.blend_mode = 0;
+    else if (0.005 > std::fabs((double)w - 1.0)) F.blend_mode = 1;
+    else if (w < 0.0f || w > 1.0f) { F.blend_mode = 2; F.lerp_w = (float)(1.0 - (double)w); }
+    else F.blend_mode = 3;
+}
+
+// The |amount| early-out of inv_warp (warp.rs:749-764): true means a plain copy.
+bool warp_is_copy(float amount, int fw, int W) {
+    float a = amount < 0.0f ? -amount : amount;
+    if (0.005 > (double)a) return true;
+    float r = (float)(fw - 1) / (float)(W - 1);
+    float q = (float)((double)amount / (double)r);
+    if (q < 0.0f) q = -q;
+    return 0.0001 > (double)q;
+}
+
+double now_ms() {
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+twx_flow_settings settings_for(const twx_retime_params& p, int vq) {
+    twx_flow_settings fs{};
+    fs.quality = vq;
+    fs.hiq = p.quality_high;
+    fs.sensitivity = timeline::read_fixed(timeline::to_fixed(p.sensitivity), 1.0f) / 100.0f;
+    return fs;
+}
+
+// The batched render of output frames [first, first + count) with the clip and the output on the
+// device.
+int retime_frames_dev(twx_handle* h, const twx_retime_params& p, const uint8_t* clip, int n_in, int first, int count,
+                      uint8_t* out, std::string& err) {
+    int st = timeline::check_params(p, err);
+    if (st) return st;
+    hipStream_t s = h->stream;
+    double t0 = now_ms();
+    timeline::Rates rates;
+    if ((st = timeline::rates(p, rates, err))) return st;
+    timeline::SpeedMap map(p, rates);
+    const int W = p.width, H = p.height, fmt = p.fmt;
+    const size_t fb = image_bytes(W, H, fmt);
+
+    struct Plan {
+        twx_selection sel;
+        int vector;  // 0 fast path, 1 vector path
+        long ia, ib; // clip indices (-1 transparent)
+    };
+    std::vector<Plan> plan(count);
+    for (int i = 0; i < count; i++) {
+        Plan& pl = plan[i];
+        if ((st = timeline::select(p, map, first + i, pl.sel, err))) return st;
+        pl.vector = (pl.sel.vectors_quality != 0 && !pl.sel.exact) ? 1 : 0;
+        if (pl.vector && pl.sel.interp == 2) {
+            err = "Weighted Blend on the vector path (the sharpening step) is not served by this backend";
+            return TWX_E_UNSUPPORTED;
+        }
+        if (pl.sel.interp < 0 || pl.sel.interp > 2) { err = "frame_interp outside 1..3"; return TWX_E_MISMATCH; }
+        pl.ia = timeline::clip_index(rates, pl.sel.a, n_in);
+        pl.ib = timeline::clip_index(rates, pl.sel.b, n_in);
+    }
+
+    DevPool pool(s);
+    // source slots: one engine image (motion rows) per clip index used; -1 is the transparent frame
+    std::map<long, int> slot_of;
+    std::vector<long> slots;
+    auto slot = [&](long idx) {
+        auto it = slot_of.find(idx);
+        if (it != slot_of.end()) return it->second;
+        int k = (int)slots.size();
+        slots.push_back(idx);
+        slot_of[idx] = k;
+        return k;
+    };
+    for (auto& pl : plan) { slot(pl.ia); slot(pl.ib); }
+    uint8_t* engine = pool.alloc<uint8_t>(fb * slots.size());
+    for (size_t k = 0; k < slots.size(); k++) {
+        if (slots[k] < 0) HIPCHECK(hipMemsetAsync(engine + k * fb, 0, fb, s));
+        else launch_ingest(s, clip + (size_t)slots[k] * fb, engine + k * fb, W, H, fmt, true);
+    }
+
+    // pyramids of the frames the vector path solves on, and the unique solves
+    std::map<int, int> pyr_of;  // slot -> pyramid index
+    std::vector<PyrIn> pyrs;
+    struct SolveKey {
+        int x, y, vq;
+        bool operator<(const SolveKey& o) const { return std::tie(x, y, vq) < std::tie(o.x, o.y, o.vq); }
+    };
+    std::map<SolveKey, int> solve_of;
+    std::vector<SolveIn> solves;
+    std::vector<int> solve_vq;
+    std::map<int, Geo> geo_of_vq;
+    auto geo_for = [&](int vq, Geo& g) -> int {
+        auto it = geo_of_vq.find(vq);
+        if (it != geo_of_vq.end()) { g = it->second; return TWX_OK; }
+        twx_flow_settings fs = settings_for(p, vq);
+        int r = geometry(fs, W, H, g, err);
+        if (!r) geo_of_vq[vq] = g;
+        return r;
+    };
+    auto pyramid = [&](int sl, int vq) -> int {
+        // one pyramid per (slot, quality); quality is constant over a run (no segment marks)
+        int key = sl * 8 + vq;
+        auto it = pyr_of.find(key);
+        if (it != pyr_of.end()) return it->second;
+        Geo g;
+        (void)geo_for(vq, g);
+        PyrIn pi{engine + (size_t)sl * fb, W, H, fmt, g, nullptr};
+        pi.levels = pool.alloc<float>(pyramid_floats(g));
+        int k = (int)pyrs.size();
+        pyrs.push_back(pi);
+        pyr_of[key] = k;
+        return k;
+    };
+    auto solve = [&](int sx, int sy, int vq) -> int {
+        SolveKey key{sx, sy, vq};
+        auto it = solve_of.find(key);
+        if (it != solve_of.end()) { h->stats.pairs_reused++; return it->second; }
+        int px = pyramid(sx, vq), py = pyramid(sy, vq);
+        Geo g = pyrs[px].geo;
+        SolveIn si{pyrs[px].levels, pyrs[py].levels, g, pool.alloc<float>((size_t)g.w0 * g.h0 * 2)};
+        int k = (int)solves.size();
+        solves.push_back(si);
+        solve_vq.push_back(vq);
+        solve_of[key] = k;
+        return k;
+    };
+    struct FrameSolves {
+        int f_ab = -1, f_ba = -1;
+    };
+    std::vector<FrameSolves> fsv(count);
+    for (int i = 0; i < count; i++) {
+        const Plan& pl = plan[i];
+        if (!pl.vector) continue;
+        Geo g;
+        if ((st = geo_for(pl.sel.vectors_quality, g))) return st;
+        int sa = slot_of[pl.ia], sb = slot_of[pl.ib];
+        int vq = pl.sel.vectors_quality;
+        if (pl.sel.interp == 0) {
+            // Nearest: one solve, the one nearest_uses_f_ab names (compose/vector.rs:170-176)
+            if (pl.sel.wa < 0.5f) fsv[i].f_ab = solve(sa, sb, vq);
+            else fsv[i].f_ba = solve(sb, sa, vq);
+        } else {
+            fsv[i].f_ba = solve(sb, sa, vq);
+            fsv[i].f_ab = solve(sa, sb, vq);
+        }
+    }
+    build_pyramids(s, pool, pyrs);
+    h->stats.pyramid_builds += pyrs.size();
+    h->stats.pair_solves += solves.size();
+    // solves grouped by quality (one settings struct per launch set)
+    {
+        std::map<int, std::vector<SolveIn>> by_vq;
+        for (size_t k = 0; k < solves.size(); k++) by_vq[solve_vq[k]].push_back(solves[k]);
+        for (auto& kv : by_vq) {
+            twx_flow_settings fs = settings_for(p, kv.first);
+            if ((st = run_solves(s, pool, kv.second, fs, h->budget / 2, err))) return st;
+        }
+    }
+
+    // compose and finish every output frame in one launch
+    int* bad = pool.alloc<int>(1);
+    HIPCHECK(hipMemsetAsync(bad, 0, sizeof(int), s));
+    std::vector<FrameJob> jobs(count);
+    for (int i = 0; i < count; i++) {
+        const Plan& pl = plan[i];
+        FrameJob& F = jobs[i];
+        memset(&F, 0, sizeof(F));
+        F.a = engine + (size_t)slot_of[pl.ia] * fb;
+        F.b = engine + (size_t)slot_of[pl.ib] * fb;
+        F.out = out + (size_t)i * fb;
+        F.bad = bad;
+        F.fmt = fmt;
+        F.W = W;
+        F.H = H;
+        F.finish = 1;
+        F.wa = pl.sel.wa;
+        F.frac = 1.0f - pl.sel.wa;
+        if (!pl.vector) {
+            F.kind = 0;
+            h->stats.fast_frames++;
+            if (pl.sel.interp == 0) {
+                F.blend_mode = pl.sel.wa >= 0.5f ? 0 : 1;
+            } else if (pl.sel.interp == 1) {
+                blend_branch((float)(1.0 - (double)pl.sel.wa), F);
+            } else {
+                // weighted_blend_noflow (blend.rs:280-300): Blend with 1 - wa in f32, float inputs + 0.0
+                blend_branch(1.0f - pl.sel.wa, F);
+                F.plus_zero = 1;
+            }
+        } else {
+            h->stats.composes++;
+            Geo g = geo_of_vq[pl.sel.vectors_quality];
+            F.fw = g.w0;
+            F.fh = g.h0;
+            if (fsv[i].f_ab >= 0) F.f_ab = solves[fsv[i].f_ab].out;
+            if (fsv[i].f_ba >= 0) F.f_ba = solves[fsv[i].f_ba].out;
+            F.warp_a_copy = warp_is_copy(F.frac, g.w0, W) ? 1 : 0;
+            F.warp_b_copy = warp_is_copy(F.wa, g.w0, W) ? 1 : 0;
+            if (pl.sel.interp == 0) {
+                F.kind = 1;
+                F.nearest_from_a = 0.5f <= pl.sel.wa ? 1 : 0;
+            } else {
+                F.kind = 2;
+                blend_branch(F.frac, F);
+            }
+        }
+    }
+    if (W < 2 || H < 2) {
+        for (auto& F : jobs)
+            if (F.kind != 0) { err = "inverse warp needs images of at least 2x2"; return TWX_E_MISMATCH; }
+    }
+    FrameJob* dj = pool.upload(jobs);
+    launch_frames(s, dj, jobs);
+    int hbad = 0;
+    HIPCHECK(hipMemcpyAsync(&hbad, bad, sizeof(int), hipMemcpyDeviceToHost, s));
+    HIPCHECK(hipStreamSynchronize(s));
+    HIPCHECK(hipGetLastError());
+    h->stats.device_ms += now_ms() - t0;
+    if (hbad) { err = "inverse warp: a sample coordinate is NaN or never converges, which is not supported"; return TWX_E_UNSUPPORTED; }
+    return TWX_OK;
+}
+
+// The Backend::solve / solve_pair path on host engine images (rows as given).
+int solve_host(twx_handle* h, const void* x, const void* y, int W, int H, int fmt, const twx_flow_settings& fs,
+               float* f_xy, float* f_yx, std::string& err) {
+    Geo g;
+    int st = geometry(fs, W, H, g, err);
+    if (st) return st;
+    if (fmt < 0 || fmt > 2) { err = "bad format code"; return TWX_E_MISMATCH; }
+    hipStream_t s = h->stream;
+    double t0 = now_ms();
+    DevPool pool(s);
+    size_t fb = image_bytes(W, H, fmt);
+    uint8_t* img = pool.alloc<uint8_t>(fb * 2);
+    HIPCHECK(hipMemcpyAsync(img, x, fb, hipMemcpyHostToDevice, s));
+    HIPCHECK(hipMemcpyAsync(img + fb, y, fb, hipMemcpyHostToDevice, s));
+    std::vector<PyrIn> pyrs(2);
+    for (int k = 0; k < 2; k++) {
+        pyrs[k] = PyrIn{img + k * fb, W, H, fmt, g, pool.alloc<float>(pyramid_floats(g))};
+    }
+    build_pyramids(s, pool, pyrs);
+    size_t fn = (size_t)g.w0 * g.h0 * 2;
+    std::vector<SolveIn> solves;
+    // solve_pair order: F_BA (X = B) then F_AB (X = A); both in one batched launch
+    if (f_yx) solves.push_back(SolveIn{pyrs[1].levels, pyrs[0].levels, g, pool.alloc<float>(fn)});
+    solves.push_back(SolveIn{pyrs[0].levels, pyrs[1].levels, g, pool.alloc<float>(fn)});
+    if ((st = run_solves(s, pool, solves, fs, h->budget / 2, err))) return st;
+    if (f_yx) {
+        HIPCHECK(hipMemcpyAsync(f_yx, solves[0].out, fn * 4, hipMemcpyDeviceToHost, s));
+        HIPCHECK(hipMemcpyAsync(f_xy, solves[1].out, fn * 4, hipMemcpyDeviceToHost, s));
+    } else {
+        HIPCHECK(hipMemcpyAsync(f_xy, solves[0].out, fn * 4, hipMemcpyDeviceToHost, s));
+    }
+    HIPCHECK(hipStreamSynchronize(s));
+    h->stats.pyramid_builds += 2;
+    h->stats.pair_solves += solves.size();
+    h->stats.solve_calls++;
+    h->stats.device_ms += now_ms() - t0;
+    return TWX_OK;
+}
+
+}  // namespace
+
+// ================================================================================================
+
+extern "C" {
+
+int twx_open(int index, twx_handle** out, char* err, size_t errlen) {
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n == 0) {
+        set_err(err, errlen, "no HIP device");
+        return TWX_E_DEVICE;
+    }
+    int dev = index < 0 ? 0 : index;
+    if (dev >= n) {
+        set_err(err, errlen, "device index out of range");
+        return TWX_E_DEVICE;
+    }
+    twx_handle* h = new twx_handle();
+    h->device = dev;
+    try {
+        HIPCHECK(hipSetDevice(dev));
+        HIPCHECK(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+        size_t free_b = 0, total_b = 0;
+        HIPCHECK(hipMemGetInfo(&free_b, &total_b));
+        h->budget = (uint64_t)(total_b * 0.75);
+        // keep freed stream-ordered memory in the pool between calls
+        hipMemPool_t pool;
+        HIPCHECK(hipDeviceGetDefaultMemPool(&pool, dev));
+        uint64_t thresh = UINT64_MAX;
+        HIPCHECK(hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &thresh));
+        std::string e;
+        int st = run_selfcheck(h->stream, e);
+        if (st) {
+            set_err(err, errlen, "device self-check refused the device: " + e);
+            (void)hipStreamDestroy(h->stream);
+            delete h;
+            return st;
+        }
+    } catch (const HipError& e) {
+        set_err(err, errlen, e.msg);
+        delete h;
+        return TWX_E_DEVICE;
+    }
+    *out = h;
+    return TWX_OK;
+}
+
+void twx_close(twx_handle* h) {
+    if (!h) return;
+    (void)hipSetDevice(h->device);
+    if (h->stream) {
+        (void)hipStreamSynchronize(h->stream);
+        (void)hipStreamDestroy(h->stream);
+    }
+    delete h;
+}
+
+int twx_selfcheck(twx_handle* h, char* err, size_t errlen) {
+    std::string e;
+    int st = TWX_OK;
+    try {
+        Guard g(h, st, e);
+        if (!g.ok()) { set_err(err, errlen, e); return st; }
+        st = run_selfcheck(h->stream, e);
+    } catch (const HipError& x) {
+        set_err(err, errlen, x.msg);
+        return TWX_E_DEVICE;
+    }
+    if (st) set_err(err, errlen, e);
+    return st;
+}
+
+uint64_t twx_memory_budget(twx_handle* h) { return h ? h->budget : 0; }
+
+void* twx_stream(twx_handle* h) { return h ? (void*)h->stream : nullptr; }
+
+int twx_flow_geometry(const twx_flow_settings* fs, int32_t width, int32_t height, int32_t* flow_w, int32_t* flow_h,
+                      int32_t* levels) {
+    Geo g;
+    std::string e;
+    int st = geometry(*fs, width, height, g, e);
+    if (st) return st;
+    if (flow_w) *flow_w = g.w0;
+    if (flow_h) *flow_h = g.h0;
+    if (levels) *levels = g.levels;
+    return TWX_OK;
+}
+
+int twx_solve(twx_handle* h, const void* x, const void* y, int32_t width, int32_t height, int32_t fmt,
+              const twx_flow_settings* fs, float* flow_out, char* err, size_t errlen) {
+    std::string e;
+    int st = TWX_OK;
+    try {
+        Guard g(h, st, e);
+        if (g.ok()) st = solve_host(h, x, y, width, height, fmt, *fs, flow_out, nullptr, e);
+    } catch (const HipError& x) {
+        e = x.msg;
+        st = TWX_E_DEVICE;
+    }
+    if (st) set_err(err, errlen, e);
+    return st;
+}
+
+int twx_solve_pair(twx_handle* h, const void* a, const void* b, int32_t width, int32_t height, int32_t fmt,
+                   const twx_flow_settings* fs, float* f_ab_out, float* f_ba_out, char* err, size_t errlen) {
+    std::string e;
+    int st = TWX_OK;
+    try {
+        Guard g(h, st, e);
+        if (g.ok()) st = solve_host(h, a, b, width, height, fmt, *fs, f_ab_out, f_ba_out, e);
+    } catch (const HipError& x) {
+        e = x.msg;
+        st = TWX_E_DEVICE;
+    }
+    if (st) set_err(err, errlen, e);
+    return st;
+}
+
+int twx_compose_vectors(twx_handle* h, int32_t interp, float wa, const void* a, const void* b, int32_t width,
+                        int32_t height, int32_t fmt, const float* f_ab, const float* f_ba, int32_t flow_w,
+                        int32_t flow_h, void* out, char* err, size_t errlen) {
+    std::string e;
+    int st = TWX_OK;
+    try {
+        Guard g(h, st, e);
+        if (!g.ok()) { set_err(err, errlen, e); return st; }
+        if (interp != 0 && interp != 1) {
+            set_err(err, errlen, "only Nearest and Blend (warping code 0) are served by this backend");
+            return TWX_E_UNSUPPORTED;
+        }
+        if (width < 2 || height < 2 || flow_w < 2 || flow_h < 2 || fmt < 0 || fmt > 2) {
+            set_err(err, errlen, "inverse warp needs images and a flow of at least 2x2");
+            return TWX_E_MISMATCH;
+        }
+        hipStream_t s = h->stream;
+        double t0 = now_ms();
+        DevPool pool(s);
+        size_t fb = image_bytes(width, height, fmt);
+        size_t fn = (size_t)flow_w * flow_h * 2;
+        uint8_t* img = pool.alloc<uint8_t>(fb * 3);
+        float* fl = pool.alloc<float>(fn * 2);
+        HIPCHECK(hipMemcpyAsync(img, a, fb, hipMemcpyHostToDevice, s));
+        HIPCHECK(hipMemcpyAsync(img + fb, b, fb, hipMemcpyHostToDevice, s));
+        FrameJob F;
+        memset(&F, 0, sizeof(F));
+        F.a = img;
+        F.b = img + fb;
+        F.out = img + 2 * fb;
+        F.fmt = fmt;
+        F.W = width;
+        F.H = height;
+        F.fw = flow_w;
+        F.fh = flow_h;
+        F.wa = wa;
+        F.frac = 1.0f - wa;
+        F.finish = 0;
+        F.warp_a_copy = warp_is_copy(F.frac, flow_w, width) ? 1 : 0;
+        F.warp_b_copy = warp_is_copy(wa, flow_w, width) ? 1 : 0;
+        bool need_ab = interp == 1 || !(0.5f <= wa);
+        bool need_ba = interp == 1 || (0.5f <= wa);
+        if (need_ab) {
+            if (!f_ab) { set_err(err, errlen, "the composition needs the flow F_AB"); return TWX_E_MISMATCH; }
+            HIPCHECK(hipMemcpyAsync(fl, f_ab, fn * 4, hipMemcpyHostToDevice, s));
+            F.f_ab = fl;
+        }
+        if (need_ba) {
+            if (!f_ba) { set_err(err, errlen, "the composition needs the flow F_BA"); return TWX_E_MISMATCH; }
+            HIPCHECK(hipMemcpyAsync(fl + fn, f_ba, fn * 4, hipMemcpyHostToDevice, s));
+            F.f_ba = fl + fn;
+        }
+        if (interp == 0) {
+            F.kind = 1;
+            F.nearest_from_a = 0.5f <= wa ? 1 : 0;
+        } else {
+            F.kind = 2;
+            blend_branch(F.frac, F);
+        }
+        int* bad = pool.alloc<int>(1);
+        HIPCHECK(hipMemsetAsync(bad, 0, sizeof(int), s));
+        F.bad = bad;
+        std::vector<FrameJob> jobs{F};
+        FrameJob* dj = pool.upload(jobs);
+        launch_frames(s, dj, jobs);
+        int hbad = 0;
+        HIPCHECK(hipMemcpyAsync(out, img + 2 * fb, fb, hipMemcpyDeviceToHost, s));
+        HIPCHECK(hipMemcpyAsync(&hbad, bad, sizeof(int), hipMemcpyDeviceToHost, s));
+        HIPCHECK(hipStreamSynchronize(s));
+        h->stats.composes++;
+        h->stats.device_ms += now_ms() - t0;
+        if (hbad) st = TWX_E_UNSUPPORTED, e = "inverse warp: a sample coordinate is NaN or never converges, which is not supported";
+    } catch (const HipError& x) {
+        e = x.msg;
+        st = TWX_E_DEVICE;
+    }
+    if (st) set_err(err, errlen, e);
+    return st;
+}
+
+int twx_select(const twx_retime_params* p, int32_t n, twx_selection* out, char* err, size_t errlen) {
+    std::string e;
+    int st = timeline::check_params(*p, e);
+    timeline::Rates r;
+    if (!st) st = timeline::rates(*p, r, e);
+    if (!st) {
+        timeline::SpeedMap map(*p, r);
+        st = timeline::select(*p, map, n, *out, e);
+    }
+    if (st) set_err(err, errlen, e);
+    return st;
+}
+
+int twx_retime_frames_dev(twx_handle* h, const twx_retime_params* p, const void* clip, int32_t n_in, int32_t first,
+                          int32_t count, void* out, char* err, size_t errlen) {
+    std::string e;
+    int st = TWX_OK;
+    try {
+        Guard g(h, st, e);
+        if (g.ok()) st = retime_frames_dev(h, *p, (const uint8_t*)clip, n_in, first, count, (uint8_t*)out, e);
+    } catch (const HipError& x) {
+        e = x.msg;
+        st = TWX_E_DEVICE;
+    }
+    if (st) set_err(err, errlen, e);
+    return st;
+}
+
+int twx_retime_frames(twx_handle* h, const twx_retime_params* p, const void* clip, int32_t n_in, int32_t first,
+                      int32_t count, void* out, char* err, size_t errlen) {
+    std::string e;
+    int st = TWX_OK;
+    try {
+        Guard g(h, st, e);
+        if (!g.ok()) { set_err(err, errlen, e); return st; }
+        if ((st = timeline::check_params(*p, e))) { set_err(err, errlen, e); return st; }
+        hipStream_t s = h->stream;
+        size_t fb = image_bytes(p->width, p->height, p->fmt);
+        DevPool pool(s);
+        uint8_t* dclip = pool.alloc<uint8_t>(fb * std::max(n_in, 1));
+        uint8_t* dout = pool.alloc<uint8_t>(fb * std::max(count, 1));
+        if (n_in > 0) HIPCHECK(hipMemcpyAsync(dclip, clip, fb * n_in, hipMemcpyHostToDevice, s));
+        st = retime_frames_dev(h, *p, dclip, n_in, first, count, dout, e);
+        if (!st) {
+            HIPCHECK(hipMemcpyAsync(out, dout, fb * count, hipMemcpyDeviceToHost, s));
+            HIPCHECK(hipStreamSynchronize(s));
+        }
+    } catch (const HipError& x) {
+        e = x.msg;
+        st = TWX_E_DEVICE;
+    }
+    if (st) set_err(err, errlen, e);
+    return st;
+}
+
+int twx_get_stats(twx_handle* h, twx_stats* out) {
+    if (!h || !out) return TWX_E_MISMATCH;
+    *out = h->stats;
+    return TWX_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,94 @@
+// Shared definitions of the HIP backend: device job tables and the host-side launch API.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/twx.h"
+
+namespace twx {
+
+constexpr int MAXL = 16;     // pyramid levels (a side of 65536 would give 16)
+constexpr int MAX_TAPS = 8;  // half kernels used: r=7 -> 8, r=5 -> 6, binomial -> 3
+
+struct Taps {
+    float k[MAX_TAPS];
+    int n;
+};
+
+// A plane operation of the pyramid build, one per frame of a batched launch (blockIdx.z).
+struct PlaneOp {
+    const void* src;
+    float* dst;
+    int sw, sh, dw, dh;
+    int fmt;  // luma: the engine image format
+    int pad;
+};
+
+// One flow solve X -> Y over a pyramid pair (flow.rs:1258-1307). Device resident table entry;
+// every per-level kernel reads its job from `jobs[blockIdx.z]` and skips levels above `start`.
+struct SolveJob {
+    const float* X[MAXL];  // band-pass levels of X (imgA of the solver), finest first
+    const float* Y[MAXL];  // band-pass levels of Y (imgB)
+    int w[MAXL], h[MAXL];
+    int levels;
+    int start;             // start level (flow.rs:179-197)
+    float damp;
+    int scale_damp;        // |damp - 1| >= 1e-6 and damp != 1
+    float* flow[2];        // flow, double buffered across the upsample: level l lives in flow[l & 1]
+    float* grad;           // 2 channels
+    float* ta;             // structure tensor, 3 channels
+    float* tb;             // right-hand side B, 2 channels
+    float* tmp;            // horizontal-pass scratch, up to 3 channels
+    int* state;            // [0] range check still active, [1] any pixel in range, [2] NaN seen
+    int max_w, max_h;
+};
+
+// One output frame of a retime run (compose.rs:93-246 fast path, compose/vector.rs:249-276 vector
+// path code 0), including the finish into wire bytes (image.rs:205-245).
+struct FrameJob {
+    const void* a;         // engine images in motion-row order (bottom row first)
+    const void* b;
+    const float* f_ab;     // flows at the working size (vector path)
+    const float* f_ba;
+    void* out;             // wire frame
+    int* bad;              // set when a warp coordinate is NaN (StageError::Unsupported)
+    int kind;              // 0 fast path, 1 vector Nearest, 2 vector Blend
+    int fmt;
+    int W, H, fw, fh;
+    float wa, frac;        // Selection::wa and 1 - wa (the amounts of the two warps)
+    float weight;          // blend weight
+    int blend_mode;        // 0 copy first, 1 copy second, 2 copy first then lerp, 3 per-sample mix
+    float lerp_w;          // lerp weight of the extrapolation branch
+    int warp_a_copy;       // the |amount| early-out of the warp of A: a plain copy
+    int warp_b_copy;
+    int nearest_from_a;    // Nearest: warp A with F_BA (wa >= 0.5), else warp B with F_AB
+    int plus_zero;         // weighted blend without flows on float images: inputs + 0.0 first
+    int finish;            // 1: unpremultiply into the wire frame, rows turned back; 0: engine pixel as is
+    int pad;
+};
+
+// Host side launchers (twx_device.hip).
+void launch_ingest(hipStream_t s, const void* wire, void* engine, int W, int H, int fmt, bool reverse_rows);
+void launch_luma(hipStream_t s, const PlaneOp* d_ops, int n, int maxw, int maxh);
+void launch_hpass1(hipStream_t s, const PlaneOp* d_ops, int n, int maxw, int maxh, const Taps& t);
+void launch_vpass1(hipStream_t s, const PlaneOp* d_ops, int n, int maxw, int maxh, const Taps& t);
+void launch_resample(hipStream_t s, const PlaneOp* d_ops, int n, int maxw, int maxh);
+void launch_subtract(hipStream_t s, const PlaneOp* d_ops, int n, int maxw, int maxh);
+void launch_solve(hipStream_t s, const SolveJob* d_jobs, const std::vector<SolveJob>& jobs, bool skip_refine);
+void launch_frames(hipStream_t s, const FrameJob* d_jobs, const std::vector<FrameJob>& jobs);
+void launch_nan_scan(hipStream_t s, const float* data, size_t n, int* flag);
+int run_selfcheck(hipStream_t s, std::string& err);
+
+// Taps of the solver's Gaussians and of the pyramid filter (conv.rs:44-86).
+Taps gaussian_taps(float r);
+Taps binomial_taps();
+
+inline size_t sample_size(int fmt) { return fmt == TWX_FMT_U8 ? 1 : (fmt == TWX_FMT_U16 ? 2 : 4); }
+inline size_t image_bytes(long w, long h, int fmt) { return (size_t)w * h * 4 * sample_size(fmt); }
+
+}  // namespace twx

@@ -1,0 +1,933 @@
+// HIP kernels of the frame-synthesis path for gfx950 (CDNA4), bit-exact with the reference CPU
+// path. Every kernel follows the CPU's operation order (plugins/retime-engine/src/flow.rs,
+// flow/conv.rs, flow/sample.rs, prep.rs, warp.rs, blend.rs, image.rs); the library is compiled
+// with -ffp-contract=off, IEEE denormals and correctly rounded division, so every float operation
+// is the one the CPU performs.
+//
+// Batching: every kernel takes a table of jobs and maps blockIdx.z onto it, so one launch serves
+// one pair, every pair of a clip, or the pairs of many clips of different sizes (blocks outside a
+// job's extent exit at once).
+
+#include <hip/hip_runtime.h>
+
+#include <climits>
+#include <cmath>
+
+#include "twx_common.hpp"
+
+namespace twx {
+
+// -------------------------------------------------------------------------------------------
+// Arithmetic helpers with the reference's semantics
+
+// flow/sample.rs:18-25: truncation, NaN and out of range give INT_MIN (the x86 cvtt value)
+__device__ __forceinline__ int32_t cvtt(float x) {
+    if (isnan(x) || x >= 2147483648.0f || x < -2147483648.0f) return INT_MIN;
+    return (int32_t)x;
+}
+// blend.rs:18-25 (64 bit truncation of a float; the callers keep the low byte or word)
+__device__ __forceinline__ int64_t cvtt64f(float x) {
+    const float two63 = 9223372036854775808.0f;
+    if (isnan(x) || x >= two63 || x < -two63) return LLONG_MIN;
+    return (int64_t)x;
+}
+// image.rs:249-255 (64 bit truncation of a double)
+__device__ __forceinline__ int64_t cvtt64d(double x) {
+    if (isnan(x) || x >= 9.223372036854775808e18 || x < -9.223372036854775808e18) return LLONG_MIN;
+    return (int64_t)x;
+}
+// image.rs:259-261: q(x) = trunc((double)x + 0.5)
+__device__ __forceinline__ int64_t qround(float x) { return cvtt64d((double)x + 0.5); }
+
+__device__ __forceinline__ float bits_f(uint32_t b) { return __uint_as_float(b); }
+
+// -------------------------------------------------------------------------------------------
+// Wire -> engine image (image.rs:173-203, frame.rs:79-111, image.rs:264-298)
+
+template <int FMT>
+__global__ void k_ingest(const uint8_t* __restrict__ wire, uint8_t* __restrict__ engine, int W, int H, int reverse) {
+    int x = blockIdx.x * blockDim.x + threadIdx.x;
+    int y = blockIdx.y;
+    if (x >= W) return;
+    long n = (long)W * H;
+    long si = (long)y * W + x;
+    int yo = reverse ? H - 1 - y : y;
+    long di = (long)yo * W + x;
+    if (FMT == TWX_FMT_U8) {
+        uchar4 c = reinterpret_cast<const uchar4*>(wire)[si];
+        float a = (float)c.w * bits_f(0x3b808081u);
+        uchar4 o;
+        o.x = c.w;
+        o.y = (uint8_t)qround((float)c.x * a);
+        o.z = (uint8_t)qround((float)c.y * a);
+        o.w = (uint8_t)qround((float)c.z * a);
+        reinterpret_cast<uchar4*>(engine)[di] = o;
+    } else if (FMT == TWX_FMT_U16) {
+        ushort4 c = reinterpret_cast<const ushort4*>(wire)[si];
+        auto q15 = [](uint32_t v) -> uint32_t { return (v * 32768u + 32767u) / 65535u; };
+        uint32_t r = q15(c.x), g = q15(c.y), b = q15(c.z), al = q15(c.w);
+        float a = (float)al * bits_f(0x387fff00u);
+        ushort4 o;
+        o.x = (uint16_t)qround((float)al * bits_f(0x3fffff00u));
+        o.y = (uint16_t)qround((float)r * a);
+        o.z = (uint16_t)qround((float)g * a);
+        o.w = (uint16_t)qround((float)b * a);
+        reinterpret_cast<ushort4*>(engine)[di] = o;
+    } else {
+        const float* pl = reinterpret_cast<const float*>(wire);  // planes G, B, R, A
+        float g = pl[si], b = pl[n + si], r = pl[2 * n + si], a = pl[3 * n + si];
+        float4 o = make_float4(a, a * r, a * g, a * b);
+        reinterpret_cast<float4*>(engine)[di] = o;
+    }
+}
+
+void launch_ingest(hipStream_t s, const void* wire, void* engine, int W, int H, int fmt, bool reverse_rows) {
+    dim3 blk(256), grd((W + 255) / 256, H);
+    if (fmt == TWX_FMT_U8) k_ingest<0><<<grd, blk, 0, s>>>((const uint8_t*)wire, (uint8_t*)engine, W, H, reverse_rows);
+    else if (fmt == TWX_FMT_U16) k_ingest<1><<<grd, blk, 0, s>>>((const uint8_t*)wire, (uint8_t*)engine, W, H, reverse_rows);
+    else k_ingest<2><<<grd, blk, 0, s>>>((const uint8_t*)wire, (uint8_t*)engine, W, H, reverse_rows);
+}
+
+// -------------------------------------------------------------------------------------------
+// Luma (prep.rs:168-235, retime path: no alpha, no video range)
+
+__global__ void k_luma(const PlaneOp* __restrict__ ops) {
+    const PlaneOp op = ops[blockIdx.z];
+    int x = blockIdx.x * blockDim.x + threadIdx.x;
+    int y = blockIdx.y;
+    if (x >= op.sw || y >= op.sh) return;
+    long i = (long)y * op.sw + x;
+    float v;
+    if (op.fmt == TWX_FMT_U8) {
+        uchar4 p = reinterpret_cast<const uchar4*>(op.src)[i];
+        float r = (float)p.y, g = (float)p.z, b = (float)p.w;
+        v = (r * 0.3f + g * 0.59f) + b * 0.11f;
+    } else if (op.fmt == TWX_FMT_U16) {
+        ushort4 p = reinterpret_cast<const ushort4*>(op.src)[i];
+        double r = p.y, g = p.z, b = p.w;
+        v = (float)(((g * 0.59 + r * 0.3) + b * 0.11) / 257.0);
+    } else {
+        float4 p = reinterpret_cast<const float4*>(op.src)[i];
+        double r = p.y, g = p.z, b = p.w;
+        v = (float)(((g * 0.59 + r * 0.3) + b * 0.11) * 255.0);
+    }
+    op.dst[i] = v;
+}
+
+void launch_luma(hipStream_t s, const PlaneOp* d_ops, int n, int maxw, int maxh) {
+    k_luma<<<dim3((maxw + 255) / 256, maxh, n), dim3(256), 0, s>>>(d_ops);
+}
+
+// -------------------------------------------------------------------------------------------
+// Separable convolution (flow/conv.rs). The element arithmetic is shared by the pyramid planes
+// and the solver's buffers.
+
+// conv.rs:99-141 border element; get(t) gives the sample or false outside the line.
+template <int C, bool LEFT_FIRST>
+__device__ __forceinline__ float border_elem(const float* __restrict__ line, long stride, int len, int i, const Taps& t) {
+    // `line` points at element 0 of the line for this channel; `stride` steps one pixel.
+    float num = t.k[0] * line[(long)i * stride];
+    float den = t.k[0];
+#pragma unroll
+    for (int j = MAX_TAPS - 1; j >= 1; j--) {
+        if (j >= t.n) continue;
+        float kj = t.k[j];
+#pragma unroll
+        for (int side = 0; side < 2; side++) {
+            bool right = (side == 0) != LEFT_FIRST;
+            int p = right ? i + j : i - j;
+            bool in = p >= 0 && p < len;
+            float xv, w;
+            if (C == 1) {
+                if (in) { xv = line[(long)p * stride]; w = kj; }
+                else { xv = 0.0f; w = 0.0f; }
+            } else {
+                float v = in ? line[(long)p * stride] : -1.0f;
+                float mask = (v != -1.0f) ? 1.0f : 0.0f;
+                xv = v;
+                w = mask * kj;
+            }
+            den = den + w;
+            num = num + xv * w;
+        }
+    }
+    return num / den;
+}
+
+// One output element of the horizontal pass (conv.rs:144-199) for pixel x, channel c.
+template <int C>
+__device__ __forceinline__ float hpass_elem(const float* __restrict__ row, int w, int x, int c, const Taps& t) {
+    int n = t.n;
+    int head = min(n, w);
+    int interior = w > 2 * n ? w - 2 * n : 0;
+    if (x >= head && x < head + interior) {
+        const float* p = row + (long)x * C + c;
+        float s = p[0] * t.k[0];
+#pragma unroll
+        for (int j = MAX_TAPS - 1; j >= 1; j--) {
+            if (j >= n) continue;
+            s = s + (p[j * C] + p[-j * C]) * t.k[j];
+        }
+        return s;
+    }
+    return border_elem<C, false>(row + c, C, w, x, t);
+}
+
+// One output element of the vertical pass, border rows above first (conv.rs:272-311, 584-606).
+template <int C>
+__device__ __forceinline__ float vpass_elem(const float* __restrict__ src, int w, int h, int x, int y, int c, const Taps& t) {
+    int n = t.n;
+    int head = min(n, h);
+    int interior = h > 2 * n ? h - 2 * n : 0;
+    long row = (long)w * C;
+    const float* col = src + (long)x * C + c;
+    if (y >= head && y < head + interior) {
+        const float* p = col + (long)y * row;
+        float s = p[0] * t.k[0];
+#pragma unroll
+        for (int j = MAX_TAPS - 1; j >= 1; j--) {
+            if (j >= n) continue;
+            s = s + (p[j * row] + p[-j * row]) * t.k[j];
+        }
+        return s;
+    }
+    return border_elem<C, true>(col, row, h, y, t);
+}
+
+__global__ void k_hpass1(const PlaneOp* __restrict__ ops, Taps t) {
+    const PlaneOp op = ops[blockIdx.z];
+    int x = blockIdx.x * blockDim.x + threadIdx.x;
+    int y = blockIdx.y;
+    if (x >= op.sw || y >= op.sh) return;
+    const float* row = (const float*)op.src + (long)y * op.sw;
+    op.dst[(long)y * op.sw + x] = hpass_elem<1>(row, op.sw, x, 0, t);
+}
+
+__global__ void k_vpass1(const PlaneOp* __restrict__ ops, Taps t) {
+    const PlaneOp op = ops[blockIdx.z];
+    int x = blockIdx.x * blockDim.x + threadIdx.x;
+    int y = blockIdx.y;
+    if (x >= op.sw || y >= op.sh) return;
+    op.dst[(long)y * op.sw + x] = vpass_elem<1>((const float*)op.src, op.sw, op.sh, x, y, 0, t);
+}
+
+void launch_hpass1(hipStream_t s, const PlaneOp* d_ops, int n, int maxw, int maxh, const Taps& t) {
+    k_hpass1<<<dim3((maxw + 255) / 256, maxh, n), dim3(256), 0, s>>>(d_ops, t);
+}
+void launch_vpass1(hipStream_t s, const PlaneOp* d_ops, int n, int maxw, int maxh, const Taps& t) {
+    k_vpass1<<<dim3((maxw + 255) / 256, maxh, n), dim3(256), 0, s>>>(d_ops, t);
+}
+
+// -------------------------------------------------------------------------------------------
+// Samplers (flow/sample.rs:31-112)
+
+__device__ __forceinline__ void cell(float x, int len, int& io, float& fo) {
+    int32_t i = cvtt(x);
+    float f = x - (float)i;
+    int32_t last = len - 1;
+    if (i >= last) { i = last - 1; f = 1.0f; }
+    int ii = i < 0 ? 0 : i;
+    int cap = len >= 2 ? len - 2 : 0;
+    io = min(ii, cap);
+    fo = f;
+}
+
+__device__ __forceinline__ float bilinear1(const float* __restrict__ img, int w, int h, float x, float y) {
+    int ix, iy;
+    float fx, fy;
+    cell(x, w, ix, fx);
+    cell(y, h, iy, fy);
+    long i = (long)iy * w + ix;
+    float p00 = img[i], p10 = img[i + 1], p01 = img[i + w], p11 = img[i + w + 1];
+    float top = (p10 - p00) * fx + p00;
+    float bot = (p11 - p01) * fx + p01;
+    return (bot - top) * fy + top;
+}
+
+__global__ void k_resample(const PlaneOp* __restrict__ ops) {
+    const PlaneOp op = ops[blockIdx.z];
+    int x = blockIdx.x * blockDim.x + threadIdx.x;
+    int y = blockIdx.y;
+    if (x >= op.dw || y >= op.dh) return;
+    float sx = ((float)op.sw - 1.0f) / (float)(op.dw - 1);
+    float sy = ((float)op.sh - 1.0f) / (float)(op.dh - 1);
+    float fy = (float)y * sy;
+    op.dst[(long)y * op.dw + x] = bilinear1((const float*)op.src, op.sw, op.sh, (float)x * sx, fy);
+}
+
+// dst = dst - src (flow.rs:228-231, band = orig - blur5(orig))
+__global__ void k_subtract(const PlaneOp* __restrict__ ops) {
+    const PlaneOp op = ops[blockIdx.z];
+    int x = blockIdx.x * blockDim.x + threadIdx.x;
+    int y = blockIdx.y;
+    if (x >= op.sw || y >= op.sh) return;
+    long i = (long)y * op.sw + x;
+    op.dst[i] = op.dst[i] - ((const float*)op.src)[i];
+}
+
+void launch_resample(hipStream_t s, const PlaneOp* d_ops, int n, int maxw, int maxh) {
+    k_resample<<<dim3((maxw + 255) / 256, maxh, n), dim3(256), 0, s>>>(d_ops);
+}
+void launch_subtract(hipStream_t s, const PlaneOp* d_ops, int n, int maxw, int maxh) {
+    k_subtract<<<dim3((maxw + 255) / 256, maxh, n), dim3(256), 0, s>>>(d_ops);
+}
+
+// -------------------------------------------------------------------------------------------
+// Solver kernels (flow.rs:281-442), one launch per step and level over all jobs
+
+#define JOB_PROLOGUE()                                                    \
+    const SolveJob& J = jobs[blockIdx.z];                                 \
+    if (level > J.start) return;                                          \
+    const int w = J.w[level], h = J.h[level];                             \
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;                  \
+    const int y = blockIdx.y;                                             \
+    if (x >= w || y >= h) return;
+
+// calc_grad + calc_a (flow.rs:283-328)
+__global__ void k_grad_tensor(const SolveJob* __restrict__ jobs, int level) {
+    JOB_PROLOGUE();
+    const float* img = J.X[level];
+    long i = (long)y * w + x;
+    float c = img[i];
+    float nx = 0.0f, dx = 0.0f, ny = 0.0f, dy = 0.0f;
+    if (x >= 1) { nx = (c - img[i - 1]) + 0.0f; dx = 1.0f; }
+    if (y >= 1) { ny = (c - img[i - w]) + 0.0f; dy = 1.0f; }
+    if (x + 1 <= w - 1) { nx = nx + (img[i + 1] - c); dx = dx + 1.0f; }
+    if (y + 1 <= h - 1) { ny = ny + (img[i + w] - c); dy = dy + 1.0f; }
+    float gx = nx / dx, gy = ny / dy;
+    J.grad[i * 2] = gx;
+    J.grad[i * 2 + 1] = gy;
+    J.ta[i * 3] = gx * gx;
+    J.ta[i * 3 + 1] = gx * gy;
+    J.ta[i * 3 + 2] = gy * gy;
+}
+
+// buffers of a job the in-place Gaussians run on
+enum Buf { BUF_TA = 0, BUF_TB = 1, BUF_FLOW = 2 };
+
+__device__ __forceinline__ float* job_buf(const SolveJob& J, int which, int level) {
+    return which == BUF_TA ? J.ta : (which == BUF_TB ? J.tb : J.flow[level & 1]);
+}
+
+template <int C>
+__global__ void k_job_hpass(const SolveJob* __restrict__ jobs, int level, int which, Taps t) {
+    JOB_PROLOGUE();
+    const float* src = job_buf(J, which, level);
+    const float* row = src + (long)y * w * C;
+#pragma unroll
+    for (int c = 0; c < C; c++) J.tmp[((long)y * w + x) * C + c] = hpass_elem<C>(row, w, x, c, t);
+}
+
+template <int C>
+__global__ void k_job_vpass(const SolveJob* __restrict__ jobs, int level, int which, Taps t) {
+    JOB_PROLOGUE();
+    float* dst = job_buf(J, which, level);
+#pragma unroll
+    for (int c = 0; c < C; c++) dst[((long)y * w + x) * C + c] = vpass_elem<C>(J.tmp, w, h, x, y, c, t);
+}
+
+// calc_b (flow.rs:334-357): I1 = X (imgA), I0 = Y (imgB)
+__global__ void k_calc_b(const SolveJob* __restrict__ jobs, int level) {
+    JOB_PROLOGUE();
+    const float* flow = J.flow[level & 1];
+    long i = (long)y * w + x;
+    float fx = (float)x, fy = (float)y;
+    float sx = fx - flow[i * 2];
+    float sy = fy - flow[i * 2 + 1];
+    float fw = (float)w, fh = (float)h;
+    float bx = 0.0f, by = 0.0f;
+    if (!(sx < 0.0f || fw - 1.0f < sx || sy < 0.0f || fh - 1.0f < sy)) {
+        float s = bilinear1(J.Y[level], w, h, sx, sy);
+        float r = s - J.X[level][i];
+        bx = r * J.grad[i * 2];
+        by = r * J.grad[i * 2 + 1];
+    }
+    J.tb[i * 2] = bx;
+    J.tb[i * 2 + 1] = by;
+}
+
+// new_disp (flow.rs:361-371)
+__global__ void k_new_disp(const SolveJob* __restrict__ jobs, int level) {
+    JOB_PROLOGUE();
+    float* flow = J.flow[level & 1];
+    long i = (long)y * w + x;
+    float aa = J.ta[i * 3], ab = J.ta[i * 3 + 1], ad = J.ta[i * 3 + 2];
+    float bx = J.tb[i * 2], by = J.tb[i * 2 + 1];
+    float t = ab * ab - ad * aa;
+    if (fabsf(t) > 40.0f) {
+        flow[i * 2] = (ab * by - ad * bx) / t + flow[i * 2];
+        flow[i * 2 + 1] = (ab * bx - aa * by) / t + flow[i * 2 + 1];
+    }
+}
+
+// all_flow_out_of_range (flow.rs:431-442): any pixel with 0 <= x+u <= w-1 and 0 <= y+v <= h-1
+__global__ void k_range_any(const SolveJob* __restrict__ jobs, int level) {
+    JOB_PROLOGUE();
+    if (J.state[0] == 0) return;
+    const float* flow = J.flow[level & 1];
+    long i = (long)y * w + x;
+    float px = (float)x + flow[i * 2];
+    float py = (float)y + flow[i * 2 + 1];
+    bool in = 0.0f <= px && px <= (float)(w - 1) && 0.0f <= py && py <= (float)(h - 1);
+    if (in) atomicOr(&J.state[1], 1);
+}
+
+// the range check result, then the damping at the start level (flow.rs:1291-1300)
+__global__ void k_range_apply(const SolveJob* __restrict__ jobs, int level) {
+    JOB_PROLOGUE();
+    float* flow = J.flow[level & 1];
+    long i = (long)y * w + x;
+    float u = flow[i * 2], v = flow[i * 2 + 1];
+    if (J.state[0] != 0 && J.state[1] == 0) { u = 0.0f; v = 0.0f; }
+    if (level == J.start && J.scale_damp) { u = J.damp * u; v = J.damp * v; }
+    flow[i * 2] = u;
+    flow[i * 2 + 1] = v;
+}
+
+__global__ void k_range_update(const SolveJob* __restrict__ jobs, int njobs, int level) {
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= njobs) return;
+    const SolveJob& J = jobs[j];
+    if (level > J.start) return;
+    if (J.state[0] != 0 && J.state[1] != 0) J.state[0] = 0;
+    J.state[1] = 0;
+}
+
+// upsample_flow (flow.rs:376-391) from level l into level l-1's buffer
+__global__ void k_upsample(const SolveJob* __restrict__ jobs, int level) {
+    const SolveJob& J = jobs[blockIdx.z];
+    if (level > J.start || level == 0) return;
+    const int sw = J.w[level], sh = J.h[level];
+    const int dw = J.w[level - 1], dh = J.h[level - 1];
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = blockIdx.y;
+    if (x >= dw || y >= dh) return;
+    const float* src = J.flow[level & 1];
+    float* dst = J.flow[(level - 1) & 1];
+    float sx = ((float)dw - 1.0f) / (float)(sw - 1);
+    float sy = ((float)dh - 1.0f) / (float)(sh - 1);
+    float rx = (float)(1.0 / (double)sx);
+    float ry = (float)(1.0 / (double)sy);
+    float fx = (float)x * rx, fy = (float)y * ry;
+    int ix, iy;
+    float ax, ay;
+    cell(fx, sw, ix, ax);
+    cell(fy, sh, iy, ay);
+    long i = ((long)iy * sw + ix) * 2;
+    long r = (long)sw * 2;
+    float out[2];
+#pragma unroll
+    for (int c = 0; c < 2; c++) {
+        float p00 = src[i + c], p10 = src[i + 2 + c], p01 = src[i + r + c], p11 = src[i + r + 2 + c];
+        float top = (p10 - p00) * ax + p00;
+        float bot = (p11 - p01) * ax + p01;
+        out[c] = (bot - top) * ay + top;
+    }
+    long o = ((long)y * dw + x) * 2;
+    dst[o] = out[0] * sx;
+    dst[o + 1] = out[1] * sy;
+}
+
+// final negation (flow.rs:1305) and the NaN scan the reference device path applies to the
+// downloaded flow (device/solver/mod.rs:575-582)
+__global__ void k_negate(const SolveJob* __restrict__ jobs) {
+    const SolveJob& J = jobs[blockIdx.z];
+    const int w = J.w[0], h = J.h[0];
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = blockIdx.y;
+    if (x >= w || y >= h) return;
+    float* flow = J.flow[0];
+    long i = ((long)y * w + x) * 2;
+    float u = -1.0f * flow[i], v = -1.0f * flow[i + 1];
+    flow[i] = u;
+    flow[i + 1] = v;
+    if (isnan(u) || isnan(v)) atomicOr(&J.state[2], 1);
+}
+
+// -------------------------------------------------------------------------------------------
+// Block matching (flow.rs:446-622)
+
+// Block::cost (flow.rs:472-524); false for None
+__device__ __forceinline__ bool bm_cost(const float* __restrict__ b, int w, int h, const float p[9], float x2, float y2,
+                                        float& out) {
+    float wm2 = (float)(w - 2);
+    float hm2 = (float)(h - 2);
+    if (x2 > wm2 || y2 > hm2) return false;
+    int32_t ix, iy;
+    float fx, fy;
+    if (x2 == wm2) { ix = cvtt(wm2 - 1.0f); fx = 1.0f; }
+    else { ix = cvtt(x2); fx = x2 - (float)ix; }
+    if (y2 == hm2) { iy = cvtt(hm2 - 1.0f); fy = 1.0f; }
+    else { iy = cvtt(y2); fy = y2 - (float)iy; }
+    if (ix < 1 || iy < 1) return false;
+    float w11 = fy * fx;
+    float w10 = fx - w11;
+    float w01 = fy - w11;
+    float w00 = ((1.0f - fx) - fy) + w11;
+    const float* base = b + ((long)iy - 1) * w + (ix - 1);
+    float sq[9];
+#pragma unroll
+    for (int bb = 0; bb < 3; bb++) {
+#pragma unroll
+        for (int aa = 0; aa < 3; aa++) {
+            const float* t0 = base + (long)bb * w + aa;
+            const float* t1 = t0 + w;
+            float q = ((w00 * t0[0] + w10 * t0[1]) + w01 * t1[0]) + w11 * t1[1];
+            float d = p[bb * 3 + aa] - q;
+            sq[bb * 3 + aa] = d * d;
+        }
+    }
+    float s = sq[0] + sq[1];
+#pragma unroll
+    for (int k = 2; k < 8; k++) s = s + sq[k];
+    out = sq[8] + s;
+    return true;
+}
+
+// Block::step (flow.rs:528-544): true when the neighbour's vector replaces the pixel's
+__device__ __forceinline__ bool bm_step(const float* __restrict__ a, const float* __restrict__ b, int w, int h, int x, int y,
+                                        float cu, float cv, float nu, float nv) {
+    float tx = (float)x - cu;
+    float ty = (float)y - cv;
+    float w1 = (float)(w - 1), h1 = (float)(h - 1);
+    if (!(0.0f <= tx && tx <= w1 && 0.0f <= ty && ty <= h1)) return false;
+    float p[9];
+    const float* pa = a + ((long)y - 1) * w + (x - 1);
+#pragma unroll
+    for (int r = 0; r < 3; r++) {
+        p[r * 3] = pa[(long)r * w];
+        p[r * 3 + 1] = pa[(long)r * w + 1];
+        p[r * 3 + 2] = pa[(long)r * w + 2];
+    }
+    float c0, c1;
+    if (!bm_cost(b, w, h, p, tx, ty, c0)) return false;
+    if (!bm_cost(b, w, h, p, (float)x - nu, (float)y - nv, c1)) return false;
+    return c1 < c0;
+}
+
+// Sweeps 1 (DIR = +1, left to right) and 2 (DIR = -1): one chain per interior row
+template <int DIR>
+__global__ void k_bm_rows(const SolveJob* __restrict__ jobs, int level) {
+    const SolveJob& J = jobs[blockIdx.z];
+    if (level > J.start) return;
+    const int w = J.w[level], h = J.h[level];
+    if (w < 3 || h < 3) return;
+    int y = 1 + blockIdx.x * blockDim.x + threadIdx.x;
+    if (y > h - 2) return;
+    float* row = J.flow[level & 1] + (long)y * w * 2;
+    const float* A = J.X[level];
+    const float* B = J.Y[level];
+    int x = DIR > 0 ? 1 : w - 2;
+    float nu = row[(x - DIR) * 2], nv = row[(x - DIR) * 2 + 1];
+    for (int k = 0; k < w - 2; k++, x += DIR) {
+        float cu = row[x * 2], cv = row[x * 2 + 1];
+        if (bm_step(A, B, w, h, x, y, cu, cv, nu, nv)) {
+            row[x * 2] = nu;
+            row[x * 2 + 1] = nv;
+        } else {
+            nu = cu;
+            nv = cv;
+        }
+    }
+}
+
+// Sweeps 3 (DIR = +1, top to bottom) and 4 (DIR = -1): one chain per interior column
+template <int DIR>
+__global__ void k_bm_cols(const SolveJob* __restrict__ jobs, int level) {
+    const SolveJob& J = jobs[blockIdx.z];
+    if (level > J.start) return;
+    const int w = J.w[level], h = J.h[level];
+    if (w < 3 || h < 3) return;
+    int x = 1 + blockIdx.x * blockDim.x + threadIdx.x;
+    if (x > w - 2) return;
+    float* flow = J.flow[level & 1];
+    const float* A = J.X[level];
+    const float* B = J.Y[level];
+    int y = DIR > 0 ? 1 : h - 2;
+    long ni = ((long)(y - DIR) * w + x) * 2;
+    float nu = flow[ni], nv = flow[ni + 1];
+    for (int k = 0; k < h - 2; k++, y += DIR) {
+        long ci = ((long)y * w + x) * 2;
+        float cu = flow[ci], cv = flow[ci + 1];
+        if (bm_step(A, B, w, h, x, y, cu, cv, nu, nv)) {
+            flow[ci] = nu;
+            flow[ci + 1] = nv;
+        } else {
+            nu = cu;
+            nv = cv;
+        }
+    }
+}
+
+// -------------------------------------------------------------------------------------------
+// The solve driver: per level, the kernel sequence of flow.rs:1271-1304 over every job.
+
+void launch_solve(hipStream_t s, const SolveJob* d_jobs, const std::vector<SolveJob>& jobs, bool skip_refine) {
+    int nj = (int)jobs.size();
+    if (nj == 0) return;
+    int max_start = 0;
+    for (auto& j : jobs) max_start = std::max(max_start, j.start);
+    const Taps g7 = gaussian_taps(7.0f), g5 = gaussian_taps(5.0f);
+    for (int l = max_start; l >= 0; l--) {
+        int mw = 0, mh = 0;
+        for (auto& j : jobs)
+            if (l <= j.start) { mw = std::max(mw, j.w[l]); mh = std::max(mh, j.h[l]); }
+        dim3 blk(256), grd((mw + 255) / 256, mh, nj);
+        k_grad_tensor<<<grd, blk, 0, s>>>(d_jobs, l);
+        k_job_hpass<3><<<grd, blk, 0, s>>>(d_jobs, l, BUF_TA, g7);
+        k_job_vpass<3><<<grd, blk, 0, s>>>(d_jobs, l, BUF_TA, g7);
+        for (int it = 0; it < 3; it++) {
+            k_calc_b<<<grd, blk, 0, s>>>(d_jobs, l);
+            k_job_hpass<2><<<grd, blk, 0, s>>>(d_jobs, l, BUF_TB, g7);
+            k_job_vpass<2><<<grd, blk, 0, s>>>(d_jobs, l, BUF_TB, g7);
+            k_new_disp<<<grd, blk, 0, s>>>(d_jobs, l);
+            k_job_hpass<2><<<grd, blk, 0, s>>>(d_jobs, l, BUF_FLOW, g5);
+            k_job_vpass<2><<<grd, blk, 0, s>>>(d_jobs, l, BUF_FLOW, g5);
+            if ((it == 1 && l == 0 && !skip_refine)) {
+                dim3 rb(64), rg((mh + 63) / 64, 1, nj), cg((mw + 63) / 64, 1, nj);
+                k_bm_rows<1><<<rg, rb, 0, s>>>(d_jobs, l);
+                k_bm_rows<-1><<<rg, rb, 0, s>>>(d_jobs, l);
+                k_bm_cols<1><<<cg, rb, 0, s>>>(d_jobs, l);
+                k_bm_cols<-1><<<cg, rb, 0, s>>>(d_jobs, l);
+            }
+        }
+        if (l != 0 && !skip_refine) {
+            dim3 rb(64), rg((mh + 63) / 64, 1, nj), cg((mw + 63) / 64, 1, nj);
+            k_bm_rows<1><<<rg, rb, 0, s>>>(d_jobs, l);
+            k_bm_rows<-1><<<rg, rb, 0, s>>>(d_jobs, l);
+            k_bm_cols<1><<<cg, rb, 0, s>>>(d_jobs, l);
+            k_bm_cols<-1><<<cg, rb, 0, s>>>(d_jobs, l);
+        }
+        k_range_any<<<grd, blk, 0, s>>>(d_jobs, l);
+        k_range_apply<<<grd, blk, 0, s>>>(d_jobs, l);
+        k_range_update<<<dim3((nj + 63) / 64), dim3(64), 0, s>>>(d_jobs, nj, l);
+        if (l > 0) {
+            int uw = 0, uh = 0;
+            for (auto& j : jobs)
+                if (l <= j.start) { uw = std::max(uw, j.w[l - 1]); uh = std::max(uh, j.h[l - 1]); }
+            k_upsample<<<dim3((uw + 255) / 256, uh, nj), blk, 0, s>>>(d_jobs, l);
+        }
+    }
+    int w0 = 0, h0 = 0;
+    for (auto& j : jobs) { w0 = std::max(w0, j.w[0]); h0 = std::max(h0, j.h[0]); }
+    k_negate<<<dim3((w0 + 255) / 256, h0, nj), dim3(256), 0, s>>>(d_jobs);
+}
+
+__global__ void k_nan_scan(const float* __restrict__ d, size_t n, int* flag) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n && isnan(d[i])) atomicOr(flag, 1);
+}
+void launch_nan_scan(hipStream_t s, const float* data, size_t n, int* flag) {
+    if (n == 0) return;
+    k_nan_scan<<<dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s>>>(data, n, flag);
+}
+
+// -------------------------------------------------------------------------------------------
+// Inverse warp (warp.rs:99-786, hold edges), blend (blend.rs:55-257), finish (image.rs:205-245)
+
+// warp.rs:178-215
+__device__ __forceinline__ bool clip_to_rect(float x, float y, float& ix, float& iy, float w1, float h1) {
+    for (int n = 0; n < 4096; n++) {
+        if (ix < 0.0f) {
+            float dy = y - iy, dxs = x - ix, num = x - 0.0f;
+            ix = 0.0f;
+            iy = y - (num / dxs) * dy;
+            continue;
+        }
+        if (ix > w1) {
+            float dy = y - iy, dxs = x - ix, num = x - w1;
+            ix = w1;
+            iy = y - (num / dxs) * dy;
+            continue;
+        }
+        if (iy < 0.0f) {
+            float dyy = y - iy, num = y - 0.0f, dx = x - ix;
+            ix = x - (num / dyy) * dx;
+            iy = 0.0f;
+            continue;
+        }
+        if (iy > h1) {
+            float dyy = y - iy, num = y - h1, dx = x - ix;
+            ix = x - (num / dyy) * dx;
+            iy = h1;
+            continue;
+        }
+        return true;
+    }
+    return false;
+}
+
+template <int FMT> struct Px;
+template <> struct Px<0> { using T = uint8_t; };
+template <> struct Px<1> { using T = uint16_t; };
+template <> struct Px<2> { using T = float; };
+
+template <int FMT>
+__device__ __forceinline__ void load4(const void* img, long i, float o[4]) {
+    if (FMT == 0) {
+        uchar4 p = reinterpret_cast<const uchar4*>(img)[i];
+        o[0] = p.x; o[1] = p.y; o[2] = p.z; o[3] = p.w;
+    } else if (FMT == 1) {
+        ushort4 p = reinterpret_cast<const ushort4*>(img)[i];
+        o[0] = p.x; o[1] = p.y; o[2] = p.z; o[3] = p.w;
+    } else {
+        float4 p = reinterpret_cast<const float4*>(img)[i];
+        o[0] = p.x; o[1] = p.y; o[2] = p.z; o[3] = p.w;
+    }
+}
+
+// warp.rs:129-146 store rules
+__device__ __forceinline__ float sat_u8(float v) {
+    int32_t i = cvtt(v + 0.5f);
+    int32_t t = (int32_t)((uint32_t)i + (uint32_t)-128);
+    int32_t w = min(max(t, -32768), 32767);
+    int32_t b = min(max(w, -128), 127);
+    return (float)(b + 128);
+}
+__device__ __forceinline__ float sat_u16(float v) {
+    int32_t i = cvtt(v + 0.5f);
+    int32_t t = (int32_t)((uint32_t)i + (uint32_t)-32768);
+    int32_t w = min(max(t, -32768), 32767);
+    return (float)(w + 32768);
+}
+template <int FMT>
+__device__ __forceinline__ float store_rule(float v) {
+    return FMT == 0 ? sat_u8(v) : (FMT == 1 ? sat_u16(v) : v);
+}
+
+// One warped pixel W(src, flow, amount) at output (x, y) (warp.rs:477-512), as the stored sample
+// values (integer formats hold exact integers in the floats). Returns false for a NaN coordinate.
+template <int FMT>
+__device__ __forceinline__ bool warp_pixel(const void* src, int W, int H, const float* __restrict__ flow, int fw, int fh,
+                                           float amount, int x, int y, float o[4]) {
+    int w1i = W - 1, h1i = H - 1;
+    float rx = (float)(fw - 1) / (float)w1i;
+    float ry = (float)(fh - 1) / (float)h1i;
+    float sx = amount / rx, sy = amount / ry;
+    float xf = (float)x, yf = (float)y;
+    // Flow::bilerp (warp.rs:415-442)
+    float qx = xf * rx, qy = yf * ry;
+    int fw1 = fw - 1, fh1 = fh - 1;
+    int32_t jx = cvtt(qx), jy = cvtt(qy);
+    float gx = qx - (float)jx, gy = qy - (float)jy;
+    if (jx >= fw1) { jx = fw1 - 1; gx = 1.0f; }
+    if (jy >= fh1) { jy = fh1 - 1; gy = 1.0f; }
+    long f00 = ((long)jy * fw + jx) * 2, f01 = f00 + (long)fw * 2;
+    float d[2];
+#pragma unroll
+    for (int k = 0; k < 2; k++) {
+        float p00 = flow[f00 + k], p10 = flow[f00 + 2 + k], p01 = flow[f01 + k], p11 = flow[f01 + 2 + k];
+        float top = (p10 - p00) * gx + p00;
+        float bot = (p11 - p01) * gx + p01;
+        d[k] = (bot - top) * gy + top;
+    }
+    float ix = d[0] * sx + xf;
+    float iy = d[1] * sy + yf;
+    if (!clip_to_rect(xf, yf, ix, iy, (float)w1i, (float)h1i) || isnan(ix) || isnan(iy)) {
+        o[0] = o[1] = o[2] = o[3] = 0.0f;
+        return false;
+    }
+    // bilerp4 (warp.rs:306-331)
+    int32_t kx = cvtt(ix), ky = cvtt(iy);
+    float ax = ix - (float)kx, ay = iy - (float)ky;
+    if (kx >= w1i) { kx = w1i - 1; ax = 1.0f; }
+    if (ky >= h1i) { ky = h1i - 1; ay = 1.0f; }
+    long i00 = (long)ky * W + kx, i01 = i00 + W;
+    float p00[4], p10[4], p01[4], p11[4];
+    load4<FMT>(src, i00, p00);
+    load4<FMT>(src, i00 + 1, p10);
+    load4<FMT>(src, i01, p01);
+    load4<FMT>(src, i01 + 1, p11);
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        float top = p00[k] + (p10[k] - p00[k]) * ax;
+        float bot = p01[k] + (p11[k] - p01[k]) * ax;
+        o[k] = store_rule<FMT>(top + (bot - top) * ay);
+    }
+    return true;
+}
+
+// lerp of one pixel, weight on the first (blend.rs:180-257)
+template <int FMT>
+__device__ __forceinline__ void lerp_pixel(float f[4], const float s[4], float w) {
+    if (FMT == 2) {
+        float a = (f[0] - s[0]) * w + s[0];
+        if (!(a > 0.0f)) { f[0] = f[1] = f[2] = f[3] = 0.0f; return; }
+        float r = (f[1] - s[1]) * w + s[1];
+        float g = (f[2] - s[2]) * w + s[2];
+        float b = (f[3] - s[3]) * w + s[3];
+        if (a > 1.0f) { r = r / a; g = g / a; b = b / a; a = 1.0f; }
+        f[1] = r + 0.0f; f[2] = g + 0.0f; f[3] = b + 0.0f; f[0] = a + 0.0f;
+    } else {
+        const float scale = FMT == 0 ? 255.0f : 65535.0f;
+        const int64_t mask = FMT == 0 ? 0xff : 0xffff;
+        float a = (float)((int32_t)f[0] - (int32_t)s[0]) * w + s[0];
+        float n = a / scale;
+        if (!(n > 0.0f)) { f[0] = f[1] = f[2] = f[3] = (float)(cvtt64f(0.0f) & mask); return; }
+        float r = (float)((int32_t)f[1] - (int32_t)s[1]) * w + s[1];
+        float g = (float)((int32_t)f[2] - (int32_t)s[2]) * w + s[2];
+        float b = (float)((int32_t)f[3] - (int32_t)s[3]) * w + s[3];
+        if (n > 1.0f) { r = r / n; g = g / n; b = b / n; a = scale; }
+        auto clamp = [&](float c) { return c > a ? a : (0.0f > c ? 0.0f : c); };
+        r = clamp(r); g = clamp(g); b = clamp(b);
+        f[1] = (float)(cvtt64f(r + 0.5f) & mask);
+        f[2] = (float)(cvtt64f(g + 0.5f) & mask);
+        f[3] = (float)(cvtt64f(b + 0.5f) & mask);
+        f[0] = (float)(cvtt64f(a + 0.5f) & mask);
+    }
+}
+
+// blend(out, first, second, w) of one pixel (blend.rs:55-114); mode decided on the host
+template <int FMT>
+__device__ __forceinline__ void blend_pixel(const float p[4], const float q[4], int mode, float w, float lw, float o[4]) {
+    if (mode == 0) { for (int k = 0; k < 4; k++) o[k] = p[k]; return; }
+    if (mode == 1) { for (int k = 0; k < 4; k++) o[k] = q[k]; return; }
+    if (mode == 2) {
+        for (int k = 0; k < 4; k++) o[k] = p[k];
+        if (lw == 1.0f) return;
+        if (lw == 0.0f) { for (int k = 0; k < 4; k++) o[k] = q[k]; return; }
+        lerp_pixel<FMT>(o, q, lw);
+        return;
+    }
+    const int64_t mask = FMT == 0 ? 0xff : 0xffff;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        if (FMT == 2) o[k] = ((q[k] - p[k]) * w + p[k]) + 0.0f;
+        else o[k] = (float)(cvtt64f(((q[k] - p[k]) * w + p[k]) + 0.5f) & mask);
+    }
+}
+
+// Image::to_frame of one pixel into the wire frame (image.rs:300-351, frame.rs:113-141)
+template <int FMT>
+__device__ __forceinline__ void finish_pixel(void* out, long n, long i, const float e[4]) {
+    if (FMT == 0) {
+        uint8_t p[4] = {(uint8_t)e[0], (uint8_t)e[1], (uint8_t)e[2], (uint8_t)e[3]};
+        uint8_t o[4] = {p[0], p[1], p[2], p[3]};
+        if (p[0] != 0) {
+            float s = 255.0f / (float)p[0];
+            for (int c = 1; c < 4; c++) {
+                float v = (float)p[c] * s;
+                if (v > 255.0f) v = 255.0f;
+                o[c] = (uint8_t)qround(v);
+            }
+        }
+        reinterpret_cast<uchar4*>(out)[i] = make_uchar4(o[1], o[2], o[3], o[0]);
+    } else if (FMT == 1) {
+        uint16_t p[4] = {(uint16_t)e[0], (uint16_t)e[1], (uint16_t)e[2], (uint16_t)e[3]};
+        uint16_t o[4] = {p[0], p[1], p[2], p[3]};
+        if (p[0] != 0) {
+            float a = (float)p[0];
+            float s = 32768.0f / a;
+            o[0] = (uint16_t)qround(a * bits_f(0x3f000080u));
+            for (int c = 1; c < 4; c++) o[c] = (uint16_t)qround((float)p[c] * s);
+        }
+        auto wire = [](uint16_t v) -> uint16_t {
+            uint32_t m = v < 32768u ? v : 32768u;
+            return (uint16_t)((m * 65535u + 16384u) / 32768u);
+        };
+        reinterpret_cast<ushort4*>(out)[i] = make_ushort4(wire(o[1]), wire(o[2]), wire(o[3]), wire(o[0]));
+    } else {
+        float o[4];
+        if ((double)e[0] < 1e-12) {
+            o[0] = o[1] = o[2] = o[3] = 0.0f;
+        } else {
+            float inv = 1.0f / e[0];
+            o[0] = e[0];
+            for (int c = 1; c < 4; c++) o[c] = e[c] * inv;
+        }
+        float* pl = reinterpret_cast<float*>(out);  // planes G, B, R, A
+        pl[i] = o[2];
+        pl[n + i] = o[3];
+        pl[2 * n + i] = o[1];
+        pl[3 * n + i] = o[0];
+    }
+}
+
+template <int FMT>
+__global__ void k_frames(const FrameJob* __restrict__ jobs) {
+    const FrameJob& F = jobs[blockIdx.z];
+    if (F.fmt != FMT) return;
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = blockIdx.y;  // motion row
+    if (x >= F.W || y >= F.H) return;
+    long n = (long)F.W * F.H;
+    long mi = (long)y * F.W + x;
+    float pa[4], pb[4], e[4];
+    bool ok = true;
+    if (F.kind == 0) {
+        load4<FMT>(F.a, mi, pa);
+        load4<FMT>(F.b, mi, pb);
+        if (FMT == 2 && F.plus_zero) {
+            for (int k = 0; k < 4; k++) { pa[k] = pa[k] + 0.0f; pb[k] = pb[k] + 0.0f; }
+        }
+        blend_pixel<FMT>(pa, pb, F.blend_mode, F.weight, F.lerp_w, e);
+    } else if (F.kind == 1) {
+        if (F.nearest_from_a) {
+            if (F.warp_a_copy) load4<FMT>(F.a, mi, e);
+            else ok = warp_pixel<FMT>(F.a, F.W, F.H, F.f_ba, F.fw, F.fh, F.frac, x, y, e);
+        } else {
+            if (F.warp_b_copy) load4<FMT>(F.b, mi, e);
+            else ok = warp_pixel<FMT>(F.b, F.W, F.H, F.f_ab, F.fw, F.fh, F.wa, x, y, e);
+        }
+    } else {
+        if (F.warp_a_copy) load4<FMT>(F.a, mi, pa);
+        else ok = warp_pixel<FMT>(F.a, F.W, F.H, F.f_ba, F.fw, F.fh, F.frac, x, y, pa);
+        if (F.warp_b_copy) load4<FMT>(F.b, mi, pb);
+        else ok = warp_pixel<FMT>(F.b, F.W, F.H, F.f_ab, F.fw, F.fh, F.wa, x, y, pb) && ok;
+        blend_pixel<FMT>(pa, pb, F.blend_mode, F.weight, F.lerp_w, e);
+    }
+    if (!ok) atomicOr(F.bad, 1);
+    if (!F.finish) {
+        typename Px<FMT>::T* o = reinterpret_cast<typename Px<FMT>::T*>(F.out) + mi * 4;
+        for (int k = 0; k < 4; k++) o[k] = (typename Px<FMT>::T)e[k];
+        return;
+    }
+    // the composed image is in motion rows: row y is the output's row H-1-y
+    long oi = (long)(F.H - 1 - y) * F.W + x;
+    finish_pixel<FMT>(F.out, n, oi, e);
+}
+
+void launch_frames(hipStream_t s, const FrameJob* d_jobs, const std::vector<FrameJob>& jobs) {
+    if (jobs.empty()) return;
+    int mw = 0, mh = 0;
+    bool has[3] = {false, false, false};
+    for (auto& f : jobs) { mw = std::max(mw, f.W); mh = std::max(mh, f.H); has[f.fmt] = true; }
+    dim3 blk(256), grd((mw + 255) / 256, mh, (unsigned)jobs.size());
+    if (has[0]) k_frames<0><<<grd, blk, 0, s>>>(d_jobs);
+    if (has[1]) k_frames<1><<<grd, blk, 0, s>>>(d_jobs);
+    if (has[2]) k_frames<2><<<grd, blk, 0, s>>>(d_jobs);
+}
+
+// -------------------------------------------------------------------------------------------
+// Taps (conv.rs:44-86), on the host
+
+Taps gaussian_taps(float r) {
+    Taps t{};
+    int n = std::max((int32_t)r + 1, 0);
+    float s3 = r / 3.0f;
+    float d = s3 * s3;
+    float sum = 0.0f;
+    for (int i = 0; i < n; i++) {
+        float fi = (float)i;
+        float e = -((fi * fi) / d);
+        float v = (float)std::pow(2.0, (double)e);
+        t.k[i] = v;
+        sum += v;
+    }
+    t.n = n;
+    if (n == 0) return t;
+    float den = sum * 2.0f - t.k[0];
+    for (int i = 0; i < n; i++) t.k[i] /= den;
+    return t;
+}
+
+Taps binomial_taps() {
+    Taps t{};
+    t.k[0] = 0.375f;
+    t.k[1] = 0.25f;
+    t.k[2] = 0.0625f;
+    t.n = 3;
+    return t;
+}
+
+}  // namespace twx
