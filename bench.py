@@ -1,0 +1,173 @@
+"""Benchmark: 1080p retime output frames/s on one MI355X.
+
+Workload: `vfx apply retime` on a synthetic 1920x1080 RGBA float32 clip of 120 frames with the
+keyed speed ramp 100 % @ 0, 25 % @ 60, 100 % @ 120 (defaults otherwise: Best vectors, Blend,
+sensitivity 70), 120 output frames. One step = the whole clip rendered through the HIP backend
+(frame selection, ingest, pyramids, every flow solve, warp+blend and finish) with the clip and the
+output resident in HBM.
+
+Prints ONE JSON line. `roofline` covers the dominant kernel (HIP-event timed around its launches
+inside the timed steps); `cpu_baseline` times the oracle (the C++ restatement of the reference
+CPU path) on a bounded sample of the same output frames and checks those frames' bytes against
+the device output.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+SPEED = "speed_percent=100@0;25@60;100@120"
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--frames", type=int, default=120)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0, help="budget of the CPU-baseline sample")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--per-pair", action="store_true", help="also time one launch sequence per output frame")
+    ap.add_argument("--no-profile", action="store_true", help="time the steps without the per-launch HIP events")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU: this build has no multi-GPU path (asked for {args.gpus}); "
+              "a clip shards by output-frame range, see DESIGN.md", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    from twixtor_amd import synth
+    from twixtor_amd.backend import HipBackend
+    from twixtor_amd.params import Retime
+
+    dev = torch.device("cuda:0")
+    spec = synth.spec_1080p(frames=args.frames)
+    gen = synth.Clip(spec, device=dev)
+    clip = torch.cat([gen.wire(t) for t in range(spec.frames)]).contiguous()
+    cfg = Retime(spec.width, spec.height, 32, "24", [SPEED])
+    params = cfg.struct()
+    fb = cfg.frame_bytes()
+    n_out = args.frames
+    out = torch.empty(fb * n_out, dtype=torch.uint8, device=dev)
+    torch.cuda.synchronize()
+
+    be = HipBackend(0)
+    be.profile(not args.no_profile)
+
+    def step():
+        be.retime_frames_dev(cfg, clip.data_ptr(), spec.frames, 0, n_out, out.data_ptr(), params)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    be.profile_reset()
+    s0 = be.stats()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    s1 = be.stats()
+    prof = be.profile_read()
+    ms_per_step = elapsed * 1000.0 / args.steps
+    fps = n_out * args.steps / elapsed
+
+    # dominant kernel by HIP-event time over the timed steps
+    kern = max(prof, key=lambda k: prof[k]["ms"]) if prof else None
+    roofline = None
+    if kern:
+        k = prof[kern]
+        avg_ms = k["ms"] / max(k["launches"], 1)
+        achieved = (k["bytes"] / max(k["launches"], 1)) / (avg_ms / 1000.0) / 1e9 if avg_ms > 0 else 0.0
+        roofline = {"bound": "hbm", "kernel": kern, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                    "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                    "avg_launch_ms": round(avg_ms, 4), "launches_per_step": k["launches"] // args.steps,
+                    "share_of_step": round(k["ms"] / args.steps / ms_per_step, 3)}
+    # whole-path algorithmic bytes per step over the step time
+    total_bytes = sum(v["bytes"] for v in prof.values()) / args.steps if prof else 0.0
+    path = {"algorithmic_GB_per_step": round(total_bytes / 1e9, 3),
+            "achieved_GBs": round(total_bytes / (ms_per_step / 1000.0) / 1e9, 1),
+            "kernels": {n: {"ms_per_step": round(v["ms"] / args.steps, 3),
+                            "GBs": round(v["bytes"] / (v["ms"] / 1000.0) / 1e9, 1) if v["ms"] > 0 else 0.0}
+                        for n, v in sorted(prof.items(), key=lambda kv: -kv[1]["ms"])}}
+    be.profile(False)
+
+    per_pair = None
+    if args.per_pair:
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for n in range(n_out):
+            be.retime_frames_dev(cfg, clip.data_ptr(), spec.frames, n, 1, out.data_ptr() + n * fb, params)
+        torch.cuda.synchronize()
+        pp = time.perf_counter() - t0
+        per_pair = {"ms_per_clip": round(pp * 1000.0, 2), "batched_speedup": round(pp * 1000.0 / ms_per_step, 2)}
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(cfg, clip, out, spec.frames, n_out, fb, args.cpu_seconds)
+
+    line = {
+        "metric": "1080p retime output frames/s",
+        "value": round(fps, 3),
+        "unit": "frames/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"retime 1920x1080 RGBA f32, {spec.frames} frames, {SPEED}, Best, Blend -> {n_out} output frames",
+                   "width": 1920, "height": 1080, "frames_in": spec.frames, "frames_out": n_out,
+                   "unique_solves_per_step": (s1["pair_solves"] - s0["pair_solves"]) // args.steps,
+                   "vector_frames_per_step": (s1["composes"] - s0["composes"]) // args.steps},
+        "roofline": roofline,
+        "path": path,
+        "per_pair": per_pair,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+def cpu_baseline(cfg, clip, out, n_in, n_out, fb, budget_s):
+    """The oracle on the host cores over a bounded, evenly spread sample of the output frames;
+    the sampled frames' bytes must equal the device output."""
+    import numpy as np
+
+    from oracle import cpuref
+
+    cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    cores = max(1, min(cores, 16))
+    cpuref.set_threads(cores)
+    host_clip = clip.cpu().numpy().tobytes()
+    host_out = out.cpu().numpy()
+    params = cfg.struct()
+    order = list(range(1, n_out, 7)) + [0]
+    done, t_sum, same = 0, 0.0, True
+    for n in order:
+        t0 = time.perf_counter()
+        got = cpuref.retime_frames(params, host_clip, n_in, n, 1)
+        t_sum += time.perf_counter() - t0
+        done += 1
+        if got != host_out[n * fb:(n + 1) * fb].tobytes():
+            same = False
+        if t_sum >= budget_s:
+            break
+    return {"value": round(done / t_sum, 4), "unit": "frames/s", "cores": cores, "kind": "port",
+            "sample": f"{done} of {n_out} output frames (every 7th), oracle rendering each frame whole as the CPU path does",
+            "bytes_equal_to_device": same}
+
+
+if __name__ == "__main__":
+    main()

@@ -160,6 +160,16 @@ int twx_get_stats(twx_handle* h, twx_stats* out);
 /* Device pointer of the handle's HIP stream (hipStream_t), so a caller can order its own work. */
 void* twx_stream(twx_handle* h);
 
+/* Per-kernel-family timing (the reference's VFX_DEVICE_REPORT counters, device/exec/mod.rs:
+ * 292-322, extended with roofline data): when enabled, every launch is bracketed by HIP events
+ * on the handle's stream and its algorithmic bytes (each step's distinct inputs read once and
+ * outputs written once) are accumulated per family. Families are numbered 0 .. n-1; reading
+ * family n returns TWX_E_MISMATCH. */
+int twx_profile_enable(twx_handle* h, int on);
+int twx_profile_reset(twx_handle* h);
+int twx_profile_read(twx_handle* h, int32_t family, const char** name, double* ms, double* bytes,
+                     uint64_t* launches);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,403 @@
+"""Pins of the oracle (oracle/cpuref.cpp) against what the reference fixes independently of it:
+bit tables, the reference's own unit-test hand values, its bit-exact model fixture (WRP1), closed
+forms, and -- strongest -- the SHA-256 of the original plugin's output for every in-scope case of
+its retime-flow parity grid. CPU only."""
+import gzip
+import hashlib
+import json
+import os
+import struct
+
+import numpy as np
+import pytest
+
+from oracle import cpuref
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def bits(a):
+    return np.asarray(a, np.float32).view(np.uint32).tolist()
+
+
+# ---- a4: kernel tables (plugins/retime-engine/src/flow/conv.rs:693-707) ----
+
+def test_gaussian_and_binomial_tables_bit_exact():
+    assert bits(cpuref.gaussian_kernel(7.0)) == [0x3e4e2a9a, 0x3e35855c, 0x3df7ca22, 0x3d831b79, 0x3cd71a9b, 0x3c08ca7a,
+                                                0x3b06de99, 0x39ce2a9a]
+    assert bits(cpuref.gaussian_kernel(5.0)) == [0x3e904ee6, 0x3e60e131, 0x3dd4bfc6, 0x3cf46273, 0x3baa6d48, 0x3a104ee6]
+    assert bits(cpuref.gaussian_kernel(2.0)) == [0x3f33bb04, 0x3e172281, 0x3ab3bb0c]
+    assert bits([0.375, 0.25, 0.0625]) == [0x3ec00000, 0x3e800000, 0x3d800000]
+
+
+# ---- a5: convolution (conv.rs module header; closed forms) ----
+
+def test_constant_power_of_two_image_is_preserved_by_the_borders():
+    # conv.rs:717-727: a power-of-two constant keeps every product and sum exact in the borders
+    w, h = 23, 19
+    d = cpuref.convolve(np.full(w * h * 2, 4.0, np.float32), w, h, 2, cpuref.gaussian_kernel(7.0))
+    assert np.all(np.abs(d - 4.0) <= 4.0 * 2e-7)
+
+
+def _conv_model(x, w, h, c, k, above_first):
+    """Brute-force model of one separable pass pair written from the module header's formulas."""
+    n = len(k)
+    x = x.reshape(h, w, c).astype(np.float32)
+    k = np.asarray(k, np.float32)
+
+    def line(v, left_first):
+        L = len(v)
+        out = np.zeros(L, np.float32)
+        head = min(n, L)
+        interior = max(L - 2 * n, 0)
+        for i in range(L):
+            if head <= i < head + interior:
+                s = np.float32(v[i] * k[0])
+                for j in range(n - 1, 0, -1):
+                    s = np.float32(s + np.float32(np.float32(v[i + j] + v[i - j]) * k[j]))
+                out[i] = s
+                continue
+            num = np.float32(k[0] * v[i])
+            den = np.float32(k[0])
+            for j in range(n - 1, 0, -1):
+                for t in ((i - j, i + j) if left_first else (i + j, i - j)):
+                    inside = 0 <= t < L
+                    if c == 1:
+                        xv, wt = (v[t], k[j]) if inside else (np.float32(0), np.float32(0))
+                    else:
+                        xv = v[t] if inside else np.float32(-1.0)
+                        wt = np.float32((np.float32(1) if xv != np.float32(-1.0) else np.float32(0)) * k[j])
+                    den = np.float32(den + wt)
+                    num = np.float32(num + np.float32(xv * wt))
+            out[i] = np.float32(num / den)
+        return out
+
+    tmp = np.zeros_like(x)
+    for y in range(h):
+        for ch in range(c):
+            tmp[y, :, ch] = line(x[y, :, ch], False)
+    out = np.zeros_like(x)
+    for xx in range(w):
+        for ch in range(c):
+            out[:, xx, ch] = line(tmp[:, xx, ch], above_first)
+    return out.reshape(-1)
+
+
+@pytest.mark.parametrize("w,h,c,r", [(5, 3, 1, 2.0), (2, 2, 2, 7.0), (17, 13, 3, 7.0), (20, 9, 2, 5.0), (11, 16, 1, 0)])
+def test_convolution_matches_the_brute_force_model(w, h, c, r):
+    rng = np.random.RandomState(w * 100 + h)
+    x = rng.uniform(-50, 300, w * h * c).astype(np.float32)
+    x[::7] = -1.0  # a real sample of -1.0 is dropped by the multi-channel border rule
+    k = cpuref.gaussian_kernel(r) if r else np.array([0.375, 0.25, 0.0625], np.float32)
+    for above in (True, False):
+        got = cpuref.convolve(x, w, h, c, k, above_first=above)
+        want = _conv_model(x, w, h, c, k, above)
+        assert bits(got) == bits(want), (w, h, c, r, above)
+
+
+def test_above_first_differs_from_below_first_somewhere():
+    # the order matters in the last bits (conv.rs:584-597); a plausible swap must fail a test
+    rng = np.random.RandomState(3)
+    x = rng.uniform(0, 255, 32 * 24).astype(np.float32)
+    k = cpuref.gaussian_kernel(7.0)
+    assert bits(cpuref.convolve(x, 32, 24, 1, k, True)) != bits(cpuref.convolve(x, 32, 24, 1, k, False))
+
+
+# ---- a6: samplers (flow/sample.rs:288-329) ----
+
+def test_bilinear_hand_values():
+    img = np.array([0.0, 10.0, 20.0, 100.0, 110.0, 120.0], np.float32)
+    assert cpuref.bilinear1(img, 3, 2, 0.0, 0.0) == 0.0
+    assert cpuref.bilinear1(img, 3, 2, 1.0, 0.0) == 10.0
+    assert cpuref.bilinear1(img, 3, 2, 0.5, 0.5) == 55.0
+    assert cpuref.bilinear1(img, 3, 2, 2.0, 1.0) == 120.0
+    assert cpuref.bilinear1(img, 3, 2, 1.75, 0.25) == 42.5
+
+
+def test_resample_is_corner_aligned():
+    s2 = np.array([0, 1, 2, 3, 4] * 2, np.float32)
+    assert cpuref.resample1(s2, 5, 2, 3, 2)[:3].tolist() == [0.0, 2.0, 4.0]
+
+
+# ---- a1/a3/a7: geometry, luma (prep.rs:400-560) ----
+
+@pytest.mark.parametrize("q,hiq,r,w0,h0", [(1, True, 3, 240, 135), (2, True, 3, 240, 135), (3, True, 2, 480, 270),
+                                           (4, True, 1, 960, 540), (5, True, 0, 1920, 1080), (4, False, 2, 480, 270),
+                                           (3, False, 3, 240, 135), (5, False, 1, 960, 540), (1, False, 3, 240, 135)])
+def test_quality_to_resolution_table(q, hiq, r, w0, h0):
+    st, g = cpuref.geometry(cpuref.flow_settings(quality=q, hiq=hiq), 1920, 1080)
+    assert st == 0 and (g["width"], g["height"]) == (w0, h0)
+
+
+def test_tiny_frames_and_level_sizes():
+    st, g = cpuref.geometry(cpuref.flow_settings(quality=3), 6, 6)
+    assert (g["width"], g["height"]) == (3, 3)
+    st, g = cpuref.geometry(cpuref.flow_settings(quality=1, hiq=False), 17, 16)
+    assert (g["width"], g["height"]) == (2, 2)
+    st, _ = cpuref.geometry(cpuref.flow_settings(quality=5), 1, 17)
+    assert st == 3  # below 2x2: Unsupported
+    st, g = cpuref.geometry(cpuref.flow_settings(quality=5), 960, 540)
+    assert g["sizes"] == [(960, 540), (480, 270), (240, 135), (120, 67), (60, 33), (30, 16), (15, 8), (7, 4)]
+    assert cpuref.geometry(cpuref.flow_settings(quality=5), 4, 4)[1]["levels"] == 1
+    assert cpuref.geometry(cpuref.flow_settings(quality=5), 8, 8)[1]["levels"] == 2
+    assert cpuref.geometry(cpuref.flow_settings(quality=5), 16, 3)[1]["levels"] == 1
+
+
+def test_luma_weights_at_each_depth():
+    im = np.array([255, 100, 200, 50] * 4, np.uint8)
+    want = (np.float32(100) * np.float32(0.3) + np.float32(200) * np.float32(0.59)) + np.float32(50) * np.float32(0.11)
+    assert bits(cpuref.luma(im, 2, 2, 0)[:1]) == bits([want])
+    im16 = np.array([65535, 1000, 30000, 7] * 4, np.uint16)
+    want16 = np.float32(((30000.0 * 0.59 + 1000.0 * 0.3) + 7.0 * 0.11) / 257.0)
+    assert bits(cpuref.luma(im16, 2, 2, 1)[:1]) == bits([want16])
+    imf = np.array([1.0, 0.25, 0.5, 0.125] * 4, np.float32)
+    wantf = np.float32(((0.5 * 0.59 + 0.25 * 0.3) + 0.125 * 0.11) * 255.0)
+    assert bits(cpuref.luma(imf, 2, 2, 2)[:1]) == bits([wantf])
+
+
+# ---- a8-a14: solver kernels (flow.rs:1387-1565) ----
+
+def test_gradient_of_ramps_and_parabola():
+    w, h = 6, 5
+    img = np.array([3.0 * x + 5.0 * y for y in range(h) for x in range(w)], np.float32)
+    g = cpuref.calc_grad(img, w, h)
+    assert np.all(g[0::2] == 3.0) and np.all(g[1::2] == 5.0)
+    img = np.array([float(x * x) for y in range(h) for x in range(w)], np.float32)
+    g = cpuref.calc_grad(img, w, h)
+    assert g[0] == 1.0 and g[2 * 3] == 6.0 and g[2 * (w - 1)] == float(2 * (w - 1) - 1) and g[1] == 0.0
+
+
+def test_right_hand_side_of_a_tiny_patch():
+    w, h = 3, 3
+    i1 = np.arange(9, dtype=np.float32)
+    i0 = np.arange(9, dtype=np.float32) * 2
+    grad = np.ones(18, np.float32)
+    b = cpuref.calc_b(i1, i0, grad, np.zeros(18, np.float32), w, h)
+    assert b[0::2].tolist() == list(range(9)) and b[1::2].tolist() == list(range(9))
+    b = cpuref.calc_b(i1, i0, grad, np.array([1.0, 0.0] * 9, np.float32), w, h)
+    assert (b[0], b[1]) == (0.0, 0.0) and b[(3 + 1) * 2] == 2.0
+    b = cpuref.calc_b(i1, i0, grad, np.array([-2.5, 0.0] * 9, np.float32), w, h)
+    assert (b[4], b[5]) == (0.0, 0.0)
+
+
+def test_new_displacement_update_and_threshold():
+    f = cpuref.new_disp([1.0, 2.0], [100.0, 0.0, 100.0], [50.0, -20.0])
+    assert f.tolist() == [1.5, 1.8] or bits(f) == bits([1.5, 1.8])
+    assert cpuref.new_disp([1.0, 2.0], [6.0, 0.0, 6.5], [50.0, -20.0]).tolist() == [1.0, 2.0]
+    f = cpuref.new_disp([0.0, 0.0], [8.0, 3.0, 8.0], [1.0, 1.0])
+    want = np.float32(np.float32(3.0 * 1.0 - 8.0 * 1.0) / np.float32(-55.0))
+    assert bits(f) == bits([want, want])
+
+
+def test_flow_upsampling_scales_the_vectors():
+    d = cpuref.upsample_flow(np.array([1.0, 2.0] * 4, np.float32), 2, 2, 3, 3)
+    assert d[0::2].tolist() == [2.0] * 9 and d[1::2].tolist() == [4.0] * 9
+    d = cpuref.upsample_flow(np.array([0.0, 0.0, 2.0, 0.0, 4.0, 0.0] * 2, np.float32), 3, 2, 5, 2)
+    assert [d[x * 2] for x in range(5)] == [2.0 * x for x in range(5)]
+
+
+def test_range_check():
+    out = np.array([10.0, 10.0] * 16, np.float32)
+    assert cpuref.lib().cref_all_flow_out_of_range(cpuref._p(out), 4, 4) == 1
+    ok = out.copy()
+    ok[10] = -1.0
+    assert cpuref.lib().cref_all_flow_out_of_range(cpuref._p(ok), 4, 4) == 1
+    ok[0] = ok[1] = 0.0
+    assert cpuref.lib().cref_all_flow_out_of_range(cpuref._p(ok), 4, 4) == 0
+
+
+def test_block_cost_golden_values():
+    w = h = 8
+    z = np.zeros(w * h, np.float32)
+    p = np.arange(1, 10, dtype=np.float32).reshape(3, 3)
+    a = z.reshape(h, w).copy()
+    a[2:5, 2:5] = p  # the patch of A around (3, 3)
+    a = a.reshape(-1)
+    assert cpuref.block_cost(a, z, w, h, 3, 3, 3.0, 3.0) == 285.0
+    assert cpuref.block_cost(a, z, w, h, 3, 3, 0.5, 3.0) is None
+    assert cpuref.block_cost(a, z, w, h, 3, 3, 3.0, 0.0) is None
+    assert cpuref.block_cost(a, z, w, h, 3, 3, 6.01, 3.0) is None
+    assert cpuref.block_cost(a, z, w, h, 3, 3, 6.0, 6.0) is not None
+    a2 = np.zeros((h, w), np.float32)
+    a2[2:5, 2:5] = 1.0
+    a2[4, 4] = 4096.0
+    assert cpuref.block_cost(a2.reshape(-1), z, w, h, 3, 3, 3.0, 3.0) == 16777224.0  # the ninth square last
+
+
+def test_block_matching_replaces_a_bad_vector_and_spares_the_border():
+    w, h = 12, 10
+    img = np.array([((x * 7 + y * 13) % 17) * 3.0 + x * 0.5 for y in range(h) for x in range(w)], np.float32)
+    flow = np.zeros(w * h * 2, np.float32)
+    c = (5 * w + 6) * 2
+    flow[c], flow[c + 1] = 2.0, 1.0
+    for x in range(w):
+        flow[x * 2] = 0.25
+        flow[((h - 1) * w + x) * 2] = 0.25
+    f = cpuref.block_match(img, img, flow, w, h)
+    assert (f[c], f[c + 1]) == (0.0, 0.0)
+    assert all(f[x * 2] == 0.25 and f[((h - 1) * w + x) * 2] == 0.25 for x in range(w))
+
+
+def test_block_matching_is_thread_count_independent():
+    w, h = 40, 30
+    rng = np.random.RandomState(5)
+    a = rng.uniform(0, 255, w * h).astype(np.float32)
+    b = np.roll(a, 2)
+    f0 = ((np.arange(w * h * 2) * 37 % 11) - 5.0).astype(np.float32) * 0.3
+    prev = cpuref.set_threads(1)
+    f1 = cpuref.block_match(a, b, f0, w, h)
+    cpuref.set_threads(4)
+    f4 = cpuref.block_match(a, b, f0, w, h)
+    cpuref.set_threads(prev)
+    assert bits(f1) == bits(f4)
+
+
+def test_start_level_table():
+    s, d = cpuref.start_level(0.7, 8)
+    assert s == 6 and abs(d - 0.9709) < 1e-4
+    assert cpuref.start_level(1.0, 8) == (7, 1.0)
+    assert cpuref.start_level(2.0, 8) == (7, 1.0)
+    assert cpuref.start_level(0.0, 8) == (0, 0.0)
+    assert cpuref.start_level(0.1, 3) == (0, 0.0)
+    assert cpuref.start_level(0.5, 8) == (6, 0.0)
+
+
+def test_solve_recovers_a_translation():
+    # analytic ground truth (in the spirit of retime-engine/tests/flow_d.rs): a smooth pattern
+    # moved by (3, -2) px gives a flow near (3, -2) in the interior, at working resolution / 2
+    w, h = 128, 96
+    yy, xx = np.mgrid[0:h + 16, 0:w + 16].astype(np.float64)
+    pat = 0.5 + 0.25 * np.sin(0.23 * xx + 0.11 * yy) + 0.2 * np.cos(0.17 * yy - 0.07 * xx)
+    a = pat[4:4 + h, 6:6 + w]
+    b = pat[4 - 2:4 - 2 + h, 6 + 3:6 + 3 + w]  # b(p) = a(p + (3, -2))  ->  X=a, Y=b: a(p) ~ b(p - (3,-2))
+    def img(p):
+        return np.stack([np.ones_like(p), p, p, p], -1).astype(np.float32).reshape(-1)
+    flow, g = cpuref.solve(img(a), img(b), w, h, 2, cpuref.flow_settings(quality=5))
+    f = flow.reshape(h, w, 2)[20:-20, 20:-20]
+    assert abs(np.median(f[..., 0]) + 3.0) < 0.3 and abs(np.median(f[..., 1]) - 2.0) < 0.3
+
+
+# ---- a16: inverse warp, the reference's bit-exact model fixture (warp/tests.rs:590-635) ----
+
+def _warp_cases():
+    with gzip.open(os.path.join(HERE, "golden", "warp_cases.bin.gz")) as fh:
+        blob = fh.read()
+    assert blob[:4] == b"WRP1"
+    p = 4
+    (n,) = struct.unpack_from("<I", blob, p)
+    p += 4
+    cases = []
+    for _ in range(n):
+        depth, w, h, fw, fh = struct.unpack_from("<5I", blob, p)
+        p += 20
+        (amount,) = struct.unpack_from("<f", blob, p)
+        p += 4
+        (flags,) = struct.unpack_from("<I", blob, p)
+        p += 4
+        nb = w * h * 4 * [1, 2, 4][depth]
+        dt = [np.uint8, np.dtype("<u2"), np.dtype("<f4")][depth]
+        src = np.frombuffer(blob, dt, w * h * 4, p)
+        p += nb
+        flow = np.frombuffer(blob, np.dtype("<f4"), fw * fh * 2, p)
+        p += fw * fh * 8
+        out = np.frombuffer(blob, dt, w * h * 4, p)
+        p += nb
+        if flags & 4:
+            p += w * h
+        cases.append(dict(depth=depth, w=w, h=h, fw=fw, fh=fh, amount=amount, flags=flags, src=src, flow=flow, out=out))
+    assert p == len(blob)
+    return cases
+
+
+def test_warp_fixture_hold_cases_bit_exact():
+    cases = _warp_cases()
+    assert len(cases) == 60
+    hold = [c for c in cases if c["flags"] & 1 and not c["flags"] & 2 and not c["flags"] & 4]
+    assert len(hold) >= 10
+    for c in hold:
+        st, got = cpuref.inv_warp(np.ascontiguousarray(c["src"]), c["w"], c["h"], c["depth"], c["flow"], c["fw"], c["fh"],
+                                  c["amount"])
+        assert st == 0
+        assert got.view(np.uint8).tobytes() == np.ascontiguousarray(c["out"]).view(np.uint8).tobytes()
+
+
+# ---- a17, a19: blend, premultiply (goldens of plugins/retime/tests/common/goldens.rs) ----
+
+def test_store_rules():
+    assert cpuref.sat_u8(254.49) == 254 and cpuref.sat_u8(254.5) == 255 and cpuref.sat_u8(-3.0) == 0
+    assert cpuref.sat_u8(float("nan")) == 255 and cpuref.sat_u8(1e12) == 255
+    assert cpuref.sat_u16(65534.5) == 65535 and cpuref.sat_u16(float("nan")) == 65535
+
+
+def test_blend_branches():
+    a = np.array([10, 20, 30, 40], np.uint8)
+    b = np.array([250, 120, 30, 0], np.uint8)
+    assert cpuref.blend(a, b, 1, 1, 0, 0.004).tolist() == a.tolist()        # snap to A
+    assert cpuref.blend(a, b, 1, 1, 0, 0.996).tolist() == b.tolist()        # snap to B
+    got = cpuref.blend(a, b, 1, 1, 0, 0.25).tolist()
+    want = [int(np.float32(np.float32(np.float32(float(q) - float(p)) * np.float32(0.25)) + np.float32(p)) + np.float32(0.5))
+            for p, q in zip(a, b)]
+    assert got == want
+
+
+def test_premultiply_round_trip_of_opaque_pixels():
+    # opaque pixels survive from_frame -> to_frame exactly at 8 and 16 bpc
+    rng = np.random.RandomState(1)
+    px = rng.randint(0, 256, (64, 4)).astype(np.uint8)
+    px[:, 3] = 255
+    eng = cpuref.from_frame(px.tobytes(), 64, 1, 0)
+    assert cpuref.to_frame(eng, 64, 1, 0) == px.tobytes()
+    px16 = rng.randint(0, 65536, (64, 4)).astype("<u2")
+    px16[:, 3] = 65535
+    eng = cpuref.from_frame(px16.tobytes(), 64, 1, 1)
+    back = np.frombuffer(cpuref.to_frame(eng, 64, 1, 1), "<u2").reshape(64, 4)
+    # the wire -> 0..32768 -> wire mapping rounds; the round trip is exact on the q15 grid
+    q15 = ((px16.astype(np.uint32) * 32768 + 32767) // 65535)
+    wire = ((np.minimum(q15, 32768) * 65535 + 16384) // 32768).astype("<u2")
+    assert (back == wire).all()
+
+
+def test_premultiply_u8_hand_value():
+    # alpha 128: a = 128 * (1/255 as f32); colour q(200 * a)
+    px = np.array([200, 100, 50, 128], np.uint8)
+    eng = cpuref.from_frame(px.tobytes(), 1, 1, 0)
+    a = np.float32(128) * np.float32(np.uint32(0x3b808081).view(np.float32))
+    want = [128] + [int(np.float64(np.float32(np.float32(c) * a)) + 0.5) for c in (200, 100, 50)]
+    assert eng.tolist() == want
+
+
+# ---- end to end: the original plugin's output hashes (parity/retime-time/results/supp/refs_sha256.txt) ----
+
+def _hash_cases():
+    with open(os.path.join(HERE, "golden", "retime_flow_cases.json")) as fh:
+        return [c for c in json.load(fh) if c["scope"] == "core"]
+
+
+def test_every_in_scope_reference_hash():
+    import parity_inputs as pin
+    from twixtor_amd.params import Retime
+
+    cases = _hash_cases()
+    assert len(cases) == 137
+    bad = []
+    for c in cases:
+        cfg = Retime(c["width"], c["height"], c["depth"], c["fps"], c["params"], c["quality"])
+        clip = pin.clip_bytes(c["source"], c["width"], c["height"], c["frames"], c["depth"])
+        out = cpuref.retime_frames(cfg.struct(), clip, c["frames"], 0, c["frames"])
+        if hashlib.sha256(out).hexdigest() != c["sha256"]:
+            bad.append(c["id"])
+    assert not bad, bad
+
+
+def test_a_planted_mistake_breaks_the_hashes():
+    """A one-ULP change of one output sample changes the hash: the pin is sensitive."""
+    import parity_inputs as pin
+    from twixtor_amd.params import Retime
+
+    c = next(c for c in _hash_cases() if c["depth"] == 32 and c["width"] >= 32)
+    cfg = Retime(c["width"], c["height"], c["depth"], c["fps"], c["params"], c["quality"])
+    clip = pin.clip_bytes(c["source"], c["width"], c["height"], c["frames"], c["depth"])
+    out = bytearray(cpuref.retime_frames(cfg.struct(), clip, c["frames"], 0, c["frames"]))
+    assert hashlib.sha256(out).hexdigest() == c["sha256"]
+    out[1000] ^= 1
+    assert hashlib.sha256(out).hexdigest() != c["sha256"]

@@ -16,7 +16,7 @@ TWX_OK, TWX_E_DEVICE, TWX_E_NAN_OUTPUT, TWX_E_UNSUPPORTED, TWX_E_MISMATCH = 0, 1
 EXPORTS = [
     "twx_open", "twx_close", "twx_selfcheck", "twx_memory_budget", "twx_flow_geometry", "twx_solve",
     "twx_solve_pair", "twx_compose_vectors", "twx_select", "twx_retime_frames_dev", "twx_retime_frames",
-    "twx_get_stats", "twx_stream",
+    "twx_get_stats", "twx_stream", "twx_profile_enable", "twx_profile_reset", "twx_profile_read",
 ]
 
 
@@ -85,6 +85,10 @@ def lib():
         L.twx_get_stats.argtypes = [P, ctypes.POINTER(Stats)]
         L.twx_stream.argtypes = [P]
         L.twx_stream.restype = P
+        L.twx_profile_enable.argtypes = [P, ctypes.c_int]
+        L.twx_profile_reset.argtypes = [P]
+        L.twx_profile_read.argtypes = [P, I32, ctypes.POINTER(ctypes.c_char_p), ctypes.POINTER(ctypes.c_double),
+                                       ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_uint64)]
         _lib = L
     return _lib
 

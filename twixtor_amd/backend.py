@@ -80,6 +80,25 @@ class HipBackend:
         _ffi.lib().twx_get_stats(self._h, ctypes.byref(s))
         return {k: getattr(s, k) for k, _ in s._fields_}
 
+    # ---- per-kernel-family HIP-event timing and algorithmic bytes ----
+    def profile(self, on: bool = True):
+        _ffi.lib().twx_profile_enable(self._h, 1 if on else 0)
+
+    def profile_reset(self):
+        _ffi.lib().twx_profile_reset(self._h)
+
+    def profile_read(self) -> dict:
+        out, fam = {}, 0
+        L = _ffi.lib()
+        while True:
+            name, ms, by, n = ctypes.c_char_p(), ctypes.c_double(), ctypes.c_double(), ctypes.c_uint64()
+            if L.twx_profile_read(self._h, fam, ctypes.byref(name), ctypes.byref(ms), ctypes.byref(by), ctypes.byref(n)):
+                break
+            if n.value:
+                out[name.value.decode()] = {"ms": ms.value, "bytes": by.value, "launches": n.value}
+            fam += 1
+        return out
+
     # ---- Backend::solve / solve_pair ----
     def solve(self, x: np.ndarray, y: np.ndarray, width: int, height: int, fmt: int, fs: _ffi.FlowSettings) -> np.ndarray:
         fw, fh, _ = flow_geometry(fs, width, height)

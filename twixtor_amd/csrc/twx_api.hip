@@ -23,6 +23,24 @@
 
 using namespace twx;
 
+// Grow-only memory for the calls on a handle: device blocks (work buffers, pyramids, flows) and
+// pinned host blocks (the job tables uploaded to the device). A call bump-allocates from the
+// start; the same call pattern reuses the same blocks, so a steady state allocates nothing.
+struct BlockArena {
+    struct Block {
+        char* p;
+        size_t cap;
+    };
+    std::vector<Block> blocks;
+    size_t cur = 0, off = 0;
+    bool pinned = false;
+    size_t min_block = 256u << 20;
+
+    void reset() { cur = 0; off = 0; }
+    char* take(size_t bytes);
+    void release();
+};
+
 struct twx_handle {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -31,6 +49,9 @@ struct twx_handle {
     std::thread::id owner;
     bool busy = false;
     twx_stats stats{};
+    Prof prof;
+    BlockArena dev;
+    BlockArena host;
 };
 
 namespace {
@@ -53,31 +74,71 @@ struct HipError {
         if (e_ != hipSuccess) throw HipError{std::string(#expr) + ": " + hipGetErrorString(e_)}; \
     } while (0)
 
-// Stream-ordered device buffers, returned to the pool when the call ends.
+}  // namespace
+
+char* BlockArena::take(size_t bytes) {
+    bytes = std::max<size_t>((bytes + 255) & ~(size_t)255, 256);
+    while (cur < blocks.size()) {
+        if (off + bytes <= blocks[cur].cap) {
+            char* r = blocks[cur].p + off;
+            off += bytes;
+            return r;
+        }
+        cur++;
+        off = 0;
+    }
+    size_t cap = std::max(bytes, min_block);
+    void* p = nullptr;
+    if (pinned) HIPCHECK(hipHostMalloc(&p, cap, hipHostMallocDefault));
+    else HIPCHECK(hipMalloc(&p, cap));
+    blocks.push_back(Block{(char*)p, cap});
+    cur = blocks.size() - 1;
+    off = bytes;
+    return (char*)p;
+}
+
+void BlockArena::release() {
+    for (auto& b : blocks) {
+        if (pinned) (void)hipHostFree(b.p);
+        else (void)hipFree(b.p);
+    }
+    blocks.clear();
+    reset();
+}
+
+namespace {
+
+// The buffers of one call: bump allocations from the handle's arenas (reset when the call
+// starts), table uploads through pinned staging. Every call synchronises its stream before it
+// returns, so the next call may reuse the memory.
 class DevPool {
   public:
-    explicit DevPool(hipStream_t s) : s_(s) {}
-    ~DevPool() {
-        for (void* p : ptrs_) (void)hipFreeAsync(p, s_);
+    DevPool(twx_handle* h) : h_(h) {
+        h->dev.reset();
+        h->host.reset();
     }
     template <class T>
     T* alloc(size_t count) {
-        void* p = nullptr;
-        size_t bytes = std::max<size_t>(count * sizeof(T), 256);
-        HIPCHECK(hipMallocAsync(&p, bytes, s_));
-        ptrs_.push_back(p);
-        return (T*)p;
+        return (T*)h_->dev.take(count * sizeof(T));
     }
     template <class T>
     T* upload(const std::vector<T>& v) {
         T* d = alloc<T>(v.size());
-        if (!v.empty()) HIPCHECK(hipMemcpyAsync(d, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s_));
+        if (!v.empty()) {
+            size_t bytes = v.size() * sizeof(T);
+            char* st = h_->host.take(bytes);
+            memcpy(st, v.data(), bytes);
+            HIPCHECK(hipMemcpyAsync(d, st, bytes, hipMemcpyHostToDevice, h_->stream));
+        }
         return d;
+    }
+    template <class T>
+    T* pinned(size_t count) {
+        return (T*)h_->host.take(count * sizeof(T));
     }
 
   private:
-    hipStream_t s_;
-    std::vector<void*> ptrs_;
+    twx_handle* h_;
 };
 
 // Serialises the calls on a handle and refuses a nested call from the same thread
@@ -101,9 +162,15 @@ class Guard {
         }
         locked_ = true;
         (void)hipSetDevice(h->device);
+        g_prof = &h->prof;
     }
     ~Guard() {
         if (!locked_) return;
+        if (h_->prof.on) {
+            (void)hipStreamSynchronize(h_->stream);
+            h_->prof.flush();
+        }
+        g_prof = nullptr;
         {
             std::lock_guard<std::mutex> g(meta_mu());
             h_->busy = false;
@@ -214,18 +281,36 @@ void build_pyramids(hipStream_t s, DevPool& pool, const std::vector<PyrIn>& in) 
     if (in.empty()) return;
     const Taps bin = binomial_taps();
     const size_t chunk = 64;
+    // per frame scratch: lum (W*H), red (W/2*H/2), tmp (W*H), blur (w0*h0); one set for the
+    // largest chunk, reused by every chunk in stream order
+    auto frame_scratch = [](const PyrIn& p) {
+        auto r = [](size_t v) { return (v + 63) & ~(size_t)63; };
+        return r((size_t)p.W * p.H) * 2 + r((size_t)(p.W / 2 + 1) * (p.H / 2 + 1)) + r((size_t)p.geo.w0 * p.geo.h0);
+    };
+    size_t max_floats = 0;
+    for (size_t c0 = 0; c0 < in.size(); c0 += chunk) {
+        size_t f = 0;
+        for (size_t i = c0; i < std::min(in.size(), c0 + chunk); i++) f += frame_scratch(in[i]);
+        max_floats = std::max(max_floats, f);
+    }
+    float* sbase = pool.alloc<float>(max_floats);
     for (size_t c0 = 0; c0 < in.size(); c0 += chunk) {
         size_t c1 = std::min(in.size(), c0 + chunk);
         int n = (int)(c1 - c0);
-        // per frame scratch: lum (W*H), red (W/2*H/2), tmp (W*H), blur (w0*h0)
         std::vector<float*> lum(n), red(n), tmp(n), blur(n);
         int maxW = 0, maxH = 0, maxR = 0, maxL = 0;
+        float* sp = sbase;
         for (int i = 0; i < n; i++) {
             const PyrIn& p = in[c0 + i];
-            lum[i] = pool.alloc<float>((size_t)p.W * p.H);
-            red[i] = pool.alloc<float>((size_t)(p.W / 2 + 1) * (p.H / 2 + 1));
-            tmp[i] = pool.alloc<float>((size_t)p.W * p.H);
-            blur[i] = pool.alloc<float>((size_t)p.geo.w0 * p.geo.h0);
+            auto r = [](size_t v) { return (v + 63) & ~(size_t)63; };
+            lum[i] = sp;
+            sp += r((size_t)p.W * p.H);
+            tmp[i] = sp;
+            sp += r((size_t)p.W * p.H);
+            red[i] = sp;
+            sp += r((size_t)(p.W / 2 + 1) * (p.H / 2 + 1));
+            blur[i] = sp;
+            sp += r((size_t)p.geo.w0 * p.geo.h0);
             maxW = std::max(maxW, p.W);
             maxH = std::max(maxH, p.H);
             maxR = std::max(maxR, p.geo.reduction);
@@ -299,11 +384,11 @@ void build_pyramids(hipStream_t s, DevPool& pool, const std::vector<PyrIn>& in) 
             }
             if (mw == 0 || mh == 0) continue;
             switch (kinds[k]) {
-                case 0: launch_luma(s, dk, n, mw, mh); break;
-                case 1: launch_hpass1(s, dk, n, mw, mh, bin); break;
-                case 2: launch_vpass1(s, dk, n, mw, mh, bin); break;
-                case 3: launch_resample(s, dk, n, mw, mh); break;
-                case 4: launch_subtract(s, dk, n, mw, mh); break;
+                case 0: launch_luma(s, dk, steps[k], mw, mh); break;
+                case 1: launch_hpass1(s, dk, steps[k], mw, mh, bin); break;
+                case 2: launch_vpass1(s, dk, steps[k], mw, mh, bin); break;
+                case 3: launch_resample(s, dk, steps[k], mw, mh); break;
+                case 4: launch_subtract(s, dk, steps[k], mw, mh); break;
             }
         }
     }
@@ -322,23 +407,32 @@ struct SolveIn {
 int run_solves(hipStream_t s, DevPool& pool, const std::vector<SolveIn>& in, const twx_flow_settings& fs,
                uint64_t scratch_budget, std::string& err) {
     if (in.empty()) return TWX_OK;
-    size_t i0 = 0;
-    while (i0 < in.size()) {
-        // chunk by the scratch bytes of each job: flow[1] 2 + grad 2 + ta 3 + tb 2 + tmp 3 floats/pixel
-        size_t i1 = i0, bytes = 0;
+    // chunks by the scratch bytes of each job: flow[1] 2 + grad 2 + ta 3 + tb 2 + tmp 3 floats/pixel
+    std::vector<std::pair<size_t, size_t>> chunks;
+    size_t max_px = 0, max_n = 0;
+    for (size_t i0 = 0; i0 < in.size();) {
+        size_t i1 = i0, px_sum = 0;
         while (i1 < in.size()) {
             size_t px = (size_t)in[i1].geo.w0 * in[i1].geo.h0;
-            size_t b = px * 12 * sizeof(float) + 4096;
-            if (i1 > i0 && bytes + b > scratch_budget) break;
-            bytes += b;
+            if (i1 > i0 && (px_sum + px) * 12 * sizeof(float) > scratch_budget) break;
+            px_sum += px;
             i1++;
         }
-        int n = (int)(i1 - i0);
+        chunks.push_back({i0, i1});
+        max_px = std::max(max_px, px_sum);
+        max_n = std::max(max_n, i1 - i0);
+        i0 = i1;
+    }
+    // one scratch set, reused by every chunk in stream order
+    float* scratch = pool.alloc<float>(max_px * 12);
+    int* state = pool.alloc<int>(max_n * 4);
+    int* hstate = pool.pinned<int>(max_n * 4);
+    for (auto& ch : chunks) {
+        int n = (int)(ch.second - ch.first);
         std::vector<SolveJob> jobs(n);
-        int* state = pool.alloc<int>((size_t)n * 4);
-        std::vector<int> state_init((size_t)n * 4, 0);
+        float* sp = scratch;
         for (int k = 0; k < n; k++) {
-            const SolveIn& si = in[i0 + k];
+            const SolveIn& si = in[ch.first + k];
             SolveJob& J = jobs[k];
             memset(&J, 0, sizeof(J));
             const float* xl = si.X;
@@ -356,28 +450,22 @@ int run_solves(hipStream_t s, DevPool& pool, const std::vector<SolveIn>& in, con
             J.scale_damp = (std::fabs((double)J.damp - 1.0) >= 1e-6 && J.damp != 1.0f) ? 1 : 0;
             size_t px = (size_t)si.geo.w0 * si.geo.h0;
             J.flow[0] = si.out;
-            J.flow[1] = pool.alloc<float>(px * 2);
-            J.grad = pool.alloc<float>(px * 2);
-            J.ta = pool.alloc<float>(px * 3);
-            J.tb = pool.alloc<float>(px * 2);
-            J.tmp = pool.alloc<float>(px * 3);
+            J.flow[1] = sp;
+            J.grad = sp + px * 2;
+            J.ta = sp + px * 4;
+            J.tb = sp + px * 7;
+            J.tmp = sp + px * 9;
+            sp += px * 12;
             J.state = state + k * 4;
             J.max_w = si.geo.w0;
             J.max_h = si.geo.h0;
-            state_init[k * 4] = 1;
-            // the flow starts at zero at the start level, in the buffer of that level's parity
-            size_t spx = (size_t)si.geo.w[J.start] * si.geo.h[J.start];
-            HIPCHECK(hipMemsetAsync(J.flow[J.start & 1], 0, spx * 2 * sizeof(float), s));
         }
-        HIPCHECK(hipMemcpyAsync(state, state_init.data(), state_init.size() * sizeof(int), hipMemcpyHostToDevice, s));
         SolveJob* d = pool.upload(jobs);
-        launch_solve(s, d, jobs, fs.skip_refine != 0);
-        std::vector<int> st((size_t)n * 4);
-        HIPCHECK(hipMemcpyAsync(st.data(), state, st.size() * sizeof(int), hipMemcpyDeviceToHost, s));
+        launch_solve(s, d, jobs, fs.skip_refine != 0);  // starts with the per-job init kernel
+        HIPCHECK(hipMemcpyAsync(hstate, state, (size_t)n * 4 * sizeof(int), hipMemcpyDeviceToHost, s));
         HIPCHECK(hipStreamSynchronize(s));
         for (int k = 0; k < n; k++)
-            if (st[k * 4 + 2]) { err = "the solved flow holds a NaN"; return TWX_E_NAN_OUTPUT; }
-        i0 = i1;
+            if (hstate[k * 4 + 2]) { err = "the solved flow holds a NaN"; return TWX_E_NAN_OUTPUT; }
     }
     return TWX_OK;
 }
@@ -417,8 +505,8 @@ twx_flow_settings settings_for(const twx_retime_params& p, int vq) {
 
 // The batched render of output frames [first, first + count) with the clip and the output on the
 // device.
-int retime_frames_dev(twx_handle* h, const twx_retime_params& p, const uint8_t* clip, int n_in, int first, int count,
-                      uint8_t* out, std::string& err) {
+int retime_frames_dev(twx_handle* h, DevPool& pool, const twx_retime_params& p, const uint8_t* clip, int n_in, int first,
+                      int count, uint8_t* out, std::string& err) {
     int st = timeline::check_params(p, err);
     if (st) return st;
     hipStream_t s = h->stream;
@@ -448,7 +536,6 @@ int retime_frames_dev(twx_handle* h, const twx_retime_params& p, const uint8_t* 
         pl.ib = timeline::clip_index(rates, pl.sel.b, n_in);
     }
 
-    DevPool pool(s);
     // source slots: one engine image (motion rows) per clip index used; -1 is the transparent frame
     std::map<long, int> slot_of;
     std::vector<long> slots;
@@ -618,7 +705,7 @@ int solve_host(twx_handle* h, const void* x, const void* y, int W, int H, int fm
     if (fmt < 0 || fmt > 2) { err = "bad format code"; return TWX_E_MISMATCH; }
     hipStream_t s = h->stream;
     double t0 = now_ms();
-    DevPool pool(s);
+    DevPool pool(h);
     size_t fb = image_bytes(W, H, fmt);
     uint8_t* img = pool.alloc<uint8_t>(fb * 2);
     HIPCHECK(hipMemcpyAsync(img, x, fb, hipMemcpyHostToDevice, s));
@@ -674,6 +761,8 @@ int twx_open(int index, twx_handle** out, char* err, size_t errlen) {
         size_t free_b = 0, total_b = 0;
         HIPCHECK(hipMemGetInfo(&free_b, &total_b));
         h->budget = (uint64_t)(total_b * 0.75);
+        h->host.pinned = true;
+        h->host.min_block = 8u << 20;
         // keep freed stream-ordered memory in the pool between calls
         hipMemPool_t pool;
         HIPCHECK(hipDeviceGetDefaultMemPool(&pool, dev));
@@ -703,6 +792,8 @@ void twx_close(twx_handle* h) {
         (void)hipStreamSynchronize(h->stream);
         (void)hipStreamDestroy(h->stream);
     }
+    h->dev.release();
+    h->host.release();
     delete h;
 }
 
@@ -785,7 +876,7 @@ int twx_compose_vectors(twx_handle* h, int32_t interp, float wa, const void* a, 
         }
         hipStream_t s = h->stream;
         double t0 = now_ms();
-        DevPool pool(s);
+        DevPool pool(h);
         size_t fb = image_bytes(width, height, fmt);
         size_t fn = (size_t)flow_w * flow_h * 2;
         uint8_t* img = pool.alloc<uint8_t>(fb * 3);
@@ -866,7 +957,10 @@ int twx_retime_frames_dev(twx_handle* h, const twx_retime_params* p, const void*
     int st = TWX_OK;
     try {
         Guard g(h, st, e);
-        if (g.ok()) st = retime_frames_dev(h, *p, (const uint8_t*)clip, n_in, first, count, (uint8_t*)out, e);
+        if (g.ok()) {
+            DevPool pool(h);
+            st = retime_frames_dev(h, pool, *p, (const uint8_t*)clip, n_in, first, count, (uint8_t*)out, e);
+        }
     } catch (const HipError& x) {
         e = x.msg;
         st = TWX_E_DEVICE;
@@ -885,11 +979,11 @@ int twx_retime_frames(twx_handle* h, const twx_retime_params* p, const void* cli
         if ((st = timeline::check_params(*p, e))) { set_err(err, errlen, e); return st; }
         hipStream_t s = h->stream;
         size_t fb = image_bytes(p->width, p->height, p->fmt);
-        DevPool pool(s);
+        DevPool pool(h);
         uint8_t* dclip = pool.alloc<uint8_t>(fb * std::max(n_in, 1));
         uint8_t* dout = pool.alloc<uint8_t>(fb * std::max(count, 1));
         if (n_in > 0) HIPCHECK(hipMemcpyAsync(dclip, clip, fb * n_in, hipMemcpyHostToDevice, s));
-        st = retime_frames_dev(h, *p, dclip, n_in, first, count, dout, e);
+        st = retime_frames_dev(h, pool, *p, dclip, n_in, first, count, dout, e);
         if (!st) {
             HIPCHECK(hipMemcpyAsync(out, dout, fb * count, hipMemcpyDeviceToHost, s));
             HIPCHECK(hipStreamSynchronize(s));
@@ -905,6 +999,30 @@ int twx_retime_frames(twx_handle* h, const twx_retime_params* p, const void* cli
 int twx_get_stats(twx_handle* h, twx_stats* out) {
     if (!h || !out) return TWX_E_MISMATCH;
     *out = h->stats;
+    return TWX_OK;
+}
+
+int twx_profile_enable(twx_handle* h, int on) {
+    if (!h) return TWX_E_MISMATCH;
+    std::lock_guard<std::mutex> g(h->mu);
+    h->prof.on = on != 0;
+    return TWX_OK;
+}
+
+int twx_profile_reset(twx_handle* h) {
+    if (!h) return TWX_E_MISMATCH;
+    std::lock_guard<std::mutex> g(h->mu);
+    h->prof.reset();
+    return TWX_OK;
+}
+
+int twx_profile_read(twx_handle* h, int32_t family, const char** name, double* ms, double* bytes, uint64_t* launches) {
+    if (!h || family < 0 || family >= F_NFAM) return TWX_E_MISMATCH;
+    std::lock_guard<std::mutex> g(h->mu);
+    if (name) *name = PROF_NAMES[family];
+    if (ms) *ms = h->prof.ms[family];
+    if (bytes) *bytes = h->prof.bytes[family];
+    if (launches) *launches = h->prof.launches[family];
     return TWX_OK;
 }
 

@@ -72,13 +72,50 @@ struct FrameJob {
     int pad;
 };
 
+// Per-kernel-family timing with HIP events on the launching stream, and the algorithmic bytes of
+// each launch (the step-boundary model: distinct inputs read once, outputs written once).
+enum ProfFam {
+    F_INGEST, F_LUMA, F_PYR_HPASS, F_PYR_VPASS, F_PYR_RESAMPLE, F_PYR_SUBTRACT, F_GRAD_TENSOR, F_GAUSS_HPASS,
+    F_GAUSS_VPASS, F_CALC_B, F_NEW_DISP, F_BM_ROWS, F_BM_COLS, F_RANGE, F_UPSAMPLE, F_NEGATE, F_FRAMES, F_NFAM
+};
+extern const char* const PROF_NAMES[F_NFAM];
+
+struct Prof {
+    bool on = false;
+    struct Rec {
+        int fam;
+        hipEvent_t a, b;
+        double bytes;
+    };
+    std::vector<Rec> pending;
+    std::vector<hipEvent_t> spare;
+    double ms[F_NFAM] = {};
+    double bytes[F_NFAM] = {};
+    uint64_t launches[F_NFAM] = {};
+    hipEvent_t take();
+    void flush();  // after a stream synchronisation: collect the elapsed times
+    void reset();
+    ~Prof();
+};
+// The profiler of the call in progress on this thread (null when off).
+extern thread_local Prof* g_prof;
+hipEvent_t prof_begin(hipStream_t s);
+void prof_end(hipStream_t s, int fam, double bytes, hipEvent_t a);
+
+#define TWX_LAUNCH(stream, fam, bytes, ...)                 \
+    do {                                                    \
+        hipEvent_t e0_ = prof_begin(stream);                \
+        __VA_ARGS__;                                        \
+        prof_end(stream, fam, (double)(bytes), e0_);        \
+    } while (0)
+
 // Host side launchers (twx_device.hip).
 void launch_ingest(hipStream_t s, const void* wire, void* engine, int W, int H, int fmt, bool reverse_rows);
-void launch_luma(hipStream_t s, const PlaneOp* d_ops, int n, int maxw, int maxh);
-void launch_hpass1(hipStream_t s, const PlaneOp* d_ops, int n, int maxw, int maxh, const Taps& t);
-void launch_vpass1(hipStream_t s, const PlaneOp* d_ops, int n, int maxw, int maxh, const Taps& t);
-void launch_resample(hipStream_t s, const PlaneOp* d_ops, int n, int maxw, int maxh);
-void launch_subtract(hipStream_t s, const PlaneOp* d_ops, int n, int maxw, int maxh);
+void launch_luma(hipStream_t s, const PlaneOp* d_ops, const std::vector<PlaneOp>& ops, int maxw, int maxh);
+void launch_hpass1(hipStream_t s, const PlaneOp* d_ops, const std::vector<PlaneOp>& ops, int maxw, int maxh, const Taps& t);
+void launch_vpass1(hipStream_t s, const PlaneOp* d_ops, const std::vector<PlaneOp>& ops, int maxw, int maxh, const Taps& t);
+void launch_resample(hipStream_t s, const PlaneOp* d_ops, const std::vector<PlaneOp>& ops, int maxw, int maxh);
+void launch_subtract(hipStream_t s, const PlaneOp* d_ops, const std::vector<PlaneOp>& ops, int maxw, int maxh);
 void launch_solve(hipStream_t s, const SolveJob* d_jobs, const std::vector<SolveJob>& jobs, bool skip_refine);
 void launch_frames(hipStream_t s, const FrameJob* d_jobs, const std::vector<FrameJob>& jobs);
 void launch_nan_scan(hipStream_t s, const float* data, size_t n, int* flag);

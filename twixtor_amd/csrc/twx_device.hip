@@ -83,9 +83,74 @@ __global__ void k_ingest(const uint8_t* __restrict__ wire, uint8_t* __restrict__
 
 void launch_ingest(hipStream_t s, const void* wire, void* engine, int W, int H, int fmt, bool reverse_rows) {
     dim3 blk(256), grd((W + 255) / 256, H);
-    if (fmt == TWX_FMT_U8) k_ingest<0><<<grd, blk, 0, s>>>((const uint8_t*)wire, (uint8_t*)engine, W, H, reverse_rows);
-    else if (fmt == TWX_FMT_U16) k_ingest<1><<<grd, blk, 0, s>>>((const uint8_t*)wire, (uint8_t*)engine, W, H, reverse_rows);
-    else k_ingest<2><<<grd, blk, 0, s>>>((const uint8_t*)wire, (uint8_t*)engine, W, H, reverse_rows);
+    double bytes = 2.0 * image_bytes(W, H, fmt);
+    if (fmt == TWX_FMT_U8)
+        TWX_LAUNCH(s, F_INGEST, bytes, k_ingest<0><<<grd, blk, 0, s>>>((const uint8_t*)wire, (uint8_t*)engine, W, H, reverse_rows));
+    else if (fmt == TWX_FMT_U16)
+        TWX_LAUNCH(s, F_INGEST, bytes, k_ingest<1><<<grd, blk, 0, s>>>((const uint8_t*)wire, (uint8_t*)engine, W, H, reverse_rows));
+    else
+        TWX_LAUNCH(s, F_INGEST, bytes, k_ingest<2><<<grd, blk, 0, s>>>((const uint8_t*)wire, (uint8_t*)engine, W, H, reverse_rows));
+}
+
+// ---- profiler (host) ----
+
+const char* const PROF_NAMES[F_NFAM] = {"ingest", "luma", "pyr_hpass", "pyr_vpass", "pyr_resample", "pyr_subtract",
+                                        "grad_tensor", "gauss_hpass", "gauss_vpass", "calc_b", "new_disp", "bm_rows",
+                                        "bm_cols", "range", "upsample", "negate", "frames"};
+thread_local Prof* g_prof = nullptr;
+
+hipEvent_t Prof::take() {
+    if (!spare.empty()) {
+        hipEvent_t e = spare.back();
+        spare.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    (void)hipEventCreate(&e);
+    return e;
+}
+
+void Prof::flush() {
+    for (auto& r : pending) {
+        float t = 0.0f;
+        if (hipEventElapsedTime(&t, r.a, r.b) == hipSuccess) {
+            ms[r.fam] += t;
+            bytes[r.fam] += r.bytes;
+            launches[r.fam] += 1;
+        }
+        spare.push_back(r.a);
+        spare.push_back(r.b);
+    }
+    pending.clear();
+}
+
+void Prof::reset() {
+    for (int i = 0; i < F_NFAM; i++) ms[i] = bytes[i] = 0, launches[i] = 0;
+}
+
+Prof::~Prof() {
+    for (auto& r : pending) { (void)hipEventDestroy(r.a); (void)hipEventDestroy(r.b); }
+    for (auto e : spare) (void)hipEventDestroy(e);
+}
+
+hipEvent_t prof_begin(hipStream_t s) {
+    if (!g_prof || !g_prof->on) return nullptr;
+    hipEvent_t a = g_prof->take();
+    (void)hipEventRecord(a, s);
+    return a;
+}
+
+void prof_end(hipStream_t s, int fam, double bytes, hipEvent_t a) {
+    if (!a || !g_prof) return;
+    hipEvent_t b = g_prof->take();
+    (void)hipEventRecord(b, s);
+    g_prof->pending.push_back(Prof::Rec{fam, a, b, bytes});
+}
+
+static double ops_px(const std::vector<PlaneOp>& ops, bool dst) {
+    double n = 0;
+    for (auto& o : ops) n += dst ? (double)o.dw * o.dh : (double)o.sw * o.sh;
+    return n;
 }
 
 // -------------------------------------------------------------------------------------------
@@ -114,8 +179,10 @@ __global__ void k_luma(const PlaneOp* __restrict__ ops) {
     op.dst[i] = v;
 }
 
-void launch_luma(hipStream_t s, const PlaneOp* d_ops, int n, int maxw, int maxh) {
-    k_luma<<<dim3((maxw + 255) / 256, maxh, n), dim3(256), 0, s>>>(d_ops);
+void launch_luma(hipStream_t s, const PlaneOp* d_ops, const std::vector<PlaneOp>& ops, int maxw, int maxh) {
+    double bytes = 0;
+    for (auto& o : ops) bytes += (double)o.sw * o.sh * (4.0 * sample_size(o.fmt) + 4.0);
+    TWX_LAUNCH(s, F_LUMA, bytes, k_luma<<<dim3((maxw + 255) / 256, maxh, (unsigned)ops.size()), dim3(256), 0, s>>>(d_ops));
 }
 
 // -------------------------------------------------------------------------------------------
@@ -211,11 +278,13 @@ __global__ void k_vpass1(const PlaneOp* __restrict__ ops, Taps t) {
     op.dst[(long)y * op.sw + x] = vpass_elem<1>((const float*)op.src, op.sw, op.sh, x, y, 0, t);
 }
 
-void launch_hpass1(hipStream_t s, const PlaneOp* d_ops, int n, int maxw, int maxh, const Taps& t) {
-    k_hpass1<<<dim3((maxw + 255) / 256, maxh, n), dim3(256), 0, s>>>(d_ops, t);
+void launch_hpass1(hipStream_t s, const PlaneOp* d_ops, const std::vector<PlaneOp>& ops, int maxw, int maxh, const Taps& t) {
+    TWX_LAUNCH(s, F_PYR_HPASS, 8.0 * ops_px(ops, false),
+               k_hpass1<<<dim3((maxw + 255) / 256, maxh, (unsigned)ops.size()), dim3(256), 0, s>>>(d_ops, t));
 }
-void launch_vpass1(hipStream_t s, const PlaneOp* d_ops, int n, int maxw, int maxh, const Taps& t) {
-    k_vpass1<<<dim3((maxw + 255) / 256, maxh, n), dim3(256), 0, s>>>(d_ops, t);
+void launch_vpass1(hipStream_t s, const PlaneOp* d_ops, const std::vector<PlaneOp>& ops, int maxw, int maxh, const Taps& t) {
+    TWX_LAUNCH(s, F_PYR_VPASS, 8.0 * ops_px(ops, false),
+               k_vpass1<<<dim3((maxw + 255) / 256, maxh, (unsigned)ops.size()), dim3(256), 0, s>>>(d_ops, t));
 }
 
 // -------------------------------------------------------------------------------------------
@@ -265,11 +334,13 @@ __global__ void k_subtract(const PlaneOp* __restrict__ ops) {
     op.dst[i] = op.dst[i] - ((const float*)op.src)[i];
 }
 
-void launch_resample(hipStream_t s, const PlaneOp* d_ops, int n, int maxw, int maxh) {
-    k_resample<<<dim3((maxw + 255) / 256, maxh, n), dim3(256), 0, s>>>(d_ops);
+void launch_resample(hipStream_t s, const PlaneOp* d_ops, const std::vector<PlaneOp>& ops, int maxw, int maxh) {
+    TWX_LAUNCH(s, F_PYR_RESAMPLE, 4.0 * ops_px(ops, false) + 4.0 * ops_px(ops, true),
+               k_resample<<<dim3((maxw + 255) / 256, maxh, (unsigned)ops.size()), dim3(256), 0, s>>>(d_ops));
 }
-void launch_subtract(hipStream_t s, const PlaneOp* d_ops, int n, int maxw, int maxh) {
-    k_subtract<<<dim3((maxw + 255) / 256, maxh, n), dim3(256), 0, s>>>(d_ops);
+void launch_subtract(hipStream_t s, const PlaneOp* d_ops, const std::vector<PlaneOp>& ops, int maxw, int maxh) {
+    TWX_LAUNCH(s, F_PYR_SUBTRACT, 12.0 * ops_px(ops, false),
+               k_subtract<<<dim3((maxw + 255) / 256, maxh, (unsigned)ops.size()), dim3(256), 0, s>>>(d_ops));
 }
 
 // -------------------------------------------------------------------------------------------
@@ -562,55 +633,76 @@ __global__ void k_bm_cols(const SolveJob* __restrict__ jobs, int level) {
 // -------------------------------------------------------------------------------------------
 // The solve driver: per level, the kernel sequence of flow.rs:1271-1304 over every job.
 
+// the flow starts at zero at the start level (in the buffer of that level's parity); the range
+// check starts active (flow.rs:1268-1270)
+__global__ void k_solve_init(const SolveJob* __restrict__ jobs) {
+    const SolveJob& J = jobs[blockIdx.z];
+    long n = (long)J.w[J.start] * J.h[J.start] * 2;
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) J.flow[J.start & 1][i] = 0.0f;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        J.state[0] = 1;
+        J.state[1] = 0;
+        J.state[2] = 0;
+    }
+}
+
 void launch_solve(hipStream_t s, const SolveJob* d_jobs, const std::vector<SolveJob>& jobs, bool skip_refine) {
     int nj = (int)jobs.size();
     if (nj == 0) return;
     int max_start = 0;
-    for (auto& j : jobs) max_start = std::max(max_start, j.start);
+    long max_init = 0;
+    for (auto& j : jobs) {
+        max_start = std::max(max_start, j.start);
+        max_init = std::max(max_init, (long)j.w[j.start] * j.h[j.start] * 2);
+    }
+    TWX_LAUNCH(s, F_RANGE, 0.0, k_solve_init<<<dim3((unsigned)((max_init + 255) / 256), 1, nj), dim3(256), 0, s>>>(d_jobs));
     const Taps g7 = gaussian_taps(7.0f), g5 = gaussian_taps(5.0f);
+    auto bm = [&](int l, int mw, int mh, double P) {
+        dim3 rb(64), rg((mh + 63) / 64, 1, nj), cg((mw + 63) / 64, 1, nj);
+        TWX_LAUNCH(s, F_BM_ROWS, 24.0 * P, k_bm_rows<1><<<rg, rb, 0, s>>>(d_jobs, l));
+        TWX_LAUNCH(s, F_BM_ROWS, 24.0 * P, k_bm_rows<-1><<<rg, rb, 0, s>>>(d_jobs, l));
+        TWX_LAUNCH(s, F_BM_COLS, 24.0 * P, k_bm_cols<1><<<cg, rb, 0, s>>>(d_jobs, l));
+        TWX_LAUNCH(s, F_BM_COLS, 24.0 * P, k_bm_cols<-1><<<cg, rb, 0, s>>>(d_jobs, l));
+    };
     for (int l = max_start; l >= 0; l--) {
         int mw = 0, mh = 0;
+        double P = 0, Pn = 0;  // pixels of this level and of the next finer one, over the jobs
         for (auto& j : jobs)
-            if (l <= j.start) { mw = std::max(mw, j.w[l]); mh = std::max(mh, j.h[l]); }
-        dim3 blk(256), grd((mw + 255) / 256, mh, nj);
-        k_grad_tensor<<<grd, blk, 0, s>>>(d_jobs, l);
-        k_job_hpass<3><<<grd, blk, 0, s>>>(d_jobs, l, BUF_TA, g7);
-        k_job_vpass<3><<<grd, blk, 0, s>>>(d_jobs, l, BUF_TA, g7);
-        for (int it = 0; it < 3; it++) {
-            k_calc_b<<<grd, blk, 0, s>>>(d_jobs, l);
-            k_job_hpass<2><<<grd, blk, 0, s>>>(d_jobs, l, BUF_TB, g7);
-            k_job_vpass<2><<<grd, blk, 0, s>>>(d_jobs, l, BUF_TB, g7);
-            k_new_disp<<<grd, blk, 0, s>>>(d_jobs, l);
-            k_job_hpass<2><<<grd, blk, 0, s>>>(d_jobs, l, BUF_FLOW, g5);
-            k_job_vpass<2><<<grd, blk, 0, s>>>(d_jobs, l, BUF_FLOW, g5);
-            if ((it == 1 && l == 0 && !skip_refine)) {
-                dim3 rb(64), rg((mh + 63) / 64, 1, nj), cg((mw + 63) / 64, 1, nj);
-                k_bm_rows<1><<<rg, rb, 0, s>>>(d_jobs, l);
-                k_bm_rows<-1><<<rg, rb, 0, s>>>(d_jobs, l);
-                k_bm_cols<1><<<cg, rb, 0, s>>>(d_jobs, l);
-                k_bm_cols<-1><<<cg, rb, 0, s>>>(d_jobs, l);
+            if (l <= j.start) {
+                mw = std::max(mw, j.w[l]);
+                mh = std::max(mh, j.h[l]);
+                P += (double)j.w[l] * j.h[l];
+                if (l > 0) Pn += (double)j.w[l - 1] * j.h[l - 1];
             }
+        dim3 blk(256), grd((mw + 255) / 256, mh, nj);
+        TWX_LAUNCH(s, F_GRAD_TENSOR, 24.0 * P, k_grad_tensor<<<grd, blk, 0, s>>>(d_jobs, l));
+        TWX_LAUNCH(s, F_GAUSS_HPASS, 24.0 * P, k_job_hpass<3><<<grd, blk, 0, s>>>(d_jobs, l, BUF_TA, g7));
+        TWX_LAUNCH(s, F_GAUSS_VPASS, 24.0 * P, k_job_vpass<3><<<grd, blk, 0, s>>>(d_jobs, l, BUF_TA, g7));
+        for (int it = 0; it < 3; it++) {
+            TWX_LAUNCH(s, F_CALC_B, 32.0 * P, k_calc_b<<<grd, blk, 0, s>>>(d_jobs, l));
+            TWX_LAUNCH(s, F_GAUSS_HPASS, 16.0 * P, k_job_hpass<2><<<grd, blk, 0, s>>>(d_jobs, l, BUF_TB, g7));
+            TWX_LAUNCH(s, F_GAUSS_VPASS, 16.0 * P, k_job_vpass<2><<<grd, blk, 0, s>>>(d_jobs, l, BUF_TB, g7));
+            TWX_LAUNCH(s, F_NEW_DISP, 36.0 * P, k_new_disp<<<grd, blk, 0, s>>>(d_jobs, l));
+            TWX_LAUNCH(s, F_GAUSS_HPASS, 16.0 * P, k_job_hpass<2><<<grd, blk, 0, s>>>(d_jobs, l, BUF_FLOW, g5));
+            TWX_LAUNCH(s, F_GAUSS_VPASS, 16.0 * P, k_job_vpass<2><<<grd, blk, 0, s>>>(d_jobs, l, BUF_FLOW, g5));
+            if (it == 1 && l == 0 && !skip_refine) bm(l, mw, mh, P);
         }
-        if (l != 0 && !skip_refine) {
-            dim3 rb(64), rg((mh + 63) / 64, 1, nj), cg((mw + 63) / 64, 1, nj);
-            k_bm_rows<1><<<rg, rb, 0, s>>>(d_jobs, l);
-            k_bm_rows<-1><<<rg, rb, 0, s>>>(d_jobs, l);
-            k_bm_cols<1><<<cg, rb, 0, s>>>(d_jobs, l);
-            k_bm_cols<-1><<<cg, rb, 0, s>>>(d_jobs, l);
-        }
-        k_range_any<<<grd, blk, 0, s>>>(d_jobs, l);
-        k_range_apply<<<grd, blk, 0, s>>>(d_jobs, l);
-        k_range_update<<<dim3((nj + 63) / 64), dim3(64), 0, s>>>(d_jobs, nj, l);
+        if (l != 0 && !skip_refine) bm(l, mw, mh, P);
+        TWX_LAUNCH(s, F_RANGE, 8.0 * P, k_range_any<<<grd, blk, 0, s>>>(d_jobs, l));
+        TWX_LAUNCH(s, F_RANGE, 16.0 * P, k_range_apply<<<grd, blk, 0, s>>>(d_jobs, l));
+        TWX_LAUNCH(s, F_RANGE, 0.0, k_range_update<<<dim3((nj + 63) / 64), dim3(64), 0, s>>>(d_jobs, nj, l));
         if (l > 0) {
             int uw = 0, uh = 0;
             for (auto& j : jobs)
                 if (l <= j.start) { uw = std::max(uw, j.w[l - 1]); uh = std::max(uh, j.h[l - 1]); }
-            k_upsample<<<dim3((uw + 255) / 256, uh, nj), blk, 0, s>>>(d_jobs, l);
+            TWX_LAUNCH(s, F_UPSAMPLE, 8.0 * P + 8.0 * Pn, k_upsample<<<dim3((uw + 255) / 256, uh, nj), blk, 0, s>>>(d_jobs, l));
         }
     }
     int w0 = 0, h0 = 0;
-    for (auto& j : jobs) { w0 = std::max(w0, j.w[0]); h0 = std::max(h0, j.h[0]); }
-    k_negate<<<dim3((w0 + 255) / 256, h0, nj), dim3(256), 0, s>>>(d_jobs);
+    double P0 = 0;
+    for (auto& j : jobs) { w0 = std::max(w0, j.w[0]); h0 = std::max(h0, j.h[0]); P0 += (double)j.w[0] * j.h[0]; }
+    TWX_LAUNCH(s, F_NEGATE, 16.0 * P0, k_negate<<<dim3((w0 + 255) / 256, h0, nj), dim3(256), 0, s>>>(d_jobs));
 }
 
 __global__ void k_nan_scan(const float* __restrict__ d, size_t n, int* flag) {
@@ -893,9 +985,17 @@ void launch_frames(hipStream_t s, const FrameJob* d_jobs, const std::vector<Fram
     bool has[3] = {false, false, false};
     for (auto& f : jobs) { mw = std::max(mw, f.W); mh = std::max(mh, f.H); has[f.fmt] = true; }
     dim3 blk(256), grd((mw + 255) / 256, mh, (unsigned)jobs.size());
-    if (has[0]) k_frames<0><<<grd, blk, 0, s>>>(d_jobs);
-    if (has[1]) k_frames<1><<<grd, blk, 0, s>>>(d_jobs);
-    if (has[2]) k_frames<2><<<grd, blk, 0, s>>>(d_jobs);
+    // per output pixel: the two source pixels and the output (Bpp each), plus the flows read at
+    // working size (8 B per flow and working pixel)
+    double bytes = 0;
+    for (auto& f : jobs) {
+        double px = (double)f.W * f.H, bpp = 4.0 * sample_size(f.fmt);
+        double flows = f.kind == 0 ? 0.0 : (f.kind == 1 ? 8.0 : 16.0) * f.fw * f.fh;
+        bytes += (f.kind == 1 ? 2.0 : 3.0) * px * bpp + flows;
+    }
+    if (has[0]) TWX_LAUNCH(s, F_FRAMES, bytes, k_frames<0><<<grd, blk, 0, s>>>(d_jobs));
+    if (has[1]) TWX_LAUNCH(s, F_FRAMES, bytes, k_frames<1><<<grd, blk, 0, s>>>(d_jobs));
+    if (has[2]) TWX_LAUNCH(s, F_FRAMES, bytes, k_frames<2><<<grd, blk, 0, s>>>(d_jobs));
 }
 
 // -------------------------------------------------------------------------------------------
