@@ -449,8 +449,8 @@ int run_solves(hipStream_t s, DevPool& pool, const std::vector<SolveIn>& in, con
             start_level(fs.sensitivity, si.geo.levels, J.start, J.damp);
             J.scale_damp = (std::fabs((double)J.damp - 1.0) >= 1e-6 && J.damp != 1.0f) ? 1 : 0;
             size_t px = (size_t)si.geo.w0 * si.geo.h0;
-            J.flow[0] = si.out;
-            J.flow[1] = sp;
+            J.flow[0] = sp;      // where each level starts (the upsample target)
+            J.flow[1] = si.out;  // where each level ends: level 0 ends in the output
             J.grad = sp + px * 2;
             J.ta = sp + px * 4;
             J.tb = sp + px * 7;

@@ -344,7 +344,159 @@ void launch_subtract(hipStream_t s, const PlaneOp* d_ops, const std::vector<Plan
 }
 
 // -------------------------------------------------------------------------------------------
-// Solver kernels (flow.rs:281-442), one launch per step and level over all jobs
+// Solver kernels (flow.rs:281-442), one launch per step and level over all jobs.
+//
+// Flow buffers of a job: a level starts in flow[0]; the three Gaussian iterations go 0 -> 1 ->
+// 0 -> 1 (no kernel reads and writes one buffer, so tiles may overlap their halos freely); the
+// upsample writes the next level into flow[0]. flow[1] is the caller's output, so level 0 ends
+// in place.
+
+// One element of a separable pass over a line held with a stride, `origin` being the line index
+// of base[0]; the interior chain or the renormalised border (conv.rs:99-199, 272-311).
+template <int R, bool LEFT_FIRST, bool MULTI>
+__device__ __forceinline__ float line_elem(const float* __restrict__ base, int stride, int origin, int len, int i,
+                                           const Taps& t) {
+    constexpr int n = R + 1;
+    const int head = min(n, len);
+    const int interior = len > 2 * n ? len - 2 * n : 0;
+    const float* p = base + (i - origin) * stride;
+    if (i >= head && i < head + interior) {
+        float s = p[0] * t.k[0];
+#pragma unroll
+        for (int j = R; j >= 1; j--) s = s + (p[j * stride] + p[-j * stride]) * t.k[j];
+        return s;
+    }
+    float num = t.k[0] * p[0];
+    float den = t.k[0];
+#pragma unroll
+    for (int j = R; j >= 1; j--) {
+        const float kj = t.k[j];
+#pragma unroll
+        for (int side = 0; side < 2; side++) {
+            const bool right = (side == 0) != LEFT_FIRST;
+            const int q = right ? i + j : i - j;
+            const bool in = q >= 0 && q < len;
+            float xv, wt;
+            if (!MULTI) {
+                xv = in ? p[(right ? j : -j) * stride] : 0.0f;
+                wt = in ? kj : 0.0f;
+            } else {
+                float v = in ? p[(right ? j : -j) * stride] : -1.0f;
+                float mask = (v != -1.0f) ? 1.0f : 0.0f;
+                xv = v;
+                wt = mask * kj;
+            }
+            den = den + wt;
+            num = num + xv * wt;
+        }
+    }
+    return num / den;
+}
+
+enum Produce { P_TENSOR = 0, P_CALCB = 1, P_NEWDISP = 2 };
+
+// The value of the field the Gaussian runs on, at (x, y):
+//  P_TENSOR: calc_grad + calc_a (flow.rs:283-328); the tile's own pixels also store the gradient
+//  P_CALCB:  calc_b (flow.rs:334-357), I1 = X, I0 = Y, with the flow in buffer `src`
+//  P_NEWDISP: new_disp (flow.rs:361-371) applied to the flow in buffer `src`
+template <int MODE, int C>
+__device__ __forceinline__ void produce(const SolveJob& J, int level, int src, int w, int h, int x, int y, bool own,
+                                        float v[C]) {
+    const long i = (long)y * w + x;
+    if (MODE == P_TENSOR) {
+        const float* img = J.X[level];
+        float c = img[i];
+        float nx = 0.0f, dx = 0.0f, ny = 0.0f, dy = 0.0f;
+        if (x >= 1) { nx = (c - img[i - 1]) + 0.0f; dx = 1.0f; }
+        if (y >= 1) { ny = (c - img[i - w]) + 0.0f; dy = 1.0f; }
+        if (x + 1 <= w - 1) { nx = nx + (img[i + 1] - c); dx = dx + 1.0f; }
+        if (y + 1 <= h - 1) { ny = ny + (img[i + w] - c); dy = dy + 1.0f; }
+        float gx = nx / dx, gy = ny / dy;
+        if (own) {
+            J.grad[i * 2] = gx;
+            J.grad[i * 2 + 1] = gy;
+        }
+        v[0] = gx * gx;
+        v[1] = gx * gy;
+        v[C - 1] = gy * gy;
+    } else if (MODE == P_CALCB) {
+        const float* flow = J.flow[src];
+        float sx = (float)x - flow[i * 2];
+        float sy = (float)y - flow[i * 2 + 1];
+        float bx = 0.0f, by = 0.0f;
+        if (!(sx < 0.0f || (float)w - 1.0f < sx || sy < 0.0f || (float)h - 1.0f < sy)) {
+            float s = bilinear1(J.Y[level], w, h, sx, sy);
+            float r = s - J.X[level][i];
+            bx = r * J.grad[i * 2];
+            by = r * J.grad[i * 2 + 1];
+        }
+        v[0] = bx;
+        v[C - 1] = by;
+    } else {
+        const float* flow = J.flow[src];
+        float u = flow[i * 2], vv = flow[i * 2 + 1];
+        float aa = J.ta[i * 3], ab = J.ta[i * 3 + 1], ad = J.ta[i * 3 + 2];
+        float bx = J.tb[i * 2], by = J.tb[i * 2 + 1];
+        float t = ab * ab - ad * aa;
+        if (fabsf(t) > 40.0f) {
+            u = (ab * by - ad * bx) / t + u;
+            vv = (ab * bx - aa * by) / t + vv;
+        }
+        v[0] = u;
+        v[C - 1] = vv;
+    }
+}
+
+constexpr int TILE_X = 64, TILE_Y = 16, TILE_THREADS = 256;
+
+// A producer step fused with the separable Gaussian that follows it (flow.rs:1277-1287): the
+// tile's region (tile plus the kernel's reach on every side) is produced into LDS, the horizontal
+// pass runs over the region's rows, the vertical pass (border rows above first) over the tile.
+// Every element is computed exactly as by the separate passes, so the bits are the same.
+template <int C, int R, int MODE>
+__global__ __launch_bounds__(TILE_THREADS) void k_tile_gauss(const SolveJob* __restrict__ jobs, int level, int src,
+                                                             int dst, Taps t) {
+    const SolveJob& J = jobs[blockIdx.z];
+    if (level > J.start) return;
+    const int w = J.w[level], h = J.h[level];
+    const int x0 = blockIdx.x * TILE_X, y0 = blockIdx.y * TILE_Y;
+    if (x0 >= w || y0 >= h) return;
+    constexpr int RW = TILE_X + 2 * R, RH = TILE_Y + 2 * R;
+    __shared__ float s_in[RH * RW * C];
+    __shared__ float s_h[RH * TILE_X * C];
+    const int tid = threadIdx.x;
+    for (int k = tid; k < RH * RW; k += TILE_THREADS) {
+        const int ry = k / RW, rx = k - ry * RW;
+        const int x = x0 - R + rx, y = y0 - R + ry;
+        if (x < 0 || x >= w || y < 0 || y >= h) continue;
+        const bool own = rx >= R && rx < R + TILE_X && ry >= R && ry < R + TILE_Y;
+        float v[C];
+        produce<MODE, C>(J, level, src, w, h, x, y, own, v);
+#pragma unroll
+        for (int c = 0; c < C; c++) s_in[(ry * RW + rx) * C + c] = v[c];
+    }
+    __syncthreads();
+    for (int k = tid; k < RH * TILE_X; k += TILE_THREADS) {
+        const int ry = k / TILE_X, hx = k - ry * TILE_X;
+        const int x = x0 + hx, y = y0 - R + ry;
+        if (x >= w || y < 0 || y >= h) continue;
+        const float* line = s_in + ry * RW * C;
+#pragma unroll
+        for (int c = 0; c < C; c++)
+            s_h[(ry * TILE_X + hx) * C + c] = line_elem<R, false, (C > 1)>(line + c, C, x0 - R, w, x, t);
+    }
+    __syncthreads();
+    float* out = MODE == P_TENSOR ? J.ta : (MODE == P_CALCB ? J.tb : J.flow[dst]);
+    for (int k = tid; k < TILE_Y * TILE_X; k += TILE_THREADS) {
+        const int hy = k / TILE_X, hx = k - hy * TILE_X;
+        const int x = x0 + hx, y = y0 + hy;
+        if (x >= w || y >= h) continue;
+        const float* col = s_h + hx * C;
+#pragma unroll
+        for (int c = 0; c < C; c++)
+            out[((long)y * w + x) * C + c] = line_elem<R, true, (C > 1)>(col + c, TILE_X * C, y0 - R, h, y, t);
+    }
+}
 
 #define JOB_PROLOGUE()                                                    \
     const SolveJob& J = jobs[blockIdx.z];                                 \
@@ -354,88 +506,11 @@ void launch_subtract(hipStream_t s, const PlaneOp* d_ops, const std::vector<Plan
     const int y = blockIdx.y;                                             \
     if (x >= w || y >= h) return;
 
-// calc_grad + calc_a (flow.rs:283-328)
-__global__ void k_grad_tensor(const SolveJob* __restrict__ jobs, int level) {
-    JOB_PROLOGUE();
-    const float* img = J.X[level];
-    long i = (long)y * w + x;
-    float c = img[i];
-    float nx = 0.0f, dx = 0.0f, ny = 0.0f, dy = 0.0f;
-    if (x >= 1) { nx = (c - img[i - 1]) + 0.0f; dx = 1.0f; }
-    if (y >= 1) { ny = (c - img[i - w]) + 0.0f; dy = 1.0f; }
-    if (x + 1 <= w - 1) { nx = nx + (img[i + 1] - c); dx = dx + 1.0f; }
-    if (y + 1 <= h - 1) { ny = ny + (img[i + w] - c); dy = dy + 1.0f; }
-    float gx = nx / dx, gy = ny / dy;
-    J.grad[i * 2] = gx;
-    J.grad[i * 2 + 1] = gy;
-    J.ta[i * 3] = gx * gx;
-    J.ta[i * 3 + 1] = gx * gy;
-    J.ta[i * 3 + 2] = gy * gy;
-}
-
-// buffers of a job the in-place Gaussians run on
-enum Buf { BUF_TA = 0, BUF_TB = 1, BUF_FLOW = 2 };
-
-__device__ __forceinline__ float* job_buf(const SolveJob& J, int which, int level) {
-    return which == BUF_TA ? J.ta : (which == BUF_TB ? J.tb : J.flow[level & 1]);
-}
-
-template <int C>
-__global__ void k_job_hpass(const SolveJob* __restrict__ jobs, int level, int which, Taps t) {
-    JOB_PROLOGUE();
-    const float* src = job_buf(J, which, level);
-    const float* row = src + (long)y * w * C;
-#pragma unroll
-    for (int c = 0; c < C; c++) J.tmp[((long)y * w + x) * C + c] = hpass_elem<C>(row, w, x, c, t);
-}
-
-template <int C>
-__global__ void k_job_vpass(const SolveJob* __restrict__ jobs, int level, int which, Taps t) {
-    JOB_PROLOGUE();
-    float* dst = job_buf(J, which, level);
-#pragma unroll
-    for (int c = 0; c < C; c++) dst[((long)y * w + x) * C + c] = vpass_elem<C>(J.tmp, w, h, x, y, c, t);
-}
-
-// calc_b (flow.rs:334-357): I1 = X (imgA), I0 = Y (imgB)
-__global__ void k_calc_b(const SolveJob* __restrict__ jobs, int level) {
-    JOB_PROLOGUE();
-    const float* flow = J.flow[level & 1];
-    long i = (long)y * w + x;
-    float fx = (float)x, fy = (float)y;
-    float sx = fx - flow[i * 2];
-    float sy = fy - flow[i * 2 + 1];
-    float fw = (float)w, fh = (float)h;
-    float bx = 0.0f, by = 0.0f;
-    if (!(sx < 0.0f || fw - 1.0f < sx || sy < 0.0f || fh - 1.0f < sy)) {
-        float s = bilinear1(J.Y[level], w, h, sx, sy);
-        float r = s - J.X[level][i];
-        bx = r * J.grad[i * 2];
-        by = r * J.grad[i * 2 + 1];
-    }
-    J.tb[i * 2] = bx;
-    J.tb[i * 2 + 1] = by;
-}
-
-// new_disp (flow.rs:361-371)
-__global__ void k_new_disp(const SolveJob* __restrict__ jobs, int level) {
-    JOB_PROLOGUE();
-    float* flow = J.flow[level & 1];
-    long i = (long)y * w + x;
-    float aa = J.ta[i * 3], ab = J.ta[i * 3 + 1], ad = J.ta[i * 3 + 2];
-    float bx = J.tb[i * 2], by = J.tb[i * 2 + 1];
-    float t = ab * ab - ad * aa;
-    if (fabsf(t) > 40.0f) {
-        flow[i * 2] = (ab * by - ad * bx) / t + flow[i * 2];
-        flow[i * 2 + 1] = (ab * bx - aa * by) / t + flow[i * 2 + 1];
-    }
-}
-
 // all_flow_out_of_range (flow.rs:431-442): any pixel with 0 <= x+u <= w-1 and 0 <= y+v <= h-1
-__global__ void k_range_any(const SolveJob* __restrict__ jobs, int level) {
+__global__ void k_range_any(const SolveJob* __restrict__ jobs, int level, int buf) {
     JOB_PROLOGUE();
     if (J.state[0] == 0) return;
-    const float* flow = J.flow[level & 1];
+    const float* flow = J.flow[buf];
     long i = (long)y * w + x;
     float px = (float)x + flow[i * 2];
     float py = (float)y + flow[i * 2 + 1];
@@ -444,9 +519,9 @@ __global__ void k_range_any(const SolveJob* __restrict__ jobs, int level) {
 }
 
 // the range check result, then the damping at the start level (flow.rs:1291-1300)
-__global__ void k_range_apply(const SolveJob* __restrict__ jobs, int level) {
+__global__ void k_range_apply(const SolveJob* __restrict__ jobs, int level, int buf) {
     JOB_PROLOGUE();
-    float* flow = J.flow[level & 1];
+    float* flow = J.flow[buf];
     long i = (long)y * w + x;
     float u = flow[i * 2], v = flow[i * 2 + 1];
     if (J.state[0] != 0 && J.state[1] == 0) { u = 0.0f; v = 0.0f; }
@@ -464,7 +539,7 @@ __global__ void k_range_update(const SolveJob* __restrict__ jobs, int njobs, int
     J.state[1] = 0;
 }
 
-// upsample_flow (flow.rs:376-391) from level l into level l-1's buffer
+// upsample_flow (flow.rs:376-391) from level l (buffer 1) into level l-1 (buffer 0)
 __global__ void k_upsample(const SolveJob* __restrict__ jobs, int level) {
     const SolveJob& J = jobs[blockIdx.z];
     if (level > J.start || level == 0) return;
@@ -473,8 +548,8 @@ __global__ void k_upsample(const SolveJob* __restrict__ jobs, int level) {
     const int x = blockIdx.x * blockDim.x + threadIdx.x;
     const int y = blockIdx.y;
     if (x >= dw || y >= dh) return;
-    const float* src = J.flow[level & 1];
-    float* dst = J.flow[(level - 1) & 1];
+    const float* src = J.flow[1];
+    float* dst = J.flow[0];
     float sx = ((float)dw - 1.0f) / (float)(sw - 1);
     float sy = ((float)dh - 1.0f) / (float)(sh - 1);
     float rx = (float)(1.0 / (double)sx);
@@ -499,15 +574,15 @@ __global__ void k_upsample(const SolveJob* __restrict__ jobs, int level) {
     dst[o + 1] = out[1] * sy;
 }
 
-// final negation (flow.rs:1305) and the NaN scan the reference device path applies to the
-// downloaded flow (device/solver/mod.rs:575-582)
+// final negation (flow.rs:1305) of level 0's flow (buffer 1, the output) and the NaN scan the
+// reference device path applies to the downloaded flow (device/solver/mod.rs:575-582)
 __global__ void k_negate(const SolveJob* __restrict__ jobs) {
     const SolveJob& J = jobs[blockIdx.z];
     const int w = J.w[0], h = J.h[0];
     const int x = blockIdx.x * blockDim.x + threadIdx.x;
     const int y = blockIdx.y;
     if (x >= w || y >= h) return;
-    float* flow = J.flow[0];
+    float* flow = J.flow[1];
     long i = ((long)y * w + x) * 2;
     float u = -1.0f * flow[i], v = -1.0f * flow[i + 1];
     flow[i] = u;
@@ -518,9 +593,20 @@ __global__ void k_negate(const SolveJob* __restrict__ jobs) {
 // -------------------------------------------------------------------------------------------
 // Block matching (flow.rs:446-622)
 
-// Block::cost (flow.rs:472-524); false for None
-__device__ __forceinline__ bool bm_cost(const float* __restrict__ b, int w, int h, const float p[9], float x2, float y2,
+// Plane element (x, y) lives at x * SX + y * SY: (1, w) for the plain layout, (h, 1) for the
+// transposed copies the row sweeps use (so that the lanes of a wave, one row each, read
+// consecutive addresses). The arithmetic does not depend on the layout.
+
+// global-memory view of a device pointer (global loads rather than flat loads)
+typedef __attribute__((address_space(1))) const float gfloat;
+typedef __attribute__((address_space(1))) float gfloat_rw;
+
+// Block::cost (flow.rs:472-524); false for None. 32-bit element indices (planes < 2^31).
+template <bool T>
+__device__ __forceinline__ bool bm_cost(const float* __restrict__ bp, int w, int h, const float p[9], float x2, float y2,
                                         float& out) {
+    const float* b = bp;
+    const long SX = T ? h : 1, SY = T ? 1 : w;
     float wm2 = (float)(w - 2);
     float hm2 = (float)(h - 2);
     if (x2 > wm2 || y2 > hm2) return false;
@@ -535,15 +621,15 @@ __device__ __forceinline__ bool bm_cost(const float* __restrict__ b, int w, int 
     float w10 = fx - w11;
     float w01 = fy - w11;
     float w00 = ((1.0f - fx) - fy) + w11;
-    const float* base = b + ((long)iy - 1) * w + (ix - 1);
+    const float* base = b + ((long)iy - 1) * SY + ((long)ix - 1) * SX;
     float sq[9];
 #pragma unroll
     for (int bb = 0; bb < 3; bb++) {
 #pragma unroll
         for (int aa = 0; aa < 3; aa++) {
-            const float* t0 = base + (long)bb * w + aa;
-            const float* t1 = t0 + w;
-            float q = ((w00 * t0[0] + w10 * t0[1]) + w01 * t1[0]) + w11 * t1[1];
+            const float* t0 = base + bb * SY + aa * SX;
+            const float* t1 = t0 + SY;
+            float q = ((w00 * t0[0] + w10 * t0[SX]) + w01 * t1[0]) + w11 * t1[SX];
             float d = p[bb * 3 + aa] - q;
             sq[bb * 3 + aa] = d * d;
         }
@@ -556,27 +642,92 @@ __device__ __forceinline__ bool bm_cost(const float* __restrict__ b, int w, int 
 }
 
 // Block::step (flow.rs:528-544): true when the neighbour's vector replaces the pixel's
+template <bool T>
 __device__ __forceinline__ bool bm_step(const float* __restrict__ a, const float* __restrict__ b, int w, int h, int x, int y,
                                         float cu, float cv, float nu, float nv) {
+    const long SX = T ? h : 1, SY = T ? 1 : w;
     float tx = (float)x - cu;
     float ty = (float)y - cv;
     float w1 = (float)(w - 1), h1 = (float)(h - 1);
     if (!(0.0f <= tx && tx <= w1 && 0.0f <= ty && ty <= h1)) return false;
     float p[9];
-    const float* pa = a + ((long)y - 1) * w + (x - 1);
+    const float* pa = a + ((long)y - 1) * SY + ((long)x - 1) * SX;
 #pragma unroll
     for (int r = 0; r < 3; r++) {
-        p[r * 3] = pa[(long)r * w];
-        p[r * 3 + 1] = pa[(long)r * w + 1];
-        p[r * 3 + 2] = pa[(long)r * w + 2];
+        p[r * 3] = pa[r * SY];
+        p[r * 3 + 1] = pa[r * SY + SX];
+        p[r * 3 + 2] = pa[r * SY + 2 * SX];
     }
     float c0, c1;
-    if (!bm_cost(b, w, h, p, tx, ty, c0)) return false;
-    if (!bm_cost(b, w, h, p, (float)x - nu, (float)y - nv, c1)) return false;
+    if (!bm_cost<T>(b, w, h, p, tx, ty, c0)) return false;
+    if (!bm_cost<T>(b, w, h, p, (float)x - nu, (float)y - nv, c1)) return false;
     return c1 < c0;
 }
 
-// Sweeps 1 (DIR = +1, left to right) and 2 (DIR = -1): one chain per interior row
+// Transposed copies for the row sweeps: the flow (buffer `buf`) into tmp, X and Y into tb's two
+// planes; OUT copies tmp back into the flow. With them the lanes of a wave (one row each) read
+// consecutive addresses, as the column sweeps do on the plain layout.
+template <bool OUT>
+__global__ void k_bm_transpose(const SolveJob* __restrict__ jobs, int level, int buf) {
+    const SolveJob& J = jobs[blockIdx.z];
+    if (level > J.start) return;
+    const int w = J.w[level], h = J.h[level];
+    if (w < 3 || h < 3) return;
+    __shared__ float t[4][32][33];
+    const int bx = blockIdx.x * 32, by = blockIdx.y * 32;
+    if (bx >= w || by >= h) return;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 256 threads: 32 x 8
+    const long px = (long)w * h;
+    float* flow = J.flow[buf];
+    float* ft = J.tmp;
+    if (!OUT) {
+        for (int r = ty; r < 32; r += 8) {
+            int x = bx + tx, y = by + r;
+            if (x < w && y < h) {
+                long i = (long)y * w + x;
+                t[0][r][tx] = flow[i * 2];
+                t[1][r][tx] = flow[i * 2 + 1];
+                t[2][r][tx] = J.X[level][i];
+                t[3][r][tx] = J.Y[level][i];
+            }
+        }
+        __syncthreads();
+        for (int r = ty; r < 32; r += 8) {
+            int x = bx + r, y = by + tx;
+            if (x < w && y < h) {
+                long o = (long)x * h + y;
+                ft[o * 2] = t[0][tx][r];
+                ft[o * 2 + 1] = t[1][tx][r];
+                J.tb[o] = t[2][tx][r];
+                J.tb[px + o] = t[3][tx][r];
+            }
+        }
+    } else {
+        for (int r = ty; r < 32; r += 8) {
+            int x = bx + r, y = by + tx;
+            if (x < w && y < h) {
+                long o = (long)x * h + y;
+                t[0][tx][r] = ft[o * 2];
+                t[1][tx][r] = ft[o * 2 + 1];
+            }
+        }
+        __syncthreads();
+        for (int r = ty; r < 32; r += 8) {
+            int x = bx + tx, y = by + r;
+            if (x < w && y < h) {
+                long i = (long)y * w + x;
+                flow[i * 2] = t[0][r][tx];
+                flow[i * 2 + 1] = t[1][r][tx];
+            }
+        }
+    }
+}
+
+// Sweeps 1 (DIR = +1, left to right) and 2 (DIR = -1) of block matching (flow.rs:551-585): one
+// serial chain per interior row, one thread per chain, on the transposed copies (flow in tmp, X
+// and Y in tb) so that adjacent rows are adjacent lanes and every load is coalesced. (Whether a
+// pixel keeps its own vector decides the next pixel's input, and in practice more than half the
+// pixels take the incoming vector, in long runs: the chain does not split usefully.)
 template <int DIR>
 __global__ void k_bm_rows(const SolveJob* __restrict__ jobs, int level) {
     const SolveJob& J = jobs[blockIdx.z];
@@ -585,16 +736,18 @@ __global__ void k_bm_rows(const SolveJob* __restrict__ jobs, int level) {
     if (w < 3 || h < 3) return;
     int y = 1 + blockIdx.x * blockDim.x + threadIdx.x;
     if (y > h - 2) return;
-    float* row = J.flow[level & 1] + (long)y * w * 2;
-    const float* A = J.X[level];
-    const float* B = J.Y[level];
+    float* ft = J.tmp;
+    const float* A = J.tb;
+    const float* B = J.tb + (long)w * h;
     int x = DIR > 0 ? 1 : w - 2;
-    float nu = row[(x - DIR) * 2], nv = row[(x - DIR) * 2 + 1];
+    long ni = ((long)(x - DIR) * h + y) * 2;
+    float nu = ft[ni], nv = ft[ni + 1];
     for (int k = 0; k < w - 2; k++, x += DIR) {
-        float cu = row[x * 2], cv = row[x * 2 + 1];
-        if (bm_step(A, B, w, h, x, y, cu, cv, nu, nv)) {
-            row[x * 2] = nu;
-            row[x * 2 + 1] = nv;
+        long ci = ((long)x * h + y) * 2;
+        float cu = ft[ci], cv = ft[ci + 1];
+        if (bm_step<true>(A, B, w, h, x, y, cu, cv, nu, nv)) {
+            ft[ci] = nu;
+            ft[ci + 1] = nv;
         } else {
             nu = cu;
             nv = cv;
@@ -602,16 +755,17 @@ __global__ void k_bm_rows(const SolveJob* __restrict__ jobs, int level) {
     }
 }
 
-// Sweeps 3 (DIR = +1, top to bottom) and 4 (DIR = -1): one chain per interior column
+// Sweeps 3 (DIR = +1, top to bottom) and 4 (DIR = -1) (flow.rs:586-614): one chain per interior
+// column, adjacent columns on adjacent lanes.
 template <int DIR>
-__global__ void k_bm_cols(const SolveJob* __restrict__ jobs, int level) {
+__global__ void k_bm_cols(const SolveJob* __restrict__ jobs, int level, int buf) {
     const SolveJob& J = jobs[blockIdx.z];
     if (level > J.start) return;
     const int w = J.w[level], h = J.h[level];
     if (w < 3 || h < 3) return;
     int x = 1 + blockIdx.x * blockDim.x + threadIdx.x;
     if (x > w - 2) return;
-    float* flow = J.flow[level & 1];
+    float* flow = J.flow[buf];
     const float* A = J.X[level];
     const float* B = J.Y[level];
     int y = DIR > 0 ? 1 : h - 2;
@@ -620,7 +774,7 @@ __global__ void k_bm_cols(const SolveJob* __restrict__ jobs, int level) {
     for (int k = 0; k < h - 2; k++, y += DIR) {
         long ci = ((long)y * w + x) * 2;
         float cu = flow[ci], cv = flow[ci + 1];
-        if (bm_step(A, B, w, h, x, y, cu, cv, nu, nv)) {
+        if (bm_step<false>(A, B, w, h, x, y, cu, cv, nu, nv)) {
             flow[ci] = nu;
             flow[ci + 1] = nv;
         } else {
@@ -633,13 +787,13 @@ __global__ void k_bm_cols(const SolveJob* __restrict__ jobs, int level) {
 // -------------------------------------------------------------------------------------------
 // The solve driver: per level, the kernel sequence of flow.rs:1271-1304 over every job.
 
-// the flow starts at zero at the start level (in the buffer of that level's parity); the range
-// check starts active (flow.rs:1268-1270)
+// the flow starts at zero at the start level, in buffer 0; the range check starts active
+// (flow.rs:1268-1270)
 __global__ void k_solve_init(const SolveJob* __restrict__ jobs) {
     const SolveJob& J = jobs[blockIdx.z];
     long n = (long)J.w[J.start] * J.h[J.start] * 2;
     long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) J.flow[J.start & 1][i] = 0.0f;
+    if (i < n) J.flow[0][i] = 0.0f;
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         J.state[0] = 1;
         J.state[1] = 0;
@@ -658,12 +812,17 @@ void launch_solve(hipStream_t s, const SolveJob* d_jobs, const std::vector<Solve
     }
     TWX_LAUNCH(s, F_RANGE, 0.0, k_solve_init<<<dim3((unsigned)((max_init + 255) / 256), 1, nj), dim3(256), 0, s>>>(d_jobs));
     const Taps g7 = gaussian_taps(7.0f), g5 = gaussian_taps(5.0f);
-    auto bm = [&](int l, int mw, int mh, double P) {
-        dim3 rb(64), rg((mh + 63) / 64, 1, nj), cg((mw + 63) / 64, 1, nj);
-        TWX_LAUNCH(s, F_BM_ROWS, 24.0 * P, k_bm_rows<1><<<rg, rb, 0, s>>>(d_jobs, l));
-        TWX_LAUNCH(s, F_BM_ROWS, 24.0 * P, k_bm_rows<-1><<<rg, rb, 0, s>>>(d_jobs, l));
-        TWX_LAUNCH(s, F_BM_COLS, 24.0 * P, k_bm_cols<1><<<cg, rb, 0, s>>>(d_jobs, l));
-        TWX_LAUNCH(s, F_BM_COLS, 24.0 * P, k_bm_cols<-1><<<cg, rb, 0, s>>>(d_jobs, l));
+    // block matching on the flow in buffer `buf`
+    auto bm = [&](int l, int mw, int mh, double P, int buf) {
+        dim3 tb(256), tg((mw + 31) / 32, (mh + 31) / 32, nj);
+        dim3 cb(64), rg((mh + 63) / 64, 1, nj), cg((mw + 63) / 64, 1, nj);
+        // transposes 24 + 16 B/px, sweeps: A 4 + B 4 + flow read and write 16 B/px each
+        TWX_LAUNCH(s, F_BM_ROWS, 24.0 * P, k_bm_transpose<false><<<tg, tb, 0, s>>>(d_jobs, l, buf));
+        TWX_LAUNCH(s, F_BM_ROWS, 24.0 * P, k_bm_rows<1><<<rg, cb, 0, s>>>(d_jobs, l));
+        TWX_LAUNCH(s, F_BM_ROWS, 24.0 * P, k_bm_rows<-1><<<rg, cb, 0, s>>>(d_jobs, l));
+        TWX_LAUNCH(s, F_BM_ROWS, 16.0 * P, k_bm_transpose<true><<<tg, tb, 0, s>>>(d_jobs, l, buf));
+        TWX_LAUNCH(s, F_BM_COLS, 24.0 * P, k_bm_cols<1><<<cg, cb, 0, s>>>(d_jobs, l, buf));
+        TWX_LAUNCH(s, F_BM_COLS, 24.0 * P, k_bm_cols<-1><<<cg, cb, 0, s>>>(d_jobs, l, buf));
     };
     for (int l = max_start; l >= 0; l--) {
         int mw = 0, mh = 0;
@@ -676,21 +835,20 @@ void launch_solve(hipStream_t s, const SolveJob* d_jobs, const std::vector<Solve
                 if (l > 0) Pn += (double)j.w[l - 1] * j.h[l - 1];
             }
         dim3 blk(256), grd((mw + 255) / 256, mh, nj);
-        TWX_LAUNCH(s, F_GRAD_TENSOR, 24.0 * P, k_grad_tensor<<<grd, blk, 0, s>>>(d_jobs, l));
-        TWX_LAUNCH(s, F_GAUSS_HPASS, 24.0 * P, k_job_hpass<3><<<grd, blk, 0, s>>>(d_jobs, l, BUF_TA, g7));
-        TWX_LAUNCH(s, F_GAUSS_VPASS, 24.0 * P, k_job_vpass<3><<<grd, blk, 0, s>>>(d_jobs, l, BUF_TA, g7));
+        dim3 tgrid((mw + TILE_X - 1) / TILE_X, (mh + TILE_Y - 1) / TILE_Y, nj), tblk(TILE_THREADS);
+        // grad + tensor, blurred r = 7: X 4 B in, grad 8 + tensor 12 B out
+        TWX_LAUNCH(s, F_GRAD_TENSOR, 24.0 * P, (k_tile_gauss<3, 7, P_TENSOR><<<tgrid, tblk, 0, s>>>(d_jobs, l, 0, 0, g7)));
         for (int it = 0; it < 3; it++) {
-            TWX_LAUNCH(s, F_CALC_B, 32.0 * P, k_calc_b<<<grd, blk, 0, s>>>(d_jobs, l));
-            TWX_LAUNCH(s, F_GAUSS_HPASS, 16.0 * P, k_job_hpass<2><<<grd, blk, 0, s>>>(d_jobs, l, BUF_TB, g7));
-            TWX_LAUNCH(s, F_GAUSS_VPASS, 16.0 * P, k_job_vpass<2><<<grd, blk, 0, s>>>(d_jobs, l, BUF_TB, g7));
-            TWX_LAUNCH(s, F_NEW_DISP, 36.0 * P, k_new_disp<<<grd, blk, 0, s>>>(d_jobs, l));
-            TWX_LAUNCH(s, F_GAUSS_HPASS, 16.0 * P, k_job_hpass<2><<<grd, blk, 0, s>>>(d_jobs, l, BUF_FLOW, g5));
-            TWX_LAUNCH(s, F_GAUSS_VPASS, 16.0 * P, k_job_vpass<2><<<grd, blk, 0, s>>>(d_jobs, l, BUF_FLOW, g5));
-            if (it == 1 && l == 0 && !skip_refine) bm(l, mw, mh, P);
+            const int src = it & 1, dst = src ^ 1;
+            // calc_b + B blur: flow 8 + X 4 + Y 4 + grad 8 in, B 8 out
+            TWX_LAUNCH(s, F_CALC_B, 32.0 * P, (k_tile_gauss<2, 7, P_CALCB><<<tgrid, tblk, 0, s>>>(d_jobs, l, src, 0, g7)));
+            // new_disp + flow blur: tensor 12 + B 8 + flow 8 in, flow 8 out
+            TWX_LAUNCH(s, F_NEW_DISP, 36.0 * P, (k_tile_gauss<2, 5, P_NEWDISP><<<tgrid, tblk, 0, s>>>(d_jobs, l, src, dst, g5)));
+            if (it == 1 && l == 0 && !skip_refine) bm(l, mw, mh, P, 0);
         }
-        if (l != 0 && !skip_refine) bm(l, mw, mh, P);
-        TWX_LAUNCH(s, F_RANGE, 8.0 * P, k_range_any<<<grd, blk, 0, s>>>(d_jobs, l));
-        TWX_LAUNCH(s, F_RANGE, 16.0 * P, k_range_apply<<<grd, blk, 0, s>>>(d_jobs, l));
+        if (l != 0 && !skip_refine) bm(l, mw, mh, P, 1);
+        TWX_LAUNCH(s, F_RANGE, 8.0 * P, k_range_any<<<grd, blk, 0, s>>>(d_jobs, l, 1));
+        TWX_LAUNCH(s, F_RANGE, 16.0 * P, k_range_apply<<<grd, blk, 0, s>>>(d_jobs, l, 1));
         TWX_LAUNCH(s, F_RANGE, 0.0, k_range_update<<<dim3((nj + 63) / 64), dim3(64), 0, s>>>(d_jobs, nj, l));
         if (l > 0) {
             int uw = 0, uh = 0;
