@@ -25,6 +25,10 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 SPEED = "speed_percent=100@0;25@60;100@120"
 
+# Measured HBM bytes over algorithmic bytes per kernel family, from the FETCH_SIZE / WRITE_SIZE
+# counter runs (copy-calibrated) recorded in profiles/calc_b_gauss_traffic_pmc.md.
+TRAFFIC_RATIO = {"calc_b_gauss": 2.00, "new_disp_gauss": 1.30, "tensor_gauss": 1.25}
+
 
 def main():
     ap = argparse.ArgumentParser()
@@ -36,7 +40,19 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--per-pair", action="store_true", help="also time one launch sequence per output frame")
     ap.add_argument("--no-profile", action="store_true", help="time the steps without the per-launch HIP events")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only run one device-to-device copy of MIB MiB (calibrates the HBM traffic counters)")
     args = ap.parse_args()
+    if args.copy_probe:
+        import torch
+
+        src = torch.ones(args.copy_probe << 18, dtype=torch.float32, device="cuda:0")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes": src.numel() * 4}))
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: this build has no multi-GPU path (asked for {args.gpus}); "
               "a clip shards by output-frame range, see DESIGN.md", file=sys.stderr)
@@ -88,8 +104,11 @@ def main():
         k = prof[kern]
         avg_ms = k["ms"] / max(k["launches"], 1)
         achieved = (k["bytes"] / max(k["launches"], 1)) / (avg_ms / 1000.0) / 1e9 if avg_ms > 0 else 0.0
+        alg_per_launch = k["bytes"] / max(k["launches"], 1)
+        traffic = round(TRAFFIC_RATIO[kern] * alg_per_launch) if kern in TRAFFIC_RATIO else None
         roofline = {"bound": "hbm", "kernel": kern, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                    "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                    "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                    "algorithmic_bytes_per_launch": round(alg_per_launch),
                     "avg_launch_ms": round(avg_ms, 4), "launches_per_step": k["launches"] // args.steps,
                     "share_of_step": round(k["ms"] / args.steps / ms_per_step, 3)}
     # whole-path algorithmic bytes per step over the step time

@@ -75,8 +75,8 @@ struct FrameJob {
 // Per-kernel-family timing with HIP events on the launching stream, and the algorithmic bytes of
 // each launch (the step-boundary model: distinct inputs read once, outputs written once).
 enum ProfFam {
-    F_INGEST, F_LUMA, F_PYR_HPASS, F_PYR_VPASS, F_PYR_RESAMPLE, F_PYR_SUBTRACT, F_GRAD_TENSOR, F_GAUSS_HPASS,
-    F_GAUSS_VPASS, F_CALC_B, F_NEW_DISP, F_BM_ROWS, F_BM_COLS, F_RANGE, F_UPSAMPLE, F_NEGATE, F_FRAMES, F_NFAM
+    F_INGEST, F_LUMA, F_PYR_HPASS, F_PYR_VPASS, F_PYR_RESAMPLE, F_PYR_SUBTRACT, F_GRAD_TENSOR, F_BM_TRANSPOSE,
+    F_SOLVE_INIT, F_CALC_B, F_NEW_DISP, F_BM_ROWS, F_BM_COLS, F_RANGE, F_UPSAMPLE, F_NEGATE, F_FRAMES, F_NFAM
 };
 extern const char* const PROF_NAMES[F_NFAM];
 

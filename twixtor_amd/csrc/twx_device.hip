@@ -95,8 +95,8 @@ void launch_ingest(hipStream_t s, const void* wire, void* engine, int W, int H, 
 // ---- profiler (host) ----
 
 const char* const PROF_NAMES[F_NFAM] = {"ingest", "luma", "pyr_hpass", "pyr_vpass", "pyr_resample", "pyr_subtract",
-                                        "grad_tensor", "gauss_hpass", "gauss_vpass", "calc_b", "new_disp", "bm_rows",
-                                        "bm_cols", "range", "upsample", "negate", "frames"};
+                                        "tensor_gauss", "bm_transpose", "solve_init", "calc_b_gauss", "new_disp_gauss",
+                                        "bm_rows", "bm_cols", "range", "upsample", "negate", "frames"};
 thread_local Prof* g_prof = nullptr;
 
 hipEvent_t Prof::take() {
@@ -810,17 +810,17 @@ void launch_solve(hipStream_t s, const SolveJob* d_jobs, const std::vector<Solve
         max_start = std::max(max_start, j.start);
         max_init = std::max(max_init, (long)j.w[j.start] * j.h[j.start] * 2);
     }
-    TWX_LAUNCH(s, F_RANGE, 0.0, k_solve_init<<<dim3((unsigned)((max_init + 255) / 256), 1, nj), dim3(256), 0, s>>>(d_jobs));
+    TWX_LAUNCH(s, F_SOLVE_INIT, 0.0, k_solve_init<<<dim3((unsigned)((max_init + 255) / 256), 1, nj), dim3(256), 0, s>>>(d_jobs));
     const Taps g7 = gaussian_taps(7.0f), g5 = gaussian_taps(5.0f);
     // block matching on the flow in buffer `buf`
     auto bm = [&](int l, int mw, int mh, double P, int buf) {
         dim3 tb(256), tg((mw + 31) / 32, (mh + 31) / 32, nj);
         dim3 cb(64), rg((mh + 63) / 64, 1, nj), cg((mw + 63) / 64, 1, nj);
         // transposes 24 + 16 B/px, sweeps: A 4 + B 4 + flow read and write 16 B/px each
-        TWX_LAUNCH(s, F_BM_ROWS, 24.0 * P, k_bm_transpose<false><<<tg, tb, 0, s>>>(d_jobs, l, buf));
+        TWX_LAUNCH(s, F_BM_TRANSPOSE, 24.0 * P, k_bm_transpose<false><<<tg, tb, 0, s>>>(d_jobs, l, buf));
         TWX_LAUNCH(s, F_BM_ROWS, 24.0 * P, k_bm_rows<1><<<rg, cb, 0, s>>>(d_jobs, l));
         TWX_LAUNCH(s, F_BM_ROWS, 24.0 * P, k_bm_rows<-1><<<rg, cb, 0, s>>>(d_jobs, l));
-        TWX_LAUNCH(s, F_BM_ROWS, 16.0 * P, k_bm_transpose<true><<<tg, tb, 0, s>>>(d_jobs, l, buf));
+        TWX_LAUNCH(s, F_BM_TRANSPOSE, 16.0 * P, k_bm_transpose<true><<<tg, tb, 0, s>>>(d_jobs, l, buf));
         TWX_LAUNCH(s, F_BM_COLS, 24.0 * P, k_bm_cols<1><<<cg, cb, 0, s>>>(d_jobs, l, buf));
         TWX_LAUNCH(s, F_BM_COLS, 24.0 * P, k_bm_cols<-1><<<cg, cb, 0, s>>>(d_jobs, l, buf));
     };
