@@ -122,13 +122,17 @@ int twx_solve_pair(twx_handle* h, const void* a, const void* b, int32_t width, i
 
 /* ---- Backend::compose_vectors (compose/vector.rs:209-284) -------------------------------- */
 
-/* Warping code 0 (inverse warps with hold edges, mode 1, no 360) for interp 0 (Nearest) and
- * 1 (Blend); wa is the weight of A (Selection::wa). For Nearest only the flow that
- * nearest_uses_f_ab names is read (the other may be NULL). Host engine images. */
+/* Warping code 0 (inverse warps with hold edges, mode 1, no 360) for interp 0 (Nearest),
+ * 1 (Blend) and 2 (Weighted Blend: each warped frame through the sharpening step against its
+ * unwarped frame, filters.rs:649-656, then the blend); wa is the weight of A (Selection::wa).
+ * For Nearest only the flow that nearest_uses_f_ab names is read (the other may be NULL).
+ * `taps_above_first` is the vertical tap order of the Weighted Blend blur (the stage set's
+ * vertical_taps, stages.rs:60-62): 1 for the retime front-end (frontend.rs:238-241), 0 for the
+ * plain CPU stages. Host engine images. */
 int twx_compose_vectors(twx_handle* h, int32_t interp, float wa, const void* a, const void* b,
                         int32_t width, int32_t height, int32_t fmt, const float* f_ab,
-                        const float* f_ba, int32_t flow_w, int32_t flow_h, void* out,
-                        char* err, size_t errlen);
+                        const float* f_ba, int32_t flow_w, int32_t flow_h,
+                        int32_t taps_above_first, void* out, char* err, size_t errlen);
 
 /* ---- the per-frame render loop of `vfx apply retime` ------------------------------------- */
 

@@ -1065,6 +1065,302 @@ static void blend(void* out, const void* a, const void* b, long w, long h, int f
 }
 
 // ------------------------------------------------------------------------------------------
+// The filters of the Weighted Blend step (filters.rs:54-712): the separable blur, the difference
+// mask and its clean-up, the masked composite and the sharpening step. Finite samples only (the
+// NaN rule of nanrule.rs is not restated).
+
+static const float BLUR_RADIUS = 3.0f;   // filters.rs:55
+static const float MASK_THRESHOLD = 0.4f;  // filters.rs:285
+
+// filters.rs:71-95: w[i] = powf(2, -(i*i)/t^2) for i <= (int)radius, over sum * 2 - w[0]
+static std::vector<float> blur_taps(float radius) {
+    float t = radius / 3.0f;
+    float t2 = t * t;
+    int count = (std::isnan(radius) || radius < 0.0f) ? 1 : (int)radius + 1;  // Rust `as i32` saturates
+    std::vector<float> w;
+    float sum = 0.0f;
+    for (int i = 0; i < count; i++) {
+        float fi = (float)i;
+        float x = fi * fi;
+        x = x / t2;
+        x = -x;
+        float p = powf(2.0f, x);
+        w.push_back(p);
+        sum += p;
+    }
+    float denom = sum * 2.0f - w[0];
+    for (auto& p : w) p /= denom;
+    return w;
+}
+
+// filters.rs:190-236, one output pixel of the 1D filter. `get(j)` gives the four lanes of pixel j
+// of a line of n pixels; the sentinel -1.0 stands for a tap outside the line, and in the edge
+// formula a lane that equals it carries no weight.
+template <class Get>
+static inline void fconv_pixel(long i, long n, const float* taps, int count, const Get& get, bool left_first,
+                               float out[4]) {
+    const float S = -1.0f;
+    const float sent[4] = {S, S, S, S};
+    long left = std::min(n, (long)count);
+    long mid = n > 2L * count ? n - 2L * count : 0;
+    const float* c = get(i);
+    float w0 = taps[0];
+    float acc[4];
+    for (int l = 0; l < 4; l++) acc[l] = c[l] * w0;
+    if (i >= left && i < left + mid) {
+        for (int k = count - 1; k >= 1; k--) {
+            const float* r = get(i + k);
+            const float* lf = get(i - k);
+            for (int l = 0; l < 4; l++) acc[l] = acc[l] + (r[l] + lf[l]) * taps[k];
+        }
+        for (int l = 0; l < 4; l++) out[l] = acc[l];
+        return;
+    }
+    float wsum[4] = {w0, w0, w0, w0};
+    for (int k = count - 1; k >= 1; k--) {
+        const float* r = i + k < n ? get(i + k) : sent;
+        const float* lf = i >= k ? get(i - k) : sent;
+        if (left_first) std::swap(r, lf);
+        float wk = taps[k];
+        for (int l = 0; l < 4; l++) {
+            float wr = (r[l] != S ? 1.0f : 0.0f) * wk;
+            wsum[l] = wsum[l] + wr;
+            float rt = wr * r[l];
+            float wl = (lf[l] != S ? 1.0f : 0.0f) * wk;
+            float lt = lf[l] * wl;
+            wsum[l] = wsum[l] + wl;
+            acc[l] = (acc[l] + rt) + lt;
+        }
+    }
+    for (int l = 0; l < 4; l++) out[l] = acc[l] / wsum[l];
+}
+
+// filters.rs:112-119: trunc(v + 0.5), clamped to 0..max, trunc(x + 0.5) again
+static inline int32_t store_int(float v, float max) {
+    int32_t t = cvtt(v + 0.5f);
+    float f = (float)t;
+    f = f > 0.0f ? f : 0.0f;
+    f = f < max ? f : max;
+    return cvtt(f + 0.5f);
+}
+template <class T> static inline T fstore(float v);
+template <> inline uint8_t fstore<uint8_t>(float v) { return (uint8_t)store_int(v, 255.0f); }
+template <> inline uint16_t fstore<uint16_t>(float v) { return (uint16_t)store_int(v, 65535.0f); }
+template <> inline float fstore<float>(float v) { return v; }
+
+// filters.rs:240-266: rows, then columns on a snapshot of the row result
+template <class T>
+static void blur_plane(T* d, long w, long h, const std::vector<float>& taps, bool above_first) {
+    if (w == 0 || h == 0) return;
+    int count = (int)taps.size();
+#pragma omp parallel for schedule(static)
+    for (long y = 0; y < h; y++) {
+        std::vector<float> line(w * 4);
+        for (long i = 0; i < w * 4; i++) line[i] = (float)d[y * w * 4 + i];
+        auto get = [&](long j) { return &line[j * 4]; };
+        for (long x = 0; x < w; x++) {
+            float v[4];
+            fconv_pixel(x, w, taps.data(), count, get, false, v);
+            for (int l = 0; l < 4; l++) d[(y * w + x) * 4 + l] = fstore<T>(v[l]);
+        }
+    }
+    std::vector<T> snap(d, d + w * h * 4);
+#pragma omp parallel for schedule(static)
+    for (long x = 0; x < w; x++) {
+        std::vector<float> col(h * 4);
+        for (long y = 0; y < h; y++)
+            for (int l = 0; l < 4; l++) col[y * 4 + l] = (float)snap[(y * w + x) * 4 + l];
+        auto get = [&](long j) { return &col[j * 4]; };
+        for (long y = 0; y < h; y++) {
+            float v[4];
+            fconv_pixel(y, h, taps.data(), count, get, above_first, v);
+            for (int l = 0; l < 4; l++) d[(y * w + x) * 4 + l] = fstore<T>(v[l]);
+        }
+    }
+}
+
+// filters.rs:272-283 (blur_with): a negative radius does nothing
+static void blur_with(void* img, long w, long h, int fmt, float radius, bool above_first) {
+    if (0.0 > (double)radius) return;
+    std::vector<float> taps = blur_taps(radius);
+    if (fmt == TWX_FMT_U8) blur_plane((uint8_t*)img, w, h, taps, above_first);
+    else if (fmt == TWX_FMT_U16) blur_plane((uint16_t*)img, w, h, taps, above_first);
+    else blur_plane((float*)img, w, h, taps, above_first);
+}
+
+// filters.rs:287-293: ((threshold * white) * threshold) * threshold * 3
+static float mask_limit(int fmt, float threshold) {
+    float white = fmt == TWX_FMT_U8 ? 255.0f : (fmt == TWX_FMT_U16 ? 65535.0f : 1.0f);
+    return threshold * white * threshold * threshold * 3.0f;
+}
+
+// filters.rs:295-335: 255 where (|dR| + |dG|) + |dB| >= limit, differences in float
+template <class T>
+static void difference_rows(const T* a, const T* b, long n, float limit, uint8_t* m) {
+#pragma omp parallel for schedule(static)
+    for (long i = 0; i < n; i++) {
+        auto d = [&](int c) { return std::fabs((float)a[i * 4 + c] - (float)b[i * 4 + c]); };
+        float sum = (d(1) + d(2)) + d(3);
+        m[i] = sum >= limit ? 0xff : 0;
+    }
+}
+static void difference_mask(const void* warped, const void* original, long w, long h, int fmt, float threshold,
+                            uint8_t* m) {
+    float limit = mask_limit(fmt, threshold);
+    long n = w * h;
+    if (fmt == TWX_FMT_U8) difference_rows((const uint8_t*)warped, (const uint8_t*)original, n, limit, m);
+    else if (fmt == TWX_FMT_U16) difference_rows((const uint16_t*)warped, (const uint16_t*)original, n, limit, m);
+    else difference_rows((const float*)warped, (const float*)original, n, limit, m);
+}
+
+// filters.rs:345-350: the ring of eight around (x + 1, r1) from rows r0, r1, r2
+static inline int ring_sum(const uint8_t* t, long w, long x, long r0, long r1, long r2) {
+    auto at = [&](long y, long dx) { return (int)t[y * w + x + dx]; };
+    return at(r0, 1) + at(r0, 0) + at(r0, 2) + at(r1, 0) + at(r1, 2) + at(r2, 0) + at(r2, 1) + at(r2, 2);
+}
+
+// filters.rs:352-375
+static void mask_prune(uint8_t* m, long w, long h, int limit) {
+    if (h < 5 || w < 3) return;
+    std::vector<uint8_t> t(m, m + w * h);
+    for (long y0 = 0; y0 < h - 4; y0++)
+        for (long x = 0; x < w - 2; x++) {
+            int sum = ring_sum(t.data(), w, x, y0, y0 + 2, y0 + 4);
+            long dst = (y0 + 2) * w + x + 1;
+            if (sum < limit && m[dst] != 0) m[dst] = 0;
+        }
+}
+
+// filters.rs:384-399
+static void mask_fill(uint8_t* m, long w, long h, int limit, uint8_t value) {
+    if (h < 3 || w < 3) return;
+    std::vector<uint8_t> t(m, m + w * h);
+    for (long y0 = 0; y0 < h - 2; y0++)
+        for (long x = 0; x < w - 2; x++) {
+            int sum = ring_sum(t.data(), w, x, y0, y0 + 1, y0 + 2);
+            long dst = (y0 + 1) * w + x + 1;
+            if (limit < sum && m[dst] == 0) m[dst] = value;
+        }
+}
+
+// filters.rs:403-417
+static void mask_dilate_rows(uint8_t* m, long w, long h) {
+    if (h < 3) return;
+    std::vector<uint8_t> t(m, m + w * h);
+    for (long y = 1; y < h - 1; y++)
+        for (long x = 0; x < w; x++)
+            if (t[(y - 1) * w + x] == 255 || t[(y + 1) * w + x] == 255) m[y * w + x] = 255;
+}
+
+// filters.rs:423-484: the rectangle tests in float, as the plugin runs them for any size
+static void mask_replicate_border(uint8_t* m, long wl, long hl) {
+    int64_t w = wl, h = hl;
+    if (w == 0 || h == 0) return;
+    float p2 = 1.0f, p3 = 1.0f, p4 = (float)(w - 2), p5 = (float)(h - 2);
+    int64_t iv14 = (int64_t)(int32_t)p5;
+    float fy0 = p3, fy1 = p5;
+    if (p5 < p3) { fy0 = p5; fy1 = p3; }
+    float fx0 = p2, fx1 = p4;
+    if (p4 < p2) { fx0 = p4; fx1 = p2; }
+    auto ok = [&](int64_t x, int64_t y) { return x >= 0 && x < w && y >= 0 && y < h; };
+    auto outside = [&](int64_t x, int64_t y) {
+        float xf = (float)x, yf = (float)y;
+        return yf < fy0 || fy1 < yf || fx1 < xf || xf < fx0;
+    };
+    for (int64_t y = 0; (float)y < fy0; y++)
+        for (int64_t x = 0; x < w; x++) {
+            int64_t sy = (int64_t)(int32_t)p3;
+            if (outside(x, y) && ok(x, y) && ok(x, sy)) m[y * w + x] = m[sy * w + x];
+        }
+    for (int64_t y = (int64_t)(int32_t)fy1; y < h; y++)
+        for (int64_t x = 0; x < w; x++)
+            if (outside(x, y) && ok(x, y) && ok(x, iv14)) m[y * w + x] = m[iv14 * w + x];
+    for (int64_t y = 0; y < h; y++) {
+        int64_t sx = (int64_t)(int32_t)p2;
+        for (int64_t x = 0; (float)x < fx0; x++)
+            if (ok(x, y) && ok(sx, y)) m[y * w + x] = m[y * w + sx];
+        int64_t first = (int64_t)(int32_t)fx1;
+        int64_t base = (int64_t)(int32_t)p4;
+        for (int64_t j = 0; first + j < w; j++)
+            if (ok(base + j, y) && ok(base, y)) m[y * w + base + j] = m[y * w + base];
+    }
+}
+
+// filters.rs:495-508
+static void clean_mask(uint8_t* m, long w, long h) {
+    mask_prune(m, w, h, 254);
+    mask_prune(m, w, h, 509);
+    mask_dilate_rows(m, w, h);
+    for (int i = 0; i < 2; i++) {
+        mask_fill(m, w, h, 254, 255);
+        mask_replicate_border(m, w, h);
+    }
+    int limit = 254, value = 128;
+    for (int i = 0; i < 2; i++) {
+        mask_fill(m, w, h, limit, (uint8_t)value);
+        mask_replicate_border(m, w, h);
+        limit = (int)((double)limit * 0.5 + 0.5);
+        value = (int)((double)value * 0.5 + 0.5);
+    }
+}
+
+// filters.rs:532-607 and 611-616: mask 0 takes src, else f = m / 255, g = 1 - f (in double)
+template <class T>
+static void composite_rows(T* d, const T* s, long n, const uint8_t* m) {
+#pragma omp parallel for schedule(static)
+    for (long i = 0; i < n; i++) {
+        uint8_t mv = m[i];
+        for (int c = 0; c < 4; c++) {
+            long k = i * 4 + c;
+            if (mv == 0) { d[k] = s[k]; continue; }
+            float f = (float)mv / 255.0f;
+            float g = (float)(1.0 - (double)f);
+            d[k] = (T)cvtt64f(((float)s[k] * g + (float)d[k] * f) + 0.5f);
+        }
+    }
+}
+static void composite_rows_f32(float* d, const float* s, long n, const uint8_t* m) {
+#pragma omp parallel for schedule(static)
+    for (long i = 0; i < n; i++) {
+        uint8_t mv = m[i];
+        for (int c = 0; c < 4; c++) {
+            long k = i * 4 + c;
+            if (mv == 0) { d[k] = s[k]; continue; }
+            float f = (float)mv / 255.0f;
+            float g = (float)(1.0 - (double)f);
+            d[k] = (g * s[k] + f * d[k]) + 0.0f;
+        }
+    }
+}
+static void composite_masked(void* dest, const void* src, long w, long h, int fmt, const uint8_t* m) {
+    long n = w * h;
+    if (fmt == TWX_FMT_U8) composite_rows((uint8_t*)dest, (const uint8_t*)src, n, m);
+    else if (fmt == TWX_FMT_U16) composite_rows((uint16_t*)dest, (const uint16_t*)src, n, m);
+    else composite_rows_f32((float*)dest, (const float*)src, n, m);
+}
+
+// filters.rs:628-633: -(1 - |0.5 - t| * 2) * 0.5, the inner value narrowed and negated in float
+static float sharpen_weight(float t) {
+    double x = 1.0 - std::fabs(0.5 - (double)t) * 2.0;
+    float neg = -(float)x;
+    return (float)((double)neg * 0.5);
+}
+
+// filters.rs:649-656: mask of the change, blurred copy, unsharp lerp, inverted mask, composite
+static void sharpen_step_with(void* frame, const void* original, long w, long h, int fmt, float t, bool above_first) {
+    size_t bytes = image_bytes(w, h, fmt);
+    std::vector<uint8_t> mask(w * h);
+    difference_mask(frame, original, w, h, fmt, MASK_THRESHOLD, mask.data());
+    clean_mask(mask.data(), w, h);
+    std::vector<uint8_t> sharp((uint8_t*)frame, (uint8_t*)frame + bytes);
+    blur_with(sharp.data(), w, h, fmt, BLUR_RADIUS, above_first);
+    std::vector<uint8_t> base((uint8_t*)frame, (uint8_t*)frame + bytes);
+    lerp(sharp.data(), base.data(), w, h, fmt, sharpen_weight(t));
+    for (auto& v : mask) v = (uint8_t)(255 - v);
+    composite_masked(frame, sharp.data(), w, h, fmt, mask.data());
+}
+
+// ------------------------------------------------------------------------------------------
 // Time model (timeline.rs, curves.rs, fixed.rs, frontend.rs)
 
 static inline int32_t float_to_fixed(double f) {  // fixed.rs:8-10 (Rust `as i32` saturates, NaN 0)
@@ -1235,7 +1531,6 @@ static int render_frame(const twx_retime_params& p, const Clip& c, int32_t n, ui
     if (sel.vectors_quality != 0 && !sel.exact) {
         // vector path (compose/vector.rs:289-314)
         if (!(sel.interp >= 0 && sel.interp <= 2)) { err = "frame_interp outside 1..3"; return TWX_E_MISMATCH; }
-        if (sel.interp == 2) { err = "oracle: Weighted Blend on the vector path is not restated"; return TWX_E_UNSUPPORTED; }
         std::vector<uint8_t> a0 = main_frame(c, rates, p.time_step, sel.a);
         std::vector<uint8_t> b0 = main_frame(c, rates, p.time_step, sel.b);
         std::vector<uint8_t> a(bytes), b(bytes);
@@ -1267,6 +1562,13 @@ static int render_frame(const twx_retime_params& p, const Clip& c, int32_t n, ui
             std::vector<uint8_t> ta(bytes, 0), tb(bytes, 0);
             if ((st = inv_warp(ta.data(), a.data(), w, h, fmt, f_ba.data(), fw, fh, frac, err))) return st;
             if ((st = inv_warp(tb.data(), b.data(), w, h, fmt, f_ab.data(), fw, fh, wa, err))) return st;
+            if (sel.interp == 2) {
+                // Weighted Blend (compose/vector.rs:268-273): the sharpening step of each warped
+                // frame against its unwarped frame, strength position frac for A and wa for B; the
+                // retime front-end's stages take the row above first (frontend.rs:238-241)
+                sharpen_step_with(ta.data(), a.data(), w, h, fmt, frac, true);
+                sharpen_step_with(tb.data(), b.data(), w, h, fmt, wa, true);
+            }
             blend(composed.data(), ta.data(), tb.data(), w, h, fmt, frac);
         }
         rows_reversed(composed.data(), w, h, fmt, out.data());
@@ -1445,8 +1747,10 @@ uint32_t cref_sat_u8(float v) { return sat_u8(v); }
 uint32_t cref_sat_u16(float v) { return sat_u16(v); }
 
 // compose_vectors, warping code 0 (compose/vector.rs:249-276), on engine images in motion rows.
+// `above_first` is the vertical tap order of the Weighted Blend blur (the stage set's
+// vertical_taps).
 int cref_compose_vectors(int interp, float wa, const void* a, const void* b, int w, int h, int fmt, const float* f_ab,
-                         const float* f_ba, int fw, int fh, void* out, char* err, size_t errlen) {
+                         const float* f_ba, int fw, int fh, int above_first, void* out, char* err, size_t errlen) {
     std::string e;
     float frac = 1.0f - wa;
     size_t bytes = image_bytes(w, h, fmt);
@@ -1454,17 +1758,53 @@ int cref_compose_vectors(int interp, float wa, const void* a, const void* b, int
     if (interp == 0) {
         if (0.5f <= wa) st = inv_warp(out, a, w, h, fmt, f_ba, fw, fh, frac, e);
         else st = inv_warp(out, b, w, h, fmt, f_ab, fw, fh, wa, e);
-    } else if (interp == 1) {
+    } else if (interp == 1 || interp == 2) {
         std::vector<uint8_t> ta(bytes), tb(bytes);
         st = inv_warp(ta.data(), a, w, h, fmt, f_ba, fw, fh, frac, e);
         if (!st) st = inv_warp(tb.data(), b, w, h, fmt, f_ab, fw, fh, wa, e);
+        if (!st && interp == 2) {
+            sharpen_step_with(ta.data(), a, w, h, fmt, frac, above_first != 0);
+            sharpen_step_with(tb.data(), b, w, h, fmt, wa, above_first != 0);
+        }
         if (!st) blend(out, ta.data(), tb.data(), w, h, fmt, frac);
     } else {
-        e = "oracle: only Nearest and Blend compositions are restated";
-        st = TWX_E_UNSUPPORTED;
+        e = "interpolation mode outside 0..2";
+        st = TWX_E_MISMATCH;
     }
     if (st) set_err(err, errlen, e);
     return st;
+}
+
+// The filters of the Weighted Blend step, for the tests.
+int cref_blur_taps(float radius, float* out, int cap) {
+    std::vector<float> t = blur_taps(radius);
+    for (int i = 0; i < (int)t.size() && i < cap; i++) out[i] = t[i];
+    return (int)t.size();
+}
+void cref_blur(void* img, int w, int h, int fmt, float radius, int above_first) {
+    blur_with(img, w, h, fmt, radius, above_first != 0);
+}
+float cref_mask_limit(int fmt, float threshold) { return mask_limit(fmt, threshold); }
+void cref_difference_mask(const void* warped, const void* original, int w, int h, int fmt, float threshold, uint8_t* m) {
+    difference_mask(warped, original, w, h, fmt, threshold, m);
+}
+// op: 0 prune(limit), 1 fill(limit, value), 2 dilate rows, 3 replicate border, 4 clean, 5 invert
+void cref_mask_op(uint8_t* m, int w, int h, int op, int limit, int value) {
+    switch (op) {
+        case 0: mask_prune(m, w, h, limit); break;
+        case 1: mask_fill(m, w, h, limit, (uint8_t)value); break;
+        case 2: mask_dilate_rows(m, w, h); break;
+        case 3: mask_replicate_border(m, w, h); break;
+        case 4: clean_mask(m, w, h); break;
+        default: for (long i = 0; i < (long)w * h; i++) m[i] = (uint8_t)(255 - m[i]); break;
+    }
+}
+void cref_composite_masked(void* dest, const void* src, int w, int h, int fmt, const uint8_t* m) {
+    composite_masked(dest, src, w, h, fmt, m);
+}
+float cref_sharpen_weight(float t) { return sharpen_weight(t); }
+void cref_sharpen_step(void* frame, const void* original, int w, int h, int fmt, float t, int above_first) {
+    sharpen_step_with(frame, original, w, h, fmt, t, above_first != 0);
 }
 
 int cref_select(const twx_retime_params* p, int n, twx_selection* out, char* err, size_t errlen) {

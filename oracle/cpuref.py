@@ -87,6 +87,8 @@ def lib():
         _lib.cref_bilinear1.restype = ctypes.c_float
         _lib.cref_sat_u8.restype = ctypes.c_uint32
         _lib.cref_sat_u16.restype = ctypes.c_uint32
+        _lib.cref_mask_limit.restype = ctypes.c_float
+        _lib.cref_sharpen_weight.restype = ctypes.c_float
     return _lib
 
 
@@ -283,17 +285,67 @@ def sat_u16(v: float) -> int:
     return lib().cref_sat_u16(ctypes.c_float(v))
 
 
-def compose_vectors(interp, wa, a, b, w, h, fmt, f_ab, f_ba, fw, fh):
+def compose_vectors(interp, wa, a, b, w, h, fmt, f_ab, f_ba, fw, fh, above_first=True):
     out = np.zeros_like(a)
     err = ctypes.create_string_buffer(256)
     nul = np.zeros(2, np.float32)
     fa = np.ascontiguousarray(f_ab if f_ab is not None else nul, np.float32)
     fb = np.ascontiguousarray(f_ba if f_ba is not None else nul, np.float32)
     st = lib().cref_compose_vectors(interp, ctypes.c_float(wa), _p(np.ascontiguousarray(a)), _p(np.ascontiguousarray(b)),
-                                    w, h, fmt, _p(fa), _p(fb), fw, fh, _p(out), err, 256)
+                                    w, h, fmt, _p(fa), _p(fb), fw, fh, int(above_first), _p(out), err, 256)
     if st:
         raise RuntimeError(err.value.decode())
     return out
+
+
+# ---- the filters of the Weighted Blend step (filters.rs) ------------------------------------
+
+def blur_taps(radius: float) -> np.ndarray:
+    out = np.zeros(64, np.float32)
+    n = lib().cref_blur_taps(ctypes.c_float(radius), _p(out), 64)
+    return out[:n].copy()
+
+
+def blur(img, w, h, fmt, radius=3.0, above_first=False):
+    d = np.ascontiguousarray(img).copy()
+    lib().cref_blur(_p(d), w, h, fmt, ctypes.c_float(radius), int(above_first))
+    return d
+
+
+def mask_limit(fmt, threshold=0.4) -> np.float32:
+    return np.float32(lib().cref_mask_limit(fmt, ctypes.c_float(threshold)))
+
+
+def difference_mask(warped, original, w, h, fmt, threshold=0.4) -> np.ndarray:
+    m = np.zeros(w * h, np.uint8)
+    lib().cref_difference_mask(_p(np.ascontiguousarray(warped)), _p(np.ascontiguousarray(original)), w, h, fmt,
+                               ctypes.c_float(threshold), _p(m))
+    return m
+
+
+MASK_PRUNE, MASK_FILL, MASK_DILATE, MASK_BORDER, MASK_CLEAN, MASK_INVERT = range(6)
+
+
+def mask_op(mask, w, h, op, limit=0, value=0) -> np.ndarray:
+    m = np.ascontiguousarray(mask, np.uint8).copy()
+    lib().cref_mask_op(_p(m), w, h, op, int(limit), int(value))
+    return m
+
+
+def composite_masked(dest, src, w, h, fmt, mask):
+    d = np.ascontiguousarray(dest).copy()
+    lib().cref_composite_masked(_p(d), _p(np.ascontiguousarray(src)), w, h, fmt, _p(np.ascontiguousarray(mask, np.uint8)))
+    return d
+
+
+def sharpen_weight(t: float) -> np.float32:
+    return np.float32(lib().cref_sharpen_weight(ctypes.c_float(t)))
+
+
+def sharpen_step(frame, original, w, h, fmt, t, above_first=False):
+    f = np.ascontiguousarray(frame).copy()
+    lib().cref_sharpen_step(_p(f), _p(np.ascontiguousarray(original)), w, h, fmt, ctypes.c_float(t), int(above_first))
+    return f
 
 
 def select(params: RetimeParams, n: int) -> Selection:

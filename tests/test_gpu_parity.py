@@ -79,7 +79,7 @@ def test_solve_sensitivity_bit_exact(backend, sens):
 
 
 @pytest.mark.parametrize("fmt", [0, 1, 2])
-@pytest.mark.parametrize("interp", [0, 1])
+@pytest.mark.parametrize("interp", [0, 1, 2])
 def test_compose_bit_exact(backend, fmt, interp):
     rng = np.random.RandomState(11 + fmt)
     w, h = 67, 45
@@ -87,16 +87,41 @@ def test_compose_bit_exact(backend, fmt, interp):
     fw, fh = 33, 22
     f_ab = (rng.uniform(-6, 6, fw * fh * 2)).astype(np.float32)
     f_ba = (rng.uniform(-6, 6, fw * fh * 2)).astype(np.float32)
-    for wa in (0.25, 0.5, 0.6666667, 0.997, 0.0031):
+    for wa in (0.25, 0.5, 0.6666667, 0.997, 0.0031, 1.25, -0.5):
         wa = float(np.float32(wa))
         want = cpuref.compose_vectors(interp, wa, a, b, w, h, fmt, f_ab, f_ba, fw, fh)
         got = backend.compose_vectors(interp, wa, a, b, w, h, fmt, f_ab, f_ba, fw, fh)
         np.testing.assert_array_equal(got.view(DT[fmt]), want.view(DT[fmt]), err_msg=f"wa {wa}")
 
 
+@pytest.mark.parametrize("w,h", [(2, 2), (2, 9), (9, 2), (3, 3), (4, 5), (5, 4), (8, 8), (9, 17), (64, 16),
+                                 (65, 17), (130, 35), (200, 120)])
+@pytest.mark.parametrize("fmt", [0, 1, 2])
+def test_weighted_blend_sizes_and_tap_orders(backend, w, h, fmt):
+    """Weighted Blend across tile borders, masks under 3 pixels on a side, and both vertical tap
+    orders of the blur, against the oracle."""
+    rng = np.random.RandomState(w * 1000 + h * 10 + fmt)
+    a, b = shifted_pair(rng, w, h, fmt, 5.0, -3.0)
+    fw, fh = max(2, w // 2), max(2, h // 2)
+    f_ab = rng.uniform(-9, 9, fw * fh * 2).astype(np.float32)
+    f_ba = rng.uniform(-9, 9, fw * fh * 2).astype(np.float32)
+    for wa in (0.3, 0.5, 0.8):
+        wa = float(np.float32(wa))
+        for above in (True, False):
+            want = cpuref.compose_vectors(2, wa, a, b, w, h, fmt, f_ab, f_ba, fw, fh, above)
+            got = backend.compose_vectors(2, wa, a, b, w, h, fmt, f_ab, f_ba, fw, fh, above)
+            np.testing.assert_array_equal(got.view(DT[fmt]), want.view(DT[fmt]),
+                                          err_msg=f"{w}x{h} fmt {fmt} wa {wa} above_first {above}")
+
+
+# the fixture's scope labels of the cases the frame-synthesis path serves (warp code 0: Nearest,
+# Blend and Weighted Blend)
+SERVED = ("core", "weighted_blend")
+
+
 def _cases():
     with open(os.path.join(HERE, "golden", "retime_flow_cases.json")) as fh:
-        return [c for c in json.load(fh) if c["scope"] == "core"]
+        return [c for c in json.load(fh) if c["scope"] in SERVED]
 
 
 def test_retime_reference_hashes(backend):

@@ -481,6 +481,39 @@ void blend_branch(float w, FrameJob& F) {
     else F.blend_mode = 3;
 }
 
+// The limit of the difference mask (filters.rs:287-293): ((0.4 * white) * 0.4) * 0.4 * 3.
+float mask_limit(int fmt) {
+    const float t = 0.4f;
+    float white = fmt == TWX_FMT_U8 ? 255.0f : (fmt == TWX_FMT_U16 ? 65535.0f : 1.0f);
+    return t * white * t * t * 3.0f;
+}
+
+// The strength of the unsharp step at position t (filters.rs:628-633): -min(t, 1 - t), with the
+// inner value narrowed to float and negated there.
+float sharpen_weight(float t) {
+    double x = 1.0 - std::fabs(0.5 - (double)t) * 2.0;
+    float neg = -(float)x;
+    return (float)((double)neg * 0.5);
+}
+
+// A Weighted Blend frame of the vector path (kind 3): buffers of the warped frames and of the two
+// change masks, the limit, the sharpening weights (frac for A, wa for B) and the blend branch.
+void setup_weighted(DevPool& pool, FrameJob& F, bool above_first) {
+    size_t fb = image_bytes(F.W, F.H, F.fmt);
+    size_t np = (size_t)F.W * F.H;
+    F.kind = 3;
+    F.ta = pool.alloc<uint8_t>(fb);
+    F.tb = pool.alloc<uint8_t>(fb);
+    for (int hh = 0; hh < 2; hh++)
+        for (int b = 0; b < 2; b++) F.mask[hh][b] = pool.alloc<uint8_t>(np);
+    F.limit = mask_limit(F.fmt);
+    F.ca = sharpen_weight(F.frac);
+    F.cb = sharpen_weight(F.wa);
+    F.above_first = above_first ? 1 : 0;
+    F.mask_small = (F.W < 3 || F.H < 3) ? 1 : 0;
+    blend_branch(F.frac, F);
+}
+
 // The |amount| early-out of inv_warp (warp.rs:749-764): true means a plain copy.
 bool warp_is_copy(float amount, int fw, int W) {
     float a = amount < 0.0f ? -amount : amount;
@@ -527,10 +560,6 @@ int retime_frames_dev(twx_handle* h, DevPool& pool, const twx_retime_params& p, 
         Plan& pl = plan[i];
         if ((st = timeline::select(p, map, first + i, pl.sel, err))) return st;
         pl.vector = (pl.sel.vectors_quality != 0 && !pl.sel.exact) ? 1 : 0;
-        if (pl.vector && pl.sel.interp == 2) {
-            err = "Weighted Blend on the vector path (the sharpening step) is not served by this backend";
-            return TWX_E_UNSUPPORTED;
-        }
         if (pl.sel.interp < 0 || pl.sel.interp > 2) { err = "frame_interp outside 1..3"; return TWX_E_MISMATCH; }
         pl.ia = timeline::clip_index(rates, pl.sel.a, n_in);
         pl.ib = timeline::clip_index(rates, pl.sel.b, n_in);
@@ -675,9 +704,13 @@ int retime_frames_dev(twx_handle* h, DevPool& pool, const twx_retime_params& p, 
             if (pl.sel.interp == 0) {
                 F.kind = 1;
                 F.nearest_from_a = 0.5f <= pl.sel.wa ? 1 : 0;
-            } else {
+            } else if (pl.sel.interp == 1) {
                 F.kind = 2;
                 blend_branch(F.frac, F);
+            } else {
+                // Weighted Blend: the retime front-end's stages take the row above first in the
+                // blur of the sharpening step (frontend.rs:238-241)
+                setup_weighted(pool, F, true);
             }
         }
     }
@@ -687,6 +720,7 @@ int retime_frames_dev(twx_handle* h, DevPool& pool, const twx_retime_params& p, 
     }
     FrameJob* dj = pool.upload(jobs);
     launch_frames(s, dj, jobs);
+    launch_weighted(s, dj, jobs);
     int hbad = 0;
     HIPCHECK(hipMemcpyAsync(&hbad, bad, sizeof(int), hipMemcpyDeviceToHost, s));
     HIPCHECK(hipStreamSynchronize(s));
@@ -860,15 +894,15 @@ int twx_solve_pair(twx_handle* h, const void* a, const void* b, int32_t width, i
 
 int twx_compose_vectors(twx_handle* h, int32_t interp, float wa, const void* a, const void* b, int32_t width,
                         int32_t height, int32_t fmt, const float* f_ab, const float* f_ba, int32_t flow_w,
-                        int32_t flow_h, void* out, char* err, size_t errlen) {
+                        int32_t flow_h, int32_t taps_above_first, void* out, char* err, size_t errlen) {
     std::string e;
     int st = TWX_OK;
     try {
         Guard g(h, st, e);
         if (!g.ok()) { set_err(err, errlen, e); return st; }
-        if (interp != 0 && interp != 1) {
-            set_err(err, errlen, "only Nearest and Blend (warping code 0) are served by this backend");
-            return TWX_E_UNSUPPORTED;
+        if (interp < 0 || interp > 2) {
+            set_err(err, errlen, "interpolation mode outside 0 (Nearest), 1 (Blend), 2 (Weighted Blend)");
+            return TWX_E_MISMATCH;
         }
         if (width < 2 || height < 2 || flow_w < 2 || flow_h < 2 || fmt < 0 || fmt > 2) {
             set_err(err, errlen, "inverse warp needs images and a flow of at least 2x2");
@@ -898,8 +932,8 @@ int twx_compose_vectors(twx_handle* h, int32_t interp, float wa, const void* a, 
         F.finish = 0;
         F.warp_a_copy = warp_is_copy(F.frac, flow_w, width) ? 1 : 0;
         F.warp_b_copy = warp_is_copy(wa, flow_w, width) ? 1 : 0;
-        bool need_ab = interp == 1 || !(0.5f <= wa);
-        bool need_ba = interp == 1 || (0.5f <= wa);
+        bool need_ab = interp != 0 || !(0.5f <= wa);
+        bool need_ba = interp != 0 || (0.5f <= wa);
         if (need_ab) {
             if (!f_ab) { set_err(err, errlen, "the composition needs the flow F_AB"); return TWX_E_MISMATCH; }
             HIPCHECK(hipMemcpyAsync(fl, f_ab, fn * 4, hipMemcpyHostToDevice, s));
@@ -913,9 +947,11 @@ int twx_compose_vectors(twx_handle* h, int32_t interp, float wa, const void* a, 
         if (interp == 0) {
             F.kind = 1;
             F.nearest_from_a = 0.5f <= wa ? 1 : 0;
-        } else {
+        } else if (interp == 1) {
             F.kind = 2;
             blend_branch(F.frac, F);
+        } else {
+            setup_weighted(pool, F, taps_above_first != 0);
         }
         int* bad = pool.alloc<int>(1);
         HIPCHECK(hipMemsetAsync(bad, 0, sizeof(int), s));
@@ -923,6 +959,7 @@ int twx_compose_vectors(twx_handle* h, int32_t interp, float wa, const void* a, 
         std::vector<FrameJob> jobs{F};
         FrameJob* dj = pool.upload(jobs);
         launch_frames(s, dj, jobs);
+        launch_weighted(s, dj, jobs);
         int hbad = 0;
         HIPCHECK(hipMemcpyAsync(out, img + 2 * fb, fb, hipMemcpyDeviceToHost, s));
         HIPCHECK(hipMemcpyAsync(&hbad, bad, sizeof(int), hipMemcpyDeviceToHost, s));

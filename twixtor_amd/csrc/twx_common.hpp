@@ -70,13 +70,24 @@ struct FrameJob {
     int plus_zero;         // weighted blend without flows on float images: inputs + 0.0 first
     int finish;            // 1: unpremultiply into the wire frame, rows turned back; 0: engine pixel as is
     int pad;
+    // Weighted Blend on the vector path (kind 3, compose/vector.rs:268-273, filters.rs:649-656):
+    // the warped frames, two buffers per change mask (ping-pong of the clean-up passes), the
+    // difference-mask limit, the sharpening weights of A and B and the blur's vertical tap order.
+    void* ta;
+    void* tb;
+    uint8_t* mask[2][2];   // [half A/B][buffer]
+    float limit;
+    float ca, cb;
+    int above_first;
+    int mask_small;        // a side under 3 pixels: the clean-up runs as one sequential pass
 };
 
 // Per-kernel-family timing with HIP events on the launching stream, and the algorithmic bytes of
 // each launch (the step-boundary model: distinct inputs read once, outputs written once).
 enum ProfFam {
     F_INGEST, F_LUMA, F_PYR_HPASS, F_PYR_VPASS, F_PYR_RESAMPLE, F_PYR_SUBTRACT, F_GRAD_TENSOR, F_BM_TRANSPOSE,
-    F_SOLVE_INIT, F_CALC_B, F_NEW_DISP, F_BM_ROWS, F_BM_COLS, F_RANGE, F_UPSAMPLE, F_NEGATE, F_FRAMES, F_NFAM
+    F_SOLVE_INIT, F_CALC_B, F_NEW_DISP, F_BM_ROWS, F_BM_COLS, F_RANGE, F_UPSAMPLE, F_NEGATE, F_FRAMES,
+    F_WB_MASK, F_WB_SHARPEN, F_NFAM
 };
 extern const char* const PROF_NAMES[F_NFAM];
 
@@ -118,6 +129,8 @@ void launch_resample(hipStream_t s, const PlaneOp* d_ops, const std::vector<Plan
 void launch_subtract(hipStream_t s, const PlaneOp* d_ops, const std::vector<PlaneOp>& ops, int maxw, int maxh);
 void launch_solve(hipStream_t s, const SolveJob* d_jobs, const std::vector<SolveJob>& jobs, bool skip_refine);
 void launch_frames(hipStream_t s, const FrameJob* d_jobs, const std::vector<FrameJob>& jobs);
+// The Weighted Blend jobs (kind 3) after launch_frames: mask clean-up, sharpening, blend, finish.
+void launch_weighted(hipStream_t s, const FrameJob* d_jobs, const std::vector<FrameJob>& jobs);
 void launch_nan_scan(hipStream_t s, const float* data, size_t n, int* flag);
 int run_selfcheck(hipStream_t s, std::string& err);
 

@@ -96,7 +96,8 @@ void launch_ingest(hipStream_t s, const void* wire, void* engine, int W, int H, 
 
 const char* const PROF_NAMES[F_NFAM] = {"ingest", "luma", "pyr_hpass", "pyr_vpass", "pyr_resample", "pyr_subtract",
                                         "tensor_gauss", "bm_transpose", "solve_init", "calc_b_gauss", "new_disp_gauss",
-                                        "bm_rows", "bm_cols", "range", "upsample", "negate", "frames"};
+                                        "bm_rows", "bm_cols", "range", "upsample", "negate", "frames",
+                                        "wb_mask", "wb_sharpen_blend"};
 thread_local Prof* g_prof = nullptr;
 
 hipEvent_t Prof::take() {
@@ -1124,6 +1125,23 @@ __global__ void k_frames(const FrameJob* __restrict__ jobs) {
         else ok = warp_pixel<FMT>(F.a, F.W, F.H, F.f_ba, F.fw, F.fh, F.frac, x, y, pa);
         if (F.warp_b_copy) load4<FMT>(F.b, mi, pb);
         else ok = warp_pixel<FMT>(F.b, F.W, F.H, F.f_ab, F.fw, F.fh, F.wa, x, y, pb) && ok;
+        if (F.kind == 3) {
+            // Weighted Blend, first stage: keep both warped frames and their difference masks
+            // against the unwarped frames (filters.rs:295-335: (|dR| + |dG|) + |dB| >= limit)
+            using T = typename Px<FMT>::T;
+            T* oa = reinterpret_cast<T*>(F.ta) + mi * 4;
+            T* ob = reinterpret_cast<T*>(F.tb) + mi * 4;
+            float a0[4], b0[4];
+            load4<FMT>(F.a, mi, a0);
+            load4<FMT>(F.b, mi, b0);
+            for (int k = 0; k < 4; k++) { oa[k] = (T)pa[k]; ob[k] = (T)pb[k]; }
+            float da = (fabsf(pa[1] - a0[1]) + fabsf(pa[2] - a0[2])) + fabsf(pa[3] - a0[3]);
+            float db = (fabsf(pb[1] - b0[1]) + fabsf(pb[2] - b0[2])) + fabsf(pb[3] - b0[3]);
+            F.mask[0][0][mi] = da >= F.limit ? 255 : 0;
+            F.mask[1][0][mi] = db >= F.limit ? 255 : 0;
+            if (!ok) atomicOr(F.bad, 1);
+            return;
+        }
         blend_pixel<FMT>(pa, pb, F.blend_mode, F.weight, F.lerp_w, e);
     }
     if (!ok) atomicOr(F.bad, 1);
@@ -1149,11 +1167,297 @@ void launch_frames(hipStream_t s, const FrameJob* d_jobs, const std::vector<Fram
     for (auto& f : jobs) {
         double px = (double)f.W * f.H, bpp = 4.0 * sample_size(f.fmt);
         double flows = f.kind == 0 ? 0.0 : (f.kind == 1 ? 8.0 : 16.0) * f.fw * f.fh;
-        bytes += (f.kind == 1 ? 2.0 : 3.0) * px * bpp + flows;
+        // kind 3 reads A and B and writes both warped frames and both difference masks
+        bytes += (f.kind == 1 ? 2.0 : (f.kind == 3 ? 4.0 : 3.0)) * px * bpp + (f.kind == 3 ? 2.0 * px : 0.0) + flows;
     }
     if (has[0]) TWX_LAUNCH(s, F_FRAMES, bytes, k_frames<0><<<grd, blk, 0, s>>>(d_jobs));
     if (has[1]) TWX_LAUNCH(s, F_FRAMES, bytes, k_frames<1><<<grd, blk, 0, s>>>(d_jobs));
     if (has[2]) TWX_LAUNCH(s, F_FRAMES, bytes, k_frames<2><<<grd, blk, 0, s>>>(d_jobs));
+}
+
+// -------------------------------------------------------------------------------------------
+// Weighted Blend on the vector path, after the first stage of k_frames (kind 3): the clean-up of
+// both change masks (filters.rs:337-508), then one tiled kernel that runs the sharpening step of
+// both warped frames (blur r = 3, unsharp lerp, masked composite; filters.rs:240-283, 532-656),
+// the blend and the finish (compose/vector.rs:268-274).
+
+enum MaskOp { M_PRUNE, M_DILATE, M_FILL };
+
+// One clean-up pass over a snapshot: buffer `src` in, the other buffer out. With `clamp` the
+// snapshot is read through the border replication that the clean-up runs after each fill: on a
+// mask of at least 3x3 pixels mask_replicate_border (filters.rs:423-484) makes
+// m(x, y) = m(clamp(x, 1, w - 2), clamp(y, 1, h - 2)).
+template <int OP>
+__global__ void k_mask_pass(const FrameJob* __restrict__ jobs, int src, int limit, int value, int clamp) {
+    const FrameJob& F = jobs[blockIdx.z >> 1];
+    if (F.kind != 3 || F.mask_small) return;
+    const int half = blockIdx.z & 1;
+    const int W = F.W, H = F.H;
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = blockIdx.y;
+    if (x >= W || y >= H) return;
+    const uint8_t* __restrict__ t = F.mask[half][src];
+    uint8_t* __restrict__ d = F.mask[half][src ^ 1];
+    auto at = [&](int xx, int yy) -> int {
+        if (clamp) { xx = min(max(xx, 1), W - 2); yy = min(max(yy, 1), H - 2); }
+        return (int)t[(long)yy * W + xx];
+    };
+    int v = at(x, y);
+    if (OP == M_PRUNE) {
+        // mask_prune (filters.rs:352-375): the rows y - 2, y and y + 2 around a set pixel
+        if (H >= 5 && x >= 1 && x <= W - 2 && y >= 2 && y <= H - 3 && v != 0) {
+            int sum = at(x, y - 2) + at(x - 1, y - 2) + at(x + 1, y - 2) + at(x - 1, y) + at(x + 1, y) +
+                      at(x - 1, y + 2) + at(x, y + 2) + at(x + 1, y + 2);
+            if (sum < limit) v = 0;
+        }
+    } else if (OP == M_DILATE) {
+        // mask_dilate_rows (filters.rs:403-417)
+        if (y >= 1 && y <= H - 2 && (at(x, y - 1) == 255 || at(x, y + 1) == 255)) v = 255;
+    } else {
+        // mask_fill (filters.rs:384-399): the full ring around an empty pixel
+        if (x >= 1 && x <= W - 2 && y >= 1 && y <= H - 2 && v == 0) {
+            int sum = at(x, y - 1) + at(x - 1, y - 1) + at(x + 1, y - 1) + at(x - 1, y) + at(x + 1, y) +
+                      at(x - 1, y + 1) + at(x, y + 1) + at(x + 1, y + 1);
+            if (limit < sum) v = value;
+        }
+    }
+    d[(long)y * W + x] = (uint8_t)v;
+}
+
+// mask_replicate_border (filters.rs:423-484) as written, rectangle tests in float, for masks with
+// a side under 3 pixels where it is not the clamp of the inner ring.
+__device__ void mask_border_seq(uint8_t* m, long wl, long hl) {
+    int64_t w = wl, h = hl;
+    float p2 = 1.0f, p3 = 1.0f, p4 = (float)(w - 2), p5 = (float)(h - 2);
+    int64_t iv14 = (int64_t)(int32_t)p5;
+    float fy0 = p3, fy1 = p5;
+    if (p5 < p3) { fy0 = p5; fy1 = p3; }
+    float fx0 = p2, fx1 = p4;
+    if (p4 < p2) { fx0 = p4; fx1 = p2; }
+    auto ok = [&](int64_t x, int64_t y) { return x >= 0 && x < w && y >= 0 && y < h; };
+    auto outside = [&](int64_t x, int64_t y) {
+        float xf = (float)x, yf = (float)y;
+        return yf < fy0 || fy1 < yf || fx1 < xf || xf < fx0;
+    };
+    int64_t sy = (int64_t)(int32_t)p3;
+    for (int64_t y = 0; (float)y < fy0; y++)
+        for (int64_t x = 0; x < w; x++)
+            if (outside(x, y) && ok(x, y) && ok(x, sy)) m[y * w + x] = m[sy * w + x];
+    for (int64_t y = (int64_t)(int32_t)fy1; y < h; y++)
+        for (int64_t x = 0; x < w; x++)
+            if (outside(x, y) && ok(x, y) && ok(x, iv14)) m[y * w + x] = m[iv14 * w + x];
+    int64_t sx = (int64_t)(int32_t)p2, first = (int64_t)(int32_t)fx1, base = (int64_t)(int32_t)p4;
+    for (int64_t y = 0; y < h; y++) {
+        for (int64_t x = 0; (float)x < fx0; x++)
+            if (ok(x, y) && ok(sx, y)) m[y * w + x] = m[y * w + sx];
+        for (int64_t j = 0; first + j < w; j++)
+            if (ok(base + j, y) && ok(base, y)) m[y * w + base + j] = m[y * w + base];
+    }
+}
+
+// The whole clean-up of a mask with a side under 3 pixels, one thread per mask: prune and fill
+// need 3 columns and 3 rows and do nothing; the row growth (3 rows) and the four border
+// replications run as written. The result goes to buffer 1.
+__global__ void k_mask_small(const FrameJob* __restrict__ jobs) {
+    const FrameJob& F = jobs[blockIdx.x >> 1];
+    if (F.kind != 3 || !F.mask_small || threadIdx.x != 0) return;
+    const int half = blockIdx.x & 1;
+    const long W = F.W, H = F.H, n = W * H;
+    uint8_t* m = F.mask[half][0];
+    uint8_t* t = F.mask[half][1];
+    if (H >= 3) {
+        for (long i = 0; i < n; i++) t[i] = m[i];
+        for (long y = 1; y < H - 1; y++)
+            for (long x = 0; x < W; x++)
+                if (t[(y - 1) * W + x] == 255 || t[(y + 1) * W + x] == 255) m[y * W + x] = 255;
+    }
+    for (int k = 0; k < 4; k++) mask_border_seq(m, W, H);
+    for (long i = 0; i < n; i++) t[i] = m[i];
+}
+
+constexpr int WB_TW = 64, WB_TH = 16, WB_R = 3;  // tile, and the reach of the r = 3 blur (4 taps)
+constexpr int WB_RW = WB_TW + 2 * WB_R, WB_RH = WB_TH + 2 * WB_R;
+
+struct BlurTaps {
+    float k[4];
+};
+
+// The store of the blur passes (filters.rs:112-119): trunc(v + 0.5), clamped, trunc again.
+template <int FMT>
+__device__ __forceinline__ float wb_store(float v) {
+    if (FMT == 2) return v;
+    const float mx = FMT == 0 ? 255.0f : 65535.0f;
+    float f = (float)cvtt(v + 0.5f);
+    f = f > 0.0f ? f : 0.0f;
+    f = f < mx ? f : mx;
+    return (float)cvtt(f + 0.5f);
+}
+
+// One output pixel of the 1D blur (filters.rs:190-236) on four lanes: `p` is the pixel in LDS,
+// `stride` the float4 distance to the next pixel of the line, i its index in a line of n. Taps
+// outside the image hold the sentinel -1.0; in the edge formula a lane equal to it carries no
+// weight, as in the reference.
+__device__ __forceinline__ float4 wb_conv(const float4* p, int stride, int i, int n, const BlurTaps& bt, bool left_first) {
+    const float S = -1.0f;
+    const int left = min(n, 4);
+    const int mid = n > 8 ? n - 8 : 0;
+    float4 c = p[0];
+    float acc[4] = {c.x * bt.k[0], c.y * bt.k[0], c.z * bt.k[0], c.w * bt.k[0]};
+    if (i >= left && i < left + mid) {
+#pragma unroll
+        for (int j = 3; j >= 1; j--) {
+            float4 r = p[j * stride], l = p[-j * stride];
+            acc[0] = acc[0] + (r.x + l.x) * bt.k[j];
+            acc[1] = acc[1] + (r.y + l.y) * bt.k[j];
+            acc[2] = acc[2] + (r.z + l.z) * bt.k[j];
+            acc[3] = acc[3] + (r.w + l.w) * bt.k[j];
+        }
+        return make_float4(acc[0], acc[1], acc[2], acc[3]);
+    }
+    float wsum[4] = {bt.k[0], bt.k[0], bt.k[0], bt.k[0]};
+#pragma unroll
+    for (int j = 3; j >= 1; j--) {
+        float4 r4 = i + j < n ? p[j * stride] : make_float4(S, S, S, S);
+        float4 l4 = i >= j ? p[-j * stride] : make_float4(S, S, S, S);
+        if (left_first) { float4 t = r4; r4 = l4; l4 = t; }
+        const float r[4] = {r4.x, r4.y, r4.z, r4.w}, l[4] = {l4.x, l4.y, l4.z, l4.w};
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            float wr = (r[k] != S ? 1.0f : 0.0f) * bt.k[j];
+            wsum[k] = wsum[k] + wr;
+            float rt = wr * r[k];
+            float wl = (l[k] != S ? 1.0f : 0.0f) * bt.k[j];
+            float lt = l[k] * wl;
+            wsum[k] = wsum[k] + wl;
+            acc[k] = (acc[k] + rt) + lt;
+        }
+    }
+    return make_float4(acc[0] / wsum[0], acc[1] / wsum[1], acc[2] / wsum[2], acc[3] / wsum[3]);
+}
+
+// A 64x16 output tile: for each warped frame, its region with a 3 pixel apron into LDS, the row
+// pass over the region rows, the column pass, the unsharp lerp against the frame and the masked
+// composite (inverted mask as the weight of the frame); then blend and finish.
+template <int FMT>
+__global__ __launch_bounds__(256) void k_weighted(const FrameJob* __restrict__ jobs, BlurTaps bt) {
+    const FrameJob& F = jobs[blockIdx.z];
+    if (F.fmt != FMT || F.kind != 3) return;
+    const int W = F.W, H = F.H;
+    const int x0 = blockIdx.x * WB_TW, y0 = blockIdx.y * WB_TH;
+    if (x0 >= W || y0 >= H) return;
+    __shared__ float4 reg[WB_RH][WB_RW];
+    __shared__ float4 hb[WB_RH][WB_TW];
+    const int tid = threadIdx.x, tx = tid & 63, ty = tid >> 6;
+    const int64_t imask = FMT == 0 ? 0xff : 0xffff;
+    float res[2][4][4];
+#pragma unroll
+    for (int half = 0; half < 2; half++) {
+        const void* img = half ? F.tb : F.ta;
+        const float c = half ? F.cb : F.ca;
+        const uint8_t* __restrict__ cm = F.mask[half][1];
+        __syncthreads();
+        for (int i = tid; i < WB_RH * WB_RW; i += 256) {
+            int r = i / WB_RW, cc = i - r * WB_RW;
+            int gy = y0 - WB_R + r, gx = x0 - WB_R + cc;
+            float v[4] = {-1.0f, -1.0f, -1.0f, -1.0f};
+            if (gx >= 0 && gx < W && gy >= 0 && gy < H) load4<FMT>(img, (long)gy * W + gx, v);
+            reg[r][cc] = make_float4(v[0], v[1], v[2], v[3]);
+        }
+        __syncthreads();
+        // lerp with weight 0 copies the frame (the sharpened copy is not needed)
+        if (c != 0.0f) {
+            for (int i = tid; i < WB_RH * WB_TW; i += 256) {
+                int r = i >> 6, j = i & 63;
+                int gy = y0 - WB_R + r, gx = x0 + j;
+                float4 o = make_float4(-1.0f, -1.0f, -1.0f, -1.0f);
+                if (gy >= 0 && gy < H && gx < W) {
+                    float4 v = wb_conv(&reg[r][j + WB_R], 1, gx, W, bt, false);
+                    o = make_float4(wb_store<FMT>(v.x), wb_store<FMT>(v.y), wb_store<FMT>(v.z), wb_store<FMT>(v.w));
+                }
+                hb[r][j] = o;
+            }
+            __syncthreads();
+        }
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const int ly = ty + 4 * q, gx = x0 + tx, gy = y0 + ly;
+            if (gx >= W || gy >= H) continue;
+            float4 f4 = reg[ly + WB_R][tx + WB_R];
+            float f[4] = {f4.x, f4.y, f4.z, f4.w};
+            float sh[4] = {f[0], f[1], f[2], f[3]};
+            if (c != 0.0f) {
+                float4 v = wb_conv(&hb[ly + WB_R][tx], WB_TW, gy, H, bt, F.above_first != 0);
+                sh[0] = wb_store<FMT>(v.x);
+                sh[1] = wb_store<FMT>(v.y);
+                sh[2] = wb_store<FMT>(v.z);
+                sh[3] = wb_store<FMT>(v.w);
+                if (c != 1.0f) lerp_pixel<FMT>(sh, f, c);
+            }
+            int mx = gx, my = gy;
+            if (!F.mask_small) { mx = min(max(mx, 1), W - 2); my = min(max(my, 1), H - 2); }
+            const int m = 255 - (int)cm[(long)my * W + mx];
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                if (m == 0) { res[half][q][k] = sh[k]; continue; }
+                float fw = (float)m / 255.0f;
+                float g = (float)(1.0 - (double)fw);
+                if (FMT == 2) res[half][q][k] = (g * sh[k] + fw * f[k]) + 0.0f;
+                else res[half][q][k] = (float)(cvtt64f((sh[k] * g + f[k] * fw) + 0.5f) & imask);
+            }
+        }
+    }
+    const long n = (long)W * H;
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        const int gx = x0 + tx, gy = y0 + ty + 4 * q;
+        if (gx >= W || gy >= H) continue;
+        float e[4];
+        blend_pixel<FMT>(res[0][q], res[1][q], F.blend_mode, F.weight, F.lerp_w, e);
+        long mi = (long)gy * W + gx;
+        if (!F.finish) {
+            typename Px<FMT>::T* o = reinterpret_cast<typename Px<FMT>::T*>(F.out) + mi * 4;
+            for (int k = 0; k < 4; k++) o[k] = (typename Px<FMT>::T)e[k];
+        } else {
+            finish_pixel<FMT>(F.out, n, (long)(H - 1 - gy) * W + gx, e);
+        }
+    }
+}
+
+void launch_weighted(hipStream_t s, const FrameJob* d_jobs, const std::vector<FrameJob>& jobs) {
+    int mw = 0, mh = 0, nw = 0;
+    bool has[3] = {false, false, false}, small = false, big = false;
+    double px = 0, img = 0;
+    for (auto& f : jobs) {
+        if (f.kind != 3) continue;
+        nw++;
+        mw = std::max(mw, f.W);
+        mh = std::max(mh, f.H);
+        has[f.fmt] = true;
+        (f.mask_small ? small : big) = true;
+        px += (double)f.W * f.H;
+        img += (double)f.W * f.H * 4 * sample_size(f.fmt);
+    }
+    if (!nw) return;
+    const unsigned nz = (unsigned)jobs.size();
+    if (big) {
+        dim3 blk(256), grd((mw + 255) / 256, mh, 2 * nz);
+        double b = 4.0 * px;  // both masks read and written once per pass
+        TWX_LAUNCH(s, F_WB_MASK, b, k_mask_pass<M_PRUNE><<<grd, blk, 0, s>>>(d_jobs, 0, 254, 0, 0));
+        TWX_LAUNCH(s, F_WB_MASK, b, k_mask_pass<M_PRUNE><<<grd, blk, 0, s>>>(d_jobs, 1, 509, 0, 0));
+        TWX_LAUNCH(s, F_WB_MASK, b, k_mask_pass<M_DILATE><<<grd, blk, 0, s>>>(d_jobs, 0, 0, 0, 0));
+        TWX_LAUNCH(s, F_WB_MASK, b, k_mask_pass<M_FILL><<<grd, blk, 0, s>>>(d_jobs, 1, 254, 255, 0));
+        TWX_LAUNCH(s, F_WB_MASK, b, k_mask_pass<M_FILL><<<grd, blk, 0, s>>>(d_jobs, 0, 254, 255, 1));
+        TWX_LAUNCH(s, F_WB_MASK, b, k_mask_pass<M_FILL><<<grd, blk, 0, s>>>(d_jobs, 1, 254, 128, 1));
+        TWX_LAUNCH(s, F_WB_MASK, b, k_mask_pass<M_FILL><<<grd, blk, 0, s>>>(d_jobs, 0, 127, 64, 1));
+    }
+    if (small) TWX_LAUNCH(s, F_WB_MASK, 4.0 * px, k_mask_small<<<dim3(2 * nz), dim3(64), 0, s>>>(d_jobs));
+    Taps g = gaussian_taps(3.0f);  // the same formula as the blur's taps (filters.rs:71-95)
+    BlurTaps bt{{g.k[0], g.k[1], g.k[2], g.k[3]}};
+    dim3 blk(256), grd((mw + WB_TW - 1) / WB_TW, (mh + WB_TH - 1) / WB_TH, nz);
+    double b = 3.0 * img + 2.0 * px;  // both warped frames and masks read, the frame written
+    if (has[0]) TWX_LAUNCH(s, F_WB_SHARPEN, b, k_weighted<0><<<grd, blk, 0, s>>>(d_jobs, bt));
+    if (has[1]) TWX_LAUNCH(s, F_WB_SHARPEN, b, k_weighted<1><<<grd, blk, 0, s>>>(d_jobs, bt));
+    if (has[2]) TWX_LAUNCH(s, F_WB_SHARPEN, b, k_weighted<2><<<grd, blk, 0, s>>>(d_jobs, bt));
 }
 
 // -------------------------------------------------------------------------------------------
