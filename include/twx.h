@@ -122,16 +122,19 @@ int twx_solve_pair(twx_handle* h, const void* a, const void* b, int32_t width, i
 
 /* ---- Backend::compose_vectors (compose/vector.rs:209-284) -------------------------------- */
 
-/* Warping code 0 (inverse warps with hold edges, mode 1, no 360) for interp 0 (Nearest),
- * 1 (Blend) and 2 (Weighted Blend: each warped frame through the sharpening step against its
- * unwarped frame, filters.rs:649-656, then the blend); wa is the weight of A (Selection::wa).
- * For Nearest only the flow that nearest_uses_f_ab names is read (the other may be NULL).
- * `taps_above_first` is the vertical tap order of the Weighted Blend blur (the stage set's
- * vertical_taps, stages.rs:60-62): 1 for the retime front-end (frontend.rs:238-241), 0 for the
- * plain CPU stages. Host engine images. */
+/* The composition of one output frame from its two frames and their flows, mode 1, no 360.
+ * `warp_mode` is EngineSettings::warp_mode: 0 the inverse warps with hold edges, 1 the inverse
+ * warps without hold edges and with coverage, the Smart Blend composite and the push-pull hole
+ * fill of the uncovered pixels (smartblend.rs:323-419); the forward warps (2, 3) are refused.
+ * `interp` 0 Nearest, 1 Blend, 2 Weighted Blend (each warped frame through the sharpening step
+ * against its unwarped frame, filters.rs:649-656, before the blend or composite); wa is the
+ * weight of A (Selection::wa). Nearest reads only the flow nearest_uses_f_ab names (the other
+ * may be NULL). `taps_above_first` is the vertical tap order of the Weighted Blend blur and of
+ * the hole fill (the stage set's vertical_taps, stages.rs:60-62): 1 for the retime front-end
+ * (frontend.rs:238-241), 0 for the plain CPU stages. Host engine images. */
 int twx_compose_vectors(twx_handle* h, int32_t interp, float wa, const void* a, const void* b,
                         int32_t width, int32_t height, int32_t fmt, const float* f_ab,
-                        const float* f_ba, int32_t flow_w, int32_t flow_h,
+                        const float* f_ba, int32_t flow_w, int32_t flow_h, int32_t warp_mode,
                         int32_t taps_above_first, void* out, char* err, size_t errlen);
 
 /* ---- the per-frame render loop of `vfx apply retime` ------------------------------------- */

@@ -931,13 +931,29 @@ static inline void flow_bilerp(const float* d, long fw, int32_t fw1, int32_t fh1
     }
 }
 
+// warp.rs:237-295 store of a faded edge sample: the stored sample scaled, trunc((double)(b * f) + 0.5)
+template <class P> static inline P store_faded(float v, float f);
+template <> inline uint8_t store_faded<uint8_t>(float v, float f) {
+    float s = (float)sat_u8(v) * f;
+    return (uint8_t)cvtt64d((double)s + 0.5);
+}
+template <> inline uint16_t store_faded<uint16_t>(float v, float f) {
+    float s = (float)sat_u16(v) * f;
+    return (uint16_t)cvtt64d((double)s + 0.5);
+}
+template <> inline float store_faded<float>(float v, float f) { return v * f; }
+
+// warp.rs:477-580, the row kernel with `hold` (clip, then sample) or without it (transparent
+// beyond one pixel, faded in the border band, with a coverage byte per pixel into `cov`).
 template <class P>
-static bool warp_run(P* out, const P* src, const float* flow, long fw, long fh, float amount, long w, long h) {
+static bool warp_run(P* out, const P* src, const float* flow, long fw, long fh, float amount, long w, long h, bool hold,
+                     uint8_t* cov) {
     int32_t w1i = (int32_t)w - 1, h1i = (int32_t)h - 1;
     float rx = (float)((int32_t)fw - 1) / (float)w1i;
     float ry = (float)((int32_t)fh - 1) / (float)h1i;
     float sx = amount / rx, sy = amount / ry;
     float w1 = (float)w1i, h1 = (float)h1i;
+    float wf = (float)w, hf = (float)h;
     int32_t fw1 = (int32_t)fw - 1, fh1 = (int32_t)fh - 1;
     bool bad = false;
 #pragma omp parallel for schedule(static) reduction(|| : bad)
@@ -951,22 +967,51 @@ static bool warp_run(P* out, const P* src, const float* flow, long fw, long fh, 
             float ix = d[0] * sx + xf;
             float iy = d[1] * sy + yf;
             P* px = out + (y * w + x) * 4;
-            if (!clip_to_rect(xf, yf, ix, iy, w1, h1) || std::isnan(ix) || std::isnan(iy)) {
-                bad = true;
-                for (int k = 0; k < 4; k++) px[k] = (P)0;
+            float s[4];
+            if (hold) {
+                if (!clip_to_rect(xf, yf, ix, iy, w1, h1) || std::isnan(ix) || std::isnan(iy)) {
+                    bad = true;
+                    for (int k = 0; k < 4; k++) px[k] = (P)0;
+                    continue;
+                }
+                bilerp4(src, w, w1i, h1i, ix, iy, s);
+                for (int k = 0; k < 4; k++) px[k] = store<P>(s[k]);
                 continue;
             }
-            float s[4];
-            bilerp4(src, w, w1i, h1i, ix, iy, s);
-            for (int k = 0; k < 4; k++) px[k] = store<P>(s[k]);
+            // pixel_fade (warp.rs:519-561)
+            uint8_t c;
+            if (!(ix > -1.0f && ix < wf && iy > -1.0f && iy < hf)) {
+                for (int k = 0; k < 4; k++) px[k] = (P)0;
+                c = 0;
+            } else if (ix >= 0.0f && ix <= w1 && iy >= 0.0f && iy <= h1) {
+                bilerp4(src, w, w1i, h1i, ix, iy, s);
+                for (int k = 0; k < 4; k++) px[k] = store<P>(s[k]);
+                c = 255;
+            } else {
+                float f = ix < 0.0f ? ix + 1.0f : (ix <= w1 ? 1.0f : wf - ix);
+                if (iy < 0.0f) f *= iy + 1.0f;
+                else if (iy > h1) f *= hf - iy;
+                if (!clip_to_rect(xf, yf, ix, iy, w1, h1) || std::isnan(ix) || std::isnan(iy)) {
+                    bad = true;
+                    for (int k = 0; k < 4; k++) px[k] = (P)0;
+                    c = 0;
+                } else {
+                    bilerp4(src, w, w1i, h1i, ix, iy, s);
+                    for (int k = 0; k < 4; k++) px[k] = store_faded<P>(s[k], f);
+                    float cv = f * 255.0f + 0.5f;
+                    c = cv <= 255.0f ? (uint8_t)cvtt64d((double)cv) : 255;
+                }
+            }
+            if (cov) cov[y * w + x] = c;
         }
     }
     return !bad;
 }
 
-// warp.rs:693-786 (hold = true, mode 1, is360 = false)
+// warp.rs:693-786 (mode 1, is360 = false): `hold` set, or the fading warp whose coverage goes to
+// `cov` (pre-filled with 255 by the caller: the copy early-outs leave it so)
 static int inv_warp(void* out, const void* src, long w, long h, int fmt, const float* flow, long fw, long fh,
-                    float amount, std::string& err) {
+                    float amount, std::string& err, bool hold = true, uint8_t* cov = nullptr) {
     if (w < 2 || h < 2 || fw < 2 || fh < 2) { err = "inverse warp needs images and a flow of at least 2x2"; return TWX_E_MISMATCH; }
     float a = amount < 0.0f ? -amount : amount;
     bool copy_only = 0.005 > (double)a;
@@ -978,9 +1023,9 @@ static int inv_warp(void* out, const void* src, long w, long h, int fmt, const f
     }
     if (copy_only) { memcpy(out, src, image_bytes(w, h, fmt)); return TWX_OK; }
     bool ok;
-    if (fmt == TWX_FMT_U8) ok = warp_run<uint8_t>((uint8_t*)out, (const uint8_t*)src, flow, fw, fh, amount, w, h);
-    else if (fmt == TWX_FMT_U16) ok = warp_run<uint16_t>((uint16_t*)out, (const uint16_t*)src, flow, fw, fh, amount, w, h);
-    else ok = warp_run<float>((float*)out, (const float*)src, flow, fw, fh, amount, w, h);
+    if (fmt == TWX_FMT_U8) ok = warp_run<uint8_t>((uint8_t*)out, (const uint8_t*)src, flow, fw, fh, amount, w, h, hold, cov);
+    else if (fmt == TWX_FMT_U16) ok = warp_run<uint16_t>((uint16_t*)out, (const uint16_t*)src, flow, fw, fh, amount, w, h, hold, cov);
+    else ok = warp_run<float>((float*)out, (const float*)src, flow, fw, fh, amount, w, h, hold, cov);
     if (!ok) { err = "inverse warp: a sample coordinate is NaN or never converges, which is not supported"; return TWX_E_UNSUPPORTED; }
     return TWX_OK;
 }
@@ -1361,6 +1406,257 @@ static void sharpen_step_with(void* frame, const void* original, long w, long h,
 }
 
 // ------------------------------------------------------------------------------------------
+// Smart Blend (smartblend.rs:72-330) and the push-pull hole fill (holefill.rs:145-491).
+// Finite samples only (the NaN rule of nanrule.rs is not restated).
+
+// smartblend.rs:84-150: the stores of the main path and of the "alpha is 0" fallback
+template <class P> struct SmartPx;
+template <> struct SmartPx<uint8_t> {
+    static constexpr float WHITE = 255.0f;
+    static uint8_t plain(float x) {
+        float y = x + 0.5f;
+        return 0.0f > y ? 0 : (y > 255.0f ? 255 : (uint8_t)cvtt64f(y));
+    }
+    static uint8_t raw(float x) { return (uint8_t)cvtt64f(x); }
+};
+template <> struct SmartPx<uint16_t> {
+    static constexpr float WHITE = 65535.0f;
+    static uint16_t plain(float x) {
+        float y = x + 0.5f;
+        return 0.0f > y ? 0 : (y > 65535.0f ? 65535 : (uint16_t)cvtt64f(y));
+    }
+    static uint16_t raw(float x) { return (uint16_t)cvtt64f(x); }
+};
+template <> struct SmartPx<float> {
+    static constexpr float WHITE = 1.0f;
+    static float plain(float x) { return x + 0.0f; }
+    static float raw(float x) { return x; }
+};
+
+// smartblend.rs:167-178: `a` over `b`, premultiplied, alpha in units of white
+static inline void over(const float a[4], const float b[4], float white, float o[4]) {
+    if (a[0] == 0.0f) { for (int k = 0; k < 4; k++) o[k] = b[k]; return; }
+    if (a[0] == white) { for (int k = 0; k < 4; k++) o[k] = a[k]; return; }
+    float ia = 1.0f - a[0] / white;
+    o[0] = (white - b[0]) * (a[0] / white) + b[0];
+    for (int k = 1; k < 4; k++) o[k] = b[k] * ia + a[k];
+}
+
+// smartblend.rs:193-212: returns true for the raw store of the fallback colour
+static inline bool smart_pixel(const float p1[4], const float p2[4], float w, float white, float v[4]) {
+    float c1[4], c2[4];
+    over(p1, p2, white, c1);
+    over(p2, p1, white, c2);
+    float amax = p1[0] > p2[0] ? p1[0] : p2[0];
+    for (int k = 0; k < 4; k++) v[k] = (c2[k] - c1[k]) * w + c1[k];
+    if (v[0] == amax) return false;
+    if (v[0] == 0.0f) {
+        const float* c = c1[0] > c2[0] ? c1 : c2;
+        for (int k = 0; k < 4; k++) v[k] = c[k];
+        return true;
+    }
+    float alpha = v[0];
+    for (int k = 1; k < 4; k++) v[k] = (v[k] * amax) / alpha;
+    v[0] = amax;
+    return false;
+}
+
+template <class P>
+static void smart_rows(P* out, const P* p1, const P* p2, long n, float w) {
+#pragma omp parallel for schedule(static)
+    for (long i = 0; i < n; i++) {
+        float a[4], b[4], v[4];
+        for (int k = 0; k < 4; k++) { a[k] = (float)p1[i * 4 + k]; b[k] = (float)p2[i * 4 + k]; }
+        bool raw = smart_pixel(a, b, w, SmartPx<P>::WHITE, v);
+        for (int k = 0; k < 4; k++) out[i * 4 + k] = raw ? SmartPx<P>::raw(v[k]) : SmartPx<P>::plain(v[k]);
+    }
+}
+
+// smartblend.rs:290-293: out = smart_blend(p1, p2, w)
+static void smart_blend(void* out, const void* p1, const void* p2, long w, long h, int fmt, float wt) {
+    long n = w * h;
+    if (fmt == TWX_FMT_U8) smart_rows((uint8_t*)out, (const uint8_t*)p1, (const uint8_t*)p2, n, wt);
+    else if (fmt == TWX_FMT_U16) smart_rows((uint16_t*)out, (const uint16_t*)p1, (const uint16_t*)p2, n, wt);
+    else smart_rows((float*)out, (const float*)p1, (const float*)p2, n, wt);
+}
+
+// holefill.rs:145-176: level sizes of the fill pyramid; the push starts at `top`
+struct HoleGeometry {
+    std::vector<std::pair<long, long>> dims;
+    int top;
+};
+static HoleGeometry hole_geometry(long w, long h) {
+    int64_t a = w, b = h;
+    int count = 1;
+    for (;;) {
+        a /= 2;
+        b /= 2;
+        if (a < 2) a = 2;
+        if (b < 2) b = 2;
+        count++;
+        if (!(a > 1 && b > 1 && (a != 2 || b != 2))) break;
+    }
+    HoleGeometry g;
+    long cw = w, ch = h;
+    for (int i = 0; i <= count; i++) {
+        g.dims.push_back({cw, ch});
+        cw = std::max(cw / 2, 2L);
+        ch = std::max(ch / 2, 2L);
+    }
+    g.top = count - 1;
+    return g;
+}
+
+static const float GAUSS_FILL_RADIUS = 5.0f;  // holefill.rs:344
+
+// holefill.rs:347-363: sol = value / weight where the weight is not 0
+static void hole_combine(std::vector<float>& sol, const std::vector<float>& v, const std::vector<float>& wt) {
+    for (size_t i = 0; i < sol.size(); i++)
+        if (wt[i] != 0.0f) sol[i] = v[i] / wt[i];
+}
+
+// holefill.rs:214-336: the weight pyramid of the mask, then the pull and push of one plane
+struct HoleFiller {
+    HoleGeometry geo;
+    std::vector<uint8_t> hole;
+    std::vector<std::vector<float>> wts;
+    bool above_first;
+    std::vector<float> k5;
+
+    void blur5(std::vector<float>& d, long w, long h) const { convolve(d.data(), w, h, 1, BINOMIAL5, 3, above_first); }
+    void gauss(std::vector<float>& d, long w, long h) const {
+        convolve(d.data(), w, h, 1, k5.data(), (int)k5.size(), above_first);
+    }
+
+    void fill(float* plane) const {
+        long w = geo.dims[0].first, h = geo.dims[0].second;
+        std::vector<std::vector<float>> vals;
+        std::vector<float> v0(plane, plane + w * h);
+        for (long i = 0; i < w * h; i++)
+            if (hole[i]) v0[i] = 0.0f;
+        vals.push_back(v0);
+        for (int k = 1; k <= geo.top; k++) {
+            auto [pw, ph] = geo.dims[k - 1];
+            auto [cw, ch] = geo.dims[k];
+            std::vector<float> tmp = vals[k - 1];
+            blur5(tmp, pw, ph);
+            vals.push_back(resample1(tmp.data(), pw, ph, cw, ch));
+        }
+        auto [tw, th] = geo.dims[geo.top];
+        std::vector<float> sol(tw * th, 0.0f);
+        for (int level = geo.top; level >= 0; level--) {
+            auto [lw, lh] = geo.dims[level];
+            hole_combine(sol, vals[level], wts[level]);
+            gauss(sol, lw, lh);
+            if (level == 0) {
+                hole_combine(sol, vals[0], wts[0]);
+            } else {
+                auto [nw, nh] = geo.dims[level - 1];
+                sol = resample1(sol.data(), lw, lh, nw, nh);
+            }
+        }
+        memcpy(plane, sol.data(), sizeof(float) * w * h);
+    }
+};
+
+// holefill.rs:90-91 and 128-134: a u8 mask value m is a hole when !(eps > (double)m)
+static const double HOLE_U8_EPS = [] { uint64_t b = 0x20202020202020fcull; double d; memcpy(&d, &b, 8); return d; }();
+
+// holefill.rs:262-296. Returns 0 with `has` false when the mask has no hole (the fill is then the
+// identity), an error for a hole in an image with a side of 1.
+static int hole_filler(long w, long h, const uint8_t* mask, bool above_first, HoleFiller& f, bool& has,
+                       std::string& err) {
+    f.hole.resize(w * h);
+    has = false;
+    for (long i = 0; i < w * h; i++) {
+        f.hole[i] = !(HOLE_U8_EPS > (double)mask[i]);
+        has = has || f.hole[i];
+    }
+    if (!has) return TWX_OK;
+    if (w < 2 || h < 2) { err = "hole fill of an image with a side of 1"; return TWX_E_UNSUPPORTED; }
+    f.above_first = above_first;
+    f.k5 = gaussian_kernel(GAUSS_FILL_RADIUS);
+    f.geo = hole_geometry(w, h);
+    std::vector<float> w0(w * h);
+    for (long i = 0; i < w * h; i++) w0[i] = f.hole[i] ? 0.0f : 1.0f;
+    f.wts.push_back(w0);
+    for (int k = 1; k <= f.geo.top; k++) {
+        auto [pw, ph] = f.geo.dims[k - 1];
+        auto [cw, ch] = f.geo.dims[k];
+        std::vector<float> tmp = f.wts[k - 1];
+        f.blur5(tmp, pw, ph);
+        f.wts.push_back(resample1(tmp.data(), pw, ph, cw, ch));
+    }
+    return TWX_OK;
+}
+
+// holefill.rs:366-386: write back of the integer depths
+template <class T> static inline T hole_store(float x, float white) {
+    float y = x + 0.5f;
+    return 0.0f > y ? (T)0 : (y > white ? (T)white : (T)cvtt64f(y));
+}
+
+// holefill.rs:444-481: every channel of the engine image, alpha first
+static int fill_holes(void* img, long w, long h, int fmt, const uint8_t* holes, bool above_first, std::string& err) {
+    HoleFiller f;
+    bool has;
+    int st = hole_filler(w, h, holes, above_first, f, has, err);
+    if (st || !has) return st;
+    long n = w * h;
+    std::vector<float> plane(n);
+#pragma omp parallel for schedule(static) firstprivate(plane)
+    for (int c = 0; c < 4; c++) {
+        for (long i = 0; i < n; i++) {
+            if (fmt == TWX_FMT_U8) plane[i] = (float)((uint8_t*)img)[i * 4 + c];
+            else if (fmt == TWX_FMT_U16) plane[i] = (float)((uint16_t*)img)[i * 4 + c];
+            else plane[i] = ((float*)img)[i * 4 + c];
+        }
+        f.fill(plane.data());
+        for (long i = 0; i < n; i++) {
+            if (fmt == TWX_FMT_U8) ((uint8_t*)img)[i * 4 + c] = hole_store<uint8_t>(plane[i], 255.0f);
+            else if (fmt == TWX_FMT_U16) ((uint16_t*)img)[i * 4 + c] = hole_store<uint16_t>(plane[i], 65535.0f);
+            else ((float*)img)[i * 4 + c] = plane[i];
+        }
+    }
+    return TWX_OK;
+}
+
+// smartblend.rs:304-316: coverage max, invert, hole fill
+static int smart_finish(void* out, long w, long h, int fmt, std::vector<uint8_t>& cov1, const std::vector<uint8_t>& cov2,
+                        bool above_first, std::string& err) {
+    for (size_t i = 0; i < cov1.size(); i++) cov1[i] = (uint8_t)(255 - std::max(cov1[i], cov2[i]));
+    return fill_holes(out, w, h, fmt, cov1.data(), above_first, err);
+}
+
+// The compositions of warp code 1 (smartblend.rs:323-419, compose/vector.rs:222-247): the fading
+// warps with coverage, the composite (Nearest: the one warp), the Weighted Blend sharpening, the
+// coverage finish and the hole fill. `flow` is the one flow of Nearest; interp 1 and 2 read both.
+static int compose_smart(int interp, float wa, const void* a, const void* b, long w, long h, int fmt, const float* f_ab,
+                         const float* f_ba, long fw, long fh, bool above_first, void* out, std::string& err) {
+    size_t bytes = image_bytes(w, h, fmt);
+    float frac = 1.0f - wa;
+    int st;
+    if (interp == 0) {
+        std::vector<uint8_t> cov(w * h, 255);
+        if (!(0.5f > wa)) st = inv_warp(out, a, w, h, fmt, f_ba, fw, fh, 1.0f - wa, err, false, cov.data());
+        else st = inv_warp(out, b, w, h, fmt, f_ab, fw, fh, wa, err, false, cov.data());
+        if (st) return st;
+        for (auto& c : cov) c = (uint8_t)(255 - c);
+        return fill_holes(out, w, h, fmt, cov.data(), above_first, err);
+    }
+    std::vector<uint8_t> ta(bytes, 0), tb(bytes, 0);
+    std::vector<uint8_t> cov1(w * h, 255), cov2(w * h, 255);
+    if ((st = inv_warp(ta.data(), a, w, h, fmt, f_ba, fw, fh, frac, err, false, cov1.data()))) return st;
+    if ((st = inv_warp(tb.data(), b, w, h, fmt, f_ab, fw, fh, wa, err, false, cov2.data()))) return st;
+    if (interp == 2) {
+        sharpen_step_with(ta.data(), a, w, h, fmt, frac, above_first);
+        sharpen_step_with(tb.data(), b, w, h, fmt, wa, above_first);
+    }
+    smart_blend(out, ta.data(), tb.data(), w, h, fmt, frac);
+    return smart_finish(out, w, h, fmt, cov1, cov2, above_first, err);
+}
+
+// ------------------------------------------------------------------------------------------
 // Time model (timeline.rs, curves.rs, fixed.rs, frontend.rs)
 
 static inline int32_t float_to_fixed(double f) {  // fixed.rs:8-10 (Rust `as i32` saturates, NaN 0)
@@ -1519,7 +1815,6 @@ static std::vector<uint8_t> main_frame(const Clip& c, const Rates& r, int32_t lt
 }
 
 static int render_frame(const twx_retime_params& p, const Clip& c, int32_t n, uint8_t* out_wire, std::string& err) {
-    if (p.smart_blend) { err = "oracle: Smart Blend is not restated"; return TWX_E_UNSUPPORTED; }
     twx_selection sel;
     Rates rates;
     int st = select_frame(p, n, sel, rates, err);
@@ -1547,7 +1842,22 @@ static int render_frame(const twx_retime_params& p, const Clip& c, int32_t n, ui
         float wa = sel.wa;
         float frac = 1.0f - wa;
         std::vector<uint8_t> composed(bytes, 0);
-        if (sel.interp == 0) {
+        if (p.smart_blend) {
+            // warping code 1 (frontend.rs:228: warp_mode = smart_blend), the fills and the
+            // sharpening blur with the row above first (frontend.rs:238-241)
+            std::vector<float> f_ba, f_ab;
+            if (sel.interp == 0) {
+                if (wa < 0.5f) st = solve_pyramids(pa, pb, fs, f_ab, err);  // nearest_uses_f_ab, code 1
+                else st = solve_pyramids(pb, pa, fs, f_ba, err);
+                if (st) return st;
+            } else {
+                if ((st = solve_pyramids(pb, pa, fs, f_ba, err))) return st;
+                if ((st = solve_pyramids(pa, pb, fs, f_ab, err))) return st;
+            }
+            if ((st = compose_smart(sel.interp, wa, a.data(), b.data(), w, h, fmt, f_ab.data(), f_ba.data(), fw, fh, true,
+                                    composed.data(), err)))
+                return st;
+        } else if (sel.interp == 0) {
             bool use_ab = wa < 0.5f;  // nearest_uses_f_ab with warping code 0
             std::vector<float> flow;
             if (use_ab) { if ((st = solve_pyramids(pa, pb, fs, flow, err))) return st; }
@@ -1739,6 +2049,44 @@ int cref_inv_warp(void* out, const void* src, int w, int h, int fmt, const float
     return st;
 }
 
+// The fading warp (hold off) with its coverage image.
+int cref_inv_warp_cov(void* out, const void* src, int w, int h, int fmt, const float* flow, int fw, int fh, float amount,
+                      uint8_t* cov, char* err, size_t errlen) {
+    std::string e;
+    memset(cov, 255, (size_t)w * h);
+    int st = inv_warp(out, src, w, h, fmt, flow, fw, fh, amount, e, false, cov);
+    if (st) set_err(err, errlen, e);
+    return st;
+}
+
+// Smart Blend and the hole fill, for the tests.
+void cref_smart_blend(void* out, const void* p1, const void* p2, int w, int h, int fmt, float wt) {
+    smart_blend(out, p1, p2, w, h, fmt, wt);
+}
+int cref_hole_geometry(int w, int h, int* dims, int cap) {
+    HoleGeometry g = hole_geometry(w, h);
+    for (int i = 0; i < (int)g.dims.size() && 2 * i + 1 < cap; i++) {
+        dims[2 * i] = (int)g.dims[i].first;
+        dims[2 * i + 1] = (int)g.dims[i].second;
+    }
+    return g.top * 1000 + (int)g.dims.size();
+}
+int cref_fill_holes(void* img, int w, int h, int fmt, const uint8_t* holes, int above_first, char* err, size_t errlen) {
+    std::string e;
+    int st = fill_holes(img, w, h, fmt, holes, above_first != 0, e);
+    if (st) set_err(err, errlen, e);
+    return st;
+}
+int cref_fill_plane(float* plane, int w, int h, const uint8_t* holes, int above_first, char* err, size_t errlen) {
+    std::string e;
+    HoleFiller f;
+    bool has;
+    int st = hole_filler(w, h, holes, above_first != 0, f, has, e);
+    if (!st && has) f.fill(plane);
+    if (st) set_err(err, errlen, e);
+    return st;
+}
+
 void cref_blend(void* out, const void* a, const void* b, int w, int h, int fmt, float wt) { blend(out, a, b, w, h, fmt, wt); }
 void cref_lerp(void* first, const void* second, int w, int h, int fmt, float wt) { lerp(first, second, w, h, fmt, wt); }
 void cref_from_frame(const uint8_t* wire, int w, int h, int fmt, void* engine) { from_frame(wire, w, h, fmt, engine); }
@@ -1750,12 +2098,18 @@ uint32_t cref_sat_u16(float v) { return sat_u16(v); }
 // `above_first` is the vertical tap order of the Weighted Blend blur (the stage set's
 // vertical_taps).
 int cref_compose_vectors(int interp, float wa, const void* a, const void* b, int w, int h, int fmt, const float* f_ab,
-                         const float* f_ba, int fw, int fh, int above_first, void* out, char* err, size_t errlen) {
+                         const float* f_ba, int fw, int fh, int warp_mode, int above_first, void* out, char* err,
+                         size_t errlen) {
     std::string e;
     float frac = 1.0f - wa;
     size_t bytes = image_bytes(w, h, fmt);
     int st;
-    if (interp == 0) {
+    if (warp_mode == 1 && interp >= 0 && interp <= 2) {
+        st = compose_smart(interp, wa, a, b, w, h, fmt, f_ab, f_ba, fw, fh, above_first != 0, out, e);
+    } else if (warp_mode != 0) {
+        e = "oracle: only the warping codes 0 and 1 are restated";
+        st = TWX_E_UNSUPPORTED;
+    } else if (interp == 0) {
         if (0.5f <= wa) st = inv_warp(out, a, w, h, fmt, f_ba, fw, fh, frac, e);
         else st = inv_warp(out, b, w, h, fmt, f_ab, fw, fh, wa, e);
     } else if (interp == 1 || interp == 2) {

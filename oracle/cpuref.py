@@ -285,14 +285,15 @@ def sat_u16(v: float) -> int:
     return lib().cref_sat_u16(ctypes.c_float(v))
 
 
-def compose_vectors(interp, wa, a, b, w, h, fmt, f_ab, f_ba, fw, fh, above_first=True):
+def compose_vectors(interp, wa, a, b, w, h, fmt, f_ab, f_ba, fw, fh, above_first=True, warp_mode=0):
     out = np.zeros_like(a)
     err = ctypes.create_string_buffer(256)
     nul = np.zeros(2, np.float32)
     fa = np.ascontiguousarray(f_ab if f_ab is not None else nul, np.float32)
     fb = np.ascontiguousarray(f_ba if f_ba is not None else nul, np.float32)
     st = lib().cref_compose_vectors(interp, ctypes.c_float(wa), _p(np.ascontiguousarray(a)), _p(np.ascontiguousarray(b)),
-                                    w, h, fmt, _p(fa), _p(fb), fw, fh, int(above_first), _p(out), err, 256)
+                                    w, h, fmt, _p(fa), _p(fb), fw, fh, int(warp_mode), int(above_first), _p(out),
+                                    err, 256)
     if st:
         raise RuntimeError(err.value.decode())
     return out
@@ -346,6 +347,50 @@ def sharpen_step(frame, original, w, h, fmt, t, above_first=False):
     f = np.ascontiguousarray(frame).copy()
     lib().cref_sharpen_step(_p(f), _p(np.ascontiguousarray(original)), w, h, fmt, ctypes.c_float(t), int(above_first))
     return f
+
+
+# ---- the fading warp, Smart Blend and the hole fill (warp.rs, smartblend.rs, holefill.rs) ----
+
+def inv_warp_cov(src, w, h, fmt, flow, fw, fh, amount):
+    out = np.zeros_like(src)
+    cov = np.zeros(w * h, np.uint8)
+    err = ctypes.create_string_buffer(256)
+    st = lib().cref_inv_warp_cov(_p(out), _p(np.ascontiguousarray(src)), w, h, fmt,
+                                 _p(np.ascontiguousarray(flow, np.float32)), fw, fh, ctypes.c_float(amount), _p(cov),
+                                 err, 256)
+    return st, out, cov
+
+
+def smart_blend(p1, p2, w, h, fmt, wt):
+    out = np.zeros_like(p1)
+    lib().cref_smart_blend(_p(out), _p(np.ascontiguousarray(p1)), _p(np.ascontiguousarray(p2)), w, h, fmt,
+                           ctypes.c_float(wt))
+    return out
+
+
+def hole_geometry(w, h):
+    dims = np.zeros(128, np.int32)
+    r = lib().cref_hole_geometry(w, h, _p(dims), 128)
+    n, top = r % 1000, r // 1000
+    return [(int(dims[2 * i]), int(dims[2 * i + 1])) for i in range(n)], top
+
+
+def fill_holes(img, w, h, fmt, holes, above_first=False):
+    d = np.ascontiguousarray(img).copy()
+    err = ctypes.create_string_buffer(256)
+    st = lib().cref_fill_holes(_p(d), w, h, fmt, _p(np.ascontiguousarray(holes, np.uint8)), int(above_first), err, 256)
+    if st:
+        raise RuntimeError(err.value.decode())
+    return d
+
+
+def fill_plane(plane, w, h, holes, above_first=False):
+    d = np.ascontiguousarray(plane, np.float32).copy()
+    err = ctypes.create_string_buffer(256)
+    st = lib().cref_fill_plane(_p(d), w, h, _p(np.ascontiguousarray(holes, np.uint8)), int(above_first), err, 256)
+    if st:
+        raise RuntimeError(err.value.decode())
+    return d
 
 
 def select(params: RetimeParams, n: int) -> Selection:

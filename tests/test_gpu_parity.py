@@ -114,9 +114,9 @@ def test_weighted_blend_sizes_and_tap_orders(backend, w, h, fmt):
                                           err_msg=f"{w}x{h} fmt {fmt} wa {wa} above_first {above}")
 
 
-# the fixture's scope labels of the cases the frame-synthesis path serves (warp code 0: Nearest,
-# Blend and Weighted Blend)
-SERVED = ("core", "weighted_blend")
+# the fixture's scope labels of the cases the frame-synthesis path serves (warping codes 0 and 1:
+# Nearest, Blend, Weighted Blend, with and without Smart Blend)
+SERVED = ("core", "weighted_blend", "smart_blend", "smart_blend+weighted_blend")
 
 
 def _cases():
@@ -149,3 +149,32 @@ def test_small_clip_matches_oracle(backend):
     want = cpuref.retime_frames(cfg.struct(), clip, 48, 0, 95)
     got = backend.retime_frames(cfg, clip, 48, 0, 95)
     assert got == want
+
+
+@pytest.mark.parametrize("w,h", [(2, 2), (3, 7), (9, 4), (16, 16), (37, 29), (65, 17), (130, 70)])
+@pytest.mark.parametrize("fmt", [0, 1, 2])
+def test_smart_blend_compositions_bit_exact(backend, w, h, fmt):
+    """Warping code 1 (Smart Blend and the hole fill) for Nearest, Blend and Weighted Blend, both
+    tap orders, flows that push samples out of the frame (holes) and flows that do not."""
+    rng = np.random.RandomState(w * 7919 + h * 31 + fmt)
+    a, b = shifted_pair(rng, w, h, fmt, 4.0, -2.0)
+    if fmt == 2:
+        # translucent and transparent pixels for the composite's alpha branches
+        for img in (a, b):
+            px = img.reshape(-1, 4)
+            px[rng.uniform(0, 1, len(px)) < 0.2, :] = 0.0
+            half = rng.uniform(0, 1, len(px)) < 0.2
+            px[half] *= np.float32(0.5)
+    fw, fh = max(2, w // 2), max(2, h // 2)
+    for spread in (0.3, 12.0):
+        f_ab = rng.uniform(-spread, spread, fw * fh * 2).astype(np.float32)
+        f_ba = rng.uniform(-spread, spread, fw * fh * 2).astype(np.float32)
+        for interp in (0, 1, 2):
+            for wa in (0.3, 0.7):
+                wa = float(np.float32(wa))
+                for above in (True, False):
+                    want = cpuref.compose_vectors(interp, wa, a, b, w, h, fmt, f_ab, f_ba, fw, fh, above, warp_mode=1)
+                    got = backend.compose_vectors(interp, wa, a, b, w, h, fmt, f_ab, f_ba, fw, fh, above, warp_mode=1)
+                    np.testing.assert_array_equal(
+                        got.view(DT[fmt]), want.view(DT[fmt]),
+                        err_msg=f"{w}x{h} fmt {fmt} spread {spread} interp {interp} wa {wa} above_first {above}")

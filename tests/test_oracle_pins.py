@@ -368,9 +368,9 @@ def test_premultiply_u8_hand_value():
 
 # ---- end to end: the original plugin's output hashes (parity/retime-time/results/supp/refs_sha256.txt) ----
 
-# the fixture's scope labels of the cases the frame-synthesis path serves (warp code 0: Nearest,
-# Blend and Weighted Blend)
-SERVED = ("core", "weighted_blend")
+# the fixture's scope labels of the cases the frame-synthesis path serves (warping codes 0 and 1:
+# Nearest, Blend, Weighted Blend, with and without Smart Blend)
+SERVED = ("core", "weighted_blend", "smart_blend", "smart_blend+weighted_blend")
 
 
 def _hash_cases():
@@ -383,7 +383,7 @@ def test_every_in_scope_reference_hash():
     from twixtor_amd.params import Retime
 
     cases = _hash_cases()
-    assert len(cases) == 151
+    assert len(cases) == 192
     bad = []
     for c in cases:
         cfg = Retime(c["width"], c["height"], c["depth"], c["fps"], c["params"], c["quality"])

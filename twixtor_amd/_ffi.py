@@ -78,7 +78,7 @@ def lib():
                                         ctypes.POINTER(I32)]
         L.twx_solve.argtypes = [P, P, P, I32, I32, I32, ctypes.POINTER(FlowSettings), P] + ERR
         L.twx_solve_pair.argtypes = [P, P, P, I32, I32, I32, ctypes.POINTER(FlowSettings), P, P] + ERR
-        L.twx_compose_vectors.argtypes = [P, I32, ctypes.c_float, P, P, I32, I32, I32, P, P, I32, I32, I32, P] + ERR
+        L.twx_compose_vectors.argtypes = [P, I32, ctypes.c_float, P, P, I32, I32, I32, P, P, I32, I32, I32, I32, P] + ERR
         L.twx_select.argtypes = [ctypes.POINTER(RetimeParams), I32, ctypes.POINTER(Selection)] + ERR
         L.twx_retime_frames_dev.argtypes = [P, ctypes.POINTER(RetimeParams), P, I32, I32, I32, P] + ERR
         L.twx_retime_frames.argtypes = [P, ctypes.POINTER(RetimeParams), P, I32, I32, I32, P] + ERR

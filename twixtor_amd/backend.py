@@ -119,9 +119,11 @@ class HipBackend:
         return f_ab, f_ba
 
     # ---- Backend::compose_vectors (warping code 0) ----
-    def compose_vectors(self, interp, wa, a, b, width, height, fmt, f_ab, f_ba, fw, fh, above_first=True) -> np.ndarray:
-        """interp 0 Nearest, 1 Blend, 2 Weighted Blend; `above_first` is the vertical tap order of
-        the Weighted Blend blur (the retime front-end's, frontend.rs:238-241, by default)."""
+    def compose_vectors(self, interp, wa, a, b, width, height, fmt, f_ab, f_ba, fw, fh, above_first=True,
+                        warp_mode=0) -> np.ndarray:
+        """interp 0 Nearest, 1 Blend, 2 Weighted Blend; warp_mode 0 inverse warps, 1 with Smart
+        Blend and hole fill; `above_first` is the vertical tap order of the Weighted Blend blur and
+        of the hole fill (the retime front-end's, frontend.rs:238-241, by default)."""
         out = np.zeros_like(a)
         a, b = np.ascontiguousarray(a), np.ascontiguousarray(b)
         fab = None if f_ab is None else np.ascontiguousarray(f_ab, np.float32)
@@ -129,7 +131,7 @@ class HipBackend:
         err = _ffi.errbuf()
         _ffi.check(_ffi.lib().twx_compose_vectors(self._h, interp, ctypes.c_float(wa), _p(a), _p(b), width, height, fmt,
                                                   None if fab is None else _p(fab), None if fba is None else _p(fba),
-                                                  fw, fh, int(above_first), _p(out), err, 512), err)
+                                                  fw, fh, int(warp_mode), int(above_first), _p(out), err, 512), err)
         return out
 
     # ---- the per-frame render loop, batched ----

@@ -514,6 +514,103 @@ void setup_weighted(DevPool& pool, FrameJob& F, bool above_first) {
     blend_branch(F.frac, F);
 }
 
+// A Smart Blend frame of the vector path (warping code 1): kind 4 Nearest, 5 Blend, 6 Weighted
+// Blend (the Weighted Blend buffers as well). The composite goes to its own image when the frame
+// is finished into wire bytes, else straight into `out`; the hole fill then works in place.
+// `has_hole` must be cleared before the first kernel.
+void setup_smart(DevPool& pool, FrameJob& F, int interp, bool above_first, int* has_hole) {
+    if (interp == 2) setup_weighted(pool, F, above_first);
+    F.kind = 4 + interp;
+    F.above_first = above_first ? 1 : 0;
+    F.comp = F.finish ? (void*)pool.alloc<uint8_t>(image_bytes(F.W, F.H, F.fmt)) : F.out;
+    F.holes = pool.alloc<uint8_t>((size_t)F.W * F.H);
+    F.has_hole = has_hole;
+    F.nearest_from_a = !(0.5f > F.wa) ? 1 : 0;  // nearest_smart (smartblend.rs:330-337)
+}
+
+// The level sizes of the hole-fill pyramid (holefill.rs:145-176): halve (at least 2) until both
+// sides are 2; the push starts at `top`.
+void hole_geometry(int w, int h, std::vector<std::pair<int, int>>& dims, int& top) {
+    int64_t a = w, b = h;
+    int count = 1;
+    for (;;) {
+        a /= 2;
+        b /= 2;
+        if (a < 2) a = 2;
+        if (b < 2) b = 2;
+        count++;
+        if (!(a > 1 && b > 1 && (a != 2 || b != 2))) break;
+    }
+    dims.clear();
+    int cw = w, ch = h;
+    for (int i = 0; i <= count; i++) {
+        dims.push_back({cw, ch});
+        cw = std::max(cw / 2, 2);
+        ch = std::max(ch / 2, 2);
+    }
+    top = count - 1;
+}
+
+// The hole fill of every Smart Blend frame of `jobs` (kinds 4-6), in chunks of frames whose
+// buffers fit a few GB; a frame without holes passes through every kernel at once.
+int run_fills(hipStream_t s, DevPool& pool, const std::vector<FrameJob>& jobs, std::string& err) {
+    std::vector<int> idx;
+    for (size_t i = 0; i < jobs.size(); i++)
+        if (jobs[i].kind >= 4) idx.push_back((int)i);
+    if (idx.empty()) return TWX_OK;
+    // per frame: 5 pull pyramids, 4 push pyramids, two level-0 scratch planes per pull plane
+    size_t per_frame = 0;
+    for (int i : idx) {
+        std::vector<std::pair<int, int>> dims;
+        int top;
+        hole_geometry(jobs[i].W, jobs[i].H, dims, top);
+        if (top + 1 > MAXFL) { err = "hole fill: a frame too large for the fill pyramid"; return TWX_E_UNSUPPORTED; }
+        size_t pyr = 0;
+        for (int l = 0; l <= top; l++) pyr += (size_t)dims[l].first * dims[l].second;
+        // every buffer rounded up to 64 floats
+        per_frame = std::max(per_frame, 4 * (9 * pyr + 10 * (size_t)jobs[i].W * jobs[i].H + 64 * (9 * (size_t)(top + 1) + 10)));
+    }
+    const size_t chunk = std::max<size_t>(1, std::min<size_t>(idx.size(), ((size_t)4 << 30) / per_frame));
+    char* scratch = pool.alloc<char>(chunk * per_frame);
+    for (size_t c0 = 0; c0 < idx.size(); c0 += chunk) {
+        std::vector<FillPlane> planes;
+        for (size_t k = c0; k < std::min(idx.size(), c0 + chunk); k++) {
+            const FrameJob& F = jobs[idx[k]];
+            std::vector<std::pair<int, int>> dims;
+            int top;
+            hole_geometry(F.W, F.H, dims, top);
+            float* base = reinterpret_cast<float*>(scratch + (k - c0) * per_frame);
+            auto take = [&](size_t n) { float* p = base; base += (n + 63) & ~(size_t)63; return p; };
+            FillPlane fp[5];
+            memset(fp, 0, sizeof(fp));
+            for (int c = 0; c < 5; c++) {
+                FillPlane& P = fp[c];
+                P.top = top;
+                P.chan = c;
+                P.gate = F.has_hole;
+                P.img = F.comp;
+                P.holes = F.holes;
+                P.fmt = F.fmt;
+                P.above_first = F.above_first;
+                for (int l = 0; l <= top; l++) {
+                    P.w[l] = dims[l].first;
+                    P.h[l] = dims[l].second;
+                    P.val[l] = take((size_t)dims[l].first * dims[l].second);
+                    if (c < 4) P.sol[l] = take((size_t)dims[l].first * dims[l].second);
+                }
+                P.t1 = take((size_t)F.W * F.H);
+                P.t2 = take((size_t)F.W * F.H);
+            }
+            for (int c = 0; c < 5; c++)
+                for (int l = 0; l <= top; l++) fp[c].wts[l] = fp[4].val[l];
+            for (int c = 0; c < 5; c++) planes.push_back(fp[c]);
+        }
+        FillPlane* dp = pool.upload(planes);
+        launch_fill(s, dp, planes);
+    }
+    return TWX_OK;
+}
+
 // The |amount| early-out of inv_warp (warp.rs:749-764): true means a plain copy.
 bool warp_is_copy(float amount, int fw, int W) {
     float a = amount < 0.0f ? -amount : amount;
@@ -663,8 +760,9 @@ int retime_frames_dev(twx_handle* h, DevPool& pool, const twx_retime_params& p, 
     }
 
     // compose and finish every output frame in one launch
-    int* bad = pool.alloc<int>(1);
-    HIPCHECK(hipMemsetAsync(bad, 0, sizeof(int), s));
+    // the NaN flag and one has-a-hole flag per output frame, cleared together
+    int* bad = pool.alloc<int>(1 + (size_t)count);
+    HIPCHECK(hipMemsetAsync(bad, 0, sizeof(int) * (1 + (size_t)count), s));
     std::vector<FrameJob> jobs(count);
     for (int i = 0; i < count; i++) {
         const Plan& pl = plan[i];
@@ -701,7 +799,11 @@ int retime_frames_dev(twx_handle* h, DevPool& pool, const twx_retime_params& p, 
             if (fsv[i].f_ba >= 0) F.f_ba = solves[fsv[i].f_ba].out;
             F.warp_a_copy = warp_is_copy(F.frac, g.w0, W) ? 1 : 0;
             F.warp_b_copy = warp_is_copy(F.wa, g.w0, W) ? 1 : 0;
-            if (pl.sel.interp == 0) {
+            if (p.smart_blend) {
+                // warping code 1 (frontend.rs:228); the retime front-end's stages take the row
+                // above first in the hole fill and the sharpening blur (frontend.rs:238-241)
+                setup_smart(pool, F, pl.sel.interp, true, bad + 1 + i);
+            } else if (pl.sel.interp == 0) {
                 F.kind = 1;
                 F.nearest_from_a = 0.5f <= pl.sel.wa ? 1 : 0;
             } else if (pl.sel.interp == 1) {
@@ -721,6 +823,8 @@ int retime_frames_dev(twx_handle* h, DevPool& pool, const twx_retime_params& p, 
     FrameJob* dj = pool.upload(jobs);
     launch_frames(s, dj, jobs);
     launch_weighted(s, dj, jobs);
+    if ((st = run_fills(s, pool, jobs, err))) return st;
+    launch_finish(s, dj, jobs);
     int hbad = 0;
     HIPCHECK(hipMemcpyAsync(&hbad, bad, sizeof(int), hipMemcpyDeviceToHost, s));
     HIPCHECK(hipStreamSynchronize(s));
@@ -894,7 +998,8 @@ int twx_solve_pair(twx_handle* h, const void* a, const void* b, int32_t width, i
 
 int twx_compose_vectors(twx_handle* h, int32_t interp, float wa, const void* a, const void* b, int32_t width,
                         int32_t height, int32_t fmt, const float* f_ab, const float* f_ba, int32_t flow_w,
-                        int32_t flow_h, int32_t taps_above_first, void* out, char* err, size_t errlen) {
+                        int32_t flow_h, int32_t warp_mode, int32_t taps_above_first, void* out, char* err,
+                        size_t errlen) {
     std::string e;
     int st = TWX_OK;
     try {
@@ -903,6 +1008,10 @@ int twx_compose_vectors(twx_handle* h, int32_t interp, float wa, const void* a, 
         if (interp < 0 || interp > 2) {
             set_err(err, errlen, "interpolation mode outside 0 (Nearest), 1 (Blend), 2 (Weighted Blend)");
             return TWX_E_MISMATCH;
+        }
+        if (warp_mode != 0 && warp_mode != 1) {
+            set_err(err, errlen, "only the inverse warps (warping codes 0 and 1) are served by this backend");
+            return TWX_E_UNSUPPORTED;
         }
         if (width < 2 || height < 2 || flow_w < 2 || flow_h < 2 || fmt < 0 || fmt > 2) {
             set_err(err, errlen, "inverse warp needs images and a flow of at least 2x2");
@@ -944,7 +1053,11 @@ int twx_compose_vectors(twx_handle* h, int32_t interp, float wa, const void* a, 
             HIPCHECK(hipMemcpyAsync(fl + fn, f_ba, fn * 4, hipMemcpyHostToDevice, s));
             F.f_ba = fl + fn;
         }
-        if (interp == 0) {
+        int* bad = pool.alloc<int>(2);
+        HIPCHECK(hipMemsetAsync(bad, 0, 2 * sizeof(int), s));
+        if (warp_mode == 1) {
+            setup_smart(pool, F, interp, taps_above_first != 0, bad + 1);
+        } else if (interp == 0) {
             F.kind = 1;
             F.nearest_from_a = 0.5f <= wa ? 1 : 0;
         } else if (interp == 1) {
@@ -953,13 +1066,13 @@ int twx_compose_vectors(twx_handle* h, int32_t interp, float wa, const void* a, 
         } else {
             setup_weighted(pool, F, taps_above_first != 0);
         }
-        int* bad = pool.alloc<int>(1);
-        HIPCHECK(hipMemsetAsync(bad, 0, sizeof(int), s));
         F.bad = bad;
         std::vector<FrameJob> jobs{F};
         FrameJob* dj = pool.upload(jobs);
         launch_frames(s, dj, jobs);
         launch_weighted(s, dj, jobs);
+        if ((st = run_fills(s, pool, jobs, e))) { set_err(err, errlen, e); return st; }
+        launch_finish(s, dj, jobs);
         int hbad = 0;
         HIPCHECK(hipMemcpyAsync(out, img + 2 * fb, fb, hipMemcpyDeviceToHost, s));
         HIPCHECK(hipMemcpyAsync(&hbad, bad, sizeof(int), hipMemcpyDeviceToHost, s));

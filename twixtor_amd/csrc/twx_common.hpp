@@ -80,6 +80,34 @@ struct FrameJob {
     float ca, cb;
     int above_first;
     int mask_small;        // a side under 3 pixels: the clean-up runs as one sequential pass
+    // Smart Blend (warping code 1, kinds 4 Nearest, 5 Blend, 6 Weighted Blend; smartblend.rs:
+    // 323-419): the composited engine image before the hole fill, the holes mask (255 - the
+    // combined coverage) and the frame's has-a-hole flag that gates the fill.
+    void* comp;
+    uint8_t* holes;
+    int* has_hole;
+    int pad2;
+};
+
+constexpr int MAXFL = 24;  // levels of the hole-fill pyramid (holefill.rs:145-176)
+
+// One plane of a batched hole fill (holefill.rs:214-336): a colour channel of a frame (chan 0..3)
+// or the frame's weight plane (chan 4). Pull: val[k] = resample(blur5(val[k - 1])); push from
+// sol[top] down to sol[0]. Every kernel of the fill returns at once when *gate is 0 (no hole).
+struct FillPlane {
+    float* val[MAXFL];        // pull pyramid (values, or the weights for chan 4)
+    float* sol[MAXFL];        // push solution per level (colour planes)
+    const float* wts[MAXFL];  // the frame's weight pyramid
+    int w[MAXFL], h[MAXFL];
+    int top;
+    int chan;
+    float* t1;                // scratch of the level-0 size
+    float* t2;
+    const int* gate;
+    void* img;                // the frame's engine image (load and store)
+    const uint8_t* holes;
+    int fmt;
+    int above_first;
 };
 
 // Per-kernel-family timing with HIP events on the launching stream, and the algorithmic bytes of
@@ -87,7 +115,7 @@ struct FrameJob {
 enum ProfFam {
     F_INGEST, F_LUMA, F_PYR_HPASS, F_PYR_VPASS, F_PYR_RESAMPLE, F_PYR_SUBTRACT, F_GRAD_TENSOR, F_BM_TRANSPOSE,
     F_SOLVE_INIT, F_CALC_B, F_NEW_DISP, F_BM_ROWS, F_BM_COLS, F_RANGE, F_UPSAMPLE, F_NEGATE, F_FRAMES,
-    F_WB_MASK, F_WB_SHARPEN, F_NFAM
+    F_WB_MASK, F_WB_SHARPEN, F_FILL_PULL, F_FILL_PUSH, F_FILL_IO, F_NFAM
 };
 extern const char* const PROF_NAMES[F_NFAM];
 
@@ -129,8 +157,14 @@ void launch_resample(hipStream_t s, const PlaneOp* d_ops, const std::vector<Plan
 void launch_subtract(hipStream_t s, const PlaneOp* d_ops, const std::vector<PlaneOp>& ops, int maxw, int maxh);
 void launch_solve(hipStream_t s, const SolveJob* d_jobs, const std::vector<SolveJob>& jobs, bool skip_refine);
 void launch_frames(hipStream_t s, const FrameJob* d_jobs, const std::vector<FrameJob>& jobs);
-// The Weighted Blend jobs (kind 3) after launch_frames: mask clean-up, sharpening, blend, finish.
+// The Weighted Blend jobs (kinds 3 and 6) after launch_frames: mask clean-up, sharpening, then
+// blend and finish (kind 3) or the Smart Blend composite into the frame's image (kind 6).
 void launch_weighted(hipStream_t s, const FrameJob* d_jobs, const std::vector<FrameJob>& jobs);
+// The hole fill of a set of planes (every level's buffers set up by the caller); `maxw`, `maxh`
+// per level over the planes.
+void launch_fill(hipStream_t s, const FillPlane* d_planes, const std::vector<FillPlane>& planes);
+// The Smart Blend frames (kinds 4-6) with finish set: the filled image into the wire frame.
+void launch_finish(hipStream_t s, const FrameJob* d_jobs, const std::vector<FrameJob>& jobs);
 void launch_nan_scan(hipStream_t s, const float* data, size_t n, int* flag);
 int run_selfcheck(hipStream_t s, std::string& err);
 

@@ -97,7 +97,7 @@ void launch_ingest(hipStream_t s, const void* wire, void* engine, int W, int H, 
 const char* const PROF_NAMES[F_NFAM] = {"ingest", "luma", "pyr_hpass", "pyr_vpass", "pyr_resample", "pyr_subtract",
                                         "tensor_gauss", "bm_transpose", "solve_init", "calc_b_gauss", "new_disp_gauss",
                                         "bm_rows", "bm_cols", "range", "upsample", "negate", "frames",
-                                        "wb_mask", "wb_sharpen_blend"};
+                                        "wb_mask", "wb_sharpen_blend", "fill_pull", "fill_push", "fill_io"};
 thread_local Prof* g_prof = nullptr;
 
 hipEvent_t Prof::take() {
@@ -241,8 +241,9 @@ __device__ __forceinline__ float hpass_elem(const float* __restrict__ row, int w
     return border_elem<C, false>(row + c, C, w, x, t);
 }
 
-// One output element of the vertical pass, border rows above first (conv.rs:272-311, 584-606).
-template <int C>
+// One output element of the vertical pass (conv.rs:272-311, 584-606): border rows above first,
+// or below first (the engine's default order) with ABOVE false.
+template <int C, bool ABOVE = true>
 __device__ __forceinline__ float vpass_elem(const float* __restrict__ src, int w, int h, int x, int y, int c, const Taps& t) {
     int n = t.n;
     int head = min(n, h);
@@ -259,7 +260,7 @@ __device__ __forceinline__ float vpass_elem(const float* __restrict__ src, int w
         }
         return s;
     }
-    return border_elem<C, true>(col, row, h, y, t);
+    return border_elem<C, ABOVE>(col, row, h, y, t);
 }
 
 __global__ void k_hpass1(const PlaneOp* __restrict__ ops, Taps t) {
@@ -946,17 +947,15 @@ __device__ __forceinline__ float store_rule(float v) {
     return FMT == 0 ? sat_u8(v) : (FMT == 1 ? sat_u16(v) : v);
 }
 
-// One warped pixel W(src, flow, amount) at output (x, y) (warp.rs:477-512), as the stored sample
-// values (integer formats hold exact integers in the floats). Returns false for a NaN coordinate.
-template <int FMT>
-__device__ __forceinline__ bool warp_pixel(const void* src, int W, int H, const float* __restrict__ flow, int fw, int fh,
-                                           float amount, int x, int y, float o[4]) {
+// The sample point of output pixel (x, y) of W(src, flow, amount) (warp.rs:477-486): the flow
+// sampled bilinearly at (x * rx, y * ry) (Flow::bilerp, warp.rs:415-442), scaled by amount / r.
+__device__ __forceinline__ void warp_point(const float* __restrict__ flow, int fw, int fh, int W, int H, float amount,
+                                           int x, int y, float& ix, float& iy) {
     int w1i = W - 1, h1i = H - 1;
     float rx = (float)(fw - 1) / (float)w1i;
     float ry = (float)(fh - 1) / (float)h1i;
     float sx = amount / rx, sy = amount / ry;
     float xf = (float)x, yf = (float)y;
-    // Flow::bilerp (warp.rs:415-442)
     float qx = xf * rx, qy = yf * ry;
     int fw1 = fw - 1, fh1 = fh - 1;
     int32_t jx = cvtt(qx), jy = cvtt(qy);
@@ -972,13 +971,14 @@ __device__ __forceinline__ bool warp_pixel(const void* src, int W, int H, const 
         float bot = (p11 - p01) * gx + p01;
         d[k] = (bot - top) * gy + top;
     }
-    float ix = d[0] * sx + xf;
-    float iy = d[1] * sy + yf;
-    if (!clip_to_rect(xf, yf, ix, iy, (float)w1i, (float)h1i) || isnan(ix) || isnan(iy)) {
-        o[0] = o[1] = o[2] = o[3] = 0.0f;
-        return false;
-    }
-    // bilerp4 (warp.rs:306-331)
+    ix = d[0] * sx + xf;
+    iy = d[1] * sy + yf;
+}
+
+// bilerp4 (warp.rs:306-331) at a point inside [0, W-1] x [0, H-1], before the store rule
+template <int FMT>
+__device__ __forceinline__ void bilerp_src(const void* src, int W, int H, float ix, float iy, float o[4]) {
+    int w1i = W - 1, h1i = H - 1;
     int32_t kx = cvtt(ix), ky = cvtt(iy);
     float ax = ix - (float)kx, ay = iy - (float)ky;
     if (kx >= w1i) { kx = w1i - 1; ax = 1.0f; }
@@ -993,9 +993,116 @@ __device__ __forceinline__ bool warp_pixel(const void* src, int W, int H, const 
     for (int k = 0; k < 4; k++) {
         float top = p00[k] + (p10[k] - p00[k]) * ax;
         float bot = p01[k] + (p11[k] - p01[k]) * ax;
-        o[k] = store_rule<FMT>(top + (bot - top) * ay);
+        o[k] = top + (bot - top) * ay;
     }
+}
+
+// One warped pixel W(src, flow, amount) at output (x, y) with hold edges (warp.rs:500-512), as the
+// stored sample values (integer formats hold exact integers in the floats). Returns false for a
+// NaN coordinate.
+template <int FMT>
+__device__ __forceinline__ bool warp_pixel(const void* src, int W, int H, const float* __restrict__ flow, int fw, int fh,
+                                           float amount, int x, int y, float o[4]) {
+    float ix, iy;
+    warp_point(flow, fw, fh, W, H, amount, x, y, ix, iy);
+    if (!clip_to_rect((float)x, (float)y, ix, iy, (float)(W - 1), (float)(H - 1)) || isnan(ix) || isnan(iy)) {
+        o[0] = o[1] = o[2] = o[3] = 0.0f;
+        return false;
+    }
+    float s[4];
+    bilerp_src<FMT>(src, W, H, ix, iy, s);
+#pragma unroll
+    for (int k = 0; k < 4; k++) o[k] = store_rule<FMT>(s[k]);
     return true;
+}
+
+// The store of a faded edge sample (warp.rs:237-295): the stored sample times f,
+// trunc((double)(b * f) + 0.5), low byte or word; float v * f.
+template <int FMT>
+__device__ __forceinline__ float store_faded(float v, float f) {
+    if (FMT == 2) return v * f;
+    float b = FMT == 0 ? sat_u8(v) : sat_u16(v);
+    float s = b * f;
+    return (float)(cvtt64d((double)s + 0.5) & (FMT == 0 ? 0xff : 0xffff));
+}
+
+// One pixel of the warp without hold edges (warp.rs:519-561): transparent one pixel or more
+// outside, faded in the border band, plain inside; `cov` gets the coverage byte. Returns false
+// only where the clip of a border-band point fails (never for finite data).
+template <int FMT>
+__device__ __forceinline__ bool warp_fade_pixel(const void* src, int W, int H, const float* __restrict__ flow, int fw,
+                                                int fh, float amount, int x, int y, float o[4], int& cov) {
+    float ix, iy;
+    warp_point(flow, fw, fh, W, H, amount, x, y, ix, iy);
+    const float wf = (float)W, hf = (float)H, w1 = (float)(W - 1), h1 = (float)(H - 1);
+    o[0] = o[1] = o[2] = o[3] = 0.0f;
+    if (!(ix > -1.0f && ix < wf && iy > -1.0f && iy < hf)) {
+        cov = 0;
+        return true;
+    }
+    float s[4];
+    if (ix >= 0.0f && ix <= w1 && iy >= 0.0f && iy <= h1) {
+        bilerp_src<FMT>(src, W, H, ix, iy, s);
+#pragma unroll
+        for (int k = 0; k < 4; k++) o[k] = store_rule<FMT>(s[k]);
+        cov = 255;
+        return true;
+    }
+    float f = ix < 0.0f ? ix + 1.0f : (ix <= w1 ? 1.0f : wf - ix);
+    if (iy < 0.0f) f *= iy + 1.0f;
+    else if (iy > h1) f *= hf - iy;
+    if (!clip_to_rect((float)x, (float)y, ix, iy, w1, h1) || isnan(ix) || isnan(iy)) {
+        cov = 0;
+        return false;
+    }
+    bilerp_src<FMT>(src, W, H, ix, iy, s);
+#pragma unroll
+    for (int k = 0; k < 4; k++) o[k] = store_faded<FMT>(s[k], f);
+    float cv = f * 255.0f + 0.5f;
+    cov = cv <= 255.0f ? (int)(cvtt64d((double)cv) & 0xff) : 255;
+    return true;
+}
+
+// `a` over `b` (smartblend.rs:167-178), premultiplied, alpha in units of white
+__device__ __forceinline__ void over4(const float a[4], const float b[4], float white, float o[4]) {
+    if (a[0] == 0.0f) { for (int k = 0; k < 4; k++) o[k] = b[k]; return; }
+    if (a[0] == white) { for (int k = 0; k < 4; k++) o[k] = a[k]; return; }
+    float ia = 1.0f - a[0] / white;
+    o[0] = (white - b[0]) * (a[0] / white) + b[0];
+    for (int k = 1; k < 4; k++) o[k] = b[k] * ia + a[k];
+}
+
+// One pixel of the Smart Blend composite (smartblend.rs:193-212) with its stores
+// (smartblend.rs:84-150): the main path `+ 0.5` and saturation (integers) or `+ 0.0` (float), the
+// "alpha is 0" fallback the plain truncation (integers) or the value (float).
+template <int FMT>
+__device__ __forceinline__ void smart_px(const float p1[4], const float p2[4], float w, float o[4]) {
+    const float white = FMT == 0 ? 255.0f : (FMT == 1 ? 65535.0f : 1.0f);
+    const int64_t m = FMT == 0 ? 0xff : 0xffff;
+    float c1[4], c2[4], v[4];
+    over4(p1, p2, white, c1);
+    over4(p2, p1, white, c2);
+    float amax = p1[0] > p2[0] ? p1[0] : p2[0];
+#pragma unroll
+    for (int k = 0; k < 4; k++) v[k] = (c2[k] - c1[k]) * w + c1[k];
+    bool raw = false;
+    if (v[0] != amax) {
+        if (v[0] == 0.0f) {
+            raw = true;
+            for (int k = 0; k < 4; k++) v[k] = c1[0] > c2[0] ? c1[k] : c2[k];
+        } else {
+            float alpha = v[0];
+            for (int k = 1; k < 4; k++) v[k] = (v[k] * amax) / alpha;
+            v[0] = amax;
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        if (FMT == 2) { o[k] = raw ? v[k] : v[k] + 0.0f; continue; }
+        if (raw) { o[k] = (float)(cvtt64f(v[k]) & m); continue; }
+        float y = v[k] + 0.5f;
+        o[k] = 0.0f > y ? 0.0f : (y > white ? white : (float)(cvtt64f(y) & m));
+    }
 }
 
 // lerp of one pixel, weight on the first (blend.rs:180-257)
@@ -1120,6 +1227,48 @@ __global__ void k_frames(const FrameJob* __restrict__ jobs) {
             if (F.warp_b_copy) load4<FMT>(F.b, mi, e);
             else ok = warp_pixel<FMT>(F.b, F.W, F.H, F.f_ab, F.fw, F.fh, F.wa, x, y, e);
         }
+    } else if (F.kind >= 4) {
+        // Smart Blend, first stage (smartblend.rs:323-419): the warps without hold edges and their
+        // coverage; Nearest keeps its one warp, Blend the composite, Weighted Blend both warps and
+        // their difference masks; the holes mask is 255 - max(cov1, cov2) (smart_finish)
+        using T = typename Px<FMT>::T;
+        int c1 = 255, c2 = 255;
+        if (F.kind == 4) {
+            if (F.nearest_from_a) {
+                if (F.warp_a_copy) load4<FMT>(F.a, mi, e);
+                else ok = warp_fade_pixel<FMT>(F.a, F.W, F.H, F.f_ba, F.fw, F.fh, F.frac, x, y, e, c1);
+            } else {
+                if (F.warp_b_copy) load4<FMT>(F.b, mi, e);
+                else ok = warp_fade_pixel<FMT>(F.b, F.W, F.H, F.f_ab, F.fw, F.fh, F.wa, x, y, e, c1);
+            }
+        } else {
+            if (F.warp_a_copy) load4<FMT>(F.a, mi, pa);
+            else ok = warp_fade_pixel<FMT>(F.a, F.W, F.H, F.f_ba, F.fw, F.fh, F.frac, x, y, pa, c1);
+            if (F.warp_b_copy) load4<FMT>(F.b, mi, pb);
+            else ok = warp_fade_pixel<FMT>(F.b, F.W, F.H, F.f_ab, F.fw, F.fh, F.wa, x, y, pb, c2) && ok;
+            if (F.kind == 5) smart_px<FMT>(pa, pb, F.frac, e);
+        }
+        if (F.kind == 6) {
+            T* oa = reinterpret_cast<T*>(F.ta) + mi * 4;
+            T* ob = reinterpret_cast<T*>(F.tb) + mi * 4;
+            float a0[4], b0[4];
+            load4<FMT>(F.a, mi, a0);
+            load4<FMT>(F.b, mi, b0);
+            for (int k = 0; k < 4; k++) { oa[k] = (T)pa[k]; ob[k] = (T)pb[k]; }
+            float da = (fabsf(pa[1] - a0[1]) + fabsf(pa[2] - a0[2])) + fabsf(pa[3] - a0[3]);
+            float db = (fabsf(pb[1] - b0[1]) + fabsf(pb[2] - b0[2])) + fabsf(pb[3] - b0[3]);
+            F.mask[0][0][mi] = da >= F.limit ? 255 : 0;
+            F.mask[1][0][mi] = db >= F.limit ? 255 : 0;
+        } else {
+            T* oc = reinterpret_cast<T*>(F.comp) + mi * 4;
+            for (int k = 0; k < 4; k++) oc[k] = (T)e[k];
+        }
+        // Nearest inverts its one coverage (smartblend.rs:336), the others the combined one
+        const int hole = 255 - (F.kind == 4 ? c1 : max(c1, c2));
+        F.holes[mi] = (uint8_t)hole;
+        if (hole != 0) *F.has_hole = 1;  // the same value from every writer
+        if (!ok) atomicOr(F.bad, 1);
+        return;
     } else {
         if (F.warp_a_copy) load4<FMT>(F.a, mi, pa);
         else ok = warp_pixel<FMT>(F.a, F.W, F.H, F.f_ba, F.fw, F.fh, F.frac, x, y, pa);
@@ -1166,9 +1315,13 @@ void launch_frames(hipStream_t s, const FrameJob* d_jobs, const std::vector<Fram
     double bytes = 0;
     for (auto& f : jobs) {
         double px = (double)f.W * f.H, bpp = 4.0 * sample_size(f.fmt);
-        double flows = f.kind == 0 ? 0.0 : (f.kind == 1 ? 8.0 : 16.0) * f.fw * f.fh;
-        // kind 3 reads A and B and writes both warped frames and both difference masks
-        bytes += (f.kind == 1 ? 2.0 : (f.kind == 3 ? 4.0 : 3.0)) * px * bpp + (f.kind == 3 ? 2.0 * px : 0.0) + flows;
+        double flows = f.kind == 0 ? 0.0 : ((f.kind == 1 || f.kind == 4) ? 8.0 : 16.0) * f.fw * f.fh;
+        // the frames read and written (Bpp each) and the u8 masks written
+        double imgs = 3.0, masks = 0.0;
+        if (f.kind == 1 || f.kind == 4) imgs = 2.0;
+        if (f.kind == 3 || f.kind == 6) { imgs = 4.0; masks = 2.0; }
+        if (f.kind >= 4) masks += 1.0;
+        bytes += imgs * px * bpp + masks * px + flows;
     }
     if (has[0]) TWX_LAUNCH(s, F_FRAMES, bytes, k_frames<0><<<grd, blk, 0, s>>>(d_jobs));
     if (has[1]) TWX_LAUNCH(s, F_FRAMES, bytes, k_frames<1><<<grd, blk, 0, s>>>(d_jobs));
@@ -1190,7 +1343,7 @@ enum MaskOp { M_PRUNE, M_DILATE, M_FILL };
 template <int OP>
 __global__ void k_mask_pass(const FrameJob* __restrict__ jobs, int src, int limit, int value, int clamp) {
     const FrameJob& F = jobs[blockIdx.z >> 1];
-    if (F.kind != 3 || F.mask_small) return;
+    if ((F.kind != 3 && F.kind != 6) || F.mask_small) return;
     const int half = blockIdx.z & 1;
     const int W = F.W, H = F.H;
     const int x = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1260,7 +1413,7 @@ __device__ void mask_border_seq(uint8_t* m, long wl, long hl) {
 // replications run as written. The result goes to buffer 1.
 __global__ void k_mask_small(const FrameJob* __restrict__ jobs) {
     const FrameJob& F = jobs[blockIdx.x >> 1];
-    if (F.kind != 3 || !F.mask_small || threadIdx.x != 0) return;
+    if ((F.kind != 3 && F.kind != 6) || !F.mask_small || threadIdx.x != 0) return;
     const int half = blockIdx.x & 1;
     const long W = F.W, H = F.H, n = W * H;
     uint8_t* m = F.mask[half][0];
@@ -1341,7 +1494,7 @@ __device__ __forceinline__ float4 wb_conv(const float4* p, int stride, int i, in
 template <int FMT>
 __global__ __launch_bounds__(256) void k_weighted(const FrameJob* __restrict__ jobs, BlurTaps bt) {
     const FrameJob& F = jobs[blockIdx.z];
-    if (F.fmt != FMT || F.kind != 3) return;
+    if (F.fmt != FMT || (F.kind != 3 && F.kind != 6)) return;
     const int W = F.W, H = F.H;
     const int x0 = blockIdx.x * WB_TW, y0 = blockIdx.y * WB_TH;
     if (x0 >= W || y0 >= H) return;
@@ -1412,8 +1565,15 @@ __global__ __launch_bounds__(256) void k_weighted(const FrameJob* __restrict__ j
         const int gx = x0 + tx, gy = y0 + ty + 4 * q;
         if (gx >= W || gy >= H) continue;
         float e[4];
-        blend_pixel<FMT>(res[0][q], res[1][q], F.blend_mode, F.weight, F.lerp_w, e);
         long mi = (long)gy * W + gx;
+        if (F.kind == 6) {
+            // Smart Blend of the sharpened frames into the image the hole fill works on
+            smart_px<FMT>(res[0][q], res[1][q], F.frac, e);
+            typename Px<FMT>::T* o = reinterpret_cast<typename Px<FMT>::T*>(F.comp) + mi * 4;
+            for (int k = 0; k < 4; k++) o[k] = (typename Px<FMT>::T)e[k];
+            continue;
+        }
+        blend_pixel<FMT>(res[0][q], res[1][q], F.blend_mode, F.weight, F.lerp_w, e);
         if (!F.finish) {
             typename Px<FMT>::T* o = reinterpret_cast<typename Px<FMT>::T*>(F.out) + mi * 4;
             for (int k = 0; k < 4; k++) o[k] = (typename Px<FMT>::T)e[k];
@@ -1428,7 +1588,7 @@ void launch_weighted(hipStream_t s, const FrameJob* d_jobs, const std::vector<Fr
     bool has[3] = {false, false, false}, small = false, big = false;
     double px = 0, img = 0;
     for (auto& f : jobs) {
-        if (f.kind != 3) continue;
+        if (f.kind != 3 && f.kind != 6) continue;
         nw++;
         mw = std::max(mw, f.W);
         mh = std::max(mh, f.H);
@@ -1458,6 +1618,178 @@ void launch_weighted(hipStream_t s, const FrameJob* d_jobs, const std::vector<Fr
     if (has[0]) TWX_LAUNCH(s, F_WB_SHARPEN, b, k_weighted<0><<<grd, blk, 0, s>>>(d_jobs, bt));
     if (has[1]) TWX_LAUNCH(s, F_WB_SHARPEN, b, k_weighted<1><<<grd, blk, 0, s>>>(d_jobs, bt));
     if (has[2]) TWX_LAUNCH(s, F_WB_SHARPEN, b, k_weighted<2><<<grd, blk, 0, s>>>(d_jobs, bt));
+}
+
+// -------------------------------------------------------------------------------------------
+// The push-pull hole fill (holefill.rs:214-336), batched over planes (frame x channel): the
+// frame's weight plane and its four colour planes are pulled alike (blur5, resample), the colour
+// planes pushed (combine, Gaussian r = 5, resample). The combine of a level is fused into the
+// resample that produces it (and into the top-level init and the final store). Each kernel
+// returns at once for a frame without holes (the fill is the identity then).
+
+// Load (holefill.rs:314-321): the channel in native units, holes zeroed; weights 0 at holes, else 1
+__global__ void k_fill_load(const FillPlane* __restrict__ planes) {
+    const FillPlane& P = planes[blockIdx.z];
+    if (!*P.gate) return;
+    const int W = P.w[0], H = P.h[0];
+    const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
+    if (x >= W || y >= H) return;
+    const long i = (long)y * W + x;
+    const bool hole = P.holes[i] != 0;  // !(eps > m) for a u8 mask
+    float v;
+    if (P.chan == 4) {
+        v = hole ? 0.0f : 1.0f;
+    } else {
+        if (P.fmt == TWX_FMT_U8) v = (float)reinterpret_cast<const uint8_t*>(P.img)[i * 4 + P.chan];
+        else if (P.fmt == TWX_FMT_U16) v = (float)reinterpret_cast<const uint16_t*>(P.img)[i * 4 + P.chan];
+        else v = reinterpret_cast<const float*>(P.img)[i * 4 + P.chan];
+        if (hole) v = 0.0f;
+    }
+    P.val[0][i] = v;
+}
+
+// The horizontal pass of the pull blur5 of level `level - 1` (push false) or of the push Gaussian
+// of level `level` (push true), into t1.
+__global__ void k_fill_hpass(const FillPlane* __restrict__ planes, int level, int push, Taps t) {
+    const FillPlane& P = planes[blockIdx.z];
+    if (!*P.gate || level > P.top || (push && P.chan == 4)) return;
+    const int l = push ? level : level - 1;
+    const int W = P.w[l], H = P.h[l];
+    const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
+    if (x >= W || y >= H) return;
+    const float* src = push ? P.sol[l] : P.val[l];
+    P.t1[(long)y * W + x] = hpass_elem<1>(src + (long)y * W, W, x, 0, t);
+}
+
+// The vertical pass of the same, t1 into t2 (pull) or into sol[level] (push); the border rows
+// take the row above or below first as the plane says.
+__global__ void k_fill_vpass(const FillPlane* __restrict__ planes, int level, int push, Taps t) {
+    const FillPlane& P = planes[blockIdx.z];
+    if (!*P.gate || level > P.top || (push && P.chan == 4)) return;
+    const int l = push ? level : level - 1;
+    const int W = P.w[l], H = P.h[l];
+    const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
+    if (x >= W || y >= H) return;
+    float v = P.above_first ? vpass_elem<1, true>(P.t1, W, H, x, y, 0, t) : vpass_elem<1, false>(P.t1, W, H, x, y, 0, t);
+    (push ? P.sol[l] : P.t2)[(long)y * W + x] = v;
+}
+
+// Pull: t2 (level - 1) resampled into val[level]. Push: sol[level] resampled into level - 1,
+// then the combine of that level (sol = value / weight where the weight is not 0).
+__global__ void k_fill_resample(const FillPlane* __restrict__ planes, int level, int push) {
+    const FillPlane& P = planes[blockIdx.z];
+    if (!*P.gate || level > P.top || level < 1 || (push && P.chan == 4)) return;
+    const int sl = push ? level : level - 1, dl = push ? level - 1 : level;
+    const int sw = P.w[sl], sh = P.h[sl], dw = P.w[dl], dh = P.h[dl];
+    const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
+    if (x >= dw || y >= dh) return;
+    const float sx = ((float)sw - 1.0f) / (float)(dw - 1);
+    const float sy = ((float)sh - 1.0f) / (float)(dh - 1);
+    const long i = (long)y * dw + x;
+    float v = bilinear1(push ? P.sol[sl] : P.t2, sw, sh, (float)x * sx, (float)y * sy);
+    if (!push) { P.val[dl][i] = v; return; }
+    const float wt = P.wts[dl][i];
+    if (wt != 0.0f) v = P.val[dl][i] / wt;
+    P.sol[dl][i] = v;
+}
+
+// The solution of the top level: the combine over zeros (holefill.rs:322-326).
+__global__ void k_fill_push_init(const FillPlane* __restrict__ planes) {
+    const FillPlane& P = planes[blockIdx.z];
+    if (!*P.gate || P.chan == 4) return;
+    const int l = P.top, W = P.w[l], H = P.h[l];
+    const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
+    if (x >= W || y >= H) return;
+    const long i = (long)y * W + x;
+    const float wt = P.wts[l][i];
+    P.sol[l][i] = wt != 0.0f ? P.val[l][i] / wt : 0.0f;
+}
+
+// The last combine of level 0 and the write back (holefill.rs:330-333, 366-386).
+__global__ void k_fill_store(const FillPlane* __restrict__ planes) {
+    const FillPlane& P = planes[blockIdx.z];
+    if (!*P.gate || P.chan == 4) return;
+    const int W = P.w[0], H = P.h[0];
+    const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
+    if (x >= W || y >= H) return;
+    const long i = (long)y * W + x;
+    const float wt = P.wts[0][i];
+    const float v = wt != 0.0f ? P.val[0][i] / wt : P.sol[0][i];
+    if (P.fmt == TWX_FMT_F32) {
+        reinterpret_cast<float*>(P.img)[i * 4 + P.chan] = v;
+        return;
+    }
+    const float white = P.fmt == TWX_FMT_U8 ? 255.0f : 65535.0f;
+    const float yv = v + 0.5f;
+    const float o = 0.0f > yv ? 0.0f : (yv > white ? white : (float)cvtt64f(yv));
+    if (P.fmt == TWX_FMT_U8) reinterpret_cast<uint8_t*>(P.img)[i * 4 + P.chan] = (uint8_t)o;
+    else reinterpret_cast<uint16_t*>(P.img)[i * 4 + P.chan] = (uint16_t)o;
+}
+
+void launch_fill(hipStream_t s, const FillPlane* d_planes, const std::vector<FillPlane>& planes) {
+    if (planes.empty()) return;
+    const unsigned np = (unsigned)planes.size();
+    int maxtop = 0;
+    for (auto& p : planes) maxtop = std::max(maxtop, p.top);
+    // the largest level sizes over the planes, and the pixels of each level over the planes
+    std::vector<int> mw(maxtop + 1, 0), mh(maxtop + 1, 0);
+    std::vector<double> px(maxtop + 1, 0.0);
+    for (auto& p : planes)
+        for (int l = 0; l <= p.top; l++) {
+            mw[l] = std::max(mw[l], p.w[l]);
+            mh[l] = std::max(mh[l], p.h[l]);
+            px[l] += (double)p.w[l] * p.h[l];
+        }
+    auto grid = [&](int l) { return dim3((mw[l] + 255) / 256, mh[l], np); };
+    const dim3 blk(256);
+    const Taps t5 = binomial_taps(), tg = gaussian_taps(5.0f);
+    TWX_LAUNCH(s, F_FILL_IO, 8.0 * px[0], k_fill_load<<<grid(0), blk, 0, s>>>(d_planes));
+    for (int k = 1; k <= maxtop; k++) {
+        TWX_LAUNCH(s, F_FILL_PULL, 8.0 * px[k - 1], k_fill_hpass<<<grid(k - 1), blk, 0, s>>>(d_planes, k, 0, t5));
+        TWX_LAUNCH(s, F_FILL_PULL, 8.0 * px[k - 1], k_fill_vpass<<<grid(k - 1), blk, 0, s>>>(d_planes, k, 0, t5));
+        TWX_LAUNCH(s, F_FILL_PULL, 4.0 * (px[k - 1] + px[k]), k_fill_resample<<<grid(k), blk, 0, s>>>(d_planes, k, 0));
+    }
+    TWX_LAUNCH(s, F_FILL_PUSH, 12.0 * px[maxtop], k_fill_push_init<<<grid(maxtop), blk, 0, s>>>(d_planes));
+    for (int l = maxtop; l >= 0; l--) {
+        TWX_LAUNCH(s, F_FILL_PUSH, 8.0 * px[l], k_fill_hpass<<<grid(l), blk, 0, s>>>(d_planes, l, 1, tg));
+        TWX_LAUNCH(s, F_FILL_PUSH, 8.0 * px[l], k_fill_vpass<<<grid(l), blk, 0, s>>>(d_planes, l, 1, tg));
+        if (l > 0)
+            TWX_LAUNCH(s, F_FILL_PUSH, 4.0 * px[l] + 16.0 * px[l - 1],
+                       k_fill_resample<<<grid(l - 1), blk, 0, s>>>(d_planes, l, 1));
+    }
+    TWX_LAUNCH(s, F_FILL_IO, 16.0 * px[0], k_fill_store<<<grid(0), blk, 0, s>>>(d_planes));
+}
+
+// The Smart Blend frames after the fill: the filled engine image into the wire frame, rows turned
+// back (image.rs:205-245).
+template <int FMT>
+__global__ void k_finish(const FrameJob* __restrict__ jobs) {
+    const FrameJob& F = jobs[blockIdx.z];
+    if (F.fmt != FMT || F.kind < 4 || !F.finish) return;
+    const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
+    if (x >= F.W || y >= F.H) return;
+    const long mi = (long)y * F.W + x;
+    float e[4];
+    load4<FMT>(F.comp, mi, e);
+    finish_pixel<FMT>(F.out, (long)F.W * F.H, (long)(F.H - 1 - y) * F.W + x, e);
+}
+
+void launch_finish(hipStream_t s, const FrameJob* d_jobs, const std::vector<FrameJob>& jobs) {
+    int mw = 0, mh = 0;
+    bool has[3] = {false, false, false};
+    double bytes = 0;
+    for (auto& f : jobs) {
+        if (f.kind < 4 || !f.finish) continue;
+        mw = std::max(mw, f.W);
+        mh = std::max(mh, f.H);
+        has[f.fmt] = true;
+        bytes += 2.0 * f.W * f.H * 4 * sample_size(f.fmt);
+    }
+    if (!mw) return;
+    dim3 blk(256), grd((mw + 255) / 256, mh, (unsigned)jobs.size());
+    if (has[0]) TWX_LAUNCH(s, F_FILL_IO, bytes, k_finish<0><<<grd, blk, 0, s>>>(d_jobs));
+    if (has[1]) TWX_LAUNCH(s, F_FILL_IO, bytes, k_finish<1><<<grd, blk, 0, s>>>(d_jobs));
+    if (has[2]) TWX_LAUNCH(s, F_FILL_IO, bytes, k_finish<2><<<grd, blk, 0, s>>>(d_jobs));
 }
 
 // -------------------------------------------------------------------------------------------

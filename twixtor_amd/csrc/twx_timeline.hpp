@@ -187,10 +187,6 @@ inline int check_params(const twx_retime_params& p, std::string& err) {
         err = "retime parameters: popup out of range";
         return TWX_E_MISMATCH;
     }
-    if (p.smart_blend) {
-        err = "Smart Blend (warping code 1) is not served by this backend";
-        return TWX_E_UNSUPPORTED;
-    }
     return TWX_OK;
 }
 
