@@ -42,6 +42,9 @@ def main():
     ap.add_argument("--no-profile", action="store_true", help="time the steps without the per-launch HIP events")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run one device-to-device copy of MIB MiB (calibrates the HBM traffic counters)")
+    ap.add_argument("--batch-clips", type=int, default=64,
+                    help="clips of the mixed-size batch leg (batched against one call per output frame); 0 skips it")
+    ap.add_argument("--batch-only", action="store_true", help="run only the mixed-size batch leg")
     args = ap.parse_args()
     if args.copy_probe:
         import torch
@@ -66,6 +69,10 @@ def main():
     from twixtor_amd.params import Retime
 
     dev = torch.device("cuda:0")
+    if args.batch_only:
+        be = HipBackend(0)
+        print(json.dumps({"batch": batch_leg(be, args.batch_clips, args.steps, args.warmup)}), flush=True)
+        return
     spec = synth.spec_1080p(frames=args.frames)
     gen = synth.Clip(spec, device=dev)
     clip = torch.cat([gen.wire(t) for t in range(spec.frames)]).contiguous()
@@ -133,6 +140,9 @@ def main():
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(cfg, clip, out, spec.frames, n_out, fb, args.cpu_seconds)
+    del clip, out
+    torch.cuda.empty_cache()
+    batch = batch_leg(be, args.batch_clips, args.steps, args.warmup) if args.batch_clips > 0 else None
 
     line = {
         "metric": "1080p retime output frames/s",
@@ -154,9 +164,64 @@ def main():
         "roofline": roofline,
         "path": path,
         "per_pair": per_pair,
+        "batch": batch,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
+
+
+def batch_leg(be, n_clips, steps, warmup):
+    """The mixed-size batch: n_clips RGBA 8-bit clips of 24 frames (widths 320..2048, heights
+    180..1152), each retimed at 33.3333 % into 70 output frames. Batched: one twx_retime_batch call
+    over every clip. Against: one call per output frame (the reference's render loop), which solves
+    the pair of every in-between frame. The output bytes of both must be equal."""
+    import torch
+
+    from twixtor_amd import synth
+    from twixtor_amd.params import Retime
+
+    dev = torch.device("cuda:0")
+    specs = synth.spec_batch(clips=n_clips)
+    n_in, n_out = 24, 70
+    cfgs = [Retime(s.width, s.height, 8, "24", ["speed_percent=33.3333"]) for s in specs]
+    structs = [c.struct() for c in cfgs]
+    clips = [synth.Clip(s, device=dev).clip(n_in).contiguous() for s in specs]
+    outs = [torch.zeros(c.frame_bytes() * n_out, dtype=torch.uint8, device=dev) for c in cfgs]
+    n = len(cfgs)
+    cptr, optr = [c.data_ptr() for c in clips], [o.data_ptr() for o in outs]
+    torch.cuda.synchronize()
+
+    def batched():
+        be.retime_batch(structs, cptr, [n_in] * n, [0] * n, [n_out] * n, optr)
+
+    for _ in range(warmup):
+        batched()
+    torch.cuda.synchronize()
+    s0 = be.stats()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        batched()
+    torch.cuda.synchronize()
+    t_batch = (time.perf_counter() - t0) / steps
+    s1 = be.stats()
+    solves = (s1["pair_solves"] - s0["pair_solves"]) // steps
+    outs2 = [torch.zeros_like(o) for o in outs]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k, cfg in enumerate(cfgs):
+        fb = cfg.frame_bytes()
+        for i in range(n_out):
+            be.retime_frames_dev(cfg, cptr[k], n_in, i, 1, outs2[k].data_ptr() + i * fb, structs[k])
+    torch.cuda.synchronize()
+    t_pp = time.perf_counter() - t0
+    same = all(torch.equal(a, b) for a, b in zip(outs, outs2))
+    px = sum(s.width * s.height for s in specs)
+    return {"workload": f"{n} RGBA 8-bit clips, widths 320-2048, heights 180-1152, {n_in} frames each, "
+                        f"speed_percent=33.3333 -> {n_out} output frames each",
+            "clips": n, "output_frames": n * n_out, "megapixels_per_output_frame_avg": round(px / n / 1e6, 3),
+            "unique_solves_per_step": solves, "batched_ms": round(t_batch * 1000.0, 2),
+            "batched_frames_per_s": round(n * n_out / t_batch, 1), "per_pair_ms": round(t_pp * 1000.0, 2),
+            "per_pair_calls": n * n_out, "speedup": round(t_pp / t_batch, 2), "bytes_equal": bool(same)}
 
 
 def cpu_baseline(cfg, clip, out, n_in, n_out, fb, budget_s):

@@ -151,6 +151,16 @@ int twx_retime_frames_dev(twx_handle* h, const twx_retime_params* p, const void*
                           int32_t n_in, int32_t first, int32_t count, void* out,
                           char* err, size_t errlen);
 
+/* Many clips, of any sizes, depths and settings, in one batched render: clip i renders its
+ * output frames [first[i], first[i] + count[i]) from clips[i] (n_in[i] wire frames) into outs[i].
+ * DEVICE pointers. Every frame pair of every clip goes through the same ingest, pyramid, solve and
+ * compose launches (the frame tables sorted by size, one launch per size class); the bytes of each
+ * clip equal a twx_retime_frames_dev call on it alone. No counterpart in the reference, whose
+ * render loop takes one output frame at a time (temporal.rs:183-250). */
+int twx_retime_batch(twx_handle* h, int32_t n_clips, const twx_retime_params* params,
+                     const void* const* clips, const int32_t* n_in, const int32_t* first,
+                     const int32_t* count, void* const* outs, char* err, size_t errlen);
+
 /* The same with host pointers (uploads the clip, downloads the frames). */
 int twx_retime_frames(twx_handle* h, const twx_retime_params* p, const void* clip, int32_t n_in,
                       int32_t first, int32_t count, void* out, char* err, size_t errlen);

@@ -178,3 +178,43 @@ def test_smart_blend_compositions_bit_exact(backend, w, h, fmt):
                     np.testing.assert_array_equal(
                         got.view(DT[fmt]), want.view(DT[fmt]),
                         err_msg=f"{w}x{h} fmt {fmt} spread {spread} interp {interp} wa {wa} above_first {above}")
+
+
+def test_batch_of_mixed_clips_equals_single_clip_calls(backend):
+    """Clips of different sizes, depths and settings in one twx_retime_batch call give the bytes of
+    one call per clip, and of one call per output frame; two of them are checked against the oracle."""
+    import torch
+
+    from twixtor_amd import synth
+
+    dev = torch.device("cuda:0")
+    setups = [((40, 30, 8), ["speed_percent=50"]), ((64, 36, 16), ["speed_percent=100@0;25@3;100@5"]),
+              ((33, 47, 32), ["speed_percent=40", "smart_blend=1"]), ((96, 54, 8), ["speed_percent=62.5", "frame_interp=3"]),
+              ((24, 20, 16), ["speed_percent=33.3333", "frame_interp=1", "vectors=3"]), ((128, 72, 8), ["speed_percent=50"])]
+    cfgs, clips, outs, counts = [], [], [], []
+    for k, ((w, h, d), params) in enumerate(setups):
+        spec = synth.spec_small(seed=10 + k, width=w, height=h, frames=6, depth=d, scale=w / 160.0)
+        clip = synth.Clip(spec, device=dev).clip().contiguous()
+        cfg = Retime(w, h, d, "24", params)
+        n_out = 11
+        cfgs.append(cfg)
+        clips.append(clip)
+        counts.append(n_out)
+        outs.append(torch.zeros(cfg.frame_bytes() * n_out, dtype=torch.uint8, device=dev))
+    structs = [c.struct() for c in cfgs]
+    backend.retime_batch(structs, [c.data_ptr() for c in clips], [6] * len(cfgs), [0] * len(cfgs), counts,
+                         [o.data_ptr() for o in outs])
+    torch.cuda.synchronize()
+    for k, cfg in enumerate(cfgs):
+        single = torch.zeros_like(outs[k])
+        backend.retime_frames_dev(cfg, clips[k].data_ptr(), 6, 0, counts[k], single.data_ptr())
+        per_frame = torch.zeros_like(outs[k])
+        fb = cfg.frame_bytes()
+        for n in range(counts[k]):
+            backend.retime_frames_dev(cfg, clips[k].data_ptr(), 6, n, 1, per_frame.data_ptr() + n * fb)
+        torch.cuda.synchronize()
+        assert torch.equal(outs[k], single), f"clip {k}: batch vs single call"
+        assert torch.equal(outs[k], per_frame), f"clip {k}: batch vs one call per frame"
+    for k in (2, 3):
+        want = cpuref.retime_frames(cfgs[k].struct(), clips[k].cpu().numpy().tobytes(), 6, 0, counts[k])
+        assert outs[k].cpu().numpy().tobytes() == want, f"clip {k} vs oracle"

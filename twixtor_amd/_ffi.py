@@ -15,7 +15,8 @@ TWX_OK, TWX_E_DEVICE, TWX_E_NAN_OUTPUT, TWX_E_UNSUPPORTED, TWX_E_MISMATCH = 0, 1
 # every entry point of include/twx.h
 EXPORTS = [
     "twx_open", "twx_close", "twx_selfcheck", "twx_memory_budget", "twx_flow_geometry", "twx_solve",
-    "twx_solve_pair", "twx_compose_vectors", "twx_select", "twx_retime_frames_dev", "twx_retime_frames",
+    "twx_solve_pair", "twx_compose_vectors", "twx_select", "twx_retime_frames_dev", "twx_retime_batch",
+    "twx_retime_frames",
     "twx_get_stats", "twx_stream", "twx_profile_enable", "twx_profile_reset", "twx_profile_read",
 ]
 
@@ -67,6 +68,14 @@ def lib():
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise ImportError(f"{LIB_PATH} is not built: run `python -m twixtor_amd.build` (no CPU fallback exists)")
+        # PyTorch-ROCm ships its own HIP runtime under the same soname (libamdhip64.so.7) as the
+        # system one libtwx links: the first one loaded serves the whole process, and torch does not
+        # work on another release's runtime, while libtwx works on either. Load torch's first when
+        # it is installed, so a process that uses both (device tensors as buffers) keeps working.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = ctypes.CDLL(LIB_PATH)
         L.twx_open.argtypes = [ctypes.c_int, ctypes.POINTER(P)] + ERR
         L.twx_close.argtypes = [P]
@@ -82,6 +91,9 @@ def lib():
         L.twx_select.argtypes = [ctypes.POINTER(RetimeParams), I32, ctypes.POINTER(Selection)] + ERR
         L.twx_retime_frames_dev.argtypes = [P, ctypes.POINTER(RetimeParams), P, I32, I32, I32, P] + ERR
         L.twx_retime_frames.argtypes = [P, ctypes.POINTER(RetimeParams), P, I32, I32, I32, P] + ERR
+        PI32 = ctypes.POINTER(ctypes.c_int32)
+        L.twx_retime_batch.argtypes = [P, I32, ctypes.POINTER(RetimeParams), ctypes.POINTER(ctypes.c_void_p), PI32, PI32,
+                                       PI32, ctypes.POINTER(ctypes.c_void_p)] + ERR
         L.twx_get_stats.argtypes = [P, ctypes.POINTER(Stats)]
         L.twx_stream.argtypes = [P]
         L.twx_stream.restype = P

@@ -151,3 +151,14 @@ class HipBackend:
         err = _ffi.errbuf()
         _ffi.check(_ffi.lib().twx_retime_frames_dev(self._h, ctypes.byref(p), ctypes.c_void_p(clip_ptr), n_in, first, count,
                                                     ctypes.c_void_p(out_ptr), err, 512), err)
+
+    def retime_batch(self, params: list, clip_ptrs: list, n_in: list, first: list, count: list, out_ptrs: list) -> None:
+        """Many clips in one batched render (device addresses): clip i's output frames
+        [first[i], first[i] + count[i]) into out_ptrs[i]. `params` are RetimeParams structs."""
+        n = len(params)
+        P = (RetimeParams * n)(*params)
+        I32 = ctypes.c_int32 * n
+        V = ctypes.c_void_p * n
+        err = _ffi.errbuf()
+        _ffi.check(_ffi.lib().twx_retime_batch(self._h, n, P, V(*clip_ptrs), I32(*n_in), I32(*first), I32(*count),
+                                               V(*out_ptrs), err, 512), err)

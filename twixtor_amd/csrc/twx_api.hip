@@ -16,6 +16,7 @@
 #include <map>
 #include <mutex>
 #include <thread>
+#include <tuple>
 #include <vector>
 
 #include "twx_common.hpp"
@@ -633,55 +634,91 @@ twx_flow_settings settings_for(const twx_retime_params& p, int vq) {
     return fs;
 }
 
-// The batched render of output frames [first, first + count) with the clip and the output on the
-// device.
-int retime_frames_dev(twx_handle* h, DevPool& pool, const twx_retime_params& p, const uint8_t* clip, int n_in, int first,
-                      int count, uint8_t* out, std::string& err) {
-    int st = timeline::check_params(p, err);
-    if (st) return st;
+// One clip of a batched render: output frames [first, first + count) of a retime run over a clip
+// of n_in wire frames, clip and output on the device.
+struct ClipRun {
+    twx_retime_params p;
+    const uint8_t* clip;
+    int n_in, first, count;
+    uint8_t* out;
+};
+
+// The batched render of every output frame of every run: one plan on the host, then one ingest of
+// every source frame, one pyramid build, one solve per flow-settings group, one compose-and-finish
+// launch set over every output frame. Frame pairs are keyed per run, so identical pairs of a clip
+// are solved once.
+int retime_batch_dev(twx_handle* h, DevPool& pool, const std::vector<ClipRun>& runs, std::string& err) {
     hipStream_t s = h->stream;
     double t0 = now_ms();
-    timeline::Rates rates;
-    if ((st = timeline::rates(p, rates, err))) return st;
-    timeline::SpeedMap map(p, rates);
-    const int W = p.width, H = p.height, fmt = p.fmt;
-    const size_t fb = image_bytes(W, H, fmt);
-
+    int st;
     struct Plan {
         twx_selection sel;
-        int vector;  // 0 fast path, 1 vector path
-        long ia, ib; // clip indices (-1 transparent)
+        int vector;   // 0 fast path, 1 vector path
+        int sa, sb;   // slots of frames a and b
     };
-    std::vector<Plan> plan(count);
-    for (int i = 0; i < count; i++) {
-        Plan& pl = plan[i];
-        if ((st = timeline::select(p, map, first + i, pl.sel, err))) return st;
-        pl.vector = (pl.sel.vectors_quality != 0 && !pl.sel.exact) ? 1 : 0;
-        if (pl.sel.interp < 0 || pl.sel.interp > 2) { err = "frame_interp outside 1..3"; return TWX_E_MISMATCH; }
-        pl.ia = timeline::clip_index(rates, pl.sel.a, n_in);
-        pl.ib = timeline::clip_index(rates, pl.sel.b, n_in);
+    struct Slot {
+        int run;
+        long idx;     // clip index, -1 transparent
+        uint8_t* eng;
+    };
+    std::vector<std::vector<Plan>> plans(runs.size());
+    std::vector<Slot> slots;
+    std::map<std::pair<int, long>, int> slot_of;
+    long total_frames = 0;
+    for (size_t r = 0; r < runs.size(); r++) {
+        const ClipRun& R = runs[r];
+        if ((st = timeline::check_params(R.p, err))) return st;
+        if (R.count < 0 || R.n_in < 0) { err = "negative frame count"; return TWX_E_MISMATCH; }
+        timeline::Rates rates;
+        if ((st = timeline::rates(R.p, rates, err))) return st;
+        timeline::SpeedMap map(R.p, rates);
+        const size_t fb = image_bytes(R.p.width, R.p.height, R.p.fmt);
+        auto slot = [&](long idx) {
+            auto key = std::make_pair((int)r, idx);
+            auto it = slot_of.find(key);
+            if (it != slot_of.end()) return it->second;
+            int k = (int)slots.size();
+            slots.push_back(Slot{(int)r, idx, pool.alloc<uint8_t>(fb)});
+            slot_of[key] = k;
+            return k;
+        };
+        plans[r].resize(R.count);
+        for (int i = 0; i < R.count; i++) {
+            Plan& pl = plans[r][i];
+            if ((st = timeline::select(R.p, map, R.first + i, pl.sel, err))) return st;
+            pl.vector = (pl.sel.vectors_quality != 0 && !pl.sel.exact) ? 1 : 0;
+            if (pl.sel.interp < 0 || pl.sel.interp > 2) { err = "frame_interp outside 1..3"; return TWX_E_MISMATCH; }
+            pl.sa = slot(timeline::clip_index(rates, pl.sel.a, R.n_in));
+            pl.sb = slot(timeline::clip_index(rates, pl.sel.b, R.n_in));
+        }
+        total_frames += R.count;
     }
 
-    // source slots: one engine image (motion rows) per clip index used; -1 is the transparent frame
-    std::map<long, int> slot_of;
-    std::vector<long> slots;
-    auto slot = [&](long idx) {
-        auto it = slot_of.find(idx);
-        if (it != slot_of.end()) return it->second;
-        int k = (int)slots.size();
-        slots.push_back(idx);
-        slot_of[idx] = k;
-        return k;
-    };
-    for (auto& pl : plan) { slot(pl.ia); slot(pl.ib); }
-    uint8_t* engine = pool.alloc<uint8_t>(fb * slots.size());
-    for (size_t k = 0; k < slots.size(); k++) {
-        if (slots[k] < 0) HIPCHECK(hipMemsetAsync(engine + k * fb, 0, fb, s));
-        else launch_ingest(s, clip + (size_t)slots[k] * fb, engine + k * fb, W, H, fmt, true);
+    // every source frame into its engine image (rows reversed for the motion path), largest first
+    {
+        std::vector<IngestOp> ops;
+        for (auto& sl : slots) {
+            const ClipRun& R = runs[sl.run];
+            const size_t fb = image_bytes(R.p.width, R.p.height, R.p.fmt);
+            IngestOp op{};
+            op.wire = sl.idx >= 0 ? R.clip + (size_t)sl.idx * fb : nullptr;
+            op.engine = sl.eng;
+            op.W = R.p.width;
+            op.H = R.p.height;
+            op.fmt = R.p.fmt;
+            op.zero = sl.idx < 0 ? 1 : 0;
+            op.reverse = 1;
+            ops.push_back(op);
+        }
+        std::stable_sort(ops.begin(), ops.end(), [](const IngestOp& a, const IngestOp& b) {
+            return (long)a.W * a.H > (long)b.W * b.H;
+        });
+        if (!ops.empty()) launch_ingest_batch(s, pool.upload(ops), ops);
     }
 
-    // pyramids of the frames the vector path solves on, and the unique solves
-    std::map<int, int> pyr_of;  // slot -> pyramid index
+    // pyramids of the frames the vector path solves on, and the unique solves; flow settings come
+    // from the run (quality flag, sensitivity) and the frame's vectors quality
+    std::map<std::pair<int, int>, int> pyr_of;  // (slot, vq) -> pyramid
     std::vector<PyrIn> pyrs;
     struct SolveKey {
         int x, y, vq;
@@ -689,28 +726,26 @@ int retime_frames_dev(twx_handle* h, DevPool& pool, const twx_retime_params& p, 
     };
     std::map<SolveKey, int> solve_of;
     std::vector<SolveIn> solves;
-    std::vector<int> solve_vq;
-    std::map<int, Geo> geo_of_vq;
-    auto geo_for = [&](int vq, Geo& g) -> int {
-        auto it = geo_of_vq.find(vq);
-        if (it != geo_of_vq.end()) { g = it->second; return TWX_OK; }
-        twx_flow_settings fs = settings_for(p, vq);
-        int r = geometry(fs, W, H, g, err);
-        if (!r) geo_of_vq[vq] = g;
-        return r;
+    std::vector<std::tuple<int, int, uint32_t>> solve_group;  // (vq, hiq, sensitivity bits)
+    std::map<std::pair<int, int>, Geo> geo_of;                 // (run, vq)
+    auto geo_for = [&](int r, int vq, Geo& g) -> int {
+        auto it = geo_of.find({r, vq});
+        if (it != geo_of.end()) { g = it->second; return TWX_OK; }
+        twx_flow_settings fs = settings_for(runs[r].p, vq);
+        int e = geometry(fs, runs[r].p.width, runs[r].p.height, g, err);
+        if (!e) geo_of[{r, vq}] = g;
+        return e;
     };
     auto pyramid = [&](int sl, int vq) -> int {
-        // one pyramid per (slot, quality); quality is constant over a run (no segment marks)
-        int key = sl * 8 + vq;
-        auto it = pyr_of.find(key);
+        auto it = pyr_of.find({sl, vq});
         if (it != pyr_of.end()) return it->second;
+        const ClipRun& R = runs[slots[sl].run];
         Geo g;
-        (void)geo_for(vq, g);
-        PyrIn pi{engine + (size_t)sl * fb, W, H, fmt, g, nullptr};
-        pi.levels = pool.alloc<float>(pyramid_floats(g));
+        (void)geo_for(slots[sl].run, vq, g);
+        PyrIn pi{slots[sl].eng, R.p.width, R.p.height, R.p.fmt, g, pool.alloc<float>(pyramid_floats(g))};
         int k = (int)pyrs.size();
         pyrs.push_back(pi);
-        pyr_of[key] = k;
+        pyr_of[{sl, vq}] = k;
         return k;
     };
     auto solve = [&](int sx, int sy, int vq) -> int {
@@ -722,116 +757,145 @@ int retime_frames_dev(twx_handle* h, DevPool& pool, const twx_retime_params& p, 
         SolveIn si{pyrs[px].levels, pyrs[py].levels, g, pool.alloc<float>((size_t)g.w0 * g.h0 * 2)};
         int k = (int)solves.size();
         solves.push_back(si);
-        solve_vq.push_back(vq);
+        twx_flow_settings fs = settings_for(runs[slots[sx].run].p, vq);
+        uint32_t sb;
+        memcpy(&sb, &fs.sensitivity, 4);
+        solve_group.push_back(std::make_tuple(vq, fs.hiq, sb));
         solve_of[key] = k;
         return k;
     };
     struct FrameSolves {
         int f_ab = -1, f_ba = -1;
     };
-    std::vector<FrameSolves> fsv(count);
-    for (int i = 0; i < count; i++) {
-        const Plan& pl = plan[i];
-        if (!pl.vector) continue;
-        Geo g;
-        if ((st = geo_for(pl.sel.vectors_quality, g))) return st;
-        int sa = slot_of[pl.ia], sb = slot_of[pl.ib];
-        int vq = pl.sel.vectors_quality;
-        if (pl.sel.interp == 0) {
-            // Nearest: one solve, the one nearest_uses_f_ab names (compose/vector.rs:170-176)
-            if (pl.sel.wa < 0.5f) fsv[i].f_ab = solve(sa, sb, vq);
-            else fsv[i].f_ba = solve(sb, sa, vq);
-        } else {
-            fsv[i].f_ba = solve(sb, sa, vq);
-            fsv[i].f_ab = solve(sa, sb, vq);
+    std::vector<std::vector<FrameSolves>> fsv(runs.size());
+    for (size_t r = 0; r < runs.size(); r++) {
+        fsv[r].resize(plans[r].size());
+        for (size_t i = 0; i < plans[r].size(); i++) {
+            const Plan& pl = plans[r][i];
+            if (!pl.vector) continue;
+            Geo g;
+            if ((st = geo_for((int)r, pl.sel.vectors_quality, g))) return st;
+            int vq = pl.sel.vectors_quality;
+            if (pl.sel.interp == 0) {
+                // Nearest: one solve, the one nearest_uses_f_ab names (compose/vector.rs:170-176)
+                if (pl.sel.wa < 0.5f) fsv[r][i].f_ab = solve(pl.sa, pl.sb, vq);
+                else fsv[r][i].f_ba = solve(pl.sb, pl.sa, vq);
+            } else {
+                fsv[r][i].f_ba = solve(pl.sb, pl.sa, vq);
+                fsv[r][i].f_ab = solve(pl.sa, pl.sb, vq);
+            }
         }
     }
-    build_pyramids(s, pool, pyrs);
+    {
+        std::vector<PyrIn> sorted = pyrs;
+        std::stable_sort(sorted.begin(), sorted.end(),
+                         [](const PyrIn& a, const PyrIn& b) { return (long)a.W * a.H > (long)b.W * b.H; });
+        build_pyramids(s, pool, sorted);
+    }
     h->stats.pyramid_builds += pyrs.size();
     h->stats.pair_solves += solves.size();
-    // solves grouped by quality (one settings struct per launch set)
     {
-        std::map<int, std::vector<SolveIn>> by_vq;
-        for (size_t k = 0; k < solves.size(); k++) by_vq[solve_vq[k]].push_back(solves[k]);
-        for (auto& kv : by_vq) {
-            twx_flow_settings fs = settings_for(p, kv.first);
+        std::map<std::tuple<int, int, uint32_t>, std::vector<SolveIn>> groups;
+        for (size_t k = 0; k < solves.size(); k++) groups[solve_group[k]].push_back(solves[k]);
+        for (auto& kv : groups) {
+            twx_flow_settings fs{};
+            fs.quality = std::get<0>(kv.first);
+            fs.hiq = std::get<1>(kv.first);
+            uint32_t sb = std::get<2>(kv.first);
+            memcpy(&fs.sensitivity, &sb, 4);
+            std::stable_sort(kv.second.begin(), kv.second.end(), [](const SolveIn& a, const SolveIn& b) {
+                return (long)a.geo.w0 * a.geo.h0 > (long)b.geo.w0 * b.geo.h0;
+            });
             if ((st = run_solves(s, pool, kv.second, fs, h->budget / 2, err))) return st;
         }
     }
 
-    // compose and finish every output frame in one launch
-    // the NaN flag and one has-a-hole flag per output frame, cleared together
-    int* bad = pool.alloc<int>(1 + (size_t)count);
-    HIPCHECK(hipMemsetAsync(bad, 0, sizeof(int) * (1 + (size_t)count), s));
-    std::vector<FrameJob> jobs(count);
-    for (int i = 0; i < count; i++) {
-        const Plan& pl = plan[i];
-        FrameJob& F = jobs[i];
-        memset(&F, 0, sizeof(F));
-        F.a = engine + (size_t)slot_of[pl.ia] * fb;
-        F.b = engine + (size_t)slot_of[pl.ib] * fb;
-        F.out = out + (size_t)i * fb;
-        F.bad = bad;
-        F.fmt = fmt;
-        F.W = W;
-        F.H = H;
-        F.finish = 1;
-        F.wa = pl.sel.wa;
-        F.frac = 1.0f - pl.sel.wa;
-        if (!pl.vector) {
-            F.kind = 0;
-            h->stats.fast_frames++;
-            if (pl.sel.interp == 0) {
-                F.blend_mode = pl.sel.wa >= 0.5f ? 0 : 1;
-            } else if (pl.sel.interp == 1) {
-                blend_branch((float)(1.0 - (double)pl.sel.wa), F);
+    // compose and finish every output frame; the NaN flag and one has-a-hole flag per output frame
+    int* bad = pool.alloc<int>(1 + (size_t)total_frames);
+    HIPCHECK(hipMemsetAsync(bad, 0, sizeof(int) * (1 + (size_t)total_frames), s));
+    std::vector<FrameJob> jobs;
+    jobs.reserve(total_frames);
+    for (size_t r = 0; r < runs.size(); r++) {
+        const ClipRun& R = runs[r];
+        const int W = R.p.width, H = R.p.height;
+        const size_t fb = image_bytes(W, H, R.p.fmt);
+        for (size_t i = 0; i < plans[r].size(); i++) {
+            const Plan& pl = plans[r][i];
+            FrameJob F;
+            memset(&F, 0, sizeof(F));
+            F.a = slots[pl.sa].eng;
+            F.b = slots[pl.sb].eng;
+            F.out = R.out + i * fb;
+            F.bad = bad;
+            F.fmt = R.p.fmt;
+            F.W = W;
+            F.H = H;
+            F.finish = 1;
+            F.wa = pl.sel.wa;
+            F.frac = 1.0f - pl.sel.wa;
+            if (!pl.vector) {
+                F.kind = 0;
+                h->stats.fast_frames++;
+                if (pl.sel.interp == 0) {
+                    F.blend_mode = pl.sel.wa >= 0.5f ? 0 : 1;
+                } else if (pl.sel.interp == 1) {
+                    blend_branch((float)(1.0 - (double)pl.sel.wa), F);
+                } else {
+                    // weighted_blend_noflow (blend.rs:280-300): Blend with 1 - wa in f32, float inputs + 0.0
+                    blend_branch(1.0f - pl.sel.wa, F);
+                    F.plus_zero = 1;
+                }
             } else {
-                // weighted_blend_noflow (blend.rs:280-300): Blend with 1 - wa in f32, float inputs + 0.0
-                blend_branch(1.0f - pl.sel.wa, F);
-                F.plus_zero = 1;
+                if (W < 2 || H < 2) { err = "inverse warp needs images of at least 2x2"; return TWX_E_MISMATCH; }
+                h->stats.composes++;
+                Geo g = geo_of[{(int)r, pl.sel.vectors_quality}];
+                F.fw = g.w0;
+                F.fh = g.h0;
+                if (fsv[r][i].f_ab >= 0) F.f_ab = solves[fsv[r][i].f_ab].out;
+                if (fsv[r][i].f_ba >= 0) F.f_ba = solves[fsv[r][i].f_ba].out;
+                F.warp_a_copy = warp_is_copy(F.frac, g.w0, W) ? 1 : 0;
+                F.warp_b_copy = warp_is_copy(F.wa, g.w0, W) ? 1 : 0;
+                if (R.p.smart_blend) {
+                    // warping code 1 (frontend.rs:228); the retime front-end's stages take the row
+                    // above first in the hole fill and the sharpening blur (frontend.rs:238-241)
+                    setup_smart(pool, F, pl.sel.interp, true, bad + 1 + jobs.size());
+                } else if (pl.sel.interp == 0) {
+                    F.kind = 1;
+                    F.nearest_from_a = 0.5f <= pl.sel.wa ? 1 : 0;
+                } else if (pl.sel.interp == 1) {
+                    F.kind = 2;
+                    blend_branch(F.frac, F);
+                } else {
+                    // Weighted Blend: the retime front-end's stages take the row above first in
+                    // the blur of the sharpening step (frontend.rs:238-241)
+                    setup_weighted(pool, F, true);
+                }
             }
-        } else {
-            h->stats.composes++;
-            Geo g = geo_of_vq[pl.sel.vectors_quality];
-            F.fw = g.w0;
-            F.fh = g.h0;
-            if (fsv[i].f_ab >= 0) F.f_ab = solves[fsv[i].f_ab].out;
-            if (fsv[i].f_ba >= 0) F.f_ba = solves[fsv[i].f_ba].out;
-            F.warp_a_copy = warp_is_copy(F.frac, g.w0, W) ? 1 : 0;
-            F.warp_b_copy = warp_is_copy(F.wa, g.w0, W) ? 1 : 0;
-            if (p.smart_blend) {
-                // warping code 1 (frontend.rs:228); the retime front-end's stages take the row
-                // above first in the hole fill and the sharpening blur (frontend.rs:238-241)
-                setup_smart(pool, F, pl.sel.interp, true, bad + 1 + i);
-            } else if (pl.sel.interp == 0) {
-                F.kind = 1;
-                F.nearest_from_a = 0.5f <= pl.sel.wa ? 1 : 0;
-            } else if (pl.sel.interp == 1) {
-                F.kind = 2;
-                blend_branch(F.frac, F);
-            } else {
-                // Weighted Blend: the retime front-end's stages take the row above first in the
-                // blur of the sharpening step (frontend.rs:238-241)
-                setup_weighted(pool, F, true);
-            }
+            jobs.push_back(F);
         }
     }
-    if (W < 2 || H < 2) {
-        for (auto& F : jobs)
-            if (F.kind != 0) { err = "inverse warp needs images of at least 2x2"; return TWX_E_MISMATCH; }
+    std::stable_sort(jobs.begin(), jobs.end(),
+                     [](const FrameJob& a, const FrameJob& b) { return (long)a.W * a.H > (long)b.W * b.H; });
+    if (!jobs.empty()) {
+        FrameJob* dj = pool.upload(jobs);
+        launch_frames(s, dj, jobs);
+        launch_weighted(s, dj, jobs);
+        if ((st = run_fills(s, pool, jobs, err))) return st;
+        launch_finish(s, dj, jobs);
     }
-    FrameJob* dj = pool.upload(jobs);
-    launch_frames(s, dj, jobs);
-    launch_weighted(s, dj, jobs);
-    if ((st = run_fills(s, pool, jobs, err))) return st;
-    launch_finish(s, dj, jobs);
-    int hbad = 0;
-    HIPCHECK(hipMemcpyAsync(&hbad, bad, sizeof(int), hipMemcpyDeviceToHost, s));
+    int* hbad = pool.pinned<int>(1);
+    HIPCHECK(hipMemcpyAsync(hbad, bad, sizeof(int), hipMemcpyDeviceToHost, s));
     HIPCHECK(hipStreamSynchronize(s));
     HIPCHECK(hipGetLastError());
     h->stats.device_ms += now_ms() - t0;
-    if (hbad) { err = "inverse warp: a sample coordinate is NaN or never converges, which is not supported"; return TWX_E_UNSUPPORTED; }
+    if (*hbad) { err = "inverse warp: a sample coordinate is NaN or never converges, which is not supported"; return TWX_E_UNSUPPORTED; }
     return TWX_OK;
+}
+
+// The batched render of output frames [first, first + count) of one clip on the device.
+int retime_frames_dev(twx_handle* h, DevPool& pool, const twx_retime_params& p, const uint8_t* clip, int n_in, int first,
+                      int count, uint8_t* out, std::string& err) {
+    return retime_batch_dev(h, pool, {ClipRun{p, clip, n_in, first, count, out}}, err);
 }
 
 // The Backend::solve / solve_pair path on host engine images (rows as given).
@@ -1110,6 +1174,33 @@ int twx_retime_frames_dev(twx_handle* h, const twx_retime_params* p, const void*
         if (g.ok()) {
             DevPool pool(h);
             st = retime_frames_dev(h, pool, *p, (const uint8_t*)clip, n_in, first, count, (uint8_t*)out, e);
+        }
+    } catch (const HipError& x) {
+        e = x.msg;
+        st = TWX_E_DEVICE;
+    }
+    if (st) set_err(err, errlen, e);
+    return st;
+}
+
+int twx_retime_batch(twx_handle* h, int32_t n_clips, const twx_retime_params* params, const void* const* clips,
+                     const int32_t* n_in, const int32_t* first, const int32_t* count, void* const* outs, char* err,
+                     size_t errlen) {
+    std::string e;
+    int st = TWX_OK;
+    try {
+        Guard g(h, st, e);
+        if (g.ok()) {
+            if (n_clips < 0 || (n_clips > 0 && (!params || !clips || !n_in || !first || !count || !outs))) {
+                e = "batch: missing arrays";
+                st = TWX_E_MISMATCH;
+            } else {
+                std::vector<ClipRun> runs;
+                for (int i = 0; i < n_clips; i++)
+                    runs.push_back(ClipRun{params[i], (const uint8_t*)clips[i], n_in[i], first[i], count[i], (uint8_t*)outs[i]});
+                DevPool pool(h);
+                st = retime_batch_dev(h, pool, runs, e);
+            }
         }
     } catch (const HipError& x) {
         e = x.msg;

@@ -148,8 +148,24 @@ void prof_end(hipStream_t s, int fam, double bytes, hipEvent_t a);
         prof_end(stream, fam, (double)(bytes), e0_);        \
     } while (0)
 
-// Host side launchers (twx_device.hip).
+// One source frame of a batch: wire frame -> engine image (rows reversed), or zeros.
+struct IngestOp {
+    const uint8_t* wire;
+    uint8_t* engine;
+    int W, H, fmt;
+    int zero;     // a frame outside the clip: transparent
+    int reverse;  // the motion path's row order
+    int pad;
+};
+
+// Consecutive ranges of a table sorted by area, largest first, in which the largest area is at
+// most twice the smallest: a mixed-size batch launches once per range, one size gives one range.
+std::vector<std::pair<int, int>> size_buckets(const std::vector<double>& areas);
+
+// Host side launchers (twx_device.hip). Tables with several frame sizes are sorted by area,
+// largest first.
 void launch_ingest(hipStream_t s, const void* wire, void* engine, int W, int H, int fmt, bool reverse_rows);
+void launch_ingest_batch(hipStream_t s, const IngestOp* d_ops, const std::vector<IngestOp>& ops);
 void launch_luma(hipStream_t s, const PlaneOp* d_ops, const std::vector<PlaneOp>& ops, int maxw, int maxh);
 void launch_hpass1(hipStream_t s, const PlaneOp* d_ops, const std::vector<PlaneOp>& ops, int maxw, int maxh, const Taps& t);
 void launch_vpass1(hipStream_t s, const PlaneOp* d_ops, const std::vector<PlaneOp>& ops, int maxw, int maxh, const Taps& t);

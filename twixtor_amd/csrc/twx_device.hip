@@ -45,10 +45,8 @@ __device__ __forceinline__ float bits_f(uint32_t b) { return __uint_as_float(b);
 // Wire -> engine image (image.rs:173-203, frame.rs:79-111, image.rs:264-298)
 
 template <int FMT>
-__global__ void k_ingest(const uint8_t* __restrict__ wire, uint8_t* __restrict__ engine, int W, int H, int reverse) {
-    int x = blockIdx.x * blockDim.x + threadIdx.x;
-    int y = blockIdx.y;
-    if (x >= W) return;
+__device__ __forceinline__ void ingest_pixel(const uint8_t* __restrict__ wire, uint8_t* __restrict__ engine, int W, int H,
+                                             int x, int y, int reverse) {
     long n = (long)W * H;
     long si = (long)y * W + x;
     int yo = reverse ? H - 1 - y : y;
@@ -78,6 +76,64 @@ __global__ void k_ingest(const uint8_t* __restrict__ wire, uint8_t* __restrict__
         float g = pl[si], b = pl[n + si], r = pl[2 * n + si], a = pl[3 * n + si];
         float4 o = make_float4(a, a * r, a * g, a * b);
         reinterpret_cast<float4*>(engine)[di] = o;
+    }
+}
+
+template <int FMT>
+__global__ void k_ingest(const uint8_t* __restrict__ wire, uint8_t* __restrict__ engine, int W, int H, int reverse) {
+    int x = blockIdx.x * blockDim.x + threadIdx.x;
+    int y = blockIdx.y;
+    if (x >= W) return;
+    ingest_pixel<FMT>(wire, engine, W, H, x, y, reverse);
+}
+
+// Every source frame of a batch in one launch per format (blockIdx.z = op); a transparent frame
+// (outside the clip) is written as zeros.
+template <int FMT>
+__global__ void k_ingest_batch(const IngestOp* __restrict__ ops) {
+    const IngestOp& op = ops[blockIdx.z];
+    if (op.fmt != FMT) return;
+    const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
+    if (x >= op.W || y >= op.H) return;
+    if (op.zero) {
+        const long i = (long)y * op.W + x;
+        if (FMT == TWX_FMT_U8) reinterpret_cast<uchar4*>(op.engine)[i] = make_uchar4(0, 0, 0, 0);
+        else if (FMT == TWX_FMT_U16) reinterpret_cast<ushort4*>(op.engine)[i] = make_ushort4(0, 0, 0, 0);
+        else reinterpret_cast<float4*>(op.engine)[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        return;
+    }
+    ingest_pixel<FMT>(op.wire, op.engine, op.W, op.H, x, y, op.reverse);
+}
+
+std::vector<std::pair<int, int>> size_buckets(const std::vector<double>& areas) {
+    std::vector<std::pair<int, int>> out;
+    int n = (int)areas.size();
+    for (int b0 = 0; b0 < n;) {
+        int b1 = b0 + 1;
+        while (b1 < n && areas[b1] * 2.0 >= areas[b0]) b1++;
+        out.push_back({b0, b1});
+        b0 = b1;
+    }
+    return out;
+}
+
+void launch_ingest_batch(hipStream_t s, const IngestOp* d_ops, const std::vector<IngestOp>& ops) {
+    std::vector<double> area;
+    for (auto& o : ops) area.push_back((double)o.W * o.H);
+    for (auto [b0, b1] : size_buckets(area)) {
+        int mw = 0, mh = 0;
+        bool has[3] = {false, false, false};
+        double bytes = 0;
+        for (int i = b0; i < b1; i++) {
+            mw = std::max(mw, ops[i].W);
+            mh = std::max(mh, ops[i].H);
+            has[ops[i].fmt] = true;
+            bytes += (ops[i].zero ? 1.0 : 2.0) * image_bytes(ops[i].W, ops[i].H, ops[i].fmt);
+        }
+        dim3 blk(256), grd((mw + 255) / 256, mh, b1 - b0);
+        if (has[0]) TWX_LAUNCH(s, F_INGEST, bytes, k_ingest_batch<0><<<grd, blk, 0, s>>>(d_ops + b0));
+        if (has[1]) TWX_LAUNCH(s, F_INGEST, bytes, k_ingest_batch<1><<<grd, blk, 0, s>>>(d_ops + b0));
+        if (has[2]) TWX_LAUNCH(s, F_INGEST, bytes, k_ingest_batch<2><<<grd, blk, 0, s>>>(d_ops + b0));
     }
 }
 
@@ -803,7 +859,7 @@ __global__ void k_solve_init(const SolveJob* __restrict__ jobs) {
     }
 }
 
-void launch_solve(hipStream_t s, const SolveJob* d_jobs, const std::vector<SolveJob>& jobs, bool skip_refine) {
+static void launch_solve_range(hipStream_t s, const SolveJob* d_jobs, const std::vector<SolveJob>& jobs, bool skip_refine) {
     int nj = (int)jobs.size();
     if (nj == 0) return;
     int max_start = 0;
@@ -863,6 +919,13 @@ void launch_solve(hipStream_t s, const SolveJob* d_jobs, const std::vector<Solve
     double P0 = 0;
     for (auto& j : jobs) { w0 = std::max(w0, j.w[0]); h0 = std::max(h0, j.h[0]); P0 += (double)j.w[0] * j.h[0]; }
     TWX_LAUNCH(s, F_NEGATE, 16.0 * P0, k_negate<<<dim3((w0 + 255) / 256, h0, nj), dim3(256), 0, s>>>(d_jobs));
+}
+
+void launch_solve(hipStream_t s, const SolveJob* d_jobs, const std::vector<SolveJob>& jobs, bool skip_refine) {
+    std::vector<double> area;
+    for (auto& j : jobs) area.push_back((double)j.w[0] * j.h[0]);
+    for (auto [b0, b1] : size_buckets(area))
+        launch_solve_range(s, d_jobs + b0, std::vector<SolveJob>(jobs.begin() + b0, jobs.begin() + b1), skip_refine);
 }
 
 __global__ void k_nan_scan(const float* __restrict__ d, size_t n, int* flag) {
@@ -1304,7 +1367,20 @@ __global__ void k_frames(const FrameJob* __restrict__ jobs) {
     finish_pixel<FMT>(F.out, n, oi, e);
 }
 
+static std::vector<std::pair<int, int>> frame_buckets(const std::vector<FrameJob>& jobs) {
+    std::vector<double> area;
+    for (auto& f : jobs) area.push_back((double)f.W * f.H);
+    return size_buckets(area);
+}
+
+static void launch_frames_range(hipStream_t s, const FrameJob* d_jobs, const std::vector<FrameJob>& jobs);
+
 void launch_frames(hipStream_t s, const FrameJob* d_jobs, const std::vector<FrameJob>& jobs) {
+    for (auto [b0, b1] : frame_buckets(jobs))
+        launch_frames_range(s, d_jobs + b0, std::vector<FrameJob>(jobs.begin() + b0, jobs.begin() + b1));
+}
+
+static void launch_frames_range(hipStream_t s, const FrameJob* d_jobs, const std::vector<FrameJob>& jobs) {
     if (jobs.empty()) return;
     int mw = 0, mh = 0;
     bool has[3] = {false, false, false};
@@ -1583,7 +1659,14 @@ __global__ __launch_bounds__(256) void k_weighted(const FrameJob* __restrict__ j
     }
 }
 
+static void launch_weighted_range(hipStream_t s, const FrameJob* d_jobs, const std::vector<FrameJob>& jobs);
+
 void launch_weighted(hipStream_t s, const FrameJob* d_jobs, const std::vector<FrameJob>& jobs) {
+    for (auto [b0, b1] : frame_buckets(jobs))
+        launch_weighted_range(s, d_jobs + b0, std::vector<FrameJob>(jobs.begin() + b0, jobs.begin() + b1));
+}
+
+static void launch_weighted_range(hipStream_t s, const FrameJob* d_jobs, const std::vector<FrameJob>& jobs) {
     int mw = 0, mh = 0, nw = 0;
     bool has[3] = {false, false, false}, small = false, big = false;
     double px = 0, img = 0;
@@ -1726,7 +1809,16 @@ __global__ void k_fill_store(const FillPlane* __restrict__ planes) {
     else reinterpret_cast<uint16_t*>(P.img)[i * 4 + P.chan] = (uint16_t)o;
 }
 
+static void launch_fill_range(hipStream_t s, const FillPlane* d_planes, const std::vector<FillPlane>& planes);
+
 void launch_fill(hipStream_t s, const FillPlane* d_planes, const std::vector<FillPlane>& planes) {
+    std::vector<double> area;
+    for (auto& p : planes) area.push_back((double)p.w[0] * p.h[0]);
+    for (auto [b0, b1] : size_buckets(area))
+        launch_fill_range(s, d_planes + b0, std::vector<FillPlane>(planes.begin() + b0, planes.begin() + b1));
+}
+
+static void launch_fill_range(hipStream_t s, const FillPlane* d_planes, const std::vector<FillPlane>& planes) {
     if (planes.empty()) return;
     const unsigned np = (unsigned)planes.size();
     int maxtop = 0;
@@ -1774,7 +1866,14 @@ __global__ void k_finish(const FrameJob* __restrict__ jobs) {
     finish_pixel<FMT>(F.out, (long)F.W * F.H, (long)(F.H - 1 - y) * F.W + x, e);
 }
 
+static void launch_finish_range(hipStream_t s, const FrameJob* d_jobs, const std::vector<FrameJob>& jobs);
+
 void launch_finish(hipStream_t s, const FrameJob* d_jobs, const std::vector<FrameJob>& jobs) {
+    for (auto [b0, b1] : frame_buckets(jobs))
+        launch_finish_range(s, d_jobs + b0, std::vector<FrameJob>(jobs.begin() + b0, jobs.begin() + b1));
+}
+
+static void launch_finish_range(hipStream_t s, const FrameJob* d_jobs, const std::vector<FrameJob>& jobs) {
     int mw = 0, mh = 0;
     bool has[3] = {false, false, false};
     double bytes = 0;

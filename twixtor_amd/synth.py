@@ -163,3 +163,16 @@ def spec_1080p(seed: int = 1, frames: int = 120) -> ClipSpec:
     rng = np.random.RandomState(seed)
     discs = _rng_discs(rng, 6, 1920, 1080, 20.0, 120.0, 24.0)
     return ClipSpec(1920, 1080, frames, 32, (0.7 * 12, -0.3 * 12), discs, 2.0 / 255.0, seed, 16.0 * 12)
+
+
+def spec_batch(seed: int = 3, clips: int = 64, frames: int = 24) -> list:
+    """The mixed-size batch: `clips` RGBA 8-bit clips of `frames` frames, widths uniform in
+    320..2048 and heights in 180..1152 (multiples of 4, seeded), each the 160x96 generator scaled by
+    width / 160 (disc radii and velocities, background velocity and lattice cell)."""
+    rng = np.random.RandomState(seed)
+    out = []
+    for i in range(clips):
+        w = int(rng.randint(80, 513)) * 4
+        h = int(rng.randint(45, 289)) * 4
+        out.append(spec_small(seed=seed * 1000 + i, width=w, height=h, frames=frames, depth=8, scale=w / 160.0))
+    return out
