@@ -505,23 +505,56 @@ __device__ __forceinline__ void produce(const SolveJob& J, int level, int src, i
     }
 }
 
-constexpr int TILE_X = 64, TILE_Y = 16, TILE_THREADS = 256;
+constexpr int TILE_X = 64, TILE_Y = 32, TILE_THREADS = 256, TILE_RUN = 8;
+
+// The interior formula (every tap inside the line) for TILE_RUN consecutive outputs of one lane
+// from a register window: win[m + R + d] is the sample at offset d of output m. The operations are
+// those of line_elem's interior branch, in the same order.
+template <int R>
+__device__ __forceinline__ void run_interior(const float* win, const Taps& t, float* out) {
+#pragma unroll
+    for (int m = 0; m < TILE_RUN; m++) {
+        float s = win[m + R] * t.k[0];
+#pragma unroll
+        for (int j = R; j >= 1; j--) s = s + (win[m + R + j] + win[m + R - j]) * t.k[j];
+        out[m] = s;
+    }
+}
+
+// Tiles of a level in an order for the 8 XCDs: the hardware deals workgroup b to XCD b % 8, so
+// tile (b % 8) * ceil(n / 8) + b / 8 gives each XCD a contiguous band of tiles (row-major within a
+// job), whose halos the XCD's own L2 then serves.
+__device__ __forceinline__ int xcd_tile(int n) {
+    const int per = (n + 7) / 8;
+    const int b = blockIdx.x;
+    return (b & 7) * per + (b >> 3);
+}
 
 // A producer step fused with the separable Gaussian that follows it (flow.rs:1277-1287): the
 // tile's region (tile plus the kernel's reach on every side) is produced into LDS, the horizontal
-// pass runs over the region's rows, the vertical pass (border rows above first) over the tile.
-// Every element is computed exactly as by the separate passes, so the bits are the same.
+// pass runs over the region's rows (results back into the region's middle columns, all reads
+// before any write), the vertical pass (border rows above first) over the tile. Each lane
+// computes runs of TILE_RUN outputs from a register window where every tap lies inside the image,
+// and the border formula element by element elsewhere. Every element is computed exactly as by
+// the separate passes, so the bits are the same.
 template <int C, int R, int MODE>
 __global__ __launch_bounds__(TILE_THREADS) void k_tile_gauss(const SolveJob* __restrict__ jobs, int level, int src,
-                                                             int dst, Taps t) {
-    const SolveJob& J = jobs[blockIdx.z];
+                                                             int dst, Taps t, int tiles_x, int tiles_y, int njobs) {
+    const int per_job = tiles_x * tiles_y;
+    const int lt = xcd_tile(per_job * njobs);
+    if (lt >= per_job * njobs) return;
+    const int job = lt / per_job, rem = lt - job * per_job;
+    const int tyi = rem / tiles_x, txi = rem - tyi * tiles_x;
+    const SolveJob& J = jobs[job];
     if (level > J.start) return;
     const int w = J.w[level], h = J.h[level];
-    const int x0 = blockIdx.x * TILE_X, y0 = blockIdx.y * TILE_Y;
+    const int x0 = txi * TILE_X, y0 = tyi * TILE_Y;
     if (x0 >= w || y0 >= h) return;
     constexpr int RW = TILE_X + 2 * R, RH = TILE_Y + 2 * R;
+    constexpr int RUNS_X = TILE_X / TILE_RUN, HITEMS = RH * RUNS_X;
+    constexpr int HPER = (HITEMS + TILE_THREADS - 1) / TILE_THREADS;
+    constexpr bool MULTI = C > 1;
     __shared__ float s_in[RH * RW * C];
-    __shared__ float s_h[RH * TILE_X * C];
     const int tid = threadIdx.x;
     for (int k = tid; k < RH * RW; k += TILE_THREADS) {
         const int ry = k / RW, rx = k - ry * RW;
@@ -534,25 +567,87 @@ __global__ __launch_bounds__(TILE_THREADS) void k_tile_gauss(const SolveJob* __r
         for (int c = 0; c < C; c++) s_in[(ry * RW + rx) * C + c] = v[c];
     }
     __syncthreads();
-    for (int k = tid; k < RH * TILE_X; k += TILE_THREADS) {
-        const int ry = k / TILE_X, hx = k - ry * TILE_X;
-        const int x = x0 + hx, y = y0 - R + ry;
-        if (x >= w || y < 0 || y >= h) continue;
-        const float* line = s_in + ry * RW * C;
+    // horizontal pass: item = (region row, run of TILE_RUN columns); results held until every
+    // lane has read its window
+    const int nh = min(R + 1, w);
+    const int hint = w > 2 * (R + 1) ? w - 2 * (R + 1) : 0;
+    float hres[HPER][TILE_RUN][C];
 #pragma unroll
-        for (int c = 0; c < C; c++)
-            s_h[(ry * TILE_X + hx) * C + c] = line_elem<R, false, (C > 1)>(line + c, C, x0 - R, w, x, t);
+    for (int q = 0; q < HPER; q++) {
+        const int it = tid + q * TILE_THREADS;
+        if (it >= HITEMS) continue;
+        const int ry = it / RUNS_X, rx0 = (it - ry * RUNS_X) * TILE_RUN;
+        const int y = y0 - R + ry, xs = x0 + rx0;
+        if (y < 0 || y >= h || xs >= w) continue;
+        const float* line = s_in + ry * RW * C;
+        if (xs >= nh && xs + TILE_RUN - 1 < nh + hint) {
+#pragma unroll
+            for (int c = 0; c < C; c++) {
+                float win[TILE_RUN + 2 * R];
+#pragma unroll
+                for (int d = 0; d < TILE_RUN + 2 * R; d++) win[d] = line[(rx0 + d) * C + c];
+                float o[TILE_RUN];
+                run_interior<R>(win, t, o);
+#pragma unroll
+                for (int m = 0; m < TILE_RUN; m++) hres[q][m][c] = o[m];
+            }
+        } else {
+#pragma unroll
+            for (int m = 0; m < TILE_RUN; m++) {
+                if (xs + m >= w) continue;
+#pragma unroll
+                for (int c = 0; c < C; c++) hres[q][m][c] = line_elem<R, false, MULTI>(line + c, C, x0 - R, w, xs + m, t);
+            }
+        }
     }
     __syncthreads();
+#pragma unroll
+    for (int q = 0; q < HPER; q++) {
+        const int it = tid + q * TILE_THREADS;
+        if (it >= HITEMS) continue;
+        const int ry = it / RUNS_X, rx0 = (it - ry * RUNS_X) * TILE_RUN;
+        const int y = y0 - R + ry, xs = x0 + rx0;
+        if (y < 0 || y >= h || xs >= w) continue;
+#pragma unroll
+        for (int m = 0; m < TILE_RUN; m++) {
+            if (xs + m >= w) continue;
+#pragma unroll
+            for (int c = 0; c < C; c++) s_in[(ry * RW + R + rx0 + m) * C + c] = hres[q][m][c];
+        }
+    }
+    __syncthreads();
+    // vertical pass: item = (column, run of TILE_RUN rows), one per lane
     float* out = MODE == P_TENSOR ? J.ta : (MODE == P_CALCB ? J.tb : J.flow[dst]);
-    for (int k = tid; k < TILE_Y * TILE_X; k += TILE_THREADS) {
-        const int hy = k / TILE_X, hx = k - hy * TILE_X;
-        const int x = x0 + hx, y = y0 + hy;
-        if (x >= w || y >= h) continue;
-        const float* col = s_h + hx * C;
+    static_assert(TILE_X * (TILE_Y / TILE_RUN) == TILE_THREADS, "one vertical run per lane");
+    const int hx = tid % TILE_X, ry0 = (tid / TILE_X) * TILE_RUN;
+    const int x = x0 + hx, ys = y0 + ry0;
+    if (x >= w || ys >= h) return;
+    const int nv = min(R + 1, h);
+    const int vint = h > 2 * (R + 1) ? h - 2 * (R + 1) : 0;
+    const float* col = s_in + (R + hx) * C;
+    if (ys >= nv && ys + TILE_RUN - 1 < nv + vint) {
+        float o[C][TILE_RUN];
+#pragma unroll
+        for (int c = 0; c < C; c++) {
+            float win[TILE_RUN + 2 * R];
+#pragma unroll
+            for (int d = 0; d < TILE_RUN + 2 * R; d++) win[d] = col[(ry0 + d) * RW * C + c];
+            run_interior<R>(win, t, o[c]);
+        }
+#pragma unroll
+        for (int m = 0; m < TILE_RUN; m++) {
+            float* p = out + ((long)(ys + m) * w + x) * C;
+#pragma unroll
+            for (int c = 0; c < C; c++) p[c] = o[c][m];
+        }
+        return;
+    }
+#pragma unroll
+    for (int m = 0; m < TILE_RUN; m++) {
+        if (ys + m >= h) break;
 #pragma unroll
         for (int c = 0; c < C; c++)
-            out[((long)y * w + x) * C + c] = line_elem<R, true, (C > 1)>(col + c, TILE_X * C, y0 - R, h, y, t);
+            out[((long)(ys + m) * w + x) * C + c] = line_elem<R, true, MULTI>(col + c, RW * C, y0 - R, h, ys + m, t);
     }
 }
 
@@ -893,15 +988,17 @@ static void launch_solve_range(hipStream_t s, const SolveJob* d_jobs, const std:
                 if (l > 0) Pn += (double)j.w[l - 1] * j.h[l - 1];
             }
         dim3 blk(256), grd((mw + 255) / 256, mh, nj);
-        dim3 tgrid((mw + TILE_X - 1) / TILE_X, (mh + TILE_Y - 1) / TILE_Y, nj), tblk(TILE_THREADS);
+        const int tx = (mw + TILE_X - 1) / TILE_X, ty = (mh + TILE_Y - 1) / TILE_Y;
+        const long ntile = (long)tx * ty * nj;
+        dim3 tgrid((unsigned)(8 * ((ntile + 7) / 8))), tblk(TILE_THREADS);
         // grad + tensor, blurred r = 7: X 4 B in, grad 8 + tensor 12 B out
-        TWX_LAUNCH(s, F_GRAD_TENSOR, 24.0 * P, (k_tile_gauss<3, 7, P_TENSOR><<<tgrid, tblk, 0, s>>>(d_jobs, l, 0, 0, g7)));
+        TWX_LAUNCH(s, F_GRAD_TENSOR, 24.0 * P, (k_tile_gauss<3, 7, P_TENSOR><<<tgrid, tblk, 0, s>>>(d_jobs, l, 0, 0, g7, tx, ty, nj)));
         for (int it = 0; it < 3; it++) {
             const int src = it & 1, dst = src ^ 1;
             // calc_b + B blur: flow 8 + X 4 + Y 4 + grad 8 in, B 8 out
-            TWX_LAUNCH(s, F_CALC_B, 32.0 * P, (k_tile_gauss<2, 7, P_CALCB><<<tgrid, tblk, 0, s>>>(d_jobs, l, src, 0, g7)));
+            TWX_LAUNCH(s, F_CALC_B, 32.0 * P, (k_tile_gauss<2, 7, P_CALCB><<<tgrid, tblk, 0, s>>>(d_jobs, l, src, 0, g7, tx, ty, nj)));
             // new_disp + flow blur: tensor 12 + B 8 + flow 8 in, flow 8 out
-            TWX_LAUNCH(s, F_NEW_DISP, 36.0 * P, (k_tile_gauss<2, 5, P_NEWDISP><<<tgrid, tblk, 0, s>>>(d_jobs, l, src, dst, g5)));
+            TWX_LAUNCH(s, F_NEW_DISP, 36.0 * P, (k_tile_gauss<2, 5, P_NEWDISP><<<tgrid, tblk, 0, s>>>(d_jobs, l, src, dst, g5, tx, ty, nj)));
             if (it == 1 && l == 0 && !skip_refine) bm(l, mw, mh, P, 0);
         }
         if (l != 0 && !skip_refine) bm(l, mw, mh, P, 1);
