@@ -817,6 +817,71 @@ __device__ __forceinline__ bool bm_step(const float* __restrict__ a, const float
     return c1 < c0;
 }
 
+// Block::cost without branches (the same arithmetic as bm_cost when `ok` comes back true): the
+// two lanes of a block-matching chain run it on different targets in one instruction stream.
+template <bool T>
+__device__ __forceinline__ float bm_cost_nb(const float* __restrict__ b, int w, int h, const float p[9], float x2, float y2,
+                                            bool& ok) {
+    const long SX = T ? h : 1, SY = T ? 1 : w;
+    const float wm2 = (float)(w - 2), hm2 = (float)(h - 2);
+    ok = !(x2 > wm2 || y2 > hm2);
+    int32_t ix, iy;
+    float fx, fy;
+    if (x2 == wm2) { ix = cvtt(wm2 - 1.0f); fx = 1.0f; }
+    else { ix = cvtt(x2); fx = x2 - (float)ix; }
+    if (y2 == hm2) { iy = cvtt(hm2 - 1.0f); fy = 1.0f; }
+    else { iy = cvtt(y2); fy = y2 - (float)iy; }
+    ok = ok && !(ix < 1 || iy < 1);
+    if (!ok) { ix = 1; iy = 1; }  // an in-bounds address; the value is not used
+    const float w11 = fy * fx;
+    const float w10 = fx - w11;
+    const float w01 = fy - w11;
+    const float w00 = ((1.0f - fx) - fy) + w11;
+    const float* base = b + ((long)iy - 1) * SY + ((long)ix - 1) * SX;
+    float sq[9];
+#pragma unroll
+    for (int bb = 0; bb < 3; bb++) {
+#pragma unroll
+        for (int aa = 0; aa < 3; aa++) {
+            const float* t0 = base + bb * SY + aa * SX;
+            const float* t1 = t0 + SY;
+            float q = ((w00 * t0[0] + w10 * t0[SX]) + w01 * t1[0]) + w11 * t1[SX];
+            float d = p[bb * 3 + aa] - q;
+            sq[bb * 3 + aa] = d * d;
+        }
+    }
+    float s = sq[0] + sq[1];
+#pragma unroll
+    for (int k = 2; k < 8; k++) s = s + sq[k];
+    return sq[8] + s;
+}
+
+// One step of a chain on a lane pair (Block::step, flow.rs:528-544): lane role 0 computes the
+// cost of the pixel's own vector (after the step's range test), role 1 the cost of the incoming
+// vector; both take the decision, c1 < c0 with both costs present.
+template <bool T>
+__device__ __forceinline__ bool bm_step_pair(const float* __restrict__ a, const float* __restrict__ b, int w, int h, int x,
+                                             int y, float cu, float cv, float nu, float nv, int role) {
+    const long SX = T ? h : 1, SY = T ? 1 : w;
+    const float tx = (float)x - cu, ty = (float)y - cv;
+    const bool in_range = 0.0f <= tx && tx <= (float)(w - 1) && 0.0f <= ty && ty <= (float)(h - 1);
+    float p[9];
+    const float* pa = a + ((long)y - 1) * SY + ((long)x - 1) * SX;
+#pragma unroll
+    for (int r = 0; r < 3; r++) {
+        p[r * 3] = pa[r * SY];
+        p[r * 3 + 1] = pa[r * SY + SX];
+        p[r * 3 + 2] = pa[r * SY + 2 * SX];
+    }
+    bool ok;
+    const float c = bm_cost_nb<T>(b, w, h, p, role ? (float)x - nu : tx, role ? (float)y - nv : ty, ok);
+    const float c_o = __shfl_xor(c, 1);
+    const int ok_o = __shfl_xor(ok ? 1 : 0, 1);
+    const float c0 = role ? c_o : c, c1 = role ? c : c_o;
+    const bool ok0 = role ? ok_o != 0 : ok, ok1 = role ? ok : ok_o != 0;
+    return in_range && ok0 && ok1 && c1 < c0;
+}
+
 // Transposed copies for the row sweeps: the flow (buffer `buf`) into tmp, X and Y into tb's two
 // planes; OUT copies tmp back into the flow. With them the lanes of a wave (one row each) read
 // consecutive addresses, as the column sweeps do on the plain layout.
@@ -887,20 +952,26 @@ __global__ void k_bm_rows(const SolveJob* __restrict__ jobs, int level) {
     if (level > J.start) return;
     const int w = J.w[level], h = J.h[level];
     if (w < 3 || h < 3) return;
-    int y = 1 + blockIdx.x * blockDim.x + threadIdx.x;
-    if (y > h - 2) return;
+    // a lane pair per chain (bm_step_pair); a pair past the last row still joins the shuffles
+    const int role = threadIdx.x & 1;
+    const int y = 1 + blockIdx.x * (blockDim.x / 2) + (threadIdx.x >> 1);
+    if (1 + blockIdx.x * (blockDim.x / 2) > h - 2) return;  // the whole block is past the rows
+    const bool live = y <= h - 2;
+    const int yy = live ? y : h - 2;  // rows past the end mirror the last row and never write
     float* ft = J.tmp;
     const float* A = J.tb;
     const float* B = J.tb + (long)w * h;
     int x = DIR > 0 ? 1 : w - 2;
-    long ni = ((long)(x - DIR) * h + y) * 2;
+    long ni = ((long)(x - DIR) * h + yy) * 2;
     float nu = ft[ni], nv = ft[ni + 1];
     for (int k = 0; k < w - 2; k++, x += DIR) {
-        long ci = ((long)x * h + y) * 2;
+        long ci = ((long)x * h + yy) * 2;
         float cu = ft[ci], cv = ft[ci + 1];
-        if (bm_step<true>(A, B, w, h, x, y, cu, cv, nu, nv)) {
-            ft[ci] = nu;
-            ft[ci + 1] = nv;
+        if (bm_step_pair<true>(A, B, w, h, x, yy, cu, cv, nu, nv, role)) {
+            if (live && role == 0) {
+                ft[ci] = nu;
+                ft[ci + 1] = nv;
+            }
         } else {
             nu = cu;
             nv = cv;
@@ -916,20 +987,25 @@ __global__ void k_bm_cols(const SolveJob* __restrict__ jobs, int level, int buf)
     if (level > J.start) return;
     const int w = J.w[level], h = J.h[level];
     if (w < 3 || h < 3) return;
-    int x = 1 + blockIdx.x * blockDim.x + threadIdx.x;
-    if (x > w - 2) return;
+    const int role = threadIdx.x & 1;
+    const int x = 1 + blockIdx.x * (blockDim.x / 2) + (threadIdx.x >> 1);
+    if (1 + blockIdx.x * (blockDim.x / 2) > w - 2) return;
+    const bool live = x <= w - 2;
+    const int xx = live ? x : w - 2;
     float* flow = J.flow[buf];
     const float* A = J.X[level];
     const float* B = J.Y[level];
     int y = DIR > 0 ? 1 : h - 2;
-    long ni = ((long)(y - DIR) * w + x) * 2;
+    long ni = ((long)(y - DIR) * w + xx) * 2;
     float nu = flow[ni], nv = flow[ni + 1];
     for (int k = 0; k < h - 2; k++, y += DIR) {
-        long ci = ((long)y * w + x) * 2;
+        long ci = ((long)y * w + xx) * 2;
         float cu = flow[ci], cv = flow[ci + 1];
-        if (bm_step<false>(A, B, w, h, x, y, cu, cv, nu, nv)) {
-            flow[ci] = nu;
-            flow[ci + 1] = nv;
+        if (bm_step_pair<false>(A, B, w, h, xx, y, cu, cv, nu, nv, role)) {
+            if (live && role == 0) {
+                flow[ci] = nu;
+                flow[ci + 1] = nv;
+            }
         } else {
             nu = cu;
             nv = cv;
@@ -968,7 +1044,8 @@ static void launch_solve_range(hipStream_t s, const SolveJob* d_jobs, const std:
     // block matching on the flow in buffer `buf`
     auto bm = [&](int l, int mw, int mh, double P, int buf) {
         dim3 tb(256), tg((mw + 31) / 32, (mh + 31) / 32, nj);
-        dim3 cb(64), rg((mh + 63) / 64, 1, nj), cg((mw + 63) / 64, 1, nj);
+        // a lane pair per chain: 32 chains per 64-lane block
+        dim3 cb(64), rg((mh + 31) / 32, 1, nj), cg((mw + 31) / 32, 1, nj);
         // transposes 24 + 16 B/px, sweeps: A 4 + B 4 + flow read and write 16 B/px each
         TWX_LAUNCH(s, F_BM_TRANSPOSE, 24.0 * P, k_bm_transpose<false><<<tg, tb, 0, s>>>(d_jobs, l, buf));
         TWX_LAUNCH(s, F_BM_ROWS, 24.0 * P, k_bm_rows<1><<<rg, cb, 0, s>>>(d_jobs, l));
