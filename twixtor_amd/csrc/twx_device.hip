@@ -820,7 +820,7 @@ __device__ __forceinline__ bool bm_step(const float* __restrict__ a, const float
 // Block::cost without branches (the same arithmetic as bm_cost when `ok` comes back true): the
 // two lanes of a block-matching chain run it on different targets in one instruction stream.
 template <bool T>
-__device__ __forceinline__ float bm_cost_nb(const float* __restrict__ b, int w, int h, const float p[9], float x2, float y2,
+__device__ __forceinline__ float bm_cost_nb(const gfloat* __restrict__ b, int w, int h, const float p[9], float x2, float y2,
                                             bool& ok) {
     const long SX = T ? h : 1, SY = T ? 1 : w;
     const float wm2 = (float)(w - 2), hm2 = (float)(h - 2);
@@ -837,14 +837,14 @@ __device__ __forceinline__ float bm_cost_nb(const float* __restrict__ b, int w, 
     const float w10 = fx - w11;
     const float w01 = fy - w11;
     const float w00 = ((1.0f - fx) - fy) + w11;
-    const float* base = b + ((long)iy - 1) * SY + ((long)ix - 1) * SX;
+    const gfloat* base = b + ((long)iy - 1) * SY + ((long)ix - 1) * SX;
     float sq[9];
 #pragma unroll
     for (int bb = 0; bb < 3; bb++) {
 #pragma unroll
         for (int aa = 0; aa < 3; aa++) {
-            const float* t0 = base + bb * SY + aa * SX;
-            const float* t1 = t0 + SY;
+            const gfloat* t0 = base + bb * SY + aa * SX;
+            const gfloat* t1 = t0 + SY;
             float q = ((w00 * t0[0] + w10 * t0[SX]) + w01 * t1[0]) + w11 * t1[SX];
             float d = p[bb * 3 + aa] - q;
             sq[bb * 3 + aa] = d * d;
@@ -859,20 +859,24 @@ __device__ __forceinline__ float bm_cost_nb(const float* __restrict__ b, int w, 
 // One step of a chain on a lane pair (Block::step, flow.rs:528-544): lane role 0 computes the
 // cost of the pixel's own vector (after the step's range test), role 1 the cost of the incoming
 // vector; both take the decision, c1 < c0 with both costs present.
+// The A patch of pixel (x, y): the 3x3 samples around it (Block::patch, flow.rs:459-470).
 template <bool T>
-__device__ __forceinline__ bool bm_step_pair(const float* __restrict__ a, const float* __restrict__ b, int w, int h, int x,
-                                             int y, float cu, float cv, float nu, float nv, int role) {
+__device__ __forceinline__ void bm_patch(const gfloat* __restrict__ a, int w, int h, int x, int y, float p[9]) {
     const long SX = T ? h : 1, SY = T ? 1 : w;
-    const float tx = (float)x - cu, ty = (float)y - cv;
-    const bool in_range = 0.0f <= tx && tx <= (float)(w - 1) && 0.0f <= ty && ty <= (float)(h - 1);
-    float p[9];
-    const float* pa = a + ((long)y - 1) * SY + ((long)x - 1) * SX;
+    const gfloat* pa = a + ((long)y - 1) * SY + ((long)x - 1) * SX;
 #pragma unroll
     for (int r = 0; r < 3; r++) {
         p[r * 3] = pa[r * SY];
         p[r * 3 + 1] = pa[r * SY + SX];
         p[r * 3 + 2] = pa[r * SY + 2 * SX];
     }
+}
+
+template <bool T>
+__device__ __forceinline__ bool bm_step_pair(const gfloat* __restrict__ b, int w, int h, int x, int y, const float p[9],
+                                             float cu, float cv, float nu, float nv, int role) {
+    const float tx = (float)x - cu, ty = (float)y - cv;
+    const bool in_range = 0.0f <= tx && tx <= (float)(w - 1) && 0.0f <= ty && ty <= (float)(h - 1);
     bool ok;
     const float c = bm_cost_nb<T>(b, w, h, p, role ? (float)x - nu : tx, role ? (float)y - nv : ty, ok);
     const float c_o = __shfl_xor(c, 1);
@@ -958,16 +962,25 @@ __global__ void k_bm_rows(const SolveJob* __restrict__ jobs, int level) {
     if (1 + blockIdx.x * (blockDim.x / 2) > h - 2) return;  // the whole block is past the rows
     const bool live = y <= h - 2;
     const int yy = live ? y : h - 2;  // rows past the end mirror the last row and never write
-    float* ft = J.tmp;
-    const float* A = J.tb;
-    const float* B = J.tb + (long)w * h;
+    gfloat_rw* ft = (gfloat_rw*)J.tmp;
+    const gfloat* A = (const gfloat*)J.tb;
+    const gfloat* B = (const gfloat*)(J.tb + (long)w * h);
     int x = DIR > 0 ? 1 : w - 2;
     long ni = ((long)(x - DIR) * h + yy) * 2;
     float nu = ft[ni], nv = ft[ni + 1];
+    // the pixel's own vector and A patch do not depend on the chain: loaded one step ahead (the
+    // step's store goes to its own pixel, never to the next one)
+    float cu = ft[((long)x * h + yy) * 2], cv = ft[((long)x * h + yy) * 2 + 1];
+    float p[9];
+    bm_patch<true>(A, w, h, x, yy, p);
     for (int k = 0; k < w - 2; k++, x += DIR) {
-        long ci = ((long)x * h + yy) * 2;
-        float cu = ft[ci], cv = ft[ci + 1];
-        if (bm_step_pair<true>(A, B, w, h, x, yy, cu, cv, nu, nv, role)) {
+        const long ci = ((long)x * h + yy) * 2;
+        const bool more = k + 1 < w - 2;
+        const int xn = more ? x + DIR : x;
+        const float cu_n = ft[((long)xn * h + yy) * 2], cv_n = ft[((long)xn * h + yy) * 2 + 1];
+        float pn[9];
+        bm_patch<true>(A, w, h, xn, yy, pn);
+        if (bm_step_pair<true>(B, w, h, x, yy, p, cu, cv, nu, nv, role)) {
             if (live && role == 0) {
                 ft[ci] = nu;
                 ft[ci + 1] = nv;
@@ -976,6 +989,10 @@ __global__ void k_bm_rows(const SolveJob* __restrict__ jobs, int level) {
             nu = cu;
             nv = cv;
         }
+        cu = cu_n;
+        cv = cv_n;
+#pragma unroll
+        for (int i = 0; i < 9; i++) p[i] = pn[i];
     }
 }
 
@@ -992,16 +1009,23 @@ __global__ void k_bm_cols(const SolveJob* __restrict__ jobs, int level, int buf)
     if (1 + blockIdx.x * (blockDim.x / 2) > w - 2) return;
     const bool live = x <= w - 2;
     const int xx = live ? x : w - 2;
-    float* flow = J.flow[buf];
-    const float* A = J.X[level];
-    const float* B = J.Y[level];
+    gfloat_rw* flow = (gfloat_rw*)J.flow[buf];
+    const gfloat* A = (const gfloat*)J.X[level];
+    const gfloat* B = (const gfloat*)J.Y[level];
     int y = DIR > 0 ? 1 : h - 2;
     long ni = ((long)(y - DIR) * w + xx) * 2;
     float nu = flow[ni], nv = flow[ni + 1];
+    float cu = flow[((long)y * w + xx) * 2], cv = flow[((long)y * w + xx) * 2 + 1];
+    float p[9];
+    bm_patch<false>(A, w, h, xx, y, p);
     for (int k = 0; k < h - 2; k++, y += DIR) {
-        long ci = ((long)y * w + xx) * 2;
-        float cu = flow[ci], cv = flow[ci + 1];
-        if (bm_step_pair<false>(A, B, w, h, xx, y, cu, cv, nu, nv, role)) {
+        const long ci = ((long)y * w + xx) * 2;
+        const bool more = k + 1 < h - 2;
+        const int yn = more ? y + DIR : y;
+        const float cu_n = flow[((long)yn * w + xx) * 2], cv_n = flow[((long)yn * w + xx) * 2 + 1];
+        float pn[9];
+        bm_patch<false>(A, w, h, xx, yn, pn);
+        if (bm_step_pair<false>(B, w, h, xx, y, p, cu, cv, nu, nv, role)) {
             if (live && role == 0) {
                 flow[ci] = nu;
                 flow[ci + 1] = nv;
@@ -1010,6 +1034,10 @@ __global__ void k_bm_cols(const SolveJob* __restrict__ jobs, int level, int buf)
             nu = cu;
             nv = cv;
         }
+        cu = cu_n;
+        cv = cv_n;
+#pragma unroll
+        for (int i = 0; i < 9; i++) p[i] = pn[i];
     }
 }
 
