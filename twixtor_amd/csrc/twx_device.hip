@@ -461,14 +461,18 @@ template <int MODE, int C>
 __device__ __forceinline__ void produce(const SolveJob& J, int level, int src, int w, int h, int x, int y, bool own,
                                         float v[C]) {
     const long i = (long)y * w + x;
+    // every load below is unconditional (in-bounds addresses, results selected), so that the
+    // producer loop can keep several pixels' loads in flight
     if (MODE == P_TENSOR) {
         const float* img = J.X[level];
-        float c = img[i];
+        const float c = img[i];
+        const float vl = img[i - (x >= 1 ? 1 : 0)], vu = img[i - (y >= 1 ? w : 0)];
+        const float vr = img[i + (x + 1 <= w - 1 ? 1 : 0)], vd = img[i + (y + 1 <= h - 1 ? w : 0)];
         float nx = 0.0f, dx = 0.0f, ny = 0.0f, dy = 0.0f;
-        if (x >= 1) { nx = (c - img[i - 1]) + 0.0f; dx = 1.0f; }
-        if (y >= 1) { ny = (c - img[i - w]) + 0.0f; dy = 1.0f; }
-        if (x + 1 <= w - 1) { nx = nx + (img[i + 1] - c); dx = dx + 1.0f; }
-        if (y + 1 <= h - 1) { ny = ny + (img[i + w] - c); dy = dy + 1.0f; }
+        if (x >= 1) { nx = (c - vl) + 0.0f; dx = 1.0f; }
+        if (y >= 1) { ny = (c - vu) + 0.0f; dy = 1.0f; }
+        if (x + 1 <= w - 1) { nx = nx + (vr - c); dx = dx + 1.0f; }
+        if (y + 1 <= h - 1) { ny = ny + (vd - c); dy = dy + 1.0f; }
         float gx = nx / dx, gy = ny / dy;
         if (own) {
             J.grad[i * 2] = gx;
@@ -479,17 +483,14 @@ __device__ __forceinline__ void produce(const SolveJob& J, int level, int src, i
         v[C - 1] = gy * gy;
     } else if (MODE == P_CALCB) {
         const float* flow = J.flow[src];
-        float sx = (float)x - flow[i * 2];
-        float sy = (float)y - flow[i * 2 + 1];
-        float bx = 0.0f, by = 0.0f;
-        if (!(sx < 0.0f || (float)w - 1.0f < sx || sy < 0.0f || (float)h - 1.0f < sy)) {
-            float s = bilinear1(J.Y[level], w, h, sx, sy);
-            float r = s - J.X[level][i];
-            bx = r * J.grad[i * 2];
-            by = r * J.grad[i * 2 + 1];
-        }
-        v[0] = bx;
-        v[C - 1] = by;
+        const float sx = (float)x - flow[i * 2];
+        const float sy = (float)y - flow[i * 2 + 1];
+        const bool in = !(sx < 0.0f || (float)w - 1.0f < sx || sy < 0.0f || (float)h - 1.0f < sy);
+        const float s = bilinear1(J.Y[level], w, h, in ? sx : 0.0f, in ? sy : 0.0f);
+        const float r = s - J.X[level][i];
+        const float gx = J.grad[i * 2], gy = J.grad[i * 2 + 1];
+        v[0] = in ? r * gx : 0.0f;
+        v[C - 1] = in ? r * gy : 0.0f;
     } else {
         const float* flow = J.flow[src];
         float u = flow[i * 2], vv = flow[i * 2 + 1];
@@ -556,15 +557,29 @@ __global__ __launch_bounds__(TILE_THREADS) void k_tile_gauss(const SolveJob* __r
     constexpr bool MULTI = C > 1;
     __shared__ float s_in[RH * RW * C];
     const int tid = threadIdx.x;
-    for (int k = tid; k < RH * RW; k += TILE_THREADS) {
-        const int ry = k / RW, rx = k - ry * RW;
-        const int x = x0 - R + rx, y = y0 - R + ry;
-        if (x < 0 || x >= w || y < 0 || y >= h) continue;
-        const bool own = rx >= R && rx < R + TILE_X && ry >= R && ry < R + TILE_Y;
-        float v[C];
-        produce<MODE, C>(J, level, src, w, h, x, y, own, v);
+    // the region, PU pixels per lane at a time with their loads in flight together; a pixel
+    // outside the image is computed at a clamped position and not stored
+    constexpr int NPROD = (RH * RW + TILE_THREADS - 1) / TILE_THREADS, PU = MODE == P_NEWDISP ? 1 : 4;
+#pragma unroll 1
+    for (int k0 = 0; k0 < NPROD; k0 += PU) {
+        float v[PU][C];
+        int slot[PU];
 #pragma unroll
-        for (int c = 0; c < C; c++) s_in[(ry * RW + rx) * C + c] = v[c];
+        for (int u = 0; u < PU; u++) {
+            const int k = tid + (k0 + u) * TILE_THREADS;
+            const int ry = k / RW, rx = k - ry * RW;
+            const int x = x0 - R + rx, y = y0 - R + ry;
+            const bool keep = k0 + u < NPROD && k < RH * RW && x >= 0 && x < w && y >= 0 && y < h;
+            const bool own = keep && rx >= R && rx < R + TILE_X && ry >= R && ry < R + TILE_Y;
+            produce<MODE, C>(J, level, src, w, h, min(max(x, 0), w - 1), min(max(y, 0), h - 1), own, v[u]);
+            slot[u] = keep ? ry * RW + rx : -1;
+        }
+#pragma unroll
+        for (int u = 0; u < PU; u++)
+            if (slot[u] >= 0) {
+#pragma unroll
+                for (int c = 0; c < C; c++) s_in[slot[u] * C + c] = v[u][c];
+            }
     }
     __syncthreads();
     // horizontal pass: item = (region row, run of TILE_RUN columns); results held until every
