@@ -282,11 +282,11 @@ void build_pyramids(hipStream_t s, DevPool& pool, const std::vector<PyrIn>& in) 
     if (in.empty()) return;
     const Taps bin = binomial_taps();
     const size_t chunk = 64;
-    // per frame scratch: lum (W*H), red (W/2*H/2), tmp (W*H), blur (w0*h0); one set for the
-    // largest chunk, reused by every chunk in stream order
-    auto frame_scratch = [](const PyrIn& p) {
-        auto r = [](size_t v) { return (v + 63) & ~(size_t)63; };
-        return r((size_t)p.W * p.H) * 2 + r((size_t)(p.W / 2 + 1) * (p.H / 2 + 1)) + r((size_t)p.geo.w0 * p.geo.h0);
+    // per frame scratch: two level buffers A (W*H) and B (half size), the blurred plane T (W*H);
+    // one set for the largest chunk, reused by every chunk in stream order
+    auto r64 = [](size_t v) { return (v + 63) & ~(size_t)63; };
+    auto frame_scratch = [&](const PyrIn& p) {
+        return r64((size_t)p.W * p.H) * 2 + r64((size_t)(p.W / 2 + 1) * (p.H / 2 + 1));
     };
     size_t max_floats = 0;
     for (size_t c0 = 0; c0 < in.size(); c0 += chunk) {
@@ -298,80 +298,68 @@ void build_pyramids(hipStream_t s, DevPool& pool, const std::vector<PyrIn>& in) 
     for (size_t c0 = 0; c0 < in.size(); c0 += chunk) {
         size_t c1 = std::min(in.size(), c0 + chunk);
         int n = (int)(c1 - c0);
-        std::vector<float*> lum(n), red(n), tmp(n), blur(n);
-        int maxW = 0, maxH = 0, maxR = 0, maxL = 0;
+        std::vector<float*> bufA(n), bufB(n), blur(n);
+        int maxR = 0, maxL = 0;
         float* sp = sbase;
         for (int i = 0; i < n; i++) {
             const PyrIn& p = in[c0 + i];
-            auto r = [](size_t v) { return (v + 63) & ~(size_t)63; };
-            lum[i] = sp;
-            sp += r((size_t)p.W * p.H);
-            tmp[i] = sp;
-            sp += r((size_t)p.W * p.H);
-            red[i] = sp;
-            sp += r((size_t)(p.W / 2 + 1) * (p.H / 2 + 1));
+            bufA[i] = sp;
+            sp += r64((size_t)p.W * p.H);
             blur[i] = sp;
-            sp += r((size_t)p.geo.w0 * p.geo.h0);
-            maxW = std::max(maxW, p.W);
-            maxH = std::max(maxH, p.H);
+            sp += r64((size_t)p.W * p.H);
+            bufB[i] = sp;
+            sp += r64((size_t)(p.W / 2 + 1) * (p.H / 2 + 1));
             maxR = std::max(maxR, p.geo.reduction);
             maxL = std::max(maxL, p.geo.levels);
         }
         // every step is one launch over the chunk; a frame without the step gets an empty op
         std::vector<std::vector<PlaneOp>> steps;
-        std::vector<int> kinds;  // 0 luma, 1 hpass, 2 vpass, 3 resample, 4 subtract
+        std::vector<int> kinds;  // 0 luma, 1 blur5 (+ band), 3 resample
         auto level_ptr = [&](int i, int l) {
             const PyrIn& p = in[c0 + i];
             float* base = p.levels;
             for (int k = 0; k < l; k++) base += (size_t)p.geo.w[k] * p.geo.h[k];
             return base;
         };
-        auto empty = PlaneOp{nullptr, nullptr, 0, 0, 0, 0, 0, 0};
-        // luma into lum (or straight into level 0 without a reduction)
+        // the buffer holding the plane of step k (reductions, then levels): A, B, A, ...
+        auto cur = [&](int i, int k) { return (k % 2 == 0) ? bufA[i] : bufB[i]; };
+        const PlaneOp empty{};
         {
             std::vector<PlaneOp> ops(n, empty);
             for (int i = 0; i < n; i++) {
                 const PyrIn& p = in[c0 + i];
-                ops[i] = PlaneOp{p.img, p.geo.reduction ? lum[i] : level_ptr(i, 0), p.W, p.H, p.W, p.H, p.fmt, 0};
+                ops[i] = PlaneOp{p.img, bufA[i], p.W, p.H, p.W, p.H, p.fmt, 0, nullptr};
             }
             steps.push_back(ops);
             kinds.push_back(0);
         }
         // reduction: R x (blur5, resample to the halved size)
         for (int r = 0; r < maxR; r++) {
-            std::vector<PlaneOp> hp(n, empty), vp(n, empty), rs(n, empty);
+            std::vector<PlaneOp> bl(n, empty), rs(n, empty);
             for (int i = 0; i < n; i++) {
                 const PyrIn& p = in[c0 + i];
                 if (r >= p.geo.reduction) continue;
                 int cw = p.W, ch = p.H;
                 for (int k = 0; k < r; k++) { cw /= 2; ch /= 2; }
-                float* cur = (r % 2 == 0) ? lum[i] : red[i];
-                float* nxt = (r + 1 == p.geo.reduction) ? level_ptr(i, 0) : ((r % 2 == 0) ? red[i] : lum[i]);
-                hp[i] = PlaneOp{cur, tmp[i], cw, ch, cw, ch, 0, 0};
-                vp[i] = PlaneOp{tmp[i], cur, cw, ch, cw, ch, 0, 0};
-                rs[i] = PlaneOp{cur, nxt, cw, ch, cw / 2, ch / 2, 0, 0};
+                bl[i] = PlaneOp{cur(i, r), blur[i], cw, ch, cw, ch, 0, 0, nullptr};
+                rs[i] = PlaneOp{blur[i], cur(i, r + 1), cw, ch, cw / 2, ch / 2, 0, 0, nullptr};
             }
-            steps.push_back(hp); kinds.push_back(1);
-            steps.push_back(vp); kinds.push_back(2);
+            steps.push_back(bl); kinds.push_back(1);
             steps.push_back(rs); kinds.push_back(3);
         }
-        // levels: blur = blur5(orig); next = resample(blur); band = orig - blur
+        // levels: blur = blur5(orig), band = orig - blur into the pyramid, next = resample(blur)
         for (int l = 0; l < maxL; l++) {
-            std::vector<PlaneOp> hp(n, empty), vp(n, empty), rs(n, empty), sb(n, empty);
+            std::vector<PlaneOp> bl(n, empty), rs(n, empty);
             for (int i = 0; i < n; i++) {
                 const PyrIn& p = in[c0 + i];
                 if (l >= p.geo.levels) continue;
+                const int k = p.geo.reduction + l;
                 int w = p.geo.w[l], h = p.geo.h[l];
-                float* orig = level_ptr(i, l);
-                hp[i] = PlaneOp{orig, tmp[i], w, h, w, h, 0, 0};
-                vp[i] = PlaneOp{tmp[i], blur[i], w, h, w, h, 0, 0};
-                if (l + 1 < p.geo.levels) rs[i] = PlaneOp{blur[i], level_ptr(i, l + 1), w, h, p.geo.w[l + 1], p.geo.h[l + 1], 0, 0};
-                sb[i] = PlaneOp{blur[i], orig, w, h, w, h, 0, 0};
+                bl[i] = PlaneOp{cur(i, k), blur[i], w, h, w, h, 0, 0, level_ptr(i, l)};
+                if (l + 1 < p.geo.levels) rs[i] = PlaneOp{blur[i], cur(i, k + 1), w, h, p.geo.w[l + 1], p.geo.h[l + 1], 0, 0, nullptr};
             }
-            steps.push_back(hp); kinds.push_back(1);
-            steps.push_back(vp); kinds.push_back(2);
+            steps.push_back(bl); kinds.push_back(1);
             steps.push_back(rs); kinds.push_back(3);
-            steps.push_back(sb); kinds.push_back(4);
         }
         std::vector<PlaneOp> flat;
         for (auto& st : steps) flat.insert(flat.end(), st.begin(), st.end());
@@ -386,10 +374,8 @@ void build_pyramids(hipStream_t s, DevPool& pool, const std::vector<PyrIn>& in) 
             if (mw == 0 || mh == 0) continue;
             switch (kinds[k]) {
                 case 0: launch_luma(s, dk, steps[k], mw, mh); break;
-                case 1: launch_hpass1(s, dk, steps[k], mw, mh, bin); break;
-                case 2: launch_vpass1(s, dk, steps[k], mw, mh, bin); break;
+                case 1: launch_blur5_tile(s, dk, steps[k], mw, mh, bin); break;
                 case 3: launch_resample(s, dk, steps[k], mw, mh); break;
-                case 4: launch_subtract(s, dk, steps[k], mw, mh); break;
             }
         }
     }

@@ -27,6 +27,7 @@ struct PlaneOp {
     int sw, sh, dw, dh;
     int fmt;  // luma: the engine image format
     int pad;
+    float* aux;  // k_blur5_tile: where the band src - blur goes (null: none)
 };
 
 // One flow solve X -> Y over a pyramid pair (flow.rs:1258-1307). Device resident table entry;
@@ -43,7 +44,7 @@ struct SolveJob {
     float* grad;           // 2 channels
     float* ta;             // structure tensor, 3 channels
     float* tb;             // right-hand side B, 2 channels
-    float* tmp;            // horizontal-pass scratch, up to 3 channels
+    float* tmp;            // block-matching scratch: the transposed flow (2 channels)
     int* state;            // [0] range check still active, [1] any pixel in range, [2] NaN seen
     int max_w, max_h;
 };
@@ -113,7 +114,7 @@ struct FillPlane {
 // Per-kernel-family timing with HIP events on the launching stream, and the algorithmic bytes of
 // each launch (the step-boundary model: distinct inputs read once, outputs written once).
 enum ProfFam {
-    F_INGEST, F_LUMA, F_PYR_HPASS, F_PYR_VPASS, F_PYR_RESAMPLE, F_PYR_SUBTRACT, F_GRAD_TENSOR, F_BM_TRANSPOSE,
+    F_INGEST, F_LUMA, F_PYR_BLUR, F_PYR_RESAMPLE, F_GRAD_TENSOR, F_BM_TRANSPOSE,
     F_SOLVE_INIT, F_CALC_B, F_NEW_DISP, F_BM_ROWS, F_BM_COLS, F_RANGE, F_UPSAMPLE, F_NEGATE, F_FRAMES,
     F_WB_MASK, F_WB_SHARPEN, F_FILL_PULL, F_FILL_PUSH, F_FILL_IO, F_NFAM
 };
@@ -167,10 +168,8 @@ std::vector<std::pair<int, int>> size_buckets(const std::vector<double>& areas);
 void launch_ingest(hipStream_t s, const void* wire, void* engine, int W, int H, int fmt, bool reverse_rows);
 void launch_ingest_batch(hipStream_t s, const IngestOp* d_ops, const std::vector<IngestOp>& ops);
 void launch_luma(hipStream_t s, const PlaneOp* d_ops, const std::vector<PlaneOp>& ops, int maxw, int maxh);
-void launch_hpass1(hipStream_t s, const PlaneOp* d_ops, const std::vector<PlaneOp>& ops, int maxw, int maxh, const Taps& t);
-void launch_vpass1(hipStream_t s, const PlaneOp* d_ops, const std::vector<PlaneOp>& ops, int maxw, int maxh, const Taps& t);
+void launch_blur5_tile(hipStream_t s, const PlaneOp* d_ops, const std::vector<PlaneOp>& ops, int maxw, int maxh, const Taps& t);
 void launch_resample(hipStream_t s, const PlaneOp* d_ops, const std::vector<PlaneOp>& ops, int maxw, int maxh);
-void launch_subtract(hipStream_t s, const PlaneOp* d_ops, const std::vector<PlaneOp>& ops, int maxw, int maxh);
 void launch_solve(hipStream_t s, const SolveJob* d_jobs, const std::vector<SolveJob>& jobs, bool skip_refine);
 void launch_frames(hipStream_t s, const FrameJob* d_jobs, const std::vector<FrameJob>& jobs);
 // The Weighted Blend jobs (kinds 3 and 6) after launch_frames: mask clean-up, sharpening, then

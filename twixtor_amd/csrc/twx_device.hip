@@ -150,7 +150,7 @@ void launch_ingest(hipStream_t s, const void* wire, void* engine, int W, int H, 
 
 // ---- profiler (host) ----
 
-const char* const PROF_NAMES[F_NFAM] = {"ingest", "luma", "pyr_hpass", "pyr_vpass", "pyr_resample", "pyr_subtract",
+const char* const PROF_NAMES[F_NFAM] = {"ingest", "luma", "pyr_blur5_band", "pyr_resample",
                                         "tensor_gauss", "bm_transpose", "solve_init", "calc_b_gauss", "new_disp_gauss",
                                         "bm_rows", "bm_cols", "range", "upsample", "negate", "frames",
                                         "wb_mask", "wb_sharpen_blend", "fill_pull", "fill_push", "fill_io"};
@@ -319,32 +319,6 @@ __device__ __forceinline__ float vpass_elem(const float* __restrict__ src, int w
     return border_elem<C, ABOVE>(col, row, h, y, t);
 }
 
-__global__ void k_hpass1(const PlaneOp* __restrict__ ops, Taps t) {
-    const PlaneOp op = ops[blockIdx.z];
-    int x = blockIdx.x * blockDim.x + threadIdx.x;
-    int y = blockIdx.y;
-    if (x >= op.sw || y >= op.sh) return;
-    const float* row = (const float*)op.src + (long)y * op.sw;
-    op.dst[(long)y * op.sw + x] = hpass_elem<1>(row, op.sw, x, 0, t);
-}
-
-__global__ void k_vpass1(const PlaneOp* __restrict__ ops, Taps t) {
-    const PlaneOp op = ops[blockIdx.z];
-    int x = blockIdx.x * blockDim.x + threadIdx.x;
-    int y = blockIdx.y;
-    if (x >= op.sw || y >= op.sh) return;
-    op.dst[(long)y * op.sw + x] = vpass_elem<1>((const float*)op.src, op.sw, op.sh, x, y, 0, t);
-}
-
-void launch_hpass1(hipStream_t s, const PlaneOp* d_ops, const std::vector<PlaneOp>& ops, int maxw, int maxh, const Taps& t) {
-    TWX_LAUNCH(s, F_PYR_HPASS, 8.0 * ops_px(ops, false),
-               k_hpass1<<<dim3((maxw + 255) / 256, maxh, (unsigned)ops.size()), dim3(256), 0, s>>>(d_ops, t));
-}
-void launch_vpass1(hipStream_t s, const PlaneOp* d_ops, const std::vector<PlaneOp>& ops, int maxw, int maxh, const Taps& t) {
-    TWX_LAUNCH(s, F_PYR_VPASS, 8.0 * ops_px(ops, false),
-               k_vpass1<<<dim3((maxw + 255) / 256, maxh, (unsigned)ops.size()), dim3(256), 0, s>>>(d_ops, t));
-}
-
 // -------------------------------------------------------------------------------------------
 // Samplers (flow/sample.rs:31-112)
 
@@ -382,23 +356,9 @@ __global__ void k_resample(const PlaneOp* __restrict__ ops) {
     op.dst[(long)y * op.dw + x] = bilinear1((const float*)op.src, op.sw, op.sh, (float)x * sx, fy);
 }
 
-// dst = dst - src (flow.rs:228-231, band = orig - blur5(orig))
-__global__ void k_subtract(const PlaneOp* __restrict__ ops) {
-    const PlaneOp op = ops[blockIdx.z];
-    int x = blockIdx.x * blockDim.x + threadIdx.x;
-    int y = blockIdx.y;
-    if (x >= op.sw || y >= op.sh) return;
-    long i = (long)y * op.sw + x;
-    op.dst[i] = op.dst[i] - ((const float*)op.src)[i];
-}
-
 void launch_resample(hipStream_t s, const PlaneOp* d_ops, const std::vector<PlaneOp>& ops, int maxw, int maxh) {
     TWX_LAUNCH(s, F_PYR_RESAMPLE, 4.0 * ops_px(ops, false) + 4.0 * ops_px(ops, true),
                k_resample<<<dim3((maxw + 255) / 256, maxh, (unsigned)ops.size()), dim3(256), 0, s>>>(d_ops));
-}
-void launch_subtract(hipStream_t s, const PlaneOp* d_ops, const std::vector<PlaneOp>& ops, int maxw, int maxh) {
-    TWX_LAUNCH(s, F_PYR_SUBTRACT, 12.0 * ops_px(ops, false),
-               k_subtract<<<dim3((maxw + 255) / 256, maxh, (unsigned)ops.size()), dim3(256), 0, s>>>(d_ops));
 }
 
 // -------------------------------------------------------------------------------------------
@@ -664,6 +624,106 @@ __global__ __launch_bounds__(TILE_THREADS) void k_tile_gauss(const SolveJob* __r
         for (int c = 0; c < C; c++)
             out[((long)(ys + m) * w + x) * C + c] = line_elem<R, true, MULTI>(col + c, RW * C, y0 - R, h, ys + m, t);
     }
+}
+
+// blur5 of a plane (conv.rs:598-629: the row pass, then the column pass with the border rows
+// above first; flow.rs:242-275 and prep.rs:252-259 run it on every pyramid level and reduction
+// step) as one LDS-tiled kernel over a batch of planes: op.src in, the blurred plane to op.dst,
+// and with op.aux set the band op.aux = src - blur (flow.rs:228-231). Each element is computed as
+// by the separate passes, so the bits are the same.
+__global__ __launch_bounds__(TILE_THREADS) void k_blur5_tile(const PlaneOp* __restrict__ ops, Taps t, int tiles_x,
+                                                             int tiles_y, int nops) {
+    constexpr int R = 2;
+    const int per_op = tiles_x * tiles_y;
+    const int lt = xcd_tile(per_op * nops);
+    if (lt >= per_op * nops) return;
+    const int job = lt / per_op, rem = lt - job * per_op;
+    const int tyi = rem / tiles_x, txi = rem - tyi * tiles_x;
+    const PlaneOp& op = ops[job];
+    const int w = op.sw, h = op.sh;
+    const int x0 = txi * TILE_X, y0 = tyi * TILE_Y;
+    if (x0 >= w || y0 >= h) return;
+    constexpr int RW = TILE_X + 2 * R, RH = TILE_Y + 2 * R;
+    constexpr int RUNS_X = TILE_X / TILE_RUN, HITEMS = RH * RUNS_X;
+    constexpr int HPER = (HITEMS + TILE_THREADS - 1) / TILE_THREADS;
+    __shared__ float s_in[RH * RW];
+    const float* __restrict__ src = (const float*)op.src;
+    const int tid = threadIdx.x;
+    for (int k = tid; k < RH * RW; k += TILE_THREADS) {
+        const int ry = k / RW, rx = k - ry * RW;
+        const int x = x0 - R + rx, y = y0 - R + ry;
+        if (x < 0 || x >= w || y < 0 || y >= h) continue;
+        s_in[k] = src[(long)y * w + x];
+    }
+    __syncthreads();
+    const int nh = min(R + 1, w);
+    const int hint = w > 2 * (R + 1) ? w - 2 * (R + 1) : 0;
+    float hres[HPER][TILE_RUN];
+#pragma unroll
+    for (int q = 0; q < HPER; q++) {
+        const int it = tid + q * TILE_THREADS;
+        if (it >= HITEMS) continue;
+        const int ry = it / RUNS_X, rx0 = (it - ry * RUNS_X) * TILE_RUN;
+        const int y = y0 - R + ry, xs = x0 + rx0;
+        if (y < 0 || y >= h || xs >= w) continue;
+        const float* line = s_in + ry * RW;
+        if (xs >= nh && xs + TILE_RUN - 1 < nh + hint) {
+            float win[TILE_RUN + 2 * R];
+#pragma unroll
+            for (int d = 0; d < TILE_RUN + 2 * R; d++) win[d] = line[rx0 + d];
+            run_interior<R>(win, t, hres[q]);
+        } else {
+#pragma unroll
+            for (int m = 0; m < TILE_RUN; m++)
+                if (xs + m < w) hres[q][m] = line_elem<R, false, false>(line, 1, x0 - R, w, xs + m, t);
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < HPER; q++) {
+        const int it = tid + q * TILE_THREADS;
+        if (it >= HITEMS) continue;
+        const int ry = it / RUNS_X, rx0 = (it - ry * RUNS_X) * TILE_RUN;
+        const int y = y0 - R + ry, xs = x0 + rx0;
+        if (y < 0 || y >= h || xs >= w) continue;
+#pragma unroll
+        for (int m = 0; m < TILE_RUN; m++)
+            if (xs + m < w) s_in[ry * RW + R + rx0 + m] = hres[q][m];
+    }
+    __syncthreads();
+    const int hx = tid % TILE_X, ry0 = (tid / TILE_X) * TILE_RUN;
+    const int x = x0 + hx, ys = y0 + ry0;
+    if (x >= w || ys >= h) return;
+    const int nv = min(R + 1, h);
+    const int vint = h > 2 * (R + 1) ? h - 2 * (R + 1) : 0;
+    const float* col = s_in + R + hx;
+    float o[TILE_RUN];
+    if (ys >= nv && ys + TILE_RUN - 1 < nv + vint) {
+        float win[TILE_RUN + 2 * R];
+#pragma unroll
+        for (int d = 0; d < TILE_RUN + 2 * R; d++) win[d] = col[(ry0 + d) * RW];
+        run_interior<R>(win, t, o);
+    } else {
+#pragma unroll
+        for (int m = 0; m < TILE_RUN; m++)
+            if (ys + m < h) o[m] = line_elem<R, true, false>(col, RW, y0 - R, h, ys + m, t);
+    }
+#pragma unroll
+    for (int m = 0; m < TILE_RUN; m++) {
+        if (ys + m >= h) break;
+        const long i = (long)(ys + m) * w + x;
+        op.dst[i] = o[m];
+        if (op.aux) op.aux[i] = src[i] - o[m];
+    }
+}
+
+void launch_blur5_tile(hipStream_t s, const PlaneOp* d_ops, const std::vector<PlaneOp>& ops, int maxw, int maxh, const Taps& t) {
+    const int tx = (maxw + TILE_X - 1) / TILE_X, ty = (maxh + TILE_Y - 1) / TILE_Y;
+    const long n = (long)tx * ty * (long)ops.size();
+    double bytes = 0;
+    for (auto& o : ops) bytes += (double)o.sw * o.sh * (o.aux ? 12.0 : 8.0);
+    TWX_LAUNCH(s, F_PYR_BLUR, bytes,
+               k_blur5_tile<<<dim3((unsigned)(8 * ((n + 7) / 8))), dim3(TILE_THREADS), 0, s>>>(d_ops, t, tx, ty, (int)ops.size()));
 }
 
 #define JOB_PROLOGUE()                                                    \
