@@ -165,6 +165,27 @@ int twx_retime_batch(twx_handle* h, int32_t n_clips, const twx_retime_params* pa
 int twx_retime_frames(twx_handle* h, const twx_retime_params* p, const void* clip, int32_t n_in,
                       int32_t first, int32_t count, void* out, char* err, size_t errlen);
 
+/* ---- the differentiable warp+blend (no counterpart in the reference) ------------------------ */
+
+/* out = blend(W(A, F_BA, 1 - wa), W(B, F_AB, wa), 1 - wa) on float images of `channels` (1..4)
+ * interleaved channels, width x height, rows top first; W is the inverse warp with hold edges
+ * of warp.rs:477-512 (flows interleaved (dx, dy) at flow_w x flow_h), the blend the float blend
+ * of blend.rs:55-114 without its snap branches, ((tB - tA) * w + tA) + 0. DEVICE pointers. */
+int twx_warp_blend(twx_handle* h, int32_t channels, int32_t width, int32_t height, const float* a,
+                   const float* b, const float* f_ab, const float* f_ba, int32_t flow_w,
+                   int32_t flow_h, float wa, float* out, char* err, size_t errlen);
+
+/* The backward pass of twx_warp_blend for the upstream gradient grad_out (same shape as out):
+ * dL/dA, dL/dB (images) and dL/dF_AB, dL/dF_BA (flows), each written when its pointer is not
+ * NULL. Sample points are recomputed, not stored. The flow gradients are summed per flow texel
+ * in a fixed order (deterministic); the image gradients are scattered with atomic adds. DEVICE
+ * pointers. */
+int twx_warp_blend_backward(twx_handle* h, int32_t channels, int32_t width, int32_t height,
+                            const float* a, const float* b, const float* f_ab, const float* f_ba,
+                            int32_t flow_w, int32_t flow_h, float wa, const float* grad_out,
+                            float* grad_a, float* grad_b, float* grad_f_ab, float* grad_f_ba,
+                            char* err, size_t errlen);
+
 /* ---- counters (Exec::summary, device/exec/mod.rs:292-322) -------------------------------- */
 
 typedef struct twx_stats {

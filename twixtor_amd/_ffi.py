@@ -11,12 +11,14 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libtwx.so")
 
 TWX_OK, TWX_E_DEVICE, TWX_E_NAN_OUTPUT, TWX_E_UNSUPPORTED, TWX_E_MISMATCH = 0, 1, 2, 3, 4
+TWX_FMT_U8, TWX_FMT_U16, TWX_FMT_F32 = 0, 1, 2
 
 # every entry point of include/twx.h
 EXPORTS = [
     "twx_open", "twx_close", "twx_selfcheck", "twx_memory_budget", "twx_flow_geometry", "twx_solve",
     "twx_solve_pair", "twx_compose_vectors", "twx_select", "twx_retime_frames_dev", "twx_retime_batch",
     "twx_retime_frames",
+    "twx_warp_blend", "twx_warp_blend_backward",
     "twx_get_stats", "twx_stream", "twx_profile_enable", "twx_profile_reset", "twx_profile_read",
 ]
 
@@ -94,6 +96,8 @@ def lib():
         PI32 = ctypes.POINTER(ctypes.c_int32)
         L.twx_retime_batch.argtypes = [P, I32, ctypes.POINTER(RetimeParams), ctypes.POINTER(ctypes.c_void_p), PI32, PI32,
                                        PI32, ctypes.POINTER(ctypes.c_void_p)] + ERR
+        L.twx_warp_blend.argtypes = [P, I32, I32, I32, P, P, P, P, I32, I32, ctypes.c_float, P] + ERR
+        L.twx_warp_blend_backward.argtypes = [P, I32, I32, I32, P, P, P, P, I32, I32, ctypes.c_float, P, P, P, P, P] + ERR
         L.twx_get_stats.argtypes = [P, ctypes.POINTER(Stats)]
         L.twx_stream.argtypes = [P]
         L.twx_stream.restype = P

@@ -1223,6 +1223,58 @@ int twx_retime_frames(twx_handle* h, const twx_retime_params* p, const void* cli
     return st;
 }
 
+static int warp_blend_entry(twx_handle* h, bool backward, int32_t channels, int32_t width, int32_t height, const float* a,
+                            const float* b, const float* f_ab, const float* f_ba, int32_t flow_w, int32_t flow_h, float wa,
+                            float* out, const float* grad_out, float* grad_a, float* grad_b, float* grad_f_ab,
+                            float* grad_f_ba, char* err, size_t errlen) {
+    std::string e;
+    int st = TWX_OK;
+    try {
+        Guard g(h, st, e);
+        if (g.ok()) {
+            hipStream_t s = h->stream;
+            DevPool pool(h);
+            const size_t px = (size_t)std::max(width, 0) * std::max(height, 0);
+            const size_t img = px * std::max(channels, 0) * sizeof(float);
+            int* bad = pool.alloc<int>(1);
+            HIPCHECK(hipMemsetAsync(bad, 0, sizeof(int), s));
+            float* gp = backward ? pool.alloc<float>(px * 4) : nullptr;
+            if (backward) {
+                if (grad_a) HIPCHECK(hipMemsetAsync(grad_a, 0, img, s));
+                if (grad_b) HIPCHECK(hipMemsetAsync(grad_b, 0, img, s));
+            }
+            st = warp_blend_run(s, backward, channels, width, height, a, b, f_ab, f_ba, flow_w, flow_h, wa, out, grad_out,
+                                grad_a, grad_b, grad_f_ab, grad_f_ba, gp, bad, e);
+            if (!st) {
+                int* hb = pool.pinned<int>(1);
+                HIPCHECK(hipMemcpyAsync(hb, bad, sizeof(int), hipMemcpyDeviceToHost, s));
+                HIPCHECK(hipStreamSynchronize(s));
+                if (*hb) { e = "warp+blend: a sample point is NaN or never converges"; st = TWX_E_UNSUPPORTED; }
+            }
+        }
+    } catch (const HipError& x) {
+        e = x.msg;
+        st = TWX_E_DEVICE;
+    }
+    if (st) set_err(err, errlen, e);
+    return st;
+}
+
+int twx_warp_blend(twx_handle* h, int32_t channels, int32_t width, int32_t height, const float* a, const float* b,
+                   const float* f_ab, const float* f_ba, int32_t flow_w, int32_t flow_h, float wa, float* out, char* err,
+                   size_t errlen) {
+    return warp_blend_entry(h, false, channels, width, height, a, b, f_ab, f_ba, flow_w, flow_h, wa, out, nullptr, nullptr,
+                            nullptr, nullptr, nullptr, err, errlen);
+}
+
+int twx_warp_blend_backward(twx_handle* h, int32_t channels, int32_t width, int32_t height, const float* a, const float* b,
+                            const float* f_ab, const float* f_ba, int32_t flow_w, int32_t flow_h, float wa,
+                            const float* grad_out, float* grad_a, float* grad_b, float* grad_f_ab, float* grad_f_ba,
+                            char* err, size_t errlen) {
+    return warp_blend_entry(h, true, channels, width, height, a, b, f_ab, f_ba, flow_w, flow_h, wa, nullptr, grad_out,
+                            grad_a, grad_b, grad_f_ab, grad_f_ba, err, errlen);
+}
+
 int twx_get_stats(twx_handle* h, twx_stats* out) {
     if (!h || !out) return TWX_E_MISMATCH;
     *out = h->stats;

@@ -116,7 +116,7 @@ struct FillPlane {
 enum ProfFam {
     F_INGEST, F_LUMA, F_PYR_BLUR, F_PYR_RESAMPLE, F_GRAD_TENSOR, F_BM_TRANSPOSE,
     F_SOLVE_INIT, F_CALC_B, F_NEW_DISP, F_BM_ROWS, F_BM_COLS, F_RANGE, F_UPSAMPLE, F_NEGATE, F_FRAMES,
-    F_WB_MASK, F_WB_SHARPEN, F_FILL_PULL, F_FILL_PUSH, F_FILL_IO, F_NFAM
+    F_WB_MASK, F_WB_SHARPEN, F_FILL_PULL, F_FILL_PUSH, F_FILL_IO, F_DIFF_FWD, F_DIFF_BWD, F_NFAM
 };
 extern const char* const PROF_NAMES[F_NFAM];
 
@@ -182,6 +182,11 @@ void launch_fill(hipStream_t s, const FillPlane* d_planes, const std::vector<Fil
 void launch_finish(hipStream_t s, const FrameJob* d_jobs, const std::vector<FrameJob>& jobs);
 void launch_nan_scan(hipStream_t s, const float* data, size_t n, int* flag);
 int run_selfcheck(hipStream_t s, std::string& err);
+// The differentiable warp+blend (twx_diff.hip): the forward into `out`, or the backward into the
+// gradients that are not null; gp_scratch holds 4 floats per pixel, `bad` is set for a NaN point.
+int warp_blend_run(hipStream_t s, bool backward, int C, int W, int H, const float* a, const float* b, const float* f_ab,
+                   const float* f_ba, int fw, int fh, float wa, float* out, const float* grad_out, float* grad_a,
+                   float* grad_b, float* grad_f_ab, float* grad_f_ba, float* gp_scratch, int* bad, std::string& err);
 
 // Taps of the solver's Gaussians and of the pyramid filter (conv.rs:44-86).
 Taps gaussian_taps(float r);

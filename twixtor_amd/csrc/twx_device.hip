@@ -153,7 +153,8 @@ void launch_ingest(hipStream_t s, const void* wire, void* engine, int W, int H, 
 const char* const PROF_NAMES[F_NFAM] = {"ingest", "luma", "pyr_blur5_band", "pyr_resample",
                                         "tensor_gauss", "bm_transpose", "solve_init", "calc_b_gauss", "new_disp_gauss",
                                         "bm_rows", "bm_cols", "range", "upsample", "negate", "frames",
-                                        "wb_mask", "wb_sharpen_blend", "fill_pull", "fill_push", "fill_io"};
+                                        "wb_mask", "wb_sharpen_blend", "fill_pull", "fill_push", "fill_io",
+                                        "warp_blend_fwd", "warp_blend_bwd"};
 thread_local Prof* g_prof = nullptr;
 
 hipEvent_t Prof::take() {

@@ -1,0 +1,347 @@
+// The differentiable warp+blend: out = blend(W(A, F_BA, 1 - wa), W(B, F_AB, wa), 1 - wa) on float
+// images of C channels (interleaved, rows top first), with W the inverse warp with hold edges of
+// warp.rs:477-512 (flow sampled bilinearly at the working size, the sample point clipped to the
+// image along its segment, clip_to_rect warp.rs:178-215, the image sampled bilinearly, bilerp4
+// warp.rs:306-331) and the blend of blend.rs:55-114 on floats, ((tB - tA) * w + tA) + 0.
+//
+// The backward pass recomputes every sample point instead of storing it: per output pixel the
+// flow sample, the clip (with the Jacobian of the clipped point with respect to the flow value
+// carried through each clip step) and the image sample, then
+//   dL/dframes: each output pixel's upstream gradient scattered to the four texels it sampled
+//               (atomic adds: the sample points are data dependent);
+//   dL/dflow:   the gradient with respect to each pixel's flow value (two floats per pixel), then
+//               gathered per flow texel over the output pixels whose flow sample uses it, in a
+//               fixed order (deterministic).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "twx_common.hpp"
+
+namespace twx {
+
+namespace {
+
+__device__ __forceinline__ int32_t cvtt_d(float x) {
+    if (isnan(x) || x >= 2147483648.0f || x < -2147483648.0f) return INT32_MIN;
+    return (int32_t)x;
+}
+
+// The flow sample of output pixel (x, y): corner (jx, jy), fractions (gx, gy), value (dx, dy).
+struct FlowSample {
+    int jx, jy;
+    float gx, gy;
+    float d[2];
+};
+
+__device__ __forceinline__ void flow_sample(const float* __restrict__ F, int fw, int fh, float rx, float ry, int x, int y,
+                                            FlowSample& s) {
+    const float qx = (float)x * rx, qy = (float)y * ry;
+    int32_t jx = cvtt_d(qx), jy = cvtt_d(qy);
+    float gx = qx - (float)jx, gy = qy - (float)jy;
+    if (jx >= fw - 1) { jx = fw - 2; gx = 1.0f; }
+    if (jy >= fh - 1) { jy = fh - 2; gy = 1.0f; }
+    const long f00 = ((long)jy * fw + jx) * 2, f01 = f00 + (long)fw * 2;
+#pragma unroll
+    for (int k = 0; k < 2; k++) {
+        const float p00 = F[f00 + k], p10 = F[f00 + 2 + k], p01 = F[f01 + k], p11 = F[f01 + 2 + k];
+        const float top = (p10 - p00) * gx + p00;
+        const float bot = (p11 - p01) * gx + p01;
+        s.d[k] = (bot - top) * gy + top;
+    }
+    s.jx = jx;
+    s.jy = jy;
+    s.gx = gx;
+    s.gy = gy;
+}
+
+// clip_to_rect (warp.rs:178-215) with J = d(ix, iy) / d(dx, dy) carried through each step.
+__device__ __forceinline__ bool clip_jac(float x, float y, float& ix, float& iy, float w1, float h1, float J[2][2]) {
+    for (int n = 0; n < 4096; n++) {
+        if (ix < 0.0f || ix > w1) {
+            const float bound = ix < 0.0f ? 0.0f : w1;
+            const float dy = y - iy, dxs = x - ix, num = x - bound;
+            const float t = num / dxs;
+            // iy' = y - t * dy, t = num / (x - ix): dt = t / dxs * d ix, d dy = -d iy
+#pragma unroll
+            for (int c = 0; c < 2; c++) {
+                const float dt = t / dxs * J[0][c];
+                J[1][c] = -(dt * dy) + t * J[1][c];
+                J[0][c] = 0.0f;
+            }
+            ix = bound;
+            iy = y - t * dy;
+            continue;
+        }
+        if (iy < 0.0f || iy > h1) {
+            const float bound = iy < 0.0f ? 0.0f : h1;
+            const float dyy = y - iy, num = y - bound, dx = x - ix;
+            const float t = num / dyy;
+#pragma unroll
+            for (int c = 0; c < 2; c++) {
+                const float dt = t / dyy * J[1][c];
+                J[0][c] = -(dt * dx) + t * J[0][c];
+                J[1][c] = 0.0f;
+            }
+            ix = x - t * dx;
+            iy = bound;
+            continue;
+        }
+        return true;
+    }
+    return false;
+}
+
+// The sample point of one warp at output pixel (x, y) and its Jacobian; false for a NaN point.
+__device__ __forceinline__ bool warp_point_jac(const float* __restrict__ F, int fw, int fh, int W, int H, float amount, int x,
+                                               int y, FlowSample& fs, float& ix, float& iy, float J[2][2]) {
+    const float rx = (float)(fw - 1) / (float)(W - 1), ry = (float)(fh - 1) / (float)(H - 1);
+    const float sx = amount / rx, sy = amount / ry;
+    flow_sample(F, fw, fh, rx, ry, x, y, fs);
+    ix = fs.d[0] * sx + (float)x;
+    iy = fs.d[1] * sy + (float)y;
+    J[0][0] = sx;
+    J[0][1] = 0.0f;
+    J[1][0] = 0.0f;
+    J[1][1] = sy;
+    if (!clip_jac((float)x, (float)y, ix, iy, (float)(W - 1), (float)(H - 1), J) || isnan(ix) || isnan(iy)) return false;
+    return true;
+}
+
+// The image sample at (ix, iy) (bilerp4): corner, fractions and whether each fraction moves with
+// its coordinate (not at the clamped last cell).
+struct ImgSample {
+    int kx, ky;
+    float ax, ay;
+    float dax, day;
+};
+
+__device__ __forceinline__ void img_cell(float ix, float iy, int W, int H, ImgSample& s) {
+    int32_t kx = cvtt_d(ix), ky = cvtt_d(iy);
+    float ax = ix - (float)kx, ay = iy - (float)ky;
+    s.dax = 1.0f;
+    s.day = 1.0f;
+    if (kx >= W - 1) { kx = W - 2; ax = 1.0f; s.dax = 0.0f; }
+    if (ky >= H - 1) { ky = H - 2; ay = 1.0f; s.day = 0.0f; }
+    s.kx = kx;
+    s.ky = ky;
+    s.ax = ax;
+    s.ay = ay;
+}
+
+struct DiffArgs {
+    const float* a;
+    const float* b;
+    const float* f_ab;
+    const float* f_ba;
+    const float* grad_out;
+    float* out;
+    float* grad_a;
+    float* grad_b;
+    float* gp_ba;   // per output pixel dL/d(flow value) of the warp of A
+    float* gp_ab;   // of the warp of B
+    float* grad_f_ab;
+    float* grad_f_ba;
+    int* bad;
+    int C, W, H, fw, fh;
+    float wa, frac;
+    int copy_a, copy_b;  // the |amount| early-out of each warp: a plain copy
+    int mode;            // the blend's branch (blend.rs:55-72): 0 copy A, 1 copy B, 3 per-sample mix
+};
+
+// One warp at (x, y): the sampled value (C channels), and with `g` the backward: the scatter into
+// the image gradient and the flow-value gradient `gflow`.
+template <bool BWD>
+__device__ __forceinline__ bool warp_one(const DiffArgs& P, const float* __restrict__ src, const float* __restrict__ F, float amount,
+                                         int copy, int x, int y, float* v, const float* g, float* gsrc, float gflow[2]) {
+    const int C = P.C, W = P.W, H = P.H;
+    const long pi = (long)y * W + x;
+    if (copy) {
+        for (int c = 0; c < C; c++) v[c] = src[pi * C + c];
+        if (BWD) {
+            if (gsrc)
+                for (int c = 0; c < C; c++) atomicAdd(&gsrc[pi * C + c], g[c]);
+            gflow[0] = gflow[1] = 0.0f;
+        }
+        return true;
+    }
+    FlowSample fs;
+    float ix, iy, J[2][2];
+    if (!warp_point_jac(F, P.fw, P.fh, W, H, amount, x, y, fs, ix, iy, J)) {
+        for (int c = 0; c < C; c++) v[c] = 0.0f;
+        if (BWD) gflow[0] = gflow[1] = 0.0f;
+        return false;
+    }
+    ImgSample s;
+    img_cell(ix, iy, W, H, s);
+    const long i00 = (long)s.ky * W + s.kx, i01 = i00 + W;
+    float dlx = 0.0f, dly = 0.0f;
+    for (int c = 0; c < C; c++) {
+        const float p00 = src[i00 * C + c], p10 = src[(i00 + 1) * C + c];
+        const float p01 = src[i01 * C + c], p11 = src[(i01 + 1) * C + c];
+        const float top = p00 + (p10 - p00) * s.ax;
+        const float bot = p01 + (p11 - p01) * s.ax;
+        v[c] = top + (bot - top) * s.ay;
+        if (BWD) {
+            const float dvx = (p10 - p00) * (1.0f - s.ay) + (p11 - p01) * s.ay;
+            const float dvy = bot - top;
+            dlx += g[c] * dvx;
+            dly += g[c] * dvy;
+            if (gsrc) {
+                atomicAdd(&gsrc[i00 * C + c], g[c] * ((1.0f - s.ax) * (1.0f - s.ay)));
+                atomicAdd(&gsrc[(i00 + 1) * C + c], g[c] * (s.ax * (1.0f - s.ay)));
+                atomicAdd(&gsrc[i01 * C + c], g[c] * ((1.0f - s.ax) * s.ay));
+                atomicAdd(&gsrc[(i01 + 1) * C + c], g[c] * (s.ax * s.ay));
+            }
+        }
+    }
+    if (BWD) {
+        dlx *= s.dax;
+        dly *= s.day;
+        gflow[0] = J[0][0] * dlx + J[1][0] * dly;
+        gflow[1] = J[0][1] * dlx + J[1][1] * dly;
+    }
+    return true;
+}
+
+__global__ void k_wb_forward(DiffArgs P) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
+    if (x >= P.W || y >= P.H) return;
+    float ta[4], tb[4];
+    bool ok = warp_one<false>(P, P.a, P.f_ba, P.frac, P.copy_a, x, y, ta, nullptr, nullptr, nullptr);
+    ok = warp_one<false>(P, P.b, P.f_ab, P.wa, P.copy_b, x, y, tb, nullptr, nullptr, nullptr) && ok;
+    if (!ok) atomicOr(P.bad, 1);
+    const long pi = (long)y * P.W + x;
+    for (int c = 0; c < P.C; c++)
+        P.out[pi * P.C + c] = P.mode == 0 ? ta[c] : (P.mode == 1 ? tb[c] : ((tb[c] - ta[c]) * P.frac + ta[c]) + 0.0f);
+}
+
+__global__ void k_wb_backward(DiffArgs P) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
+    if (x >= P.W || y >= P.H) return;
+    const long pi = (long)y * P.W + x;
+    float g[4], ga[4], gb[4], v[4];
+    for (int c = 0; c < P.C; c++) {
+        g[c] = P.grad_out[pi * P.C + c];
+        // out = tA (1 - w) + tB w, or one of the two
+        ga[c] = P.mode == 0 ? g[c] : (P.mode == 1 ? 0.0f : g[c] * (1.0f - P.frac));
+        gb[c] = P.mode == 0 ? 0.0f : (P.mode == 1 ? g[c] : g[c] * P.frac);
+    }
+    float gfa[2], gfb[2];
+    bool ok = warp_one<true>(P, P.a, P.f_ba, P.frac, P.copy_a, x, y, v, ga, P.grad_a, gfa);
+    ok = warp_one<true>(P, P.b, P.f_ab, P.wa, P.copy_b, x, y, v, gb, P.grad_b, gfb) && ok;
+    if (!ok) atomicOr(P.bad, 1);
+    P.gp_ba[pi * 2] = gfa[0];
+    P.gp_ba[pi * 2 + 1] = gfa[1];
+    P.gp_ab[pi * 2] = gfb[0];
+    P.gp_ab[pi * 2 + 1] = gfb[1];
+}
+
+// dL/dflow per flow texel (i, j): the sum over the output pixels whose flow sample has (i, j) as
+// a corner of the corner's bilinear weight times the pixel's flow-value gradient, in a fixed order.
+__global__ void k_wb_flow_gather(DiffArgs P, int which) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
+    const int fw = P.fw, fh = P.fh, W = P.W, H = P.H;
+    if (i >= fw || j >= fh) return;
+    const float* gp = which ? P.gp_ab : P.gp_ba;
+    float* out = which ? P.grad_f_ab : P.grad_f_ba;
+    if (!out) return;
+    const float rx = (float)(fw - 1) / (float)(W - 1), ry = (float)(fh - 1) / (float)(H - 1);
+    // output columns whose cell index is i - 1 or i (with the last-cell clamp), and rows likewise
+    const int x0 = max(0, (int)floorf((float)(i - 1) / rx) - 2), x1 = min(W - 1, (int)ceilf((float)(i + 1) / rx) + 2);
+    const int y0 = max(0, (int)floorf((float)(j - 1) / ry) - 2), y1 = min(H - 1, (int)ceilf((float)(j + 1) / ry) + 2);
+    float s0 = 0.0f, s1 = 0.0f;
+    for (int y = y0; y <= y1; y++) {
+        const float qy = (float)y * ry;
+        int32_t jy = cvtt_d(qy);
+        float gy = qy - (float)jy;
+        if (jy >= fh - 1) { jy = fh - 2; gy = 1.0f; }
+        float wy;
+        if (jy == j) wy = 1.0f - gy;
+        else if (jy + 1 == j) wy = gy;
+        else continue;
+        for (int x = x0; x <= x1; x++) {
+            const float qx = (float)x * rx;
+            int32_t jx = cvtt_d(qx);
+            float gx = qx - (float)jx;
+            if (jx >= fw - 1) { jx = fw - 2; gx = 1.0f; }
+            float wx;
+            if (jx == i) wx = 1.0f - gx;
+            else if (jx + 1 == i) wx = gx;
+            else continue;
+            const float wgt = wx * wy;
+            const long pi = (long)y * W + x;
+            s0 += gp[pi * 2] * wgt;
+            s1 += gp[pi * 2 + 1] * wgt;
+        }
+    }
+    out[((long)j * fw + i) * 2] = s0;
+    out[((long)j * fw + i) * 2 + 1] = s1;
+}
+
+}  // namespace
+
+// The |amount| early-out of inv_warp (warp.rs:749-764).
+static bool diff_warp_is_copy(float amount, int fw, int W) {
+    float a = amount < 0.0f ? -amount : amount;
+    if (0.005 > (double)a) return true;
+    float r = (float)(fw - 1) / (float)(W - 1);
+    float q = (float)((double)amount / (double)r);
+    if (q < 0.0f) q = -q;
+    return 0.0001 > (double)q;
+}
+
+int warp_blend_run(hipStream_t s, bool backward, int C, int W, int H, const float* a, const float* b, const float* f_ab,
+                   const float* f_ba, int fw, int fh, float wa, float* out, const float* grad_out, float* grad_a,
+                   float* grad_b, float* grad_f_ab, float* grad_f_ba, float* gp_scratch, int* bad, std::string& err) {
+    if (C < 1 || C > 4 || W < 2 || H < 2 || fw < 2 || fh < 2) {
+        err = "warp+blend: 1 to 4 channels, images and flows of at least 2x2";
+        return TWX_E_MISMATCH;
+    }
+    DiffArgs P{};
+    P.a = a;
+    P.b = b;
+    P.f_ab = f_ab;
+    P.f_ba = f_ba;
+    P.out = out;
+    P.grad_out = grad_out;
+    P.grad_a = grad_a;
+    P.grad_b = grad_b;
+    P.grad_f_ab = grad_f_ab;
+    P.grad_f_ba = grad_f_ba;
+    P.gp_ba = gp_scratch;
+    P.gp_ab = gp_scratch + (size_t)W * H * 2;
+    P.bad = bad;
+    P.C = C;
+    P.W = W;
+    P.H = H;
+    P.fw = fw;
+    P.fh = fh;
+    P.wa = wa;
+    P.frac = 1.0f - wa;
+    P.copy_a = diff_warp_is_copy(P.frac, fw, W) ? 1 : 0;
+    P.copy_b = diff_warp_is_copy(wa, fw, W) ? 1 : 0;
+    const float w = P.frac;
+    if (0.005 > (double)std::fabs(w)) P.mode = 0;
+    else if (0.005 > std::fabs((double)w - 1.0)) P.mode = 1;
+    else if (w < 0.0f || w > 1.0f) {
+        err = "warp+blend: a blend weight 1 - wa outside [0, 1] (the extrapolating lerp) has no backward here";
+        return TWX_E_UNSUPPORTED;
+    } else P.mode = 3;
+    dim3 blk(256), grd((W + 255) / 256, H);
+    const double img = 4.0 * C * (double)W * H, fl = 8.0 * (double)fw * fh;
+    if (!backward) {
+        TWX_LAUNCH(s, F_DIFF_FWD, 3.0 * img + 2.0 * fl, k_wb_forward<<<grd, blk, 0, s>>>(P));
+        return TWX_OK;
+    }
+    TWX_LAUNCH(s, F_DIFF_BWD, 4.0 * img + 2.0 * fl + 16.0 * (double)W * H, k_wb_backward<<<grd, blk, 0, s>>>(P));
+    dim3 fgrd((fw + 255) / 256, fh);
+    if (grad_f_ba) TWX_LAUNCH(s, F_DIFF_BWD, 8.0 * (double)W * H + fl, k_wb_flow_gather<<<fgrd, blk, 0, s>>>(P, 0));
+    if (grad_f_ab) TWX_LAUNCH(s, F_DIFF_BWD, 8.0 * (double)W * H + fl, k_wb_flow_gather<<<fgrd, blk, 0, s>>>(P, 1));
+    return TWX_OK;
+}
+
+}  // namespace twx

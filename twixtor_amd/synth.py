@@ -176,3 +176,13 @@ def spec_batch(seed: int = 3, clips: int = 64, frames: int = 24) -> list:
         h = int(rng.randint(45, 289)) * 4
         out.append(spec_small(seed=seed * 1000 + i, width=w, height=h, frames=frames, depth=8, scale=w / 160.0))
     return out
+
+
+def spec_720p_2x(seed: int = 4, frames: int = 9) -> ClipSpec:
+    """1280x720 RGB float32 with alpha 1, rendered at twice the rate of the frame pairs it feeds:
+    frames 2k and 2k + 2 form a pair and frame 2k + 1 is its true middle frame. The noise
+    background translates (2.1, -0.9) px/frame; 6 discs with r uniform in 20-80 px and seeded
+    velocities up to 8 px/frame; no per-pixel noise (the middle frame is exact)."""
+    rng = np.random.RandomState(seed)
+    discs = _rng_discs(rng, 6, 1280, 720, 20.0, 80.0, 8.0)
+    return ClipSpec(1280, 720, frames, 32, (2.1, -0.9), discs, 0.0, seed, 16.0 * 8)
