@@ -45,6 +45,9 @@ def main():
     ap.add_argument("--batch-clips", type=int, default=64,
                     help="clips of the mixed-size batch leg (batched against one call per output frame); 0 skips it")
     ap.add_argument("--batch-only", action="store_true", help="run only the mixed-size batch leg")
+    ap.add_argument("--diff-steps", type=int, default=50,
+                    help="refinement steps of the differentiable warp+blend leg (1280x720); 0 skips it")
+    ap.add_argument("--diff-only", action="store_true", help="run only the differentiable warp+blend leg")
     args = ap.parse_args()
     if args.copy_probe:
         import torch
@@ -72,6 +75,9 @@ def main():
     if args.batch_only:
         be = HipBackend(0)
         print(json.dumps({"batch": batch_leg(be, args.batch_clips, args.steps, args.warmup)}), flush=True)
+        return
+    if args.diff_only:
+        print(json.dumps({"diff": diff_leg(args.diff_steps)}), flush=True)
         return
     spec = synth.spec_1080p(frames=args.frames)
     gen = synth.Clip(spec, device=dev)
@@ -123,6 +129,8 @@ def main():
     path = {"algorithmic_GB_per_step": round(total_bytes / 1e9, 3),
             "achieved_GBs": round(total_bytes / (ms_per_step / 1000.0) / 1e9, 1),
             "kernels": {n: {"ms_per_step": round(v["ms"] / args.steps, 3),
+                            "algorithmic_bytes_per_step": round(v["bytes"] / args.steps),
+                            "launches_per_step": v["launches"] // args.steps,
                             "GBs": round(v["bytes"] / (v["ms"] / 1000.0) / 1e9, 1) if v["ms"] > 0 else 0.0}
                         for n, v in sorted(prof.items(), key=lambda kv: -kv[1]["ms"])}}
     be.profile(False)
@@ -143,6 +151,7 @@ def main():
     del clip, out
     torch.cuda.empty_cache()
     batch = batch_leg(be, args.batch_clips, args.steps, args.warmup) if args.batch_clips > 0 else None
+    diff = diff_leg(args.diff_steps) if args.diff_steps > 0 else None
 
     line = {
         "metric": "1080p retime output frames/s",
@@ -165,6 +174,7 @@ def main():
         "path": path,
         "per_pair": per_pair,
         "batch": batch,
+        "diff": diff,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
@@ -222,6 +232,44 @@ def batch_leg(be, n_clips, steps, warmup):
             "unique_solves_per_step": solves, "batched_ms": round(t_batch * 1000.0, 2),
             "batched_frames_per_s": round(n * n_out / t_batch, 1), "per_pair_ms": round(t_pp * 1000.0, 2),
             "per_pair_calls": n * n_out, "speedup": round(t_pp / t_batch, 2), "bytes_equal": bool(same)}
+
+
+def diff_leg(steps):
+    """The differentiable warp+blend: pair 0 of the 1280x720 RGB clip rendered at twice the pair
+    rate, solver flows plus N(0, 0.5^2) px noise, `steps` Adam steps of photometric refinement
+    (forward, L2 loss against the true middle frame, backward, update; each step recomputes from
+    the flows). Reports the time per step, the loss before and after, and the HIP-event time and
+    algorithmic bytes of the forward and backward kernels."""
+    import torch
+
+    from twixtor_amd import diff, synth
+
+    dev = torch.device("cuda:0")
+    clip = synth.Clip(synth.spec_720p_2x(frames=3), device="cpu")
+    a, b, truth, fab, fba = diff.photometric_case(clip, 0)
+    h = diff.handle(dev)
+    diff.refine_flows(a, b, truth, fab, fba, 0.5, steps=2)  # warm-up
+    torch.cuda.synchronize()
+    h.profile(True)
+    h.profile_reset()
+    t0 = time.perf_counter()
+    _, _, losses = diff.refine_flows(a, b, truth, fab, fba, 0.5, steps=steps)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    prof = h.profile_read()
+    h.profile(False)
+    kern = {}
+    for name, v in prof.items():
+        n = max(v["launches"], 1)
+        avg = v["ms"] / n
+        gbs = (v["bytes"] / n) / (avg / 1000.0) / 1e9 if avg > 0 else 0.0
+        kern[name] = {"launches": v["launches"], "avg_launch_ms": round(avg, 4),
+                      "algorithmic_bytes_per_launch": round(v["bytes"] / n), "GBs": round(gbs, 1),
+                      "frac_of_hbm_peak": round(gbs / HBM_PEAK_GBS, 4)}
+    return {"workload": "1280x720 RGB f32 pair (frames 0 and 2 of a clip at twice the pair rate, truth frame 1), "
+                        "flows 640x360 from the solver + N(0, 0.5^2) px, blend at wa = 0.5, L2 loss, Adam lr 0.05",
+            "steps": steps, "ms_per_step": round(el * 1000.0 / steps, 3),
+            "loss_first": losses[0], "loss_last": losses[-1], "kernels": kern}
 
 
 def cpu_baseline(cfg, clip, out, n_in, n_out, fb, budget_s):

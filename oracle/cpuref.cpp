@@ -1657,6 +1657,485 @@ static int compose_smart(int interp, float wa, const void* a, const void* b, lon
 }
 
 // ------------------------------------------------------------------------------------------
+// Forward warp (forward.rs, forward/mesh.rs, forward/raster.rs, forward/pyramid.rs): the displaced
+// vertex grid, the triangle scan converter, the running mean per pixel in triangle order, the
+// textured (mip pyramid) and the Gouraud (vertex colour) variants. Finite colours only: the NaN
+// rule of the colour arithmetic (nanrule.rs) is not restated.
+
+// mesh.rs:11-31
+struct FwGrid {
+    long w, h;
+    float rx, ry, sx, sy;
+};
+static FwGrid fw_grid(long w, long h, long fw, long fh, float amount) {
+    FwGrid g;
+    g.w = w;
+    g.h = h;
+    g.rx = (float)((int32_t)fw - 1) / (float)((int32_t)w - 1);
+    g.ry = (float)((int32_t)fh - 1) / (float)((int32_t)h - 1);
+    g.sx = amount / g.rx;
+    g.sy = amount / g.ry;
+    return g;
+}
+
+// mesh.rs:33-52: x' = fx * sx + c, y' = fy * sy + r; rows 1.. and columns 1.. get the tie
+// breaking epsilon as a double addition, narrowed
+static std::vector<float> fw_positions(const FwGrid& g, const float* flow, long fw, long fh) {
+    std::vector<float> pos((size_t)g.w * g.h * 2);
+    for (long r = 0; r < g.h; r++) {
+        float fy = (float)(int32_t)r * g.ry, rf = (float)(int32_t)r;
+        for (long c = 0; c < g.w; c++) {
+            float cf = (float)(int32_t)c, f[2];
+            flow_bilerp(flow, fw, (int32_t)fw - 1, (int32_t)fh - 1, cf * g.rx, fy, f);
+            float px = f[0] * g.sx + cf, py = f[1] * g.sy + rf;
+            if (r != 0 && c >= 1) {
+                px = (float)((double)px + 0.0001);
+                py = (float)((double)py + 0.0001);
+            }
+            pos[(r * g.w + c) * 2] = px;
+            pos[(r * g.w + c) * 2 + 1] = py;
+        }
+    }
+    return pos;
+}
+
+// mesh.rs:54-88: the mip level of vertex (c, r) from the distances to its 8 displaced neighbours
+static float fw_mip_level(const FwGrid& g, const float* flow, long fw, long fh, int32_t c, int32_t r) {
+    int32_t w = (int32_t)g.w, h = (int32_t)g.h;
+    int32_t up = r - 1 < 0 ? 1 : r - 1, down = h - 1 < r + 1 ? h - 2 : r + 1;
+    int32_t left = c - 1 < 0 ? 1 : c - 1, right = w - 1 < c + 1 ? w - 2 : c + 1;
+    float f[2];
+    flow_bilerp(flow, fw, (int32_t)fw - 1, (int32_t)fh - 1, (float)c * g.rx, (float)r * g.ry, f);
+    float xc = f[0] * g.sx + (float)c, yc = f[1] * g.sy + (float)r;
+    auto len = [&](int32_t ci, int32_t ri) {
+        float n[2];
+        flow_bilerp(flow, fw, (int32_t)fw - 1, (int32_t)fh - 1, (float)ci * g.rx, (float)ri * g.ry, n);
+        float dx = (n[0] * g.sx + (float)ci) - xc, dy = (n[1] * g.sy + (float)ri) - yc;
+        return sqrtf(dy * dy + dx * dx);
+    };
+    float ru = len(right, up), lu = len(left, up), ld = len(left, down), rd = len(right, down);
+    float u = len(c, up), dn = len(c, down), l = len(left, r), rr = len(right, r);
+    float diag = ((ru + lu) + ld) + rd;
+    double t = (double)diag * 0.7071067811865476;
+    t += (double)u;
+    t += (double)dn;
+    t += (double)l;
+    t += (double)rr;
+    float m = (float)((double)(float)t * 0.125);
+    float inv_ln2 = 1.0f / logf(2.0f);
+    return -(logf(m) * inv_ln2) + 0.0f;
+}
+
+// pyramid.rs:20-35
+static int fw_level_count(long w, long h) {
+    int n = 1;
+    int32_t a = (int32_t)std::max(w, 2L), b = (int32_t)std::max(h, 2L);
+    for (;;) {
+        a /= 2;
+        if (a < 2) break;
+        b /= 2;
+        if (b < 2) break;
+        n++;
+    }
+    return n;
+}
+
+// pyramid.rs:37-95: levels 1.. (blur5 of the previous level with the filters.rs edge rule, rows
+// then columns, right tap first; then the corner aligned resample with the warp stores)
+struct FwPyramid {
+    int count = 1;
+    std::vector<std::vector<uint8_t>> levels;  // levels 1..count-1
+    std::vector<std::pair<long, long>> dims;   // dims[0] = the source
+};
+template <class P>
+static void fw_resample(P* dst, long dw, long dh, const P* src, long sw, long sh) {
+    float sx = ((float)(int32_t)sw - 1.0f) / (float)((int32_t)dw - 1);
+    float sy = ((float)(int32_t)sh - 1.0f) / (float)((int32_t)dh - 1);
+    for (long y = 0; y < dh; y++) {
+        float fy = (float)(int32_t)y * sy;
+        for (long x = 0; x < dw; x++) {
+            float v[4];
+            bilerp4(src, sw, (int32_t)sw - 1, (int32_t)sh - 1, (float)(int32_t)x * sx, fy, v);
+            for (int k = 0; k < 4; k++) dst[(y * dw + x) * 4 + k] = store<P>(v[k]);
+        }
+    }
+}
+template <class P>
+static FwPyramid fw_pyramid(const P* src, long w, long h) {
+    FwPyramid p;
+    p.count = fw_level_count(w, h);
+    p.dims.push_back({w, h});
+    long cw = std::max(w, 2L), ch = std::max(h, 2L);
+    const std::vector<float> taps(BINOMIAL5, BINOMIAL5 + 3);
+    for (int i = 1; i < p.count; i++) {
+        const P* prev = i == 1 ? src : (const P*)p.levels[i - 2].data();
+        long pw = p.dims[i - 1].first, ph = p.dims[i - 1].second;
+        std::vector<P> tmp(prev, prev + pw * ph * 4);
+        blur_plane(tmp.data(), pw, ph, taps, false);
+        cw = std::max(cw / 2, 2L);
+        ch = std::max(ch / 2, 2L);
+        std::vector<uint8_t> lvl((size_t)cw * ch * 4 * sizeof(P));
+        fw_resample((P*)lvl.data(), cw, ch, tmp.data(), pw, ph);
+        p.levels.push_back(std::move(lvl));
+        p.dims.push_back({cw, ch});
+    }
+    return p;
+}
+// pyramid.rs:97-110: one bilinear tap at (w1 * u, h1 * v)
+template <class P>
+static void fw_tap(const P* img, long w, long h, float u, float v, float out[4]) {
+    int32_t w1 = (int32_t)w - 1, h1 = (int32_t)h - 1;
+    bilerp4(img, w, w1, h1, (float)w1 * u, (float)h1 * v, out);
+}
+// pyramid.rs:68-92: the blend of the two levels around d
+template <class P>
+static void fw_sample(const FwPyramid& p, const P* src, float u, float v, float d, float out[4]) {
+    auto level = [&](int i) { return i == 0 ? src : (const P*)p.levels[i - 1].data(); };
+    if (0.0f < d) {
+        if (d < (float)(p.count - 1)) {
+            int32_t i = cvtt(d);
+            float frac = d - (float)i;
+            float a[4], b[4];
+            fw_tap(level(i), p.dims[i].first, p.dims[i].second, u, v, a);
+            fw_tap(level(i + 1), p.dims[i + 1].first, p.dims[i + 1].second, u, v, b);
+            for (int k = 0; k < 4; k++) out[k] = (b[k] - a[k]) * frac + a[k];
+            return;
+        }
+        int last = p.count - 1;
+        fw_tap(level(last), p.dims[last].first, p.dims[last].second, u, v, out);
+        return;
+    }
+    fw_tap(src, p.dims[0].first, p.dims[0].second, u, v, out);
+}
+
+// raster.rs: a vertex with its position and payload (u, v, d of the textured variant; the four
+// colours of the Gouraud variant, whose two passes over the same pixels are carried together)
+struct FwVtx {
+    float x, y;
+    float p[4];
+};
+struct FwEdge {
+    float ay = 0, by = 0, x = 0, dx = 0;
+    float p[4] = {0, 0, 0, 0}, dp[4] = {0, 0, 0, 0};
+};
+
+// raster.rs:59-86
+static inline void fw_edge_init(const FwVtx& a, const FwVtx& b, float y, FwEdge e[2], uint32_t& count) {
+    if (a.y <= y && y < b.y) {
+        int slot = count != 0 ? 1 : 0;
+        count++;
+        float dx = b.x - a.x;
+        float inv = 1.0f / (b.y - a.y);
+        float t = ((y - 1.0f) - a.y) * inv;
+        FwEdge& E = e[slot];
+        E.ay = a.y;
+        E.by = b.y;
+        E.x = dx * t + a.x;
+        E.dx = dx * inv;
+        for (int k = 0; k < 4; k++) {
+            float dk = b.p[k] - a.p[k];
+            E.p[k] = dk * t + a.p[k];
+            E.dp[k] = dk * inv;
+        }
+    }
+}
+
+// raster.rs:88-112: ties keep the given order except v2.y == v0.y < v1.y -> v2, v0, v1
+static inline void fw_sort_by_y(const FwVtx v[3], FwVtx o[3]) {
+    float y0 = v[0].y, y1 = v[1].y, y2 = v[2].y;
+    int a, b, c;
+    if (y1 < y0) {
+        if (y2 < y0) {
+            if (y2 < y1) { a = 2; b = 1; c = 0; } else { a = 1; b = 2; c = 0; }
+        } else { a = 1; b = 0; c = 2; }
+    } else if (y2 < y1) {
+        if (y0 < y2) { a = 0; b = 2; c = 1; } else { a = 2; b = 0; c = 1; }
+    } else { a = 0; b = 1; c = 2; }
+    o[0] = v[a];
+    o[1] = v[b];
+    o[2] = v[c];
+}
+
+// raster.rs:114-151: one span of the main pass
+template <class Hit>
+static inline void fw_span(const FwEdge& l, const FwEdge& r, float y, int32_t w1, const Hit& hit) {
+    int32_t x0 = std::max(cvtt(ceilf(l.x)), 0);
+    int32_t x1 = std::min(cvtt(floorf(r.x)), w1);
+    float wid = r.x - l.x;
+    if (x0 > x1) return;
+    if (!((double)wid > 1e-08)) return;
+    int32_t yi = cvtt(y);
+    float p[4];
+    if (x0 == x1) {
+        float t = ((float)x0 - l.x) * (1.0f / wid);
+        for (int k = 0; k < 4; k++) p[k] = (r.p[k] - l.p[k]) * t + l.p[k];
+        hit(x0, yi, p);
+        return;
+    }
+    float inv = 1.0f / wid, dp[4];
+    float t = ((float)x0 - l.x) * inv;
+    for (int k = 0; k < 4; k++) {
+        dp[k] = r.p[k] - l.p[k];
+        p[k] = dp[k] * t + l.p[k];
+    }
+    int32_t x = x0;
+    while (x < x1) {
+        hit(x, yi, p);
+        for (int k = 0; k < 4; k++) p[k] = p[k] + dp[k] * inv;
+        x++;
+    }
+    hit(x, yi, p);
+}
+
+// raster.rs:153-300: rows y_start..=y_end with closed spans, then the degenerate last row pass.
+// Returns false when that pass would write outside the image (nothing is written for it).
+template <class Hit>
+static bool fw_triangle(const FwVtx tri[3], int32_t w1, int32_t h1, const Hit& hit) {
+    float w1f = (float)w1, h1f = (float)h1;
+    const FwVtx* t = tri;
+    bool ok = (0.0f <= t[0].x || 0.0f <= t[1].x || 0.0f <= t[2].x) && (0.0f <= t[0].y || 0.0f <= t[1].y || 0.0f <= t[2].y) &&
+              (t[0].x <= w1f || t[1].x <= w1f || t[2].x <= w1f) && (t[0].y <= h1f || t[1].y <= h1f || t[2].y <= h1f);
+    if (!ok) return true;
+    FwVtx vs[3];
+    fw_sort_by_y(tri, vs);
+    float y = ceilf(vs[0].y), y_end = floorf(vs[2].y);
+    if (y < 0.0f) y = 0.0f;
+    if (h1f < y_end) y_end = h1f;
+    if (y <= y_end) {
+        int vi = 0;
+        uint32_t count = 0;
+        FwEdge e[2];
+        for (;;) {
+            if (vi < 3) {
+                for (;;) {
+                    if (y < vs[vi].y) break;
+                    if (count == 1) {
+                        if (e[0].ay <= y && e[0].by <= y) count = 0;
+                    } else if (count == 2) {
+                        if (y < e[1].ay || y < e[1].by) {
+                            if (!(y < e[0].ay || y < e[0].by)) {
+                                e[0] = e[1];
+                                count = 1;
+                            }
+                        } else if (e[0].ay <= y && e[0].by <= y) {
+                            count = 0;
+                        } else {
+                            count = 1;
+                        }
+                    }
+                    static const int NEXT[3][2] = {{1, 2}, {2, 0}, {0, 1}};
+                    fw_edge_init(vs[vi], vs[NEXT[vi][0]], y, e, count);
+                    fw_edge_init(vs[vi], vs[NEXT[vi][1]], y, e, count);
+                    vi++;
+                    if (vi >= 3) break;
+                }
+            }
+            for (int k = 0; k < 2; k++) {
+                e[k].x += e[k].dx;
+                for (int q = 0; q < 4; q++) e[k].p[q] = e[k].p[q] + e[k].dp[q];
+            }
+            if (count == 2) {
+                if (e[1].x < e[0].x) fw_span(e[1], e[0], y, w1, hit);
+                else fw_span(e[0], e[1], y, w1, hit);
+            }
+            y += 1.0f;
+            if (!(y <= y_end)) break;
+        }
+    }
+    const FwVtx& mid = vs[1];
+    const FwVtx& bot = vs[2];
+    bool inside = true;
+    if (mid.y == h1f && mid.y == bot.y) {
+        const FwVtx& l = bot.x <= mid.x ? bot : mid;
+        const FwVtx& r = bot.x <= mid.x ? mid : bot;
+        float yy = l.y, yt = (float)cvtt(yy);
+        if (0.0f <= yt && yt <= h1f && !((double)(yy - yt) > 0.01)) {
+            int32_t x0 = std::max(cvtt(ceilf(l.x)), 0);
+            int32_t x1 = std::min(cvtt(floorf(r.x)), w1);
+            float wid = r.x - l.x;
+            if ((double)wid > 1e-05) {
+                float inv = 1.0f / wid;
+                int32_t yi = cvtt(yt);
+                auto emit = [&](int32_t x, const float* p) {
+                    if (x < 0 || x > w1 || yi < 0 || yi > h1) inside = false;
+                    else hit(x, yi, p);
+                };
+                float p[4], dp[4];
+                if (x0 == x1) {
+                    float tt = ((float)x0 - l.x) * inv;
+                    for (int k = 0; k < 4; k++) p[k] = (r.p[k] - l.p[k]) * tt + l.p[k];
+                    emit(x0, p);
+                } else {
+                    float tt = ((float)x0 - l.x) * inv;
+                    for (int k = 0; k < 4; k++) {
+                        dp[k] = r.p[k] - l.p[k];
+                        p[k] = dp[k] * tt + l.p[k];
+                    }
+                    int32_t x = x0;
+                    while (x < x1) {
+                        emit(x, p);
+                        for (int k = 0; k < 4; k++) p[k] = p[k] + dp[k] * inv;
+                        x++;
+                    }
+                    emit(x, p);
+                }
+            }
+        }
+    }
+    return inside;
+}
+
+// forward.rs:104-128: the running mean with the saturating u8 hit counter
+static inline void fw_fold(float* acc, uint8_t& n, const float c[4]) {
+    if (n == 0) {
+        for (int k = 0; k < 4; k++) acc[k] = c[k] + 0.0f;
+        n = 1;
+    } else if (n == 0xff) {
+        for (int k = 0; k < 4; k++) acc[k] = (acc[k] * 255.0f + c[k]) * 0.00390625f + 0.0f;
+    } else {
+        float nf = (float)n;
+        float r = (float)(1.0 / ((double)nf + 1.0));
+        for (int k = 0; k < 4; k++) acc[k] = (nf * acc[k] + c[k]) * r + 0.0f;
+        n = (uint8_t)(n + 1);
+    }
+}
+
+// forward.rs:183-195
+static bool fw_flow_is_zero(const float* flow, long n) {
+    for (long i = 0; i < n; i++) {
+        float v = flow[i];
+        float a = v < 0.0f ? -v : v;
+        if (0.0001 < (double)a) return false;
+    }
+    return true;
+}
+
+template <class P> static inline P fw_convert(float a);
+template <> inline uint8_t fw_convert<uint8_t>(float a) { return (uint8_t)cvtt64f(a + 0.5f); }
+template <> inline uint16_t fw_convert<uint16_t>(float a) { return (uint16_t)cvtt64f(a + 0.5f); }
+template <> inline float fw_convert<float>(float a) { return a + 0.0f; }
+
+// forward.rs:229-320 (textured, pyramid on) and :327-373 (Gouraud): cells row-major, T1 = (v00,
+// v01, v10) then T2 = (v11, v01, v10); the running mean, the conversion and the 255/0 coverage
+template <class P>
+static int fw_run(P* out, const P* src, long w, long h, const float* flow, long fw, long fh, float amount, bool gouraud,
+                  uint8_t* cov, std::string& err) {
+    FwGrid g = fw_grid(w, h, fw, fh, amount);
+    std::vector<float> pos = fw_positions(g, flow, fw, fh);
+    int32_t w1 = (int32_t)w - 1, h1 = (int32_t)h - 1;
+    float w1f = (float)w1, h1f = (float)h1;
+    std::vector<float> acc((size_t)w * h * 4, 0.0f);
+    std::vector<uint8_t> cnt((size_t)w * h, 0);
+    FwPyramid pyr;
+    std::vector<float> mips;
+    if (!gouraud) {
+        pyr = fw_pyramid(src, w, h);
+        if (pyr.count > 1) {
+            mips.resize((size_t)w * h);
+            for (long r = 0; r < h; r++)
+                for (long c = 0; c < w; c++) mips[r * w + c] = fw_mip_level(g, flow, fw, fh, (int32_t)c, (int32_t)r);
+        }
+    }
+    auto vtx = [&](long r, long c) {
+        FwVtx v;
+        v.x = pos[(r * w + c) * 2];
+        v.y = pos[(r * w + c) * 2 + 1];
+        if (gouraud) {
+            for (int k = 0; k < 4; k++) v.p[k] = to_f(src[(r * w + c) * 4 + k]);
+        } else {
+            v.p[0] = (float)(int32_t)c / w1f;
+            v.p[1] = (float)(int32_t)r / h1f;
+            v.p[2] = mips.empty() ? 0.0f : mips[r * w + c];
+            v.p[3] = 0.0f;
+        }
+        return v;
+    };
+    auto hit = [&](int32_t x, int32_t y, const float* p) {
+        long i = (long)y * w + x;
+        float c[4];
+        if (gouraud) {
+            for (int k = 0; k < 4; k++) c[k] = p[k];
+        } else if (0.0f < p[2]) {
+            fw_sample(pyr, src, p[0], p[1], p[2], c);
+        } else {
+            fw_tap(src, w, h, p[0], p[1], c);
+        }
+        fw_fold(&acc[i * 4], cnt[i], c);
+    };
+    bool inside = true;
+    for (long r = 0; r < h - 1; r++) {
+        for (long c = 0; c < w - 1; c++) {
+            FwVtx v00 = vtx(r, c), v01 = vtx(r, c + 1), v10 = vtx(r + 1, c), v11 = vtx(r + 1, c + 1);
+            FwVtx t1[3] = {v00, v01, v10}, t2[3] = {v11, v01, v10};
+            inside = fw_triangle(t1, w1, h1, hit) && inside;
+            inside = fw_triangle(t2, w1, h1, hit) && inside;
+        }
+    }
+    if (!inside) {
+        err = "forward warp: the degenerate last row pass of the rasteriser would write outside the image, which is not "
+              "supported";
+        return TWX_E_UNSUPPORTED;
+    }
+    for (long i = 0; i < w * h * 4; i++) out[i] = fw_convert<P>(acc[i]);
+    if (cov)
+        for (long i = 0; i < w * h; i++) cov[i] = cnt[i] != 0 ? 0xff : 0;
+    return TWX_OK;
+}
+
+static int forward_warp(void* out, const void* src, long w, long h, int fmt, const float* flow, long fw, long fh,
+                        float amount, bool gouraud, uint8_t* cov, std::string& err) {
+    if (w < 2 || h < 2 || fw < 2 || fh < 2) {
+        err = "forward warp needs images and a flow of at least 2x2";
+        return TWX_E_MISMATCH;
+    }
+    float a = 0.0f > amount ? -amount : amount;
+    if (1e-05 > (double)a || fw_flow_is_zero(flow, fw * fh * 2)) {
+        memcpy(out, src, image_bytes(w, h, fmt));
+        if (cov) memset(cov, 0xff, (size_t)w * h);
+        return TWX_OK;
+    }
+    if (fmt == TWX_FMT_U8) return fw_run((uint8_t*)out, (const uint8_t*)src, w, h, flow, fw, fh, amount, gouraud, cov, err);
+    if (fmt == TWX_FMT_U16) return fw_run((uint16_t*)out, (const uint16_t*)src, w, h, flow, fw, fh, amount, gouraud, cov, err);
+    return fw_run((float*)out, (const float*)src, w, h, flow, fw, fh, amount, gouraud, cov, err);
+}
+
+// The compositions of warping codes 2 and 3 (compose/vector.rs:117-147, forward.rs:391-572):
+// each frame pushed with its own flow (A with F_AB, B with F_BA; Nearest the one flow), code 3
+// without coverage (holes transparent), code 2 with coverage, the Smart Blend composite and the
+// hole fill. `gouraud` selects the forward warp variant of the stage set (stages::GouraudForward).
+static int compose_forward(int code, int interp, float wa, const void* a, const void* b, long w, long h, int fmt,
+                           const float* f_ab, const float* f_ba, long fw, long fh, bool above_first, bool gouraud,
+                           void* out, std::string& err) {
+    size_t bytes = image_bytes(w, h, fmt);
+    float frac = 1.0f - wa;
+    bool fill = code == 2;
+    int st;
+    if (interp == 0) {
+        const float* flow = (0.5f <= wa) ? f_ab : f_ba;
+        std::vector<uint8_t> cov(fill ? w * h : 0);
+        if (!(0.5f > wa)) st = forward_warp(out, a, w, h, fmt, flow, fw, fh, 1.0f - wa, gouraud, fill ? cov.data() : nullptr, err);
+        else st = forward_warp(out, b, w, h, fmt, flow, fw, fh, wa, gouraud, fill ? cov.data() : nullptr, err);
+        if (st || !fill) return st;
+        for (auto& c : cov) c = (uint8_t)(255 - c);
+        return fill_holes(out, w, h, fmt, cov.data(), above_first, err);
+    }
+    std::vector<uint8_t> ta(bytes, 0), tb(bytes, 0);
+    std::vector<uint8_t> cov1(fill ? w * h : 0), cov2(fill ? w * h : 0);
+    if ((st = forward_warp(ta.data(), a, w, h, fmt, f_ab, fw, fh, frac, gouraud, fill ? cov1.data() : nullptr, err))) return st;
+    if ((st = forward_warp(tb.data(), b, w, h, fmt, f_ba, fw, fh, wa, gouraud, fill ? cov2.data() : nullptr, err))) return st;
+    if (interp == 2) {
+        sharpen_step_with(ta.data(), a, w, h, fmt, frac, above_first);
+        sharpen_step_with(tb.data(), b, w, h, fmt, wa, above_first);
+    }
+    if (!fill) {
+        blend(out, ta.data(), tb.data(), w, h, fmt, frac);
+        return TWX_OK;
+    }
+    smart_blend(out, ta.data(), tb.data(), w, h, fmt, frac);
+    return smart_finish(out, w, h, fmt, cov1, cov2, above_first, err);
+}
+
+// ------------------------------------------------------------------------------------------
 // Time model (timeline.rs, curves.rs, fixed.rs, frontend.rs)
 
 static inline int32_t float_to_fixed(double f) {  // fixed.rs:8-10 (Rust `as i32` saturates, NaN 0)
@@ -2049,6 +2528,52 @@ int cref_inv_warp(void* out, const void* src, int w, int h, int fmt, const float
     return st;
 }
 
+// The forward warp (forward.rs:229-373): `pyramid` 1 the textured variant, 0 the Gouraud variant;
+// `cov` (w * h bytes) may be null.
+int cref_forward_warp(void* out, const void* src, int w, int h, int fmt, const float* flow, int fw, int fh, float amount,
+                      int pyramid, uint8_t* cov, char* err, size_t errlen) {
+    std::string e;
+    int st = forward_warp(out, src, w, h, fmt, flow, fw, fh, amount, pyramid == 0, cov, e);
+    if (st) set_err(err, errlen, e);
+    return st;
+}
+
+// One triangle through the scan converter (raster.rs:153-300): the hits (x, y, payload[4]) into
+// `hits` (at most `cap`), their count returned; *outside set when the last row pass would write
+// outside the image.
+int cref_fw_triangle(const float* vtx /* 3 x (x, y, p0..p3) */, int w1, int h1, float* hits, int cap, int* outside) {
+    FwVtx t[3];
+    for (int i = 0; i < 3; i++) {
+        t[i].x = vtx[i * 6];
+        t[i].y = vtx[i * 6 + 1];
+        for (int k = 0; k < 4; k++) t[i].p[k] = vtx[i * 6 + 2 + k];
+    }
+    int n = 0;
+    bool inside = fw_triangle(t, w1, h1, [&](int32_t x, int32_t y, const float* p) {
+        if (n < cap) {
+            hits[n * 6] = (float)x;
+            hits[n * 6 + 1] = (float)y;
+            for (int k = 0; k < 4; k++) hits[n * 6 + 2 + k] = p[k];
+        }
+        n++;
+    });
+    *outside = inside ? 0 : 1;
+    return n;
+}
+
+// The vertex positions (mesh.rs:33-52) and mip levels (mesh.rs:54-88) of a w x h grid.
+void cref_fw_mesh(const float* flow, int fw, int fh, int w, int h, float amount, float* pos, float* mips) {
+    FwGrid g = fw_grid(w, h, fw, fh, amount);
+    std::vector<float> p = fw_positions(g, flow, fw, fh);
+    memcpy(pos, p.data(), p.size() * sizeof(float));
+    if (mips)
+        for (int r = 0; r < h; r++)
+            for (int c = 0; c < w; c++) mips[r * w + c] = fw_mip_level(g, flow, fw, fh, c, r);
+}
+
+// The running mean step (forward.rs:104-128) on one pixel.
+void cref_fw_fold(float* acc, uint8_t* n, const float* c) { fw_fold(acc, *n, c); }
+
 // The fading warp (hold off) with its coverage image.
 int cref_inv_warp_cov(void* out, const void* src, int w, int h, int fmt, const float* flow, int fw, int fh, float amount,
                       uint8_t* cov, char* err, size_t errlen) {
@@ -2098,17 +2623,19 @@ uint32_t cref_sat_u16(float v) { return sat_u16(v); }
 // `above_first` is the vertical tap order of the Weighted Blend blur (the stage set's
 // vertical_taps).
 int cref_compose_vectors(int interp, float wa, const void* a, const void* b, int w, int h, int fmt, const float* f_ab,
-                         const float* f_ba, int fw, int fh, int warp_mode, int above_first, void* out, char* err,
-                         size_t errlen) {
+                         const float* f_ba, int fw, int fh, int warp_mode, int above_first, int gouraud, void* out,
+                         char* err, size_t errlen) {
     std::string e;
     float frac = 1.0f - wa;
     size_t bytes = image_bytes(w, h, fmt);
     int st;
     if (warp_mode == 1 && interp >= 0 && interp <= 2) {
         st = compose_smart(interp, wa, a, b, w, h, fmt, f_ab, f_ba, fw, fh, above_first != 0, out, e);
+    } else if ((warp_mode == 2 || warp_mode == 3) && interp >= 0 && interp <= 2) {
+        st = compose_forward(warp_mode, interp, wa, a, b, w, h, fmt, f_ab, f_ba, fw, fh, above_first != 0, gouraud != 0, out, e);
     } else if (warp_mode != 0) {
-        e = "oracle: only the warping codes 0 and 1 are restated";
-        st = TWX_E_UNSUPPORTED;
+        e = "warping code outside 0..3";
+        st = TWX_E_MISMATCH;
     } else if (interp == 0) {
         if (0.5f <= wa) st = inv_warp(out, a, w, h, fmt, f_ba, fw, fh, frac, e);
         else st = inv_warp(out, b, w, h, fmt, f_ab, fw, fh, wa, e);

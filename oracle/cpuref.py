@@ -285,15 +285,15 @@ def sat_u16(v: float) -> int:
     return lib().cref_sat_u16(ctypes.c_float(v))
 
 
-def compose_vectors(interp, wa, a, b, w, h, fmt, f_ab, f_ba, fw, fh, above_first=True, warp_mode=0):
+def compose_vectors(interp, wa, a, b, w, h, fmt, f_ab, f_ba, fw, fh, above_first=True, warp_mode=0, gouraud=True):
     out = np.zeros_like(a)
     err = ctypes.create_string_buffer(256)
     nul = np.zeros(2, np.float32)
     fa = np.ascontiguousarray(f_ab if f_ab is not None else nul, np.float32)
     fb = np.ascontiguousarray(f_ba if f_ba is not None else nul, np.float32)
     st = lib().cref_compose_vectors(interp, ctypes.c_float(wa), _p(np.ascontiguousarray(a)), _p(np.ascontiguousarray(b)),
-                                    w, h, fmt, _p(fa), _p(fb), fw, fh, int(warp_mode), int(above_first), _p(out),
-                                    err, 256)
+                                    w, h, fmt, _p(fa), _p(fb), fw, fh, int(warp_mode), int(above_first), int(gouraud),
+                                    _p(out), err, 256)
     if st:
         raise RuntimeError(err.value.decode())
     return out
@@ -347,6 +347,44 @@ def sharpen_step(frame, original, w, h, fmt, t, above_first=False):
     f = np.ascontiguousarray(frame).copy()
     lib().cref_sharpen_step(_p(f), _p(np.ascontiguousarray(original)), w, h, fmt, ctypes.c_float(t), int(above_first))
     return f
+
+
+# ---- the forward warp (forward.rs, forward/mesh.rs, forward/raster.rs, forward/pyramid.rs) ----
+
+def forward_warp(src, w, h, fmt, flow, fw, fh, amount, pyramid=False, coverage=True):
+    """(status, out, coverage or None); `pyramid` selects the textured variant, else Gouraud."""
+    out = np.zeros_like(src)
+    cov = np.zeros(w * h, np.uint8) if coverage else None
+    err = ctypes.create_string_buffer(256)
+    st = lib().cref_forward_warp(_p(out), _p(np.ascontiguousarray(src)), w, h, fmt,
+                                 _p(np.ascontiguousarray(flow, np.float32)), fw, fh, ctypes.c_float(amount),
+                                 int(pyramid), _p(cov) if cov is not None else None, err, 256)
+    return st, out, cov
+
+
+def fw_triangle(vtx, w1, h1, cap=4096):
+    """Hits of one triangle: vtx is 3 x (x, y, p0, p1, p2, p3); returns (hits (n x 6), outside)."""
+    v = np.ascontiguousarray(vtx, np.float32).reshape(3, 6)
+    hits = np.zeros((cap, 6), np.float32)
+    outside = ctypes.c_int32(0)
+    n = lib().cref_fw_triangle(_p(v), int(w1), int(h1), _p(hits), cap, ctypes.byref(outside))
+    return hits[:min(n, cap)], bool(outside.value)
+
+
+def fw_mesh(flow, fw, fh, w, h, amount, mips=False):
+    pos = np.zeros((h, w, 2), np.float32)
+    m = np.zeros((h, w), np.float32) if mips else None
+    lib().cref_fw_mesh(_p(np.ascontiguousarray(flow, np.float32)), fw, fh, w, h, ctypes.c_float(amount), _p(pos),
+                       _p(m) if m is not None else None)
+    return pos, m
+
+
+def fw_fold(acc, n, c):
+    """One running-mean step: acc (4 floats), n (hit count), colour c; returns (acc, n)."""
+    a = np.ascontiguousarray(acc, np.float32).copy()
+    cnt = ctypes.c_uint8(n)
+    lib().cref_fw_fold(_p(a), ctypes.byref(cnt), _p(np.ascontiguousarray(c, np.float32)))
+    return a, cnt.value
 
 
 # ---- the fading warp, Smart Blend and the hole fill (warp.rs, smartblend.rs, holefill.rs) ----

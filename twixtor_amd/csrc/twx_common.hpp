@@ -116,7 +116,7 @@ struct FillPlane {
 enum ProfFam {
     F_INGEST, F_LUMA, F_PYR_BLUR, F_PYR_RESAMPLE, F_GRAD_TENSOR, F_BM_TRANSPOSE,
     F_SOLVE_INIT, F_CALC_B, F_NEW_DISP, F_BM_ROWS, F_BM_COLS, F_RANGE, F_UPSAMPLE, F_NEGATE, F_FRAMES,
-    F_WB_MASK, F_WB_SHARPEN, F_FILL_PULL, F_FILL_PUSH, F_FILL_IO, F_DIFF_FWD, F_DIFF_BWD, F_NFAM
+    F_WB_MASK, F_WB_SHARPEN, F_FILL_PULL, F_FILL_PUSH, F_FILL_IO, F_DIFF_FWD, F_DIFF_BWD, F_DIFF_GATHER, F_NFAM
 };
 extern const char* const PROF_NAMES[F_NFAM];
 

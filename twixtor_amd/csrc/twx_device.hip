@@ -154,7 +154,7 @@ const char* const PROF_NAMES[F_NFAM] = {"ingest", "luma", "pyr_blur5_band", "pyr
                                         "tensor_gauss", "bm_transpose", "solve_init", "calc_b_gauss", "new_disp_gauss",
                                         "bm_rows", "bm_cols", "range", "upsample", "negate", "frames",
                                         "wb_mask", "wb_sharpen_blend", "fill_pull", "fill_push", "fill_io",
-                                        "warp_blend_fwd", "warp_blend_bwd"};
+                                        "warp_blend_fwd", "warp_blend_bwd", "warp_blend_flow_gather"};
 thread_local Prof* g_prof = nullptr;
 
 hipEvent_t Prof::take() {

@@ -337,10 +337,12 @@ int warp_blend_run(hipStream_t s, bool backward, int C, int W, int H, const floa
         TWX_LAUNCH(s, F_DIFF_FWD, 3.0 * img + 2.0 * fl, k_wb_forward<<<grd, blk, 0, s>>>(P));
         return TWX_OK;
     }
-    TWX_LAUNCH(s, F_DIFF_BWD, 4.0 * img + 2.0 * fl + 16.0 * (double)W * H, k_wb_backward<<<grd, blk, 0, s>>>(P));
+    // reads A, B and the upstream gradient, writes dL/dA and dL/dB, reads both flows, writes the
+    // two per-pixel flow-value gradients
+    TWX_LAUNCH(s, F_DIFF_BWD, 5.0 * img + 2.0 * fl + 16.0 * (double)W * H, k_wb_backward<<<grd, blk, 0, s>>>(P));
     dim3 fgrd((fw + 255) / 256, fh);
-    if (grad_f_ba) TWX_LAUNCH(s, F_DIFF_BWD, 8.0 * (double)W * H + fl, k_wb_flow_gather<<<fgrd, blk, 0, s>>>(P, 0));
-    if (grad_f_ab) TWX_LAUNCH(s, F_DIFF_BWD, 8.0 * (double)W * H + fl, k_wb_flow_gather<<<fgrd, blk, 0, s>>>(P, 1));
+    if (grad_f_ba) TWX_LAUNCH(s, F_DIFF_GATHER, 8.0 * (double)W * H + fl, k_wb_flow_gather<<<fgrd, blk, 0, s>>>(P, 0));
+    if (grad_f_ab) TWX_LAUNCH(s, F_DIFF_GATHER, 8.0 * (double)W * H + fl, k_wb_flow_gather<<<fgrd, blk, 0, s>>>(P, 1));
     return TWX_OK;
 }
 
