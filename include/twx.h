@@ -125,17 +125,38 @@ int twx_solve_pair(twx_handle* h, const void* a, const void* b, int32_t width, i
 /* The composition of one output frame from its two frames and their flows, mode 1, no 360.
  * `warp_mode` is EngineSettings::warp_mode: 0 the inverse warps with hold edges, 1 the inverse
  * warps without hold edges and with coverage, the Smart Blend composite and the push-pull hole
- * fill of the uncovered pixels (smartblend.rs:323-419); the forward warps (2, 3) are refused.
+ * fill of the uncovered pixels (smartblend.rs:323-419), 3 the forward warps (each frame pushed
+ * with its own flow, holes transparent), 2 the forward warps with coverage, the Smart Blend
+ * composite and the hole fill (compose/vector.rs:117-147, forward.rs:391-572).
  * `interp` 0 Nearest, 1 Blend, 2 Weighted Blend (each warped frame through the sharpening step
  * against its unwarped frame, filters.rs:649-656, before the blend or composite); wa is the
- * weight of A (Selection::wa). Nearest reads only the flow nearest_uses_f_ab names (the other
- * may be NULL). `taps_above_first` is the vertical tap order of the Weighted Blend blur and of
- * the hole fill (the stage set's vertical_taps, stages.rs:60-62): 1 for the retime front-end
- * (frontend.rs:238-241), 0 for the plain CPU stages. Host engine images. */
+ * weight of A (Selection::wa). Nearest reads only the flow nearest_uses_f_ab names
+ * (compose/vector.rs:61-64; the other may be NULL). `taps_above_first` is the vertical tap order
+ * of the Weighted Blend blur and of the hole fill (the stage set's vertical_taps,
+ * stages.rs:60-62): 1 for the retime and vectors-in front-ends (frontend.rs:238-241,
+ * vectors-in/src/glue.rs:242), 0 for the plain CPU stages. `forward_gouraud` is the forward warp
+ * variant of the stage set: 1 the Gouraud variant (stages::GouraudForward, stages.rs:101-130,
+ * which vectors-in and retime-fgbg install), 0 the textured variant (refused with
+ * TWX_E_UNSUPPORTED for warping codes 2 and 3). Host engine images. */
 int twx_compose_vectors(twx_handle* h, int32_t interp, float wa, const void* a, const void* b,
                         int32_t width, int32_t height, int32_t fmt, const float* f_ab,
                         const float* f_ba, int32_t flow_w, int32_t flow_h, int32_t warp_mode,
-                        int32_t taps_above_first, void* out, char* err, size_t errlen);
+                        int32_t taps_above_first, int32_t forward_gouraud, void* out, char* err,
+                        size_t errlen);
+
+/* ---- Backend::forward_warp_with (device/exec/backend.rs:34-36, forward.rs:229-373) ------------ */
+
+/* The forward warp of `src` by `flow` times `amount`: the source pixels as the vertices of a
+ * displaced mesh, its triangles scan converted in cell order, the running mean of the colours per
+ * covered pixel in triangle order. `pyramid` is ForwardOpts::pyramid: 0 the Gouraud variant
+ * (vertex colours), 1 the textured variant (refused with TWX_E_UNSUPPORTED). `coverage` (width *
+ * height bytes: 255 covered, 0 hole) may be NULL. The early outs copy `src` with full coverage.
+ * TWX_E_UNSUPPORTED as the CPU path for the degenerate last row pass that would write outside the
+ * image, and for what this device does not serve: a triangle over 65536 covered pixels, NaN
+ * colours. Host buffers. */
+int twx_forward_warp(twx_handle* h, const void* src, int32_t width, int32_t height, int32_t fmt,
+                     const float* flow, int32_t flow_w, int32_t flow_h, float amount,
+                     int32_t pyramid, void* out, uint8_t* coverage, char* err, size_t errlen);
 
 /* ---- the per-frame render loop of `vfx apply retime` ------------------------------------- */
 

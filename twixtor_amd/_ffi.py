@@ -18,7 +18,7 @@ EXPORTS = [
     "twx_open", "twx_close", "twx_selfcheck", "twx_memory_budget", "twx_flow_geometry", "twx_solve",
     "twx_solve_pair", "twx_compose_vectors", "twx_select", "twx_retime_frames_dev", "twx_retime_batch",
     "twx_retime_frames",
-    "twx_warp_blend", "twx_warp_blend_backward",
+    "twx_forward_warp", "twx_warp_blend", "twx_warp_blend_backward",
     "twx_get_stats", "twx_stream", "twx_profile_enable", "twx_profile_reset", "twx_profile_read",
 ]
 
@@ -89,7 +89,8 @@ def lib():
                                         ctypes.POINTER(I32)]
         L.twx_solve.argtypes = [P, P, P, I32, I32, I32, ctypes.POINTER(FlowSettings), P] + ERR
         L.twx_solve_pair.argtypes = [P, P, P, I32, I32, I32, ctypes.POINTER(FlowSettings), P, P] + ERR
-        L.twx_compose_vectors.argtypes = [P, I32, ctypes.c_float, P, P, I32, I32, I32, P, P, I32, I32, I32, I32, P] + ERR
+        L.twx_compose_vectors.argtypes = [P, I32, ctypes.c_float, P, P, I32, I32, I32, P, P, I32, I32, I32, I32, I32, P] + ERR
+        L.twx_forward_warp.argtypes = [P, P, I32, I32, I32, P, I32, I32, ctypes.c_float, I32, P, P] + ERR
         L.twx_select.argtypes = [ctypes.POINTER(RetimeParams), I32, ctypes.POINTER(Selection)] + ERR
         L.twx_retime_frames_dev.argtypes = [P, ctypes.POINTER(RetimeParams), P, I32, I32, I32, P] + ERR
         L.twx_retime_frames.argtypes = [P, ctypes.POINTER(RetimeParams), P, I32, I32, I32, P] + ERR

@@ -120,10 +120,12 @@ class HipBackend:
 
     # ---- Backend::compose_vectors (warping code 0) ----
     def compose_vectors(self, interp, wa, a, b, width, height, fmt, f_ab, f_ba, fw, fh, above_first=True,
-                        warp_mode=0) -> np.ndarray:
+                        warp_mode=0, gouraud=True) -> np.ndarray:
         """interp 0 Nearest, 1 Blend, 2 Weighted Blend; warp_mode 0 inverse warps, 1 with Smart
-        Blend and hole fill; `above_first` is the vertical tap order of the Weighted Blend blur and
-        of the hole fill (the retime front-end's, frontend.rs:238-241, by default)."""
+        Blend and hole fill, 3 forward warps, 2 forward warps with Smart Blend and hole fill;
+        `above_first` is the vertical tap order of the Weighted Blend blur and of the hole fill (the
+        retime front-end's, frontend.rs:238-241, by default); `gouraud` the forward warp variant
+        (stages::GouraudForward, the one vectors-in and retime-fgbg use)."""
         out = np.zeros_like(a)
         a, b = np.ascontiguousarray(a), np.ascontiguousarray(b)
         fab = None if f_ab is None else np.ascontiguousarray(f_ab, np.float32)
@@ -131,8 +133,22 @@ class HipBackend:
         err = _ffi.errbuf()
         _ffi.check(_ffi.lib().twx_compose_vectors(self._h, interp, ctypes.c_float(wa), _p(a), _p(b), width, height, fmt,
                                                   None if fab is None else _p(fab), None if fba is None else _p(fba),
-                                                  fw, fh, int(warp_mode), int(above_first), _p(out), err, 512), err)
+                                                  fw, fh, int(warp_mode), int(above_first), int(gouraud), _p(out), err,
+                                                  512), err)
         return out
+
+    # ---- Backend::forward_warp_with ----
+    def forward_warp(self, src, width, height, fmt, flow, fw, fh, amount, pyramid=False, coverage=True):
+        """(out, coverage or None) of the forward warp (forward.rs:229-373); `pyramid` False is the
+        Gouraud variant."""
+        src = np.ascontiguousarray(src)
+        out = np.zeros_like(src)
+        cov = np.zeros(width * height, np.uint8) if coverage else None
+        flow = np.ascontiguousarray(flow, np.float32)
+        err = _ffi.errbuf()
+        _ffi.check(_ffi.lib().twx_forward_warp(self._h, _p(src), width, height, fmt, _p(flow), fw, fh, ctypes.c_float(amount),
+                                               int(pyramid), _p(out), None if cov is None else _p(cov), err, 512), err)
+        return out, cov
 
     # ---- the per-frame render loop, batched ----
     def retime_frames(self, cfg: Retime, clip: bytes, n_in: int, first: int, count: int) -> bytes:

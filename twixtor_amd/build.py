@@ -15,7 +15,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libtwx.so")
-SOURCES = ["twx_device.hip", "twx_api.hip", "twx_selfcheck.hip", "twx_diff.hip"]
+SOURCES = ["twx_device.hip", "twx_api.hip", "twx_selfcheck.hip", "twx_diff.hip", "twx_forward.hip"]
 ARCH = os.environ.get("TWX_OFFLOAD_ARCH", "gfx950")
 
 FLAGS = [

@@ -1046,10 +1046,128 @@ int twx_solve_pair(twx_handle* h, const void* a, const void* b, int32_t width, i
     return st;
 }
 
+// The Gouraud forward warp of a device image into `out` (and `cov`, may be null); the flow on the
+// device, with its host copy for the early-out test (forward.rs:245-256). `dflags` is a device int
+// the caller cleared; its bits 8 and 16 are read by the caller after its final synchronisation.
+static int forward_warp_dev(twx_handle* h, DevPool& pool, const void* src, int W, int H, int fmt, const float* d_flow,
+                            const float* h_flow, int fw, int fh, float amount, void* out, uint8_t* cov, int* dflags,
+                            std::string& err) {
+    hipStream_t s = h->stream;
+    const float a = 0.0f > amount ? -amount : amount;
+    if (1e-05 > (double)a || fw_flow_is_zero(h_flow, (long)fw * fh * 2)) {
+        HIPCHECK(hipMemcpyAsync(out, src, image_bytes(W, H, fmt), hipMemcpyDeviceToDevice, s));
+        if (cov) HIPCHECK(hipMemsetAsync(cov, 0xff, (size_t)W * H, s));
+        return TWX_OK;
+    }
+    FwArgs A{};
+    A.src = src;
+    A.out = out;
+    A.cov = cov;
+    A.flow = d_flow;
+    A.W = W;
+    A.H = H;
+    A.fw = fw;
+    A.fh = fh;
+    A.fmt = fmt;
+    A.flags = dflags;
+    fw_grid(A, amount);
+    const long n = (long)W * H;
+    A.pos = pool.alloc<float2>(n);
+    A.counts = pool.alloc<uint32_t>(n);
+    A.fill = pool.alloc<uint32_t>(n);
+    A.offsets = pool.alloc<uint64_t>(n);
+    A.block_totals = pool.alloc<uint64_t>(fw_scan_blocks(n));
+    A.total = pool.alloc<uint64_t>(1);
+    HIPCHECK(hipMemsetAsync(A.counts, 0, n * sizeof(uint32_t), s));
+    HIPCHECK(hipMemsetAsync(A.fill, 0, n * sizeof(uint32_t), s));
+    launch_fw_count(s, A);
+    uint64_t* ht = pool.pinned<uint64_t>(1);
+    int* hf = pool.pinned<int>(1);
+    HIPCHECK(hipMemcpyAsync(ht, A.total, sizeof(uint64_t), hipMemcpyDeviceToHost, s));
+    HIPCHECK(hipMemcpyAsync(hf, dflags, sizeof(int), hipMemcpyDeviceToHost, s));
+    HIPCHECK(hipStreamSynchronize(s));
+    if (*hf & 1) {
+        err = "forward warp: the degenerate last row pass of the rasteriser would write outside the image, which is not "
+              "supported";
+        return TWX_E_UNSUPPORTED;
+    }
+    if (*hf & 2) {
+        err = "forward warp: a triangle of the displaced mesh covers more than 65536 pixels (no device kernel for that)";
+        return TWX_E_UNSUPPORTED;
+    }
+    if (*hf & 4) {
+        err = "forward warp: a NaN colour (the NaN rule of the colour arithmetic has no device kernel)";
+        return TWX_E_UNSUPPORTED;
+    }
+    const uint64_t total = *ht;
+    if (total * 20 > h->budget / 2) {
+        err = "forward warp: the mesh covers too many pixels for the device memory budget";
+        return TWX_E_UNSUPPORTED;
+    }
+    A.rec_t = pool.alloc<uint32_t>(std::max<uint64_t>(total, 1));
+    A.rec_c = pool.alloc<float4>(std::max<uint64_t>(total, 1));
+    launch_fw_emit_fold(s, A, total);
+    return TWX_OK;
+}
+
+static int check_fw_flags(int f, std::string& err) {
+    if (f & 24) {
+        err = "forward warp: the device hit lists were inconsistent";
+        return TWX_E_DEVICE;
+    }
+    return TWX_OK;
+}
+
+int twx_forward_warp(twx_handle* h, const void* src, int32_t width, int32_t height, int32_t fmt, const float* flow,
+                     int32_t flow_w, int32_t flow_h, float amount, int32_t pyramid, void* out, uint8_t* coverage,
+                     char* err, size_t errlen) {
+    std::string e;
+    int st = TWX_OK;
+    try {
+        Guard g(h, st, e);
+        if (g.ok()) {
+            if (width < 2 || height < 2 || flow_w < 2 || flow_h < 2 || fmt < 0 || fmt > 2) {
+                e = "forward warp needs images and a flow of at least 2x2";
+                st = TWX_E_MISMATCH;
+            } else if (pyramid) {
+                e = "forward warp: the textured variant (ForwardOpts::pyramid) has no device kernel here; the Gouraud "
+                    "variant has";
+                st = TWX_E_UNSUPPORTED;
+            } else {
+                hipStream_t s = h->stream;
+                DevPool pool(h);
+                const size_t fb = image_bytes(width, height, fmt), fn = (size_t)flow_w * flow_h * 2;
+                uint8_t* img = pool.alloc<uint8_t>(fb * 2);
+                float* fl = pool.alloc<float>(fn);
+                uint8_t* cov = coverage ? pool.alloc<uint8_t>((size_t)width * height) : nullptr;
+                int* flags = pool.alloc<int>(1);
+                HIPCHECK(hipMemcpyAsync(img, src, fb, hipMemcpyHostToDevice, s));
+                HIPCHECK(hipMemcpyAsync(fl, flow, fn * 4, hipMemcpyHostToDevice, s));
+                HIPCHECK(hipMemsetAsync(flags, 0, sizeof(int), s));
+                st = forward_warp_dev(h, pool, img, width, height, fmt, fl, flow, flow_w, flow_h, amount, img + fb, cov,
+                                      flags, e);
+                if (!st) {
+                    int* hf = pool.pinned<int>(1);
+                    HIPCHECK(hipMemcpyAsync(out, img + fb, fb, hipMemcpyDeviceToHost, s));
+                    if (cov) HIPCHECK(hipMemcpyAsync(coverage, cov, (size_t)width * height, hipMemcpyDeviceToHost, s));
+                    HIPCHECK(hipMemcpyAsync(hf, flags, sizeof(int), hipMemcpyDeviceToHost, s));
+                    HIPCHECK(hipStreamSynchronize(s));
+                    st = check_fw_flags(*hf, e);
+                }
+            }
+        }
+    } catch (const HipError& x) {
+        e = x.msg;
+        st = TWX_E_DEVICE;
+    }
+    if (st) set_err(err, errlen, e);
+    return st;
+}
+
 int twx_compose_vectors(twx_handle* h, int32_t interp, float wa, const void* a, const void* b, int32_t width,
                         int32_t height, int32_t fmt, const float* f_ab, const float* f_ba, int32_t flow_w,
-                        int32_t flow_h, int32_t warp_mode, int32_t taps_above_first, void* out, char* err,
-                        size_t errlen) {
+                        int32_t flow_h, int32_t warp_mode, int32_t taps_above_first, int32_t forward_gouraud,
+                        void* out, char* err, size_t errlen) {
     std::string e;
     int st = TWX_OK;
     try {
@@ -1059,12 +1177,18 @@ int twx_compose_vectors(twx_handle* h, int32_t interp, float wa, const void* a, 
             set_err(err, errlen, "interpolation mode outside 0 (Nearest), 1 (Blend), 2 (Weighted Blend)");
             return TWX_E_MISMATCH;
         }
-        if (warp_mode != 0 && warp_mode != 1) {
-            set_err(err, errlen, "only the inverse warps (warping codes 0 and 1) are served by this backend");
+        if (warp_mode < 0 || warp_mode > 3) {
+            set_err(err, errlen, "warping code outside 0..3");
+            return TWX_E_MISMATCH;
+        }
+        const bool forward = warp_mode >= 2;
+        if (forward && !forward_gouraud) {
+            set_err(err, errlen, "the textured forward warp (warping codes 2 and 3 without stages::GouraudForward) has no "
+                                 "device kernel here");
             return TWX_E_UNSUPPORTED;
         }
         if (width < 2 || height < 2 || flow_w < 2 || flow_h < 2 || fmt < 0 || fmt > 2) {
-            set_err(err, errlen, "inverse warp needs images and a flow of at least 2x2");
+            set_err(err, errlen, "the warps need images and a flow of at least 2x2");
             return TWX_E_MISMATCH;
         }
         hipStream_t s = h->stream;
@@ -1091,8 +1215,10 @@ int twx_compose_vectors(twx_handle* h, int32_t interp, float wa, const void* a, 
         F.finish = 0;
         F.warp_a_copy = warp_is_copy(F.frac, flow_w, width) ? 1 : 0;
         F.warp_b_copy = warp_is_copy(wa, flow_w, width) ? 1 : 0;
-        bool need_ab = interp != 0 || !(0.5f <= wa);
-        bool need_ba = interp != 0 || (0.5f <= wa);
+        // compose/vector.rs:61-64 (nearest_uses_f_ab): the one flow of Nearest
+        const bool nearest_ab = (wa < 0.5f && !forward) || (0.5f <= wa && forward);
+        bool need_ab = interp != 0 || nearest_ab;
+        bool need_ba = interp != 0 || !nearest_ab;
         if (need_ab) {
             if (!f_ab) { set_err(err, errlen, "the composition needs the flow F_AB"); return TWX_E_MISMATCH; }
             HIPCHECK(hipMemcpyAsync(fl, f_ab, fn * 4, hipMemcpyHostToDevice, s));
@@ -1103,9 +1229,50 @@ int twx_compose_vectors(twx_handle* h, int32_t interp, float wa, const void* a, 
             HIPCHECK(hipMemcpyAsync(fl + fn, f_ba, fn * 4, hipMemcpyHostToDevice, s));
             F.f_ba = fl + fn;
         }
-        int* bad = pool.alloc<int>(2);
-        HIPCHECK(hipMemsetAsync(bad, 0, 2 * sizeof(int), s));
-        if (warp_mode == 1) {
+        int* bad = pool.alloc<int>(3);
+        HIPCHECK(hipMemsetAsync(bad, 0, 3 * sizeof(int), s));
+        bool direct = false;  // warping code 3 Nearest: the pushed frame is the output
+        if (forward) {
+            // forward.rs:391-572: each frame pushed with its own flow (A with F_AB at 1 - wa, B with
+            // F_BA at wa; Nearest the one flow), with coverage for code 2
+            const bool fill = warp_mode == 2;
+            const size_t np = (size_t)width * height;
+            if (interp == 0) {
+                const bool from_a = !(0.5f > wa);
+                void* dst = fill ? (void*)pool.alloc<uint8_t>(fb) : F.out;
+                uint8_t* cov = fill ? pool.alloc<uint8_t>(np) : nullptr;
+                st = forward_warp_dev(h, pool, from_a ? F.a : F.b, width, height, fmt, from_a ? F.f_ab : F.f_ba,
+                                      from_a ? f_ab : f_ba, flow_w, flow_h, from_a ? 1.0f - wa : wa, dst, cov, bad + 2, e);
+                F.pre_a = dst;
+                F.pre_cov_a = cov;
+                direct = !fill;
+            } else {
+                void* ta = pool.alloc<uint8_t>(fb);
+                void* tb = pool.alloc<uint8_t>(fb);
+                uint8_t* c1 = fill ? pool.alloc<uint8_t>(np) : nullptr;
+                uint8_t* c2 = fill ? pool.alloc<uint8_t>(np) : nullptr;
+                st = forward_warp_dev(h, pool, F.a, width, height, fmt, F.f_ab, f_ab, flow_w, flow_h, F.frac, ta, c1,
+                                      bad + 2, e);
+                if (!st)
+                    st = forward_warp_dev(h, pool, F.b, width, height, fmt, F.f_ba, f_ba, flow_w, flow_h, wa, tb, c2,
+                                          bad + 2, e);
+                F.pre_a = ta;
+                F.pre_b = tb;
+                F.pre_cov_a = c1;
+                F.pre_cov_b = c2;
+            }
+            if (st) { set_err(err, errlen, e); return st; }
+            if (fill) {
+                setup_smart(pool, F, interp, taps_above_first != 0, bad + 1);
+            } else if (interp == 0) {
+                F.kind = 1;
+            } else if (interp == 1) {
+                F.kind = 2;
+                blend_branch(F.frac, F);
+            } else {
+                setup_weighted(pool, F, taps_above_first != 0);
+            }
+        } else if (warp_mode == 1) {
             setup_smart(pool, F, interp, taps_above_first != 0, bad + 1);
         } else if (interp == 0) {
             F.kind = 1;
@@ -1117,19 +1284,22 @@ int twx_compose_vectors(twx_handle* h, int32_t interp, float wa, const void* a, 
             setup_weighted(pool, F, taps_above_first != 0);
         }
         F.bad = bad;
-        std::vector<FrameJob> jobs{F};
-        FrameJob* dj = pool.upload(jobs);
-        launch_frames(s, dj, jobs);
-        launch_weighted(s, dj, jobs);
-        if ((st = run_fills(s, pool, jobs, e))) { set_err(err, errlen, e); return st; }
-        launch_finish(s, dj, jobs);
-        int hbad = 0;
+        if (!direct) {
+            std::vector<FrameJob> jobs{F};
+            FrameJob* dj = pool.upload(jobs);
+            launch_frames(s, dj, jobs);
+            launch_weighted(s, dj, jobs);
+            if ((st = run_fills(s, pool, jobs, e))) { set_err(err, errlen, e); return st; }
+            launch_finish(s, dj, jobs);
+        }
+        int* hbad = pool.pinned<int>(3);
         HIPCHECK(hipMemcpyAsync(out, img + 2 * fb, fb, hipMemcpyDeviceToHost, s));
-        HIPCHECK(hipMemcpyAsync(&hbad, bad, sizeof(int), hipMemcpyDeviceToHost, s));
+        HIPCHECK(hipMemcpyAsync(hbad, bad, 3 * sizeof(int), hipMemcpyDeviceToHost, s));
         HIPCHECK(hipStreamSynchronize(s));
         h->stats.composes++;
         h->stats.device_ms += now_ms() - t0;
-        if (hbad) st = TWX_E_UNSUPPORTED, e = "inverse warp: a sample coordinate is NaN or never converges, which is not supported";
+        if (hbad[0]) st = TWX_E_UNSUPPORTED, e = "inverse warp: a sample coordinate is NaN or never converges, which is not supported";
+        else st = check_fw_flags(hbad[2], e);
     } catch (const HipError& x) {
         e = x.msg;
         st = TWX_E_DEVICE;

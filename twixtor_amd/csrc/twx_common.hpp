@@ -88,6 +88,13 @@ struct FrameJob {
     uint8_t* holes;
     int* has_hole;
     int pad2;
+    // Forward warps (warping codes 2 and 3, forward.rs:391-572): the pushed frames computed before
+    // this job (engine images) and their coverages; when pre_a is set the job reads them in place
+    // of its inverse warps (Nearest: pre_a is the one pushed frame).
+    const void* pre_a;
+    const void* pre_b;
+    const uint8_t* pre_cov_a;
+    const uint8_t* pre_cov_b;
 };
 
 constexpr int MAXFL = 24;  // levels of the hole-fill pyramid (holefill.rs:145-176)
@@ -116,7 +123,8 @@ struct FillPlane {
 enum ProfFam {
     F_INGEST, F_LUMA, F_PYR_BLUR, F_PYR_RESAMPLE, F_GRAD_TENSOR, F_BM_TRANSPOSE,
     F_SOLVE_INIT, F_CALC_B, F_NEW_DISP, F_BM_ROWS, F_BM_COLS, F_RANGE, F_UPSAMPLE, F_NEGATE, F_FRAMES,
-    F_WB_MASK, F_WB_SHARPEN, F_FILL_PULL, F_FILL_PUSH, F_FILL_IO, F_DIFF_FWD, F_DIFF_BWD, F_DIFF_GATHER, F_NFAM
+    F_WB_MASK, F_WB_SHARPEN, F_FILL_PULL, F_FILL_PUSH, F_FILL_IO, F_DIFF_FWD, F_DIFF_BWD, F_DIFF_GATHER,
+    F_FW_MESH, F_FW_RASTER, F_FW_SCAN, F_FW_FOLD, F_NFAM
 };
 extern const char* const PROF_NAMES[F_NFAM];
 
@@ -187,6 +195,35 @@ int run_selfcheck(hipStream_t s, std::string& err);
 int warp_blend_run(hipStream_t s, bool backward, int C, int W, int H, const float* a, const float* b, const float* f_ab,
                    const float* f_ba, int fw, int fh, float wa, float* out, const float* grad_out, float* grad_a,
                    float* grad_b, float* grad_f_ab, float* grad_f_ba, float* gp_scratch, int* bad, std::string& err);
+
+// The Gouraud forward warp (twx_forward.hip): device buffers of one warp. The caller allocates
+// pos (W*H), counts and fill (W*H, zeroed), offsets (W*H), block_totals (fw_scan_blocks(W*H)),
+// total (1) and flags (1, zeroed); after launch_fw_count and a read of *total, rec_t and rec_c
+// (total records each), then launch_fw_emit_fold. flags: 1 the last row pass would write outside
+// the image, 2 a triangle over the per-triangle hit cap, 4 a NaN colour, 8 a repeated hit.
+struct FwArgs {
+    const void* src;
+    void* out;
+    uint8_t* cov;  // may be null
+    const float* flow;
+    float2* pos;
+    uint32_t* counts;
+    uint32_t* fill;
+    uint64_t* offsets;
+    uint64_t* block_totals;
+    uint64_t* total;
+    uint32_t* rec_t;
+    float4* rec_c;
+    int* flags;
+    int W, H, fw, fh, fmt;
+    float rx, ry, sx, sy;
+    uint64_t ntri;
+};
+size_t fw_scan_blocks(long n);
+bool fw_flow_is_zero(const float* flow, long n);  // forward.rs:183-195 on a host flow
+void fw_grid(FwArgs& A, float amount);             // mesh.rs:11-31
+void launch_fw_count(hipStream_t s, const FwArgs& A);
+void launch_fw_emit_fold(hipStream_t s, const FwArgs& A, uint64_t records);
 
 // Taps of the solver's Gaussians and of the pyramid filter (conv.rs:44-86).
 Taps gaussian_taps(float r);

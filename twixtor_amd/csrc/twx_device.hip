@@ -154,7 +154,8 @@ const char* const PROF_NAMES[F_NFAM] = {"ingest", "luma", "pyr_blur5_band", "pyr
                                         "tensor_gauss", "bm_transpose", "solve_init", "calc_b_gauss", "new_disp_gauss",
                                         "bm_rows", "bm_cols", "range", "upsample", "negate", "frames",
                                         "wb_mask", "wb_sharpen_blend", "fill_pull", "fill_push", "fill_io",
-                                        "warp_blend_fwd", "warp_blend_bwd", "warp_blend_flow_gather"};
+                                        "warp_blend_fwd", "warp_blend_bwd", "warp_blend_flow_gather",
+                                        "fw_mesh", "fw_raster", "fw_scan", "fw_fold"};
 thread_local Prof* g_prof = nullptr;
 
 hipEvent_t Prof::take() {
@@ -1560,6 +1561,8 @@ __global__ void k_frames(const FrameJob* __restrict__ jobs) {
             for (int k = 0; k < 4; k++) { pa[k] = pa[k] + 0.0f; pb[k] = pb[k] + 0.0f; }
         }
         blend_pixel<FMT>(pa, pb, F.blend_mode, F.weight, F.lerp_w, e);
+    } else if (F.kind == 1 && F.pre_a) {
+        load4<FMT>(F.pre_a, mi, e);
     } else if (F.kind == 1) {
         if (F.nearest_from_a) {
             if (F.warp_a_copy) load4<FMT>(F.a, mi, e);
@@ -1574,7 +1577,16 @@ __global__ void k_frames(const FrameJob* __restrict__ jobs) {
         // their difference masks; the holes mask is 255 - max(cov1, cov2) (smart_finish)
         using T = typename Px<FMT>::T;
         int c1 = 255, c2 = 255;
-        if (F.kind == 4) {
+        if (F.pre_a) {
+            // pushed frames and their coverages (warping code 2)
+            load4<FMT>(F.pre_a, mi, F.kind == 4 ? e : pa);
+            c1 = F.pre_cov_a[mi];
+            if (F.kind != 4) {
+                load4<FMT>(F.pre_b, mi, pb);
+                c2 = F.pre_cov_b[mi];
+                if (F.kind == 5) smart_px<FMT>(pa, pb, F.frac, e);
+            }
+        } else if (F.kind == 4) {
             if (F.nearest_from_a) {
                 if (F.warp_a_copy) load4<FMT>(F.a, mi, e);
                 else ok = warp_fade_pixel<FMT>(F.a, F.W, F.H, F.f_ba, F.fw, F.fh, F.frac, x, y, e, c1);
@@ -1611,10 +1623,15 @@ __global__ void k_frames(const FrameJob* __restrict__ jobs) {
         if (!ok) atomicOr(F.bad, 1);
         return;
     } else {
-        if (F.warp_a_copy) load4<FMT>(F.a, mi, pa);
-        else ok = warp_pixel<FMT>(F.a, F.W, F.H, F.f_ba, F.fw, F.fh, F.frac, x, y, pa);
-        if (F.warp_b_copy) load4<FMT>(F.b, mi, pb);
-        else ok = warp_pixel<FMT>(F.b, F.W, F.H, F.f_ab, F.fw, F.fh, F.wa, x, y, pb) && ok;
+        if (F.pre_a) {
+            load4<FMT>(F.pre_a, mi, pa);
+            load4<FMT>(F.pre_b, mi, pb);
+        } else {
+            if (F.warp_a_copy) load4<FMT>(F.a, mi, pa);
+            else ok = warp_pixel<FMT>(F.a, F.W, F.H, F.f_ba, F.fw, F.fh, F.frac, x, y, pa);
+            if (F.warp_b_copy) load4<FMT>(F.b, mi, pb);
+            else ok = warp_pixel<FMT>(F.b, F.W, F.H, F.f_ab, F.fw, F.fh, F.wa, x, y, pb) && ok;
+        }
         if (F.kind == 3) {
             // Weighted Blend, first stage: keep both warped frames and their difference masks
             // against the unwarped frames (filters.rs:295-335: (|dR| + |dG|) + |dB| >= limit)
