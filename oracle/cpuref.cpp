@@ -2136,6 +2136,151 @@ static int compose_forward(int code, int interp, float wa, const void* a, const 
 }
 
 // ------------------------------------------------------------------------------------------
+// Vector image coding (vectors.rs:57-130, :474-583) and the flow steps of Vectors In
+// (compose/vector.rs:335-397): decode, the hole fill of the flow, Vec Scale, and the flow
+// inversion through the forward warp.
+
+static inline void vec_clamp_unit(float& v) {  // vectors.rs:132-141 (a NaN falls through)
+    if (v > 1.0f) v = 1.0f;
+    else if (-1.0f > v) v = -1.0f;
+}
+
+// vectors.rs:57-130: the flow sampled at every output pixel, scaled to output pixels over `scale`
+static int encode_vectors(void* dst, long w, long h, int fmt, const float* flow, long fw, long fh, float scale,
+                          std::string& err) {
+    if (w < 2 || h < 2 || fw < 2 || fh < 2) {
+        err = "vector encode needs images and a flow of at least 2x2";
+        return TWX_E_MISMATCH;
+    }
+    float dw1 = (float)((int32_t)w - 1), dh1 = (float)((int32_t)h - 1);
+    float fw1 = (float)((int32_t)fw - 1), fh1 = (float)((int32_t)fh - 1);
+    float sx = fw1 / dw1, sy = fh1 / dh1;
+    float kx = (dw1 / fw1) / scale, ky = (dh1 / fh1) / scale;
+    for (long y = 0; y < h; y++)
+        for (long x = 0; x < w; x++) {
+            float f[2];
+            flow_bilerp(flow, fw, (int32_t)fw - 1, (int32_t)fh - 1, (float)x * sx, (float)y * sy, f);
+            float a = f[0] * kx, b = f[1] * ky;
+            long i = (y * w + x) * 4;
+            if (fmt == TWX_FMT_F32) {
+                float* o = (float*)dst;
+                o[i] = 1.0f;
+                o[i + 1] = (a + 1.0f) * 0.5f;
+                o[i + 2] = (b + 1.0f) * 0.5f;
+                o[i + 3] = 0.0f;
+                continue;
+            }
+            vec_clamp_unit(a);
+            vec_clamp_unit(b);
+            a = a + 1.0f;
+            b = b + 1.0f;
+            if (fmt == TWX_FMT_U8) {
+                uint8_t* o = (uint8_t*)dst;
+                o[i] = 0xff;
+                o[i + 1] = (uint8_t)cvtt64d((double)a * 127.0 + 0.5);
+                o[i + 2] = (uint8_t)cvtt64d((double)b * 127.0 + 0.5);
+                o[i + 3] = 0;
+            } else {
+                uint16_t* o = (uint16_t*)dst;
+                o[i] = 0xffff;
+                o[i + 1] = (uint16_t)cvtt64d((double)a * 32767.0 + 0.5);
+                o[i + 2] = (uint16_t)cvtt64d((double)b * 32767.0 + 0.5);
+                o[i + 3] = 0;
+            }
+        }
+    return TWX_OK;
+}
+
+// vectors.rs:474-521: alpha 0 is the zero vector; integers ((c * 2) / den - 1) * scale in double
+static void decode_vectors(const void* vec, long w, long h, int fmt, float max_displace, float* flow) {
+    double md = (double)max_displace;
+    for (long i = 0; i < w * h; i++) {
+        float* o = flow + i * 2;
+        if (fmt == TWX_FMT_F32) {
+            const float* p = (const float*)vec + i * 4;
+            if (p[0] == 0.0f) { o[0] = o[1] = 0.0f; continue; }
+            o[0] = (p[1] * 2.0f - 1.0f) * max_displace;
+            o[1] = (p[2] * 2.0f - 1.0f) * max_displace;
+        } else if (fmt == TWX_FMT_U16) {
+            const uint16_t* p = (const uint16_t*)vec + i * 4;
+            if (p[0] == 0) { o[0] = o[1] = 0.0f; continue; }
+            o[0] = (float)((((double)p[1] * 2.0) / 65534.0 - 1.0) * md);
+            o[1] = (float)((((double)p[2] * 2.0) / 65534.0 - 1.0) * md);
+        } else {
+            const uint8_t* p = (const uint8_t*)vec + i * 4;
+            if (p[0] == 0) { o[0] = o[1] = 0.0f; continue; }
+            o[0] = (float)((((double)p[1] * 2.0) / 254.0 - 1.0) * md);
+            o[1] = (float)((((double)p[2] * 2.0) / 254.0 - 1.0) * md);
+        }
+    }
+}
+
+// vectors.rs:543-549 with holefill.rs:124-133: the inverted alpha of the vector image as a hole
+// flag (u8: any non-zero; u16 and f32: at least 3.1e-5 or unordered)
+static std::vector<uint8_t> vector_holes(const void* vec, long w, long h, int fmt) {
+    std::vector<uint8_t> holes((size_t)w * h);
+    const double wide_eps = [] { uint64_t b = 0x3f0040bfe3b03e21ull; double d; memcpy(&d, &b, 8); return d; }();
+    for (long i = 0; i < w * h; i++) {
+        bool hole;
+        if (fmt == TWX_FMT_U8) hole = (uint8_t)(0xff - ((const uint8_t*)vec)[i * 4]) != 0;
+        else if (fmt == TWX_FMT_U16) hole = !(wide_eps > (double)(uint16_t)(0xffff - ((const uint16_t*)vec)[i * 4]));
+        else hole = !(wide_eps > (double)(1.0f - ((const float*)vec)[i * 4]));
+        holes[i] = hole ? 1 : 0;
+    }
+    return holes;
+}
+
+// holefill.rs:506-521: the fill of a two channel float flow
+static int fill_flow_holes(float* flow, long w, long h, const uint8_t* holes, bool above_first, std::string& err) {
+    HoleFiller f;
+    bool has;
+    int st = hole_filler(w, h, holes, above_first, f, has, err);
+    if (st || !has) return st;
+    std::vector<float> plane((size_t)w * h);
+    for (int c = 0; c < 2; c++) {
+        for (long i = 0; i < w * h; i++) plane[i] = flow[i * 2 + c];
+        f.fill(plane.data());
+        for (long i = 0; i < w * h; i++) flow[i * 2 + c] = plane[i];
+    }
+    return TWX_OK;
+}
+
+// compose/vector.rs:335-345 with vectors.rs:566-571: decode, fill, Vec Scale
+static int flow_from_vector_image(const void* vec, long w, long h, int fmt, float max_displace, float scale_x,
+                                  float scale_y, bool above_first, float* flow, std::string& err) {
+    decode_vectors(vec, w, h, fmt, max_displace, flow);
+    std::vector<uint8_t> holes = vector_holes(vec, w, h, fmt);
+    int st = fill_flow_holes(flow, w, h, holes.data(), above_first, err);
+    if (st) return st;
+    for (long i = 0; i < w * h; i++) {
+        flow[i * 2] = scale_x * flow[i * 2];
+        flow[i * 2 + 1] = scale_y * flow[i * 2 + 1];
+    }
+    return TWX_OK;
+}
+
+// compose/vector.rs:370-397: T = (-1, -fx, -fy, -1) pushed along the flow itself (amount 1, with
+// coverage), the uncovered pixels filled; `gouraud` is the stage set's forward variant
+static int invert_flow(const float* flow, long w, long h, bool gouraud, bool above_first, float* out, std::string& err) {
+    std::vector<float> t((size_t)w * h * 4), pushed((size_t)w * h * 4);
+    for (long i = 0; i < w * h; i++) {
+        t[i * 4] = -1.0f * 1.0f;
+        t[i * 4 + 1] = -1.0f * flow[i * 2];
+        t[i * 4 + 2] = -1.0f * flow[i * 2 + 1];
+        t[i * 4 + 3] = -1.0f * 1.0f;
+    }
+    std::vector<uint8_t> cov((size_t)w * h);
+    int st = forward_warp(pushed.data(), t.data(), w, h, TWX_FMT_F32, flow, w, h, 1.0f, gouraud, cov.data(), err);
+    if (st) return st;
+    for (auto& c : cov) c = (uint8_t)(255 - c);
+    for (long i = 0; i < w * h; i++) {
+        out[i * 2] = pushed[i * 4 + 1];
+        out[i * 2 + 1] = pushed[i * 4 + 2];
+    }
+    return fill_flow_holes(out, w, h, cov.data(), above_first, err);
+}
+
+// ------------------------------------------------------------------------------------------
 // Time model (timeline.rs, curves.rs, fixed.rs, frontend.rs)
 
 static inline int32_t float_to_fixed(double f) {  // fixed.rs:8-10 (Rust `as i32` saturates, NaN 0)
@@ -2573,6 +2718,41 @@ void cref_fw_mesh(const float* flow, int fw, int fh, int w, int h, float amount,
 
 // The running mean step (forward.rs:104-128) on one pixel.
 void cref_fw_fold(float* acc, uint8_t* n, const float* c) { fw_fold(acc, *n, c); }
+
+// Vector image coding and the Vectors In flow steps.
+int cref_encode_vectors(void* dst, int w, int h, int fmt, const float* flow, int fw, int fh, float scale, char* err,
+                        size_t errlen) {
+    std::string e;
+    int st = encode_vectors(dst, w, h, fmt, flow, fw, fh, scale, e);
+    if (st) set_err(err, errlen, e);
+    return st;
+}
+void cref_decode_vectors(const void* vec, int w, int h, int fmt, float max_displace, float* flow) {
+    decode_vectors(vec, w, h, fmt, max_displace, flow);
+}
+void cref_vector_holes(const void* vec, int w, int h, int fmt, uint8_t* holes) {
+    std::vector<uint8_t> v = vector_holes(vec, w, h, fmt);
+    memcpy(holes, v.data(), v.size());
+}
+int cref_fill_flow_holes(float* flow, int w, int h, const uint8_t* holes, int above_first, char* err, size_t errlen) {
+    std::string e;
+    int st = fill_flow_holes(flow, w, h, holes, above_first != 0, e);
+    if (st) set_err(err, errlen, e);
+    return st;
+}
+int cref_flow_from_vector_image(const void* vec, int w, int h, int fmt, float max_displace, float sx, float sy,
+                                int above_first, float* flow, char* err, size_t errlen) {
+    std::string e;
+    int st = flow_from_vector_image(vec, w, h, fmt, max_displace, sx, sy, above_first != 0, flow, e);
+    if (st) set_err(err, errlen, e);
+    return st;
+}
+int cref_invert_flow(const float* flow, int w, int h, int gouraud, int above_first, float* out, char* err, size_t errlen) {
+    std::string e;
+    int st = invert_flow(flow, w, h, gouraud != 0, above_first != 0, out, e);
+    if (st) set_err(err, errlen, e);
+    return st;
+}
 
 // The fading warp (hold off) with its coverage image.
 int cref_inv_warp_cov(void* out, const void* src, int w, int h, int fmt, const float* flow, int fw, int fh, float amount,

@@ -387,6 +387,54 @@ def fw_fold(acc, n, c):
     return a, cnt.value
 
 
+# ---- vector image coding and the Vectors In flow steps (vectors.rs, compose/vector.rs) ----
+
+def encode_vectors(w, h, fmt, flow, fw, fh, scale):
+    out = np.zeros(w * h * 4, DTYPE[fmt])
+    err = ctypes.create_string_buffer(256)
+    st = lib().cref_encode_vectors(_p(out), w, h, fmt, _p(np.ascontiguousarray(flow, np.float32)), fw, fh,
+                                   ctypes.c_float(scale), err, 256)
+    if st:
+        raise RuntimeError(err.value.decode())
+    return out
+
+
+def decode_vectors(vec, w, h, fmt, max_displace):
+    flow = np.zeros(w * h * 2, np.float32)
+    lib().cref_decode_vectors(_p(np.ascontiguousarray(vec)), w, h, fmt, ctypes.c_float(max_displace), _p(flow))
+    return flow
+
+
+def vector_holes(vec, w, h, fmt):
+    holes = np.zeros(w * h, np.uint8)
+    lib().cref_vector_holes(_p(np.ascontiguousarray(vec)), w, h, fmt, _p(holes))
+    return holes
+
+
+def fill_flow_holes(flow, w, h, holes, above_first=True):
+    f = np.ascontiguousarray(flow, np.float32).copy()
+    err = ctypes.create_string_buffer(256)
+    st = lib().cref_fill_flow_holes(_p(f), w, h, _p(np.ascontiguousarray(holes, np.uint8)), int(above_first), err, 256)
+    return st, f
+
+
+def flow_from_vector_image(vec, w, h, fmt, max_displace, scale=(1.0, 1.0), above_first=True):
+    flow = np.zeros(w * h * 2, np.float32)
+    err = ctypes.create_string_buffer(256)
+    st = lib().cref_flow_from_vector_image(_p(np.ascontiguousarray(vec)), w, h, fmt, ctypes.c_float(max_displace),
+                                           ctypes.c_float(scale[0]), ctypes.c_float(scale[1]), int(above_first),
+                                           _p(flow), err, 256)
+    return st, flow
+
+
+def invert_flow(flow, w, h, gouraud=True, above_first=True):
+    out = np.zeros(w * h * 2, np.float32)
+    err = ctypes.create_string_buffer(256)
+    st = lib().cref_invert_flow(_p(np.ascontiguousarray(flow, np.float32)), w, h, int(gouraud), int(above_first),
+                                _p(out), err, 256)
+    return st, out
+
+
 # ---- the fading warp, Smart Blend and the hole fill (warp.rs, smartblend.rs, holefill.rs) ----
 
 def inv_warp_cov(src, w, h, fmt, flow, fw, fh, amount):
