@@ -158,6 +158,50 @@ int twx_forward_warp(twx_handle* h, const void* src, int32_t width, int32_t heig
                      const float* flow, int32_t flow_w, int32_t flow_h, float amount,
                      int32_t pyramid, void* out, uint8_t* coverage, char* err, size_t errlen);
 
+/* ---- vector image coding (vectors.rs) and the Vectors In flow steps (compose/vector.rs) ---- */
+
+/* encode_vectors (vectors.rs:57-130): `flow` (flow_w x flow_h, any size of at least 2x2) sampled
+ * at every pixel of a width x height engine image of `fmt`, scaled to pixels of that size over
+ * `scale` (Max Displace): A white, R and G the centred components, B 0. Host buffers. No
+ * counterpart in the reference's device backend (the reference encodes on the CPU). */
+int twx_encode_vectors(twx_handle* h, const float* flow, int32_t flow_w, int32_t flow_h,
+                       int32_t width, int32_t height, int32_t fmt, float scale, void* out,
+                       char* err, size_t errlen);
+
+/* decode_vectors (vectors.rs:474-521): the flow (width x height x 2) of an encoded engine image
+ * with `max_displace`; alpha 0 is the zero vector. `holes_out` (may be NULL) receives the hole
+ * flags of the inverted alpha (vectors.rs:543-549 as the hole fill reads them,
+ * holefill.rs:124-133). Host buffers. */
+int twx_decode_vectors(twx_handle* h, const void* vec, int32_t width, int32_t height, int32_t fmt,
+                       float max_displace, float* flow_out, uint8_t* holes_out, char* err,
+                       size_t errlen);
+
+/* flow_from_vector_image_with (compose/vector.rs:335-345): decode, the push-pull fill of the
+ * holes (holefill.rs:506-521, the vertical tap order `taps_above_first`), Vec Scale
+ * (vectors.rs:566-571). Host buffers. */
+int twx_flow_from_vectors(twx_handle* h, const void* vec, int32_t width, int32_t height,
+                          int32_t fmt, float max_displace, float scale_x, float scale_y,
+                          int32_t taps_above_first, float* flow_out, char* err, size_t errlen);
+
+/* invert_flow_on (compose/vector.rs:370-397): T = (-1, -fx, -fy, -1) pushed along the flow by
+ * the forward warp with coverage (amount 1), the uncovered pixels filled. `forward_gouraud` is
+ * the stage set's forward variant (1 for Vectors In; 0, the textured one, is refused with
+ * TWX_E_UNSUPPORTED). Host buffers. */
+int twx_invert_flow(twx_handle* h, const float* flow, int32_t width, int32_t height,
+                    int32_t forward_gouraud, int32_t taps_above_first, float* out, char* err,
+                    size_t errlen);
+
+/* The Create Vectors pass over a clip (create_vectors.rs:162-179 with the row flips of
+ * create-vectors/src/glue.rs:517-566): output k is solve(X = frame frame_x[k], Y = frame
+ * frame_y[k]) on the flipped engine images, encoded at the frame size with `max_displace`,
+ * flipped back and written as a wire frame (image.rs:205-245). Identical (X, Y) pairs are solved
+ * once. `clip` (n_in wire frames) and `out` (count wire frames) are DEVICE pointers; frame_x and
+ * frame_y are host arrays of count indices into the clip. */
+int twx_create_vectors_dev(twx_handle* h, const twx_flow_settings* fs, int32_t width,
+                           int32_t height, int32_t fmt, const void* clip, int32_t n_in,
+                           int32_t count, const int32_t* frame_x, const int32_t* frame_y,
+                           float max_displace, void* out, char* err, size_t errlen);
+
 /* ---- the per-frame render loop of `vfx apply retime` ------------------------------------- */
 
 /* Frame selection of output frame `n` (compose.rs:76-99 and timeline.rs:447-464). */

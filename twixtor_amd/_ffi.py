@@ -18,7 +18,8 @@ EXPORTS = [
     "twx_open", "twx_close", "twx_selfcheck", "twx_memory_budget", "twx_flow_geometry", "twx_solve",
     "twx_solve_pair", "twx_compose_vectors", "twx_select", "twx_retime_frames_dev", "twx_retime_batch",
     "twx_retime_frames",
-    "twx_forward_warp", "twx_warp_blend", "twx_warp_blend_backward",
+    "twx_forward_warp", "twx_encode_vectors", "twx_decode_vectors", "twx_flow_from_vectors", "twx_invert_flow",
+    "twx_create_vectors_dev", "twx_warp_blend", "twx_warp_blend_backward",
     "twx_get_stats", "twx_stream", "twx_profile_enable", "twx_profile_reset", "twx_profile_read",
 ]
 
@@ -97,6 +98,13 @@ def lib():
         PI32 = ctypes.POINTER(ctypes.c_int32)
         L.twx_retime_batch.argtypes = [P, I32, ctypes.POINTER(RetimeParams), ctypes.POINTER(ctypes.c_void_p), PI32, PI32,
                                        PI32, ctypes.POINTER(ctypes.c_void_p)] + ERR
+        F32 = ctypes.c_float
+        L.twx_encode_vectors.argtypes = [P, P, I32, I32, I32, I32, I32, F32, P] + ERR
+        L.twx_decode_vectors.argtypes = [P, P, I32, I32, I32, F32, P, P] + ERR
+        L.twx_flow_from_vectors.argtypes = [P, P, I32, I32, I32, F32, F32, F32, I32, P] + ERR
+        L.twx_invert_flow.argtypes = [P, P, I32, I32, I32, I32, P] + ERR
+        L.twx_create_vectors_dev.argtypes = [P, ctypes.POINTER(FlowSettings), I32, I32, I32, P, I32, I32, PI32, PI32, F32,
+                                             P] + ERR
         L.twx_warp_blend.argtypes = [P, I32, I32, I32, P, P, P, P, I32, I32, ctypes.c_float, P] + ERR
         L.twx_warp_blend_backward.argtypes = [P, I32, I32, I32, P, P, P, P, I32, I32, ctypes.c_float, P, P, P, P, P] + ERR
         L.twx_get_stats.argtypes = [P, ctypes.POINTER(Stats)]

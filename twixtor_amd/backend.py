@@ -150,6 +150,47 @@ class HipBackend:
                                                int(pyramid), _p(out), None if cov is None else _p(cov), err, 512), err)
         return out, cov
 
+    # ---- vector image coding and the Vectors In flow steps ----
+    def encode_vectors(self, flow, fw, fh, width, height, fmt, scale) -> np.ndarray:
+        out = np.zeros(width * height * 4, DTYPES[fmt])
+        err = _ffi.errbuf()
+        _ffi.check(_ffi.lib().twx_encode_vectors(self._h, _p(np.ascontiguousarray(flow, np.float32)), fw, fh, width,
+                                                 height, fmt, ctypes.c_float(scale), _p(out), err, 512), err)
+        return out
+
+    def decode_vectors(self, vec, width, height, fmt, max_displace):
+        """(flow, hole flags) of an encoded vector image."""
+        flow = np.zeros(width * height * 2, np.float32)
+        holes = np.zeros(width * height, np.uint8)
+        err = _ffi.errbuf()
+        _ffi.check(_ffi.lib().twx_decode_vectors(self._h, _p(np.ascontiguousarray(vec)), width, height, fmt,
+                                                 ctypes.c_float(max_displace), _p(flow), _p(holes), err, 512), err)
+        return flow, holes
+
+    def flow_from_vectors(self, vec, width, height, fmt, max_displace, scale=(1.0, 1.0), above_first=True):
+        flow = np.zeros(width * height * 2, np.float32)
+        err = _ffi.errbuf()
+        _ffi.check(_ffi.lib().twx_flow_from_vectors(self._h, _p(np.ascontiguousarray(vec)), width, height, fmt,
+                                                    ctypes.c_float(max_displace), ctypes.c_float(scale[0]),
+                                                    ctypes.c_float(scale[1]), int(above_first), _p(flow), err, 512), err)
+        return flow
+
+    def invert_flow(self, flow, width, height, gouraud=True, above_first=True):
+        out = np.zeros(width * height * 2, np.float32)
+        err = _ffi.errbuf()
+        _ffi.check(_ffi.lib().twx_invert_flow(self._h, _p(np.ascontiguousarray(flow, np.float32)), width, height,
+                                              int(gouraud), int(above_first), _p(out), err, 512), err)
+        return out
+
+    def create_vectors_dev(self, fs, width, height, fmt, clip_ptr, n_in, frame_x, frame_y, max_displace, out_ptr):
+        """The Create Vectors pass over a device-resident clip into device-resident wire frames."""
+        n = len(frame_x)
+        I = ctypes.c_int32 * n
+        err = _ffi.errbuf()
+        _ffi.check(_ffi.lib().twx_create_vectors_dev(self._h, ctypes.byref(fs), width, height, fmt,
+                                                     ctypes.c_void_p(clip_ptr), n_in, n, I(*frame_x), I(*frame_y),
+                                                     ctypes.c_float(max_displace), ctypes.c_void_p(out_ptr), err, 512), err)
+
     # ---- the per-frame render loop, batched ----
     def retime_frames(self, cfg: Retime, clip: bytes, n_in: int, first: int, count: int) -> bytes:
         p = cfg.struct()

@@ -538,64 +538,87 @@ void hole_geometry(int w, int h, std::vector<std::pair<int, int>>& dims, int& to
     top = count - 1;
 }
 
-// The hole fill of every Smart Blend frame of `jobs` (kinds 4-6), in chunks of frames whose
-// buffers fit a few GB; a frame without holes passes through every kernel at once.
-int run_fills(hipStream_t s, DevPool& pool, const std::vector<FrameJob>& jobs, std::string& err) {
-    std::vector<int> idx;
-    for (size_t i = 0; i < jobs.size(); i++)
-        if (jobs[i].kind >= 4) idx.push_back((int)i);
-    if (idx.empty()) return TWX_OK;
-    // per frame: 5 pull pyramids, 4 push pyramids, two level-0 scratch planes per pull plane
-    size_t per_frame = 0;
-    for (int i : idx) {
+// One image of a batched hole fill: `chans` planes of `img` (4 of an engine image in its format,
+// or 2 of a flow with fmt FILL_FLOW2), the holes mask and the has-a-hole flag that gates it.
+struct FillItem {
+    int W, H;
+    void* img;
+    int fmt, chans;
+    const uint8_t* holes;
+    const int* gate;
+    int above_first;
+};
+
+// The push-pull hole fill of a set of images in chunks whose buffers fit a few GB; an image
+// without holes passes through every kernel at once.
+int fill_items(hipStream_t s, DevPool& pool, const std::vector<FillItem>& items, std::string& err) {
+    if (items.empty()) return TWX_OK;
+    // per image: its pull pyramids (values of each plane and the weights), push pyramids, two
+    // level-0 scratch planes per pull plane
+    size_t per_item = 0;
+    for (const FillItem& it : items) {
         std::vector<std::pair<int, int>> dims;
         int top;
-        hole_geometry(jobs[i].W, jobs[i].H, dims, top);
-        if (top + 1 > MAXFL) { err = "hole fill: a frame too large for the fill pyramid"; return TWX_E_UNSUPPORTED; }
+        hole_geometry(it.W, it.H, dims, top);
+        if (top + 1 > MAXFL) { err = "hole fill: an image too large for the fill pyramid"; return TWX_E_UNSUPPORTED; }
         size_t pyr = 0;
         for (int l = 0; l <= top; l++) pyr += (size_t)dims[l].first * dims[l].second;
+        const size_t np = (size_t)it.chans + 1;
         // every buffer rounded up to 64 floats
-        per_frame = std::max(per_frame, 4 * (9 * pyr + 10 * (size_t)jobs[i].W * jobs[i].H + 64 * (9 * (size_t)(top + 1) + 10)));
+        per_item = std::max(per_item, 4 * ((2 * np - 1) * pyr + 2 * np * (size_t)it.W * it.H +
+                                           64 * ((2 * np - 1) * (size_t)(top + 1) + 2 * np)));
     }
-    const size_t chunk = std::max<size_t>(1, std::min<size_t>(idx.size(), ((size_t)4 << 30) / per_frame));
-    char* scratch = pool.alloc<char>(chunk * per_frame);
-    for (size_t c0 = 0; c0 < idx.size(); c0 += chunk) {
+    const size_t chunk = std::max<size_t>(1, std::min<size_t>(items.size(), ((size_t)4 << 30) / per_item));
+    char* scratch = pool.alloc<char>(chunk * per_item);
+    for (size_t c0 = 0; c0 < items.size(); c0 += chunk) {
         std::vector<FillPlane> planes;
-        for (size_t k = c0; k < std::min(idx.size(), c0 + chunk); k++) {
-            const FrameJob& F = jobs[idx[k]];
+        for (size_t k = c0; k < std::min(items.size(), c0 + chunk); k++) {
+            const FillItem& it = items[k];
             std::vector<std::pair<int, int>> dims;
             int top;
-            hole_geometry(F.W, F.H, dims, top);
-            float* base = reinterpret_cast<float*>(scratch + (k - c0) * per_frame);
+            hole_geometry(it.W, it.H, dims, top);
+            float* base = reinterpret_cast<float*>(scratch + (k - c0) * per_item);
             auto take = [&](size_t n) { float* p = base; base += (n + 63) & ~(size_t)63; return p; };
             FillPlane fp[5];
             memset(fp, 0, sizeof(fp));
-            for (int c = 0; c < 5; c++) {
-                FillPlane& P = fp[c];
+            // the colour planes 0..chans-1 and the weight plane (chan 4)
+            int chans[5], nplanes = 0;
+            for (int c = 0; c < it.chans; c++) chans[nplanes++] = c;
+            chans[nplanes++] = 4;
+            for (int q = 0; q < nplanes; q++) {
+                FillPlane& P = fp[q];
                 P.top = top;
-                P.chan = c;
-                P.gate = F.has_hole;
-                P.img = F.comp;
-                P.holes = F.holes;
-                P.fmt = F.fmt;
-                P.above_first = F.above_first;
+                P.chan = chans[q];
+                P.gate = it.gate;
+                P.img = it.img;
+                P.holes = it.holes;
+                P.fmt = it.fmt;
+                P.above_first = it.above_first;
                 for (int l = 0; l <= top; l++) {
                     P.w[l] = dims[l].first;
                     P.h[l] = dims[l].second;
                     P.val[l] = take((size_t)dims[l].first * dims[l].second);
-                    if (c < 4) P.sol[l] = take((size_t)dims[l].first * dims[l].second);
+                    if (P.chan < 4) P.sol[l] = take((size_t)dims[l].first * dims[l].second);
                 }
-                P.t1 = take((size_t)F.W * F.H);
-                P.t2 = take((size_t)F.W * F.H);
+                P.t1 = take((size_t)it.W * it.H);
+                P.t2 = take((size_t)it.W * it.H);
             }
-            for (int c = 0; c < 5; c++)
-                for (int l = 0; l <= top; l++) fp[c].wts[l] = fp[4].val[l];
-            for (int c = 0; c < 5; c++) planes.push_back(fp[c]);
+            for (int q = 0; q < nplanes; q++)
+                for (int l = 0; l <= top; l++) fp[q].wts[l] = fp[nplanes - 1].val[l];
+            for (int q = 0; q < nplanes; q++) planes.push_back(fp[q]);
         }
         FillPlane* dp = pool.upload(planes);
         launch_fill(s, dp, planes);
     }
     return TWX_OK;
+}
+
+// The hole fill of every Smart Blend frame of `jobs` (kinds 4-6).
+int run_fills(hipStream_t s, DevPool& pool, const std::vector<FrameJob>& jobs, std::string& err) {
+    std::vector<FillItem> items;
+    for (const FrameJob& F : jobs)
+        if (F.kind >= 4) items.push_back(FillItem{F.W, F.H, F.comp, F.fmt, 4, F.holes, F.has_hole, F.above_first});
+    return fill_items(s, pool, items, err);
 }
 
 // The |amount| early-out of inv_warp (warp.rs:749-764): true means a plain copy.
@@ -1162,6 +1185,257 @@ int twx_forward_warp(twx_handle* h, const void* src, int32_t width, int32_t heig
     }
     if (st) set_err(err, errlen, e);
     return st;
+}
+
+// ---- vector image coding and the Vectors In flow steps ----------------------------------------
+
+}  // extern "C"
+
+// The common frame of an entry point: the handle's lock and device, HIP errors as status 1, the
+// message into `err`.
+template <class F>
+static int entry(twx_handle* h, char* err, size_t errlen, F&& body) {
+    std::string e;
+    int st = TWX_OK;
+    try {
+        Guard g(h, st, e);
+        if (g.ok()) st = body(e);
+    } catch (const HipError& x) {
+        e = x.msg;
+        st = TWX_E_DEVICE;
+    }
+    if (st) set_err(err, errlen, e);
+    return st;
+}
+
+extern "C" {
+
+int twx_encode_vectors(twx_handle* h, const float* flow, int32_t flow_w, int32_t flow_h, int32_t width, int32_t height,
+                       int32_t fmt, float scale, void* out, char* err, size_t errlen) {
+    return entry(h, err, errlen, [&](std::string& e) -> int {
+        if (width < 2 || height < 2 || flow_w < 2 || flow_h < 2 || fmt < 0 || fmt > 2) {
+            e = "vector encode needs images and a flow of at least 2x2";
+            return TWX_E_MISMATCH;
+        }
+        hipStream_t s = h->stream;
+        DevPool pool(h);
+        const size_t fb = image_bytes(width, height, fmt), fn = (size_t)flow_w * flow_h * 2;
+        float* fl = pool.alloc<float>(fn);
+        uint8_t* img = pool.alloc<uint8_t>(fb);
+        HIPCHECK(hipMemcpyAsync(fl, flow, fn * 4, hipMemcpyHostToDevice, s));
+        VecJob J{};
+        J.flow = fl;
+        J.out = img;
+        J.fw = flow_w;
+        J.fh = flow_h;
+        J.W = width;
+        J.H = height;
+        J.fmt = fmt;
+        J.finish = 0;
+        vec_job_scales(J, scale);
+        std::vector<VecJob> jobs{J};
+        launch_vec_encode(s, pool.upload(jobs), jobs);
+        HIPCHECK(hipMemcpyAsync(out, img, fb, hipMemcpyDeviceToHost, s));
+        HIPCHECK(hipStreamSynchronize(s));
+        return TWX_OK;
+    });
+}
+
+int twx_decode_vectors(twx_handle* h, const void* vec, int32_t width, int32_t height, int32_t fmt, float max_displace,
+                       float* flow_out, uint8_t* holes_out, char* err, size_t errlen) {
+    return entry(h, err, errlen, [&](std::string& e) -> int {
+        if (width < 1 || height < 1 || fmt < 0 || fmt > 2) {
+            e = "vector decode: bad size or format";
+            return TWX_E_MISMATCH;
+        }
+        hipStream_t s = h->stream;
+        DevPool pool(h);
+        const size_t fb = image_bytes(width, height, fmt), n = (size_t)width * height;
+        uint8_t* img = pool.alloc<uint8_t>(fb);
+        float* fl = pool.alloc<float>(n * 2);
+        uint8_t* holes = pool.alloc<uint8_t>(n);
+        int* gate = pool.alloc<int>(1);
+        HIPCHECK(hipMemcpyAsync(img, vec, fb, hipMemcpyHostToDevice, s));
+        launch_vec_decode(s, img, width, height, fmt, max_displace, fl, holes, gate);
+        HIPCHECK(hipMemcpyAsync(flow_out, fl, n * 8, hipMemcpyDeviceToHost, s));
+        if (holes_out) HIPCHECK(hipMemcpyAsync(holes_out, holes, n, hipMemcpyDeviceToHost, s));
+        HIPCHECK(hipStreamSynchronize(s));
+        return TWX_OK;
+    });
+}
+
+// The flow of a Vectors In layer on the device: decode, the hole fill of the alpha-0 holes,
+// Vec Scale (compose/vector.rs:335-345). `flow` (width * height * 2) is written.
+static int flow_from_vectors_dev(hipStream_t s, DevPool& pool, const void* d_vec, int W, int H, int fmt, float md,
+                                 float sx, float sy, bool above_first, float* d_flow, std::string& err) {
+    const size_t n = (size_t)W * H;
+    uint8_t* holes = pool.alloc<uint8_t>(n);
+    int* gate = pool.alloc<int>(1);
+    HIPCHECK(hipMemsetAsync(gate, 0, sizeof(int), s));
+    launch_vec_decode(s, d_vec, W, H, fmt, md, d_flow, holes, gate);
+    if (W >= 2 && H >= 2) {
+        std::vector<FillItem> items{FillItem{W, H, d_flow, FILL_FLOW2, 2, holes, gate, above_first ? 1 : 0}};
+        int st = fill_items(s, pool, items, err);
+        if (st) return st;
+    } else {
+        int* hg = pool.pinned<int>(1);
+        HIPCHECK(hipMemcpyAsync(hg, gate, sizeof(int), hipMemcpyDeviceToHost, s));
+        HIPCHECK(hipStreamSynchronize(s));
+        if (*hg) {
+            err = "hole fill of an image with a side of 1";
+            return TWX_E_UNSUPPORTED;
+        }
+    }
+    launch_vec_scale(s, d_flow, (long)n, sx, sy);
+    return TWX_OK;
+}
+
+int twx_flow_from_vectors(twx_handle* h, const void* vec, int32_t width, int32_t height, int32_t fmt, float max_displace,
+                          float scale_x, float scale_y, int32_t taps_above_first, float* flow_out, char* err,
+                          size_t errlen) {
+    return entry(h, err, errlen, [&](std::string& e) -> int {
+        if (width < 1 || height < 1 || fmt < 0 || fmt > 2) {
+            e = "vector decode: bad size or format";
+            return TWX_E_MISMATCH;
+        }
+        hipStream_t s = h->stream;
+        DevPool pool(h);
+        const size_t fb = image_bytes(width, height, fmt), n = (size_t)width * height;
+        uint8_t* img = pool.alloc<uint8_t>(fb);
+        float* fl = pool.alloc<float>(n * 2);
+        HIPCHECK(hipMemcpyAsync(img, vec, fb, hipMemcpyHostToDevice, s));
+        int st = flow_from_vectors_dev(s, pool, img, width, height, fmt, max_displace, scale_x, scale_y,
+                                       taps_above_first != 0, fl, e);
+        if (st) return st;
+        HIPCHECK(hipMemcpyAsync(flow_out, fl, n * 8, hipMemcpyDeviceToHost, s));
+        HIPCHECK(hipStreamSynchronize(s));
+        return TWX_OK;
+    });
+}
+
+int twx_invert_flow(twx_handle* h, const float* flow, int32_t width, int32_t height, int32_t forward_gouraud,
+                    int32_t taps_above_first, float* out, char* err, size_t errlen) {
+    return entry(h, err, errlen, [&](std::string& e) -> int {
+        if (width < 2 || height < 2) {
+            e = "forward warp needs images and a flow of at least 2x2";
+            return TWX_E_UNSUPPORTED;
+        }
+        if (!forward_gouraud) {
+            e = "the flow inversion through the textured forward warp has no device kernel here";
+            return TWX_E_UNSUPPORTED;
+        }
+        hipStream_t s = h->stream;
+        DevPool pool(h);
+        const size_t n = (size_t)width * height;
+        float* fl = pool.alloc<float>(n * 2);
+        float* t = pool.alloc<float>(n * 4);
+        float* pushed = pool.alloc<float>(n * 4);
+        uint8_t* cov = pool.alloc<uint8_t>(n);
+        uint8_t* holes = pool.alloc<uint8_t>(n);
+        float* res = pool.alloc<float>(n * 2);
+        int* flags = pool.alloc<int>(2);
+        HIPCHECK(hipMemcpyAsync(fl, flow, n * 8, hipMemcpyHostToDevice, s));
+        HIPCHECK(hipMemsetAsync(flags, 0, 2 * sizeof(int), s));
+        launch_invert_prep(s, fl, t, (long)n);
+        int st = forward_warp_dev(h, pool, t, width, height, TWX_FMT_F32, fl, flow, width, height, 1.0f, pushed, cov,
+                                  flags, e);
+        if (st) return st;
+        launch_invert_post(s, pushed, cov, res, holes, flags + 1, (long)n);
+        std::vector<FillItem> items{FillItem{width, height, res, FILL_FLOW2, 2, holes, flags + 1, taps_above_first ? 1 : 0}};
+        if ((st = fill_items(s, pool, items, e))) return st;
+        int* hf = pool.pinned<int>(1);
+        HIPCHECK(hipMemcpyAsync(out, res, n * 8, hipMemcpyDeviceToHost, s));
+        HIPCHECK(hipMemcpyAsync(hf, flags, sizeof(int), hipMemcpyDeviceToHost, s));
+        HIPCHECK(hipStreamSynchronize(s));
+        return check_fw_flags(*hf, e);
+    });
+}
+
+int twx_create_vectors_dev(twx_handle* h, const twx_flow_settings* fs, int32_t width, int32_t height, int32_t fmt,
+                           const void* clip, int32_t n_in, int32_t count, const int32_t* frame_x, const int32_t* frame_y,
+                           float max_displace, void* out, char* err, size_t errlen) {
+    return entry(h, err, errlen, [&](std::string& e) -> int {
+        if (width < 2 || height < 2) {
+            e = "create vectors: frames with a side of 1 take the zero vector image on the CPU";
+            return TWX_E_UNSUPPORTED;
+        }
+        if (fmt < 0 || fmt > 2 || count < 0 || n_in < 0 || !fs || (count > 0 && (!frame_x || !frame_y))) {
+            e = "create vectors: bad arguments";
+            return TWX_E_MISMATCH;
+        }
+        for (int k = 0; k < count; k++)
+            if (frame_x[k] < 0 || frame_x[k] >= n_in || frame_y[k] < 0 || frame_y[k] >= n_in) {
+                e = "create vectors: a frame index outside the clip";
+                return TWX_E_MISMATCH;
+            }
+        Geo g;
+        int st = geometry(*fs, width, height, g, e);
+        if (st) return st;
+        hipStream_t s = h->stream;
+        double t0 = now_ms();
+        DevPool pool(h);
+        const size_t fb = image_bytes(width, height, fmt);
+        // the engine images (flipped rows) and pyramids of the frames used, once each
+        std::map<int, int> slot_of;
+        std::vector<IngestOp> ops;
+        std::vector<PyrIn> pyrs;
+        auto slot = [&](int idx) {
+            auto it = slot_of.find(idx);
+            if (it != slot_of.end()) return it->second;
+            IngestOp op{};
+            op.wire = (const uint8_t*)clip + (size_t)idx * fb;
+            op.engine = pool.alloc<uint8_t>(fb);
+            op.W = width;
+            op.H = height;
+            op.fmt = fmt;
+            op.reverse = 1;
+            ops.push_back(op);
+            pyrs.push_back(PyrIn{op.engine, width, height, fmt, g, pool.alloc<float>(pyramid_floats(g))});
+            int k = (int)pyrs.size() - 1;
+            slot_of[idx] = k;
+            return k;
+        };
+        std::map<std::pair<int, int>, int> solve_of;
+        std::vector<SolveIn> solves;
+        std::vector<int> solve_for(count);
+        for (int k = 0; k < count; k++) {
+            auto key = std::make_pair(frame_x[k], frame_y[k]);
+            auto it = solve_of.find(key);
+            if (it != solve_of.end()) {
+                solve_for[k] = it->second;
+                h->stats.pairs_reused++;
+                continue;
+            }
+            int px = slot(frame_x[k]), py = slot(frame_y[k]);
+            solves.push_back(SolveIn{pyrs[px].levels, pyrs[py].levels, g, pool.alloc<float>((size_t)g.w0 * g.h0 * 2)});
+            solve_for[k] = (int)solves.size() - 1;
+            solve_of[key] = solve_for[k];
+        }
+        if (!ops.empty()) launch_ingest_batch(s, pool.upload(ops), ops);
+        build_pyramids(s, pool, pyrs);
+        h->stats.pyramid_builds += pyrs.size();
+        h->stats.pair_solves += solves.size();
+        if ((st = run_solves(s, pool, solves, *fs, h->budget / 2, e))) return st;
+        std::vector<VecJob> jobs;
+        for (int k = 0; k < count; k++) {
+            VecJob J{};
+            J.flow = solves[solve_for[k]].out;
+            J.out = (uint8_t*)out + (size_t)k * fb;
+            J.fw = g.w0;
+            J.fh = g.h0;
+            J.W = width;
+            J.H = height;
+            J.fmt = fmt;
+            J.finish = 1;
+            vec_job_scales(J, max_displace);
+            jobs.push_back(J);
+        }
+        if (!jobs.empty()) launch_vec_encode(s, pool.upload(jobs), jobs);
+        HIPCHECK(hipStreamSynchronize(s));
+        HIPCHECK(hipGetLastError());
+        h->stats.device_ms += now_ms() - t0;
+        return TWX_OK;
+    });
 }
 
 int twx_compose_vectors(twx_handle* h, int32_t interp, float wa, const void* a, const void* b, int32_t width,

@@ -98,6 +98,7 @@ struct FrameJob {
 };
 
 constexpr int MAXFL = 24;  // levels of the hole-fill pyramid (holefill.rs:145-176)
+constexpr int FILL_FLOW2 = 3;  // FillPlane.fmt of a two-channel float flow (holefill.rs:506-521)
 
 // One plane of a batched hole fill (holefill.rs:214-336): a colour channel of a frame (chan 0..3)
 // or the frame's weight plane (chan 4). Pull: val[k] = resample(blur5(val[k - 1])); push from
@@ -124,7 +125,7 @@ enum ProfFam {
     F_INGEST, F_LUMA, F_PYR_BLUR, F_PYR_RESAMPLE, F_GRAD_TENSOR, F_BM_TRANSPOSE,
     F_SOLVE_INIT, F_CALC_B, F_NEW_DISP, F_BM_ROWS, F_BM_COLS, F_RANGE, F_UPSAMPLE, F_NEGATE, F_FRAMES,
     F_WB_MASK, F_WB_SHARPEN, F_FILL_PULL, F_FILL_PUSH, F_FILL_IO, F_DIFF_FWD, F_DIFF_BWD, F_DIFF_GATHER,
-    F_FW_MESH, F_FW_RASTER, F_FW_SCAN, F_FW_FOLD, F_NFAM
+    F_FW_MESH, F_FW_RASTER, F_FW_SCAN, F_FW_FOLD, F_VEC, F_NFAM
 };
 extern const char* const PROF_NAMES[F_NFAM];
 
@@ -224,6 +225,28 @@ bool fw_flow_is_zero(const float* flow, long n);  // forward.rs:183-195 on a hos
 void fw_grid(FwArgs& A, float amount);             // mesh.rs:11-31
 void launch_fw_count(hipStream_t s, const FwArgs& A);
 void launch_fw_emit_fold(hipStream_t s, const FwArgs& A, uint64_t records);
+
+// Vector image coding (vectors.rs:57-130): one output image per job (blockIdx.z). The flow at
+// the working size is sampled at every output pixel; `finish` 1 writes the wire frame with the
+// rows turned back (Create Vectors runs on flipped frames), 0 the engine image as is.
+struct VecJob {
+    const float* flow;
+    void* out;
+    int fw, fh, W, H, fmt, finish;
+    float sx, sy, kx, ky;
+};
+void vec_job_scales(VecJob& J, float scale);  // vectors.rs:70-76
+void launch_vec_encode(hipStream_t s, const VecJob* d_jobs, const std::vector<VecJob>& jobs);
+// vectors.rs:474-521 and the hole flags of the inverted alpha (vectors.rs:543-549, holefill.rs:
+// 124-133); *gate is set when a hole exists
+void launch_vec_decode(hipStream_t s, const void* vec, int W, int H, int fmt, float max_displace, float* flow,
+                       uint8_t* holes, int* gate);
+void launch_vec_scale(hipStream_t s, float* flow, long n, float sx, float sy);  // vectors.rs:566-571
+// compose/vector.rs:370-397: T = (-1, -fx, -fy, -1); after the forward warp, the holes of the
+// inverted coverage and the two flow channels of the pushed image
+void launch_invert_prep(hipStream_t s, const float* flow, float* t, long n);
+void launch_invert_post(hipStream_t s, const float* pushed, const uint8_t* cov, float* flow, uint8_t* holes, int* gate,
+                        long n);
 
 // Taps of the solver's Gaussians and of the pyramid filter (conv.rs:44-86).
 Taps gaussian_taps(float r);

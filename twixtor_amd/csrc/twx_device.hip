@@ -155,7 +155,7 @@ const char* const PROF_NAMES[F_NFAM] = {"ingest", "luma", "pyr_blur5_band", "pyr
                                         "bm_rows", "bm_cols", "range", "upsample", "negate", "frames",
                                         "wb_mask", "wb_sharpen_blend", "fill_pull", "fill_push", "fill_io",
                                         "warp_blend_fwd", "warp_blend_bwd", "warp_blend_flow_gather",
-                                        "fw_mesh", "fw_raster", "fw_scan", "fw_fold"};
+                                        "fw_mesh", "fw_raster", "fw_scan", "fw_fold", "vector_coding"};
 thread_local Prof* g_prof = nullptr;
 
 hipEvent_t Prof::take() {
@@ -2018,7 +2018,8 @@ __global__ void k_fill_load(const FillPlane* __restrict__ planes) {
     if (P.chan == 4) {
         v = hole ? 0.0f : 1.0f;
     } else {
-        if (P.fmt == TWX_FMT_U8) v = (float)reinterpret_cast<const uint8_t*>(P.img)[i * 4 + P.chan];
+        if (P.fmt == FILL_FLOW2) v = reinterpret_cast<const float*>(P.img)[i * 2 + P.chan];
+        else if (P.fmt == TWX_FMT_U8) v = (float)reinterpret_cast<const uint8_t*>(P.img)[i * 4 + P.chan];
         else if (P.fmt == TWX_FMT_U16) v = (float)reinterpret_cast<const uint16_t*>(P.img)[i * 4 + P.chan];
         else v = reinterpret_cast<const float*>(P.img)[i * 4 + P.chan];
         if (hole) v = 0.0f;
@@ -2093,6 +2094,10 @@ __global__ void k_fill_store(const FillPlane* __restrict__ planes) {
     const long i = (long)y * W + x;
     const float wt = P.wts[0][i];
     const float v = wt != 0.0f ? P.val[0][i] / wt : P.sol[0][i];
+    if (P.fmt == FILL_FLOW2) {
+        reinterpret_cast<float*>(P.img)[i * 2 + P.chan] = v;
+        return;
+    }
     if (P.fmt == TWX_FMT_F32) {
         reinterpret_cast<float*>(P.img)[i * 4 + P.chan] = v;
         return;
@@ -2184,6 +2189,176 @@ static void launch_finish_range(hipStream_t s, const FrameJob* d_jobs, const std
     if (has[0]) TWX_LAUNCH(s, F_FILL_IO, bytes, k_finish<0><<<grd, blk, 0, s>>>(d_jobs));
     if (has[1]) TWX_LAUNCH(s, F_FILL_IO, bytes, k_finish<1><<<grd, blk, 0, s>>>(d_jobs));
     if (has[2]) TWX_LAUNCH(s, F_FILL_IO, bytes, k_finish<2><<<grd, blk, 0, s>>>(d_jobs));
+}
+
+// -------------------------------------------------------------------------------------------
+// Vector image coding (vectors.rs:57-130, :474-549) and the flow steps of Vectors In
+// (compose/vector.rs:335-397)
+
+// Flow::bilerp (warp.rs:415-442)
+__device__ __forceinline__ void flow_bilerp(const float* __restrict__ flow, int fw, int fh, float x, float y, float d[2]) {
+    int32_t jx = cvtt(x), jy = cvtt(y);
+    float gx = x - (float)jx, gy = y - (float)jy;
+    if (jx >= fw - 1) { jx = fw - 2; gx = 1.0f; }
+    if (jy >= fh - 1) { jy = fh - 2; gy = 1.0f; }
+    const long f00 = ((long)jy * fw + jx) * 2, f01 = f00 + (long)fw * 2;
+#pragma unroll
+    for (int k = 0; k < 2; k++) {
+        const float p00 = flow[f00 + k], p10 = flow[f00 + 2 + k], p01 = flow[f01 + k], p11 = flow[f01 + 2 + k];
+        const float top = (p10 - p00) * gx + p00;
+        const float bot = (p11 - p01) * gx + p01;
+        d[k] = (bot - top) * gy + top;
+    }
+}
+
+__device__ __forceinline__ void clamp_unit(float& v) {  // vectors.rs:132-141 (a NaN falls through)
+    if (v > 1.0f) v = 1.0f;
+    else if (-1.0f > v) v = -1.0f;
+}
+
+template <int FMT>
+__global__ void k_vec_encode(const VecJob* __restrict__ jobs) {
+    const VecJob& J = jobs[blockIdx.z];
+    if (J.fmt != FMT) return;
+    const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
+    if (x >= J.W || y >= J.H) return;
+    float d[2];
+    flow_bilerp(J.flow, J.fw, J.fh, (float)x * J.sx, (float)y * J.sy, d);
+    float a = d[0] * J.kx, b = d[1] * J.ky;
+    float e[4];
+    if (FMT == 2) {
+        e[0] = 1.0f;
+        e[1] = (a + 1.0f) * 0.5f;
+        e[2] = (b + 1.0f) * 0.5f;
+        e[3] = 0.0f;
+    } else {
+        clamp_unit(a);
+        clamp_unit(b);
+        a = a + 1.0f;
+        b = b + 1.0f;
+        const double q = FMT == 0 ? 127.0 : 32767.0;
+        const uint32_t mask = FMT == 0 ? 0xffu : 0xffffu;  // the low byte or word of the 64-bit truncation
+        e[0] = FMT == 0 ? 255.0f : 65535.0f;
+        e[1] = (float)((uint32_t)cvtt64d((double)a * q + 0.5) & mask);
+        e[2] = (float)((uint32_t)cvtt64d((double)b * q + 0.5) & mask);
+        e[3] = 0.0f;
+    }
+    const long n = (long)J.W * J.H;
+    if (J.finish) {
+        finish_pixel<FMT>(J.out, n, (long)(J.H - 1 - y) * J.W + x, e);
+        return;
+    }
+    typename Px<FMT>::T* o = reinterpret_cast<typename Px<FMT>::T*>(J.out) + ((long)y * J.W + x) * 4;
+#pragma unroll
+    for (int k = 0; k < 4; k++) o[k] = (typename Px<FMT>::T)e[k];
+}
+
+void vec_job_scales(VecJob& J, float scale) {
+    const float dw1 = (float)(J.W - 1), dh1 = (float)(J.H - 1), fw1 = (float)(J.fw - 1), fh1 = (float)(J.fh - 1);
+    J.sx = fw1 / dw1;
+    J.sy = fh1 / dh1;
+    J.kx = (dw1 / fw1) / scale;
+    J.ky = (dh1 / fh1) / scale;
+}
+
+void launch_vec_encode(hipStream_t s, const VecJob* d_jobs, const std::vector<VecJob>& jobs) {
+    int mw = 0, mh = 0;
+    bool has[3] = {false, false, false};
+    double bytes = 0;
+    for (auto& j : jobs) {
+        mw = std::max(mw, j.W);
+        mh = std::max(mh, j.H);
+        has[j.fmt] = true;
+        bytes += (double)j.fw * j.fh * 8 + (double)image_bytes(j.W, j.H, j.fmt);
+    }
+    if (!mw) return;
+    dim3 blk(256), grd((mw + 255) / 256, mh, (unsigned)jobs.size());
+    if (has[0]) TWX_LAUNCH(s, F_VEC, bytes, k_vec_encode<0><<<grd, blk, 0, s>>>(d_jobs));
+    if (has[1]) TWX_LAUNCH(s, F_VEC, bytes, k_vec_encode<1><<<grd, blk, 0, s>>>(d_jobs));
+    if (has[2]) TWX_LAUNCH(s, F_VEC, bytes, k_vec_encode<2><<<grd, blk, 0, s>>>(d_jobs));
+}
+
+template <int FMT>
+__global__ void k_vec_decode(const void* __restrict__ vec, long n, float md, float* __restrict__ flow,
+                             uint8_t* __restrict__ holes, int* gate) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    // the hole flag of the inverted alpha: u8 any non-zero, u16 and f32 at least 3.1e-5 or unordered
+    const double wide_eps = __longlong_as_double(0x3f0040bfe3b03e21ll);
+    float o0, o1;
+    bool hole;
+    if (FMT == 2) {
+        const float* p = reinterpret_cast<const float*>(vec) + i * 4;
+        if (p[0] == 0.0f) { o0 = o1 = 0.0f; }
+        else {
+            o0 = (p[1] * 2.0f - 1.0f) * md;
+            o1 = (p[2] * 2.0f - 1.0f) * md;
+        }
+        hole = !(wide_eps > (double)(1.0f - p[0]));
+    } else {
+        using T = typename Px<FMT>::T;
+        const T* p = reinterpret_cast<const T*>(vec) + i * 4;
+        const double den = FMT == 0 ? 254.0 : 65534.0;
+        const double mdd = (double)md;
+        if (p[0] == 0) { o0 = o1 = 0.0f; }
+        else {
+            o0 = (float)((((double)p[1] * 2.0) / den - 1.0) * mdd);
+            o1 = (float)((((double)p[2] * 2.0) / den - 1.0) * mdd);
+        }
+        hole = FMT == 0 ? p[0] != 0xff : !(wide_eps > (double)(uint16_t)(0xffff - p[0]));
+    }
+    flow[i * 2] = o0;
+    flow[i * 2 + 1] = o1;
+    holes[i] = hole ? 1 : 0;
+    if (hole) *gate = 1;  // the same value from every writer
+}
+
+void launch_vec_decode(hipStream_t s, const void* vec, int W, int H, int fmt, float max_displace, float* flow,
+                       uint8_t* holes, int* gate) {
+    const long n = (long)W * H;
+    const unsigned g = (unsigned)((n + 255) / 256);
+    const double bytes = (double)image_bytes(W, H, fmt) + 9.0 * n;
+    if (fmt == 0) TWX_LAUNCH(s, F_VEC, bytes, k_vec_decode<0><<<g, 256, 0, s>>>(vec, n, max_displace, flow, holes, gate));
+    else if (fmt == 1) TWX_LAUNCH(s, F_VEC, bytes, k_vec_decode<1><<<g, 256, 0, s>>>(vec, n, max_displace, flow, holes, gate));
+    else TWX_LAUNCH(s, F_VEC, bytes, k_vec_decode<2><<<g, 256, 0, s>>>(vec, n, max_displace, flow, holes, gate));
+}
+
+__global__ void k_vec_scale(float* __restrict__ flow, long n, float sx, float sy) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    flow[i * 2] = sx * flow[i * 2];
+    flow[i * 2 + 1] = sy * flow[i * 2 + 1];
+}
+
+void launch_vec_scale(hipStream_t s, float* flow, long n, float sx, float sy) {
+    TWX_LAUNCH(s, F_VEC, 16.0 * n, k_vec_scale<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(flow, n, sx, sy));
+}
+
+__global__ void k_invert_prep(const float* __restrict__ flow, float* __restrict__ t, long n) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    reinterpret_cast<float4*>(t)[i] = make_float4(-1.0f * 1.0f, -1.0f * flow[i * 2], -1.0f * flow[i * 2 + 1], -1.0f * 1.0f);
+}
+
+__global__ void k_invert_post(const float* __restrict__ pushed, const uint8_t* __restrict__ cov, float* __restrict__ flow,
+                              uint8_t* __restrict__ holes, int* gate, long n) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float4 p = reinterpret_cast<const float4*>(pushed)[i];
+    flow[i * 2] = p.y;
+    flow[i * 2 + 1] = p.z;
+    const uint8_t hole = (uint8_t)(255 - cov[i]);
+    holes[i] = hole;
+    if (hole) *gate = 1;
+}
+
+void launch_invert_prep(hipStream_t s, const float* flow, float* t, long n) {
+    TWX_LAUNCH(s, F_VEC, 24.0 * n, k_invert_prep<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(flow, t, n));
+}
+
+void launch_invert_post(hipStream_t s, const float* pushed, const uint8_t* cov, float* flow, uint8_t* holes, int* gate,
+                        long n) {
+    TWX_LAUNCH(s, F_VEC, 26.0 * n, k_invert_post<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(pushed, cov, flow, holes, gate, n));
 }
 
 // -------------------------------------------------------------------------------------------
