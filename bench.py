@@ -49,6 +49,9 @@ def main():
     ap.add_argument("--diff-steps", type=int, default=50,
                     help="refinement steps of the differentiable warp+blend leg (1280x720); 0 skips it")
     ap.add_argument("--diff-only", action="store_true", help="run only the differentiable warp+blend leg")
+    ap.add_argument("--vectors-frames", type=int, default=60,
+                    help="frames of the 3840x2160 Create Vectors / Vectors In leg; 0 skips it")
+    ap.add_argument("--vectors-only", action="store_true", help="run only the Create Vectors / Vectors In leg")
     args = ap.parse_args()
     if args.copy_probe:
         import torch
@@ -79,6 +82,9 @@ def main():
         return
     if args.diff_only:
         print(json.dumps({"diff": diff_leg(args.diff_steps)}), flush=True)
+        return
+    if args.vectors_only:
+        print(json.dumps({"vectors": vectors_leg(HipBackend(0), args.vectors_frames, args.steps, args.warmup)}), flush=True)
         return
     spec = synth.spec_1080p(frames=args.frames)
     gen = synth.Clip(spec, device=dev)
@@ -153,6 +159,7 @@ def main():
     torch.cuda.empty_cache()
     batch = batch_leg(be, args.batch_clips, args.steps, args.warmup) if args.batch_clips > 0 else None
     diff = diff_leg(args.diff_steps) if args.diff_steps > 0 else None
+    vectors = vectors_leg(be, args.vectors_frames, args.steps, args.warmup) if args.vectors_frames > 1 else None
 
     line = {
         "metric": "1080p retime output frames/s",
@@ -176,6 +183,7 @@ def main():
         "per_pair": per_pair,
         "batch": batch,
         "diff": diff,
+        "vectors": vectors,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
@@ -271,6 +279,89 @@ def diff_leg(steps):
                         "flows 640x360 from the solver + N(0, 0.5^2) px, blend at wa = 0.5, L2 loss, Adam lr 0.05",
             "steps": steps, "ms_per_step": round(el * 1000.0 / steps, 3),
             "loss_first": losses[0], "loss_last": losses[-1], "kernels": kern}
+
+
+def _families(prof, steps=1):
+    out = {}
+    for name, v in sorted(prof.items(), key=lambda kv: -kv[1]["ms"]):
+        n = max(v["launches"], 1)
+        gbs = v["bytes"] / (v["ms"] / 1000.0) / 1e9 if v["ms"] > 0 else 0.0
+        out[name] = {"ms": round(v["ms"] / steps, 3), "launches": v["launches"] // steps, "GBs": round(gbs, 1),
+                     "frac_of_hbm_peak": round(gbs / HBM_PEAK_GBS, 4)}
+    return out
+
+
+def vectors_leg(be, frames, steps, warmup):
+    """Create Vectors and the Vectors In read-back at 3840x2160 RGBA f32 (BASELINE.json
+    configs[2]): a clip under a global affine motion with 10 discs; Create Vectors with Normal
+    (half res) vectors, track Previous, Max Displace 2048: output n = the encoded flow of frame n
+    towards frame n - 1, for n = 1 .. frames - 1, one twx_create_vectors_dev call over the clip
+    (device-resident). The read-back (Vectors In at 50 %, warping 2, Blend, one vector layer) of
+    two in-between frames: the layer's flow (decode, hole fill, Vec Scale), its inversion through
+    the Gouraud forward warp, and the warping code 2 composition; these calls take host buffers,
+    so their wall time includes PCIe, and the kernel time is reported beside it."""
+    import numpy as np
+    import torch
+
+    from twixtor_amd import synth
+    from twixtor_amd.backend import flow_settings
+
+    dev = torch.device("cuda:0")
+    spec = synth.spec_4k_vectors(frames=frames)
+    gen = synth.Clip(spec, device=dev)
+    W, H, fmt = spec.width, spec.height, 2
+    clip = torch.cat([gen.wire(t) for t in range(frames)]).contiguous()
+    fb = clip.numel() // frames
+    xs, ys = list(range(1, frames)), list(range(0, frames - 1))
+    out = torch.empty(fb * len(xs), dtype=torch.uint8, device=dev)
+    fs = flow_settings(quality=4)
+    md = 2048.0
+    torch.cuda.synchronize()
+
+    def run():
+        be.create_vectors_dev(fs, W, H, fmt, clip.data_ptr(), frames, xs, ys, md, out.data_ptr())
+
+    for _ in range(warmup):
+        run()
+    be.profile(True)
+    be.profile_reset()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        run()
+    torch.cuda.synchronize()
+    t_cv = (time.perf_counter() - t0) / steps
+    prof_cv = be.profile_read()
+
+    def engine(k):  # wire (planes G, B, R, A) -> engine A,R,G,B (alpha 1 or the encoded white)
+        p = clip[k * fb:(k + 1) * fb].view(torch.float32).reshape(4, H, W) if k >= 0 else None
+        return torch.stack([p[3], p[2] * p[3], p[0] * p[3], p[1] * p[3]], -1).cpu().numpy().reshape(-1)
+
+    be.profile_reset()
+    rb_wall, rb = [], None
+    for k in (1, frames // 2):
+        enc = out[(k - 1) * fb:k * fb].view(torch.float32).reshape(4, H, W)
+        vec = torch.stack([enc[3], enc[2], enc[0], enc[1]], -1).cpu().numpy().reshape(-1)
+        a, b = engine(k - 1), engine(k)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        f_ba = be.flow_from_vectors(vec, W, H, fmt, md, (1.0, 1.0), True)
+        f_ab = be.invert_flow(f_ba, W, H, True, True)
+        rb = be.compose_vectors(1, 0.5, a, b, W, H, fmt, f_ab, f_ba, W, H, True, warp_mode=2, gouraud=True)
+        rb_wall.append(time.perf_counter() - t0)
+    prof_rb = be.profile_read()
+    be.profile(False)
+    kern_rb = sum(v["ms"] for v in prof_rb.values()) / len(rb_wall)
+    del clip, out
+    torch.cuda.empty_cache()
+    return {"workload": f"Create Vectors 3840x2160 RGBA f32, {frames} frames (affine background 0.2 deg, x1.002, (3, -1) px per "
+                        "frame; 10 discs up to 40 px/frame), Normal vectors (working 1920x1080), track Previous, Max Displace "
+                        f"2048 -> {len(xs)} encoded frames; Vectors In read-back at 50 %, warping 2, Blend",
+            "create_vectors": {"output_frames": len(xs), "ms": round(t_cv * 1000.0, 2),
+                               "frames_per_s": round(len(xs) / t_cv, 1), "kernels": _families(prof_cv, steps)},
+            "read_back": {"frames": len(rb_wall), "wall_ms_per_frame_pcie_inclusive": round(1000.0 * float(np.mean(rb_wall)), 2),
+                          "kernel_ms_per_frame": round(kern_rb, 3), "kernels": _families(prof_rb, len(rb_wall)),
+                          "finite": bool(np.isfinite(rb).all())}}
 
 
 def cpu_baseline(cfg, clip, out, n_in, n_out, fb, budget_s):

@@ -37,6 +37,8 @@ class ClipSpec:
     noise_sigma: float = 0.0       # per-pixel Gaussian noise, in units of the [0, 1] range
     seed: int = 0
     cell: float = 16.0             # lattice spacing of the coarsest octave, in pixels (scaled by size)
+    bg_rotation_deg: float = 0.0   # background rotation per frame about the frame centre
+    bg_zoom: float = 1.0           # background scale factor per frame about the frame centre
 
 
 def _lattice(gen: torch.Generator, gh: int, gw: int, device) -> torch.Tensor:
@@ -90,6 +92,14 @@ class Clip:
         vx, vy = s.bg_velocity
         X = self.X - vx * t
         Y = self.Y - vy * t
+        if s.bg_rotation_deg != 0.0 or s.bg_zoom != 1.0:
+            # the background point that lands on (X, Y): undo the rotation and zoom about the centre
+            cx, cy = (s.width - 1) / 2.0, (s.height - 1) / 2.0
+            ang = -math.radians(s.bg_rotation_deg) * t
+            z = s.bg_zoom ** (-t)
+            dx, dy = X - cx, Y - cy
+            X = (math.cos(ang) * dx - math.sin(ang) * dy) * z + cx
+            Y = (math.sin(ang) * dx + math.cos(ang) * dy) * z + cy
         wsum = sum(w for _, _, w in self.lats)
         chans = []
         for ch in range(3):
@@ -186,3 +196,12 @@ def spec_720p_2x(seed: int = 4, frames: int = 9) -> ClipSpec:
     rng = np.random.RandomState(seed)
     discs = _rng_discs(rng, 6, 1280, 720, 20.0, 80.0, 8.0)
     return ClipSpec(1280, 720, frames, 32, (2.1, -0.9), discs, 0.0, seed, 16.0 * 8)
+
+
+def spec_4k_vectors(seed: int = 2, frames: int = 60) -> ClipSpec:
+    """3840x2160 RGBA float32, 60 frames: the noise background under a global affine motion
+    (0.2 degrees and x1.002 per frame about the centre, (3, -1) px per frame) and 10 discs with r
+    uniform in 30-160 px and seeded velocities up to 40 px/frame."""
+    rng = np.random.RandomState(seed)
+    discs = _rng_discs(rng, 10, 3840, 2160, 30.0, 160.0, 40.0)
+    return ClipSpec(3840, 2160, frames, 32, (3.0, -1.0), discs, 0.0, seed, 16.0 * 24, 0.2, 1.002)
