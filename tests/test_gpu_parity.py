@@ -218,3 +218,26 @@ def test_batch_of_mixed_clips_equals_single_clip_calls(backend):
     for k in (2, 3):
         want = cpuref.retime_frames(cfgs[k].struct(), clips[k].cpu().numpy().tobytes(), 6, 0, counts[k])
         assert outs[k].cpu().numpy().tobytes() == want, f"clip {k} vs oracle"
+
+
+def test_every_served_retime_time_hash_on_device(backend):
+    """The retime-time grid on the device (the time map, the rates and the fast path at three
+    depths, through twx_retime_frames): the same 279 hashes as the oracle's pin, and the same 6
+    Weighted Blend cases over non-finite float samples refused (status 3)."""
+    from test_oracle_pins import TIME_CPU_PATH_ONLY, time_case_run, time_case_want, time_cases
+
+    from twixtor_amd._ffi import StageError
+
+    bad, refused = [], 0
+    for c in time_cases():
+        try:
+            out = time_case_run(c, lambda cfg, clip, n: backend.retime_frames(cfg, clip, n, 0, n))
+        except StageError as e:
+            # Weighted Blend without vectors over non-finite float samples: refused as by the oracle
+            assert e.status == 3 and c["source"].startswith("special") and "frame_interp=3" in c["params"], (c["id"], e)
+            refused += 1
+            continue
+        if hashlib.sha256(out).hexdigest() != time_case_want(c):
+            bad.append(c["id"])
+    assert not bad, bad
+    assert refused == 6

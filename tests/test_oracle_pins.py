@@ -406,3 +406,58 @@ def test_a_planted_mistake_breaks_the_hashes():
     assert hashlib.sha256(out).hexdigest() == c["sha256"]
     out[1000] ^= 1
     assert hashlib.sha256(out).hexdigest() != c["sha256"]
+
+
+# ---- the retime-time grid: the time map, the rates and the fast path at three depths ----
+
+# The reference CPU path's output differs from the original plugin's in these cases: NaN payload
+# bits of two special-value cases (graded "pass" by the reference's NaN-aware float comparison) and
+# the two 6000-frame rate cases (not graded against the original by the reference). For them the
+# pin is the reference CPU path's own recorded hash of the same case.
+TIME_CPU_PATH_ONLY = {
+    "retime.special-special32-s50-i2.special32-5x2.d32.f42c3e54",
+    "retime.special-special32-s25-i2.special32-4x3.d32.eb6d579c",
+    "retime.long6000-rate29_97.ident-1x1.d8.d634b990",
+    "retime.long6000-rate29_97003.ident-1x1.d8.9b45a592",
+}
+
+
+def time_cases():
+    with open(os.path.join(HERE, "golden", "retime_time_cases.json")) as fh:
+        return json.load(fh)
+
+
+def time_case_run(c, render):
+    import retime_time_inputs as rti
+    from twixtor_amd.params import Retime
+
+    cfg = Retime(c["width"], c["height"], c["depth"], c["fps"], c["params"] + ["vectors=1"])
+    clip = rti.clip_bytes(c["source"], c["width"], c["height"], c["frames"], c["depth"])
+    return render(cfg, clip, c["frames"])
+
+
+def time_case_want(c):
+    return c["cpu_path_sha256"] if c["id"] in TIME_CPU_PATH_ONLY else c["sha256"]
+
+
+def test_every_served_retime_time_hash():
+    """285 cases of the reference's retime-time grid in the parameters this backend serves (Speed
+    mode, constant and keyed speeds, Nearest / Blend / Weighted Blend, rates with In FPS is Out FPS
+    off, long clips, extreme integer values and float special values): 275 equal the original
+    plugin's SHA-256, 4 the reference CPU path's own (TIME_CPU_PATH_ONLY); the oracle refuses the
+    6 Weighted Blend cases over non-finite float samples, whose NaN rule it does not restate."""
+    cases = time_cases()
+    assert len(cases) == 285
+    bad, refused, n_orig = [], 0, 0
+    for c in cases:
+        try:
+            out = time_case_run(c, lambda cfg, clip, n: cpuref.retime_frames(cfg.struct(), clip, n, 0, n))
+        except RuntimeError as e:
+            assert "non-finite float input to the weighted blend" in str(e), (c["id"], e)
+            refused += 1
+            continue
+        if hashlib.sha256(out).hexdigest() != time_case_want(c):
+            bad.append(c["id"])
+        n_orig += c["id"] not in TIME_CPU_PATH_ONLY
+    assert not bad, bad
+    assert refused == 6 and n_orig == 275

@@ -897,6 +897,11 @@ int retime_batch_dev(twx_handle* h, DevPool& pool, const std::vector<ClipRun>& r
     HIPCHECK(hipStreamSynchronize(s));
     HIPCHECK(hipGetLastError());
     h->stats.device_ms += now_ms() - t0;
+    if (*hbad & 2) {
+        err = "weighted blend without vectors: non-finite or huge float samples (the NaN rule of blend.rs:280-300) are not "
+              "served";
+        return TWX_E_UNSUPPORTED;
+    }
     if (*hbad) { err = "inverse warp: a sample coordinate is NaN or never converges, which is not supported"; return TWX_E_UNSUPPORTED; }
     return TWX_OK;
 }

@@ -1558,7 +1558,15 @@ __global__ void k_frames(const FrameJob* __restrict__ jobs) {
         load4<FMT>(F.a, mi, pa);
         load4<FMT>(F.b, mi, pb);
         if (FMT == 2 && F.plus_zero) {
-            for (int k = 0; k < 4; k++) { pa[k] = pa[k] + 0.0f; pb[k] = pb[k] + 0.0f; }
+            // blend.rs:280-300 takes another route for non-finite or huge samples (its NaN rule),
+            // which is not served here: flag the frame (status 3)
+            bool odd = false;
+            for (int k = 0; k < 4; k++) {
+                odd |= !isfinite(pa[k]) || !isfinite(pb[k]) || fabsf(pa[k]) > 1e30f || fabsf(pb[k]) > 1e30f;
+                pa[k] = pa[k] + 0.0f;
+                pb[k] = pb[k] + 0.0f;
+            }
+            if (odd) atomicOr(F.bad, 2);
         }
         blend_pixel<FMT>(pa, pb, F.blend_mode, F.weight, F.lerp_w, e);
     } else if (F.kind == 1 && F.pre_a) {
