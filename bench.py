@@ -40,7 +40,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="budget of the CPU-baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--per-pair", action="store_true", help="also time one launch sequence per output frame")
-    ap.add_argument("--no-profile", action="store_true", help="time the steps without the per-launch HIP events")
+    ap.add_argument("--no-profile", action="store_true", help="skip the profiled pass (per-launch HIP events)")
+    ap.add_argument("--no-graphs", action="store_true", help="time plain launches instead of the captured graph")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run one device-to-device copy of MIB MiB (calibrates the HBM traffic counters)")
     ap.add_argument("--batch-clips", type=int, default=64,
@@ -97,15 +98,16 @@ def main():
     torch.cuda.synchronize()
 
     be = HipBackend(0)
-    be.profile(not args.no_profile)
+    be.graphs(not args.no_graphs)
 
     def step():
         be.retime_frames_dev(cfg, clip.data_ptr(), spec.frames, 0, n_out, out.data_ptr(), params)
 
-    for _ in range(args.warmup):
+    # timed steps: the whole clip per step; with graphs on, the first call (warm-up) captures the
+    # batched render and the timed steps replay it
+    for _ in range(max(args.warmup, 1)):
         step()
     torch.cuda.synchronize()
-    be.profile_reset()
     s0 = be.stats()
     t0 = time.perf_counter()
     for _ in range(args.steps):
@@ -113,9 +115,23 @@ def main():
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
     s1 = be.stats()
-    prof = be.profile_read()
     ms_per_step = elapsed * 1000.0 / args.steps
     fps = n_out * args.steps / elapsed
+    # the per-kernel HIP-event timing: the same steps again as plain launches, each launch
+    # bracketed by events on the library's stream (kernel durations do not depend on the graph)
+    prof, ms_profiled = {}, None
+    if not args.no_profile:
+        be.profile(True)
+        step()
+        torch.cuda.synchronize()
+        be.profile_reset()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            step()
+        torch.cuda.synchronize()
+        ms_profiled = (time.perf_counter() - t0) * 1000.0 / args.steps
+        prof = be.profile_read()
+        be.profile(False)
 
     # dominant kernel by HIP-event time over the timed steps
     kern = max(prof, key=lambda k: prof[k]["ms"]) if prof else None
@@ -140,10 +156,10 @@ def main():
                             "launches_per_step": v["launches"] // args.steps,
                             "GBs": round(v["bytes"] / (v["ms"] / 1000.0) / 1e9, 1) if v["ms"] > 0 else 0.0}
                         for n, v in sorted(prof.items(), key=lambda kv: -kv[1]["ms"])}}
-    be.profile(False)
 
     per_pair = None
     if args.per_pair:
+        be.graphs(False)  # one plain launch sequence per output frame, as the reference's loop
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         for n in range(n_out):
@@ -151,6 +167,7 @@ def main():
         torch.cuda.synchronize()
         pp = time.perf_counter() - t0
         per_pair = {"ms_per_clip": round(pp * 1000.0, 2), "batched_speedup": round(pp * 1000.0 / ms_per_step, 2)}
+        be.graphs(not args.no_graphs)
 
     cpu = None
     if not args.no_cpu:
@@ -177,7 +194,9 @@ def main():
         "config": {"workload": f"retime 1920x1080 RGBA f32, {spec.frames} frames, {SPEED}, Best, Blend -> {n_out} output frames",
                    "width": 1920, "height": 1080, "frames_in": spec.frames, "frames_out": n_out,
                    "unique_solves_per_step": (s1["pair_solves"] - s0["pair_solves"]) // args.steps,
-                   "vector_frames_per_step": (s1["composes"] - s0["composes"]) // args.steps},
+                   "vector_frames_per_step": (s1["composes"] - s0["composes"]) // args.steps,
+                   "graph_replays_timed": s1["graph_replays"] - s0["graph_replays"],
+                   "ms_per_step_profiled_plain_launches": None if ms_profiled is None else round(ms_profiled, 3)},
         "roofline": roofline,
         "path": path,
         "per_pair": per_pair,
@@ -192,8 +211,9 @@ def main():
 def batch_leg(be, n_clips, steps, warmup):
     """The mixed-size batch: n_clips RGBA 8-bit clips of 24 frames (widths 320..2048, heights
     180..1152), each retimed at 33.3333 % into 70 output frames. Batched: one twx_retime_batch call
-    over every clip. Against: one call per output frame (the reference's render loop), which solves
-    the pair of every in-between frame. The output bytes of both must be equal."""
+    over every clip (captured into a graph by the warm-up call and replayed by the timed ones).
+    Against: one plain call per output frame (the reference's render loop), which solves the pair
+    of every in-between frame. The output bytes of both must be equal."""
     import torch
 
     from twixtor_amd import synth
@@ -225,6 +245,7 @@ def batch_leg(be, n_clips, steps, warmup):
     s1 = be.stats()
     solves = (s1["pair_solves"] - s0["pair_solves"]) // steps
     outs2 = [torch.zeros_like(o) for o in outs]
+    be.graphs(False)  # the per-pair form: one plain launch sequence per output frame
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for k, cfg in enumerate(cfgs):
@@ -233,6 +254,7 @@ def batch_leg(be, n_clips, steps, warmup):
             be.retime_frames_dev(cfg, cptr[k], n_in, i, 1, outs2[k].data_ptr() + i * fb, structs[k])
     torch.cuda.synchronize()
     t_pp = time.perf_counter() - t0
+    be.graphs(True)
     same = all(torch.equal(a, b) for a, b in zip(outs, outs2))
     px = sum(s.width * s.height for s in specs)
     return {"workload": f"{n} RGBA 8-bit clips, widths 320-2048, heights 180-1152, {n_in} frames each, "

@@ -257,8 +257,17 @@ typedef struct twx_stats {
     uint64_t solve_calls, pair_solves, pairs_reused, composes, fast_frames;
     uint64_t pyramid_builds, pyramid_hits;
     double device_ms;
+    uint64_t graph_replays;  /* batched renders replayed from a captured graph */
 } twx_stats;
 int twx_get_stats(twx_handle* h, twx_stats* out);
+
+/* Graph capture of batched renders (on by default; no counterpart in the reference, whose render
+ * loop launches per frame): a twx_retime_frames_dev or twx_retime_batch call runs on memory of its
+ * own and captures its launches (ingest, pyramids, every solve, compose, finish) into one hipGraph;
+ * a later call with the same parameters, pointers and frame ranges replays it in one launch. The
+ * captured plan holds its own device and pinned memory until another plan replaces it or graphs are
+ * turned off. With profiling on, calls take the plain path (per-launch events). */
+int twx_set_graphs(twx_handle* h, int32_t on);
 
 /* Device pointer of the handle's HIP stream (hipStream_t), so a caller can order its own work. */
 void* twx_stream(twx_handle* h);

@@ -80,6 +80,10 @@ class HipBackend:
         _ffi.lib().twx_get_stats(self._h, ctypes.byref(s))
         return {k: getattr(s, k) for k, _ in s._fields_}
 
+    def graphs(self, on: bool = True):
+        """Graph capture and replay of batched renders (twx_set_graphs)."""
+        _ffi.lib().twx_set_graphs(self._h, 1 if on else 0)
+
     # ---- per-kernel-family HIP-event timing and algorithmic bytes ----
     def profile(self, on: bool = True):
         _ffi.lib().twx_profile_enable(self._h, 1 if on else 0)

@@ -14,6 +14,7 @@
 #include <chrono>
 #include <cmath>
 #include <map>
+#include <memory>
 #include <mutex>
 #include <thread>
 #include <tuple>
@@ -42,6 +43,24 @@ struct BlockArena {
     void release();
 };
 
+// A batched render captured into a graph, with its own memory (the arenas the captured launches
+// and table copies point into), replayed when a call repeats the same arguments.
+struct GraphPlan {
+    std::string key;
+    hipGraphExec_t exec = nullptr;
+    BlockArena dev, host;
+    int* bad = nullptr;     // the render's device flags (see BatchFlags)
+    int* states = nullptr;
+    size_t nsolves = 0;
+    twx_stats delta{};      // the counters one render adds
+    GraphPlan() { host.pinned = true; }
+    ~GraphPlan() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        dev.release();
+        host.release();
+    }
+};
+
 struct twx_handle {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -53,6 +72,9 @@ struct twx_handle {
     Prof prof;
     BlockArena dev;
     BlockArena host;
+    bool graphs = true;                // capture batched renders into graphs and replay them
+    std::unique_ptr<GraphPlan> graph;  // the last captured render
+    uint64_t graph_replays = 0;
 };
 
 namespace {
@@ -114,32 +136,37 @@ namespace {
 // returns, so the next call may reuse the memory.
 class DevPool {
   public:
-    DevPool(twx_handle* h) : h_(h) {
-        h->dev.reset();
-        h->host.reset();
+    DevPool(twx_handle* h) : dev_(h->dev), host_(h->host), s_(h->stream) { reset(); }
+    // over other arenas (a graph plan's own memory)
+    DevPool(BlockArena& dev, BlockArena& host, hipStream_t s) : dev_(dev), host_(host), s_(s) { reset(); }
+    void reset() {
+        dev_.reset();
+        host_.reset();
     }
     template <class T>
     T* alloc(size_t count) {
-        return (T*)h_->dev.take(count * sizeof(T));
+        return (T*)dev_.take(count * sizeof(T));
     }
     template <class T>
     T* upload(const std::vector<T>& v) {
         T* d = alloc<T>(v.size());
         if (!v.empty()) {
             size_t bytes = v.size() * sizeof(T);
-            char* st = h_->host.take(bytes);
+            char* st = host_.take(bytes);
             memcpy(st, v.data(), bytes);
-            HIPCHECK(hipMemcpyAsync(d, st, bytes, hipMemcpyHostToDevice, h_->stream));
+            HIPCHECK(hipMemcpyAsync(d, st, bytes, hipMemcpyHostToDevice, s_));
         }
         return d;
     }
     template <class T>
     T* pinned(size_t count) {
-        return (T*)h_->host.take(count * sizeof(T));
+        return (T*)host_.take(count * sizeof(T));
     }
 
   private:
-    twx_handle* h_;
+    BlockArena& dev_;
+    BlockArena& host_;
+    hipStream_t s_;
 };
 
 // Serialises the calls on a handle and refuses a nested call from the same thread
@@ -390,13 +417,14 @@ struct SolveIn {
     float* out;      // w0 * h0 * 2 floats, the flow X(p) ~ Y(p + out(p))
 };
 
-// Runs every solve, in chunks that fit the scratch budget. Returns a status.
-int run_solves(hipStream_t s, DevPool& pool, const std::vector<SolveIn>& in, const twx_flow_settings& fs,
-               uint64_t scratch_budget, std::string& err) {
+// Enqueues every solve, in chunks that fit the scratch budget, with no host synchronisation:
+// solve k keeps its state in states[4k .. 4k + 3] (device; [2] is set for a NaN in its flow).
+int enqueue_solves(hipStream_t s, DevPool& pool, const std::vector<SolveIn>& in, const twx_flow_settings& fs,
+                   uint64_t scratch_budget, int* states, std::string& err) {
     if (in.empty()) return TWX_OK;
     // chunks by the scratch bytes of each job: flow[1] 2 + grad 2 + ta 3 + tb 2 + tmp 3 floats/pixel
     std::vector<std::pair<size_t, size_t>> chunks;
-    size_t max_px = 0, max_n = 0;
+    size_t max_px = 0;
     for (size_t i0 = 0; i0 < in.size();) {
         size_t i1 = i0, px_sum = 0;
         while (i1 < in.size()) {
@@ -407,13 +435,10 @@ int run_solves(hipStream_t s, DevPool& pool, const std::vector<SolveIn>& in, con
         }
         chunks.push_back({i0, i1});
         max_px = std::max(max_px, px_sum);
-        max_n = std::max(max_n, i1 - i0);
         i0 = i1;
     }
     // one scratch set, reused by every chunk in stream order
     float* scratch = pool.alloc<float>(max_px * 12);
-    int* state = pool.alloc<int>(max_n * 4);
-    int* hstate = pool.pinned<int>(max_n * 4);
     for (auto& ch : chunks) {
         int n = (int)(ch.second - ch.first);
         std::vector<SolveJob> jobs(n);
@@ -443,18 +468,36 @@ int run_solves(hipStream_t s, DevPool& pool, const std::vector<SolveIn>& in, con
             J.tb = sp + px * 7;
             J.tmp = sp + px * 9;
             sp += px * 12;
-            J.state = state + k * 4;
+            J.state = states + (ch.first + k) * 4;
             J.max_w = si.geo.w0;
             J.max_h = si.geo.h0;
         }
         SolveJob* d = pool.upload(jobs);
         launch_solve(s, d, jobs, fs.skip_refine != 0);  // starts with the per-job init kernel
-        HIPCHECK(hipMemcpyAsync(hstate, state, (size_t)n * 4 * sizeof(int), hipMemcpyDeviceToHost, s));
-        HIPCHECK(hipStreamSynchronize(s));
-        for (int k = 0; k < n; k++)
-            if (hstate[k * 4 + 2]) { err = "the solved flow holds a NaN"; return TWX_E_NAN_OUTPUT; }
     }
+    (void)err;
     return TWX_OK;
+}
+
+// After the stream's work: TWX_E_NAN_OUTPUT when a solved flow holds a NaN (the reference's
+// download_flow refusal, device/solver/mod.rs:575-582).
+int check_solve_states(hipStream_t s, DevPool& pool, const int* states, size_t n, std::string& err) {
+    if (!n) return TWX_OK;
+    int* hs = pool.pinned<int>(n * 4);
+    HIPCHECK(hipMemcpyAsync(hs, states, n * 4 * sizeof(int), hipMemcpyDeviceToHost, s));
+    HIPCHECK(hipStreamSynchronize(s));
+    for (size_t k = 0; k < n; k++)
+        if (hs[k * 4 + 2]) { err = "the solved flow holds a NaN"; return TWX_E_NAN_OUTPUT; }
+    return TWX_OK;
+}
+
+// Runs every solve and checks the flows. Returns a status.
+int run_solves(hipStream_t s, DevPool& pool, const std::vector<SolveIn>& in, const twx_flow_settings& fs,
+               uint64_t scratch_budget, std::string& err) {
+    int* states = pool.alloc<int>(std::max<size_t>(in.size(), 1) * 4);
+    int st = enqueue_solves(s, pool, in, fs, scratch_budget, states, err);
+    if (st) return st;
+    return check_solve_states(s, pool, states, in.size(), err);
 }
 
 // ---- frame composition tables -----------------------------------------------------------------
@@ -652,13 +695,23 @@ struct ClipRun {
     uint8_t* out;
 };
 
+// What a batched render leaves on the device for the host to check after the stream's work: the
+// per-frame flags (bad[0]: 1 a NaN warp coordinate, 2 non-finite samples in the Weighted Blend
+// without vectors) and the solves' states.
+struct BatchFlags {
+    int* bad = nullptr;
+    int* states = nullptr;
+    size_t nsolves = 0;
+};
+
 // The batched render of every output frame of every run: one plan on the host, then one ingest of
 // every source frame, one pyramid build, one solve per flow-settings group, one compose-and-finish
 // launch set over every output frame. Frame pairs are keyed per run, so identical pairs of a clip
-// are solved once.
-int retime_batch_dev(twx_handle* h, DevPool& pool, const std::vector<ClipRun>& runs, std::string& err) {
+// are solved once. Everything is enqueued on the handle's stream with no host synchronisation and
+// no memset, so the same call can be captured into a graph; `delta` receives the counters.
+int retime_batch_enqueue(twx_handle* h, DevPool& pool, const std::vector<ClipRun>& runs, BatchFlags& flags,
+                         twx_stats& delta, std::string& err) {
     hipStream_t s = h->stream;
-    double t0 = now_ms();
     int st;
     struct Plan {
         twx_selection sel;
@@ -760,7 +813,7 @@ int retime_batch_dev(twx_handle* h, DevPool& pool, const std::vector<ClipRun>& r
     auto solve = [&](int sx, int sy, int vq) -> int {
         SolveKey key{sx, sy, vq};
         auto it = solve_of.find(key);
-        if (it != solve_of.end()) { h->stats.pairs_reused++; return it->second; }
+        if (it != solve_of.end()) { delta.pairs_reused++; return it->second; }
         int px = pyramid(sx, vq), py = pyramid(sy, vq);
         Geo g = pyrs[px].geo;
         SolveIn si{pyrs[px].levels, pyrs[py].levels, g, pool.alloc<float>((size_t)g.w0 * g.h0 * 2)};
@@ -801,11 +854,14 @@ int retime_batch_dev(twx_handle* h, DevPool& pool, const std::vector<ClipRun>& r
                          [](const PyrIn& a, const PyrIn& b) { return (long)a.W * a.H > (long)b.W * b.H; });
         build_pyramids(s, pool, sorted);
     }
-    h->stats.pyramid_builds += pyrs.size();
-    h->stats.pair_solves += solves.size();
+    delta.pyramid_builds += pyrs.size();
+    delta.pair_solves += solves.size();
+    flags.nsolves = solves.size();
+    flags.states = pool.alloc<int>(std::max<size_t>(solves.size(), 1) * 4);
     {
         std::map<std::tuple<int, int, uint32_t>, std::vector<SolveIn>> groups;
         for (size_t k = 0; k < solves.size(); k++) groups[solve_group[k]].push_back(solves[k]);
+        size_t state_off = 0;
         for (auto& kv : groups) {
             twx_flow_settings fs{};
             fs.quality = std::get<0>(kv.first);
@@ -815,13 +871,15 @@ int retime_batch_dev(twx_handle* h, DevPool& pool, const std::vector<ClipRun>& r
             std::stable_sort(kv.second.begin(), kv.second.end(), [](const SolveIn& a, const SolveIn& b) {
                 return (long)a.geo.w0 * a.geo.h0 > (long)b.geo.w0 * b.geo.h0;
             });
-            if ((st = run_solves(s, pool, kv.second, fs, h->budget / 2, err))) return st;
+            if ((st = enqueue_solves(s, pool, kv.second, fs, h->budget / 2, flags.states + state_off * 4, err))) return st;
+            state_off += kv.second.size();
         }
     }
 
     // compose and finish every output frame; the NaN flag and one has-a-hole flag per output frame
     int* bad = pool.alloc<int>(1 + (size_t)total_frames);
-    HIPCHECK(hipMemsetAsync(bad, 0, sizeof(int) * (1 + (size_t)total_frames), s));
+    launch_zero(s, bad, 1 + (size_t)total_frames);
+    flags.bad = bad;
     std::vector<FrameJob> jobs;
     jobs.reserve(total_frames);
     for (size_t r = 0; r < runs.size(); r++) {
@@ -844,7 +902,7 @@ int retime_batch_dev(twx_handle* h, DevPool& pool, const std::vector<ClipRun>& r
             F.frac = 1.0f - pl.sel.wa;
             if (!pl.vector) {
                 F.kind = 0;
-                h->stats.fast_frames++;
+                delta.fast_frames++;
                 if (pl.sel.interp == 0) {
                     F.blend_mode = pl.sel.wa >= 0.5f ? 0 : 1;
                 } else if (pl.sel.interp == 1) {
@@ -856,7 +914,7 @@ int retime_batch_dev(twx_handle* h, DevPool& pool, const std::vector<ClipRun>& r
                 }
             } else {
                 if (W < 2 || H < 2) { err = "inverse warp needs images of at least 2x2"; return TWX_E_MISMATCH; }
-                h->stats.composes++;
+                delta.composes++;
                 Geo g = geo_of[{(int)r, pl.sel.vectors_quality}];
                 F.fw = g.w0;
                 F.fh = g.h0;
@@ -892,11 +950,19 @@ int retime_batch_dev(twx_handle* h, DevPool& pool, const std::vector<ClipRun>& r
         if ((st = run_fills(s, pool, jobs, err))) return st;
         launch_finish(s, dj, jobs);
     }
+    return TWX_OK;
+}
+
+// After the stream's work of a batched render: the NaN flags of the solves and the frame flags.
+// `pool` only provides the pinned read-back buffers.
+int retime_batch_check(twx_handle* h, DevPool& pool, const BatchFlags& flags, std::string& err) {
+    hipStream_t s = h->stream;
     int* hbad = pool.pinned<int>(1);
-    HIPCHECK(hipMemcpyAsync(hbad, bad, sizeof(int), hipMemcpyDeviceToHost, s));
-    HIPCHECK(hipStreamSynchronize(s));
+    HIPCHECK(hipMemcpyAsync(hbad, flags.bad, sizeof(int), hipMemcpyDeviceToHost, s));
+    int st = check_solve_states(s, pool, flags.states, flags.nsolves, err);  // synchronises the stream
+    if (!flags.nsolves) HIPCHECK(hipStreamSynchronize(s));
     HIPCHECK(hipGetLastError());
-    h->stats.device_ms += now_ms() - t0;
+    if (st) return st;
     if (*hbad & 2) {
         err = "weighted blend without vectors: non-finite or huge float samples (the NaN rule of blend.rs:280-300) are not "
               "served";
@@ -906,10 +972,128 @@ int retime_batch_dev(twx_handle* h, DevPool& pool, const std::vector<ClipRun>& r
     return TWX_OK;
 }
 
+static void add_stats(twx_stats& a, const twx_stats& d) {
+    a.pair_solves += d.pair_solves;
+    a.pairs_reused += d.pairs_reused;
+    a.composes += d.composes;
+    a.fast_frames += d.fast_frames;
+    a.pyramid_builds += d.pyramid_builds;
+}
+
+// The batched render, enqueued and checked in one go (no graph).
+int retime_batch_dev(twx_handle* h, DevPool& pool, const std::vector<ClipRun>& runs, std::string& err) {
+    double t0 = now_ms();
+    BatchFlags flags;
+    twx_stats delta{};
+    int st = retime_batch_enqueue(h, pool, runs, flags, delta, err);
+    if (!st) st = retime_batch_check(h, pool, flags, err);
+    add_stats(h->stats, delta);
+    h->stats.device_ms += now_ms() - t0;
+    return st;
+}
+
 // The batched render of output frames [first, first + count) of one clip on the device.
 int retime_frames_dev(twx_handle* h, DevPool& pool, const twx_retime_params& p, const uint8_t* clip, int n_in, int first,
                       int count, uint8_t* out, std::string& err) {
     return retime_batch_dev(h, pool, {ClipRun{p, clip, n_in, first, count, out}}, err);
+}
+
+// The arguments that decide a batched render's launches: the parameter blocks (keyframes by value),
+// the device pointers and the frame ranges.
+static std::string batch_key(const std::vector<ClipRun>& runs) {
+    std::string k;
+    auto put = [&](const void* p, size_t n) { k.append((const char*)p, n); };
+    for (const ClipRun& r : runs) {
+        twx_retime_params p = r.p;
+        p.speed_keys = nullptr;
+        put(&p, sizeof(p));
+        if (r.p.n_speed_keys > 0 && r.p.speed_keys) put(r.p.speed_keys, sizeof(twx_key) * (size_t)r.p.n_speed_keys);
+        put(&r.clip, sizeof(r.clip));
+        put(&r.n_in, sizeof(r.n_in));
+        put(&r.first, sizeof(r.first));
+        put(&r.count, sizeof(r.count));
+        put(&r.out, sizeof(r.out));
+    }
+    return k;
+}
+
+// A batched render through the graph cache: a call that repeats the arguments of the last captured
+// render replays its graph; otherwise the render runs on a new plan's own memory (this call's
+// output), then the same launch sequence is captured over the same memory (the same allocation
+// sequence gives the same addresses; the table uploads stay in the plan's pinned staging) and kept
+// for the next call. With profiling on or graphs off, the plain path runs on the shared arenas.
+int retime_batch_run(twx_handle* h, const std::vector<ClipRun>& runs, std::string& err) {
+    if (!h->graphs || h->prof.on) {
+        DevPool pool(h);
+        return retime_batch_dev(h, pool, runs, err);
+    }
+    hipStream_t s = h->stream;
+    const double t0 = now_ms();
+    std::string key = batch_key(runs);
+    if (h->graph && h->graph->exec && h->graph->key == key) {
+        GraphPlan& g = *h->graph;
+        HIPCHECK(hipGraphLaunch(g.exec, s));
+        DevPool readback(h);
+        BatchFlags f;
+        f.bad = g.bad;
+        f.states = g.states;
+        f.nsolves = g.nsolves;
+        int st = retime_batch_check(h, readback, f, err);
+        add_stats(h->stats, g.delta);
+        h->stats.device_ms += now_ms() - t0;
+        h->graph_replays++;
+        return st;
+    }
+    h->graph.reset();
+    auto g = std::make_unique<GraphPlan>();
+    g->key = key;
+    {
+        DevPool pool(g->dev, g->host, s);
+        BatchFlags f;
+        twx_stats delta{};
+        int st = retime_batch_enqueue(h, pool, runs, f, delta, err);
+        if (!st) {
+            DevPool readback(h);
+            st = retime_batch_check(h, readback, f, err);
+        }
+        add_stats(h->stats, delta);
+        h->stats.device_ms += now_ms() - t0;
+        if (st) return st;
+        g->delta = delta;
+    }
+    DevPool pool(g->dev, g->host, s);
+    BatchFlags f;
+    twx_stats delta{};
+    hipGraph_t graph = nullptr;
+    HIPCHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    std::string e2;
+    int st = TWX_OK;
+    try {
+        st = retime_batch_enqueue(h, pool, runs, f, delta, e2);
+    } catch (const HipError&) {
+        (void)hipStreamEndCapture(s, &graph);
+        if (graph) (void)hipGraphDestroy(graph);
+        return TWX_OK;  // this call's output is complete; no graph is kept
+    }
+    const hipError_t ec = hipStreamEndCapture(s, &graph);
+    if (st || ec != hipSuccess || !graph) {
+        if (graph) (void)hipGraphDestroy(graph);
+        (void)hipGetLastError();
+        return TWX_OK;
+    }
+    hipGraphExec_t exec = nullptr;
+    const hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (ei != hipSuccess) {
+        (void)hipGetLastError();
+        return TWX_OK;
+    }
+    g->exec = exec;
+    g->bad = f.bad;
+    g->states = f.states;
+    g->nsolves = f.nsolves;
+    h->graph = std::move(g);
+    return TWX_OK;
 }
 
 // The Backend::solve / solve_pair path on host engine images (rows as given).
@@ -1606,10 +1790,7 @@ int twx_retime_frames_dev(twx_handle* h, const twx_retime_params* p, const void*
     int st = TWX_OK;
     try {
         Guard g(h, st, e);
-        if (g.ok()) {
-            DevPool pool(h);
-            st = retime_frames_dev(h, pool, *p, (const uint8_t*)clip, n_in, first, count, (uint8_t*)out, e);
-        }
+        if (g.ok()) st = retime_batch_run(h, {ClipRun{*p, (const uint8_t*)clip, n_in, first, count, (uint8_t*)out}}, e);
     } catch (const HipError& x) {
         e = x.msg;
         st = TWX_E_DEVICE;
@@ -1633,8 +1814,7 @@ int twx_retime_batch(twx_handle* h, int32_t n_clips, const twx_retime_params* pa
                 std::vector<ClipRun> runs;
                 for (int i = 0; i < n_clips; i++)
                     runs.push_back(ClipRun{params[i], (const uint8_t*)clips[i], n_in[i], first[i], count[i], (uint8_t*)outs[i]});
-                DevPool pool(h);
-                st = retime_batch_dev(h, pool, runs, e);
+                st = retime_batch_run(h, runs, e);
             }
         }
     } catch (const HipError& x) {
@@ -1724,9 +1904,25 @@ int twx_warp_blend_backward(twx_handle* h, int32_t channels, int32_t width, int3
                             grad_a, grad_b, grad_f_ab, grad_f_ba, err, errlen);
 }
 
+int twx_set_graphs(twx_handle* h, int32_t on) {
+    std::string e;
+    int st = TWX_OK;
+    try {
+        Guard g(h, st, e);
+        if (g.ok()) {
+            h->graphs = on != 0;
+            if (!h->graphs) h->graph.reset();
+        }
+    } catch (const HipError&) {
+        st = TWX_E_DEVICE;
+    }
+    return st;
+}
+
 int twx_get_stats(twx_handle* h, twx_stats* out) {
     if (!h || !out) return TWX_E_MISMATCH;
     *out = h->stats;
+    out->graph_replays = h->graph_replays;
     return TWX_OK;
 }
 

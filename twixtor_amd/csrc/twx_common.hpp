@@ -190,6 +190,8 @@ void launch_fill(hipStream_t s, const FillPlane* d_planes, const std::vector<Fil
 // The Smart Blend frames (kinds 4-6) with finish set: the filled image into the wire frame.
 void launch_finish(hipStream_t s, const FrameJob* d_jobs, const std::vector<FrameJob>& jobs);
 void launch_nan_scan(hipStream_t s, const float* data, size_t n, int* flag);
+// Clears n ints with a kernel (a memset node would not replay correctly inside a captured graph).
+void launch_zero(hipStream_t s, int* p, size_t n);
 int run_selfcheck(hipStream_t s, std::string& err);
 // The differentiable warp+blend (twx_diff.hip): the forward into `out`, or the backward into the
 // gradients that are not null; gp_scratch holds 4 floats per pixel, `bad` is set for a NaN point.

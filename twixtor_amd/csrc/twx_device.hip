@@ -2369,6 +2369,15 @@ void launch_invert_post(hipStream_t s, const float* pushed, const uint8_t* cov, 
     TWX_LAUNCH(s, F_VEC, 26.0 * n, k_invert_post<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(pushed, cov, flow, holes, gate, n));
 }
 
+__global__ void k_zero_ints(int* __restrict__ p, size_t n) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
+void launch_zero(hipStream_t s, int* p, size_t n) {
+    if (n) k_zero_ints<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(p, n);
+}
+
 // -------------------------------------------------------------------------------------------
 // Taps (conv.rs:44-86), on the host
 
