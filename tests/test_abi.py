@@ -110,3 +110,12 @@ def test_flow_geometry_through_the_abi():
     assert flow_geometry(flow_settings(quality=4), 3840, 2160) == (1920, 1080, 9)
     with pytest.raises(_ffi.StageError):
         flow_geometry(flow_settings(quality=4), 1, 17)
+
+
+def test_tile_shape_switch_checks_its_argument():
+    L = _ffi.lib()
+    for v in (-2, 4, 99):
+        assert L.twx_set_tile_shape(v) == _ffi.TWX_E_MISMATCH
+    for v in (3, 0, -1):
+        assert L.twx_set_tile_shape(v) == _ffi.TWX_OK
+    assert L.twx_tile_choices(None, 0) >= 0

@@ -20,7 +20,7 @@ EXPORTS = [
     "twx_retime_frames",
     "twx_forward_warp", "twx_encode_vectors", "twx_decode_vectors", "twx_flow_from_vectors", "twx_invert_flow",
     "twx_create_vectors_dev", "twx_warp_blend", "twx_warp_blend_backward",
-    "twx_set_graphs", "twx_get_stats", "twx_stream", "twx_profile_enable", "twx_profile_reset", "twx_profile_read",
+    "twx_set_graphs", "twx_set_tile_shape", "twx_tile_choices", "twx_get_stats", "twx_stream", "twx_profile_enable", "twx_profile_reset", "twx_profile_read",
 ]
 
 
@@ -109,6 +109,8 @@ def lib():
         L.twx_warp_blend.argtypes = [P, I32, I32, I32, P, P, P, P, I32, I32, ctypes.c_float, P] + ERR
         L.twx_warp_blend_backward.argtypes = [P, I32, I32, I32, P, P, P, P, I32, I32, ctypes.c_float, P, P, P, P, P] + ERR
         L.twx_set_graphs.argtypes = [P, I32]
+        L.twx_set_tile_shape.argtypes = [I32]
+        L.twx_tile_choices.argtypes = [PI32, I32]
         L.twx_get_stats.argtypes = [P, ctypes.POINTER(Stats)]
         L.twx_stream.argtypes = [P]
         L.twx_stream.restype = P

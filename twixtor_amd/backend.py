@@ -84,6 +84,21 @@ class HipBackend:
         """Graph capture and replay of batched renders (twx_set_graphs)."""
         _ffi.lib().twx_set_graphs(self._h, 1 if on else 0)
 
+    @staticmethod
+    def tile_shape(variant: int = -1):
+        """Force tile variant 0..3 of the fused solve kernels for the process (-1: measured choice)."""
+        st = _ffi.lib().twx_set_tile_shape(int(variant))
+        if st:
+            raise _ffi.StageError(st, f"tile variant {variant} outside -1..3")
+
+    @staticmethod
+    def tile_choices() -> list:
+        """The autotuner's choices: (kernel, log2 level area, log2 job count, variant) per class."""
+        n = _ffi.lib().twx_tile_choices(None, 0)
+        buf = (ctypes.c_int32 * max(4 * n, 4))()
+        n = _ffi.lib().twx_tile_choices(buf, 4 * n)
+        return [tuple(buf[4 * i:4 * i + 4]) for i in range(n)]
+
     # ---- per-kernel-family HIP-event timing and algorithmic bytes ----
     def profile(self, on: bool = True):
         _ffi.lib().twx_profile_enable(self._h, 1 if on else 0)

@@ -1904,6 +1904,10 @@ int twx_warp_blend_backward(twx_handle* h, int32_t channels, int32_t width, int3
                             grad_a, grad_b, grad_f_ab, grad_f_ba, err, errlen);
 }
 
+int twx_set_tile_shape(int32_t variant) { return tile_force(variant) == 0 ? TWX_OK : TWX_E_MISMATCH; }
+
+int twx_tile_choices(int32_t* out, int32_t cap) { return tile_tuning_report(out, out ? cap : 0); }
+
 int twx_set_graphs(twx_handle* h, int32_t on) {
     std::string e;
     int st = TWX_OK;

@@ -192,6 +192,11 @@ void launch_finish(hipStream_t s, const FrameJob* d_jobs, const std::vector<Fram
 void launch_nan_scan(hipStream_t s, const float* data, size_t n, int* flag);
 // Clears n ints with a kernel (a memset node would not replay correctly inside a captured graph).
 void launch_zero(hipStream_t s, int* p, size_t n);
+// Tile shapes of the fused producer + Gaussian kernels: variant v in 0..3 forced for every
+// launch (-1 returns to measured choice); the measured choices as (kernel, log2 level area,
+// log2 job count, variant) quadruples, their number returned.
+int tile_force(int v);
+int tile_tuning_report(int* out, int cap);
 int run_selfcheck(hipStream_t s, std::string& err);
 // The differentiable warp+blend (twx_diff.hip): the forward into `out`, or the backward into the
 // gradients that are not null; gp_scratch holds 4 floats per pixel, `bad` is set for a NaN point.

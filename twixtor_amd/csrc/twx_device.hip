@@ -10,6 +10,11 @@
 
 #include <hip/hip_runtime.h>
 
+#include <atomic>
+#include <map>
+#include <mutex>
+#include <tuple>
+
 #include <climits>
 #include <cmath>
 
@@ -500,8 +505,8 @@ __device__ __forceinline__ int xcd_tile(int n) {
 // computes runs of TILE_RUN outputs from a register window where every tap lies inside the image,
 // and the border formula element by element elsewhere. Every element is computed exactly as by
 // the separate passes, so the bits are the same.
-template <int C, int R, int MODE>
-__global__ __launch_bounds__(TILE_THREADS) void k_tile_gauss(const SolveJob* __restrict__ jobs, int level, int src,
+template <int C, int R, int MODE, int TY, int PUV>
+__global__ __launch_bounds__(TILE_X * TY / TILE_RUN) void k_tile_gauss(const SolveJob* __restrict__ jobs, int level, int src,
                                                              int dst, Taps t, int tiles_x, int tiles_y, int njobs) {
     const int per_job = tiles_x * tiles_y;
     const int lt = xcd_tile(per_job * njobs);
@@ -511,28 +516,29 @@ __global__ __launch_bounds__(TILE_THREADS) void k_tile_gauss(const SolveJob* __r
     const SolveJob& J = jobs[job];
     if (level > J.start) return;
     const int w = J.w[level], h = J.h[level];
-    const int x0 = txi * TILE_X, y0 = tyi * TILE_Y;
+    const int x0 = txi * TILE_X, y0 = tyi * TY;
+    constexpr int NT = TILE_X * TY / TILE_RUN;  // one vertical run of TILE_RUN rows per lane
     if (x0 >= w || y0 >= h) return;
-    constexpr int RW = TILE_X + 2 * R, RH = TILE_Y + 2 * R;
+    constexpr int RW = TILE_X + 2 * R, RH = TY + 2 * R;
     constexpr int RUNS_X = TILE_X / TILE_RUN, HITEMS = RH * RUNS_X;
-    constexpr int HPER = (HITEMS + TILE_THREADS - 1) / TILE_THREADS;
+    constexpr int HPER = (HITEMS + NT - 1) / NT;
     constexpr bool MULTI = C > 1;
     __shared__ float s_in[RH * RW * C];
     const int tid = threadIdx.x;
     // the region, PU pixels per lane at a time with their loads in flight together; a pixel
     // outside the image is computed at a clamped position and not stored
-    constexpr int NPROD = (RH * RW + TILE_THREADS - 1) / TILE_THREADS, PU = MODE == P_NEWDISP ? 1 : 4;
+    constexpr int NPROD = (RH * RW + NT - 1) / NT, PU = PUV;
 #pragma unroll 1
     for (int k0 = 0; k0 < NPROD; k0 += PU) {
         float v[PU][C];
         int slot[PU];
 #pragma unroll
         for (int u = 0; u < PU; u++) {
-            const int k = tid + (k0 + u) * TILE_THREADS;
+            const int k = tid + (k0 + u) * NT;
             const int ry = k / RW, rx = k - ry * RW;
             const int x = x0 - R + rx, y = y0 - R + ry;
             const bool keep = k0 + u < NPROD && k < RH * RW && x >= 0 && x < w && y >= 0 && y < h;
-            const bool own = keep && rx >= R && rx < R + TILE_X && ry >= R && ry < R + TILE_Y;
+            const bool own = keep && rx >= R && rx < R + TILE_X && ry >= R && ry < R + TY;
             produce<MODE, C>(J, level, src, w, h, min(max(x, 0), w - 1), min(max(y, 0), h - 1), own, v[u]);
             slot[u] = keep ? ry * RW + rx : -1;
         }
@@ -551,7 +557,7 @@ __global__ __launch_bounds__(TILE_THREADS) void k_tile_gauss(const SolveJob* __r
     float hres[HPER][TILE_RUN][C];
 #pragma unroll
     for (int q = 0; q < HPER; q++) {
-        const int it = tid + q * TILE_THREADS;
+        const int it = tid + q * NT;
         if (it >= HITEMS) continue;
         const int ry = it / RUNS_X, rx0 = (it - ry * RUNS_X) * TILE_RUN;
         const int y = y0 - R + ry, xs = x0 + rx0;
@@ -580,7 +586,7 @@ __global__ __launch_bounds__(TILE_THREADS) void k_tile_gauss(const SolveJob* __r
     __syncthreads();
 #pragma unroll
     for (int q = 0; q < HPER; q++) {
-        const int it = tid + q * TILE_THREADS;
+        const int it = tid + q * NT;
         if (it >= HITEMS) continue;
         const int ry = it / RUNS_X, rx0 = (it - ry * RUNS_X) * TILE_RUN;
         const int y = y0 - R + ry, xs = x0 + rx0;
@@ -595,7 +601,6 @@ __global__ __launch_bounds__(TILE_THREADS) void k_tile_gauss(const SolveJob* __r
     __syncthreads();
     // vertical pass: item = (column, run of TILE_RUN rows), one per lane
     float* out = MODE == P_TENSOR ? J.ta : (MODE == P_CALCB ? J.tb : J.flow[dst]);
-    static_assert(TILE_X * (TILE_Y / TILE_RUN) == TILE_THREADS, "one vertical run per lane");
     const int hx = tid % TILE_X, ry0 = (tid / TILE_X) * TILE_RUN;
     const int x = x0 + hx, ys = y0 + ry0;
     if (x >= w || ys >= h) return;
@@ -1135,6 +1140,116 @@ __global__ void k_solve_init(const SolveJob* __restrict__ jobs) {
     }
 }
 
+// The tile shapes of the fused producer + Gaussian kernels, picked by measurement: a tile of
+// 64 x TY outputs (TY 32 or 16, 256 or 128 lanes) whose region producer keeps PU pixels' loads in
+// flight per lane (4 or 1). Every variant computes every output element with the same operations,
+// so the choice changes time, not bits. The first launch of a (kernel, level size, job count)
+// class times each variant once on the live data (the kernels are idempotent: their inputs and
+// outputs are distinct buffers) and keeps the fastest for the process; launches while a graph is
+// being captured use the default shape until a plain launch has tuned their class; a shape
+// forced through tile_force applies to every launch.
+struct TileVariant {
+    int ty, pu;
+};
+static const TileVariant TILE_VARIANTS[4] = {{32, 4}, {32, 1}, {16, 4}, {16, 1}};
+
+static std::mutex& tune_mu() {
+    static std::mutex m;
+    return m;
+}
+static std::map<std::tuple<int, int, int>, int>& tune_table() {
+    static std::map<std::tuple<int, int, int>, int> t;
+    return t;
+}
+static std::atomic<int> g_tile_force{-1};
+
+int tile_force(int v) {
+    if (v < -1 || v > 3) return -1;
+    g_tile_force.store(v);
+    return 0;
+}
+
+template <int C, int R, int MODE, int TY, int PU>
+static void tile_launch_v(hipStream_t s, const SolveJob* d_jobs, int l, int src, int dst, const Taps& t, int mw, int mh, int nj) {
+    const int tx = (mw + TILE_X - 1) / TILE_X, ty = (mh + TY - 1) / TY;
+    const long ntile = (long)tx * ty * nj;
+    k_tile_gauss<C, R, MODE, TY, PU><<<dim3((unsigned)(8 * ((ntile + 7) / 8))), dim3(TILE_X * TY / TILE_RUN), 0, s>>>(
+        d_jobs, l, src, dst, t, tx, ty, nj);
+}
+
+template <int C, int R, int MODE>
+static void tile_launch_idx(int v, hipStream_t s, const SolveJob* d_jobs, int l, int src, int dst, const Taps& t, int mw,
+                            int mh, int nj) {
+    switch (v) {
+        case 1: tile_launch_v<C, R, MODE, 32, 1>(s, d_jobs, l, src, dst, t, mw, mh, nj); break;
+        case 2: tile_launch_v<C, R, MODE, 16, 4>(s, d_jobs, l, src, dst, t, mw, mh, nj); break;
+        case 3: tile_launch_v<C, R, MODE, 16, 1>(s, d_jobs, l, src, dst, t, mw, mh, nj); break;
+        default: tile_launch_v<C, R, MODE, 32, 4>(s, d_jobs, l, src, dst, t, mw, mh, nj); break;
+    }
+}
+
+static int ceil_log2(long v) {
+    int k = 0;
+    while ((1L << k) < v) k++;
+    return k;
+}
+
+template <int C, int R, int MODE>
+static void tile_launch(hipStream_t s, const SolveJob* d_jobs, int l, int src, int dst, const Taps& t, int mw, int mh,
+                        int nj) {
+    const int forced = g_tile_force.load();
+    if (forced >= 0) {
+        tile_launch_idx<C, R, MODE>(forced, s, d_jobs, l, src, dst, t, mw, mh, nj);
+        return;
+    }
+    const auto key = std::make_tuple(MODE, ceil_log2((long)mw * mh), ceil_log2(nj));
+    int choice = -1;
+    {
+        std::lock_guard<std::mutex> g(tune_mu());
+        auto it = tune_table().find(key);
+        if (it != tune_table().end()) choice = it->second;
+    }
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    (void)hipStreamIsCapturing(s, &cs);
+    if (choice < 0 && cs == hipStreamCaptureStatusNone) {
+        hipEvent_t ev[5];
+        for (auto& e : ev) (void)hipEventCreate(&e);
+        (void)hipEventRecord(ev[0], s);
+        for (int v = 0; v < 4; v++) {
+            tile_launch_idx<C, R, MODE>(v, s, d_jobs, l, src, dst, t, mw, mh, nj);
+            (void)hipEventRecord(ev[v + 1], s);
+        }
+        (void)hipEventSynchronize(ev[4]);
+        float best = 0.0f;
+        for (int v = 0; v < 4; v++) {
+            float ms = 0.0f;
+            (void)hipEventElapsedTime(&ms, ev[v], ev[v + 1]);
+            if (choice < 0 || ms < best) { best = ms; choice = v; }
+        }
+        for (auto& e : ev) (void)hipEventDestroy(e);
+        std::lock_guard<std::mutex> g(tune_mu());
+        tune_table()[key] = choice;
+    }
+    // with MODE new_disp the default keeps one pixel's loads in flight (its producer is heavier)
+    if (choice < 0) choice = MODE == P_NEWDISP ? 1 : 0;
+    tile_launch_idx<C, R, MODE>(choice, s, d_jobs, l, src, dst, t, mw, mh, nj);
+}
+
+int tile_tuning_report(int* out, int cap) {
+    std::lock_guard<std::mutex> g(tune_mu());
+    int n = 0;
+    for (auto& kv : tune_table()) {
+        if (n * 4 + 3 < cap) {
+            out[n * 4] = std::get<0>(kv.first);
+            out[n * 4 + 1] = std::get<1>(kv.first);
+            out[n * 4 + 2] = std::get<2>(kv.first);
+            out[n * 4 + 3] = kv.second;
+        }
+        n++;
+    }
+    return n;
+}
+
 static void launch_solve_range(hipStream_t s, const SolveJob* d_jobs, const std::vector<SolveJob>& jobs, bool skip_refine) {
     int nj = (int)jobs.size();
     if (nj == 0) return;
@@ -1170,17 +1285,14 @@ static void launch_solve_range(hipStream_t s, const SolveJob* d_jobs, const std:
                 if (l > 0) Pn += (double)j.w[l - 1] * j.h[l - 1];
             }
         dim3 blk(256), grd((mw + 255) / 256, mh, nj);
-        const int tx = (mw + TILE_X - 1) / TILE_X, ty = (mh + TILE_Y - 1) / TILE_Y;
-        const long ntile = (long)tx * ty * nj;
-        dim3 tgrid((unsigned)(8 * ((ntile + 7) / 8))), tblk(TILE_THREADS);
         // grad + tensor, blurred r = 7: X 4 B in, grad 8 + tensor 12 B out
-        TWX_LAUNCH(s, F_GRAD_TENSOR, 24.0 * P, (k_tile_gauss<3, 7, P_TENSOR><<<tgrid, tblk, 0, s>>>(d_jobs, l, 0, 0, g7, tx, ty, nj)));
+        TWX_LAUNCH(s, F_GRAD_TENSOR, 24.0 * P, tile_launch<3, 7, P_TENSOR>(s, d_jobs, l, 0, 0, g7, mw, mh, nj));
         for (int it = 0; it < 3; it++) {
             const int src = it & 1, dst = src ^ 1;
             // calc_b + B blur: flow 8 + X 4 + Y 4 + grad 8 in, B 8 out
-            TWX_LAUNCH(s, F_CALC_B, 32.0 * P, (k_tile_gauss<2, 7, P_CALCB><<<tgrid, tblk, 0, s>>>(d_jobs, l, src, 0, g7, tx, ty, nj)));
+            TWX_LAUNCH(s, F_CALC_B, 32.0 * P, tile_launch<2, 7, P_CALCB>(s, d_jobs, l, src, 0, g7, mw, mh, nj));
             // new_disp + flow blur: tensor 12 + B 8 + flow 8 in, flow 8 out
-            TWX_LAUNCH(s, F_NEW_DISP, 36.0 * P, (k_tile_gauss<2, 5, P_NEWDISP><<<tgrid, tblk, 0, s>>>(d_jobs, l, src, dst, g5, tx, ty, nj)));
+            TWX_LAUNCH(s, F_NEW_DISP, 36.0 * P, tile_launch<2, 5, P_NEWDISP>(s, d_jobs, l, src, dst, g5, mw, mh, nj));
             if (it == 1 && l == 0 && !skip_refine) bm(l, mw, mh, P, 0);
         }
         if (l != 0 && !skip_refine) bm(l, mw, mh, P, 1);
