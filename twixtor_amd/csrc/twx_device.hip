@@ -30,6 +30,13 @@ __device__ __forceinline__ int32_t cvtt(float x) {
     if (isnan(x) || x >= 2147483648.0f || x < -2147483648.0f) return INT_MIN;
     return (int32_t)x;
 }
+// the hardware's float -> int32 truncation (v_cvt_i32_f32: NaN gives 0, out of range saturates),
+// for callers that only use the result where cvtt gives the same value
+__device__ __forceinline__ int32_t trunc_hw(float x) {
+    int32_t r;
+    asm("v_cvt_i32_f32 %0, %1" : "=v"(r) : "v"(x));
+    return r;
+}
 // blend.rs:18-25 (64 bit truncation of a float; the callers keep the low byte or word)
 __device__ __forceinline__ int64_t cvtt64f(float x) {
     const float two63 = 9223372036854775808.0f;
@@ -832,102 +839,62 @@ __global__ void k_negate(const SolveJob* __restrict__ jobs) {
 // transposed copies the row sweeps use (so that the lanes of a wave, one row each, read
 // consecutive addresses). The arithmetic does not depend on the layout.
 
-// global-memory view of a device pointer (global loads rather than flat loads)
+// global-memory views of device pointers (global loads rather than flat loads)
 typedef __attribute__((address_space(1))) const float gfloat;
 typedef __attribute__((address_space(1))) float gfloat_rw;
+typedef __attribute__((address_space(1))) const char gchar;
 
-// Block::cost (flow.rs:472-524); false for None. 32-bit element indices (planes < 2^31).
-template <bool T>
-__device__ __forceinline__ bool bm_cost(const float* __restrict__ bp, int w, int h, const float p[9], float x2, float y2,
-                                        float& out) {
-    const float* b = bp;
-    const long SX = T ? h : 1, SY = T ? 1 : w;
-    float wm2 = (float)(w - 2);
-    float hm2 = (float)(h - 2);
-    if (x2 > wm2 || y2 > hm2) return false;
-    int32_t ix, iy;
-    float fx, fy;
-    if (x2 == wm2) { ix = cvtt(wm2 - 1.0f); fx = 1.0f; }
-    else { ix = cvtt(x2); fx = x2 - (float)ix; }
-    if (y2 == hm2) { iy = cvtt(hm2 - 1.0f); fy = 1.0f; }
-    else { iy = cvtt(y2); fy = y2 - (float)iy; }
-    if (ix < 1 || iy < 1) return false;
-    float w11 = fy * fx;
-    float w10 = fx - w11;
-    float w01 = fy - w11;
-    float w00 = ((1.0f - fx) - fy) + w11;
-    const float* base = b + ((long)iy - 1) * SY + ((long)ix - 1) * SX;
-    float sq[9];
-#pragma unroll
-    for (int bb = 0; bb < 3; bb++) {
-#pragma unroll
-        for (int aa = 0; aa < 3; aa++) {
-            const float* t0 = base + bb * SY + aa * SX;
-            const float* t1 = t0 + SY;
-            float q = ((w00 * t0[0] + w10 * t0[SX]) + w01 * t1[0]) + w11 * t1[SX];
-            float d = p[bb * 3 + aa] - q;
-            sq[bb * 3 + aa] = d * d;
-        }
-    }
-    float s = sq[0] + sq[1];
-#pragma unroll
-    for (int k = 2; k < 8; k++) s = s + sq[k];
-    out = sq[8] + s;
-    return true;
+// element i of a plane whose base is wave-uniform: a 32-bit byte offset from that base (planes
+// hold fewer than 2^30 elements), so the address needs no 64-bit arithmetic per lane
+__device__ __forceinline__ float ldg(const gfloat* base, uint32_t i) { return *(const gfloat*)((gchar*)base + i * 4u); }
+
+// A plane is read in runs: `run` r of a plane starts at element r * RS, and its elements are
+// consecutive (RS = h with element (x, y) at x * h + y for the transposed copies: runs are columns;
+// RS = w for the plain layout: runs are rows). Element e of run r, as a byte address whose
+// consecutive elements the compiler can fetch in one wide load.
+__device__ __forceinline__ gchar* run_at(const gfloat* base, uint32_t RS, int r, int e) {
+    return (gchar*)base + ((uint32_t)r * RS + (uint32_t)e) * 4u;
 }
 
-// Block::step (flow.rs:528-544): true when the neighbour's vector replaces the pixel's
+// Block::cost (flow.rs:472-524) without branches: `ok` false where the reference returns None (the
+// value is then meaningless). The operations and their order are those of the reference; the 4 x 4
+// texels the nine bilinear samples use are read once, as four runs of four.
 template <bool T>
-__device__ __forceinline__ bool bm_step(const float* __restrict__ a, const float* __restrict__ b, int w, int h, int x, int y,
-                                        float cu, float cv, float nu, float nv) {
-    const long SX = T ? h : 1, SY = T ? 1 : w;
-    float tx = (float)x - cu;
-    float ty = (float)y - cv;
-    float w1 = (float)(w - 1), h1 = (float)(h - 1);
-    if (!(0.0f <= tx && tx <= w1 && 0.0f <= ty && ty <= h1)) return false;
-    float p[9];
-    const float* pa = a + ((long)y - 1) * SY + ((long)x - 1) * SX;
-#pragma unroll
-    for (int r = 0; r < 3; r++) {
-        p[r * 3] = pa[r * SY];
-        p[r * 3 + 1] = pa[r * SY + SX];
-        p[r * 3 + 2] = pa[r * SY + 2 * SX];
-    }
-    float c0, c1;
-    if (!bm_cost<T>(b, w, h, p, tx, ty, c0)) return false;
-    if (!bm_cost<T>(b, w, h, p, (float)x - nu, (float)y - nv, c1)) return false;
-    return c1 < c0;
-}
-
-// Block::cost without branches (the same arithmetic as bm_cost when `ok` comes back true): the
-// two lanes of a block-matching chain run it on different targets in one instruction stream.
-template <bool T>
-__device__ __forceinline__ float bm_cost_nb(const gfloat* __restrict__ b, int w, int h, const float p[9], float x2, float y2,
-                                            bool& ok) {
-    const long SX = T ? h : 1, SY = T ? 1 : w;
+__device__ __forceinline__ float bm_cost(const gfloat* __restrict__ b, int w, int h, const float p[9], float x2, float y2,
+                                         bool& ok) {
     const float wm2 = (float)(w - 2), hm2 = (float)(h - 2);
     ok = !(x2 > wm2 || y2 > hm2);
-    int32_t ix, iy;
-    float fx, fy;
-    if (x2 == wm2) { ix = cvtt(wm2 - 1.0f); fx = 1.0f; }
-    else { ix = cvtt(x2); fx = x2 - (float)ix; }
-    if (y2 == hm2) { iy = cvtt(hm2 - 1.0f); fy = 1.0f; }
-    else { iy = cvtt(y2); fy = y2 - (float)iy; }
+    // cvtt(wm2 - 1) is w - 3 (w >= 3); elsewhere the hardware truncation (NaN gives 0, out of
+    // range saturates) equals cvtt wherever the cost exists, and gives ix < 1 or x2 > wm2 wherever
+    // cvtt does, so `ok` comes out the same
+    const int32_t tx2 = trunc_hw(x2), ty2 = trunc_hw(y2);
+    const int32_t ix = x2 == wm2 ? w - 3 : tx2, iy = y2 == hm2 ? h - 3 : ty2;
+    const float fx = x2 == wm2 ? 1.0f : x2 - (float)tx2, fy = y2 == hm2 ? 1.0f : y2 - (float)ty2;
     ok = ok && !(ix < 1 || iy < 1);
-    if (!ok) { ix = 1; iy = 1; }  // an in-bounds address; the value is not used
     const float w11 = fy * fx;
     const float w10 = fx - w11;
     const float w01 = fy - w11;
     const float w00 = ((1.0f - fx) - fy) + w11;
-    const gfloat* base = b + ((long)iy - 1) * SY + ((long)ix - 1) * SX;
+    // texels (ix - 1 .. ix + 2, iy - 1 .. iy + 2), inside the plane when ok (ix <= w - 3 and
+    // iy <= h - 3 then); otherwise the first four elements, read four times, and not used
+    uint32_t RS = T ? h : w;
+    int r0 = T ? ix - 1 : iy - 1, e0 = T ? iy - 1 : ix - 1;
+    if (!ok) { RS = 0; r0 = 0; e0 = 0; }
+    float t[4][4];
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+        gchar* q = run_at(b, RS, r0 + r, e0);
+#pragma unroll
+        for (int i = 0; i < 4; i++) t[r][i] = *(const gfloat*)(q + 4 * i);
+    }
+    // texel (ix - 1 + dx, iy - 1 + dy)
+    auto tx = [&](int dx, int dy) { return T ? t[dx][dy] : t[dy][dx]; };
     float sq[9];
 #pragma unroll
     for (int bb = 0; bb < 3; bb++) {
 #pragma unroll
         for (int aa = 0; aa < 3; aa++) {
-            const gfloat* t0 = base + bb * SY + aa * SX;
-            const gfloat* t1 = t0 + SY;
-            float q = ((w00 * t0[0] + w10 * t0[SX]) + w01 * t1[0]) + w11 * t1[SX];
+            float q = ((w00 * tx(aa, bb) + w10 * tx(aa + 1, bb)) + w01 * tx(aa, bb + 1)) + w11 * tx(aa + 1, bb + 1);
             float d = p[bb * 3 + aa] - q;
             sq[bb * 3 + aa] = d * d;
         }
@@ -938,35 +905,25 @@ __device__ __forceinline__ float bm_cost_nb(const gfloat* __restrict__ b, int w,
     return sq[8] + s;
 }
 
-// One step of a chain on a lane pair (Block::step, flow.rs:528-544): lane role 0 computes the
-// cost of the pixel's own vector (after the step's range test), role 1 the cost of the incoming
-// vector; both take the decision, c1 < c0 with both costs present.
-// The A patch of pixel (x, y): the 3x3 samples around it (Block::patch, flow.rs:459-470).
+// The buffers of a block-matching sweep in its walk coordinates: element (a, c) (a along the
+// chains, c across them) sits at a * m + c. T = true: the row sweeps on the transposed copies
+// (flow in tmp, X and Y in tb); T = false: the column sweeps on the plain planes.
 template <bool T>
-__device__ __forceinline__ void bm_patch(const gfloat* __restrict__ a, int w, int h, int x, int y, float p[9]) {
-    const long SX = T ? h : 1, SY = T ? 1 : w;
-    const gfloat* pa = a + ((long)y - 1) * SY + ((long)x - 1) * SX;
-#pragma unroll
-    for (int r = 0; r < 3; r++) {
-        p[r * 3] = pa[r * SY];
-        p[r * 3 + 1] = pa[r * SY + SX];
-        p[r * 3 + 2] = pa[r * SY + 2 * SX];
+struct BmView {
+    int w, h, n_al, m;
+    gfloat_rw* fl;
+    const gfloat* A;
+    const gfloat* B;
+    __device__ BmView(const SolveJob& J, int level, int buf) {
+        w = J.w[level];
+        h = J.h[level];
+        n_al = T ? w : h;
+        m = T ? h : w;
+        fl = (gfloat_rw*)(T ? J.tmp : J.flow[buf]);
+        A = (const gfloat*)(T ? J.tb : J.X[level]);
+        B = (const gfloat*)(T ? J.tb + (long)w * h : J.Y[level]);
     }
-}
-
-template <bool T>
-__device__ __forceinline__ bool bm_step_pair(const gfloat* __restrict__ b, int w, int h, int x, int y, const float p[9],
-                                             float cu, float cv, float nu, float nv, int role) {
-    const float tx = (float)x - cu, ty = (float)y - cv;
-    const bool in_range = 0.0f <= tx && tx <= (float)(w - 1) && 0.0f <= ty && ty <= (float)(h - 1);
-    bool ok;
-    const float c = bm_cost_nb<T>(b, w, h, p, role ? (float)x - nu : tx, role ? (float)y - nv : ty, ok);
-    const float c_o = __shfl_xor(c, 1);
-    const int ok_o = __shfl_xor(ok ? 1 : 0, 1);
-    const float c0 = role ? c_o : c, c1 = role ? c : c_o;
-    const bool ok0 = role ? ok_o != 0 : ok, ok1 = role ? ok : ok_o != 0;
-    return in_range && ok0 && ok1 && c1 < c0;
-}
+};
 
 // Transposed copies for the row sweeps: the flow (buffer `buf`) into tmp, X and Y into tb's two
 // planes; OUT copies tmp back into the flow. With them the lanes of a wave (one row each) read
@@ -1027,99 +984,107 @@ __global__ void k_bm_transpose(const SolveJob* __restrict__ jobs, int level, int
     }
 }
 
-// Sweeps 1 (DIR = +1, left to right) and 2 (DIR = -1) of block matching (flow.rs:551-585): one
-// serial chain per interior row, one thread per chain, on the transposed copies (flow in tmp, X
-// and Y in tb) so that adjacent rows are adjacent lanes and every load is coalesced. (Whether a
-// pixel keeps its own vector decides the next pixel's input, and in practice more than half the
-// pixels take the incoming vector, in long runs: the chain does not split usefully.)
-template <int DIR>
-__global__ void k_bm_rows(const SolveJob* __restrict__ jobs, int level) {
+// The four sweeps of block matching (flow.rs:551-614) as one kernel: T = true runs sweeps 1
+// (DIR = +1, left to right) and 2 (DIR = -1) with one serial chain per interior row, T = false
+// sweeps 3 (top to bottom) and 4 with one chain per interior column (BmView: adjacent chains are
+// adjacent lanes, so every load is coalesced). Whether a pixel keeps its own vector decides the
+// next pixel's input, and in practice more than half the pixels take the incoming vector, in long
+// runs: a chain does not split usefully. A lane pair runs each chain: role 0 computes the cost of
+// the pixel's own vector, role 1 the incoming vector's, in one instruction stream, and both take
+// the decision (Block::step, flow.rs:528-544). The A patch slides along the chain, one new run
+// per step; the loop runs in unrolled chunks of CH steps so that the values loaded one step ahead
+// need fewer register copies.
+template <bool T, int DIR, int CH>
+__global__ __launch_bounds__(64) void k_bm_sweep(const SolveJob* __restrict__ jobs, int level, int buf) {
     const SolveJob& J = jobs[blockIdx.z];
     if (level > J.start) return;
-    const int w = J.w[level], h = J.h[level];
-    if (w < 3 || h < 3) return;
-    // a lane pair per chain (bm_step_pair); a pair past the last row still joins the shuffles
-    const int role = threadIdx.x & 1;
-    const int y = 1 + blockIdx.x * (blockDim.x / 2) + (threadIdx.x >> 1);
-    if (1 + blockIdx.x * (blockDim.x / 2) > h - 2) return;  // the whole block is past the rows
-    const bool live = y <= h - 2;
-    const int yy = live ? y : h - 2;  // rows past the end mirror the last row and never write
-    gfloat_rw* ft = (gfloat_rw*)J.tmp;
-    const gfloat* A = (const gfloat*)J.tb;
-    const gfloat* B = (const gfloat*)(J.tb + (long)w * h);
-    int x = DIR > 0 ? 1 : w - 2;
-    long ni = ((long)(x - DIR) * h + yy) * 2;
-    float nu = ft[ni], nv = ft[ni + 1];
-    // the pixel's own vector and A patch do not depend on the chain: loaded one step ahead (the
-    // step's store goes to its own pixel, never to the next one)
-    float cu = ft[((long)x * h + yy) * 2], cv = ft[((long)x * h + yy) * 2 + 1];
-    float p[9];
-    bm_patch<true>(A, w, h, x, yy, p);
-    for (int k = 0; k < w - 2; k++, x += DIR) {
-        const long ci = ((long)x * h + yy) * 2;
-        const bool more = k + 1 < w - 2;
-        const int xn = more ? x + DIR : x;
-        const float cu_n = ft[((long)xn * h + yy) * 2], cv_n = ft[((long)xn * h + yy) * 2 + 1];
-        float pn[9];
-        bm_patch<true>(A, w, h, xn, yy, pn);
-        if (bm_step_pair<true>(B, w, h, x, yy, p, cu, cv, nu, nv, role)) {
-            if (live && role == 0) {
-                ft[ci] = nu;
-                ft[ci + 1] = nv;
-            }
-        } else {
-            nu = cu;
-            nv = cv;
-        }
-        cu = cu_n;
-        cv = cv_n;
+    if (J.w[level] < 3 || J.h[level] < 3) return;
+    const BmView<T> V(J, level, buf);
+    const int n_al = V.n_al, m = V.m;
+    const int c0 = 1 + blockIdx.x * 32;
+    if (c0 > m - 2) return;  // the whole workgroup is past the chains (uniform)
+    const int n = n_al - 2;
+    const int nch = (n + CH - 1) / CH;
+    const int a0 = DIR > 0 ? 1 : n_al - 2;
+    const int lane = threadIdx.x;
+    const int role = lane & 1;
+    const int cc = c0 + (lane >> 1);
+    const bool live = cc <= m - 2;
+    const int ce = live ? cc : m - 2;
+    const uint32_t RS = m;
+    int a = a0;
+    uint32_t e = (uint32_t)a * m + ce;
+    const uint32_t ni = (uint32_t)(a - DIR) * m + ce;
+    float nu = ldg(V.fl, 2 * ni), nv = ldg(V.fl, 2 * ni + 1);
+    float cu = ldg(V.fl, 2 * e), cv = ldg(V.fl, 2 * e + 1);
+    // the A patch (Block::patch, flow.rs:459-470) as three runs: win[r][i] = A at along a - 1 + r,
+    // across ce - 1 + i
+    float win[3][3];
 #pragma unroll
-        for (int i = 0; i < 9; i++) p[i] = pn[i];
+    for (int r = 0; r < 3; r++) {
+        gchar* q = run_at(V.A, RS, a - 1 + r, ce - 1);
+#pragma unroll
+        for (int i = 0; i < 3; i++) win[r][i] = *(const gfloat*)(q + 4 * i);
     }
-}
-
-// Sweeps 3 (DIR = +1, top to bottom) and 4 (DIR = -1) (flow.rs:586-614): one chain per interior
-// column, adjacent columns on adjacent lanes.
-template <int DIR>
-__global__ void k_bm_cols(const SolveJob* __restrict__ jobs, int level, int buf) {
-    const SolveJob& J = jobs[blockIdx.z];
-    if (level > J.start) return;
-    const int w = J.w[level], h = J.h[level];
-    if (w < 3 || h < 3) return;
-    const int role = threadIdx.x & 1;
-    const int x = 1 + blockIdx.x * (blockDim.x / 2) + (threadIdx.x >> 1);
-    if (1 + blockIdx.x * (blockDim.x / 2) > w - 2) return;
-    const bool live = x <= w - 2;
-    const int xx = live ? x : w - 2;
-    gfloat_rw* flow = (gfloat_rw*)J.flow[buf];
-    const gfloat* A = (const gfloat*)J.X[level];
-    const gfloat* B = (const gfloat*)J.Y[level];
-    int y = DIR > 0 ? 1 : h - 2;
-    long ni = ((long)(y - DIR) * w + xx) * 2;
-    float nu = flow[ni], nv = flow[ni + 1];
-    float cu = flow[((long)y * w + xx) * 2], cv = flow[((long)y * w + xx) * 2 + 1];
-    float p[9];
-    bm_patch<false>(A, w, h, xx, y, p);
-    for (int k = 0; k < h - 2; k++, y += DIR) {
-        const long ci = ((long)y * w + xx) * 2;
-        const bool more = k + 1 < h - 2;
-        const int yn = more ? y + DIR : y;
-        const float cu_n = flow[((long)yn * w + xx) * 2], cv_n = flow[((long)yn * w + xx) * 2 + 1];
-        float pn[9];
-        bm_patch<false>(A, w, h, xx, yn, pn);
-        if (bm_step_pair<false>(B, w, h, xx, y, p, cu, cv, nu, nv, role)) {
-            if (live && role == 0) {
-                flow[ci] = nu;
-                flow[ci + 1] = nv;
-            }
-        } else {
-            nu = cu;
-            nv = cv;
-        }
-        cu = cu_n;
-        cv = cv_n;
+    asm volatile("" ::"v"(nu), "v"(nv), "v"(cu), "v"(cv), "v"(win[0][0]), "v"(win[0][1]), "v"(win[0][2]),
+                 "v"(win[1][0]), "v"(win[1][1]), "v"(win[1][2]), "v"(win[2][0]), "v"(win[2][1]), "v"(win[2][2]));
+    for (int j = 0; j < nch; j++) {
 #pragma unroll
-        for (int i = 0; i < 9; i++) p[i] = pn[i];
+        for (int st = 0; st < CH; st++) {
+            const int k = j * CH + st;
+            if (k >= n) break;
+            const int an = k + 1 < n ? a + DIR : a;
+            const uint32_t en = (uint32_t)an * m + ce;
+            const float cu_n = ldg(V.fl, 2 * en), cv_n = ldg(V.fl, 2 * en + 1);
+            // the run entering the patch at the next step (a + 2 DIR), clamped at the chain's end
+            const int ar = min(max(an + DIR, 0), n_al - 1);
+            float nr[3];
+            {
+                gchar* q = run_at(V.A, RS, ar, ce - 1);
+#pragma unroll
+                for (int i = 0; i < 3; i++) nr[i] = *(const gfloat*)(q + 4 * i);
+            }
+            float p[9];
+#pragma unroll
+            for (int bb = 0; bb < 3; bb++)
+#pragma unroll
+                for (int aa = 0; aa < 3; aa++) p[bb * 3 + aa] = T ? win[aa][bb] : win[bb][aa];
+            const int x = T ? a : ce, y = T ? ce : a;
+            const float tx = (float)x - cu, ty = (float)y - cv;
+            const bool in_range = 0.0f <= tx && tx <= (float)(V.w - 1) && 0.0f <= ty && ty <= (float)(V.h - 1);
+            bool ok;
+            const float c = bm_cost<T>(V.B, V.w, V.h, p, role ? (float)x - nu : tx, role ? (float)y - nv : ty, ok);
+            // the partner lane's values by a quad permutation [1, 0, 3, 2] (every lane active)
+            const float c_o = __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(c), 0xB1, 0xF, 0xF, false));
+            const int ok_o = __builtin_amdgcn_mov_dpp(ok ? 1 : 0, 0xB1, 0xF, 0xF, false);
+            const float cost0 = role ? c_o : c, cost1 = role ? c : c_o;
+            const bool ok0 = role ? ok_o != 0 : ok, ok1 = role ? ok : ok_o != 0;
+            if (in_range && ok0 && ok1 && cost1 < cost0) {
+                if (live && role == 0) {
+                    *(gfloat_rw*)((gchar*)V.fl + 8u * e) = nu;
+                    *(gfloat_rw*)((gchar*)V.fl + 8u * e + 4u) = nv;
+                }
+            } else {
+                nu = cu;
+                nv = cv;
+            }
+            e = en;
+            a += DIR;
+            cu = cu_n;
+            cv = cv_n;
+#pragma unroll
+            for (int i = 0; i < 3; i++) {
+                if (DIR > 0) {
+                    win[0][i] = win[1][i];
+                    win[1][i] = win[2][i];
+                    win[2][i] = nr[i];
+                } else {
+                    win[2][i] = win[1][i];
+                    win[1][i] = win[0][i];
+                    win[0][i] = nr[i];
+                }
+            }
+        }
     }
 }
 
@@ -1148,11 +1113,6 @@ __global__ void k_solve_init(const SolveJob* __restrict__ jobs) {
 // outputs are distinct buffers) and keeps the fastest for the process; launches while a graph is
 // being captured use the default shape until a plain launch has tuned their class; a shape
 // forced through tile_force applies to every launch.
-struct TileVariant {
-    int ty, pu;
-};
-static const TileVariant TILE_VARIANTS[4] = {{32, 4}, {32, 1}, {16, 4}, {16, 1}};
-
 static std::mutex& tune_mu() {
     static std::mutex m;
     return m;
@@ -1250,6 +1210,9 @@ int tile_tuning_report(int* out, int cap) {
     return n;
 }
 
+// steps per unrolled chunk of a block-matching chain (a multiple of the patch's three runs)
+constexpr int BM_CHUNK = 6;
+
 static void launch_solve_range(hipStream_t s, const SolveJob* d_jobs, const std::vector<SolveJob>& jobs, bool skip_refine) {
     int nj = (int)jobs.size();
     if (nj == 0) return;
@@ -1268,11 +1231,11 @@ static void launch_solve_range(hipStream_t s, const SolveJob* d_jobs, const std:
         dim3 cb(64), rg((mh + 31) / 32, 1, nj), cg((mw + 31) / 32, 1, nj);
         // transposes 24 + 16 B/px, sweeps: A 4 + B 4 + flow read and write 16 B/px each
         TWX_LAUNCH(s, F_BM_TRANSPOSE, 24.0 * P, k_bm_transpose<false><<<tg, tb, 0, s>>>(d_jobs, l, buf));
-        TWX_LAUNCH(s, F_BM_ROWS, 24.0 * P, k_bm_rows<1><<<rg, cb, 0, s>>>(d_jobs, l));
-        TWX_LAUNCH(s, F_BM_ROWS, 24.0 * P, k_bm_rows<-1><<<rg, cb, 0, s>>>(d_jobs, l));
+        TWX_LAUNCH(s, F_BM_ROWS, 24.0 * P, (k_bm_sweep<true, 1, BM_CHUNK><<<rg, cb, 0, s>>>(d_jobs, l, buf)));
+        TWX_LAUNCH(s, F_BM_ROWS, 24.0 * P, (k_bm_sweep<true, -1, BM_CHUNK><<<rg, cb, 0, s>>>(d_jobs, l, buf)));
         TWX_LAUNCH(s, F_BM_TRANSPOSE, 16.0 * P, k_bm_transpose<true><<<tg, tb, 0, s>>>(d_jobs, l, buf));
-        TWX_LAUNCH(s, F_BM_COLS, 24.0 * P, k_bm_cols<1><<<cg, cb, 0, s>>>(d_jobs, l, buf));
-        TWX_LAUNCH(s, F_BM_COLS, 24.0 * P, k_bm_cols<-1><<<cg, cb, 0, s>>>(d_jobs, l, buf));
+        TWX_LAUNCH(s, F_BM_COLS, 24.0 * P, (k_bm_sweep<false, 1, BM_CHUNK><<<cg, cb, 0, s>>>(d_jobs, l, buf)));
+        TWX_LAUNCH(s, F_BM_COLS, 24.0 * P, (k_bm_sweep<false, -1, BM_CHUNK><<<cg, cb, 0, s>>>(d_jobs, l, buf)));
     };
     for (int l = max_start; l >= 0; l--) {
         int mw = 0, mh = 0;
