@@ -157,7 +157,7 @@ def main():
                             "GBs": round(v["bytes"] / (v["ms"] / 1000.0) / 1e9, 1) if v["ms"] > 0 else 0.0}
                         for n, v in sorted(prof.items(), key=lambda kv: -kv[1]["ms"])}}
     # the tile shapes the autotuner picked for the fused solve kernels, per level area class
-    shapes = {0: "64x32 pu4", 1: "64x32 pu1", 2: "64x16 pu4", 3: "64x16 pu1"}
+    shapes = {0: "64x32 pu4", 1: "64x32 pu1", 2: "64x16 pu4", 3: "64x16 pu1", 4: "64x64 pu4", 5: "64x64 pu1"}
     path["tile_choices"] = [{"kernel": ("tensor", "calc_b", "new_disp")[k], "log2_area": a, "log2_jobs": j,
                              "shape": shapes[v]} for k, a, j, v in HipBackend.tile_choices()]
 

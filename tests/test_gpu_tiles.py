@@ -1,5 +1,5 @@
 """Tile shapes of the fused solve kernels (twx_set_tile_shape, twx_tile_choices): every one of the
-four shapes gives the oracle's flows bit for bit, at sizes whose last tile row and column are
+six shapes gives the oracle's flows bit for bit, at sizes whose last tile row and column are
 partial and on a mixed-size batch (several jobs per launch); the measured choice (the default)
 records a choice per kernel and size class and still gives the oracle's bits."""
 import numpy as np
@@ -28,7 +28,7 @@ def solve_equal(backend, w, h, fmt, quality, tag):
     np.testing.assert_array_equal(got_ba.view(np.uint32), want_ba.view(np.uint32), err_msg=f"F_BA {tag}")
 
 
-@pytest.mark.parametrize("variant", [0, 1, 2, 3])
+@pytest.mark.parametrize("variant", [0, 1, 2, 3, 4, 5])
 def test_every_tile_shape_gives_the_oracle_flows(backend, variant):
     HipBackend.tile_shape(variant)
     try:
@@ -39,9 +39,10 @@ def test_every_tile_shape_gives_the_oracle_flows(backend, variant):
         HipBackend.tile_shape(-1)
 
 
-@pytest.mark.parametrize("variant", [2, 3])
+@pytest.mark.parametrize("variant", [2, 3, 4, 5])
 def test_tile_shapes_on_a_mixed_size_batch(backend, variant):
-    """Jobs of several sizes in one launch under the 64 x 16 shapes give the default shape's bytes."""
+    """Jobs of several sizes in one launch under the 64 x 16 and 64 x 64 shapes give the default
+    shape's bytes."""
     dev = torch.device("cuda:0")
     setups = [((40, 30, 8), ["speed_percent=50"]), ((96, 54, 16), ["speed_percent=40"]),
               ((33, 47, 32), ["speed_percent=62.5", "smart_blend=1"])]
@@ -80,7 +81,7 @@ def test_measured_choice_records_each_kernel_and_keeps_the_bits(backend):
     assert choices, "no choice recorded"
     kernels = {c[0] for c in choices}
     assert kernels == {0, 1, 2}
-    assert all(0 <= c[3] <= 3 for c in choices)
+    assert all(0 <= c[3] <= 5 for c in choices)
     # a second call of the same classes adds nothing
     n = len(choices)
     solve_equal(backend, w, h, 2, 5, "measured choice, second call")

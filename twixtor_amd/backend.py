@@ -86,10 +86,10 @@ class HipBackend:
 
     @staticmethod
     def tile_shape(variant: int = -1):
-        """Force tile variant 0..3 of the fused solve kernels for the process (-1: measured choice)."""
+        """Force tile variant 0..5 of the fused solve kernels for the process (-1: measured choice)."""
         st = _ffi.lib().twx_set_tile_shape(int(variant))
         if st:
-            raise _ffi.StageError(st, f"tile variant {variant} outside -1..3")
+            raise _ffi.StageError(st, f"tile variant {variant} outside -1..5")
 
     @staticmethod
     def tile_choices() -> list:

@@ -944,6 +944,7 @@ int retime_batch_enqueue(twx_handle* h, DevPool& pool, const std::vector<ClipRun
     std::stable_sort(jobs.begin(), jobs.end(),
                      [](const FrameJob& a, const FrameJob& b) { return (long)a.W * a.H > (long)b.W * b.H; });
     if (!jobs.empty()) {
+        for (auto& F : jobs) frame_job_ratios(F);
         FrameJob* dj = pool.upload(jobs);
         launch_frames(s, dj, jobs);
         launch_weighted(s, dj, jobs);
@@ -1749,6 +1750,7 @@ int twx_compose_vectors(twx_handle* h, int32_t interp, float wa, const void* a, 
         F.bad = bad;
         if (!direct) {
             std::vector<FrameJob> jobs{F};
+            frame_job_ratios(jobs[0]);
             FrameJob* dj = pool.upload(jobs);
             launch_frames(s, dj, jobs);
             launch_weighted(s, dj, jobs);

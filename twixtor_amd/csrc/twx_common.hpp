@@ -95,7 +95,15 @@ struct FrameJob {
     const void* pre_b;
     const uint8_t* pre_cov_a;
     const uint8_t* pre_cov_b;
+    // The warps' per-frame ratios (warp.rs:500-512), the same for every pixel: flow step per pixel
+    // (fw - 1) / (W - 1) and (fh - 1) / (H - 1), and each warp's amount over them ([0] the warp of
+    // A by frac, [1] the warp of B by wa). Filled by frame_job_ratios before upload.
+    float rx, ry;
+    float sx[2], sy[2];
 };
+
+// Fills the per-frame warp ratios of a job (IEEE division on the host, as on the device).
+void frame_job_ratios(FrameJob& F);
 
 constexpr int MAXFL = 24;  // levels of the hole-fill pyramid (holefill.rs:145-176)
 constexpr int FILL_FLOW2 = 3;  // FillPlane.fmt of a two-channel float flow (holefill.rs:506-521)

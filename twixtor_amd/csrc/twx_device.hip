@@ -1106,8 +1106,9 @@ __global__ void k_solve_init(const SolveJob* __restrict__ jobs) {
 }
 
 // The tile shapes of the fused producer + Gaussian kernels, picked by measurement: a tile of
-// 64 x TY outputs (TY 32 or 16, 256 or 128 lanes) whose region producer keeps PU pixels' loads in
-// flight per lane (4 or 1). Every variant computes every output element with the same operations,
+// 64 x TY outputs (TY 32, 16 or 64: 256, 128 or 512 lanes) whose region producer keeps PU pixels'
+// loads in flight per lane (4 or 1). Taller tiles recompute less of the region's halo and need
+// more LDS per workgroup. Every variant computes every output element with the same operations,
 // so the choice changes time, not bits. The first launch of a (kernel, level size, job count)
 // class times each variant once on the live data (the kernels are idempotent: their inputs and
 // outputs are distinct buffers) and keeps the fastest for the process; launches while a graph is
@@ -1122,9 +1123,10 @@ static std::map<std::tuple<int, int, int>, int>& tune_table() {
     return t;
 }
 static std::atomic<int> g_tile_force{-1};
+constexpr int TILE_NVAR = 6;
 
 int tile_force(int v) {
-    if (v < -1 || v > 3) return -1;
+    if (v < -1 || v >= TILE_NVAR) return -1;
     g_tile_force.store(v);
     return 0;
 }
@@ -1144,6 +1146,8 @@ static void tile_launch_idx(int v, hipStream_t s, const SolveJob* d_jobs, int l,
         case 1: tile_launch_v<C, R, MODE, 32, 1>(s, d_jobs, l, src, dst, t, mw, mh, nj); break;
         case 2: tile_launch_v<C, R, MODE, 16, 4>(s, d_jobs, l, src, dst, t, mw, mh, nj); break;
         case 3: tile_launch_v<C, R, MODE, 16, 1>(s, d_jobs, l, src, dst, t, mw, mh, nj); break;
+        case 4: tile_launch_v<C, R, MODE, 64, 4>(s, d_jobs, l, src, dst, t, mw, mh, nj); break;
+        case 5: tile_launch_v<C, R, MODE, 64, 1>(s, d_jobs, l, src, dst, t, mw, mh, nj); break;
         default: tile_launch_v<C, R, MODE, 32, 4>(s, d_jobs, l, src, dst, t, mw, mh, nj); break;
     }
 }
@@ -1172,16 +1176,16 @@ static void tile_launch(hipStream_t s, const SolveJob* d_jobs, int l, int src, i
     hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
     (void)hipStreamIsCapturing(s, &cs);
     if (choice < 0 && cs == hipStreamCaptureStatusNone) {
-        hipEvent_t ev[5];
+        hipEvent_t ev[TILE_NVAR + 1];
         for (auto& e : ev) (void)hipEventCreate(&e);
         (void)hipEventRecord(ev[0], s);
-        for (int v = 0; v < 4; v++) {
+        for (int v = 0; v < TILE_NVAR; v++) {
             tile_launch_idx<C, R, MODE>(v, s, d_jobs, l, src, dst, t, mw, mh, nj);
             (void)hipEventRecord(ev[v + 1], s);
         }
-        (void)hipEventSynchronize(ev[4]);
+        (void)hipEventSynchronize(ev[TILE_NVAR]);
         float best = 0.0f;
-        for (int v = 0; v < 4; v++) {
+        for (int v = 0; v < TILE_NVAR; v++) {
             float ms = 0.0f;
             (void)hipEventElapsedTime(&ms, ev[v], ev[v + 1]);
             if (choice < 0 || ms < best) { best = ms; choice = v; }
@@ -1366,12 +1370,10 @@ __device__ __forceinline__ float store_rule(float v) {
 
 // The sample point of output pixel (x, y) of W(src, flow, amount) (warp.rs:477-486): the flow
 // sampled bilinearly at (x * rx, y * ry) (Flow::bilerp, warp.rs:415-442), scaled by amount / r.
-__device__ __forceinline__ void warp_point(const float* __restrict__ flow, int fw, int fh, int W, int H, float amount,
-                                           int x, int y, float& ix, float& iy) {
-    int w1i = W - 1, h1i = H - 1;
-    float rx = (float)(fw - 1) / (float)w1i;
-    float ry = (float)(fh - 1) / (float)h1i;
-    float sx = amount / rx, sy = amount / ry;
+// rx = (fw - 1) / (W - 1), ry = (fh - 1) / (H - 1) and sx = amount / rx, sy = amount / ry are the
+// same for every pixel of a frame: the host computes them once (frame_job_ratios).
+__device__ __forceinline__ void warp_point(const float* __restrict__ flow, int fw, int fh, float rx, float ry, float sx,
+                                           float sy, int x, int y, float& ix, float& iy) {
     float xf = (float)x, yf = (float)y;
     float qx = xf * rx, qy = yf * ry;
     int fw1 = fw - 1, fh1 = fh - 1;
@@ -1419,9 +1421,9 @@ __device__ __forceinline__ void bilerp_src(const void* src, int W, int H, float 
 // NaN coordinate.
 template <int FMT>
 __device__ __forceinline__ bool warp_pixel(const void* src, int W, int H, const float* __restrict__ flow, int fw, int fh,
-                                           float amount, int x, int y, float o[4]) {
+                                           const FrameJob& F, int wi, int x, int y, float o[4]) {
     float ix, iy;
-    warp_point(flow, fw, fh, W, H, amount, x, y, ix, iy);
+    warp_point(flow, fw, fh, F.rx, F.ry, F.sx[wi], F.sy[wi], x, y, ix, iy);
     if (!clip_to_rect((float)x, (float)y, ix, iy, (float)(W - 1), (float)(H - 1)) || isnan(ix) || isnan(iy)) {
         o[0] = o[1] = o[2] = o[3] = 0.0f;
         return false;
@@ -1448,9 +1450,9 @@ __device__ __forceinline__ float store_faded(float v, float f) {
 // only where the clip of a border-band point fails (never for finite data).
 template <int FMT>
 __device__ __forceinline__ bool warp_fade_pixel(const void* src, int W, int H, const float* __restrict__ flow, int fw,
-                                                int fh, float amount, int x, int y, float o[4], int& cov) {
+                                                int fh, const FrameJob& F, int wi, int x, int y, float o[4], int& cov) {
     float ix, iy;
-    warp_point(flow, fw, fh, W, H, amount, x, y, ix, iy);
+    warp_point(flow, fw, fh, F.rx, F.ry, F.sx[wi], F.sy[wi], x, y, ix, iy);
     const float wf = (float)W, hf = (float)H, w1 = (float)(W - 1), h1 = (float)(H - 1);
     o[0] = o[1] = o[2] = o[3] = 0.0f;
     if (!(ix > -1.0f && ix < wf && iy > -1.0f && iy < hf)) {
@@ -1649,10 +1651,10 @@ __global__ void k_frames(const FrameJob* __restrict__ jobs) {
     } else if (F.kind == 1) {
         if (F.nearest_from_a) {
             if (F.warp_a_copy) load4<FMT>(F.a, mi, e);
-            else ok = warp_pixel<FMT>(F.a, F.W, F.H, F.f_ba, F.fw, F.fh, F.frac, x, y, e);
+            else ok = warp_pixel<FMT>(F.a, F.W, F.H, F.f_ba, F.fw, F.fh, F, 0, x, y, e);
         } else {
             if (F.warp_b_copy) load4<FMT>(F.b, mi, e);
-            else ok = warp_pixel<FMT>(F.b, F.W, F.H, F.f_ab, F.fw, F.fh, F.wa, x, y, e);
+            else ok = warp_pixel<FMT>(F.b, F.W, F.H, F.f_ab, F.fw, F.fh, F, 1, x, y, e);
         }
     } else if (F.kind >= 4) {
         // Smart Blend, first stage (smartblend.rs:323-419): the warps without hold edges and their
@@ -1672,16 +1674,16 @@ __global__ void k_frames(const FrameJob* __restrict__ jobs) {
         } else if (F.kind == 4) {
             if (F.nearest_from_a) {
                 if (F.warp_a_copy) load4<FMT>(F.a, mi, e);
-                else ok = warp_fade_pixel<FMT>(F.a, F.W, F.H, F.f_ba, F.fw, F.fh, F.frac, x, y, e, c1);
+                else ok = warp_fade_pixel<FMT>(F.a, F.W, F.H, F.f_ba, F.fw, F.fh, F, 0, x, y, e, c1);
             } else {
                 if (F.warp_b_copy) load4<FMT>(F.b, mi, e);
-                else ok = warp_fade_pixel<FMT>(F.b, F.W, F.H, F.f_ab, F.fw, F.fh, F.wa, x, y, e, c1);
+                else ok = warp_fade_pixel<FMT>(F.b, F.W, F.H, F.f_ab, F.fw, F.fh, F, 1, x, y, e, c1);
             }
         } else {
             if (F.warp_a_copy) load4<FMT>(F.a, mi, pa);
-            else ok = warp_fade_pixel<FMT>(F.a, F.W, F.H, F.f_ba, F.fw, F.fh, F.frac, x, y, pa, c1);
+            else ok = warp_fade_pixel<FMT>(F.a, F.W, F.H, F.f_ba, F.fw, F.fh, F, 0, x, y, pa, c1);
             if (F.warp_b_copy) load4<FMT>(F.b, mi, pb);
-            else ok = warp_fade_pixel<FMT>(F.b, F.W, F.H, F.f_ab, F.fw, F.fh, F.wa, x, y, pb, c2) && ok;
+            else ok = warp_fade_pixel<FMT>(F.b, F.W, F.H, F.f_ab, F.fw, F.fh, F, 1, x, y, pb, c2) && ok;
             if (F.kind == 5) smart_px<FMT>(pa, pb, F.frac, e);
         }
         if (F.kind == 6) {
@@ -1711,9 +1713,9 @@ __global__ void k_frames(const FrameJob* __restrict__ jobs) {
             load4<FMT>(F.pre_b, mi, pb);
         } else {
             if (F.warp_a_copy) load4<FMT>(F.a, mi, pa);
-            else ok = warp_pixel<FMT>(F.a, F.W, F.H, F.f_ba, F.fw, F.fh, F.frac, x, y, pa);
+            else ok = warp_pixel<FMT>(F.a, F.W, F.H, F.f_ba, F.fw, F.fh, F, 0, x, y, pa);
             if (F.warp_b_copy) load4<FMT>(F.b, mi, pb);
-            else ok = warp_pixel<FMT>(F.b, F.W, F.H, F.f_ab, F.fw, F.fh, F.wa, x, y, pb) && ok;
+            else ok = warp_pixel<FMT>(F.b, F.W, F.H, F.f_ab, F.fw, F.fh, F, 1, x, y, pb) && ok;
         }
         if (F.kind == 3) {
             // Weighted Blend, first stage: keep both warped frames and their difference masks
@@ -1743,6 +1745,15 @@ __global__ void k_frames(const FrameJob* __restrict__ jobs) {
     // the composed image is in motion rows: row y is the output's row H-1-y
     long oi = (long)(F.H - 1 - y) * F.W + x;
     finish_pixel<FMT>(F.out, n, oi, e);
+}
+
+void frame_job_ratios(FrameJob& F) {
+    F.rx = (float)(F.fw - 1) / (float)(F.W - 1);
+    F.ry = (float)(F.fh - 1) / (float)(F.H - 1);
+    F.sx[0] = F.frac / F.rx;
+    F.sy[0] = F.frac / F.ry;
+    F.sx[1] = F.wa / F.rx;
+    F.sy[1] = F.wa / F.ry;
 }
 
 static std::vector<std::pair<int, int>> frame_buckets(const std::vector<FrameJob>& jobs) {
