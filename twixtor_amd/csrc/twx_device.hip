@@ -37,6 +37,23 @@ __device__ __forceinline__ int32_t trunc_hw(float x) {
     asm("v_cvt_i32_f32 %0, %1" : "=v"(r) : "v"(x));
     return r;
 }
+// global-memory views of device pointers (global loads rather than flat loads)
+typedef __attribute__((address_space(1))) const float gfloat;
+typedef __attribute__((address_space(1))) float gfloat_rw;
+typedef __attribute__((address_space(1))) const char gchar;
+
+// element i of a plane whose base is wave-uniform: a 32-bit byte offset from that base (planes
+// hold fewer than 2^30 elements), so the address needs no 64-bit arithmetic per lane
+__device__ __forceinline__ float ldg(const gfloat* base, uint32_t i) { return *(const gfloat*)((gchar*)base + i * 4u); }
+
+// the N consecutive elements of an N-channel pixel i (one wide load where the compiler can)
+template <int N>
+__device__ __forceinline__ void ldgn(const gfloat* base, uint32_t i, float out[N]) {
+    gchar* q = (gchar*)base + i * (4u * N);
+#pragma unroll
+    for (int j = 0; j < N; j++) out[j] = *(const gfloat*)(q + 4 * j);
+}
+
 // blend.rs:18-25 (64 bit truncation of a float; the callers keep the low byte or word)
 __device__ __forceinline__ int64_t cvtt64f(float x) {
     const float two63 = 9223372036854775808.0f;
@@ -336,8 +353,11 @@ __device__ __forceinline__ float vpass_elem(const float* __restrict__ src, int w
 // -------------------------------------------------------------------------------------------
 // Samplers (flow/sample.rs:31-112)
 
+// Every caller samples inside its plane (x below 2^31 or NaN), where the hardware truncation gives
+// the io and fo of cvtt: equal values in [-2^31, 2^31), and for NaN cvtt's INT_MIN and the
+// hardware's 0 both end at io = 0 with fo NaN.
 __device__ __forceinline__ void cell(float x, int len, int& io, float& fo) {
-    int32_t i = cvtt(x);
+    int32_t i = trunc_hw(x);
     float f = x - (float)i;
     int32_t last = len - 1;
     if (i >= last) { i = last - 1; f = 1.0f; }
@@ -427,56 +447,121 @@ __device__ __forceinline__ float line_elem(const float* __restrict__ base, int s
 
 enum Produce { P_TENSOR = 0, P_CALCB = 1, P_NEWDISP = 2 };
 
-// The value of the field the Gaussian runs on, at (x, y):
+// The value of the field the Gaussian runs on, at PU pixels (px[u], py[u]) of a lane at once:
 //  P_TENSOR: calc_grad + calc_a (flow.rs:283-328); the tile's own pixels also store the gradient
 //  P_CALCB:  calc_b (flow.rs:334-357), I1 = X, I0 = Y, with the flow in buffer `src`
 //  P_NEWDISP: new_disp (flow.rs:361-371) applied to the flow in buffer `src`
-template <int MODE, int C>
-__device__ __forceinline__ void produce(const SolveJob& J, int level, int src, int w, int h, int x, int y, bool own,
-                                        float v[C]) {
-    const long i = (long)y * w + x;
-    // every load below is unconditional (in-bounds addresses, results selected), so that the
-    // producer loop can keep several pixels' loads in flight
+// The pixels' loads are issued together, phase by phase (calc_b's samples of Y depend on its flow
+// loads), through global-address-space pointers and 32-bit offsets; every load is unconditional
+// (in-bounds addresses, results selected). The arithmetic of each pixel is the reference's.
+template <int MODE, int C, int PU>
+__device__ __forceinline__ void produce(const SolveJob& J, int level, int src, int w, int h, const int px[PU],
+                                        const int py[PU], const bool own[PU], float v[PU][C]) {
+    uint32_t e[PU];
+#pragma unroll
+    for (int u = 0; u < PU; u++) e[u] = (uint32_t)py[u] * w + px[u];
     if (MODE == P_TENSOR) {
-        const float* img = J.X[level];
-        const float c = img[i];
-        const float vl = img[i - (x >= 1 ? 1 : 0)], vu = img[i - (y >= 1 ? w : 0)];
-        const float vr = img[i + (x + 1 <= w - 1 ? 1 : 0)], vd = img[i + (y + 1 <= h - 1 ? w : 0)];
-        float nx = 0.0f, dx = 0.0f, ny = 0.0f, dy = 0.0f;
-        if (x >= 1) { nx = (c - vl) + 0.0f; dx = 1.0f; }
-        if (y >= 1) { ny = (c - vu) + 0.0f; dy = 1.0f; }
-        if (x + 1 <= w - 1) { nx = nx + (vr - c); dx = dx + 1.0f; }
-        if (y + 1 <= h - 1) { ny = ny + (vd - c); dy = dy + 1.0f; }
-        float gx = nx / dx, gy = ny / dy;
-        if (own) {
-            J.grad[i * 2] = gx;
-            J.grad[i * 2 + 1] = gy;
+        const gfloat* img = (const gfloat*)J.X[level];
+        float c[PU], vl[PU], vu[PU], vr[PU], vd[PU];
+#pragma unroll
+        for (int u = 0; u < PU; u++) {
+            const int x = px[u], y = py[u];
+            c[u] = ldg(img, e[u]);
+            vl[u] = ldg(img, e[u] - (x >= 1 ? 1 : 0));
+            vu[u] = ldg(img, e[u] - (y >= 1 ? w : 0));
+            vr[u] = ldg(img, e[u] + (x + 1 <= w - 1 ? 1 : 0));
+            vd[u] = ldg(img, e[u] + (y + 1 <= h - 1 ? w : 0));
         }
-        v[0] = gx * gx;
-        v[1] = gx * gy;
-        v[C - 1] = gy * gy;
+#pragma unroll
+        for (int u = 0; u < PU; u++) {
+            const int x = px[u], y = py[u];
+            float nx = 0.0f, dx = 0.0f, ny = 0.0f, dy = 0.0f;
+            if (x >= 1) { nx = (c[u] - vl[u]) + 0.0f; dx = 1.0f; }
+            if (y >= 1) { ny = (c[u] - vu[u]) + 0.0f; dy = 1.0f; }
+            if (x + 1 <= w - 1) { nx = nx + (vr[u] - c[u]); dx = dx + 1.0f; }
+            if (y + 1 <= h - 1) { ny = ny + (vd[u] - c[u]); dy = dy + 1.0f; }
+            const float gx = nx / dx, gy = ny / dy;
+            if (own[u]) {
+                *(gfloat_rw*)((gchar*)J.grad + 8u * e[u]) = gx;
+                *(gfloat_rw*)((gchar*)J.grad + 8u * e[u] + 4u) = gy;
+            }
+            v[u][0] = gx * gx;
+            v[u][1] = gx * gy;
+            v[u][C - 1] = gy * gy;
+        }
     } else if (MODE == P_CALCB) {
-        const float* flow = J.flow[src];
-        const float sx = (float)x - flow[i * 2];
-        const float sy = (float)y - flow[i * 2 + 1];
-        const bool in = !(sx < 0.0f || (float)w - 1.0f < sx || sy < 0.0f || (float)h - 1.0f < sy);
-        const float s = bilinear1(J.Y[level], w, h, in ? sx : 0.0f, in ? sy : 0.0f);
-        const float r = s - J.X[level][i];
-        const float gx = J.grad[i * 2], gy = J.grad[i * 2 + 1];
-        v[0] = in ? r * gx : 0.0f;
-        v[C - 1] = in ? r * gy : 0.0f;
-    } else {
-        const float* flow = J.flow[src];
-        float u = flow[i * 2], vv = flow[i * 2 + 1];
-        float aa = J.ta[i * 3], ab = J.ta[i * 3 + 1], ad = J.ta[i * 3 + 2];
-        float bx = J.tb[i * 2], by = J.tb[i * 2 + 1];
-        float t = ab * ab - ad * aa;
-        if (fabsf(t) > 40.0f) {
-            u = (ab * by - ad * bx) / t + u;
-            vv = (ab * bx - aa * by) / t + vv;
+        const gfloat* fl = (const gfloat*)J.flow[src];
+        const gfloat* X = (const gfloat*)J.X[level];
+        const gfloat* Y = (const gfloat*)J.Y[level];
+        const gfloat* G = (const gfloat*)J.grad;
+        float fu[PU], fv[PU], xv[PU], gx[PU], gy[PU];
+#pragma unroll
+        for (int u = 0; u < PU; u++) {
+            float f2[2], g2[2];
+            ldgn<2>(fl, e[u], f2);
+            ldgn<2>(G, e[u], g2);
+            fu[u] = f2[0];
+            fv[u] = f2[1];
+            xv[u] = ldg(X, e[u]);
+            gx[u] = g2[0];
+            gy[u] = g2[1];
         }
-        v[0] = u;
-        v[C - 1] = vv;
+        bool in[PU];
+        float fx[PU], fy[PU], p00[PU], p10[PU], p01[PU], p11[PU];
+#pragma unroll
+        for (int u = 0; u < PU; u++) {
+            const float sx = (float)px[u] - fu[u];
+            const float sy = (float)py[u] - fv[u];
+            in[u] = !(sx < 0.0f || (float)w - 1.0f < sx || sy < 0.0f || (float)h - 1.0f < sy);
+            // bilinear1 at (sx, sy), or at (0, 0) outside
+            int ix, iy;
+            cell(in[u] ? sx : 0.0f, w, ix, fx[u]);
+            cell(in[u] ? sy : 0.0f, h, iy, fy[u]);
+            const uint32_t i = (uint32_t)iy * w + ix;
+            p00[u] = ldg(Y, i);
+            p10[u] = ldg(Y, i + 1);
+            p01[u] = ldg(Y, i + w);
+            p11[u] = ldg(Y, i + w + 1);
+        }
+#pragma unroll
+        for (int u = 0; u < PU; u++) {
+            const float top = (p10[u] - p00[u]) * fx[u] + p00[u];
+            const float bot = (p11[u] - p01[u]) * fx[u] + p01[u];
+            const float s = (bot - top) * fy[u] + top;
+            const float r = s - xv[u];
+            v[u][0] = in[u] ? r * gx[u] : 0.0f;
+            v[u][C - 1] = in[u] ? r * gy[u] : 0.0f;
+        }
+    } else {
+        const gfloat* fl = (const gfloat*)J.flow[src];
+        const gfloat* ta = (const gfloat*)J.ta;
+        const gfloat* tb = (const gfloat*)J.tb;
+        float fu[PU], fv[PU], aa[PU], ab[PU], ad[PU], bx[PU], by[PU];
+#pragma unroll
+        for (int u = 0; u < PU; u++) {
+            float f2[2], a3[3], b2[2];
+            ldgn<2>(fl, e[u], f2);
+            ldgn<3>(ta, e[u], a3);
+            ldgn<2>(tb, e[u], b2);
+            fu[u] = f2[0];
+            fv[u] = f2[1];
+            aa[u] = a3[0];
+            ab[u] = a3[1];
+            ad[u] = a3[2];
+            bx[u] = b2[0];
+            by[u] = b2[1];
+        }
+#pragma unroll
+        for (int u = 0; u < PU; u++) {
+            float uu = fu[u], vv = fv[u];
+            const float t = ab[u] * ab[u] - ad[u] * aa[u];
+            if (fabsf(t) > 40.0f) {
+                uu = (ab[u] * by[u] - ad[u] * bx[u]) / t + uu;
+                vv = (ab[u] * bx[u] - aa[u] * by[u]) / t + vv;
+            }
+            v[u][0] = uu;
+            v[u][C - 1] = vv;
+        }
     }
 }
 
@@ -535,20 +620,26 @@ __global__ __launch_bounds__(TILE_X * TY / TILE_RUN) void k_tile_gauss(const Sol
     // the region, PU pixels per lane at a time with their loads in flight together; a pixel
     // outside the image is computed at a clamped position and not stored
     constexpr int NPROD = (RH * RW + NT - 1) / NT, PU = PUV;
+    // region element k = tid + j * NT of the lane's j-th pixel, as (ry, rx), stepped without division
+    int ry = tid / RW, rx = tid - (tid / RW) * RW;
 #pragma unroll 1
     for (int k0 = 0; k0 < NPROD; k0 += PU) {
         float v[PU][C];
-        int slot[PU];
+        int slot[PU], px[PU], py[PU];
+        bool own[PU];
 #pragma unroll
         for (int u = 0; u < PU; u++) {
-            const int k = tid + (k0 + u) * NT;
-            const int ry = k / RW, rx = k - ry * RW;
             const int x = x0 - R + rx, y = y0 - R + ry;
-            const bool keep = k0 + u < NPROD && k < RH * RW && x >= 0 && x < w && y >= 0 && y < h;
-            const bool own = keep && rx >= R && rx < R + TILE_X && ry >= R && ry < R + TY;
-            produce<MODE, C>(J, level, src, w, h, min(max(x, 0), w - 1), min(max(y, 0), h - 1), own, v[u]);
+            const bool keep = ry < RH && x >= 0 && x < w && y >= 0 && y < h;
+            own[u] = keep && rx >= R && rx < R + TILE_X && ry >= R && ry < R + TY;
+            px[u] = min(max(x, 0), w - 1);
+            py[u] = min(max(y, 0), h - 1);
             slot[u] = keep ? ry * RW + rx : -1;
+            rx += NT % RW;
+            ry += NT / RW;
+            if (rx >= RW) { rx -= RW; ry++; }
         }
+        produce<MODE, C, PU>(J, level, src, w, h, px, py, own, v);
 #pragma unroll
         for (int u = 0; u < PU; u++)
             if (slot[u] >= 0) {
@@ -839,15 +930,6 @@ __global__ void k_negate(const SolveJob* __restrict__ jobs) {
 // transposed copies the row sweeps use (so that the lanes of a wave, one row each, read
 // consecutive addresses). The arithmetic does not depend on the layout.
 
-// global-memory views of device pointers (global loads rather than flat loads)
-typedef __attribute__((address_space(1))) const float gfloat;
-typedef __attribute__((address_space(1))) float gfloat_rw;
-typedef __attribute__((address_space(1))) const char gchar;
-
-// element i of a plane whose base is wave-uniform: a 32-bit byte offset from that base (planes
-// hold fewer than 2^30 elements), so the address needs no 64-bit arithmetic per lane
-__device__ __forceinline__ float ldg(const gfloat* base, uint32_t i) { return *(const gfloat*)((gchar*)base + i * 4u); }
-
 // A plane is read in runs: `run` r of a plane starts at element r * RS, and its elements are
 // consecutive (RS = h with element (x, y) at x * h + y for the transposed copies: runs are columns;
 // RS = w for the plain layout: runs are rows). Element e of run r, as a byte address whose
@@ -1110,7 +1192,7 @@ __global__ void k_solve_init(const SolveJob* __restrict__ jobs) {
 // loads in flight per lane (4 or 1). Taller tiles recompute less of the region's halo and need
 // more LDS per workgroup. Every variant computes every output element with the same operations,
 // so the choice changes time, not bits. The first launch of a (kernel, level size, job count)
-// class times each variant once on the live data (the kernels are idempotent: their inputs and
+// class times each variant twice on the live data (the kernels are idempotent: their inputs and
 // outputs are distinct buffers) and keeps the fastest for the process; launches while a graph is
 // being captured use the default shape until a plain launch has tuned their class; a shape
 // forced through tile_force applies to every launch.
@@ -1176,18 +1258,22 @@ static void tile_launch(hipStream_t s, const SolveJob* d_jobs, int l, int src, i
     hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
     (void)hipStreamIsCapturing(s, &cs);
     if (choice < 0 && cs == hipStreamCaptureStatusNone) {
-        hipEvent_t ev[TILE_NVAR + 1];
+        // two rounds over the shapes; each shape's time is the faster of its two launches
+        constexpr int NL = 2 * TILE_NVAR;
+        hipEvent_t ev[NL + 1];
         for (auto& e : ev) (void)hipEventCreate(&e);
         (void)hipEventRecord(ev[0], s);
-        for (int v = 0; v < TILE_NVAR; v++) {
-            tile_launch_idx<C, R, MODE>(v, s, d_jobs, l, src, dst, t, mw, mh, nj);
-            (void)hipEventRecord(ev[v + 1], s);
+        for (int i = 0; i < NL; i++) {
+            tile_launch_idx<C, R, MODE>(i % TILE_NVAR, s, d_jobs, l, src, dst, t, mw, mh, nj);
+            (void)hipEventRecord(ev[i + 1], s);
         }
-        (void)hipEventSynchronize(ev[TILE_NVAR]);
+        (void)hipEventSynchronize(ev[NL]);
         float best = 0.0f;
         for (int v = 0; v < TILE_NVAR; v++) {
-            float ms = 0.0f;
-            (void)hipEventElapsedTime(&ms, ev[v], ev[v + 1]);
+            float a = 0.0f, b = 0.0f;
+            (void)hipEventElapsedTime(&a, ev[v], ev[v + 1]);
+            (void)hipEventElapsedTime(&b, ev[v + TILE_NVAR], ev[v + TILE_NVAR + 1]);
+            const float ms = a < b ? a : b;
             if (choice < 0 || ms < best) { best = ms; choice = v; }
         }
         for (auto& e : ev) (void)hipEventDestroy(e);
