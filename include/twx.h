@@ -273,7 +273,7 @@ int twx_set_graphs(twx_handle* h, int32_t on);
  * counterpart in the reference, whose passes run per row on the CPU): 64 x 32, 64 x 16 or 64 x 64
  * outputs per workgroup, with 4 or 1 producer pixels in flight per lane (variants 0: 32/4, 1: 32/1,
  * 2: 16/4, 3: 16/1, 4: 64/4, 5: 64/1). By default the first plain launch of each (kernel, level
- * area class, job count class) times the six on the live data and keeps the fastest for the
+ * area class, job count class) times the six (twice each) on the live data and keeps the fastest for the
  * process; every variant gives the same bits. twx_set_tile_shape forces one variant for every
  * launch (-1: measured choice again; other values TWX_E_MISMATCH). twx_tile_choices writes up to
  * cap/4 quadruples (kernel 0 tensor, 1 calc_b, 2 new_disp; ceil log2 of level area; ceil log2 of

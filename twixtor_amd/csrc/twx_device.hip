@@ -46,12 +46,17 @@ typedef __attribute__((address_space(1))) const char gchar;
 // hold fewer than 2^30 elements), so the address needs no 64-bit arithmetic per lane
 __device__ __forceinline__ float ldg(const gfloat* base, uint32_t i) { return *(const gfloat*)((gchar*)base + i * 4u); }
 
-// the N consecutive elements of an N-channel pixel i (one wide load where the compiler can)
+// N consecutive elements starting at element i (one wide load where the compiler can)
 template <int N>
-__device__ __forceinline__ void ldgn(const gfloat* base, uint32_t i, float out[N]) {
-    gchar* q = (gchar*)base + i * (4u * N);
+__device__ __forceinline__ void ldrun(const gfloat* base, uint32_t i, float out[N]) {
+    gchar* q = (gchar*)base + i * 4u;
 #pragma unroll
     for (int j = 0; j < N; j++) out[j] = *(const gfloat*)(q + 4 * j);
+}
+// the N channels of pixel i of an N-channel plane
+template <int N>
+__device__ __forceinline__ void ldgn(const gfloat* base, uint32_t i, float out[N]) {
+    ldrun<N>(base, i * N, out);
 }
 
 // blend.rs:18-25 (64 bit truncation of a float; the callers keep the low byte or word)
@@ -353,11 +358,11 @@ __device__ __forceinline__ float vpass_elem(const float* __restrict__ src, int w
 // -------------------------------------------------------------------------------------------
 // Samplers (flow/sample.rs:31-112)
 
-// Every caller samples inside its plane (x below 2^31 or NaN), where the hardware truncation gives
-// the io and fo of cvtt: equal values in [-2^31, 2^31), and for NaN cvtt's INT_MIN and the
-// hardware's 0 both end at io = 0 with fo NaN.
+template <bool FINITE = false>
 __device__ __forceinline__ void cell(float x, int len, int& io, float& fo) {
-    int32_t i = trunc_hw(x);
+    // FINITE: the caller guarantees x in [0, len - 1] or NaN, and len >= 2; there the hardware
+    // truncation gives cvtt's io and fo (for NaN both end at io = 0, fo NaN)
+    int32_t i = FINITE ? trunc_hw(x) : cvtt(x);
     float f = x - (float)i;
     int32_t last = len - 1;
     if (i >= last) { i = last - 1; f = 1.0f; }
@@ -515,8 +520,8 @@ __device__ __forceinline__ void produce(const SolveJob& J, int level, int src, i
             in[u] = !(sx < 0.0f || (float)w - 1.0f < sx || sy < 0.0f || (float)h - 1.0f < sy);
             // bilinear1 at (sx, sy), or at (0, 0) outside
             int ix, iy;
-            cell(in[u] ? sx : 0.0f, w, ix, fx[u]);
-            cell(in[u] ? sy : 0.0f, h, iy, fy[u]);
+            cell<true>(in[u] ? sx : 0.0f, w, ix, fx[u]);
+            cell<true>(in[u] ? sy : 0.0f, h, iy, fy[u]);
             const uint32_t i = (uint32_t)iy * w + ix;
             p00[u] = ldg(Y, i);
             p10[u] = ldg(Y, i + 1);
@@ -1421,16 +1426,22 @@ template <> struct Px<0> { using T = uint8_t; };
 template <> struct Px<1> { using T = uint16_t; };
 template <> struct Px<2> { using T = float; };
 
+// Pixel i of an engine image (images hold fewer than 2^28 pixels: a 32-bit byte offset from the
+// image's base).
 template <int FMT>
-__device__ __forceinline__ void load4(const void* img, long i, float o[4]) {
+__device__ __forceinline__ void load4(const void* img, uint32_t i, float o[4]) {
+    typedef unsigned char u8x4 __attribute__((ext_vector_type(4)));
+    typedef unsigned short u16x4 __attribute__((ext_vector_type(4)));
+    typedef float f32x4 __attribute__((ext_vector_type(4)));
+    gchar* q = (gchar*)img + i * (uint32_t)(4 * (FMT == 0 ? 1 : (FMT == 1 ? 2 : 4)));
     if (FMT == 0) {
-        uchar4 p = reinterpret_cast<const uchar4*>(img)[i];
+        const u8x4 p = *(const __attribute__((address_space(1))) u8x4*)q;
         o[0] = p.x; o[1] = p.y; o[2] = p.z; o[3] = p.w;
     } else if (FMT == 1) {
-        ushort4 p = reinterpret_cast<const ushort4*>(img)[i];
+        const u16x4 p = *(const __attribute__((address_space(1))) u16x4*)q;
         o[0] = p.x; o[1] = p.y; o[2] = p.z; o[3] = p.w;
     } else {
-        float4 p = reinterpret_cast<const float4*>(img)[i];
+        const f32x4 p = *(const __attribute__((address_space(1))) f32x4*)q;
         o[0] = p.x; o[1] = p.y; o[2] = p.z; o[3] = p.w;
     }
 }
@@ -1463,15 +1474,21 @@ __device__ __forceinline__ void warp_point(const float* __restrict__ flow, int f
     float xf = (float)x, yf = (float)y;
     float qx = xf * rx, qy = yf * ry;
     int fw1 = fw - 1, fh1 = fh - 1;
-    int32_t jx = cvtt(qx), jy = cvtt(qy);
+    // qx, qy lie in [0, fw - 1] x [0, fh - 1] (the vector path needs frames of 2 x 2 or more, so
+    // rx and ry are finite): plain truncation is cvtt there
+    int32_t jx = (int32_t)qx, jy = (int32_t)qy;
     float gx = qx - (float)jx, gy = qy - (float)jy;
     if (jx >= fw1) { jx = fw1 - 1; gx = 1.0f; }
     if (jy >= fh1) { jy = fh1 - 1; gy = 1.0f; }
-    long f00 = ((long)jy * fw + jx) * 2, f01 = f00 + (long)fw * 2;
+    const uint32_t q00 = (uint32_t)jy * fw + jx;
+    const gfloat* gf = (const gfloat*)flow;
+    float t0[4], t1[4];  // the two flow pixels of the cell's top row, then of its bottom row
+    ldrun<4>(gf, 2 * q00, t0);
+    ldrun<4>(gf, 2 * (q00 + fw), t1);
     float d[2];
 #pragma unroll
     for (int k = 0; k < 2; k++) {
-        float p00 = flow[f00 + k], p10 = flow[f00 + 2 + k], p01 = flow[f01 + k], p11 = flow[f01 + 2 + k];
+        float p00 = t0[k], p10 = t0[2 + k], p01 = t1[k], p11 = t1[2 + k];
         float top = (p10 - p00) * gx + p00;
         float bot = (p11 - p01) * gx + p01;
         d[k] = (bot - top) * gy + top;
@@ -1484,11 +1501,11 @@ __device__ __forceinline__ void warp_point(const float* __restrict__ flow, int f
 template <int FMT>
 __device__ __forceinline__ void bilerp_src(const void* src, int W, int H, float ix, float iy, float o[4]) {
     int w1i = W - 1, h1i = H - 1;
-    int32_t kx = cvtt(ix), ky = cvtt(iy);
+    int32_t kx = (int32_t)ix, ky = (int32_t)iy;  // the point is inside the image: cvtt's values
     float ax = ix - (float)kx, ay = iy - (float)ky;
     if (kx >= w1i) { kx = w1i - 1; ax = 1.0f; }
     if (ky >= h1i) { ky = h1i - 1; ay = 1.0f; }
-    long i00 = (long)ky * W + kx, i01 = i00 + W;
+    uint32_t i00 = (uint32_t)ky * W + kx, i01 = i00 + W;
     float p00[4], p10[4], p01[4], p11[4];
     load4<FMT>(src, i00, p00);
     load4<FMT>(src, i00 + 1, p10);
