@@ -1071,6 +1071,77 @@ __global__ void k_bm_transpose(const SolveJob* __restrict__ jobs, int level, int
     }
 }
 
+// The state of one lane of a block-matching chain and its steps (Block::step, flow.rs:528-544),
+// run CH at a time with the loop unrolled: the patch runs sit in ring slots that rotate with the
+// step (slot of patch run r at step s: (s + r) mod 3 for DIR = +1, (r - s) mod 3 for DIR = -1;
+// CH is a multiple of 3, so every chunk starts from the same assignment), and the decision
+// updates the incoming vector by selects, so values move between registers only at chunk ends.
+template <bool T, int DIR>
+struct BmChain {
+    const BmView<T>& V;
+    int m, n_al, n, ce;
+    bool live;
+    int role;
+    int a;
+    uint32_t e;
+    float cu, cv, nu, nv;
+
+    template <int CH, bool CHECK>
+    __device__ __forceinline__ void chunk(int k0, float (&ring)[3][3]) {
+        static_assert(CH % 3 == 0, "the ring's rotation repeats every three steps");
+#pragma unroll
+        for (int st = 0; st < CH; st++) {
+            const int k = k0 + st;
+            if (CHECK && k >= n) break;
+            const int an = k + 1 < n ? a + DIR : a;
+            const uint32_t en = (uint32_t)an * m + ce;
+            const float cu_n = ldg(V.fl, 2 * en), cv_n = ldg(V.fl, 2 * en + 1);
+            // the run entering the patch at the next step (a + 2 DIR), clamped at the chain's end
+            const int ar = min(max(an + DIR, 0), n_al - 1);
+            float nr[3];
+            {
+                gchar* q = run_at(V.A, (uint32_t)m, ar, ce - 1);
+#pragma unroll
+                for (int i = 0; i < 3; i++) nr[i] = *(const gfloat*)(q + 4 * i);
+            }
+            const int s0 = DIR > 0 ? st % 3 : (3 - st % 3) % 3;        // slot of patch run 0
+            const int s1 = DIR > 0 ? (st + 1) % 3 : (4 - st % 3) % 3;  // run 1
+            const int s2 = DIR > 0 ? (st + 2) % 3 : (5 - st % 3) % 3;  // run 2
+            const int sl[3] = {s0, s1, s2};
+            float p[9];
+#pragma unroll
+            for (int bb = 0; bb < 3; bb++)
+#pragma unroll
+                for (int aa = 0; aa < 3; aa++) p[bb * 3 + aa] = T ? ring[sl[aa]][bb] : ring[sl[bb]][aa];
+            const int x = T ? a : ce, y = T ? ce : a;
+            const float tx = (float)x - cu, ty = (float)y - cv;
+            const bool in_range = 0.0f <= tx && tx <= (float)(V.w - 1) && 0.0f <= ty && ty <= (float)(V.h - 1);
+            bool ok;
+            const float c = bm_cost<T>(V.B, V.w, V.h, p, role ? (float)x - nu : tx, role ? (float)y - nv : ty, ok);
+            // the partner lane's values by a quad permutation [1, 0, 3, 2] (every lane active)
+            const float c_o = __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(c), 0xB1, 0xF, 0xF, false));
+            const int ok_o = __builtin_amdgcn_mov_dpp(ok ? 1 : 0, 0xB1, 0xF, 0xF, false);
+            const float cost0 = role ? c_o : c, cost1 = role ? c : c_o;
+            const bool ok0 = role ? ok_o != 0 : ok, ok1 = role ? ok : ok_o != 0;
+            const bool take = in_range && ok0 && ok1 && cost1 < cost0;
+            if (take && live && role == 0) {
+                *(gfloat_rw*)((gchar*)V.fl + 8u * e) = nu;
+                *(gfloat_rw*)((gchar*)V.fl + 8u * e + 4u) = nv;
+            }
+            nu = take ? nu : cu;
+            nv = take ? nv : cv;
+            // the run leaving the patch (run 0 for DIR = +1, run 2 for DIR = -1) makes room for the
+            // entering one
+#pragma unroll
+            for (int i = 0; i < 3; i++) ring[DIR > 0 ? s0 : s2][i] = nr[i];
+            e = en;
+            a += DIR;
+            cu = cu_n;
+            cv = cv_n;
+        }
+    }
+};
+
 // The four sweeps of block matching (flow.rs:551-614) as one kernel: T = true runs sweeps 1
 // (DIR = +1, left to right) and 2 (DIR = -1) with one serial chain per interior row, T = false
 // sweeps 3 (top to bottom) and 4 with one chain per interior column (BmView: adjacent chains are
@@ -1091,7 +1162,6 @@ __global__ __launch_bounds__(64) void k_bm_sweep(const SolveJob* __restrict__ jo
     const int c0 = 1 + blockIdx.x * 32;
     if (c0 > m - 2) return;  // the whole workgroup is past the chains (uniform)
     const int n = n_al - 2;
-    const int nch = (n + CH - 1) / CH;
     const int a0 = DIR > 0 ? 1 : n_al - 2;
     const int lane = threadIdx.x;
     const int role = lane & 1;
@@ -1104,75 +1174,21 @@ __global__ __launch_bounds__(64) void k_bm_sweep(const SolveJob* __restrict__ jo
     const uint32_t ni = (uint32_t)(a - DIR) * m + ce;
     float nu = ldg(V.fl, 2 * ni), nv = ldg(V.fl, 2 * ni + 1);
     float cu = ldg(V.fl, 2 * e), cv = ldg(V.fl, 2 * e + 1);
-    // the A patch (Block::patch, flow.rs:459-470) as three runs: win[r][i] = A at along a - 1 + r,
-    // across ce - 1 + i
-    float win[3][3];
+    // the A patch (Block::patch, flow.rs:459-470) as three runs of three: run r is A at along
+    // a - 1 + r, across ce - 1 .. ce + 1; the runs live in a ring whose slots rotate with the step
+    float ring[3][3];
 #pragma unroll
     for (int r = 0; r < 3; r++) {
         gchar* q = run_at(V.A, RS, a - 1 + r, ce - 1);
 #pragma unroll
-        for (int i = 0; i < 3; i++) win[r][i] = *(const gfloat*)(q + 4 * i);
+        for (int i = 0; i < 3; i++) ring[r][i] = *(const gfloat*)(q + 4 * i);
     }
-    asm volatile("" ::"v"(nu), "v"(nv), "v"(cu), "v"(cv), "v"(win[0][0]), "v"(win[0][1]), "v"(win[0][2]),
-                 "v"(win[1][0]), "v"(win[1][1]), "v"(win[1][2]), "v"(win[2][0]), "v"(win[2][1]), "v"(win[2][2]));
-    for (int j = 0; j < nch; j++) {
-#pragma unroll
-        for (int st = 0; st < CH; st++) {
-            const int k = j * CH + st;
-            if (k >= n) break;
-            const int an = k + 1 < n ? a + DIR : a;
-            const uint32_t en = (uint32_t)an * m + ce;
-            const float cu_n = ldg(V.fl, 2 * en), cv_n = ldg(V.fl, 2 * en + 1);
-            // the run entering the patch at the next step (a + 2 DIR), clamped at the chain's end
-            const int ar = min(max(an + DIR, 0), n_al - 1);
-            float nr[3];
-            {
-                gchar* q = run_at(V.A, RS, ar, ce - 1);
-#pragma unroll
-                for (int i = 0; i < 3; i++) nr[i] = *(const gfloat*)(q + 4 * i);
-            }
-            float p[9];
-#pragma unroll
-            for (int bb = 0; bb < 3; bb++)
-#pragma unroll
-                for (int aa = 0; aa < 3; aa++) p[bb * 3 + aa] = T ? win[aa][bb] : win[bb][aa];
-            const int x = T ? a : ce, y = T ? ce : a;
-            const float tx = (float)x - cu, ty = (float)y - cv;
-            const bool in_range = 0.0f <= tx && tx <= (float)(V.w - 1) && 0.0f <= ty && ty <= (float)(V.h - 1);
-            bool ok;
-            const float c = bm_cost<T>(V.B, V.w, V.h, p, role ? (float)x - nu : tx, role ? (float)y - nv : ty, ok);
-            // the partner lane's values by a quad permutation [1, 0, 3, 2] (every lane active)
-            const float c_o = __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(c), 0xB1, 0xF, 0xF, false));
-            const int ok_o = __builtin_amdgcn_mov_dpp(ok ? 1 : 0, 0xB1, 0xF, 0xF, false);
-            const float cost0 = role ? c_o : c, cost1 = role ? c : c_o;
-            const bool ok0 = role ? ok_o != 0 : ok, ok1 = role ? ok : ok_o != 0;
-            if (in_range && ok0 && ok1 && cost1 < cost0) {
-                if (live && role == 0) {
-                    *(gfloat_rw*)((gchar*)V.fl + 8u * e) = nu;
-                    *(gfloat_rw*)((gchar*)V.fl + 8u * e + 4u) = nv;
-                }
-            } else {
-                nu = cu;
-                nv = cv;
-            }
-            e = en;
-            a += DIR;
-            cu = cu_n;
-            cv = cv_n;
-#pragma unroll
-            for (int i = 0; i < 3; i++) {
-                if (DIR > 0) {
-                    win[0][i] = win[1][i];
-                    win[1][i] = win[2][i];
-                    win[2][i] = nr[i];
-                } else {
-                    win[2][i] = win[1][i];
-                    win[1][i] = win[0][i];
-                    win[0][i] = nr[i];
-                }
-            }
-        }
-    }
+    asm volatile("" ::"v"(nu), "v"(nv), "v"(cu), "v"(cv), "v"(ring[0][0]), "v"(ring[0][1]), "v"(ring[0][2]),
+                 "v"(ring[1][0]), "v"(ring[1][1]), "v"(ring[1][2]), "v"(ring[2][0]), "v"(ring[2][1]), "v"(ring[2][2]));
+    BmChain<T, DIR> ch{V, m, n_al, n, ce, live, role, a, e, cu, cv, nu, nv};
+    const int nfull = n / CH;
+    for (int j = 0; j < nfull; j++) ch.template chunk<CH, false>(j * CH, ring);
+    if (nfull * CH < n) ch.template chunk<CH, true>(nfull * CH, ring);
 }
 
 // -------------------------------------------------------------------------------------------
