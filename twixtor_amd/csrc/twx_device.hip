@@ -846,26 +846,37 @@ void launch_blur5_tile(hipStream_t s, const PlaneOp* d_ops, const std::vector<Pl
 
 // all_flow_out_of_range (flow.rs:431-442): any pixel with 0 <= x+u <= w-1 and 0 <= y+v <= h-1
 __global__ void k_range_any(const SolveJob* __restrict__ jobs, int level, int buf) {
-    JOB_PROLOGUE();
-    if (J.state[0] == 0) return;
+    const SolveJob& J = jobs[blockIdx.z];
+    if (level > J.start || J.state[0] == 0) return;  // the check ran out at an earlier level
+    const int w = J.w[level], h = J.h[level];
     const float* flow = J.flow[buf];
-    long i = (long)y * w + x;
-    float px = (float)x + flow[i * 2];
-    float py = (float)y + flow[i * 2 + 1];
-    bool in = 0.0f <= px && px <= (float)(w - 1) && 0.0f <= py && py <= (float)(h - 1);
-    if (in) atomicOr(&J.state[1], 1);
+    // a grid-stride pass: few blocks, since after the first level that finds a pixel in range the
+    // launch does nothing
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < w * h; i += gridDim.x * blockDim.x) {
+        const int y = i / w, x = i - y * w;
+        float px = (float)x + flow[(long)i * 2];
+        float py = (float)y + flow[(long)i * 2 + 1];
+        bool in = 0.0f <= px && px <= (float)(w - 1) && 0.0f <= py && py <= (float)(h - 1);
+        if (in) atomicOr(&J.state[1], 1);
+    }
 }
 
 // the range check result, then the damping at the start level (flow.rs:1291-1300)
 __global__ void k_range_apply(const SolveJob* __restrict__ jobs, int level, int buf) {
-    JOB_PROLOGUE();
+    const SolveJob& J = jobs[blockIdx.z];
+    if (level > J.start) return;
+    const bool zero = J.state[0] != 0 && J.state[1] == 0;
+    const bool damp = level == J.start && J.scale_damp;
+    if (!zero && !damp) return;  // the flow stays as it is
+    const int w = J.w[level], h = J.h[level];
     float* flow = J.flow[buf];
-    long i = (long)y * w + x;
-    float u = flow[i * 2], v = flow[i * 2 + 1];
-    if (J.state[0] != 0 && J.state[1] == 0) { u = 0.0f; v = 0.0f; }
-    if (level == J.start && J.scale_damp) { u = J.damp * u; v = J.damp * v; }
-    flow[i * 2] = u;
-    flow[i * 2 + 1] = v;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < w * h; i += gridDim.x * blockDim.x) {
+        float u = flow[(long)i * 2], v = flow[(long)i * 2 + 1];
+        if (zero) { u = 0.0f; v = 0.0f; }
+        if (damp) { u = J.damp * u; v = J.damp * v; }
+        flow[(long)i * 2] = u;
+        flow[(long)i * 2 + 1] = v;
+    }
 }
 
 __global__ void k_range_update(const SolveJob* __restrict__ jobs, int njobs, int level) {
@@ -1351,11 +1362,13 @@ static void launch_solve_range(hipStream_t s, const SolveJob* d_jobs, const std:
     for (int l = max_start; l >= 0; l--) {
         int mw = 0, mh = 0;
         double P = 0, Pn = 0;  // pixels of this level and of the next finer one, over the jobs
+        double Ps = 0;         // pixels of the jobs that start at this level
         for (auto& j : jobs)
             if (l <= j.start) {
                 mw = std::max(mw, j.w[l]);
                 mh = std::max(mh, j.h[l]);
                 P += (double)j.w[l] * j.h[l];
+                if (l == j.start) Ps += (double)j.w[l] * j.h[l];
                 if (l > 0) Pn += (double)j.w[l - 1] * j.h[l - 1];
             }
         dim3 blk(256), grd((mw + 255) / 256, mh, nj);
@@ -1370,8 +1383,11 @@ static void launch_solve_range(hipStream_t s, const SolveJob* d_jobs, const std:
             if (it == 1 && l == 0 && !skip_refine) bm(l, mw, mh, P, 0);
         }
         if (l != 0 && !skip_refine) bm(l, mw, mh, P, 1);
-        TWX_LAUNCH(s, F_RANGE, 8.0 * P, k_range_any<<<grd, blk, 0, s>>>(d_jobs, l, 1));
-        TWX_LAUNCH(s, F_RANGE, 16.0 * P, k_range_apply<<<grd, blk, 0, s>>>(d_jobs, l, 1));
+        // the check reads the flow while no level has found a pixel in range, and the flow is only
+        // rewritten when it is zeroed or damped: in practice both at the start level alone
+        const dim3 rgrd((unsigned)std::min<long>(64, ((long)mw * mh + 255) / 256), 1, nj);
+        TWX_LAUNCH(s, F_RANGE, 8.0 * Ps, k_range_any<<<rgrd, blk, 0, s>>>(d_jobs, l, 1));
+        TWX_LAUNCH(s, F_RANGE, 16.0 * Ps, k_range_apply<<<rgrd, blk, 0, s>>>(d_jobs, l, 1));
         TWX_LAUNCH(s, F_RANGE, 0.0, k_range_update<<<dim3((nj + 63) / 64), dim3(64), 0, s>>>(d_jobs, nj, l));
         if (l > 0) {
             int uw = 0, uh = 0;
