@@ -388,6 +388,15 @@ void build_pyramids(hipStream_t s, DevPool& pool, const std::vector<PyrIn>& in) 
             steps.push_back(bl); kinds.push_back(1);
             steps.push_back(rs); kinds.push_back(3);
         }
+        // the resample ratios (every target is 2 x 2 or larger: reductions stop at 2 pixels and
+        // pyramid levels are 3 or more)
+        for (size_t k = 0; k < steps.size(); k++)
+            if (kinds[k] == 3)
+                for (auto& op : steps[k])
+                    if (op.src) {
+                        op.rx = ((float)op.sw - 1.0f) / (float)(op.dw - 1);
+                        op.ry = ((float)op.sh - 1.0f) / (float)(op.dh - 1);
+                    }
         std::vector<PlaneOp> flat;
         for (auto& st : steps) flat.insert(flat.end(), st.begin(), st.end());
         PlaneOp* d = pool.upload(flat);
@@ -471,6 +480,7 @@ int enqueue_solves(hipStream_t s, DevPool& pool, const std::vector<SolveIn>& in,
             J.state = states + (ch.first + k) * 4;
             J.max_w = si.geo.w0;
             J.max_h = si.geo.h0;
+            solve_job_ratios(J);
         }
         SolveJob* d = pool.upload(jobs);
         launch_solve(s, d, jobs, fs.skip_refine != 0);  // starts with the per-job init kernel

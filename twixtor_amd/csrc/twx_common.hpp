@@ -28,6 +28,7 @@ struct PlaneOp {
     int fmt;  // luma: the engine image format
     int pad;
     float* aux;  // k_blur5_tile: where the band src - blur goes (null: none)
+    float rx, ry;  // k_resample: (sw - 1) / (dw - 1) and (sh - 1) / (dh - 1), computed on the host
 };
 
 // One flow solve X -> Y over a pyramid pair (flow.rs:1258-1307). Device resident table entry;
@@ -47,7 +48,14 @@ struct SolveJob {
     float* tmp;            // block-matching scratch: the transposed flow (2 channels)
     int* state;            // [0] range check still active, [1] any pixel in range, [2] NaN seen
     int max_w, max_h;
+    // upsample_flow from level l to l - 1 (flow.rs:376-391), the same for every pixel: the scales
+    // sx = (dw - 1) / (sw - 1), sy and their reciprocals (through double), at index l. Filled by
+    // solve_job_ratios before upload.
+    float up[MAXL][4];
 };
+
+// Fills a job's per-level upsample ratios (IEEE division on the host, as on the device).
+void solve_job_ratios(SolveJob& J);
 
 // One output frame of a retime run (compose.rs:93-246 fast path, compose/vector.rs:249-276 vector
 // path code 0), including the finish into wire bytes (image.rs:205-245).

@@ -389,8 +389,7 @@ __global__ void k_resample(const PlaneOp* __restrict__ ops) {
     int x = blockIdx.x * blockDim.x + threadIdx.x;
     int y = blockIdx.y;
     if (x >= op.dw || y >= op.dh) return;
-    float sx = ((float)op.sw - 1.0f) / (float)(op.dw - 1);
-    float sy = ((float)op.sh - 1.0f) / (float)(op.dh - 1);
+    const float sx = op.rx, sy = op.ry;
     float fy = (float)y * sy;
     op.dst[(long)y * op.dw + x] = bilinear1((const float*)op.src, op.sw, op.sh, (float)x * sx, fy);
 }
@@ -899,10 +898,7 @@ __global__ void k_upsample(const SolveJob* __restrict__ jobs, int level) {
     if (x >= dw || y >= dh) return;
     const float* src = J.flow[1];
     float* dst = J.flow[0];
-    float sx = ((float)dw - 1.0f) / (float)(sw - 1);
-    float sy = ((float)dh - 1.0f) / (float)(sh - 1);
-    float rx = (float)(1.0 / (double)sx);
-    float ry = (float)(1.0 / (double)sy);
+    const float sx = J.up[level][0], sy = J.up[level][1], rx = J.up[level][2], ry = J.up[level][3];
     float fx = (float)x * rx, fy = (float)y * ry;
     int ix, iy;
     float ax, ay;
@@ -921,6 +917,18 @@ __global__ void k_upsample(const SolveJob* __restrict__ jobs, int level) {
     long o = ((long)y * dw + x) * 2;
     dst[o] = out[0] * sx;
     dst[o + 1] = out[1] * sy;
+}
+
+void solve_job_ratios(SolveJob& J) {
+    for (int l = 1; l < J.levels && l < MAXL; l++) {
+        const int sw = J.w[l], sh = J.h[l], dw = J.w[l - 1], dh = J.h[l - 1];
+        const float sx = ((float)dw - 1.0f) / (float)(sw - 1);
+        const float sy = ((float)dh - 1.0f) / (float)(sh - 1);
+        J.up[l][0] = sx;
+        J.up[l][1] = sy;
+        J.up[l][2] = (float)(1.0 / (double)sx);
+        J.up[l][3] = (float)(1.0 / (double)sy);
+    }
 }
 
 // final negation (flow.rs:1305) of level 0's flow (buffer 1, the output) and the NaN scan the
