@@ -965,9 +965,9 @@ __device__ __forceinline__ gchar* run_at(const gfloat* base, uint32_t RS, int r,
 // Block::cost (flow.rs:472-524) without branches: `ok` false where the reference returns None (the
 // value is then meaningless). The operations and their order are those of the reference; the 4 x 4
 // texels the nine bilinear samples use are read once, as four runs of four.
-template <bool T>
+template <bool T, typename Between>
 __device__ __forceinline__ float bm_cost(const gfloat* __restrict__ b, int w, int h, const float p[9], float x2, float y2,
-                                         bool& ok) {
+                                         bool& ok, Between&& between) {
     const float wm2 = (float)(w - 2), hm2 = (float)(h - 2);
     ok = !(x2 > wm2 || y2 > hm2);
     // cvtt(wm2 - 1) is w - 3 (w >= 3); elsewhere the hardware truncation (NaN gives 0, out of
@@ -993,6 +993,7 @@ __device__ __forceinline__ float bm_cost(const gfloat* __restrict__ b, int w, in
 #pragma unroll
         for (int i = 0; i < 4; i++) t[r][i] = *(const gfloat*)(q + 4 * i);
     }
+    between();  // the caller's work between the texel loads and their use
     // texel (ix - 1 + dx, iy - 1 + dy)
     auto tx = [&](int dx, int dy) { return T ? t[dx][dy] : t[dy][dx]; };
     float sq[9];
@@ -1090,14 +1091,25 @@ __global__ void k_bm_transpose(const SolveJob* __restrict__ jobs, int level, int
     }
 }
 
+// steps per unrolled chunk of a block-matching chain (a multiple of the patch's three runs), and
+// how many steps ahead a chain prefetches its lines
+constexpr int BM_CHUNK = 6;
+#ifndef TWX_BM_PREFETCH_ROWS
+#define TWX_BM_PREFETCH_ROWS 4
+#endif
+#ifndef TWX_BM_PREFETCH_COLS
+#define TWX_BM_PREFETCH_COLS 0
+#endif
+
 // The state of one lane of a block-matching chain and its steps (Block::step, flow.rs:528-544),
 // run CH at a time with the loop unrolled: the patch runs sit in ring slots that rotate with the
 // step (slot of patch run r at step s: (s + r) mod 3 for DIR = +1, (r - s) mod 3 for DIR = -1;
 // CH is a multiple of 3, so every chunk starts from the same assignment), and the decision
 // updates the incoming vector by selects, so values move between registers only at chunk ends.
-template <bool T, int DIR>
+template <bool T, int DIR, int PD>
 struct BmChain {
     const BmView<T>& V;
+    float pf[3];  // the prefetched words of the last step
     int m, n_al, n, ce;
     bool live;
     int role;
@@ -1135,8 +1147,29 @@ struct BmChain {
             const int x = T ? a : ce, y = T ? ce : a;
             const float tx = (float)x - cu, ty = (float)y - cv;
             const bool in_range = 0.0f <= tx && tx <= (float)(V.w - 1) && 0.0f <= ty && ty <= (float)(V.h - 1);
+            // Every step touches new lines of A, B and the flow; a load that misses L2 costs the
+            // step its full latency, since the step waits on its dependent B texels and the memory
+            // counter retires loads in order. So each step asks, right after its own B loads, for
+            // one word of each line the chain will touch PD steps on (the leading B run around the
+            // lane's vector of now, the A run, the flow). The words are consumed one step later,
+            // after that step's own wait has covered them, so they cost the chain no wait of their
+            // own, and by their step the lines sit in L2.
+            auto prefetch = [&]() {
+                if (PD == 0) return;
+                asm volatile("" ::"v"(pf[0]), "v"(pf[1]), "v"(pf[2]));
+                const int ap = a + PD * DIR;
+                const float pu = role ? nu : cu, pv = role ? nv : cv;
+                const float al = (float)ap - (T ? pu : pv), ac = (float)ce - (T ? pv : pu);
+                const int rb = min(max(trunc_hw(fminf(fmaxf(al, -8.0f), (float)n_al + 8.0f)) + (DIR > 0 ? 2 : -1), 0), n_al - 1);
+                const int eb = min(max(trunc_hw(fminf(fmaxf(ac, -8.0f), (float)m + 8.0f)) - 1, 0), m - 1);
+                const int ra = min(max(ap + 2 * DIR, 0), n_al - 1);
+                const int rf = min(max(ap + DIR, 0), n_al - 1);
+                pf[0] = *(const gfloat*)run_at(V.B, (uint32_t)m, rb, eb);
+                pf[1] = *(const gfloat*)run_at(V.A, (uint32_t)m, ra, ce - 1);
+                pf[2] = ldg(V.fl, 2 * ((uint32_t)rf * m + ce));
+            };
             bool ok;
-            const float c = bm_cost<T>(V.B, V.w, V.h, p, role ? (float)x - nu : tx, role ? (float)y - nv : ty, ok);
+            const float c = bm_cost<T>(V.B, V.w, V.h, p, role ? (float)x - nu : tx, role ? (float)y - nv : ty, ok, prefetch);
             // the partner lane's values by a quad permutation [1, 0, 3, 2] (every lane active)
             const float c_o = __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(c), 0xB1, 0xF, 0xF, false));
             const int ok_o = __builtin_amdgcn_mov_dpp(ok ? 1 : 0, 0xB1, 0xF, 0xF, false);
@@ -1204,10 +1237,11 @@ __global__ __launch_bounds__(64) void k_bm_sweep(const SolveJob* __restrict__ jo
     }
     asm volatile("" ::"v"(nu), "v"(nv), "v"(cu), "v"(cv), "v"(ring[0][0]), "v"(ring[0][1]), "v"(ring[0][2]),
                  "v"(ring[1][0]), "v"(ring[1][1]), "v"(ring[1][2]), "v"(ring[2][0]), "v"(ring[2][1]), "v"(ring[2][2]));
-    BmChain<T, DIR> ch{V, m, n_al, n, ce, live, role, a, e, cu, cv, nu, nv};
+    BmChain<T, DIR, T ? TWX_BM_PREFETCH_ROWS : TWX_BM_PREFETCH_COLS> ch{V, {0.0f, 0.0f, 0.0f}, m, n_al, n, ce, live, role, a, e, cu, cv, nu, nv};
     const int nfull = n / CH;
     for (int j = 0; j < nfull; j++) ch.template chunk<CH, false>(j * CH, ring);
     if (nfull * CH < n) ch.template chunk<CH, true>(nfull * CH, ring);
+    asm volatile("" ::"v"(ch.pf[0]), "v"(ch.pf[1]), "v"(ch.pf[2]));
 }
 
 // -------------------------------------------------------------------------------------------
@@ -1340,8 +1374,7 @@ int tile_tuning_report(int* out, int cap) {
     return n;
 }
 
-// steps per unrolled chunk of a block-matching chain (a multiple of the patch's three runs)
-constexpr int BM_CHUNK = 6;
+
 
 static void launch_solve_range(hipStream_t s, const SolveJob* d_jobs, const std::vector<SolveJob>& jobs, bool skip_refine) {
     int nj = (int)jobs.size();
