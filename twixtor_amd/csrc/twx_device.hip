@@ -1094,12 +1094,9 @@ __global__ void k_bm_transpose(const SolveJob* __restrict__ jobs, int level, int
 // steps per unrolled chunk of a block-matching chain (a multiple of the patch's three runs), and
 // how many steps ahead a chain prefetches its lines
 constexpr int BM_CHUNK = 6;
-#ifndef TWX_BM_PREFETCH_ROWS
-#define TWX_BM_PREFETCH_ROWS 4
-#endif
-#ifndef TWX_BM_PREFETCH_COLS
-#define TWX_BM_PREFETCH_COLS 0
-#endif
+// (measured: the row sweeps gain from prefetching four steps ahead, the column sweeps lose at
+// every distance tried and do not prefetch)
+constexpr int BM_PREFETCH_ROWS = 4, BM_PREFETCH_COLS = 0;
 
 // The state of one lane of a block-matching chain and its steps (Block::step, flow.rs:528-544),
 // run CH at a time with the loop unrolled: the patch runs sit in ring slots that rotate with the
@@ -1237,7 +1234,7 @@ __global__ __launch_bounds__(64) void k_bm_sweep(const SolveJob* __restrict__ jo
     }
     asm volatile("" ::"v"(nu), "v"(nv), "v"(cu), "v"(cv), "v"(ring[0][0]), "v"(ring[0][1]), "v"(ring[0][2]),
                  "v"(ring[1][0]), "v"(ring[1][1]), "v"(ring[1][2]), "v"(ring[2][0]), "v"(ring[2][1]), "v"(ring[2][2]));
-    BmChain<T, DIR, T ? TWX_BM_PREFETCH_ROWS : TWX_BM_PREFETCH_COLS> ch{V, {0.0f, 0.0f, 0.0f}, m, n_al, n, ce, live, role, a, e, cu, cv, nu, nv};
+    BmChain<T, DIR, T ? BM_PREFETCH_ROWS : BM_PREFETCH_COLS> ch{V, {0.0f, 0.0f, 0.0f}, m, n_al, n, ce, live, role, a, e, cu, cv, nu, nv};
     const int nfull = n / CH;
     for (int j = 0; j < nfull; j++) ch.template chunk<CH, false>(j * CH, ring);
     if (nfull * CH < n) ch.template chunk<CH, true>(nfull * CH, ring);
