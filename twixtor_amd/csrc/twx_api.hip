@@ -656,8 +656,18 @@ int fill_items(hipStream_t s, DevPool& pool, const std::vector<FillItem>& items,
                 P.t1 = take((size_t)it.W * it.H);
                 P.t2 = take((size_t)it.W * it.H);
             }
-            for (int q = 0; q < nplanes; q++)
+            // the coarse levels that fit in one workgroup's LDS (from level 1 on)
+            int lo = top + 1;
+            for (int l = top; l >= 1; l--) {
+                size_t pyr_l = 0;
+                for (int k = l; k <= top; k++) pyr_l += (size_t)dims[k].first * dims[k].second;
+                if (3 * pyr_l + 2 * (size_t)dims[l].first * dims[l].second > (size_t)FILL_LDS_FLOATS) break;
+                lo = l;
+            }
+            for (int q = 0; q < nplanes; q++) {
+                fp[q].lo = lo;
                 for (int l = 0; l <= top; l++) fp[q].wts[l] = fp[nplanes - 1].val[l];
+            }
             for (int q = 0; q < nplanes; q++) planes.push_back(fp[q]);
         }
         FillPlane* dp = pool.upload(planes);

@@ -133,7 +133,12 @@ struct FillPlane {
     const uint8_t* holes;
     int fmt;
     int above_first;
+    int lo;                   // levels lo .. top run in one workgroup in LDS (none when lo > top)
 };
+
+// LDS floats of the fill's coarse levels: the value, weight and solution pyramids of levels
+// lo .. top plus two planes of level lo's size
+constexpr int FILL_LDS_FLOATS = 32768;
 
 // Per-kernel-family timing with HIP events on the launching stream, and the algorithmic bytes of
 // each launch (the step-boundary model: distinct inputs read once, outputs written once).

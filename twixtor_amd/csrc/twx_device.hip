@@ -2269,8 +2269,9 @@ static void launch_weighted_range(hipStream_t s, const FrameJob* d_jobs, const s
 // The push-pull hole fill (holefill.rs:214-336), batched over planes (frame x channel): the
 // frame's weight plane and its four colour planes are pulled alike (blur5, resample), the colour
 // planes pushed (combine, Gaussian r = 5, resample). The combine of a level is fused into the
-// resample that produces it (and into the top-level init and the final store). Each kernel
-// returns at once for a frame without holes (the fill is the identity then).
+// resample that produces it (and into the top-level init and the final store). The levels from
+// a plane's `lo` up run in k_fill_coarse, in LDS; the per-level kernels handle the levels below.
+// Each kernel returns at once for a frame without holes (the fill is the identity then).
 
 // Load (holefill.rs:314-321): the channel in native units, holes zeroed; weights 0 at holes, else 1
 __global__ void k_fill_load(const FillPlane* __restrict__ planes) {
@@ -2299,6 +2300,7 @@ __global__ void k_fill_load(const FillPlane* __restrict__ planes) {
 __global__ void k_fill_hpass(const FillPlane* __restrict__ planes, int level, int push, Taps t) {
     const FillPlane& P = planes[blockIdx.z];
     if (!*P.gate || level > P.top || (push && P.chan == 4)) return;
+    if (push ? level >= P.lo : level > P.lo) return;  // a coarse level: k_fill_coarse
     const int l = push ? level : level - 1;
     const int W = P.w[l], H = P.h[l];
     const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
@@ -2312,6 +2314,7 @@ __global__ void k_fill_hpass(const FillPlane* __restrict__ planes, int level, in
 __global__ void k_fill_vpass(const FillPlane* __restrict__ planes, int level, int push, Taps t) {
     const FillPlane& P = planes[blockIdx.z];
     if (!*P.gate || level > P.top || (push && P.chan == 4)) return;
+    if (push ? level >= P.lo : level > P.lo) return;  // a coarse level: k_fill_coarse
     const int l = push ? level : level - 1;
     const int W = P.w[l], H = P.h[l];
     const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
@@ -2325,6 +2328,7 @@ __global__ void k_fill_vpass(const FillPlane* __restrict__ planes, int level, in
 __global__ void k_fill_resample(const FillPlane* __restrict__ planes, int level, int push) {
     const FillPlane& P = planes[blockIdx.z];
     if (!*P.gate || level > P.top || level < 1 || (push && P.chan == 4)) return;
+    if (level > P.lo) return;  // inside the coarse levels: k_fill_coarse
     const int sl = push ? level : level - 1, dl = push ? level - 1 : level;
     const int sw = P.w[sl], sh = P.h[sl], dw = P.w[dl], dh = P.h[dl];
     const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
@@ -2342,13 +2346,104 @@ __global__ void k_fill_resample(const FillPlane* __restrict__ planes, int level,
 // The solution of the top level: the combine over zeros (holefill.rs:322-326).
 __global__ void k_fill_push_init(const FillPlane* __restrict__ planes) {
     const FillPlane& P = planes[blockIdx.z];
-    if (!*P.gate || P.chan == 4) return;
+    if (!*P.gate || P.chan == 4 || P.lo <= P.top) return;
     const int l = P.top, W = P.w[l], H = P.h[l];
     const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
     if (x >= W || y >= H) return;
     const long i = (long)y * W + x;
     const float wt = P.wts[l][i];
     P.sol[l][i] = wt != 0.0f ? P.val[l][i] / wt : 0.0f;
+}
+
+// The coarse levels lo .. top of a colour plane in one workgroup, every plane in LDS: the pull of
+// the values and of the frame's weights from level lo up (blur5, resample), the top level's
+// solution, and the push down to level lo (Gaussian r = 5, resample, combine), whose solution
+// goes back to memory; the same operations, in the same order, as the per-level kernels.
+__global__ __launch_bounds__(1024) void k_fill_coarse(const FillPlane* __restrict__ planes, Taps t5, Taps tg) {
+    const FillPlane& P = planes[blockIdx.x];
+    if (!*P.gate || P.chan == 4 || P.lo > P.top) return;  // the same for the whole workgroup
+    __shared__ float lds[FILL_LDS_FLOATS];
+    const int lo = P.lo, top = P.top, tid = threadIdx.x, nt = blockDim.x;
+    float* val[MAXFL];
+    float* wts[MAXFL];
+    float* sol[MAXFL];
+    float* q = lds;
+    for (int l = lo; l <= top; l++) {
+        const int a = P.w[l] * P.h[l];
+        val[l] = q;
+        wts[l] = q + a;
+        sol[l] = q + 2 * a;
+        q += 3 * a;
+    }
+    float* t1 = q;
+    float* t2 = q + P.w[lo] * P.h[lo];
+    {
+        const int a = P.w[lo] * P.h[lo];
+        for (int i = tid; i < a; i += nt) {
+            val[lo][i] = P.val[lo][i];
+            wts[lo][i] = P.wts[lo][i];
+        }
+    }
+    __syncthreads();
+    // pull: level k from level k - 1, for the values and the weights alike
+    for (int k = lo + 1; k <= top; k++) {
+        const int W = P.w[k - 1], H = P.h[k - 1], dw = P.w[k], dh = P.h[k];
+        const float sx = ((float)W - 1.0f) / (float)(dw - 1);
+        const float sy = ((float)H - 1.0f) / (float)(dh - 1);
+        for (int pl = 0; pl < 2; pl++) {
+            const float* src = pl ? wts[k - 1] : val[k - 1];
+            float* dst = pl ? wts[k] : val[k];
+            for (int i = tid; i < W * H; i += nt) {
+                const int y = i / W, x = i - y * W;
+                t1[i] = hpass_elem<1>(src + y * W, W, x, 0, t5);
+            }
+            __syncthreads();
+            for (int i = tid; i < W * H; i += nt) {
+                const int y = i / W, x = i - y * W;
+                t2[i] = P.above_first ? vpass_elem<1, true>(t1, W, H, x, y, 0, t5) : vpass_elem<1, false>(t1, W, H, x, y, 0, t5);
+            }
+            __syncthreads();
+            for (int i = tid; i < dw * dh; i += nt) {
+                const int y = i / dw, x = i - y * dw;
+                dst[i] = bilinear1(t2, W, H, (float)x * sx, (float)y * sy);
+            }
+            __syncthreads();
+        }
+    }
+    // the top level's solution: the combine over zeros
+    {
+        const int a = P.w[top] * P.h[top];
+        for (int i = tid; i < a; i += nt) sol[top][i] = wts[top][i] != 0.0f ? val[top][i] / wts[top][i] : 0.0f;
+    }
+    __syncthreads();
+    // push: the Gaussian of level l, then its resample into level l - 1 with that level's combine
+    for (int l = top; l >= lo; l--) {
+        const int W = P.w[l], H = P.h[l];
+        for (int i = tid; i < W * H; i += nt) {
+            const int y = i / W, x = i - y * W;
+            t1[i] = hpass_elem<1>(sol[l] + y * W, W, x, 0, tg);
+        }
+        __syncthreads();
+        for (int i = tid; i < W * H; i += nt) {
+            const int y = i / W, x = i - y * W;
+            sol[l][i] = P.above_first ? vpass_elem<1, true>(t1, W, H, x, y, 0, tg) : vpass_elem<1, false>(t1, W, H, x, y, 0, tg);
+        }
+        __syncthreads();
+        if (l == lo) break;
+        const int dw = P.w[l - 1], dh = P.h[l - 1];
+        const float sx = ((float)W - 1.0f) / (float)(dw - 1);
+        const float sy = ((float)H - 1.0f) / (float)(dh - 1);
+        for (int i = tid; i < dw * dh; i += nt) {
+            const int y = i / dw, x = i - y * dw;
+            float v = bilinear1(sol[l], W, H, (float)x * sx, (float)y * sy);
+            const float wt = wts[l - 1][i];
+            if (wt != 0.0f) v = val[l - 1][i] / wt;
+            sol[l - 1][i] = v;
+        }
+        __syncthreads();
+    }
+    const int a = P.w[lo] * P.h[lo];
+    for (int i = tid; i < a; i += nt) P.sol[lo][i] = sol[lo][i];
 }
 
 // The last combine of level 0 and the write back (holefill.rs:330-333, 366-386).
@@ -2402,14 +2497,29 @@ static void launch_fill_range(hipStream_t s, const FillPlane* d_planes, const st
     auto grid = [&](int l) { return dim3((mw[l] + 255) / 256, mh[l], np); };
     const dim3 blk(256);
     const Taps t5 = binomial_taps(), tg = gaussian_taps(5.0f);
+    // levels above `hi` run only inside k_fill_coarse; the per-level kernels stop there
+    int hi = 0;
+    bool coarse = false, init = false;
+    double lo_px = 0.0;
+    for (auto& p : planes) {
+        hi = std::max(hi, std::min(p.top, p.lo));
+        if (p.lo <= p.top) {
+            coarse = true;
+            if (p.chan != 4) lo_px += (double)p.w[p.lo] * p.h[p.lo];
+        } else {
+            init = true;
+        }
+    }
     TWX_LAUNCH(s, F_FILL_IO, 8.0 * px[0], k_fill_load<<<grid(0), blk, 0, s>>>(d_planes));
-    for (int k = 1; k <= maxtop; k++) {
+    for (int k = 1; k <= hi; k++) {
         TWX_LAUNCH(s, F_FILL_PULL, 8.0 * px[k - 1], k_fill_hpass<<<grid(k - 1), blk, 0, s>>>(d_planes, k, 0, t5));
         TWX_LAUNCH(s, F_FILL_PULL, 8.0 * px[k - 1], k_fill_vpass<<<grid(k - 1), blk, 0, s>>>(d_planes, k, 0, t5));
         TWX_LAUNCH(s, F_FILL_PULL, 4.0 * (px[k - 1] + px[k]), k_fill_resample<<<grid(k), blk, 0, s>>>(d_planes, k, 0));
     }
-    TWX_LAUNCH(s, F_FILL_PUSH, 12.0 * px[maxtop], k_fill_push_init<<<grid(maxtop), blk, 0, s>>>(d_planes));
-    for (int l = maxtop; l >= 0; l--) {
+    // the coarse kernel reads level lo's values and weights and writes its solution
+    if (coarse) TWX_LAUNCH(s, F_FILL_PUSH, 12.0 * lo_px, k_fill_coarse<<<dim3(np), dim3(1024), 0, s>>>(d_planes, t5, tg));
+    if (init) TWX_LAUNCH(s, F_FILL_PUSH, 12.0 * px[maxtop], k_fill_push_init<<<grid(maxtop), blk, 0, s>>>(d_planes));
+    for (int l = hi; l >= 0; l--) {
         TWX_LAUNCH(s, F_FILL_PUSH, 8.0 * px[l], k_fill_hpass<<<grid(l), blk, 0, s>>>(d_planes, l, 1, tg));
         TWX_LAUNCH(s, F_FILL_PUSH, 8.0 * px[l], k_fill_vpass<<<grid(l), blk, 0, s>>>(d_planes, l, 1, tg));
         if (l > 0)
