@@ -48,16 +48,18 @@ struct FwE {
     float p[4], dp[4];
 };
 
-// raster.rs:59-86: activates the edge a -> b for a.y <= y < b.y, state set for scanline y - 1
+// raster.rs:59-86: activates the edge a -> b for a.y <= y < b.y, state set for scanline y - 1,
+// into e1 when an edge is active already, else into e0. The edge pair lives in two named
+// structs (no runtime-indexed private arrays, which would go to scratch memory).
 template <bool PAY>
-__device__ __forceinline__ void fw_edge_init(const FwV& a, const FwV& b, float y, FwE e[2], uint32_t& count) {
+__device__ __forceinline__ void fw_edge_init(const FwV& a, const FwV& b, float y, FwE& e0, FwE& e1, uint32_t& count) {
     if (a.y <= y && y < b.y) {
-        const int slot = count != 0 ? 1 : 0;
+        const bool second = count != 0;
         count++;
         const float dx = b.x - a.x;
         const float inv = 1.0f / (b.y - a.y);
         const float t = ((y - 1.0f) - a.y) * inv;
-        FwE& E = e[slot];
+        FwE E = second ? e1 : e0;
         E.ay = a.y;
         E.by = b.y;
         E.x = dx * t + a.x;
@@ -70,7 +72,19 @@ __device__ __forceinline__ void fw_edge_init(const FwV& a, const FwV& b, float y
                 E.dp[k] = dk * inv;
             }
         }
+        if (second) e1 = E;
+        else e0 = E;
     }
+}
+
+// one of three vertices by a runtime index, field by field (selects, no indexed private array)
+__device__ __forceinline__ FwV fw_pick(const FwV& v0, const FwV& v1, const FwV& v2, int a) {
+    FwV o;
+    o.x = a == 0 ? v0.x : (a == 1 ? v1.x : v2.x);
+    o.y = a == 0 ? v0.y : (a == 1 ? v1.y : v2.y);
+#pragma unroll
+    for (int k = 0; k < 4; k++) o.p[k] = a == 0 ? v0.p[k] : (a == 1 ? v1.p[k] : v2.p[k]);
+    return o;
 }
 
 // raster.rs:88-112: ties keep the given order except v2.y == v0.y < v1.y -> v2, v0, v1
@@ -84,9 +98,33 @@ __device__ __forceinline__ void fw_sort_by_y(const FwV v[3], FwV o[3]) {
     } else if (y2 < y1) {
         if (y0 < y2) { a = 0; b = 2; c = 1; } else { a = 2; b = 0; c = 1; }
     } else { a = 0; b = 1; c = 2; }
-    o[0] = v[a];
-    o[1] = v[b];
-    o[2] = v[c];
+    o[0] = fw_pick(v[0], v[1], v[2], a);
+    o[1] = fw_pick(v[0], v[1], v[2], b);
+    o[2] = fw_pick(v[0], v[1], v[2], c);
+}
+
+// raster.rs:153-214: on reaching vertex K, the edge pair pruned of the edges that ended, then
+// K's two edges (0: 1, 2), (1: 2, 0), (2: 0, 1) activated
+template <bool PAY, int K>
+__device__ __forceinline__ void fw_vertex_edges(const FwV vs[3], float y, FwE& e0, FwE& e1, uint32_t& count) {
+    if (count == 1) {
+        if (e0.ay <= y && e0.by <= y) count = 0;
+    } else if (count == 2) {
+        if (y < e1.ay || y < e1.by) {
+            if (!(y < e0.ay || y < e0.by)) {
+                e0 = e1;
+                count = 1;
+            }
+        } else if (e0.ay <= y && e0.by <= y) {
+            count = 0;
+        } else {
+            count = 1;
+        }
+    }
+    constexpr int N0 = K == 0 ? 1 : (K == 1 ? 2 : 0);
+    constexpr int N1 = K == 0 ? 2 : (K == 1 ? 0 : 1);
+    fw_edge_init<PAY>(vs[K], vs[N0], y, e0, e1, count);
+    fw_edge_init<PAY>(vs[K], vs[N1], y, e0, e1, count);
 }
 
 // raster.rs:114-151: one span of the main pass. `hit(x, y, p)` returns false to stop the triangle.
@@ -144,53 +182,47 @@ __device__ int fw_triangle(const FwV tri[3], int32_t w1, int32_t h1, Hit& hit) {
     if (y < 0.0f) y = 0.0f;
     if (h1f < y_end) y_end = h1f;
     if (y <= y_end) {
-        int vi = 0;
+        int vi = 0;  // the next vertex to reach; vertices are reached in order, several per row
         uint32_t count = 0;
-        FwE e[2];
+        FwE e0, e1;
+        e0.ay = e0.by = e0.x = e0.dx = 0.0f;
 #pragma unroll
-        for (int s = 0; s < 2; s++) {
-            e[s].ay = e[s].by = e[s].x = e[s].dx = 0.0f;
-#pragma unroll
-            for (int k = 0; k < 4; k++) e[s].p[k] = e[s].dp[k] = 0.0f;
-        }
+        for (int k = 0; k < 4; k++) e0.p[k] = e0.dp[k] = 0.0f;
+        e1 = e0;
         for (;;) {
-            if (vi < 3) {
-                for (;;) {
-                    if (y < vs[vi].y) break;
-                    if (count == 1) {
-                        if (e[0].ay <= y && e[0].by <= y) count = 0;
-                    } else if (count == 2) {
-                        if (y < e[1].ay || y < e[1].by) {
-                            if (!(y < e[0].ay || y < e[0].by)) {
-                                e[0] = e[1];
-                                count = 1;
-                            }
-                        } else if (e[0].ay <= y && e[0].by <= y) {
-                            count = 0;
-                        } else {
-                            count = 1;
-                        }
-                    }
-                    // vertex vi's two edges: (0: 1, 2), (1: 2, 0), (2: 0, 1)
-                    const int n0 = vi == 0 ? 1 : (vi == 1 ? 2 : 0);
-                    const int n1 = vi == 0 ? 2 : (vi == 1 ? 0 : 1);
-                    fw_edge_init<PAY>(vs[vi], vs[n0], y, e, count);
-                    fw_edge_init<PAY>(vs[vi], vs[n1], y, e, count);
-                    vi++;
-                    if (vi >= 3) break;
-                }
+            if (vi == 0 && !(y < vs[0].y)) {
+                fw_vertex_edges<PAY, 0>(vs, y, e0, e1, count);
+                vi = 1;
             }
+            if (vi == 1 && !(y < vs[1].y)) {
+                fw_vertex_edges<PAY, 1>(vs, y, e0, e1, count);
+                vi = 2;
+            }
+            if (vi == 2 && !(y < vs[2].y)) {
+                fw_vertex_edges<PAY, 2>(vs, y, e0, e1, count);
+                vi = 3;
+            }
+            e0.x += e0.dx;
+            e1.x += e1.dx;
+            if (PAY) {
 #pragma unroll
-            for (int s = 0; s < 2; s++) {
-                e[s].x += e[s].dx;
-                if (PAY) {
-#pragma unroll
-                    for (int k = 0; k < 4; k++) e[s].p[k] = e[s].p[k] + e[s].dp[k];
+                for (int k = 0; k < 4; k++) {
+                    e0.p[k] = e0.p[k] + e0.dp[k];
+                    e1.p[k] = e1.p[k] + e1.dp[k];
                 }
             }
             if (count == 2) {
-                const bool go = e[1].x < e[0].x ? fw_span<PAY>(e[1], e[0], y, w1, hit) : fw_span<PAY>(e[0], e[1], y, w1, hit);
-                if (!go) return 2;
+                // the left and right edge by value (a select per field)
+                const bool swap = e1.x < e0.x;
+                FwE l, r;
+                l.x = swap ? e1.x : e0.x;
+                r.x = swap ? e0.x : e1.x;
+#pragma unroll
+                for (int k = 0; k < 4; k++) {
+                    l.p[k] = swap ? e1.p[k] : e0.p[k];
+                    r.p[k] = swap ? e0.p[k] : e1.p[k];
+                }
+                if (!fw_span<PAY>(l, r, y, w1, hit)) return 2;
             }
             y += 1.0f;
             if (!(y <= y_end)) break;
@@ -201,8 +233,9 @@ __device__ int fw_triangle(const FwV tri[3], int32_t w1, int32_t h1, Hit& hit) {
     const FwV& bot = vs[2];
     int status = 0;
     if (mid.y == h1f && mid.y == bot.y) {
-        const FwV& l = bot.x <= mid.x ? bot : mid;
-        const FwV& r = bot.x <= mid.x ? mid : bot;
+        const bool bl = bot.x <= mid.x;
+        const FwV l = fw_pick(bot, mid, mid, bl ? 0 : 1);
+        const FwV r = fw_pick(mid, bot, bot, bl ? 0 : 1);
         const float yy = l.y, yt = (float)fw_cvtt(yy);
         if (0.0f <= yt && yt <= h1f && !((double)(yy - yt) > 0.01)) {
             const int32_t x0 = max(fw_cvtt(ceilf(l.x)), 0);
@@ -308,34 +341,28 @@ __global__ void k_fw_mesh(FwArgs A) {
         px = (float)((double)px + 0.0001);
         py = (float)((double)py + 0.0001);
     }
-    A.pos[(long)r * A.W + c] = make_float2(px, py);
+    const long i = (long)r * A.W + c;
+    A.pos[i] = make_float2(px, py);
+    if (A.fmt == TWX_FMT_F32) {
+        // a NaN colour at any vertex (every pixel is a vertex of some triangle when W, H >= 2)
+        float col[4];
+        fw_texel<2>(A.src, i, col);
+        if (isnan(col[0]) || isnan(col[1]) || isnan(col[2]) || isnan(col[3])) atomicOr(A.flags, 4);
+    }
 }
 
 // Per-triangle work cap: a triangle that would cover more pixels stops and the warp is refused.
 constexpr uint32_t FW_TRI_CAP = 1u << 16;
 
 // flags: 1 the last row pass would write outside, 2 a triangle over FW_TRI_CAP hits, 4 a NaN
-// colour, 8 two hits of one triangle on one pixel, 16 the emit pass disagreed with the count pass
-template <int FMT>
+// colour (set by k_fw_mesh), 8 two hits of one triangle on one pixel, 16 the emit pass disagreed
+// with the count pass
 __global__ void k_fw_count(FwArgs A) {
     const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= A.ntri) return;
     FwV tri[3];
     bool nan_colour;
-    fw_vertices<FMT, false>(A, (uint32_t)t, tri, nan_colour);
-    if (FMT == 2) {
-        // the colours only matter for the NaN check here
-        const uint32_t cell = (uint32_t)(t >> 1), W1 = (uint32_t)A.W - 1;
-        const uint32_t r = cell / W1, c = cell - r * W1;
-        const long i00 = (long)r * A.W + c;
-        const long idx[3] = {(t & 1) ? i00 + A.W + 1 : i00, i00 + 1, i00 + A.W};
-        for (int q = 0; q < 3; q++) {
-            float col[4];
-            fw_texel<FMT>(A.src, idx[q], col);
-            for (int k = 0; k < 4; k++) nan_colour |= isnan(col[k]);
-        }
-        if (nan_colour) atomicOr(A.flags, 4);
-    }
+    fw_vertices<0, false>(A, (uint32_t)t, tri, nan_colour);
     uint32_t n = 0;
     auto hit = [&](int32_t x, int32_t y, const float*) -> bool {
         if (++n > FW_TRI_CAP) return false;
@@ -560,11 +587,10 @@ void fw_grid(FwArgs& A, float amount) {
 void launch_fw_count(hipStream_t s, const FwArgs& A) {
     const double px = (double)A.W * A.H, ss = (double)sample_size(A.fmt);
     dim3 blk(256);
-    TWX_LAUNCH(s, F_FW_MESH, 8.0 * A.fw * A.fh + 8.0 * px, k_fw_mesh<<<dim3((A.W + 255) / 256, A.H), blk, 0, s>>>(A));
+    const double nan_scan = A.fmt == TWX_FMT_F32 ? 4.0 * ss * px : 0.0;
+    TWX_LAUNCH(s, F_FW_MESH, 8.0 * A.fw * A.fh + 8.0 * px + nan_scan, k_fw_mesh<<<dim3((A.W + 255) / 256, A.H), blk, 0, s>>>(A));
     const dim3 tg((unsigned)((A.ntri + 255) / 256));
-    if (A.fmt == TWX_FMT_U8) TWX_LAUNCH(s, F_FW_RASTER, 8.0 * px + 4.0 * px, k_fw_count<0><<<tg, blk, 0, s>>>(A));
-    else if (A.fmt == TWX_FMT_U16) TWX_LAUNCH(s, F_FW_RASTER, 8.0 * px + 4.0 * px, k_fw_count<1><<<tg, blk, 0, s>>>(A));
-    else TWX_LAUNCH(s, F_FW_RASTER, 8.0 * px + 4.0 * px + 4.0 * ss * px, k_fw_count<2><<<tg, blk, 0, s>>>(A));
+    TWX_LAUNCH(s, F_FW_RASTER, 8.0 * px + 4.0 * px, k_fw_count<<<tg, blk, 0, s>>>(A));
     const long n = (long)A.W * A.H;
     const int nb = (int)fw_scan_blocks(n);
     TWX_LAUNCH(s, F_FW_SCAN, 12.0 * px, k_fw_scan_blocks<<<nb, FW_SCAN_THREADS, 0, s>>>(A.counts, A.offsets, A.block_totals, n));
