@@ -735,34 +735,25 @@ __global__ __launch_bounds__(TILE_X * TY / TILE_RUN) void k_tile_gauss(const Sol
     }
 }
 
-// blur5 of a plane (conv.rs:598-629: the row pass, then the column pass with the border rows
-// above first; flow.rs:242-275 and prep.rs:252-259 run it on every pyramid level and reduction
-// step) as one LDS-tiled kernel over a batch of planes: op.src in, the blurred plane to op.dst,
-// and with op.aux set the band op.aux = src - blur (flow.rs:228-231). Each element is computed as
-// by the separate passes, so the bits are the same.
-__global__ __launch_bounds__(TILE_THREADS) void k_blur5_tile(const PlaneOp* __restrict__ ops, Taps t, int tiles_x,
-                                                             int tiles_y, int nops) {
-    constexpr int R = 2;
-    const int per_op = tiles_x * tiles_y;
-    const int lt = xcd_tile(per_op * nops);
-    if (lt >= per_op * nops) return;
-    const int job = lt / per_op, rem = lt - job * per_op;
-    const int tyi = rem / tiles_x, txi = rem - tyi * tiles_x;
-    const PlaneOp& op = ops[job];
-    const int w = op.sw, h = op.sh;
-    const int x0 = txi * TILE_X, y0 = tyi * TILE_Y;
-    if (x0 >= w || y0 >= h) return;
+// The separable pass of one TILE_X x TILE_Y tile of a one-channel plane (conv.rs:144-199,
+// 272-311): the tile and R pixels around it into LDS (s_in, (TILE_Y + 2R) x (TILE_X + 2R)
+// floats), the row pass with the border columns right first, then the column pass with the border
+// rows above first (ABOVE) or below first. Lane (column, run of TILE_RUN rows) receives its run's
+// outputs in o (rows inside the plane only) at column x from row ys; the return is false for a
+// lane with no output. Every element is computed as by hpass_elem and vpass_elem, so the bits are
+// the same. Every lane of the block calls it (it holds barriers).
+template <int R, bool ABOVE>
+__device__ __forceinline__ bool sep_tile(const float* __restrict__ src, int w, int h, int x0, int y0, const Taps& t,
+                                         float* s_in, float o[TILE_RUN], int& x, int& ys) {
     constexpr int RW = TILE_X + 2 * R, RH = TILE_Y + 2 * R;
     constexpr int RUNS_X = TILE_X / TILE_RUN, HITEMS = RH * RUNS_X;
     constexpr int HPER = (HITEMS + TILE_THREADS - 1) / TILE_THREADS;
-    __shared__ float s_in[RH * RW];
-    const float* __restrict__ src = (const float*)op.src;
     const int tid = threadIdx.x;
     for (int k = tid; k < RH * RW; k += TILE_THREADS) {
         const int ry = k / RW, rx = k - ry * RW;
-        const int x = x0 - R + rx, y = y0 - R + ry;
-        if (x < 0 || x >= w || y < 0 || y >= h) continue;
-        s_in[k] = src[(long)y * w + x];
+        const int xx = x0 - R + rx, yy = y0 - R + ry;
+        if (xx < 0 || xx >= w || yy < 0 || yy >= h) continue;
+        s_in[k] = src[(long)yy * w + xx];
     }
     __syncthreads();
     const int nh = min(R + 1, w);
@@ -773,8 +764,8 @@ __global__ __launch_bounds__(TILE_THREADS) void k_blur5_tile(const PlaneOp* __re
         const int it = tid + q * TILE_THREADS;
         if (it >= HITEMS) continue;
         const int ry = it / RUNS_X, rx0 = (it - ry * RUNS_X) * TILE_RUN;
-        const int y = y0 - R + ry, xs = x0 + rx0;
-        if (y < 0 || y >= h || xs >= w) continue;
+        const int yy = y0 - R + ry, xs = x0 + rx0;
+        if (yy < 0 || yy >= h || xs >= w) continue;
         const float* line = s_in + ry * RW;
         if (xs >= nh && xs + TILE_RUN - 1 < nh + hint) {
             float win[TILE_RUN + 2 * R];
@@ -793,20 +784,20 @@ __global__ __launch_bounds__(TILE_THREADS) void k_blur5_tile(const PlaneOp* __re
         const int it = tid + q * TILE_THREADS;
         if (it >= HITEMS) continue;
         const int ry = it / RUNS_X, rx0 = (it - ry * RUNS_X) * TILE_RUN;
-        const int y = y0 - R + ry, xs = x0 + rx0;
-        if (y < 0 || y >= h || xs >= w) continue;
+        const int yy = y0 - R + ry, xs = x0 + rx0;
+        if (yy < 0 || yy >= h || xs >= w) continue;
 #pragma unroll
         for (int m = 0; m < TILE_RUN; m++)
             if (xs + m < w) s_in[ry * RW + R + rx0 + m] = hres[q][m];
     }
     __syncthreads();
     const int hx = tid % TILE_X, ry0 = (tid / TILE_X) * TILE_RUN;
-    const int x = x0 + hx, ys = y0 + ry0;
-    if (x >= w || ys >= h) return;
+    x = x0 + hx;
+    ys = y0 + ry0;
+    if (x >= w || ys >= h) return false;
     const int nv = min(R + 1, h);
     const int vint = h > 2 * (R + 1) ? h - 2 * (R + 1) : 0;
     const float* col = s_in + R + hx;
-    float o[TILE_RUN];
     if (ys >= nv && ys + TILE_RUN - 1 < nv + vint) {
         float win[TILE_RUN + 2 * R];
 #pragma unroll
@@ -815,8 +806,32 @@ __global__ __launch_bounds__(TILE_THREADS) void k_blur5_tile(const PlaneOp* __re
     } else {
 #pragma unroll
         for (int m = 0; m < TILE_RUN; m++)
-            if (ys + m < h) o[m] = line_elem<R, true, false>(col, RW, y0 - R, h, ys + m, t);
+            if (ys + m < h) o[m] = line_elem<R, ABOVE, false>(col, RW, y0 - R, h, ys + m, t);
     }
+    return true;
+}
+
+// blur5 of a plane (conv.rs:598-629: the row pass, then the column pass with the border rows
+// above first; flow.rs:242-275 and prep.rs:252-259 run it on every pyramid level and reduction
+// step) as one LDS-tiled kernel over a batch of planes: op.src in, the blurred plane to op.dst,
+// and with op.aux set the band op.aux = src - blur (flow.rs:228-231).
+__global__ __launch_bounds__(TILE_THREADS) void k_blur5_tile(const PlaneOp* __restrict__ ops, Taps t, int tiles_x,
+                                                             int tiles_y, int nops) {
+    constexpr int R = 2;
+    const int per_op = tiles_x * tiles_y;
+    const int lt = xcd_tile(per_op * nops);
+    if (lt >= per_op * nops) return;
+    const int job = lt / per_op, rem = lt - job * per_op;
+    const int tyi = rem / tiles_x, txi = rem - tyi * tiles_x;
+    const PlaneOp& op = ops[job];
+    const int w = op.sw, h = op.sh;
+    const int x0 = txi * TILE_X, y0 = tyi * TILE_Y;
+    if (x0 >= w || y0 >= h) return;
+    __shared__ float s_in[(TILE_Y + 2 * R) * (TILE_X + 2 * R)];
+    const float* __restrict__ src = (const float*)op.src;
+    float o[TILE_RUN];
+    int x, ys;
+    if (!sep_tile<R, true>(src, w, h, x0, y0, t, s_in, o, x, ys)) return;
 #pragma unroll
     for (int m = 0; m < TILE_RUN; m++) {
         if (ys + m >= h) break;
@@ -2295,36 +2310,42 @@ __global__ void k_fill_load(const FillPlane* __restrict__ planes) {
     P.val[0][i] = v;
 }
 
-// The horizontal pass of the pull blur5 of level `level - 1` (push false) or of the push Gaussian
-// of level `level` (push true), into t1.
-__global__ void k_fill_hpass(const FillPlane* __restrict__ planes, int level, int push, Taps t) {
-    const FillPlane& P = planes[blockIdx.z];
-    if (!*P.gate || level > P.top || (push && P.chan == 4)) return;
-    if (push ? level >= P.lo : level > P.lo) return;  // a coarse level: k_fill_coarse
+// The separable pass of a fill level as LDS tiles: the pull blur5 of level `level - 1`'s values
+// into t2 (push false, R = 2), or the push Gaussian (r = 5, R = 5) of level `level`'s solution
+// into t1 (push true); the border rows above or below first as the plane says. Levels above the
+// plane's `lo` run in k_fill_coarse.
+template <int R>
+__global__ __launch_bounds__(TILE_THREADS) void k_fill_tile(const FillPlane* __restrict__ planes, int level, int push, Taps t,
+                                                            int tiles_x, int tiles_y, int np) {
+    const int per = tiles_x * tiles_y;
+    const int lt = xcd_tile(per * np);
+    if (lt >= per * np) return;
+    const int q = lt / per, rem = lt - q * per;
+    const int tyi = rem / tiles_x, txi = rem - tyi * tiles_x;
+    const FillPlane& P = planes[q];
+    if (!*P.gate || level > P.top || level > P.lo || (push && P.chan == 4)) return;
     const int l = push ? level : level - 1;
-    const int W = P.w[l], H = P.h[l];
-    const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
-    if (x >= W || y >= H) return;
+    const int w = P.w[l], h = P.h[l];
+    const int x0 = txi * TILE_X, y0 = tyi * TILE_Y;
+    if (x0 >= w || y0 >= h) return;
+    __shared__ float s_in[(TILE_Y + 2 * R) * (TILE_X + 2 * R)];
     const float* src = push ? P.sol[l] : P.val[l];
-    P.t1[(long)y * W + x] = hpass_elem<1>(src + (long)y * W, W, x, 0, t);
+    float* dst = push ? P.t1 : P.t2;
+    float o[TILE_RUN];
+    int x, ys;
+    const bool any = P.above_first ? sep_tile<R, true>(src, w, h, x0, y0, t, s_in, o, x, ys)
+                                   : sep_tile<R, false>(src, w, h, x0, y0, t, s_in, o, x, ys);
+    if (!any) return;
+#pragma unroll
+    for (int m = 0; m < TILE_RUN; m++) {
+        if (ys + m >= h) break;
+        dst[(long)(ys + m) * w + x] = o[m];
+    }
 }
 
-// The vertical pass of the same, t1 into t2 (pull) or into sol[level] (push); the border rows
-// take the row above or below first as the plane says.
-__global__ void k_fill_vpass(const FillPlane* __restrict__ planes, int level, int push, Taps t) {
-    const FillPlane& P = planes[blockIdx.z];
-    if (!*P.gate || level > P.top || (push && P.chan == 4)) return;
-    if (push ? level >= P.lo : level > P.lo) return;  // a coarse level: k_fill_coarse
-    const int l = push ? level : level - 1;
-    const int W = P.w[l], H = P.h[l];
-    const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
-    if (x >= W || y >= H) return;
-    float v = P.above_first ? vpass_elem<1, true>(P.t1, W, H, x, y, 0, t) : vpass_elem<1, false>(P.t1, W, H, x, y, 0, t);
-    (push ? P.sol[l] : P.t2)[(long)y * W + x] = v;
-}
-
-// Pull: t2 (level - 1) resampled into val[level]. Push: sol[level] resampled into level - 1,
-// then the combine of that level (sol = value / weight where the weight is not 0).
+// Pull: t2 (level - 1 blurred) resampled into val[level]. Push: t1 (level's solution after its
+// Gaussian) resampled into level - 1, then the combine of that level (sol = value / weight where
+// the weight is not 0).
 __global__ void k_fill_resample(const FillPlane* __restrict__ planes, int level, int push) {
     const FillPlane& P = planes[blockIdx.z];
     if (!*P.gate || level > P.top || level < 1 || (push && P.chan == 4)) return;
@@ -2336,7 +2357,7 @@ __global__ void k_fill_resample(const FillPlane* __restrict__ planes, int level,
     const float sx = ((float)sw - 1.0f) / (float)(dw - 1);
     const float sy = ((float)sh - 1.0f) / (float)(dh - 1);
     const long i = (long)y * dw + x;
-    float v = bilinear1(push ? P.sol[sl] : P.t2, sw, sh, (float)x * sx, (float)y * sy);
+    float v = bilinear1(push ? P.t1 : P.t2, sw, sh, (float)x * sx, (float)y * sy);
     if (!push) { P.val[dl][i] = v; return; }
     const float wt = P.wts[dl][i];
     if (wt != 0.0f) v = P.val[dl][i] / wt;
@@ -2358,30 +2379,32 @@ __global__ void k_fill_push_init(const FillPlane* __restrict__ planes) {
 // The coarse levels lo .. top of a colour plane in one workgroup, every plane in LDS: the pull of
 // the values and of the frame's weights from level lo up (blur5, resample), the top level's
 // solution, and the push down to level lo (Gaussian r = 5, resample, combine), whose solution
-// goes back to memory; the same operations, in the same order, as the per-level kernels.
+// (before its Gaussian) goes back to memory; the same operations, in the same order, as the
+// per-level kernels.
 __global__ __launch_bounds__(1024) void k_fill_coarse(const FillPlane* __restrict__ planes, Taps t5, Taps tg) {
     const FillPlane& P = planes[blockIdx.x];
     if (!*P.gate || P.chan == 4 || P.lo > P.top) return;  // the same for the whole workgroup
     __shared__ float lds[FILL_LDS_FLOATS];
     const int lo = P.lo, top = P.top, tid = threadIdx.x, nt = blockDim.x;
-    float* val[MAXFL];
-    float* wts[MAXFL];
-    float* sol[MAXFL];
-    float* q = lds;
-    for (int l = lo; l <= top; l++) {
-        const int a = P.w[l] * P.h[l];
-        val[l] = q;
-        wts[l] = q + a;
-        sol[l] = q + 2 * a;
-        q += 3 * a;
-    }
-    float* t1 = q;
-    float* t2 = q + P.w[lo] * P.h[lo];
+    // level l's value, weight and solution planes follow each other from lds + base(l); the
+    // offsets are computed, not kept in runtime-indexed arrays (those would go to scratch memory)
+    auto base = [&](int l) {
+        int b = 0;
+        for (int k = lo; k < l; k++) b += 3 * P.w[k] * P.h[k];
+        return b;
+    };
+    auto val = [&](int l) { return lds + base(l); };
+    auto wts = [&](int l) { return lds + base(l) + P.w[l] * P.h[l]; };
+    auto sol = [&](int l) { return lds + base(l) + 2 * P.w[l] * P.h[l]; };
+    float* t1 = lds + base(top + 1);
+    float* t2 = t1 + P.w[lo] * P.h[lo];
     {
         const int a = P.w[lo] * P.h[lo];
+        float* v = val(lo);
+        float* wt = wts(lo);
         for (int i = tid; i < a; i += nt) {
-            val[lo][i] = P.val[lo][i];
-            wts[lo][i] = P.wts[lo][i];
+            v[i] = P.val[lo][i];
+            wt[i] = P.wts[lo][i];
         }
     }
     __syncthreads();
@@ -2391,8 +2414,8 @@ __global__ __launch_bounds__(1024) void k_fill_coarse(const FillPlane* __restric
         const float sx = ((float)W - 1.0f) / (float)(dw - 1);
         const float sy = ((float)H - 1.0f) / (float)(dh - 1);
         for (int pl = 0; pl < 2; pl++) {
-            const float* src = pl ? wts[k - 1] : val[k - 1];
-            float* dst = pl ? wts[k] : val[k];
+            const float* src = pl ? wts(k - 1) : val(k - 1);
+            float* dst = pl ? wts(k) : val(k);
             for (int i = tid; i < W * H; i += nt) {
                 const int y = i / W, x = i - y * W;
                 t1[i] = hpass_elem<1>(src + y * W, W, x, 0, t5);
@@ -2413,37 +2436,45 @@ __global__ __launch_bounds__(1024) void k_fill_coarse(const FillPlane* __restric
     // the top level's solution: the combine over zeros
     {
         const int a = P.w[top] * P.h[top];
-        for (int i = tid; i < a; i += nt) sol[top][i] = wts[top][i] != 0.0f ? val[top][i] / wts[top][i] : 0.0f;
+        const float* v = val(top);
+        const float* wt = wts(top);
+        float* so = sol(top);
+        for (int i = tid; i < a; i += nt) so[i] = wt[i] != 0.0f ? v[i] / wt[i] : 0.0f;
     }
     __syncthreads();
-    // push: the Gaussian of level l, then its resample into level l - 1 with that level's combine
-    for (int l = top; l >= lo; l--) {
+    // push: the Gaussian of level l, then its resample into level l - 1 with that level's
+    // combine, down to level lo (whose Gaussian k_fill_tile runs)
+    for (int l = top; l > lo; l--) {
         const int W = P.w[l], H = P.h[l];
+        float* so = sol(l);
         for (int i = tid; i < W * H; i += nt) {
             const int y = i / W, x = i - y * W;
-            t1[i] = hpass_elem<1>(sol[l] + y * W, W, x, 0, tg);
+            t1[i] = hpass_elem<1>(so + y * W, W, x, 0, tg);
         }
         __syncthreads();
         for (int i = tid; i < W * H; i += nt) {
             const int y = i / W, x = i - y * W;
-            sol[l][i] = P.above_first ? vpass_elem<1, true>(t1, W, H, x, y, 0, tg) : vpass_elem<1, false>(t1, W, H, x, y, 0, tg);
+            so[i] = P.above_first ? vpass_elem<1, true>(t1, W, H, x, y, 0, tg) : vpass_elem<1, false>(t1, W, H, x, y, 0, tg);
         }
         __syncthreads();
-        if (l == lo) break;
         const int dw = P.w[l - 1], dh = P.h[l - 1];
         const float sx = ((float)W - 1.0f) / (float)(dw - 1);
         const float sy = ((float)H - 1.0f) / (float)(dh - 1);
+        const float* dv = val(l - 1);
+        const float* dwt = wts(l - 1);
+        float* dso = sol(l - 1);
         for (int i = tid; i < dw * dh; i += nt) {
             const int y = i / dw, x = i - y * dw;
-            float v = bilinear1(sol[l], W, H, (float)x * sx, (float)y * sy);
-            const float wt = wts[l - 1][i];
-            if (wt != 0.0f) v = val[l - 1][i] / wt;
-            sol[l - 1][i] = v;
+            float v = bilinear1(so, W, H, (float)x * sx, (float)y * sy);
+            const float wt = dwt[i];
+            if (wt != 0.0f) v = dv[i] / wt;
+            dso[i] = v;
         }
         __syncthreads();
     }
     const int a = P.w[lo] * P.h[lo];
-    for (int i = tid; i < a; i += nt) P.sol[lo][i] = sol[lo][i];
+    const float* so = sol(lo);
+    for (int i = tid; i < a; i += nt) P.sol[lo][i] = so[i];
 }
 
 // The last combine of level 0 and the write back (holefill.rs:330-333, 366-386).
@@ -2455,7 +2486,7 @@ __global__ void k_fill_store(const FillPlane* __restrict__ planes) {
     if (x >= W || y >= H) return;
     const long i = (long)y * W + x;
     const float wt = P.wts[0][i];
-    const float v = wt != 0.0f ? P.val[0][i] / wt : P.sol[0][i];
+    const float v = wt != 0.0f ? P.val[0][i] / wt : P.t1[i];  // t1: level 0's solution after its Gaussian
     if (P.fmt == FILL_FLOW2) {
         reinterpret_cast<float*>(P.img)[i * 2 + P.chan] = v;
         return;
@@ -2510,18 +2541,26 @@ static void launch_fill_range(hipStream_t s, const FillPlane* d_planes, const st
             init = true;
         }
     }
+    // the tiles of level l over the planes, in XCD order
+    auto tiles = [&](int l, int& tx, int& ty) {
+        tx = (mw[l] + TILE_X - 1) / TILE_X;
+        ty = (mh[l] + TILE_Y - 1) / TILE_Y;
+        const long n = (long)tx * ty * np;
+        return dim3((unsigned)(8 * ((n + 7) / 8)));
+    };
+    int tx, ty;
     TWX_LAUNCH(s, F_FILL_IO, 8.0 * px[0], k_fill_load<<<grid(0), blk, 0, s>>>(d_planes));
     for (int k = 1; k <= hi; k++) {
-        TWX_LAUNCH(s, F_FILL_PULL, 8.0 * px[k - 1], k_fill_hpass<<<grid(k - 1), blk, 0, s>>>(d_planes, k, 0, t5));
-        TWX_LAUNCH(s, F_FILL_PULL, 8.0 * px[k - 1], k_fill_vpass<<<grid(k - 1), blk, 0, s>>>(d_planes, k, 0, t5));
+        const dim3 tg = tiles(k - 1, tx, ty);
+        TWX_LAUNCH(s, F_FILL_PULL, 8.0 * px[k - 1], k_fill_tile<2><<<tg, TILE_THREADS, 0, s>>>(d_planes, k, 0, t5, tx, ty, (int)np));
         TWX_LAUNCH(s, F_FILL_PULL, 4.0 * (px[k - 1] + px[k]), k_fill_resample<<<grid(k), blk, 0, s>>>(d_planes, k, 0));
     }
     // the coarse kernel reads level lo's values and weights and writes its solution
     if (coarse) TWX_LAUNCH(s, F_FILL_PUSH, 12.0 * lo_px, k_fill_coarse<<<dim3(np), dim3(1024), 0, s>>>(d_planes, t5, tg));
     if (init) TWX_LAUNCH(s, F_FILL_PUSH, 12.0 * px[maxtop], k_fill_push_init<<<grid(maxtop), blk, 0, s>>>(d_planes));
     for (int l = hi; l >= 0; l--) {
-        TWX_LAUNCH(s, F_FILL_PUSH, 8.0 * px[l], k_fill_hpass<<<grid(l), blk, 0, s>>>(d_planes, l, 1, tg));
-        TWX_LAUNCH(s, F_FILL_PUSH, 8.0 * px[l], k_fill_vpass<<<grid(l), blk, 0, s>>>(d_planes, l, 1, tg));
+        const dim3 gt = tiles(l, tx, ty);
+        TWX_LAUNCH(s, F_FILL_PUSH, 8.0 * px[l], k_fill_tile<5><<<gt, TILE_THREADS, 0, s>>>(d_planes, l, 1, tg, tx, ty, (int)np));
         if (l > 0)
             TWX_LAUNCH(s, F_FILL_PUSH, 4.0 * px[l] + 16.0 * px[l - 1],
                        k_fill_resample<<<grid(l - 1), blk, 0, s>>>(d_planes, l, 1));
