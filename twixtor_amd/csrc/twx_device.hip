@@ -2288,26 +2288,48 @@ static void launch_weighted_range(hipStream_t s, const FrameJob* d_jobs, const s
 // a plane's `lo` up run in k_fill_coarse, in LDS; the per-level kernels handle the levels below.
 // Each kernel returns at once for a frame without holes (the fill is the identity then).
 
-// Load (holefill.rs:314-321): the channel in native units, holes zeroed; weights 0 at holes, else 1
+// Load (holefill.rs:314-321): the channels in native units, holes zeroed; weights 0 at holes,
+// else 1. One thread per pixel of an image writes level 0 of all its planes (the colour planes
+// follow each other from the image's channel 0 plane, then its weight plane); the blocks of the
+// other planes return at once.
 __global__ void k_fill_load(const FillPlane* __restrict__ planes) {
     const FillPlane& P = planes[blockIdx.z];
-    if (!*P.gate) return;
+    if (!*P.gate || P.chan != 0) return;
     const int W = P.w[0], H = P.h[0];
     const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
     if (x >= W || y >= H) return;
     const long i = (long)y * W + x;
     const bool hole = P.holes[i] != 0;  // !(eps > m) for a u8 mask
-    float v;
-    if (P.chan == 4) {
-        v = hole ? 0.0f : 1.0f;
+    const int nc = P.fmt == FILL_FLOW2 ? 2 : 4;
+    float v[4];
+    if (P.fmt == FILL_FLOW2) {
+        const float2 f = reinterpret_cast<const float2*>(P.img)[i];
+        v[0] = f.x;
+        v[1] = f.y;
+        v[2] = v[3] = 0.0f;
+    } else if (P.fmt == TWX_FMT_U8) {
+        const uchar4 q = reinterpret_cast<const uchar4*>(P.img)[i];
+        v[0] = (float)q.x;
+        v[1] = (float)q.y;
+        v[2] = (float)q.z;
+        v[3] = (float)q.w;
+    } else if (P.fmt == TWX_FMT_U16) {
+        const ushort4 q = reinterpret_cast<const ushort4*>(P.img)[i];
+        v[0] = (float)q.x;
+        v[1] = (float)q.y;
+        v[2] = (float)q.z;
+        v[3] = (float)q.w;
     } else {
-        if (P.fmt == FILL_FLOW2) v = reinterpret_cast<const float*>(P.img)[i * 2 + P.chan];
-        else if (P.fmt == TWX_FMT_U8) v = (float)reinterpret_cast<const uint8_t*>(P.img)[i * 4 + P.chan];
-        else if (P.fmt == TWX_FMT_U16) v = (float)reinterpret_cast<const uint16_t*>(P.img)[i * 4 + P.chan];
-        else v = reinterpret_cast<const float*>(P.img)[i * 4 + P.chan];
-        if (hole) v = 0.0f;
+        const float4 q = reinterpret_cast<const float4*>(P.img)[i];
+        v[0] = q.x;
+        v[1] = q.y;
+        v[2] = q.z;
+        v[3] = q.w;
     }
-    P.val[0][i] = v;
+#pragma unroll
+    for (int c = 0; c < 4; c++)
+        if (c < nc) planes[blockIdx.z + c].val[0][i] = hole ? 0.0f : v[c];
+    planes[blockIdx.z + nc].val[0][i] = hole ? 0.0f : 1.0f;
 }
 
 // The separable pass of a fill level as LDS tiles: the pull blur5 of level `level - 1`'s values
@@ -2477,29 +2499,48 @@ __global__ __launch_bounds__(1024) void k_fill_coarse(const FillPlane* __restric
     for (int i = tid; i < a; i += nt) P.sol[lo][i] = so[i];
 }
 
-// The last combine of level 0 and the write back (holefill.rs:330-333, 366-386).
+// The last combine of level 0 and the write back (holefill.rs:330-333, 366-386), one thread per
+// pixel of an image for all its colour planes (the whole pixel is written); the blocks of the
+// other planes return at once.
 __global__ void k_fill_store(const FillPlane* __restrict__ planes) {
     const FillPlane& P = planes[blockIdx.z];
-    if (!*P.gate || P.chan == 4) return;
+    if (!*P.gate || P.chan != 0) return;
     const int W = P.w[0], H = P.h[0];
     const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
     if (x >= W || y >= H) return;
     const long i = (long)y * W + x;
-    const float wt = P.wts[0][i];
-    const float v = wt != 0.0f ? P.val[0][i] / wt : P.t1[i];  // t1: level 0's solution after its Gaussian
+    const int nc = P.fmt == FILL_FLOW2 ? 2 : 4;
+    const float wt = P.wts[0][i];  // the image's weight plane, shared by its colour planes
+    float v[4];
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+        if (c >= nc) {
+            v[c] = 0.0f;
+            continue;
+        }
+        const FillPlane& Q = planes[blockIdx.z + c];
+        v[c] = wt != 0.0f ? Q.val[0][i] / wt : Q.t1[i];  // t1: level 0's solution after its Gaussian
+    }
     if (P.fmt == FILL_FLOW2) {
-        reinterpret_cast<float*>(P.img)[i * 2 + P.chan] = v;
+        reinterpret_cast<float2*>(P.img)[i] = make_float2(v[0], v[1]);
         return;
     }
     if (P.fmt == TWX_FMT_F32) {
-        reinterpret_cast<float*>(P.img)[i * 4 + P.chan] = v;
+        reinterpret_cast<float4*>(P.img)[i] = make_float4(v[0], v[1], v[2], v[3]);
         return;
     }
     const float white = P.fmt == TWX_FMT_U8 ? 255.0f : 65535.0f;
-    const float yv = v + 0.5f;
-    const float o = 0.0f > yv ? 0.0f : (yv > white ? white : (float)cvtt64f(yv));
-    if (P.fmt == TWX_FMT_U8) reinterpret_cast<uint8_t*>(P.img)[i * 4 + P.chan] = (uint8_t)o;
-    else reinterpret_cast<uint16_t*>(P.img)[i * 4 + P.chan] = (uint16_t)o;
+    const int64_t wi = P.fmt == TWX_FMT_U8 ? 255 : 65535;
+    uint32_t o[4];
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+        // clamped to [0, white]; a NaN sample truncates to INT64_MIN, whose low bits are 0
+        const float yv = v[c] + 0.5f;
+        const int64_t q = 0.0f > yv ? 0 : (yv > white ? wi : cvtt64f(yv));
+        o[c] = (uint32_t)(q & wi);
+    }
+    if (P.fmt == TWX_FMT_U8) reinterpret_cast<uchar4*>(P.img)[i] = make_uchar4(o[0], o[1], o[2], o[3]);
+    else reinterpret_cast<ushort4*>(P.img)[i] = make_ushort4(o[0], o[1], o[2], o[3]);
 }
 
 static void launch_fill_range(hipStream_t s, const FillPlane* d_planes, const std::vector<FillPlane>& planes);
