@@ -151,7 +151,7 @@ def test_small_clip_matches_oracle(backend):
     assert got == want
 
 
-@pytest.mark.parametrize("w,h", [(2, 2), (3, 7), (9, 4), (16, 16), (37, 29), (65, 17), (130, 70)])
+@pytest.mark.parametrize("w,h", [(2, 2), (3, 7), (9, 4), (16, 16), (37, 29), (65, 17), (130, 70), (260, 150)])
 @pytest.mark.parametrize("fmt", [0, 1, 2])
 def test_smart_blend_compositions_bit_exact(backend, w, h, fmt):
     """Warping code 1 (Smart Blend and the hole fill) for Nearest, Blend and Weighted Blend, both

@@ -656,13 +656,18 @@ int fill_items(hipStream_t s, DevPool& pool, const std::vector<FillItem>& items,
                 P.t1 = take((size_t)it.W * it.H);
                 P.t2 = take((size_t)it.W * it.H);
             }
-            // the coarse levels that fit in one workgroup's LDS (from level 1 on)
+            // the coarse levels that fit in one workgroup's LDS (from level 1 on; the 2x2 top level
+            // always does)
             int lo = top + 1;
             for (int l = top; l >= 1; l--) {
                 size_t pyr_l = 0;
                 for (int k = l; k <= top; k++) pyr_l += (size_t)dims[k].first * dims[k].second;
                 if (3 * pyr_l + 2 * (size_t)dims[l].first * dims[l].second > (size_t)FILL_LDS_FLOATS) break;
                 lo = l;
+            }
+            if (lo > top) {
+                err = "hole fill: the top pyramid level does not fit in LDS";
+                return TWX_E_DEVICE;
             }
             for (int q = 0; q < nplanes; q++) {
                 fp[q].lo = lo;
@@ -1291,6 +1296,10 @@ static int forward_warp_dev(twx_handle* h, DevPool& pool, const void* src, int W
         HIPCHECK(hipMemcpyAsync(out, src, image_bytes(W, H, fmt), hipMemcpyDeviceToDevice, s));
         if (cov) HIPCHECK(hipMemsetAsync(cov, 0xff, (size_t)W * H, s));
         return TWX_OK;
+    }
+    if ((long)W * H > (long)UINT32_MAX) {  // the rasteriser's vertices carry 32-bit pixel indices
+        err = "forward warp: an image of more than 2^32 - 1 pixels";
+        return TWX_E_UNSUPPORTED;
     }
     FwArgs A{};
     A.src = src;

@@ -133,7 +133,7 @@ struct FillPlane {
     const uint8_t* holes;
     int fmt;
     int above_first;
-    int lo;                   // levels lo .. top run in one workgroup in LDS (none when lo > top)
+    int lo;                   // levels lo .. top (1 <= lo <= top) run in one workgroup in LDS
 };
 
 // LDS floats of the fill's coarse levels: the value, weight and solution pyramids of levels

@@ -2345,7 +2345,7 @@ __global__ __launch_bounds__(TILE_THREADS) void k_fill_tile(const FillPlane* __r
     const int q = lt / per, rem = lt - q * per;
     const int tyi = rem / tiles_x, txi = rem - tyi * tiles_x;
     const FillPlane& P = planes[q];
-    if (!*P.gate || level > P.top || level > P.lo || (push && P.chan == 4)) return;
+    if (!*P.gate || level > P.lo || (push && P.chan == 4)) return;
     const int l = push ? level : level - 1;
     const int w = P.w[l], h = P.h[l];
     const int x0 = txi * TILE_X, y0 = tyi * TILE_Y;
@@ -2386,26 +2386,14 @@ __global__ void k_fill_resample(const FillPlane* __restrict__ planes, int level,
     P.sol[dl][i] = v;
 }
 
-// The solution of the top level: the combine over zeros (holefill.rs:322-326).
-__global__ void k_fill_push_init(const FillPlane* __restrict__ planes) {
-    const FillPlane& P = planes[blockIdx.z];
-    if (!*P.gate || P.chan == 4 || P.lo <= P.top) return;
-    const int l = P.top, W = P.w[l], H = P.h[l];
-    const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
-    if (x >= W || y >= H) return;
-    const long i = (long)y * W + x;
-    const float wt = P.wts[l][i];
-    P.sol[l][i] = wt != 0.0f ? P.val[l][i] / wt : 0.0f;
-}
-
 // The coarse levels lo .. top of a colour plane in one workgroup, every plane in LDS: the pull of
 // the values and of the frame's weights from level lo up (blur5, resample), the top level's
-// solution, and the push down to level lo (Gaussian r = 5, resample, combine), whose solution
+// solution (the combine over zeros, holefill.rs:322-326), and the push down to level lo (Gaussian r = 5, resample, combine), whose solution
 // (before its Gaussian) goes back to memory; the same operations, in the same order, as the
 // per-level kernels.
 __global__ __launch_bounds__(1024) void k_fill_coarse(const FillPlane* __restrict__ planes, Taps t5, Taps tg) {
     const FillPlane& P = planes[blockIdx.x];
-    if (!*P.gate || P.chan == 4 || P.lo > P.top) return;  // the same for the whole workgroup
+    if (!*P.gate || P.chan == 4) return;  // the same for the whole workgroup
     __shared__ float lds[FILL_LDS_FLOATS];
     const int lo = P.lo, top = P.top, tid = threadIdx.x, nt = blockDim.x;
     // level l's value, weight and solution planes follow each other from lds + base(l); the
@@ -2569,18 +2557,13 @@ static void launch_fill_range(hipStream_t s, const FillPlane* d_planes, const st
     auto grid = [&](int l) { return dim3((mw[l] + 255) / 256, mh[l], np); };
     const dim3 blk(256);
     const Taps t5 = binomial_taps(), tg = gaussian_taps(5.0f);
-    // levels above `hi` run only inside k_fill_coarse; the per-level kernels stop there
+    // levels above `hi` run only inside k_fill_coarse (every plane has 1 <= lo <= top: its 2x2
+    // top level always fits in LDS); the per-level kernels stop there
     int hi = 0;
-    bool coarse = false, init = false;
     double lo_px = 0.0;
     for (auto& p : planes) {
-        hi = std::max(hi, std::min(p.top, p.lo));
-        if (p.lo <= p.top) {
-            coarse = true;
-            if (p.chan != 4) lo_px += (double)p.w[p.lo] * p.h[p.lo];
-        } else {
-            init = true;
-        }
+        hi = std::max(hi, p.lo);
+        if (p.chan != 4) lo_px += (double)p.w[p.lo] * p.h[p.lo];
     }
     // the tiles of level l over the planes, in XCD order
     auto tiles = [&](int l, int& tx, int& ty) {
@@ -2597,8 +2580,7 @@ static void launch_fill_range(hipStream_t s, const FillPlane* d_planes, const st
         TWX_LAUNCH(s, F_FILL_PULL, 4.0 * (px[k - 1] + px[k]), k_fill_resample<<<grid(k), blk, 0, s>>>(d_planes, k, 0));
     }
     // the coarse kernel reads level lo's values and weights and writes its solution
-    if (coarse) TWX_LAUNCH(s, F_FILL_PUSH, 12.0 * lo_px, k_fill_coarse<<<dim3(np), dim3(1024), 0, s>>>(d_planes, t5, tg));
-    if (init) TWX_LAUNCH(s, F_FILL_PUSH, 12.0 * px[maxtop], k_fill_push_init<<<grid(maxtop), blk, 0, s>>>(d_planes));
+    TWX_LAUNCH(s, F_FILL_PUSH, 12.0 * lo_px, k_fill_coarse<<<dim3(np), dim3(1024), 0, s>>>(d_planes, t5, tg));
     for (int l = hi; l >= 0; l--) {
         const dim3 gt = tiles(l, tx, ty);
         TWX_LAUNCH(s, F_FILL_PUSH, 8.0 * px[l], k_fill_tile<5><<<gt, TILE_THREADS, 0, s>>>(d_planes, l, 1, tg, tx, ty, (int)np));

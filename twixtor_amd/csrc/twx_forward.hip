@@ -38,9 +38,12 @@ __device__ __forceinline__ int64_t fw_cvtt64(float x) {
     return (int64_t)x;
 }
 
+// A vertex: its displaced position and its pixel, whose colour is read where an edge or the
+// last-row pass needs it (keeping the colours of all three vertices live took registers enough
+// to halve the emit pass's occupancy).
 struct FwV {
     float x, y;
-    float p[4];
+    uint32_t idx;
 };
 
 struct FwE {
@@ -51,8 +54,9 @@ struct FwE {
 // raster.rs:59-86: activates the edge a -> b for a.y <= y < b.y, state set for scanline y - 1,
 // into e1 when an edge is active already, else into e0. The edge pair lives in two named
 // structs (no runtime-indexed private arrays, which would go to scratch memory).
-template <bool PAY>
-__device__ __forceinline__ void fw_edge_init(const FwV& a, const FwV& b, float y, FwE& e0, FwE& e1, uint32_t& count) {
+template <bool PAY, class Tex>
+__device__ __forceinline__ void fw_edge_init(const FwV& a, const FwV& b, float y, FwE& e0, FwE& e1, uint32_t& count,
+                                             const Tex& tex) {
     if (a.y <= y && y < b.y) {
         const bool second = count != 0;
         count++;
@@ -65,10 +69,13 @@ __device__ __forceinline__ void fw_edge_init(const FwV& a, const FwV& b, float y
         E.x = dx * t + a.x;
         E.dx = dx * inv;
         if (PAY) {
+            float pa[4], pb[4];
+            tex(a.idx, pa);
+            tex(b.idx, pb);
 #pragma unroll
             for (int k = 0; k < 4; k++) {
-                const float dk = b.p[k] - a.p[k];
-                E.p[k] = dk * t + a.p[k];
+                const float dk = pb[k] - pa[k];
+                E.p[k] = dk * t + pa[k];
                 E.dp[k] = dk * inv;
             }
         }
@@ -82,8 +89,7 @@ __device__ __forceinline__ FwV fw_pick(const FwV& v0, const FwV& v1, const FwV& 
     FwV o;
     o.x = a == 0 ? v0.x : (a == 1 ? v1.x : v2.x);
     o.y = a == 0 ? v0.y : (a == 1 ? v1.y : v2.y);
-#pragma unroll
-    for (int k = 0; k < 4; k++) o.p[k] = a == 0 ? v0.p[k] : (a == 1 ? v1.p[k] : v2.p[k]);
+    o.idx = a == 0 ? v0.idx : (a == 1 ? v1.idx : v2.idx);
     return o;
 }
 
@@ -105,8 +111,8 @@ __device__ __forceinline__ void fw_sort_by_y(const FwV v[3], FwV o[3]) {
 
 // raster.rs:153-214: on reaching vertex K, the edge pair pruned of the edges that ended, then
 // K's two edges (0: 1, 2), (1: 2, 0), (2: 0, 1) activated
-template <bool PAY, int K>
-__device__ __forceinline__ void fw_vertex_edges(const FwV vs[3], float y, FwE& e0, FwE& e1, uint32_t& count) {
+template <bool PAY, int K, class Tex>
+__device__ __forceinline__ void fw_vertex_edges(const FwV vs[3], float y, FwE& e0, FwE& e1, uint32_t& count, const Tex& tex) {
     if (count == 1) {
         if (e0.ay <= y && e0.by <= y) count = 0;
     } else if (count == 2) {
@@ -123,8 +129,8 @@ __device__ __forceinline__ void fw_vertex_edges(const FwV vs[3], float y, FwE& e
     }
     constexpr int N0 = K == 0 ? 1 : (K == 1 ? 2 : 0);
     constexpr int N1 = K == 0 ? 2 : (K == 1 ? 0 : 1);
-    fw_edge_init<PAY>(vs[K], vs[N0], y, e0, e1, count);
-    fw_edge_init<PAY>(vs[K], vs[N1], y, e0, e1, count);
+    fw_edge_init<PAY>(vs[K], vs[N0], y, e0, e1, count, tex);
+    fw_edge_init<PAY>(vs[K], vs[N1], y, e0, e1, count, tex);
 }
 
 // raster.rs:114-151: one span of the main pass. `hit(x, y, p)` returns false to stop the triangle.
@@ -168,8 +174,8 @@ __device__ __forceinline__ bool fw_span(const FwE& l, const FwE& r, float y, int
 
 // raster.rs:153-300. Returns 0, or 1 when the last row pass would write outside the image, or 2
 // when `hit` stopped the triangle.
-template <bool PAY, class Hit>
-__device__ int fw_triangle(const FwV tri[3], int32_t w1, int32_t h1, Hit& hit) {
+template <bool PAY, class Hit, class Tex>
+__device__ int fw_triangle(const FwV tri[3], int32_t w1, int32_t h1, Hit& hit, const Tex& tex) {
     const float w1f = (float)w1, h1f = (float)h1;
     const bool ok = (0.0f <= tri[0].x || 0.0f <= tri[1].x || 0.0f <= tri[2].x) &&
                     (0.0f <= tri[0].y || 0.0f <= tri[1].y || 0.0f <= tri[2].y) &&
@@ -191,15 +197,15 @@ __device__ int fw_triangle(const FwV tri[3], int32_t w1, int32_t h1, Hit& hit) {
         e1 = e0;
         for (;;) {
             if (vi == 0 && !(y < vs[0].y)) {
-                fw_vertex_edges<PAY, 0>(vs, y, e0, e1, count);
+                fw_vertex_edges<PAY, 0>(vs, y, e0, e1, count, tex);
                 vi = 1;
             }
             if (vi == 1 && !(y < vs[1].y)) {
-                fw_vertex_edges<PAY, 1>(vs, y, e0, e1, count);
+                fw_vertex_edges<PAY, 1>(vs, y, e0, e1, count, tex);
                 vi = 2;
             }
             if (vi == 2 && !(y < vs[2].y)) {
-                fw_vertex_edges<PAY, 2>(vs, y, e0, e1, count);
+                fw_vertex_edges<PAY, 2>(vs, y, e0, e1, count, tex);
                 vi = 3;
             }
             e0.x += e0.dx;
@@ -245,11 +251,16 @@ __device__ int fw_triangle(const FwV tri[3], int32_t w1, int32_t h1, Hit& hit) {
                 const float inv = 1.0f / wid;
                 const int32_t yi = fw_cvtt(yt);
                 float p[4] = {0.0f, 0.0f, 0.0f, 0.0f}, dp[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+                float lp[4], rp[4];
+                if (PAY) {
+                    tex(l.idx, lp);
+                    tex(r.idx, rp);
+                }
                 const float tt = ((float)x0 - l.x) * inv;
                 if (x0 == x1) {
                     if (PAY) {
 #pragma unroll
-                        for (int k = 0; k < 4; k++) p[k] = (r.p[k] - l.p[k]) * tt + l.p[k];
+                        for (int k = 0; k < 4; k++) p[k] = (rp[k] - lp[k]) * tt + lp[k];
                     }
                     if (x0 < 0 || x0 > w1 || yi < 0 || yi > h1) status = 1;
                     else if (!hit(x0, yi, p)) return 2;
@@ -257,8 +268,8 @@ __device__ int fw_triangle(const FwV tri[3], int32_t w1, int32_t h1, Hit& hit) {
                     if (PAY) {
 #pragma unroll
                         for (int k = 0; k < 4; k++) {
-                            dp[k] = r.p[k] - l.p[k];
-                            p[k] = dp[k] * tt + l.p[k];
+                            dp[k] = rp[k] - lp[k];
+                            p[k] = dp[k] * tt + lp[k];
                         }
                     }
                     int32_t x = x0;
@@ -293,26 +304,17 @@ __device__ __forceinline__ void fw_texel(const void* img, long i, float o[4]) {
 }
 
 // The vertices of triangle t: cell (r, c) = t / 2, T1 = (v00, v01, v10), T2 = (v11, v01, v10).
-template <int FMT, bool PAY>
-__device__ __forceinline__ void fw_vertices(const FwArgs& A, uint32_t t, FwV tri[3], bool& nan_colour) {
+__device__ __forceinline__ void fw_vertices(const FwArgs& A, uint32_t t, FwV tri[3]) {
     const uint32_t cell = t >> 1, W1 = (uint32_t)A.W - 1;
     const uint32_t r = cell / W1, c = cell - r * W1;
-    const long i00 = (long)r * A.W + c, i01 = i00 + 1, i10 = i00 + A.W, i11 = i10 + 1;
-    const long idx[3] = {(t & 1) ? i11 : i00, i01, i10};
-    nan_colour = false;
+    const uint32_t i00 = r * (uint32_t)A.W + c, i01 = i00 + 1, i10 = i00 + (uint32_t)A.W, i11 = i10 + 1;
+    const uint32_t idx[3] = {(t & 1) ? i11 : i00, i01, i10};
 #pragma unroll
     for (int q = 0; q < 3; q++) {
         const float2 p = A.pos[idx[q]];
         tri[q].x = p.x;
         tri[q].y = p.y;
-        if (PAY) {
-            fw_texel<FMT>(A.src, idx[q], tri[q].p);
-#pragma unroll
-            for (int k = 0; k < 4; k++) nan_colour |= isnan(tri[q].p[k]);
-        } else {
-#pragma unroll
-            for (int k = 0; k < 4; k++) tri[q].p[k] = 0.0f;
-        }
+        tri[q].idx = idx[q];
     }
 }
 
@@ -361,15 +363,15 @@ __global__ void k_fw_count(FwArgs A) {
     const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= A.ntri) return;
     FwV tri[3];
-    bool nan_colour;
-    fw_vertices<0, false>(A, (uint32_t)t, tri, nan_colour);
+    fw_vertices(A, (uint32_t)t, tri);
     uint32_t n = 0;
     auto hit = [&](int32_t x, int32_t y, const float*) -> bool {
         if (++n > FW_TRI_CAP) return false;
         atomicAdd(&A.counts[(long)y * A.W + x], 1u);
         return true;
     };
-    const int st = fw_triangle<false>(tri, A.W - 1, A.H - 1, hit);
+    auto tex = [](uint32_t, float*) {};
+    const int st = fw_triangle<false>(tri, A.W - 1, A.H - 1, hit, tex);
     if (st == 1) atomicOr(A.flags, 1);
     if (st == 2) atomicOr(A.flags, 2);
 }
@@ -379,8 +381,8 @@ __global__ void k_fw_emit(FwArgs A) {
     const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= A.ntri) return;
     FwV tri[3];
-    bool nan_colour;
-    fw_vertices<FMT, true>(A, (uint32_t)t, tri, nan_colour);
+    fw_vertices(A, (uint32_t)t, tri);
+    auto tex = [&](uint32_t i, float o[4]) { fw_texel<FMT>(A.src, i, o); };
     auto hit = [&](int32_t x, int32_t y, const float* p) -> bool {
         const long i = (long)y * A.W + x;
         const uint32_t k = atomicAdd(&A.fill[i], 1u);
@@ -393,7 +395,7 @@ __global__ void k_fw_emit(FwArgs A) {
         A.rec_c[slot] = make_float4(p[0], p[1], p[2], p[3]);
         return true;
     };
-    fw_triangle<true>(tri, A.W - 1, A.H - 1, hit);
+    fw_triangle<true>(tri, A.W - 1, A.H - 1, hit, tex);
 }
 
 // forward.rs:104-128
