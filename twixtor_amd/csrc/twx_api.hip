@@ -309,12 +309,12 @@ void build_pyramids(hipStream_t s, DevPool& pool, const std::vector<PyrIn>& in) 
     if (in.empty()) return;
     const Taps bin = binomial_taps();
     const size_t chunk = 64;
-    // per frame scratch: two level buffers A (W*H) and B (half size), the blurred plane T (W*H);
-    // one set for the largest chunk, reused by every chunk in stream order
+    // per frame scratch: the planes of steps 1, 3, ... in B (half size) and of steps 2, 4, ... in A
+    // (quarter size); one set for the largest chunk, reused by every chunk in stream order (step 0
+    // reads the engine image, and the blurred planes stay in LDS)
     auto r64 = [](size_t v) { return (v + 63) & ~(size_t)63; };
-    auto frame_scratch = [&](const PyrIn& p) {
-        return r64((size_t)p.W * p.H) * 2 + r64((size_t)(p.W / 2 + 1) * (p.H / 2 + 1));
-    };
+    auto quarter = [&](const PyrIn& p) { return r64((size_t)(p.W / 4 + 1) * (p.H / 4 + 1)); };
+    auto frame_scratch = [&](const PyrIn& p) { return quarter(p) + r64((size_t)(p.W / 2 + 1) * (p.H / 2 + 1)); };
     size_t max_floats = 0;
     for (size_t c0 = 0; c0 < in.size(); c0 += chunk) {
         size_t f = 0;
@@ -325,15 +325,13 @@ void build_pyramids(hipStream_t s, DevPool& pool, const std::vector<PyrIn>& in) 
     for (size_t c0 = 0; c0 < in.size(); c0 += chunk) {
         size_t c1 = std::min(in.size(), c0 + chunk);
         int n = (int)(c1 - c0);
-        std::vector<float*> bufA(n), bufB(n), blur(n);
+        std::vector<float*> bufA(n), bufB(n);
         int maxR = 0, maxL = 0;
         float* sp = sbase;
         for (int i = 0; i < n; i++) {
             const PyrIn& p = in[c0 + i];
             bufA[i] = sp;
-            sp += r64((size_t)p.W * p.H);
-            blur[i] = sp;
-            sp += r64((size_t)p.W * p.H);
+            sp += quarter(p);
             bufB[i] = sp;
             sp += r64((size_t)(p.W / 2 + 1) * (p.H / 2 + 1));
             maxR = std::max(maxR, p.geo.reduction);
@@ -341,7 +339,7 @@ void build_pyramids(hipStream_t s, DevPool& pool, const std::vector<PyrIn>& in) 
         }
         // every step is one launch over the chunk; a frame without the step gets an empty op
         std::vector<std::vector<PlaneOp>> steps;
-        std::vector<int> kinds;  // 0 luma, 1 blur5 (+ band), 3 resample
+        std::vector<int> kinds;  // 2: blur5 (+ band) with the resample of the blurred plane
         auto level_ptr = [&](int i, int l) {
             const PyrIn& p = in[c0 + i];
             float* base = p.levels;
@@ -351,52 +349,53 @@ void build_pyramids(hipStream_t s, DevPool& pool, const std::vector<PyrIn>& in) 
         // the buffer holding the plane of step k (reductions, then levels): A, B, A, ...
         auto cur = [&](int i, int k) { return (k % 2 == 0) ? bufA[i] : bufB[i]; };
         const PlaneOp empty{};
-        {
-            std::vector<PlaneOp> ops(n, empty);
-            for (int i = 0; i < n; i++) {
-                const PyrIn& p = in[c0 + i];
-                ops[i] = PlaneOp{p.img, bufA[i], p.W, p.H, p.W, p.H, p.fmt, 0, nullptr};
+        // a frame's first step reads its engine image and takes the luma there (prep.rs:168-235)
+        auto source = [&](int i, int k, PlaneOp& op) {
+            if (k == 0) {
+                op.src = in[c0 + i].img;
+                op.fmt = in[c0 + i].fmt;
+                op.luma = 1;
             }
-            steps.push_back(ops);
-            kinds.push_back(0);
-        }
+        };
         // reduction: R x (blur5, resample to the halved size)
         for (int r = 0; r < maxR; r++) {
-            std::vector<PlaneOp> bl(n, empty), rs(n, empty);
+            std::vector<PlaneOp> op(n, empty);
             for (int i = 0; i < n; i++) {
                 const PyrIn& p = in[c0 + i];
                 if (r >= p.geo.reduction) continue;
                 int cw = p.W, ch = p.H;
                 for (int k = 0; k < r; k++) { cw /= 2; ch /= 2; }
-                bl[i] = PlaneOp{cur(i, r), blur[i], cw, ch, cw, ch, 0, 0, nullptr};
-                rs[i] = PlaneOp{blur[i], cur(i, r + 1), cw, ch, cw / 2, ch / 2, 0, 0, nullptr};
+                op[i] = PlaneOp{cur(i, r), cur(i, r + 1), cw, ch, cw / 2, ch / 2, 0, 0, nullptr};
+                source(i, r, op[i]);
             }
-            steps.push_back(bl); kinds.push_back(1);
-            steps.push_back(rs); kinds.push_back(3);
+            steps.push_back(op); kinds.push_back(2);
         }
         // levels: blur = blur5(orig), band = orig - blur into the pyramid, next = resample(blur)
+        // (none after the last level)
         for (int l = 0; l < maxL; l++) {
-            std::vector<PlaneOp> bl(n, empty), rs(n, empty);
+            std::vector<PlaneOp> op(n, empty);
             for (int i = 0; i < n; i++) {
                 const PyrIn& p = in[c0 + i];
                 if (l >= p.geo.levels) continue;
                 const int k = p.geo.reduction + l;
                 int w = p.geo.w[l], h = p.geo.h[l];
-                bl[i] = PlaneOp{cur(i, k), blur[i], w, h, w, h, 0, 0, level_ptr(i, l)};
-                if (l + 1 < p.geo.levels) rs[i] = PlaneOp{blur[i], cur(i, k + 1), w, h, p.geo.w[l + 1], p.geo.h[l + 1], 0, 0, nullptr};
+                const bool next = l + 1 < p.geo.levels;
+                op[i] = PlaneOp{cur(i, k), next ? cur(i, k + 1) : nullptr, w, h, next ? p.geo.w[l + 1] : 0,
+                                next ? p.geo.h[l + 1] : 0, 0, 0, level_ptr(i, l)};
+                source(i, k, op[i]);
             }
-            steps.push_back(bl); kinds.push_back(1);
-            steps.push_back(rs); kinds.push_back(3);
+            steps.push_back(op); kinds.push_back(2);
         }
-        // the resample ratios (every target is 2 x 2 or larger: reductions stop at 2 pixels and
-        // pyramid levels are 3 or more)
+        // the resample ratios: every target is 2 x 2 or larger and halves its source (reductions
+        // stop at 2 pixels, pyramid levels are 3 or more), so both ratios are at least 2, which
+        // the fused kernel's tiling relies on
         for (size_t k = 0; k < steps.size(); k++)
-            if (kinds[k] == 3)
-                for (auto& op : steps[k])
-                    if (op.src) {
-                        op.rx = ((float)op.sw - 1.0f) / (float)(op.dw - 1);
-                        op.ry = ((float)op.sh - 1.0f) / (float)(op.dh - 1);
-                    }
+            for (auto& op : steps[k])
+                if (op.src && op.dst && kinds[k] == 2) {
+                    op.rx = ((float)op.sw - 1.0f) / (float)(op.dw - 1);
+                    op.ry = ((float)op.sh - 1.0f) / (float)(op.dh - 1);
+                    if (!(op.rx >= 2.0f && op.ry >= 2.0f)) throw HipError{"pyramid: a level that does not halve"};
+                }
         std::vector<PlaneOp> flat;
         for (auto& st : steps) flat.insert(flat.end(), st.begin(), st.end());
         PlaneOp* d = pool.upload(flat);
@@ -409,9 +408,7 @@ void build_pyramids(hipStream_t s, DevPool& pool, const std::vector<PyrIn>& in) 
             }
             if (mw == 0 || mh == 0) continue;
             switch (kinds[k]) {
-                case 0: launch_luma(s, dk, steps[k], mw, mh); break;
-                case 1: launch_blur5_tile(s, dk, steps[k], mw, mh, bin); break;
-                case 3: launch_resample(s, dk, steps[k], mw, mh); break;
+                case 2: launch_blur5_resample(s, dk, steps[k], mw, mh, bin); break;
             }
         }
     }

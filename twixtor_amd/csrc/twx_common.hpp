@@ -25,10 +25,10 @@ struct PlaneOp {
     const void* src;
     float* dst;
     int sw, sh, dw, dh;
-    int fmt;  // luma: the engine image format
-    int pad;
-    float* aux;  // k_blur5_tile: where the band src - blur goes (null: none)
-    float rx, ry;  // k_resample: (sw - 1) / (dw - 1) and (sh - 1) / (dh - 1), computed on the host
+    int fmt;   // with luma: the engine image format
+    int luma;  // k_blur5_resample: src is an engine image whose luma is the plane
+    float* aux;  // k_blur5_resample: where the band src - blur goes (null: none)
+    float rx, ry;  // k_blur5_resample: (sw - 1) / (dw - 1) and (sh - 1) / (dh - 1), computed on the host
 };
 
 // One flow solve X -> Y over a pyramid pair (flow.rs:1258-1307). Device resident table entry;
@@ -143,7 +143,7 @@ constexpr int FILL_LDS_FLOATS = 32768;
 // Per-kernel-family timing with HIP events on the launching stream, and the algorithmic bytes of
 // each launch (the step-boundary model: distinct inputs read once, outputs written once).
 enum ProfFam {
-    F_INGEST, F_LUMA, F_PYR_BLUR, F_PYR_RESAMPLE, F_GRAD_TENSOR, F_BM_TRANSPOSE,
+    F_INGEST, F_PYR_BLUR, F_GRAD_TENSOR, F_BM_TRANSPOSE,
     F_SOLVE_INIT, F_CALC_B, F_NEW_DISP, F_BM_ROWS, F_BM_COLS, F_RANGE, F_UPSAMPLE, F_NEGATE, F_FRAMES,
     F_WB_MASK, F_WB_SHARPEN, F_FILL_PULL, F_FILL_PUSH, F_FILL_IO, F_DIFF_FWD, F_DIFF_BWD, F_DIFF_GATHER,
     F_FW_MESH, F_FW_RASTER, F_FW_SCAN, F_FW_FOLD, F_VEC, F_NFAM
@@ -197,9 +197,8 @@ std::vector<std::pair<int, int>> size_buckets(const std::vector<double>& areas);
 // largest first.
 void launch_ingest(hipStream_t s, const void* wire, void* engine, int W, int H, int fmt, bool reverse_rows);
 void launch_ingest_batch(hipStream_t s, const IngestOp* d_ops, const std::vector<IngestOp>& ops);
-void launch_luma(hipStream_t s, const PlaneOp* d_ops, const std::vector<PlaneOp>& ops, int maxw, int maxh);
-void launch_blur5_tile(hipStream_t s, const PlaneOp* d_ops, const std::vector<PlaneOp>& ops, int maxw, int maxh, const Taps& t);
-void launch_resample(hipStream_t s, const PlaneOp* d_ops, const std::vector<PlaneOp>& ops, int maxw, int maxh);
+void launch_blur5_resample(hipStream_t s, const PlaneOp* d_ops, const std::vector<PlaneOp>& ops, int maxw, int maxh,
+                           const Taps& t);
 void launch_solve(hipStream_t s, const SolveJob* d_jobs, const std::vector<SolveJob>& jobs, bool skip_refine);
 void launch_frames(hipStream_t s, const FrameJob* d_jobs, const std::vector<FrameJob>& jobs);
 // The Weighted Blend jobs (kinds 3 and 6) after launch_frames: mask clean-up, sharpening, then

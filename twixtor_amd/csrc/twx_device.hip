@@ -184,7 +184,7 @@ void launch_ingest(hipStream_t s, const void* wire, void* engine, int W, int H, 
 
 // ---- profiler (host) ----
 
-const char* const PROF_NAMES[F_NFAM] = {"ingest", "luma", "pyr_blur5_band", "pyr_resample",
+const char* const PROF_NAMES[F_NFAM] = {"ingest", "pyr_luma_blur5_band_resample",
                                         "tensor_gauss", "bm_transpose", "solve_init", "calc_b_gauss", "new_disp_gauss",
                                         "bm_rows", "bm_cols", "range", "upsample", "negate", "frames",
                                         "wb_mask", "wb_sharpen_blend", "fill_pull", "fill_push", "fill_io",
@@ -240,42 +240,23 @@ void prof_end(hipStream_t s, int fam, double bytes, hipEvent_t a) {
     g_prof->pending.push_back(Prof::Rec{fam, a, b, bytes});
 }
 
-static double ops_px(const std::vector<PlaneOp>& ops, bool dst) {
-    double n = 0;
-    for (auto& o : ops) n += dst ? (double)o.dw * o.dh : (double)o.sw * o.sh;
-    return n;
-}
-
 // -------------------------------------------------------------------------------------------
 // Luma (prep.rs:168-235, retime path: no alpha, no video range)
 
-__global__ void k_luma(const PlaneOp* __restrict__ ops) {
-    const PlaneOp op = ops[blockIdx.z];
-    int x = blockIdx.x * blockDim.x + threadIdx.x;
-    int y = blockIdx.y;
-    if (x >= op.sw || y >= op.sh) return;
-    long i = (long)y * op.sw + x;
-    float v;
-    if (op.fmt == TWX_FMT_U8) {
-        uchar4 p = reinterpret_cast<const uchar4*>(op.src)[i];
+__device__ __forceinline__ float luma_at(const void* img, int fmt, long i) {
+    if (fmt == TWX_FMT_U8) {
+        uchar4 p = reinterpret_cast<const uchar4*>(img)[i];
         float r = (float)p.y, g = (float)p.z, b = (float)p.w;
-        v = (r * 0.3f + g * 0.59f) + b * 0.11f;
-    } else if (op.fmt == TWX_FMT_U16) {
-        ushort4 p = reinterpret_cast<const ushort4*>(op.src)[i];
-        double r = p.y, g = p.z, b = p.w;
-        v = (float)(((g * 0.59 + r * 0.3) + b * 0.11) / 257.0);
-    } else {
-        float4 p = reinterpret_cast<const float4*>(op.src)[i];
-        double r = p.y, g = p.z, b = p.w;
-        v = (float)(((g * 0.59 + r * 0.3) + b * 0.11) * 255.0);
+        return (r * 0.3f + g * 0.59f) + b * 0.11f;
     }
-    op.dst[i] = v;
-}
-
-void launch_luma(hipStream_t s, const PlaneOp* d_ops, const std::vector<PlaneOp>& ops, int maxw, int maxh) {
-    double bytes = 0;
-    for (auto& o : ops) bytes += (double)o.sw * o.sh * (4.0 * sample_size(o.fmt) + 4.0);
-    TWX_LAUNCH(s, F_LUMA, bytes, k_luma<<<dim3((maxw + 255) / 256, maxh, (unsigned)ops.size()), dim3(256), 0, s>>>(d_ops));
+    if (fmt == TWX_FMT_U16) {
+        ushort4 p = reinterpret_cast<const ushort4*>(img)[i];
+        double r = p.y, g = p.z, b = p.w;
+        return (float)(((g * 0.59 + r * 0.3) + b * 0.11) / 257.0);
+    }
+    float4 p = reinterpret_cast<const float4*>(img)[i];
+    double r = p.y, g = p.z, b = p.w;
+    return (float)(((g * 0.59 + r * 0.3) + b * 0.11) * 255.0);
 }
 
 // -------------------------------------------------------------------------------------------
@@ -382,21 +363,6 @@ __device__ __forceinline__ float bilinear1(const float* __restrict__ img, int w,
     float top = (p10 - p00) * fx + p00;
     float bot = (p11 - p01) * fx + p01;
     return (bot - top) * fy + top;
-}
-
-__global__ void k_resample(const PlaneOp* __restrict__ ops) {
-    const PlaneOp op = ops[blockIdx.z];
-    int x = blockIdx.x * blockDim.x + threadIdx.x;
-    int y = blockIdx.y;
-    if (x >= op.dw || y >= op.dh) return;
-    const float sx = op.rx, sy = op.ry;
-    float fy = (float)y * sy;
-    op.dst[(long)y * op.dw + x] = bilinear1((const float*)op.src, op.sw, op.sh, (float)x * sx, fy);
-}
-
-void launch_resample(hipStream_t s, const PlaneOp* d_ops, const std::vector<PlaneOp>& ops, int maxw, int maxh) {
-    TWX_LAUNCH(s, F_PYR_RESAMPLE, 4.0 * ops_px(ops, false) + 4.0 * ops_px(ops, true),
-               k_resample<<<dim3((maxw + 255) / 256, maxh, (unsigned)ops.size()), dim3(256), 0, s>>>(d_ops));
 }
 
 // -------------------------------------------------------------------------------------------
@@ -737,14 +703,14 @@ __global__ __launch_bounds__(TILE_X * TY / TILE_RUN) void k_tile_gauss(const Sol
 
 // The separable pass of one TILE_X x TILE_Y tile of a one-channel plane (conv.rs:144-199,
 // 272-311): the tile and R pixels around it into LDS (s_in, (TILE_Y + 2R) x (TILE_X + 2R)
-// floats), the row pass with the border columns right first, then the column pass with the border
+// floats; load(i) gives the plane's element i), the row pass with the border columns right first, then the column pass with the border
 // rows above first (ABOVE) or below first. Lane (column, run of TILE_RUN rows) receives its run's
 // outputs in o (rows inside the plane only) at column x from row ys; the return is false for a
 // lane with no output. Every element is computed as by hpass_elem and vpass_elem, so the bits are
 // the same. Every lane of the block calls it (it holds barriers).
-template <int R, bool ABOVE>
-__device__ __forceinline__ bool sep_tile(const float* __restrict__ src, int w, int h, int x0, int y0, const Taps& t,
-                                         float* s_in, float o[TILE_RUN], int& x, int& ys) {
+template <int R, bool ABOVE, class Load>
+__device__ __forceinline__ bool sep_tile(const Load& load, int w, int h, int x0, int y0, const Taps& t, float* s_in,
+                                         float o[TILE_RUN], int& x, int& ys) {
     constexpr int RW = TILE_X + 2 * R, RH = TILE_Y + 2 * R;
     constexpr int RUNS_X = TILE_X / TILE_RUN, HITEMS = RH * RUNS_X;
     constexpr int HPER = (HITEMS + TILE_THREADS - 1) / TILE_THREADS;
@@ -753,7 +719,7 @@ __device__ __forceinline__ bool sep_tile(const float* __restrict__ src, int w, i
         const int ry = k / RW, rx = k - ry * RW;
         const int xx = x0 - R + rx, yy = y0 - R + ry;
         if (xx < 0 || xx >= w || yy < 0 || yy >= h) continue;
-        s_in[k] = src[(long)yy * w + xx];
+        s_in[k] = load((long)yy * w + xx);
     }
     __syncthreads();
     const int nh = min(R + 1, w);
@@ -811,12 +777,20 @@ __device__ __forceinline__ bool sep_tile(const float* __restrict__ src, int w, i
     return true;
 }
 
-// blur5 of a plane (conv.rs:598-629: the row pass, then the column pass with the border rows
-// above first; flow.rs:242-275 and prep.rs:252-259 run it on every pyramid level and reduction
-// step) as one LDS-tiled kernel over a batch of planes: op.src in, the blurred plane to op.dst,
-// and with op.aux set the band op.aux = src - blur (flow.rs:228-231).
-__global__ __launch_bounds__(TILE_THREADS) void k_blur5_tile(const PlaneOp* __restrict__ ops, Taps t, int tiles_x,
-                                                             int tiles_y, int nops) {
+// blur5 with its band and the resample to the next level in one pass (conv.rs:598-629,
+// flow.rs:228-275; the resample is the bilinear of flow/sample.rs:31-112): tiles at a pitch of
+// PYR_PX x PYR_PY blur a TILE_X x TILE_Y region, so the 2 x 2 footprint of every next-level
+// sample whose cell lies in the tile's pitch is inside the region, and the blurred plane stays in
+// LDS. op.aux (the band; may be null) takes src - blur over the tile's pitch, op.dst (the next
+// level, op.dw x op.dh; null for the last level) the samples whose cells the tile holds. The blur
+// is computed as by the separate row and column passes (sep_tile) and each sample as by
+// bilinear1, so the bits are the same. The ratios op.rx, op.ry are at least 2 (every level
+// halves), which bounds a tile's samples to PYR_CX x PYR_CY candidates from
+// (floor((x0 - 2) / rx), floor((y0 - 2) / ry)).
+constexpr int PYR_PX = TILE_X - 1, PYR_PY = TILE_Y - 1, PYR_CX = 36, PYR_CY = 20;
+
+__global__ __launch_bounds__(TILE_THREADS) void k_blur5_resample(const PlaneOp* __restrict__ ops, Taps t, int tiles_x,
+                                                                 int tiles_y, int nops) {
     constexpr int R = 2;
     const int per_op = tiles_x * tiles_y;
     const int lt = xcd_tile(per_op * nops);
@@ -825,29 +799,61 @@ __global__ __launch_bounds__(TILE_THREADS) void k_blur5_tile(const PlaneOp* __re
     const int tyi = rem / tiles_x, txi = rem - tyi * tiles_x;
     const PlaneOp& op = ops[job];
     const int w = op.sw, h = op.sh;
-    const int x0 = txi * TILE_X, y0 = tyi * TILE_Y;
+    const int x0 = txi * PYR_PX, y0 = tyi * PYR_PY;
     if (x0 >= w || y0 >= h) return;
     __shared__ float s_in[(TILE_Y + 2 * R) * (TILE_X + 2 * R)];
+    __shared__ float s_blur[TILE_Y * TILE_X];
+    // the plane: a float plane, or (op.luma) the luma of an engine image of format op.fmt
     const float* __restrict__ src = (const float*)op.src;
+    auto plain = [&](long i) { return src[i]; };
+    auto luma = [&](long i) { return luma_at(op.src, op.fmt, i); };
     float o[TILE_RUN];
     int x, ys;
-    if (!sep_tile<R, true>(src, w, h, x0, y0, t, s_in, o, x, ys)) return;
+    const bool any = op.luma ? sep_tile<R, true>(luma, w, h, x0, y0, t, s_in, o, x, ys)
+                             : sep_tile<R, true>(plain, w, h, x0, y0, t, s_in, o, x, ys);
+    if (any) {
 #pragma unroll
-    for (int m = 0; m < TILE_RUN; m++) {
-        if (ys + m >= h) break;
-        const long i = (long)(ys + m) * w + x;
-        op.dst[i] = o[m];
-        if (op.aux) op.aux[i] = src[i] - o[m];
+        for (int m = 0; m < TILE_RUN; m++) {
+            if (ys + m >= h) break;
+            s_blur[(ys + m - y0) * TILE_X + (x - x0)] = o[m];
+            if (op.aux && x < x0 + PYR_PX && ys + m < y0 + PYR_PY) {
+                const long i = (long)(ys + m) * w + x;
+                op.aux[i] = (op.luma ? luma(i) : src[i]) - o[m];
+            }
+        }
+    }
+    if (!op.dst) return;  // the same for the whole block
+    __syncthreads();
+    const float sx = op.rx, sy = op.ry;
+    const int xg = max(0, (int)floorf((float)(x0 - 2) / sx));
+    const int yg = max(0, (int)floorf((float)(y0 - 2) / sy));
+    for (int c = threadIdx.x; c < PYR_CX * PYR_CY; c += TILE_THREADS) {
+        const int oy = yg + c / PYR_CX, ox = xg + c % PYR_CX;
+        if (ox >= op.dw || oy >= op.dh) continue;
+        int ix, iy;
+        float fx, fy;
+        cell((float)ox * sx, w, ix, fx);
+        cell((float)oy * sy, h, iy, fy);
+        if (ix < x0 || ix >= x0 + PYR_PX || iy < y0 || iy >= y0 + PYR_PY) continue;
+        const float* bp = s_blur + (iy - y0) * TILE_X + (ix - x0);
+        const float p00 = bp[0], p10 = bp[1], p01 = bp[TILE_X], p11 = bp[TILE_X + 1];
+        const float top = (p10 - p00) * fx + p00;
+        const float bot = (p11 - p01) * fx + p01;
+        op.dst[(long)oy * op.dw + ox] = (bot - top) * fy + top;
     }
 }
 
-void launch_blur5_tile(hipStream_t s, const PlaneOp* d_ops, const std::vector<PlaneOp>& ops, int maxw, int maxh, const Taps& t) {
-    const int tx = (maxw + TILE_X - 1) / TILE_X, ty = (maxh + TILE_Y - 1) / TILE_Y;
+void launch_blur5_resample(hipStream_t s, const PlaneOp* d_ops, const std::vector<PlaneOp>& ops, int maxw, int maxh,
+                           const Taps& t) {
+    const int tx = (maxw + PYR_PX - 1) / PYR_PX, ty = (maxh + PYR_PY - 1) / PYR_PY;
     const long n = (long)tx * ty * (long)ops.size();
     double bytes = 0;
-    for (auto& o : ops) bytes += (double)o.sw * o.sh * (o.aux ? 12.0 : 8.0);
+    for (auto& o : ops)
+        bytes += (double)o.sw * o.sh * ((o.luma ? 4.0 * sample_size(o.fmt) : 4.0) + (o.aux ? 4.0 : 0.0)) +
+                 (o.dst ? 4.0 * o.dw * o.dh : 0.0);
     TWX_LAUNCH(s, F_PYR_BLUR, bytes,
-               k_blur5_tile<<<dim3((unsigned)(8 * ((n + 7) / 8))), dim3(TILE_THREADS), 0, s>>>(d_ops, t, tx, ty, (int)ops.size()));
+               k_blur5_resample<<<dim3((unsigned)(8 * ((n + 7) / 8))), dim3(TILE_THREADS), 0, s>>>(d_ops, t, tx, ty,
+                                                                                                   (int)ops.size()));
 }
 
 #define JOB_PROLOGUE()                                                    \
@@ -2355,8 +2361,9 @@ __global__ __launch_bounds__(TILE_THREADS) void k_fill_tile(const FillPlane* __r
     float* dst = push ? P.t1 : P.t2;
     float o[TILE_RUN];
     int x, ys;
-    const bool any = P.above_first ? sep_tile<R, true>(src, w, h, x0, y0, t, s_in, o, x, ys)
-                                   : sep_tile<R, false>(src, w, h, x0, y0, t, s_in, o, x, ys);
+    auto load = [&](long i) { return src[i]; };
+    const bool any = P.above_first ? sep_tile<R, true>(load, w, h, x0, y0, t, s_in, o, x, ys)
+                                   : sep_tile<R, false>(load, w, h, x0, y0, t, s_in, o, x, ys);
     if (!any) return;
 #pragma unroll
     for (int m = 0; m < TILE_RUN; m++) {
