@@ -917,7 +917,7 @@ __global__ void k_upsample(const SolveJob* __restrict__ jobs, int level) {
     const int x = blockIdx.x * blockDim.x + threadIdx.x;
     const int y = blockIdx.y;
     if (x >= dw || y >= dh) return;
-    const float* src = J.flow[1];
+    const gfloat* src = (const gfloat*)J.flow[1];
     float* dst = J.flow[0];
     const float sx = J.up[level][0], sy = J.up[level][1], rx = J.up[level][2], ry = J.up[level][3];
     float fx = (float)x * rx, fy = (float)y * ry;
@@ -925,19 +925,21 @@ __global__ void k_upsample(const SolveJob* __restrict__ jobs, int level) {
     float ax, ay;
     cell(fx, sw, ix, ax);
     cell(fy, sh, iy, ay);
-    long i = ((long)iy * sw + ix) * 2;
-    long r = (long)sw * 2;
+    // the cell's two flow pixels of its top row, then of its bottom row (a level's flow is under
+    // 2^32 bytes: working sizes are at most 16384 x 16384)
+    const uint32_t q = (uint32_t)iy * (uint32_t)sw + (uint32_t)ix;
+    float t0[4], t1[4];
+    ldrun<4>(src, 2 * q, t0);
+    ldrun<4>(src, 2 * (q + (uint32_t)sw), t1);
     float out[2];
 #pragma unroll
     for (int c = 0; c < 2; c++) {
-        float p00 = src[i + c], p10 = src[i + 2 + c], p01 = src[i + r + c], p11 = src[i + r + 2 + c];
+        float p00 = t0[c], p10 = t0[2 + c], p01 = t1[c], p11 = t1[2 + c];
         float top = (p10 - p00) * ax + p00;
         float bot = (p11 - p01) * ax + p01;
         out[c] = (bot - top) * ay + top;
     }
-    long o = ((long)y * dw + x) * 2;
-    dst[o] = out[0] * sx;
-    dst[o + 1] = out[1] * sy;
+    reinterpret_cast<float2*>(dst)[(long)y * dw + x] = make_float2(out[0] * sx, out[1] * sy);
 }
 
 void solve_job_ratios(SolveJob& J) {
@@ -1814,12 +1816,15 @@ __device__ __forceinline__ void finish_pixel(void* out, long n, long i, const fl
     }
 }
 
+// The output frames' blocks: FR_BX x FR_BY pixels (rows of FR_BX pixels, FR_BY of them).
+constexpr int FR_BX = 64, FR_BY = 4;
+
 template <int FMT>
 __global__ void k_frames(const FrameJob* __restrict__ jobs) {
     const FrameJob& F = jobs[blockIdx.z];
     if (F.fmt != FMT) return;
-    const int x = blockIdx.x * blockDim.x + threadIdx.x;
-    const int y = blockIdx.y;  // motion row
+    const int x = blockIdx.x * FR_BX + (int)threadIdx.x % FR_BX;
+    const int y = blockIdx.y * FR_BY + (int)threadIdx.x / FR_BX;  // motion row
     if (x >= F.W || y >= F.H) return;
     long n = (long)F.W * F.H;
     long mi = (long)y * F.W + x;
@@ -1968,7 +1973,7 @@ static void launch_frames_range(hipStream_t s, const FrameJob* d_jobs, const std
     int mw = 0, mh = 0;
     bool has[3] = {false, false, false};
     for (auto& f : jobs) { mw = std::max(mw, f.W); mh = std::max(mh, f.H); has[f.fmt] = true; }
-    dim3 blk(256), grd((mw + 255) / 256, mh, (unsigned)jobs.size());
+    dim3 blk(FR_BX * FR_BY), grd((mw + FR_BX - 1) / FR_BX, (mh + FR_BY - 1) / FR_BY, (unsigned)jobs.size());
     // per output pixel: the two source pixels and the output (Bpp each), plus the flows read at
     // working size (8 B per flow and working pixel)
     double bytes = 0;
