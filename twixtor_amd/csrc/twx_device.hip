@@ -1816,15 +1816,16 @@ __device__ __forceinline__ void finish_pixel(void* out, long n, long i, const fl
     }
 }
 
-// The output frames' blocks: FR_BX x FR_BY pixels (rows of FR_BX pixels, FR_BY of them).
-constexpr int FR_BX = 64, FR_BY = 4;
-
 template <int FMT>
 __global__ void k_frames(const FrameJob* __restrict__ jobs) {
+    // one block per 256 pixels of a row: the hardware deals consecutive blocks to the 8 XCDs in
+    // turn, so each XCD keeps a vertical strip of the frame and the source rows that vertically
+    // neighbouring pixels share stay in its L2 (HBM fetch 1.03x the algorithmic bytes; 64x4
+    // blocks ran 2.6 % faster at 1.45x, and 64x4 blocks in per-XCD bands 7 % slower at 0.93x)
     const FrameJob& F = jobs[blockIdx.z];
     if (F.fmt != FMT) return;
-    const int x = blockIdx.x * FR_BX + (int)threadIdx.x % FR_BX;
-    const int y = blockIdx.y * FR_BY + (int)threadIdx.x / FR_BX;  // motion row
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = blockIdx.y;  // motion row
     if (x >= F.W || y >= F.H) return;
     long n = (long)F.W * F.H;
     long mi = (long)y * F.W + x;
@@ -1973,7 +1974,7 @@ static void launch_frames_range(hipStream_t s, const FrameJob* d_jobs, const std
     int mw = 0, mh = 0;
     bool has[3] = {false, false, false};
     for (auto& f : jobs) { mw = std::max(mw, f.W); mh = std::max(mh, f.H); has[f.fmt] = true; }
-    dim3 blk(FR_BX * FR_BY), grd((mw + FR_BX - 1) / FR_BX, (mh + FR_BY - 1) / FR_BY, (unsigned)jobs.size());
+    dim3 blk(256), grd((mw + 255) / 256, mh, (unsigned)jobs.size());
     // per output pixel: the two source pixels and the output (Bpp each), plus the flows read at
     // working size (8 B per flow and working pixel)
     double bytes = 0;

@@ -207,9 +207,23 @@ __device__ __forceinline__ bool warp_one(const DiffArgs& P, const float* __restr
     return true;
 }
 
+// Pixel (x, y) of block b of a row-major grid of 256-pixel row segments, the blocks taken in XCD
+// order: the hardware deals block b to XCD b % 8, and segment (b % 8) * ceil(n / 8) + b / 8 gives
+// each XCD a contiguous band of rows, whose source rows its own L2 then serves to the rows below
+// (with 5 segments per 1280-pixel row the plain order spread each source row over every XCD).
+__device__ __forceinline__ bool wb_pixel(const DiffArgs& P, int& x, int& y) {
+    const int per_row = (P.W + 255) / 256, n = per_row * P.H;
+    const int per = (n + 7) / 8, b = blockIdx.x;
+    const int seg = (b & 7) * per + (b >> 3);
+    if (seg >= n) return false;
+    y = seg / per_row;
+    x = (seg - y * per_row) * 256 + threadIdx.x;
+    return x < P.W;
+}
+
 __global__ void k_wb_forward(DiffArgs P) {
-    const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
-    if (x >= P.W || y >= P.H) return;
+    int x, y;
+    if (!wb_pixel(P, x, y)) return;
     float ta[4], tb[4];
     bool ok = warp_one<false>(P, P.a, P.f_ba, P.frac, P.copy_a, x, y, ta, nullptr, nullptr, nullptr);
     ok = warp_one<false>(P, P.b, P.f_ab, P.wa, P.copy_b, x, y, tb, nullptr, nullptr, nullptr) && ok;
@@ -220,8 +234,8 @@ __global__ void k_wb_forward(DiffArgs P) {
 }
 
 __global__ void k_wb_backward(DiffArgs P) {
-    const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
-    if (x >= P.W || y >= P.H) return;
+    int x, y;
+    if (!wb_pixel(P, x, y)) return;
     const long pi = (long)y * P.W + x;
     float g[4], ga[4], gb[4], v[4];
     for (int c = 0; c < P.C; c++) {
@@ -331,7 +345,8 @@ int warp_blend_run(hipStream_t s, bool backward, int C, int W, int H, const floa
         err = "warp+blend: a blend weight 1 - wa outside [0, 1] (the extrapolating lerp) has no backward here";
         return TWX_E_UNSUPPORTED;
     } else P.mode = 3;
-    dim3 blk(256), grd((W + 255) / 256, H);
+    const long segs = (long)((W + 255) / 256) * H;
+    dim3 blk(256), grd((unsigned)(8 * ((segs + 7) / 8)));  // wb_pixel's XCD order
     const double img = 4.0 * C * (double)W * H, fl = 8.0 * (double)fw * fh;
     if (!backward) {
         TWX_LAUNCH(s, F_DIFF_FWD, 3.0 * img + 2.0 * fl, k_wb_forward<<<grd, blk, 0, s>>>(P));
