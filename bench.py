@@ -65,8 +65,8 @@ def main():
         print(json.dumps({"copy_probe_bytes": src.numel() * 4}))
         return
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU: this build has no multi-GPU path (asked for {args.gpus}); "
-              "a clip shards by output-frame range, see DESIGN.md", file=sys.stderr)
+        print(f"bench.py runs on one GPU: there is no multi-GPU path (asked for {args.gpus}); a clip would "
+              "shard by output-frame range, each GPU rendering its frames from the solves they need", file=sys.stderr)
         sys.exit(2)
 
     import numpy as np
@@ -318,8 +318,8 @@ def _families(prof, steps=1):
 
 
 def vectors_leg(be, frames, steps, warmup):
-    """Create Vectors and the Vectors In read-back at 3840x2160 RGBA f32 (BASELINE.json
-    configs[2]): a clip under a global affine motion with 10 discs; Create Vectors with Normal
+    """Create Vectors and the Vectors In read-back at 3840x2160 RGBA f32: a clip under a
+    global affine motion with 10 discs; Create Vectors with Normal
     (half res) vectors, track Previous, Max Displace 2048: output n = the encoded flow of frame n
     towards frame n - 1, for n = 1 .. frames - 1, one twx_create_vectors_dev call over the clip
     (device-resident). The read-back (Vectors In at 50 %, warping 2, Blend, one vector layer) of
