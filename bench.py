@@ -27,7 +27,7 @@ SPEED = "speed_percent=100@0;25@60;100@120"
 
 # Measured HBM bytes over algorithmic bytes per kernel family, from the FETCH_SIZE / WRITE_SIZE
 # counter runs (copy-calibrated) recorded in profiles/retime_1080p_hbm_traffic_pmc.md.
-TRAFFIC_RATIO = {"bm_rows": 1.42, "bm_cols": 1.28, "calc_b_gauss": 1.04, "new_disp_gauss": 1.07, "tensor_gauss": 1.02,
+TRAFFIC_RATIO = {"bm_rows": 1.42, "bm_cols": 1.28, "calc_b_gauss": 1.05, "new_disp_gauss": 1.05, "tensor_gauss": 1.02,
                  "frames": 1.03, "bm_transpose": 1.30}
 
 

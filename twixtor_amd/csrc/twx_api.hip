@@ -470,9 +470,9 @@ int enqueue_solves(hipStream_t s, DevPool& pool, const std::vector<SolveIn>& in,
             J.flow[0] = sp;      // where each level starts (the upsample target)
             J.flow[1] = si.out;  // where each level ends: level 0 ends in the output
             J.grad = sp + px * 2;
-            J.ta = sp + px * 4;
+            J.tmp = sp + px * 4;  // 8-byte aligned (flow pixels are moved as float2 through it)
             J.tb = sp + px * 7;
-            J.tmp = sp + px * 9;
+            J.ta = sp + px * 9;
             sp += px * 12;
             J.state = states + (ch.first + k) * 4;
             J.max_w = si.geo.w0;

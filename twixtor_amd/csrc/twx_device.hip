@@ -1069,15 +1069,16 @@ __global__ void k_bm_transpose(const SolveJob* __restrict__ jobs, int level, int
     if (bx >= w || by >= h) return;
     const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 256 threads: 32 x 8
     const long px = (long)w * h;
-    float* flow = J.flow[buf];
-    float* ft = J.tmp;
+    float2* flow = reinterpret_cast<float2*>(J.flow[buf]);  // a flow pixel (u, v) as one access
+    float2* ft = reinterpret_cast<float2*>(J.tmp);
     if (!OUT) {
         for (int r = ty; r < 32; r += 8) {
             int x = bx + tx, y = by + r;
             if (x < w && y < h) {
                 long i = (long)y * w + x;
-                t[0][r][tx] = flow[i * 2];
-                t[1][r][tx] = flow[i * 2 + 1];
+                const float2 f = flow[i];
+                t[0][r][tx] = f.x;
+                t[1][r][tx] = f.y;
                 t[2][r][tx] = J.X[level][i];
                 t[3][r][tx] = J.Y[level][i];
             }
@@ -1087,8 +1088,7 @@ __global__ void k_bm_transpose(const SolveJob* __restrict__ jobs, int level, int
             int x = bx + r, y = by + tx;
             if (x < w && y < h) {
                 long o = (long)x * h + y;
-                ft[o * 2] = t[0][tx][r];
-                ft[o * 2 + 1] = t[1][tx][r];
+                ft[o] = make_float2(t[0][tx][r], t[1][tx][r]);
                 J.tb[o] = t[2][tx][r];
                 J.tb[px + o] = t[3][tx][r];
             }
@@ -1098,8 +1098,9 @@ __global__ void k_bm_transpose(const SolveJob* __restrict__ jobs, int level, int
             int x = bx + r, y = by + tx;
             if (x < w && y < h) {
                 long o = (long)x * h + y;
-                t[0][tx][r] = ft[o * 2];
-                t[1][tx][r] = ft[o * 2 + 1];
+                const float2 f = ft[o];
+                t[0][tx][r] = f.x;
+                t[1][tx][r] = f.y;
             }
         }
         __syncthreads();
@@ -1107,8 +1108,7 @@ __global__ void k_bm_transpose(const SolveJob* __restrict__ jobs, int level, int
             int x = bx + tx, y = by + r;
             if (x < w && y < h) {
                 long i = (long)y * w + x;
-                flow[i * 2] = t[0][r][tx];
-                flow[i * 2 + 1] = t[1][r][tx];
+                flow[i] = make_float2(t[0][r][tx], t[1][r][tx]);
             }
         }
     }
