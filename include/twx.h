@@ -17,6 +17,13 @@
  *    device/cl.rs:20-24). One in-order HIP stream per handle; the caller serialises calls on a
  *    handle. A nested call from a stage callback is refused with TWX_E_UNSUPPORTED
  *    (exec/backend.rs:151-159).
+ *  - Ordering of DEVICE pointer arguments: the handle's stream is non-blocking. An entry point
+ *    that takes device buffers first makes its stream wait (on the device) for all work queued on
+ *    the legacy default stream before the call; work the caller queued on any other stream that
+ *    writes the inputs or touches the outputs must be complete before the call. When the call
+ *    returns, its outputs are complete.
+ *  - A frame of 2^28 pixels or more is refused with TWX_E_UNSUPPORTED (the kernels address
+ *    frames by 32-bit byte offsets, 16 bytes a pixel at most).
  *  - Status codes map onto StageError (device/image.rs:14-23): any non-zero status makes the
  *    engine rerun the stage on the CPU. `err`/`errlen` receive a message (may be NULL/0).
  */
@@ -260,6 +267,15 @@ typedef struct twx_stats {
     uint64_t graph_replays;  /* batched renders replayed from a captured graph */
 } twx_stats;
 int twx_get_stats(twx_handle* h, twx_stats* out);
+
+/* Backend::cache_stats (device/exec/backend.rs:42-45): the counters of the resident pyramid cache
+ * behind twx_solve and twx_solve_pair (hits, misses, evictions, device bytes held; any pointer may
+ * be NULL). A pyramid is found again only for an image with the same size, format and sample bits
+ * and the same working geometry (device/exec/cache.rs:1-27); least recently used entries go first,
+ * within a quarter of twx_memory_budget. pyramid_builds / pyramid_hits of twx_stats count the
+ * same misses and hits. */
+int twx_cache_stats(twx_handle* h, uint64_t* hits, uint64_t* misses, uint64_t* evictions,
+                    uint64_t* bytes_held);
 
 /* Graph capture of batched renders (on by default; no counterpart in the reference, whose render
  * loop launches per frame): a twx_retime_frames_dev or twx_retime_batch call runs on memory of its

@@ -119,3 +119,28 @@ def test_tile_shape_switch_checks_its_argument():
     for v in (5, 0, -1):
         assert L.twx_set_tile_shape(v) == _ffi.TWX_OK
     assert L.twx_tile_choices(None, 0) >= 0
+
+
+def test_frames_of_2_28_pixels_are_refused_on_the_host():
+    """The frame kernels address pixels by 32-bit byte offsets (16 bytes a pixel at most): sizes
+    of 2^28 pixels and more are refused with status 3 before any device work, never wrapped."""
+    from twixtor_amd.backend import flow_geometry, flow_settings
+
+    with pytest.raises(_ffi.StageError) as e:
+        flow_geometry(flow_settings(quality=5), 16384, 16384)
+    assert e.value.status == _ffi.TWX_E_UNSUPPORTED
+    assert flow_geometry(flow_settings(quality=4), 16384, 16383)[:2] == (8192, 8191)
+    cfg = Retime(20480, 16384, 32, "24", ["speed_percent=50"])
+    with pytest.raises(_ffi.StageError) as e:
+        twx_select(cfg.struct(), 1)
+    assert e.value.status == _ffi.TWX_E_UNSUPPORTED
+
+
+def test_host_buffers_are_checked_before_the_library_is_called():
+    from twixtor_amd.backend import _c_buffer
+
+    a = np.zeros((4, 6), np.float32, order="F")
+    out = _c_buffer(a, np.float32, 24, "x")
+    assert out.flags.c_contiguous and out.size == 24
+    with pytest.raises(ValueError):
+        _c_buffer(a, np.float32, 25, "x")

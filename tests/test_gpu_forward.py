@@ -103,8 +103,10 @@ def test_forward_warp_refusals(backend):
         backend.forward_warp(src, 20, 16, 0, flow, 20, 16, 0.5, pyramid=True)
     assert e.value.status == 3
     with pytest.raises(StageError) as e:
-        backend.forward_warp(src, 1, 16, 0, flow, 20, 16, 0.5)
+        backend.forward_warp(src[:1 * 16 * 4], 1, 16, 0, flow, 20, 16, 0.5)
     assert e.value.status == 4
+    with pytest.raises(ValueError):  # a buffer shorter than its stated size never reaches the library
+        backend.forward_warp(src[:-4], 20, 16, 0, flow, 20, 16, 0.5)
     # a NaN colour of a float image: the device refuses (the CPU's NaN rule is not served)
     f = engine(rng, 20, 16, 2)
     f[5 * 4 + 1] = np.nan

@@ -10,6 +10,7 @@ import numpy as np
 import pytest
 
 from oracle import cpuref
+from twixtor_amd._ffi import TWX_E_UNSUPPORTED, StageError
 from twixtor_amd.backend import flow_settings
 from twixtor_amd.params import Retime
 
@@ -59,7 +60,11 @@ def test_solve_pair_bit_exact(backend, w, h, fmt):
             try:
                 want_ab, want_ba, g = cpuref.solve_pair(a, b, w, h, fmt, ofs)
             except RuntimeError:
-                continue  # working size below 2x2: refused by both (checked in the ABI tests)
+                # working size below 2x2: the CPU path refuses, so must the device
+                with pytest.raises(StageError) as refused:
+                    backend.solve_pair(a, b, w, h, fmt, fs)
+                assert refused.value.status == TWX_E_UNSUPPORTED, (w, h, fmt, quality, skip)
+                continue
             got_ab, got_ba = backend.solve_pair(a, b, w, h, fmt, fs)
             np.testing.assert_array_equal(got_ba.view(np.uint32), want_ba.view(np.uint32),
                                           err_msg=f"F_BA {w}x{h} fmt {fmt} q {quality} skip {skip}")
@@ -241,3 +246,36 @@ def test_every_served_retime_time_hash_on_device(backend):
             bad.append(c["id"])
     assert not bad, bad
     assert refused == 6
+
+
+def test_pyramid_cache_serves_the_shared_frame(backend):
+    """A per-frame caller solving pairs (0, 1), (1, 2), (2, 3) of a clip, as the reference's
+    executor does: each pair's first frame was the previous pair's second, so its pyramid comes
+    from the resident cache (device/exec/backend.rs:191-208). Two of the three calls hit for that
+    frame; the flows equal the oracle's bit for bit, and a changed sample is a miss."""
+    rng = np.random.RandomState(5)
+    w, h, fmt = 96, 64, 2
+    frames = []
+    base = random_engine(rng, w + 16, h + 16, fmt).reshape(h + 16, w + 16, 4)
+    for k in range(4):
+        frames.append(np.ascontiguousarray(base[8 - k:8 - k + h, 8 + 2 * k:8 + 2 * k + w]).reshape(-1))
+    fs, ofs = flow_settings(), cpuref.flow_settings()
+    hits0, misses0, _, _ = backend.cache_stats()
+    for k in range(3):
+        got_ab, got_ba = backend.solve_pair(frames[k], frames[k + 1], w, h, fmt, fs)
+        want_ab, want_ba, _ = cpuref.solve_pair(frames[k], frames[k + 1], w, h, fmt, ofs)
+        np.testing.assert_array_equal(got_ab.view(np.uint32), want_ab.view(np.uint32), err_msg=f"F_AB pair {k}")
+        np.testing.assert_array_equal(got_ba.view(np.uint32), want_ba.view(np.uint32), err_msg=f"F_BA pair {k}")
+    hits, misses, _, held = backend.cache_stats()
+    assert hits - hits0 == 2 and misses - misses0 == 4, (hits - hits0, misses - misses0)
+    assert held > 0
+    # one sample's bits changed: the pyramid is built again, and the flow is that image's
+    changed = frames[3].copy()
+    changed[17] = np.float32(-0.0) if changed[17] == 0 else np.float32(0.0)
+    got = backend.solve(frames[2], changed, w, h, fmt, fs)
+    want, _ = cpuref.solve(frames[2], changed, w, h, fmt, ofs)
+    np.testing.assert_array_equal(got.view(np.uint32), want.view(np.uint32))
+    hits2, misses2, _, _ = backend.cache_stats()
+    assert (hits2 - hits, misses2 - misses) == (1, 1)
+    st = backend.stats()
+    assert st["pyramid_hits"] >= 3

@@ -20,7 +20,7 @@ EXPORTS = [
     "twx_retime_frames",
     "twx_forward_warp", "twx_encode_vectors", "twx_decode_vectors", "twx_flow_from_vectors", "twx_invert_flow",
     "twx_create_vectors_dev", "twx_warp_blend", "twx_warp_blend_backward",
-    "twx_set_graphs", "twx_set_tile_shape", "twx_tile_choices", "twx_get_stats", "twx_stream", "twx_profile_enable", "twx_profile_reset", "twx_profile_read",
+    "twx_set_graphs", "twx_set_tile_shape", "twx_tile_choices", "twx_get_stats", "twx_cache_stats", "twx_stream", "twx_profile_enable", "twx_profile_reset", "twx_profile_read",
 ]
 
 
@@ -112,6 +112,8 @@ def lib():
         L.twx_set_tile_shape.argtypes = [I32]
         L.twx_tile_choices.argtypes = [PI32, I32]
         L.twx_get_stats.argtypes = [P, ctypes.POINTER(Stats)]
+        PU64 = ctypes.POINTER(ctypes.c_uint64)
+        L.twx_cache_stats.argtypes = [P, PU64, PU64, PU64, PU64]
         L.twx_stream.argtypes = [P]
         L.twx_stream.restype = P
         L.twx_profile_enable.argtypes = [P, ctypes.c_int]

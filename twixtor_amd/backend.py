@@ -23,6 +23,37 @@ def _p(a: np.ndarray):
     return ctypes.c_void_p(a.ctypes.data)
 
 
+def _c_buffer(a, dtype, count: int, what: str, cast: bool = True) -> np.ndarray:
+    """`a` as a C-contiguous array of `dtype` holding exactly `count` elements; a ValueError
+    before the library is called otherwise (the library reads and writes by the element count).
+    `cast` converts the values (flows given in another float type); without it the buffer's bytes
+    are taken as they are (an image given as raw bytes)."""
+    if cast:
+        out = np.ascontiguousarray(a, dtype=dtype)
+    else:
+        out = np.ascontiguousarray(a)
+        if out.dtype != np.dtype(dtype):
+            if out.nbytes % np.dtype(dtype).itemsize:
+                raise ValueError(f"{what}: {out.nbytes} bytes are not whole {np.dtype(dtype).name} samples")
+            out = out.reshape(-1).view(dtype)
+    if out.size != count:
+        raise ValueError(f"{what}: {out.size} elements of {np.dtype(dtype).name}, expected {count}")
+    return out
+
+
+def caller_ready() -> None:
+    """Finishes the caller's torch work on a non-default stream before a device-pointer call: the
+    library's stream waits by itself only for the legacy default stream (include/twx.h)."""
+    import sys
+
+    torch = sys.modules.get("torch")
+    if torch is None or not torch.cuda.is_initialized():
+        return
+    s = torch.cuda.current_stream()
+    if s.cuda_stream != 0:
+        s.synchronize()
+
+
 def flow_settings(quality=4, hiq=True, sensitivity=None) -> _ffi.FlowSettings:
     fs = _ffi.FlowSettings()
     fs.quality = quality
@@ -119,10 +150,17 @@ class HipBackend:
         return out
 
     # ---- Backend::solve / solve_pair ----
+    @staticmethod
+    def _image(a, width, height, fmt, what):
+        """An engine image (width x height x A,R,G,B of the format's sample type)."""
+        if fmt not in DTYPES:
+            raise ValueError(f"{what}: format code {fmt} outside 0..2")
+        return _c_buffer(a, DTYPES[fmt], width * height * 4, what, cast=False)
+
     def solve(self, x: np.ndarray, y: np.ndarray, width: int, height: int, fmt: int, fs: _ffi.FlowSettings) -> np.ndarray:
         fw, fh, _ = flow_geometry(fs, width, height)
         out = np.zeros(fw * fh * 2, np.float32)
-        x, y = np.ascontiguousarray(x), np.ascontiguousarray(y)
+        x, y = self._image(x, width, height, fmt, "solve X"), self._image(y, width, height, fmt, "solve Y")
         err = _ffi.errbuf()
         _ffi.check(_ffi.lib().twx_solve(self._h, _p(x), _p(y), width, height, fmt, ctypes.byref(fs), _p(out), err, 512), err)
         return out
@@ -131,11 +169,18 @@ class HipBackend:
         fw, fh, _ = flow_geometry(fs, width, height)
         f_ab = np.zeros(fw * fh * 2, np.float32)
         f_ba = np.zeros(fw * fh * 2, np.float32)
-        a, b = np.ascontiguousarray(a), np.ascontiguousarray(b)
+        a, b = self._image(a, width, height, fmt, "solve_pair A"), self._image(b, width, height, fmt, "solve_pair B")
         err = _ffi.errbuf()
         _ffi.check(_ffi.lib().twx_solve_pair(self._h, _p(a), _p(b), width, height, fmt, ctypes.byref(fs), _p(f_ab), _p(f_ba),
                                              err, 512), err)
         return f_ab, f_ba
+
+    def cache_stats(self) -> tuple:
+        """Backend::cache_stats (device/exec/backend.rs:42-45): (hits, misses, evictions, bytes
+        held) of the resident pyramid cache behind solve and solve_pair."""
+        v = [ctypes.c_uint64() for _ in range(4)]
+        _ffi.check(_ffi.lib().twx_cache_stats(self._h, *[ctypes.byref(x) for x in v]), _ffi.errbuf())
+        return tuple(x.value for x in v)
 
     # ---- Backend::compose_vectors (warping code 0) ----
     def compose_vectors(self, interp, wa, a, b, width, height, fmt, f_ab, f_ba, fw, fh, above_first=True,
@@ -145,10 +190,10 @@ class HipBackend:
         `above_first` is the vertical tap order of the Weighted Blend blur and of the hole fill (the
         retime front-end's, frontend.rs:238-241, by default); `gouraud` the forward warp variant
         (stages::GouraudForward, the one vectors-in and retime-fgbg use)."""
+        a, b = self._image(a, width, height, fmt, "compose A"), self._image(b, width, height, fmt, "compose B")
         out = np.zeros_like(a)
-        a, b = np.ascontiguousarray(a), np.ascontiguousarray(b)
-        fab = None if f_ab is None else np.ascontiguousarray(f_ab, np.float32)
-        fba = None if f_ba is None else np.ascontiguousarray(f_ba, np.float32)
+        fab = None if f_ab is None else _c_buffer(f_ab, np.float32, fw * fh * 2, "compose F_AB")
+        fba = None if f_ba is None else _c_buffer(f_ba, np.float32, fw * fh * 2, "compose F_BA")
         err = _ffi.errbuf()
         _ffi.check(_ffi.lib().twx_compose_vectors(self._h, interp, ctypes.c_float(wa), _p(a), _p(b), width, height, fmt,
                                                   None if fab is None else _p(fab), None if fba is None else _p(fba),
@@ -160,10 +205,10 @@ class HipBackend:
     def forward_warp(self, src, width, height, fmt, flow, fw, fh, amount, pyramid=False, coverage=True):
         """(out, coverage or None) of the forward warp (forward.rs:229-373); `pyramid` False is the
         Gouraud variant."""
-        src = np.ascontiguousarray(src)
+        src = self._image(src, width, height, fmt, "forward warp source")
         out = np.zeros_like(src)
         cov = np.zeros(width * height, np.uint8) if coverage else None
-        flow = np.ascontiguousarray(flow, np.float32)
+        flow = _c_buffer(flow, np.float32, fw * fh * 2, "forward warp flow")
         err = _ffi.errbuf()
         _ffi.check(_ffi.lib().twx_forward_warp(self._h, _p(src), width, height, fmt, _p(flow), fw, fh, ctypes.c_float(amount),
                                                int(pyramid), _p(out), None if cov is None else _p(cov), err, 512), err)
@@ -171,41 +216,50 @@ class HipBackend:
 
     # ---- vector image coding and the Vectors In flow steps ----
     def encode_vectors(self, flow, fw, fh, width, height, fmt, scale) -> np.ndarray:
+        if fmt not in DTYPES:
+            raise ValueError(f"encode: format code {fmt} outside 0..2")
         out = np.zeros(width * height * 4, DTYPES[fmt])
+        flow = _c_buffer(flow, np.float32, fw * fh * 2, "encode flow")
         err = _ffi.errbuf()
-        _ffi.check(_ffi.lib().twx_encode_vectors(self._h, _p(np.ascontiguousarray(flow, np.float32)), fw, fh, width,
-                                                 height, fmt, ctypes.c_float(scale), _p(out), err, 512), err)
+        _ffi.check(_ffi.lib().twx_encode_vectors(self._h, _p(flow), fw, fh, width, height, fmt, ctypes.c_float(scale),
+                                                 _p(out), err, 512), err)
         return out
 
     def decode_vectors(self, vec, width, height, fmt, max_displace):
         """(flow, hole flags) of an encoded vector image."""
+        vec = self._image(vec, width, height, fmt, "decode vector image")
         flow = np.zeros(width * height * 2, np.float32)
         holes = np.zeros(width * height, np.uint8)
         err = _ffi.errbuf()
-        _ffi.check(_ffi.lib().twx_decode_vectors(self._h, _p(np.ascontiguousarray(vec)), width, height, fmt,
-                                                 ctypes.c_float(max_displace), _p(flow), _p(holes), err, 512), err)
+        _ffi.check(_ffi.lib().twx_decode_vectors(self._h, _p(vec), width, height, fmt, ctypes.c_float(max_displace),
+                                                 _p(flow), _p(holes), err, 512), err)
         return flow, holes
 
     def flow_from_vectors(self, vec, width, height, fmt, max_displace, scale=(1.0, 1.0), above_first=True):
+        vec = self._image(vec, width, height, fmt, "Vectors In vector image")
         flow = np.zeros(width * height * 2, np.float32)
         err = _ffi.errbuf()
-        _ffi.check(_ffi.lib().twx_flow_from_vectors(self._h, _p(np.ascontiguousarray(vec)), width, height, fmt,
-                                                    ctypes.c_float(max_displace), ctypes.c_float(scale[0]),
-                                                    ctypes.c_float(scale[1]), int(above_first), _p(flow), err, 512), err)
+        _ffi.check(_ffi.lib().twx_flow_from_vectors(self._h, _p(vec), width, height, fmt, ctypes.c_float(max_displace),
+                                                    ctypes.c_float(scale[0]), ctypes.c_float(scale[1]), int(above_first),
+                                                    _p(flow), err, 512), err)
         return flow
 
     def invert_flow(self, flow, width, height, gouraud=True, above_first=True):
+        flow = _c_buffer(flow, np.float32, width * height * 2, "inverted flow")
         out = np.zeros(width * height * 2, np.float32)
         err = _ffi.errbuf()
-        _ffi.check(_ffi.lib().twx_invert_flow(self._h, _p(np.ascontiguousarray(flow, np.float32)), width, height,
-                                              int(gouraud), int(above_first), _p(out), err, 512), err)
+        _ffi.check(_ffi.lib().twx_invert_flow(self._h, _p(flow), width, height, int(gouraud), int(above_first), _p(out),
+                                              err, 512), err)
         return out
 
     def create_vectors_dev(self, fs, width, height, fmt, clip_ptr, n_in, frame_x, frame_y, max_displace, out_ptr):
         """The Create Vectors pass over a device-resident clip into device-resident wire frames."""
         n = len(frame_x)
+        if len(frame_y) != n:
+            raise ValueError("create vectors: frame_x and frame_y differ in length")
         I = ctypes.c_int32 * n
         err = _ffi.errbuf()
+        caller_ready()
         _ffi.check(_ffi.lib().twx_create_vectors_dev(self._h, ctypes.byref(fs), width, height, fmt,
                                                      ctypes.c_void_p(clip_ptr), n_in, n, I(*frame_x), I(*frame_y),
                                                      ctypes.c_float(max_displace), ctypes.c_void_p(out_ptr), err, 512), err)
@@ -214,7 +268,9 @@ class HipBackend:
     def retime_frames(self, cfg: Retime, clip: bytes, n_in: int, first: int, count: int) -> bytes:
         p = cfg.struct()
         src = np.frombuffer(clip, np.uint8)
-        out = np.zeros(cfg.frame_bytes() * count, np.uint8)
+        if src.size != cfg.frame_bytes() * n_in:
+            raise ValueError(f"retime: a clip of {src.size} bytes, expected {n_in} frames of {cfg.frame_bytes()}")
+        out = np.zeros(cfg.frame_bytes() * max(count, 0), np.uint8)
         err = _ffi.errbuf()
         _ffi.check(_ffi.lib().twx_retime_frames(self._h, ctypes.byref(p), _p(src), n_in, first, count, _p(out), err, 512), err)
         return out.tobytes()
@@ -225,6 +281,7 @@ class HipBackend:
         tensor.data_ptr() on the handle's device). Blocks until the frames are written."""
         p = params if params is not None else cfg.struct()
         err = _ffi.errbuf()
+        caller_ready()
         _ffi.check(_ffi.lib().twx_retime_frames_dev(self._h, ctypes.byref(p), ctypes.c_void_p(clip_ptr), n_in, first, count,
                                                     ctypes.c_void_p(out_ptr), err, 512), err)
 
@@ -232,9 +289,12 @@ class HipBackend:
         """Many clips in one batched render (device addresses): clip i's output frames
         [first[i], first[i] + count[i]) into out_ptrs[i]. `params` are RetimeParams structs."""
         n = len(params)
+        if not all(len(x) == n for x in (clip_ptrs, n_in, first, count, out_ptrs)):
+            raise ValueError("batch: the per-clip lists differ in length")
         P = (RetimeParams * n)(*params)
         I32 = ctypes.c_int32 * n
         V = ctypes.c_void_p * n
         err = _ffi.errbuf()
+        caller_ready()
         _ffi.check(_ffi.lib().twx_retime_batch(self._h, n, P, V(*clip_ptrs), I32(*n_in), I32(*first), I32(*count),
                                                V(*out_ptrs), err, 512), err)

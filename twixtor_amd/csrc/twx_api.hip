@@ -61,9 +61,41 @@ struct GraphPlan {
     }
 };
 
+// The resident pyramids behind twx_solve / twx_solve_pair (the reference's frame cache,
+// device/exec/cache.rs:1-140, used by backend.rs:191-208): an entry is the device pyramid of one
+// host engine image, found again only by the image's exact bytes (its size, format and every
+// sample's bits; a digest of a few samples filters first) and the working geometry the settings
+// give, so a hit is always a pyramid made from these very samples. Least recently used first out,
+// within a quarter of the handle's device budget (the reference's share, backend.rs:147) and at
+// most MAX_ENTRIES entries (the keys are host copies of the images).
+struct PyrEntry {
+    std::vector<uint8_t> key;  // the host image's bytes
+    int W = 0, H = 0, fmt = 0, reduction = 0, levels = 0;
+    uint64_t digest = 0;
+    float* dev = nullptr;      // the pyramid, finest level first
+    size_t bytes = 0;          // device bytes of `dev`
+    bool ready = false;        // the pyramid's build has completed
+};
+
+struct PyramidCache {
+    static constexpr size_t MAX_ENTRIES = 64;
+    std::vector<std::unique_ptr<PyrEntry>> entries;  // most recently used last
+    uint64_t budget = 0, used = 0;
+    uint64_t hits = 0, misses = 0, evictions = 0;
+    void drop(size_t i) {
+        (void)hipFree(entries[i]->dev);
+        used -= entries[i]->bytes;
+        entries.erase(entries.begin() + (long)i);
+    }
+    void clear() {
+        while (!entries.empty()) drop(entries.size() - 1);
+    }
+};
+
 struct twx_handle {
     int device = 0;
-    hipStream_t stream = nullptr;
+    hipStream_t stream = nullptr;  // non-blocking: ordered after the caller's work by `caller_ev`
+    hipEvent_t caller_ev = nullptr;
     uint64_t budget = 0;
     std::mutex mu;
     std::thread::id owner;
@@ -75,6 +107,7 @@ struct twx_handle {
     bool graphs = true;                // capture batched renders into graphs and replay them
     std::unique_ptr<GraphPlan> graph;  // the last captured render
     uint64_t graph_replays = 0;
+    PyramidCache pyramids;             // twx_solve / twx_solve_pair
 };
 
 namespace {
@@ -216,6 +249,16 @@ class Guard {
     bool locked_ = false;
 };
 
+// The handle's stream does not wait for the legacy default stream by itself (it is created
+// non-blocking). An entry point that reads or writes the caller's device buffers first makes it
+// wait for everything queued on the default stream so far, so device work the caller queued there
+// (a fill of the output, the upload of the clip) is complete before the call's kernels touch the
+// buffers. Work on other streams of the caller is the caller's to finish before the call.
+void after_caller(twx_handle* h) {
+    HIPCHECK(hipEventRecord(h->caller_ev, nullptr));
+    HIPCHECK(hipStreamWaitEvent(h->stream, h->caller_ev, 0));
+}
+
 // ---- geometry (prep.rs:63-146, flow.rs:118-197) ------------------------------------------------
 
 struct Geo {
@@ -232,6 +275,7 @@ int geometry(const twx_flow_settings& fs, int W, int H, Geo& g, std::string& err
         err = "prep, track alpha, video range luma and wrap-around solves are not served by this backend";
         return TWX_E_UNSUPPORTED;
     }
+    if (int st = check_frame_pixels(W, H, err)) return st;
     int q = fs.quality;
     int r = (q <= 2) ? 3 : (q == 3 ? 2 : (q == 4 ? 1 : 0));
     if (!fs.hiq) r = (r == 0) ? 1 : (r == 1 ? 2 : (r == 2 ? 3 : r));
@@ -1119,6 +1163,30 @@ int retime_batch_run(twx_handle* h, const std::vector<ClipRun>& runs, std::strin
     return TWX_OK;
 }
 
+// A digest of an engine image for the pyramid cache's first filter: its size, format and 4096
+// evenly spaced 8-byte words (FNV-1a); equal images always have equal digests. Matches are then
+// confirmed byte for byte.
+uint64_t image_digest(const uint8_t* p, size_t bytes, int W, int H, int fmt) {
+    uint64_t hsh = 0xcbf29ce484222325ull;
+    auto eat = [&](uint64_t v) {
+        for (int i = 0; i < 8; i++) {
+            hsh ^= (v >> (8 * i)) & 0xff;
+            hsh *= 0x100000001b3ull;
+        }
+    };
+    eat((uint64_t)W);
+    eat((uint64_t)H);
+    eat((uint64_t)fmt);
+    const size_t words = bytes / 8, step = std::max<size_t>(words / 4096, 1);
+    for (size_t i = 0; i < words; i += step) {
+        uint64_t v;
+        memcpy(&v, p + i * 8, 8);
+        eat(v);
+    }
+    for (size_t i = words * 8; i < bytes; i++) eat(p[i]);
+    return hsh;
+}
+
 // The Backend::solve / solve_pair path on host engine images (rows as given).
 int solve_host(twx_handle* h, const void* x, const void* y, int W, int H, int fmt, const twx_flow_settings& fs,
                float* f_xy, float* f_yx, std::string& err) {
@@ -1129,20 +1197,91 @@ int solve_host(twx_handle* h, const void* x, const void* y, int W, int H, int fm
     hipStream_t s = h->stream;
     double t0 = now_ms();
     DevPool pool(h);
-    size_t fb = image_bytes(W, H, fmt);
-    uint8_t* img = pool.alloc<uint8_t>(fb * 2);
-    HIPCHECK(hipMemcpyAsync(img, x, fb, hipMemcpyHostToDevice, s));
-    HIPCHECK(hipMemcpyAsync(img + fb, y, fb, hipMemcpyHostToDevice, s));
-    std::vector<PyrIn> pyrs(2);
+    const size_t fb = image_bytes(W, H, fmt);
+    const size_t pyr_bytes = pyramid_floats(g) * sizeof(float);
+    PyramidCache& C = h->pyramids;
+    for (size_t i = C.entries.size(); i-- > 0;)  // builds an earlier call did not complete
+        if (!C.entries[i]->ready) C.drop(i);
+    // the pyramid of each image: found in the cache, or built (into a new cache entry when it fits
+    // the cache, else into this call's memory)
+    const float* lv[2] = {nullptr, nullptr};
+    PyrEntry* used_now[2] = {nullptr, nullptr};
+    std::vector<PyrIn> build;
+    const void* imgs[2] = {x, y};
+    std::vector<PyrEntry*> made;  // entries whose pyramids this call builds
     for (int k = 0; k < 2; k++) {
-        pyrs[k] = PyrIn{img + k * fb, W, H, fmt, g, pool.alloc<float>(pyramid_floats(g))};
+        const uint8_t* src = (const uint8_t*)imgs[k];
+        if (k == 1 && memcmp(x, y, fb) == 0) {  // solve(X, X): one pyramid
+            lv[1] = lv[0];
+            C.hits++;
+            h->stats.pyramid_hits++;
+            break;
+        }
+        const uint64_t d = image_digest(src, fb, W, H, fmt);
+        PyrEntry* hit = nullptr;
+        for (size_t i = 0; i < C.entries.size(); i++) {
+            PyrEntry& E = *C.entries[i];
+            if (E.ready && E.digest == d && E.W == W && E.H == H && E.fmt == fmt && E.reduction == g.reduction &&
+                E.levels == g.levels && memcmp(E.key.data(), src, fb) == 0) {
+                std::unique_ptr<PyrEntry> keep = std::move(C.entries[i]);
+                C.entries.erase(C.entries.begin() + (long)i);
+                C.entries.push_back(std::move(keep));
+                hit = C.entries.back().get();
+                break;
+            }
+        }
+        if (hit) {
+            C.hits++;
+            h->stats.pyramid_hits++;
+            lv[k] = hit->dev;
+            used_now[k] = hit;
+            continue;
+        }
+        C.misses++;
+        h->stats.pyramid_builds++;
+        uint8_t* dimg = pool.alloc<uint8_t>(fb);
+        HIPCHECK(hipMemcpyAsync(dimg, src, fb, hipMemcpyHostToDevice, s));
+        float* dst = nullptr;
+        if (pyr_bytes <= C.budget) {
+            // room for the new entry: the least recently used entries this call does not read go
+            for (size_t i = 0; i < C.entries.size() &&
+                               (C.used + pyr_bytes > C.budget || C.entries.size() >= PyramidCache::MAX_ENTRIES);) {
+                if (C.entries[i].get() == used_now[0]) { i++; continue; }
+                C.drop(i);
+                C.evictions++;
+            }
+            if (C.used + pyr_bytes <= C.budget && C.entries.size() < PyramidCache::MAX_ENTRIES) {
+                auto E = std::make_unique<PyrEntry>();
+                HIPCHECK(hipMalloc(&E->dev, pyr_bytes));
+                E->key.assign(src, src + fb);
+                E->W = W;
+                E->H = H;
+                E->fmt = fmt;
+                E->reduction = g.reduction;
+                E->levels = g.levels;
+                E->digest = d;
+                E->bytes = pyr_bytes;
+                C.used += pyr_bytes;
+                dst = E->dev;
+                used_now[k] = E.get();
+                made.push_back(E.get());
+                C.entries.push_back(std::move(E));
+            }
+        }
+        if (!dst) dst = pool.alloc<float>(pyramid_floats(g));
+        lv[k] = dst;
+        build.push_back(PyrIn{dimg, W, H, fmt, g, dst});
     }
-    build_pyramids(s, pool, pyrs);
+    build_pyramids(s, pool, build);
+    // an entry becomes findable once its pyramid is complete; one whose build did not complete
+    // (an exception leaves this function) stays unready and is dropped by the next eviction
+    HIPCHECK(hipStreamSynchronize(s));
+    for (PyrEntry* E : made) E->ready = true;
     size_t fn = (size_t)g.w0 * g.h0 * 2;
     std::vector<SolveIn> solves;
     // solve_pair order: F_BA (X = B) then F_AB (X = A); both in one batched launch
-    if (f_yx) solves.push_back(SolveIn{pyrs[1].levels, pyrs[0].levels, g, pool.alloc<float>(fn)});
-    solves.push_back(SolveIn{pyrs[0].levels, pyrs[1].levels, g, pool.alloc<float>(fn)});
+    if (f_yx) solves.push_back(SolveIn{lv[1], lv[0], g, pool.alloc<float>(fn)});
+    solves.push_back(SolveIn{lv[0], lv[1], g, pool.alloc<float>(fn)});
     if ((st = run_solves(s, pool, solves, fs, h->budget / 2, err))) return st;
     if (f_yx) {
         HIPCHECK(hipMemcpyAsync(f_yx, solves[0].out, fn * 4, hipMemcpyDeviceToHost, s));
@@ -1151,7 +1290,6 @@ int solve_host(twx_handle* h, const void* x, const void* y, int W, int H, int fm
         HIPCHECK(hipMemcpyAsync(f_xy, solves[0].out, fn * 4, hipMemcpyDeviceToHost, s));
     }
     HIPCHECK(hipStreamSynchronize(s));
-    h->stats.pyramid_builds += 2;
     h->stats.pair_solves += solves.size();
     h->stats.solve_calls++;
     h->stats.device_ms += now_ms() - t0;
@@ -1181,9 +1319,11 @@ int twx_open(int index, twx_handle** out, char* err, size_t errlen) {
     try {
         HIPCHECK(hipSetDevice(dev));
         HIPCHECK(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+        HIPCHECK(hipEventCreateWithFlags(&h->caller_ev, hipEventDisableTiming));
         size_t free_b = 0, total_b = 0;
         HIPCHECK(hipMemGetInfo(&free_b, &total_b));
         h->budget = (uint64_t)(total_b * 0.75);
+        h->pyramids.budget = h->budget / 4;
         h->host.pinned = true;
         h->host.min_block = 8u << 20;
         // keep freed stream-ordered memory in the pool between calls
@@ -1195,6 +1335,7 @@ int twx_open(int index, twx_handle** out, char* err, size_t errlen) {
         int st = run_selfcheck(h->stream, e);
         if (st) {
             set_err(err, errlen, "device self-check refused the device: " + e);
+            (void)hipEventDestroy(h->caller_ev);
             (void)hipStreamDestroy(h->stream);
             delete h;
             return st;
@@ -1215,6 +1356,8 @@ void twx_close(twx_handle* h) {
         (void)hipStreamSynchronize(h->stream);
         (void)hipStreamDestroy(h->stream);
     }
+    if (h->caller_ev) (void)hipEventDestroy(h->caller_ev);
+    h->pyramids.clear();
     h->dev.release();
     h->host.release();
     delete h;
@@ -1368,6 +1511,7 @@ int twx_forward_warp(twx_handle* h, const void* src, int32_t width, int32_t heig
             if (width < 2 || height < 2 || flow_w < 2 || flow_h < 2 || fmt < 0 || fmt > 2) {
                 e = "forward warp needs images and a flow of at least 2x2";
                 st = TWX_E_MISMATCH;
+            } else if ((st = check_frame_pixels(width, height, e))) {
             } else if (pyramid) {
                 e = "forward warp: the textured variant (ForwardOpts::pyramid) has no device kernel here; the Gouraud "
                     "variant has";
@@ -1433,6 +1577,7 @@ int twx_encode_vectors(twx_handle* h, const float* flow, int32_t flow_w, int32_t
             e = "vector encode needs images and a flow of at least 2x2";
             return TWX_E_MISMATCH;
         }
+        if (int st = check_frame_pixels(width, height, e)) return st;
         hipStream_t s = h->stream;
         DevPool pool(h);
         const size_t fb = image_bytes(width, height, fmt), fn = (size_t)flow_w * flow_h * 2;
@@ -1464,6 +1609,7 @@ int twx_decode_vectors(twx_handle* h, const void* vec, int32_t width, int32_t he
             e = "vector decode: bad size or format";
             return TWX_E_MISMATCH;
         }
+        if (int st = check_frame_pixels(width, height, e)) return st;
         hipStream_t s = h->stream;
         DevPool pool(h);
         const size_t fb = image_bytes(width, height, fmt), n = (size_t)width * height;
@@ -1514,6 +1660,7 @@ int twx_flow_from_vectors(twx_handle* h, const void* vec, int32_t width, int32_t
             e = "vector decode: bad size or format";
             return TWX_E_MISMATCH;
         }
+        if (int st = check_frame_pixels(width, height, e)) return st;
         hipStream_t s = h->stream;
         DevPool pool(h);
         const size_t fb = image_bytes(width, height, fmt), n = (size_t)width * height;
@@ -1536,6 +1683,7 @@ int twx_invert_flow(twx_handle* h, const float* flow, int32_t width, int32_t hei
             e = "forward warp needs images and a flow of at least 2x2";
             return TWX_E_UNSUPPORTED;
         }
+        if (int st = check_frame_pixels(width, height, e)) return st;
         if (!forward_gouraud) {
             e = "the flow inversion through the textured forward warp has no device kernel here";
             return TWX_E_UNSUPPORTED;
@@ -1587,6 +1735,7 @@ int twx_create_vectors_dev(twx_handle* h, const twx_flow_settings* fs, int32_t w
         Geo g;
         int st = geometry(*fs, width, height, g, e);
         if (st) return st;
+        after_caller(h);
         hipStream_t s = h->stream;
         double t0 = now_ms();
         DevPool pool(h);
@@ -1681,6 +1830,7 @@ int twx_compose_vectors(twx_handle* h, int32_t interp, float wa, const void* a, 
             set_err(err, errlen, "the warps need images and a flow of at least 2x2");
             return TWX_E_MISMATCH;
         }
+        if ((st = check_frame_pixels(width, height, e))) { set_err(err, errlen, e); return st; }
         hipStream_t s = h->stream;
         double t0 = now_ms();
         DevPool pool(h);
@@ -1818,7 +1968,10 @@ int twx_retime_frames_dev(twx_handle* h, const twx_retime_params* p, const void*
     int st = TWX_OK;
     try {
         Guard g(h, st, e);
-        if (g.ok()) st = retime_batch_run(h, {ClipRun{*p, (const uint8_t*)clip, n_in, first, count, (uint8_t*)out}}, e);
+        if (g.ok()) {
+            after_caller(h);
+            st = retime_batch_run(h, {ClipRun{*p, (const uint8_t*)clip, n_in, first, count, (uint8_t*)out}}, e);
+        }
     } catch (const HipError& x) {
         e = x.msg;
         st = TWX_E_DEVICE;
@@ -1842,6 +1995,7 @@ int twx_retime_batch(twx_handle* h, int32_t n_clips, const twx_retime_params* pa
                 std::vector<ClipRun> runs;
                 for (int i = 0; i < n_clips; i++)
                     runs.push_back(ClipRun{params[i], (const uint8_t*)clips[i], n_in[i], first[i], count[i], (uint8_t*)outs[i]});
+                after_caller(h);
                 st = retime_batch_run(h, runs, e);
             }
         }
@@ -1889,6 +2043,7 @@ static int warp_blend_entry(twx_handle* h, bool backward, int32_t channels, int3
     try {
         Guard g(h, st, e);
         if (g.ok()) {
+            after_caller(h);
             hipStream_t s = h->stream;
             DevPool pool(h);
             const size_t px = (size_t)std::max(width, 0) * std::max(height, 0);
@@ -1949,6 +2104,16 @@ int twx_set_graphs(twx_handle* h, int32_t on) {
         st = TWX_E_DEVICE;
     }
     return st;
+}
+
+int twx_cache_stats(twx_handle* h, uint64_t* hits, uint64_t* misses, uint64_t* evictions, uint64_t* bytes_held) {
+    if (!h) return TWX_E_MISMATCH;
+    std::lock_guard<std::mutex> g(h->mu);
+    if (hits) *hits = h->pyramids.hits;
+    if (misses) *misses = h->pyramids.misses;
+    if (evictions) *evictions = h->pyramids.evictions;
+    if (bytes_held) *bytes_held = h->pyramids.used;
+    return TWX_OK;
 }
 
 int twx_get_stats(twx_handle* h, twx_stats* out) {

@@ -315,6 +315,7 @@ int warp_blend_run(hipStream_t s, bool backward, int C, int W, int H, const floa
         err = "warp+blend: 1 to 4 channels, images and flows of at least 2x2";
         return TWX_E_MISMATCH;
     }
+    if (int st = check_frame_pixels(W, H, err)) return st;
     DiffArgs P{};
     P.a = a;
     P.b = b;

@@ -14,6 +14,7 @@
 #include <vector>
 
 #include "../../include/twx.h"
+#include "twx_common.hpp"
 
 namespace twx {
 namespace timeline {
@@ -187,7 +188,7 @@ inline int check_params(const twx_retime_params& p, std::string& err) {
         err = "retime parameters: popup out of range";
         return TWX_E_MISMATCH;
     }
-    return TWX_OK;
+    return check_frame_pixels(p.width, p.height, err);
 }
 
 }  // namespace timeline
