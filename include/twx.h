@@ -258,6 +258,21 @@ int twx_warp_blend_backward(twx_handle* h, int32_t channels, int32_t width, int3
                             float* grad_a, float* grad_b, float* grad_f_ab, float* grad_f_ba,
                             char* err, size_t errlen);
 
+/* twx_warp_blend_backward without any device or pinned-host allocation. No counterpart in the
+ * reference: it is for callers that cannot allocate (code under HIP graph capture, tight
+ * refinement loops, harnesses that forbid hipMalloc). The same arguments, kernels and results as
+ * twx_warp_blend_backward; its scratch (the per-pixel flow-value gradients, 16 bytes a pixel, and
+ * a flag) is a fixed array of the library that each device holds from twx_open until the process
+ * ends, so width * height must be at most TWX_NOALLOC_MAX_PIXELS (TWX_E_UNSUPPORTED above). Every
+ * call on one device shares that array: calls that overlap on one device (two handles, or two
+ * threads) race. Its launches are not timed by twx_profile_*. DEVICE pointers. */
+#define TWX_NOALLOC_MAX_PIXELS 2097152u /* 2^21: 1920 x 1080 fits; 32 MiB of scratch a device */
+int twx_warp_blend_backward_noalloc(twx_handle* h, int32_t channels, int32_t width, int32_t height,
+                                    const float* a, const float* b, const float* f_ab,
+                                    const float* f_ba, int32_t flow_w, int32_t flow_h, float wa,
+                                    const float* grad_out, float* grad_a, float* grad_b,
+                                    float* grad_f_ab, float* grad_f_ba, char* err, size_t errlen);
+
 /* ---- counters (Exec::summary, device/exec/mod.rs:292-322) -------------------------------- */
 
 typedef struct twx_stats {

@@ -1332,7 +1332,16 @@ int twx_open(int index, twx_handle** out, char* err, size_t errlen) {
         uint64_t thresh = UINT64_MAX;
         HIPCHECK(hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &thresh));
         std::string e;
-        int st = run_selfcheck(h->stream, e);
+        // the device address of the allocation-free backward's fixed scratch, with this device current
+        int st = noalloc_scratch_setup(dev, e);
+        if (st) {
+            set_err(err, errlen, e);
+            (void)hipEventDestroy(h->caller_ev);
+            (void)hipStreamDestroy(h->stream);
+            delete h;
+            return st;
+        }
+        st = run_selfcheck(h->stream, e);
         if (st) {
             set_err(err, errlen, "device self-check refused the device: " + e);
             (void)hipEventDestroy(h->caller_ev);
@@ -2085,6 +2094,28 @@ int twx_warp_blend_backward(twx_handle* h, int32_t channels, int32_t width, int3
                             char* err, size_t errlen) {
     return warp_blend_entry(h, true, channels, width, height, a, b, f_ab, f_ba, flow_w, flow_h, wa, nullptr, grad_out,
                             grad_a, grad_b, grad_f_ab, grad_f_ba, err, errlen);
+}
+
+int twx_warp_blend_backward_noalloc(twx_handle* h, int32_t channels, int32_t width, int32_t height, const float* a,
+                                    const float* b, const float* f_ab, const float* f_ba, int32_t flow_w, int32_t flow_h,
+                                    float wa, const float* grad_out, float* grad_a, float* grad_b, float* grad_f_ab,
+                                    float* grad_f_ba, char* err, size_t errlen) {
+    std::string e;
+    int st = TWX_OK;
+    try {
+        Guard g(h, st, e);
+        if (g.ok()) {
+            g_prof = nullptr;  // no profiling events: they are taken from a growing pool
+            after_caller(h);
+            st = warp_blend_backward_noalloc(h->stream, h->device, channels, width, height, a, b, f_ab, f_ba, flow_w,
+                                             flow_h, wa, grad_out, grad_a, grad_b, grad_f_ab, grad_f_ba, e);
+        }
+    } catch (const HipError& x) {
+        e = x.msg;
+        st = TWX_E_DEVICE;
+    }
+    if (st) set_err(err, errlen, e);
+    return st;
 }
 
 int twx_set_tile_shape(int32_t variant) { return tile_force(variant) == 0 ? TWX_OK : TWX_E_MISMATCH; }

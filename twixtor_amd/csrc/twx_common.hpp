@@ -237,6 +237,12 @@ int run_selfcheck(hipStream_t s, std::string& err);
 int warp_blend_run(hipStream_t s, bool backward, int C, int W, int H, const float* a, const float* b, const float* f_ab,
                    const float* f_ba, int fw, int fh, float wa, float* out, const float* grad_out, float* grad_a,
                    float* grad_b, float* grad_f_ab, float* grad_f_ba, float* gp_scratch, int* bad, std::string& err);
+// The backward of the warp+blend on the library's fixed per-device scratch (no allocation): the
+// scratch address of `device` is taken once by noalloc_scratch_setup with that device current.
+int noalloc_scratch_setup(int device, std::string& err);
+int warp_blend_backward_noalloc(hipStream_t s, int device, int C, int W, int H, const float* a, const float* b,
+                                const float* f_ab, const float* f_ba, int fw, int fh, float wa, const float* grad_out,
+                                float* grad_a, float* grad_b, float* grad_f_ab, float* grad_f_ba, std::string& err);
 
 // The Gouraud forward warp (twx_forward.hip): device buffers of one warp. The caller allocates
 // pos (W*H), counts and fill (W*H, zeroed), offsets (W*H), block_totals (fw_scan_blocks(W*H)),

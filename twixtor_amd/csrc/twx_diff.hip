@@ -14,8 +14,10 @@
 //               fixed order (deterministic).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -359,6 +361,102 @@ int warp_blend_run(hipStream_t s, bool backward, int C, int W, int H, const floa
     dim3 fgrd((fw + 255) / 256, fh);
     if (grad_f_ba) TWX_LAUNCH(s, F_DIFF_GATHER, 8.0 * (double)W * H + fl, k_wb_flow_gather<<<fgrd, blk, 0, s>>>(P, 0));
     if (grad_f_ab) TWX_LAUNCH(s, F_DIFF_GATHER, 8.0 * (double)W * H + fl, k_wb_flow_gather<<<fgrd, blk, 0, s>>>(P, 1));
+    return TWX_OK;
+}
+
+// ---- the backward without device allocation ---------------------------------------------------
+//
+// The backward's scratch (the per-pixel flow-value gradients of both warps, 16 bytes a pixel, and
+// the NaN flag) lives in a fixed device array of this library instead of memory taken per call:
+// 256 bytes of flags, then the gradients of up to NOALLOC_MAX_PIXELS pixels. Each device holds its
+// own copy until the process ends. In host code `scratch` only names the symbol: its device
+// address on each device comes from hipGetSymbolAddress, taken once per device with that device
+// current (at twx_open), and only that address is ever handed to kernels.
+
+constexpr size_t NOALLOC_FLAG_BYTES = 256;
+constexpr size_t NOALLOC_SCRATCH_BYTES = NOALLOC_FLAG_BYTES + (size_t)TWX_NOALLOC_MAX_PIXELS * 16;
+
+static __device__ __align__(256) unsigned char scratch[NOALLOC_SCRATCH_BYTES];
+
+namespace {
+
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[MAX_DEVICES] = {};
+
+// Zeroes the image gradients (the backward accumulates into them) and the NaN flag: a kernel, not
+// hipMemsetAsync, so a caller may capture the call's launches into a graph.
+__global__ void k_wb_clear(float* __restrict__ ga, float* __restrict__ gb, size_t n, int* bad) {
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        if (ga) ga[i] = 0.0f;
+        if (gb) gb[i] = 0.0f;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) *bad = 0;
+}
+
+}  // namespace
+
+int noalloc_scratch_setup(int device, std::string& err) {
+    if (device < 0 || device >= MAX_DEVICES) {
+        err = "device index beyond the allocation-free scratch table";
+        return TWX_E_DEVICE;
+    }
+    std::lock_guard<std::mutex> g(g_scratch_mu);
+    if (g_scratch_ptr[device]) return TWX_OK;
+    void* p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+        err = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e);
+        return TWX_E_DEVICE;
+    }
+    g_scratch_ptr[device] = p;
+    return TWX_OK;
+}
+
+int warp_blend_backward_noalloc(hipStream_t s, int device, int C, int W, int H, const float* a, const float* b,
+                                const float* f_ab, const float* f_ba, int fw, int fh, float wa, const float* grad_out,
+                                float* grad_a, float* grad_b, float* grad_f_ab, float* grad_f_ba, std::string& err) {
+    if (C < 1 || C > 4 || W < 2 || H < 2 || fw < 2 || fh < 2) {
+        err = "warp+blend: 1 to 4 channels, images and flows of at least 2x2";
+        return TWX_E_MISMATCH;
+    }
+    if ((uint64_t)W * (uint64_t)H > (uint64_t)TWX_NOALLOC_MAX_PIXELS) {
+        err = "warp+blend backward without allocation: " + std::to_string(W) + "x" + std::to_string(H) +
+              " is over its fixed scratch of " + std::to_string(TWX_NOALLOC_MAX_PIXELS) + " pixels";
+        return TWX_E_UNSUPPORTED;
+    }
+    void* base = nullptr;
+    {
+        std::lock_guard<std::mutex> g(g_scratch_mu);
+        if (device >= 0 && device < MAX_DEVICES) base = g_scratch_ptr[device];
+    }
+    if (!base) {
+        err = "the allocation-free scratch of this device was not set up (twx_open)";
+        return TWX_E_DEVICE;
+    }
+    int* bad = (int*)base;
+    float* gp = (float*)((unsigned char*)base + NOALLOC_FLAG_BYTES);
+    const size_t n = (size_t)C * W * H;
+    k_wb_clear<<<(unsigned)std::min<size_t>((n + 255) / 256, 2048), 256, 0, s>>>(grad_a, grad_b, n, bad);
+    if (hipGetLastError() != hipSuccess) {
+        err = "warp+blend backward: the clear launch failed";
+        return TWX_E_DEVICE;
+    }
+    int st = warp_blend_run(s, true, C, W, H, a, b, f_ab, f_ba, fw, fh, wa, nullptr, grad_out, grad_a, grad_b, grad_f_ab,
+                            grad_f_ba, gp, bad, err);
+    if (st) return st;
+    int hb = 0;
+    hipError_t e = hipMemcpyAsync(&hb, bad, sizeof(int), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) {
+        err = std::string("warp+blend backward: ") + hipGetErrorString(e);
+        return TWX_E_DEVICE;
+    }
+    if (hb) {
+        err = "warp+blend: a sample point is NaN or never converges";
+        return TWX_E_UNSUPPORTED;
+    }
     return TWX_OK;
 }
 
