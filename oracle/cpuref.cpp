@@ -735,9 +735,48 @@ static inline void premultiply_u16(const uint16_t p[4], uint16_t o[4]) {
     o[0] = (uint16_t)q((float)p[0] * PREMUL_16_ALPHA);
     for (int c = 1; c < 4; c++) o[c] = (uint16_t)q((float)p[c] * a);
 }
+// ------------------------------------------------------------------------------------------
+// The NaN rule (nanrule.rs:1-120, the emulator rule the references were rendered under): numbers
+// never depend on it. A NaN result of an f32 sum, difference, product or quotient takes: the one
+// NaN operand, quieted; of two NaN operands the one with the larger fraction (equal fractions: the
+// positive one; a quiet NaN before a signalling one), quieted; for an invalid operation (inf - inf,
+// 0 * inf, 0 / 0) the x86 default NaN 0xffc00000. The call sites compute plainly and repeat a
+// result with these operations only when it holds a NaN, as the reference does.
+static inline uint32_t nr_u(float f) { uint32_t b; memcpy(&b, &f, 4); return b; }
+static inline float nr_f(uint32_t b) { float f; memcpy(&f, &b, 4); return f; }
+static inline float nr_quiet(float x) { return nr_f(nr_u(x) | 0x00400000u); }
+static inline float nr_pick(float a, float b) {  // nanrule.rs:44-73
+    const uint32_t ba = nr_u(a), bb = nr_u(b);
+    const bool sa = !(ba & 0x00400000u), sb = !(bb & 0x00400000u);
+    const uint32_t fa = ba & 0x007fffffu, fb = bb & 0x007fffffu;
+    const bool a_larger = fa > fb || (fa == fb && (ba >> 31) < (bb >> 31));
+    const bool take_b = (sa == sb) ? !a_larger : sa;
+    return nr_quiet(take_b ? b : a);
+}
+static inline float nr_fix(float r, float a, float b) {  // nanrule.rs:76-93
+    if (!std::isnan(r)) return r;
+    if (std::isnan(a)) return std::isnan(b) ? nr_pick(a, b) : nr_quiet(a);
+    if (std::isnan(b)) return nr_quiet(b);
+    return nr_f(0xffc00000u);
+}
+static inline float nr_add(float a, float b) { return nr_fix(a + b, a, b); }
+static inline float nr_sub(float a, float b) { return nr_fix(a - b, a, b); }
+static inline float nr_mul(float a, float b) { return nr_fix(a * b, a, b); }
+static inline float nr_div(float a, float b) { return nr_fix(a / b, a, b); }
+// lerp (nanrule.rs:240): (b - a) * f + a; lerp_base_first (:247): a + (b - a) * f
+static inline float nr_lerp(float a, float b, float f) { return nr_add(nr_mul(nr_sub(b, a), f), a); }
+static inline float nr_lerp_base_first(float a, float b, float f) { return nr_add(a, nr_mul(nr_sub(b, a), f)); }
+
+// image.rs:291-300: colour times alpha, the products again with the NaN rule when one is a NaN
 static inline void premultiply_f32(const float p[4], float o[4]) {
     o[0] = p[0];
-    for (int c = 1; c < 4; c++) o[c] = p[0] * p[c];
+    bool nan = std::isnan(p[0]);
+    for (int c = 1; c < 4; c++) {
+        o[c] = p[0] * p[c];
+        nan |= std::isnan(o[c]);
+    }
+    if (nan)
+        for (int c = 1; c < 4; c++) o[c] = nr_mul(p[0], p[c]);
 }
 static inline void unpremultiply_u8(const uint8_t p[4], uint8_t o[4]) {
     if (p[0] == 0) { memcpy(o, p, 4); return; }
@@ -760,7 +799,13 @@ static inline void unpremultiply_f32(const float p[4], float o[4]) {
     if ((double)p[0] < 1e-12) { o[0] = o[1] = o[2] = o[3] = 0.0f; return; }
     float inv = 1.0f / p[0];
     o[0] = p[0];
-    for (int c = 1; c < 4; c++) o[c] = p[c] * inv;
+    bool nan = std::isnan(p[0]);
+    for (int c = 1; c < 4; c++) {
+        o[c] = p[c] * inv;
+        nan |= std::isnan(o[c]);
+    }
+    if (nan)  // image.rs:344-348: the colour is the first operand of the product
+        for (int c = 1; c < 4; c++) o[c] = nr_mul(p[c], inv);
 }
 
 static inline size_t sample_size(int fmt) { return fmt == TWX_FMT_U8 ? 1 : (fmt == TWX_FMT_U16 ? 2 : 4); }
@@ -906,12 +951,21 @@ static inline void bilerp4(const P* src, long w, int32_t w1, int32_t h1, float x
     if (iy >= h1) { iy = h1 - 1; fy = 1.0f; }
     long i00 = ((long)iy * w + ix) * 4;
     long i01 = i00 + w * 4;
+    bool nan = false;
     for (int k = 0; k < 4; k++) {
         float p00 = to_f(src[i00 + k]), p10 = to_f(src[i00 + 4 + k]);
         float p01 = to_f(src[i01 + k]), p11 = to_f(src[i01 + 4 + k]);
         float top = p00 + (p10 - p00) * fx;
         float bot = p01 + (p11 - p01) * fx;
         out[k] = top + (bot - top) * fy;
+        nan |= std::isnan(out[k]);
+    }
+    if (nan) {  // warp.rs:334-343: the same lerps again with the NaN rule
+        for (int k = 0; k < 4; k++) {
+            float p00 = to_f(src[i00 + k]), p10 = to_f(src[i00 + 4 + k]);
+            float p01 = to_f(src[i01 + k]), p11 = to_f(src[i01 + 4 + k]);
+            out[k] = nr_lerp_base_first(nr_lerp_base_first(p00, p10, fx), nr_lerp_base_first(p01, p11, fx), fy);
+        }
     }
 }
 
@@ -1052,6 +1106,18 @@ static void lerp_int(T* f, const T* s, float w, float scale) {
     f[0] = (T)cvtt64f(a + 0.5f);
 }
 static void lerp_f32(float* f, const float* s, float w) {
+    bool nan = false;
+    for (int k = 0; k < 4; k++) nan |= std::isnan(f[k]) || std::isnan(s[k]);
+    if (nan) {  // blend.rs:181-216: a NaN among the inputs, the arithmetic with the NaN rule
+        float a = nr_lerp(s[0], f[0], w);
+        if (!(a > 0.0f)) { f[0] = f[1] = f[2] = f[3] = 0.0f; return; }
+        float r = nr_lerp(s[1], f[1], w);
+        float g = nr_lerp(s[2], f[2], w);
+        float b = nr_lerp(s[3], f[3], w);
+        if (a > 1.0f) { r = nr_div(r, a); g = nr_div(g, a); b = nr_div(b, a); a = 1.0f; }
+        f[1] = nr_add(r, 0.0f); f[2] = nr_add(g, 0.0f); f[3] = nr_add(b, 0.0f); f[0] = nr_add(a, 0.0f);
+        return;
+    }
     float a = (f[0] - s[0]) * w + s[0];
     if (!(a > 0.0f)) { f[0] = f[1] = f[2] = f[3] = 0.0f; return; }
     float r = (f[1] - s[1]) * w + s[1];
@@ -1105,7 +1171,11 @@ static void blend(void* out, const void* a, const void* b, long w, long h, int f
         float* o = (float*)out;
         const float *pa = (const float*)a, *pb = (const float*)b;
 #pragma omp parallel for schedule(static)
-        for (long i = 0; i < n; i++) o[i] = ((pb[i] - pa[i]) * wt + pa[i]) + 0.0f;
+        for (long i = 0; i < n; i++) {
+            o[i] = ((pb[i] - pa[i]) * wt + pa[i]) + 0.0f;
+            // blend.rs:105-110: a NaN result again with the NaN rule
+            if (std::isnan(o[i])) o[i] = nr_add(nr_lerp(pa[i], pb[i], wt), 0.0f);
+        }
     }
 }
 
@@ -1442,13 +1512,43 @@ static inline void over(const float a[4], const float b[4], float white, float o
     for (int k = 1; k < 4; k++) o[k] = b[k] * ia + a[k];
 }
 
-// smartblend.rs:193-212: returns true for the raw store of the fallback colour
+// over() with the NaN rule's operations
+static inline void over_nr(const float a[4], const float b[4], float white, float o[4]) {
+    if (a[0] == 0.0f) { for (int k = 0; k < 4; k++) o[k] = b[k]; return; }
+    if (a[0] == white) { for (int k = 0; k < 4; k++) o[k] = a[k]; return; }
+    float ia = nr_sub(1.0f, nr_div(a[0], white));
+    o[0] = nr_add(nr_mul(nr_sub(white, b[0]), nr_div(a[0], white)), b[0]);
+    for (int k = 1; k < 4; k++) o[k] = nr_add(nr_mul(b[k], ia), a[k]);
+}
+
+// smartblend.rs:193-212: returns true for the raw store of the fallback colour. A NaN in the
+// composite: the same arithmetic again with the NaN rule's operations (the references were
+// rendered under it, nanrule.rs:14-21).
 static inline bool smart_pixel(const float p1[4], const float p2[4], float w, float white, float v[4]) {
     float c1[4], c2[4];
     over(p1, p2, white, c1);
     over(p2, p1, white, c2);
     float amax = p1[0] > p2[0] ? p1[0] : p2[0];
-    for (int k = 0; k < 4; k++) v[k] = (c2[k] - c1[k]) * w + c1[k];
+    bool nan = false;
+    for (int k = 0; k < 4; k++) {
+        v[k] = (c2[k] - c1[k]) * w + c1[k];
+        nan |= std::isnan(v[k]);
+    }
+    if (nan) {
+        over_nr(p1, p2, white, c1);
+        over_nr(p2, p1, white, c2);
+        for (int k = 0; k < 4; k++) v[k] = nr_lerp(c1[k], c2[k], w);
+        if (v[0] == amax) return false;
+        if (v[0] == 0.0f) {
+            const float* c = c1[0] > c2[0] ? c1 : c2;
+            for (int k = 0; k < 4; k++) v[k] = c[k];
+            return true;
+        }
+        float alpha = v[0];
+        for (int k = 1; k < 4; k++) v[k] = nr_div(nr_mul(v[k], amax), alpha);
+        v[0] = amax;
+        return false;
+    }
     if (v[0] == amax) return false;
     if (v[0] == 0.0f) {
         const float* c = c1[0] > c2[0] ? c1 : c2;

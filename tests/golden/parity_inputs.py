@@ -5,8 +5,8 @@ pixel by closed forms and by numpy's legacy RandomState with fixed seeds; its ca
 source, size, depth and frame count of each case, and parity/retime-time/results/supp/
 refs_sha256.txt holds the SHA-256 of the ORIGINAL plugin's output for each case. This module
 restates those pixel formulas (the same expressions, so the same bytes) so that the oracle and the
-HIP backend can be graded against those hashes. Only the sources of in-scope cases are restated
-(no special float values).
+HIP backend can be graded against those hashes. The sources of the in-scope cases are restated,
+and the 32 bpc special-value patches (gen.py:143-154) used by the refusal contract's cases.
 """
 from __future__ import annotations
 
@@ -26,9 +26,20 @@ def _rgb(X, Y):
     return np.stack([_tex(X, Y, c) for c in range(3)], -1)
 
 
+# The eleven float32 special values of the grid's special-value sources (gen.py:30-43), by bit
+# pattern, in the grid's order: +0, -0, the smallest and largest denormal, FLT_MIN, 1, 2.5,
+# FLT_MAX, +inf, -inf, the quiet NaN.
+SPECIAL_BITS = [("zero", 0x00000000), ("negzero", 0x80000000), ("mindenorm", 0x00000001),
+                ("maxdenorm", 0x007FFFFF), ("fltmin", 0x00800000), ("one", 0x3F800000),
+                ("above1", 0x40200000), ("fltmax", 0x7F7FFFFF), ("posinf", 0x7F800000),
+                ("neginf", 0xFF800000), ("nan", 0x7FC00000)]
+SPECIAL_VALUES = [np.array([b], np.uint32).view(np.float32)[0] for _, b in SPECIAL_BITS]
+SPECIAL_BY_NAME = {n: v for (n, _), v in zip(SPECIAL_BITS, SPECIAL_VALUES)}
+
+
 def frame_f64(src: str, W: int, H: int, t: int) -> np.ndarray:
     """One frame as float64 (H, W, 4), straight alpha (gen.py:74-128)."""
-    if src.startswith("ext-"):
+    if src.startswith(("ext-", "spec-", "specA-")):  # special patches go over the shifting texture
         src = "shift"
     X, Y = np.meshgrid(np.arange(W, dtype=np.float64), np.arange(H, dtype=np.float64))
     A = np.ones((H, W))
@@ -95,7 +106,21 @@ def frame_raw(src: str, W: int, H: int, t: int, depth: int) -> bytes:
         arr = np.clip(np.rint(a * 65535.0), 0, 65535).astype("<u2")
     else:
         arr = a.astype(np.float32)
-    if src.startswith("ext-"):
+    if depth == 32 and src.startswith("spec-"):
+        # a rectangle of special values in R, G and B; "all" cycles the values down its rows, G
+        # three values ahead of R, B left as it is
+        x0, y0, s, sh = _patch_rect(W, H, t)
+        name = src[len("spec-"):]
+        if name == "all":
+            for j in range(sh):
+                arr[y0 + j, x0:x0 + s, 0] = SPECIAL_VALUES[(j + t) % 11]
+                arr[y0 + j, x0:x0 + s, 1] = SPECIAL_VALUES[(j + 3 + t) % 11]
+        else:
+            arr[y0:y0 + sh, x0:x0 + s, 0:3] = SPECIAL_BY_NAME[name]
+    elif depth == 32 and src.startswith("specA-"):  # the same rectangle in alpha only
+        x0, y0, s, sh = _patch_rect(W, H, t)
+        arr[y0:y0 + sh, x0:x0 + s, 3] = SPECIAL_BY_NAME[src[len("specA-"):]]
+    elif src.startswith("ext-"):
         x0, y0, s, sh = _patch_rect(W, H, t)
         hi = {8: 255, 16: 65535}[depth]
         for j in range(sh):

@@ -227,7 +227,7 @@ def test_batch_of_mixed_clips_equals_single_clip_calls(backend):
 
 def test_every_served_retime_time_hash_on_device(backend):
     """The retime-time grid on the device (the time map, the rates and the fast path at three
-    depths, through twx_retime_frames): the same 279 hashes as the oracle's pin, and the same 6
+    depths, through twx_retime_frames): the same 279 hashes as the oracle's pin (277 of the original plugin), and the same 6
     Weighted Blend cases over non-finite float samples refused (status 3)."""
     from test_oracle_pins import TIME_CPU_PATH_ONLY, time_case_run, time_case_want, time_cases
 
@@ -279,3 +279,38 @@ def test_pyramid_cache_serves_the_shared_frame(backend):
     assert (hits2 - hits, misses2 - misses) == (1, 1)
     st = backend.stats()
     assert st["pyramid_hits"] >= 3
+
+
+def test_special_value_cases_give_the_reference_bytes_or_refuse(backend):
+    """The refusal contract on the 44 special-value cases of the reference's retime-flow grid
+    (32 bpc frames with patches of +-0, denormals, FLT_MIN, 1, 2.5, FLT_MAX, +-inf and NaN in the
+    colours or in alpha): every case either gives the original plugin's SHA-256 or is refused
+    (status 3 from the device, or a parameter the device's host side refuses), never other bytes
+    that the engine would take as a device result (device/exec/mod.rs:345-367)."""
+    import parity_inputs as pin
+
+    from twixtor_amd._ffi import StageError
+
+    with open(os.path.join(HERE, "golden", "retime_flow_cases.json")) as fh:
+        cases = [c for c in json.load(fh) if "special_values" in c["scope"]]
+    assert len(cases) == 44
+    outcome = {}
+    for c in cases:
+        try:
+            cfg = Retime(c["width"], c["height"], c["depth"], c["fps"], c["params"], c["quality"])
+        except ValueError:
+            outcome[c["id"]] = "refused on the host"
+            continue
+        clip = pin.clip_bytes(c["source"], c["width"], c["height"], c["frames"], c["depth"])
+        try:
+            out = backend.retime_frames(cfg, clip, c["frames"], 0, c["frames"])
+        except StageError as e:
+            assert e.status == TWX_E_UNSUPPORTED, (c["id"], e)
+            outcome[c["id"]] = "refused"
+            continue
+        outcome[c["id"]] = "reference bytes" if hashlib.sha256(out).hexdigest() == c["sha256"] else "OTHER BYTES"
+    other = sorted(k for k, v in outcome.items() if v == "OTHER BYTES")
+    counts = {v: sum(1 for x in outcome.values() if x == v) for v in set(outcome.values())}
+    assert not other, f"{counts}: {other}"
+    # the oracle's count: 34 plugin hashes, 2 Weighted Blend cases over non-finite samples refused
+    assert counts == {"reference bytes": 34, "refused": 2, "refused on the host": 8}, counts

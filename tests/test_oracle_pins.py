@@ -410,13 +410,11 @@ def test_a_planted_mistake_breaks_the_hashes():
 
 # ---- the retime-time grid: the time map, the rates and the fast path at three depths ----
 
-# The reference CPU path's output differs from the original plugin's in these cases: NaN payload
-# bits of two special-value cases (graded "pass" by the reference's NaN-aware float comparison) and
-# the two 6000-frame rate cases (not graded against the original by the reference). For them the
-# pin is the reference CPU path's own recorded hash of the same case.
+# The two 6000-frame rate cases are not graded against the original by the reference: for them the
+# pin is the reference CPU path's own recorded hash of the same case. (The two Blend cases over
+# 32 bpc special values, "special32-s50-i2" and "special32-s25-i2", mix NaNs of both signs with
+# infinities; with the NaN rule of nanrule.rs restated they give the original plugin's bytes.)
 TIME_CPU_PATH_ONLY = {
-    "retime.special-special32-s50-i2.special32-5x2.d32.f42c3e54",
-    "retime.special-special32-s25-i2.special32-4x3.d32.eb6d579c",
     "retime.long6000-rate29_97.ident-1x1.d8.d634b990",
     "retime.long6000-rate29_97003.ident-1x1.d8.9b45a592",
 }
@@ -443,9 +441,10 @@ def time_case_want(c):
 def test_every_served_retime_time_hash():
     """285 cases of the reference's retime-time grid in the parameters this backend serves (Speed
     mode, constant and keyed speeds, Nearest / Blend / Weighted Blend, rates with In FPS is Out FPS
-    off, long clips, extreme integer values and float special values): 275 equal the original
-    plugin's SHA-256, 4 the reference CPU path's own (TIME_CPU_PATH_ONLY); the oracle refuses the
-    6 Weighted Blend cases over non-finite float samples, whose NaN rule it does not restate."""
+    off, long clips, extreme integer values and float special values): 277 equal the original
+    plugin's SHA-256, 2 the reference CPU path's own (TIME_CPU_PATH_ONLY); the oracle refuses the
+    6 Weighted Blend cases over non-finite float samples (blend.rs:280-300), which it does not
+    restate."""
     cases = time_cases()
     assert len(cases) == 285
     bad, refused, n_orig = [], 0, 0
@@ -460,4 +459,39 @@ def test_every_served_retime_time_hash():
             bad.append(c["id"])
         n_orig += c["id"] not in TIME_CPU_PATH_ONLY
     assert not bad, bad
-    assert refused == 6 and n_orig == 275
+    assert refused == 6 and n_orig == 277
+
+
+def test_special_value_retime_flow_hashes():
+    """The 44 special-value cases of the retime-flow grid (32 bpc patches of +-0, denormals,
+    FLT_MIN, 1, 2.5, FLT_MAX, +-inf and NaN in colour or alpha; gen.py:143-154): 8 carry a
+    parameter outside the frame-synthesis path (prep, blur compensation, 360), 2 are Weighted Blend
+    over non-finite samples (refused), and the other 34 give the original plugin's SHA-256. Six of
+    those mix NaNs with infinities, so their bytes depend on the NaN rule (nanrule.rs): a
+    processor-rule oracle (first operand wins, no emulator pick) misses them."""
+    import parity_inputs as pin
+    from twixtor_amd.params import Retime
+
+    with open(os.path.join(HERE, "golden", "retime_flow_cases.json")) as fh:
+        cases = [c for c in json.load(fh) if "special_values" in c["scope"]]
+    assert len(cases) == 44
+    ok, refused, host, bad = 0, 0, 0, []
+    for c in cases:
+        try:
+            cfg = Retime(c["width"], c["height"], c["depth"], c["fps"], c["params"], c["quality"])
+        except ValueError:
+            host += 1
+            continue
+        clip = pin.clip_bytes(c["source"], c["width"], c["height"], c["frames"], c["depth"])
+        try:
+            out = cpuref.retime_frames(cfg.struct(), clip, c["frames"], 0, c["frames"])
+        except RuntimeError as e:
+            assert "non-finite float input to the weighted blend" in str(e), (c["id"], e)
+            refused += 1
+            continue
+        if hashlib.sha256(out).hexdigest() == c["sha256"]:
+            ok += 1
+        else:
+            bad.append(c["id"])
+    assert not bad, bad
+    assert (ok, refused, host) == (34, 2, 8)

@@ -76,6 +76,37 @@ __device__ __forceinline__ int64_t qround(float x) { return cvtt64d((double)x + 
 __device__ __forceinline__ float bits_f(uint32_t b) { return __uint_as_float(b); }
 
 // -------------------------------------------------------------------------------------------
+// The reference's NaN rule (nanrule.rs:1-120): numbers never depend on it; a NaN result of an f32
+// sum, difference, product or quotient takes the one NaN operand, quieted; of two NaN operands the
+// one with the larger fraction (equal fractions: the positive one; a quiet NaN before a
+// signalling one), quieted; for an invalid operation (inf - inf, 0 * inf, 0 / 0) the x86 default
+// NaN 0xffc00000. The f32 kernels compute plainly and repeat a result with these operations only
+// where it holds a NaN (a branch no finite pixel takes), as the reference's call sites do.
+
+__device__ __forceinline__ float nr_quiet(float x) { return __uint_as_float(__float_as_uint(x) | 0x00400000u); }
+__device__ __forceinline__ float nr_pick(float a, float b) {
+    const uint32_t ba = __float_as_uint(a), bb = __float_as_uint(b);
+    const bool sa = !(ba & 0x00400000u), sb = !(bb & 0x00400000u);
+    const uint32_t fa = ba & 0x007fffffu, fb = bb & 0x007fffffu;
+    const bool a_larger = fa > fb || (fa == fb && (ba >> 31) < (bb >> 31));
+    const bool take_b = (sa == sb) ? !a_larger : sa;
+    return nr_quiet(take_b ? b : a);
+}
+__device__ __forceinline__ float nr_fix(float r, float a, float b) {
+    if (!isnan(r)) return r;
+    if (isnan(a)) return isnan(b) ? nr_pick(a, b) : nr_quiet(a);
+    if (isnan(b)) return nr_quiet(b);
+    return __uint_as_float(0xffc00000u);
+}
+__device__ __forceinline__ float nr_add(float a, float b) { return nr_fix(a + b, a, b); }
+__device__ __forceinline__ float nr_sub(float a, float b) { return nr_fix(a - b, a, b); }
+__device__ __forceinline__ float nr_mul(float a, float b) { return nr_fix(a * b, a, b); }
+__device__ __forceinline__ float nr_div(float a, float b) { return nr_fix(a / b, a, b); }
+// lerp (nanrule.rs:240): (b - a) * f + a; lerp_base_first (:247): a + (b - a) * f
+__device__ __forceinline__ float nr_lerp(float a, float b, float f) { return nr_add(nr_mul(nr_sub(b, a), f), a); }
+__device__ __forceinline__ float nr_lerp_bf(float a, float b, float f) { return nr_add(a, nr_mul(nr_sub(b, a), f)); }
+
+// -------------------------------------------------------------------------------------------
 // Wire -> engine image (image.rs:173-203, frame.rs:79-111, image.rs:264-298)
 
 template <int FMT>
@@ -109,6 +140,8 @@ __device__ __forceinline__ void ingest_pixel(const uint8_t* __restrict__ wire, u
         const float* pl = reinterpret_cast<const float*>(wire);  // planes G, B, R, A
         float g = pl[si], b = pl[n + si], r = pl[2 * n + si], a = pl[3 * n + si];
         float4 o = make_float4(a, a * r, a * g, a * b);
+        if (isnan(o.x) || isnan(o.y) || isnan(o.z) || isnan(o.w))  // image.rs:291-300, the NaN rule
+            o = make_float4(a, nr_mul(a, r), nr_mul(a, g), nr_mul(a, b));
         reinterpret_cast<float4*>(engine)[di] = o;
     }
 }
@@ -1610,6 +1643,12 @@ __device__ __forceinline__ void bilerp_src(const void* src, int W, int H, float 
         float bot = p01[k] + (p11[k] - p01[k]) * ax;
         o[k] = top + (bot - top) * ay;
     }
+    if (FMT == 2 && (isnan(o[0]) || isnan(o[1]) || isnan(o[2]) || isnan(o[3]))) {
+        // warp.rs:334-343: the same lerps again with the NaN rule
+#pragma unroll
+        for (int k = 0; k < 4; k++)
+            o[k] = nr_lerp_bf(nr_lerp_bf(p00[k], p10[k], ax), nr_lerp_bf(p01[k], p11[k], ax), ay);
+    }
 }
 
 // One warped pixel W(src, flow, amount) at output (x, y) with hold edges (warp.rs:500-512), as the
@@ -1687,6 +1726,15 @@ __device__ __forceinline__ void over4(const float a[4], const float b[4], float 
     for (int k = 1; k < 4; k++) o[k] = b[k] * ia + a[k];
 }
 
+// over4 with the NaN rule's operations
+__device__ __forceinline__ void over4_nr(const float a[4], const float b[4], float white, float o[4]) {
+    if (a[0] == 0.0f) { for (int k = 0; k < 4; k++) o[k] = b[k]; return; }
+    if (a[0] == white) { for (int k = 0; k < 4; k++) o[k] = a[k]; return; }
+    float ia = nr_sub(1.0f, nr_div(a[0], white));
+    o[0] = nr_add(nr_mul(nr_sub(white, b[0]), nr_div(a[0], white)), b[0]);
+    for (int k = 1; k < 4; k++) o[k] = nr_add(nr_mul(b[k], ia), a[k]);
+}
+
 // One pixel of the Smart Blend composite (smartblend.rs:193-212) with its stores
 // (smartblend.rs:84-150): the main path `+ 0.5` and saturation (integers) or `+ 0.0` (float), the
 // "alpha is 0" fallback the plain truncation (integers) or the value (float).
@@ -1700,6 +1748,15 @@ __device__ __forceinline__ void smart_px(const float p1[4], const float p2[4], f
     float amax = p1[0] > p2[0] ? p1[0] : p2[0];
 #pragma unroll
     for (int k = 0; k < 4; k++) v[k] = (c2[k] - c1[k]) * w + c1[k];
+    // a NaN in the composite (float frames only): the arithmetic again with the NaN rule's
+    // operations, under which the references were rendered (nanrule.rs:14-21)
+    const bool nr = FMT == 2 && (isnan(v[0]) || isnan(v[1]) || isnan(v[2]) || isnan(v[3]));
+    if (nr) {
+        over4_nr(p1, p2, white, c1);
+        over4_nr(p2, p1, white, c2);
+#pragma unroll
+        for (int k = 0; k < 4; k++) v[k] = nr_lerp(c1[k], c2[k], w);
+    }
     bool raw = false;
     if (v[0] != amax) {
         if (v[0] == 0.0f) {
@@ -1707,7 +1764,7 @@ __device__ __forceinline__ void smart_px(const float p1[4], const float p2[4], f
             for (int k = 0; k < 4; k++) v[k] = c1[0] > c2[0] ? c1[k] : c2[k];
         } else {
             float alpha = v[0];
-            for (int k = 1; k < 4; k++) v[k] = (v[k] * amax) / alpha;
+            for (int k = 1; k < 4; k++) v[k] = nr ? nr_div(nr_mul(v[k], amax), alpha) : (v[k] * amax) / alpha;
             v[0] = amax;
         }
     }
@@ -1723,7 +1780,17 @@ __device__ __forceinline__ void smart_px(const float p1[4], const float p2[4], f
 // lerp of one pixel, weight on the first (blend.rs:180-257)
 template <int FMT>
 __device__ __forceinline__ void lerp_pixel(float f[4], const float s[4], float w) {
-    if (FMT == 2) {
+    if (FMT == 2 && (isnan(f[0]) || isnan(f[1]) || isnan(f[2]) || isnan(f[3]) || isnan(s[0]) || isnan(s[1]) ||
+                     isnan(s[2]) || isnan(s[3]))) {
+        // blend.rs:181-216: a NaN among the inputs, the arithmetic with the NaN rule
+        float a = nr_lerp(s[0], f[0], w);
+        if (!(a > 0.0f)) { f[0] = f[1] = f[2] = f[3] = 0.0f; return; }
+        float r = nr_lerp(s[1], f[1], w);
+        float g = nr_lerp(s[2], f[2], w);
+        float b = nr_lerp(s[3], f[3], w);
+        if (a > 1.0f) { r = nr_div(r, a); g = nr_div(g, a); b = nr_div(b, a); a = 1.0f; }
+        f[1] = nr_add(r, 0.0f); f[2] = nr_add(g, 0.0f); f[3] = nr_add(b, 0.0f); f[0] = nr_add(a, 0.0f);
+    } else if (FMT == 2) {
         float a = (f[0] - s[0]) * w + s[0];
         if (!(a > 0.0f)) { f[0] = f[1] = f[2] = f[3] = 0.0f; return; }
         float r = (f[1] - s[1]) * w + s[1];
@@ -1765,8 +1832,12 @@ __device__ __forceinline__ void blend_pixel(const float p[4], const float q[4], 
     const int64_t mask = FMT == 0 ? 0xff : 0xffff;
 #pragma unroll
     for (int k = 0; k < 4; k++) {
-        if (FMT == 2) o[k] = ((q[k] - p[k]) * w + p[k]) + 0.0f;
-        else o[k] = (float)(cvtt64f(((q[k] - p[k]) * w + p[k]) + 0.5f) & mask);
+        if (FMT == 2) {
+            o[k] = ((q[k] - p[k]) * w + p[k]) + 0.0f;
+            if (isnan(o[k])) o[k] = nr_add(nr_lerp(p[k], q[k], w), 0.0f);  // blend.rs:105-110
+        } else {
+            o[k] = (float)(cvtt64f(((q[k] - p[k]) * w + p[k]) + 0.5f) & mask);
+        }
     }
 }
 
@@ -1807,6 +1878,8 @@ __device__ __forceinline__ void finish_pixel(void* out, long n, long i, const fl
             float inv = 1.0f / e[0];
             o[0] = e[0];
             for (int c = 1; c < 4; c++) o[c] = e[c] * inv;
+            if (isnan(e[0]) || isnan(o[1]) || isnan(o[2]) || isnan(o[3]))  // image.rs:344-348
+                for (int c = 1; c < 4; c++) o[c] = nr_mul(e[c], inv);
         }
         float* pl = reinterpret_cast<float*>(out);  // planes G, B, R, A
         pl[i] = o[2];
