@@ -86,6 +86,29 @@ typedef struct twx_retime_params {
     const twx_key* speed_keys;
 } twx_retime_params;
 
+/* The parameters of one `vfx apply vectors-in` run (plugins/vectors-in/src/lib.rs:82-130 for ids
+ * and defaults; glue.rs:190-253 for the derivation). Constant values; keyframes only for Speed %. */
+typedef struct twx_vin_params {
+    int32_t width, height, fmt;  /* frame size and wire depth code */
+    int32_t time_step;           /* ticks per output frame (also the layers' frame duration) */
+    double fps;                  /* composition rate */
+    int32_t frame_interp;        /* "Frame Interp" 1..3, default 2 (Blend) */
+    int32_t warping;             /* "Warping" 1 Inverse, 2 Forward (default) */
+    int32_t display;             /* "Display" 1..6, default 1; 2 shows the retimed layer */
+    int32_t mark_segments;       /* "Src: Mark Segments" 1..5, default 1 */
+    int32_t time_remap_mode;     /* 1 Speed (default), 2 Frame Number */
+    double speed;                /* "Speed %" constant, used when n_speed_keys == 0 */
+    int32_t n_speed_keys;
+    const twx_key* speed_keys;
+    double frame_num;            /* "Frame Num" */
+    int32_t in_fps_is_out_fps;   /* checkbox, default 1 (input 30 fps) */
+    double input_frame_rate;     /* slider, used when in_fps_is_out_fps == 0 */
+    double max_displace;         /* "Vectors: Max Displace", default 2048 */
+    double vec_scale_x, vec_scale_y;
+    double blur;                 /* "Motion Blur Compensation"; non-zero is refused */
+    int32_t quality_high;
+} twx_vin_params;
+
 /* The frame selection of one output frame (plugins/retime-engine/src/timeline.rs:359-380). */
 typedef struct twx_selection {
     float s0, s1;

@@ -2640,6 +2640,197 @@ static int render_frame(const twx_retime_params& p, const Clip& c, int32_t n, ui
     return TWX_OK;
 }
 
+// ------------------------------------------------------------------------------------------
+// The Vectors In render (plugins/vectors-in/src/glue.rs:380-495, vectorin.rs:1-230,
+// timemap.rs:1-70): the time map of that effect, the flows from its two vector layers, the
+// core of compose.rs:122-130 with the forward warps of stages::GouraudForward and the taps above
+// first (glue.rs:253). Every frame a render reads is turned upside down before the render sees it
+// and the result is turned back (glue.rs:398-410).
+
+extern "C" int cref_compose_vectors(int interp, float wa, const void* a, const void* b, int w, int h, int fmt,
+                                    const float* f_ab, const float* f_ba, int fw, int fh, int warp_mode, int above_first,
+                                    int gouraud, void* out, char* err, size_t errlen);
+
+// `vfx apply vectors-in` parameters (plugins/vectors-in/src/lib.rs:82-130), the struct of the C ABI
+using cref_vin_params = twx_vin_params;
+
+// One layer of a render: wire frames, frame i showing at ticks [i * time_step, (i + 1) * time_step)
+// (crates/fx-cli/src/temporal.rs:21-23); `wire == nullptr` is a layer that is not set.
+struct VinLayer {
+    const uint8_t* wire;
+    int32_t n;
+};
+
+// The stage set and the row order of a render: the effect's glue installs the Gouraud forward
+// warp and the taps above first and turns every frame upside down (glue.rs:57-75, :253, :398-410):
+// {true, true, true}. The engine's plain stage set on rows as given, {false, false, false}, is the
+// configuration the reference CPU path recorded its grid hashes with (results/base.grid.tsv).
+struct VinStages {
+    bool gouraud, above_first, flip;
+};
+
+// The layer's frame at `time`, premultiplied (rows reversed with `flip`); transparent outside the
+// clip (clip.rs:170-172, host.rs:123-127).
+static std::vector<uint8_t> vin_frame(const VinLayer& L, long w, long h, int fmt, int32_t lts, int64_t time, bool flip) {
+    const size_t fb = image_bytes(w, h, fmt);
+    std::vector<uint8_t> img(fb, 0), up(fb, 0);
+    const int64_t idx = (int64_t)std::floor((double)time / (double)lts);  // TimeBase::frame_of
+    if (L.wire && idx >= 0 && idx < L.n) from_frame(L.wire + (size_t)idx * fb, w, h, fmt, img.data());
+    if (!flip) return img;
+    rows_reversed(img.data(), w, h, fmt, up.data());
+    return up;
+}
+
+static int vin_render_frame(const cref_vin_params& p, const VinLayer& input, const VinLayer& source, const VinLayer& prev,
+                            const VinLayer& next, const VinStages& vs, int32_t n, uint8_t* out_wire, std::string& err) {
+    const long w = p.width, h = p.height;
+    const int fmt = p.fmt;
+    const size_t bytes = image_bytes(w, h, fmt);
+    const VinLayer& layer = source.wire ? source : input;  // the retimed layer (glue.rs:86-89 IDS)
+    // the rates (glue.rs:200-209): input 30 fps with "In FPS is Out FPS", else the slider; output the
+    // composition's; Rates::new without its equal-rates switch (timeline.rs:108-133)
+    double in_fps = p.in_fps_is_out_fps ? 30.0 : (double)((float)p.input_frame_rate * 1.0f);
+    double out_fps = p.fps;
+    in_fps = snap(in_fps);
+    out_fps = snap(out_fps);
+    const bool rate_error = 0.01 > out_fps || 0.01 >= in_fps;
+    // the probes (glue.rs:268-300): neither vector layer is the size mismatch fill, as a rate error
+    if (rate_error || (!prev.wire && !next.wire)) {
+        const size_t ss = sample_size(fmt);
+        for (long i = 0; i < w * h; i++) {  // image.rs:391-425: red at half alpha, R, G, B, A order
+            uint8_t* q = out_wire;
+            if (fmt == TWX_FMT_U8) {
+                q[i * 4 + 0] = 0xff; q[i * 4 + 1] = 0; q[i * 4 + 2] = 0; q[i * 4 + 3] = 0x80;
+            } else if (fmt == TWX_FMT_U16) {
+                uint16_t v[4] = {q15_to_u16(0x8000), 0, 0, q15_to_u16(0x4000)};
+                memcpy(q + i * 8, v, 8);
+            } else {
+                float* pl = (float*)q;  // planes G, B, R, A
+                pl[i] = 0.0f; pl[w * h + i] = 0.0f; pl[2 * w * h + i] = 1.0f; pl[3 * w * h + i] = 0.5f;
+            }
+            (void)ss;
+        }
+        return TWX_OK;
+    }
+    Rates r{in_fps, out_fps, (float)(((double)p.time_step * out_fps) / in_fps), p.time_step};
+    const int32_t cur = n * p.time_step;
+    if (p.display == 2) {  // the retimed layer at current_time through the prepare and the finish
+        std::vector<uint8_t> img = vin_frame(layer, w, h, fmt, p.time_step, cur, false);
+        to_frame(img.data(), w, h, fmt, out_wire);
+        return TWX_OK;
+    }
+    if (p.blur != 0.0) { err = "oracle: the blur compensation tail is not restated"; return TWX_E_UNSUPPORTED; }
+    // the time map (timemap.rs:56-70): src_pos of RemapTimeMap without the source time offset,
+    // T(k) = trunc((f64)dur * (f64)k + 0.5)
+    twx_retime_params rp{};
+    rp.time_step = p.time_step;
+    rp.speed = p.speed;
+    rp.n_speed_keys = p.n_speed_keys;
+    rp.speed_keys = p.speed_keys;
+    auto src_pos_at = [&](float t, float& s) -> int {
+        if (p.time_remap_mode == 2) {  // timeline.rs:274-279, a constant Frame Num
+            const float v = (float)p.frame_num;
+            s = (float)((r.in_fps / r.out_fps) * (double)v);
+            return TWX_OK;
+        }
+        return src_pos(rp, r, t, s, err);
+    };
+    auto T = [&](float k) -> int64_t { return cvtt_i32d((double)r.dur * (double)k + 0.5); };
+    const float curf = (float)cur, ltsf = (float)p.time_step;
+    const float t0 = curf / ltsf, t1 = ((float)p.time_step + curf) / ltsf;
+    float s0, s1;
+    int st;
+    if ((st = src_pos_at(t0, s0))) return st;
+    if ((st = src_pos_at(t1, s1))) return st;
+    // the constant VECTORS_QUALITY 4 (prerender.rs:50), 0 for a frame with a side of 1
+    int vq = (w >= 2 && h >= 2) ? 4 : 0;
+    int interp = p.frame_interp - 1;
+    // the segment override (timeline.rs:418-440) with a constant Mark Segments: both markers equal
+    const int mark = (p.mark_segments >= 1 && p.mark_segments <= 5) ? p.mark_segments - 1 : 0;
+    if (mark == 3) { vq = 0; interp = 0; }
+    if (mark == 4) { vq = 0; interp = 1; }
+    const int32_t a = cvtt_i32f(s0);
+    const int32_t b = (int32_t)((uint32_t)a + 1u);
+    const float wa = 1.0f - (s0 - (float)a);
+    const bool exact = exact_position(s0, s1, (float)p.blur);
+    std::vector<uint8_t> out(bytes, 0);
+    if (vq != 0 && !exact) {
+        if (interp < 0 || interp > 2) { err = "frame_interp outside 1..3"; return TWX_E_MISMATCH; }
+        const int warp_mode = p.warping == 2 ? 2 : 0;  // glue.rs:96-103
+        // the loaders (vectorin.rs:200-225): Max Displace read once, Vec Scale at the load's time
+        const float md = (1.0f / 1.0f) * fixed_reader(float_to_fixed(p.max_displace), 1.0f);
+        const float sx = fixed_reader(float_to_fixed(p.vec_scale_x), 1.0f);
+        const float sy = fixed_reader(float_to_fixed(p.vec_scale_y), 1.0f);
+        const size_t fn = (size_t)w * h * 2;
+        std::vector<float> f_ab, f_ba;
+        if (next.wire) {
+            std::vector<uint8_t> vec = vin_frame(next, w, h, fmt, p.time_step, T((float)a), vs.flip);
+            f_ab.assign(fn, 0.0f);
+            if ((st = flow_from_vector_image(vec.data(), w, h, fmt, md, sx, sy, vs.above_first, f_ab.data(), err))) return st;
+        }
+        if (prev.wire) {
+            std::vector<uint8_t> vec = vin_frame(prev, w, h, fmt, p.time_step, T((float)b), vs.flip);
+            f_ba.assign(fn, 0.0f);
+            if ((st = flow_from_vector_image(vec.data(), w, h, fmt, md, sx, sy, vs.above_first, f_ba.data(), err))) return st;
+        }
+        // complete_flows (compose/vector.rs:430-456): the missing flow is the other's inversion, made
+        // only when the composition reads it
+        bool need_ab = true, need_ba = true;
+        if (interp == 0) {
+            const bool ab = (wa < 0.5f && warp_mode < 2) || (0.5f <= wa && warp_mode > 1);  // nearest_uses_f_ab
+            need_ab = ab;
+            need_ba = !ab;
+        }
+        if (f_ab.empty() && need_ab) {
+            f_ab.assign(fn, 0.0f);
+            if ((st = invert_flow(f_ba.data(), w, h, vs.gouraud, vs.above_first, f_ab.data(), err))) return st;
+        }
+        if (f_ba.empty() && need_ba) {
+            f_ba.assign(fn, 0.0f);
+            if ((st = invert_flow(f_ab.data(), w, h, vs.gouraud, vs.above_first, f_ba.data(), err))) return st;
+        }
+        if (f_ab.empty()) f_ab.assign(fn, 0.0f);  // not read by this composition
+        if (f_ba.empty()) f_ba.assign(fn, 0.0f);
+        std::vector<uint8_t> ia = vin_frame(layer, w, h, fmt, p.time_step, T((float)a), vs.flip);
+        std::vector<uint8_t> ib = vin_frame(layer, w, h, fmt, p.time_step, T((float)b), vs.flip);
+        char e[512] = {0};
+        st = cref_compose_vectors(interp, wa, ia.data(), ib.data(), (int)w, (int)h, fmt, f_ab.data(), f_ba.data(), (int)w,
+                                  (int)h, warp_mode, vs.above_first ? 1 : 0, vs.gouraud ? 1 : 0, out.data(), e, sizeof e);
+        if (st) { err = e; return st; }
+    } else {
+        std::vector<uint8_t> ia = vin_frame(layer, w, h, fmt, p.time_step, T((float)a), vs.flip);
+        std::vector<uint8_t> ib = vin_frame(layer, w, h, fmt, p.time_step, T((float)b), vs.flip);
+        if (interp == 0) {
+            memcpy(out.data(), wa >= 0.5f ? ia.data() : ib.data(), bytes);
+        } else if (interp == 1 || interp == 2) {
+            // Blend, and Weighted Blend without flows (blend.rs:280-300: the plain Blend with
+            // 1 - wa in f32, the float inputs plus 0.0; non-finite or huge samples not restated)
+            const float t = interp == 1 ? (float)(1.0 - (double)wa) : 1.0f - wa;
+            if (interp == 2 && fmt == TWX_FMT_F32) {
+                float* fa = (float*)ia.data();
+                float* fb = (float*)ib.data();
+                for (long i = 0; i < w * h * 4; i++) {
+                    if (!std::isfinite(fa[i]) || !std::isfinite(fb[i]) || std::fabs(fa[i]) > 1e30f || std::fabs(fb[i]) > 1e30f) {
+                        err = "oracle: non-finite float input to the weighted blend is not restated";
+                        return TWX_E_UNSUPPORTED;
+                    }
+                    fa[i] = fa[i] + 0.0f;
+                    fb[i] = fb[i] + 0.0f;
+                }
+            }
+            blend(out.data(), ia.data(), ib.data(), w, h, fmt, t);
+        } else {
+            err = "frame_interp outside 1..3";
+            return TWX_E_MISMATCH;
+        }
+    }
+    std::vector<uint8_t> back(bytes);
+    if (vs.flip) rows_reversed(out.data(), w, h, fmt, back.data());
+    else back = out;
+    to_frame(back.data(), w, h, fmt, out_wire);
+    return TWX_OK;
+}
+
 }  // namespace cref
 
 // ==========================================================================================
@@ -2994,6 +3185,24 @@ int cref_retime_frames(const twx_retime_params* p, const uint8_t* clip, int n_in
     for (int i = 0; i < count; i++) {
         std::string e;
         int st = render_frame(*p, c, first + i, out + (size_t)i * fb, e);
+        if (st) { set_err(err, errlen, "frame " + std::to_string(first + i) + ": " + e); return st; }
+    }
+    return TWX_OK;
+}
+
+// Output frames [first, first + count) of `vfx apply vectors-in`: `input` the footage, `source`
+// the Color Source layer, `prev` and `next` the vector layers (a null pointer: the layer is not
+// set), each with its frame count; gouraud, above_first, flip: the VinStages of the render (the
+// effect's own: 1, 1, 1).
+int cref_vectors_in_frames(const cref_vin_params* p, const uint8_t* input, int n_input, const uint8_t* source, int n_source,
+                           const uint8_t* prev, int n_prev, const uint8_t* next, int n_next, int gouraud, int above_first,
+                           int flip, int first, int count, uint8_t* out, char* err, size_t errlen) {
+    const size_t fb = image_bytes(p->width, p->height, p->fmt);
+    VinLayer li{input, n_input}, ls{source, n_source}, lp{prev, n_prev}, ln{next, n_next};
+    const VinStages vs{gouraud != 0, above_first != 0, flip != 0};
+    for (int i = 0; i < count; i++) {
+        std::string e;
+        int st = vin_render_frame(*p, li, ls, lp, ln, vs, first + i, out + (size_t)i * fb, e);
         if (st) { set_err(err, errlen, "frame " + std::to_string(first + i) + ": " + e); return st; }
     }
     return TWX_OK;

@@ -492,6 +492,37 @@ def frame_index(params: RetimeParams, pos: int, n_in: int) -> int:
     return lib().cref_frame_index(ctypes.byref(params), pos, n_in)
 
 
+VIN_EFFECT = (True, True, True)  # the effect's own stages: Gouraud forward warp, taps above first, rows flipped
+VIN_PLAIN = (False, False, False)  # the engine's plain stage set on rows as given
+
+
+def vectors_in_frames(params, input_clip: bytes, n_input: int, source=None, prev=None, nxt=None, first: int = 0,
+                      count: int = 1, stages=VIN_EFFECT) -> bytes:
+    """Output frames [first, first + count) of `vfx apply vectors-in` (`params` a twx_vin_params
+    structure); `source`, `prev` and `nxt` are (clip bytes, frame count) or None for a layer that is
+    not set; `stages` = (Gouraud forward warp, taps above first, rows flipped). Raises on a
+    non-zero status."""
+    fb = params.width * params.height * 4 * {FMT_U8: 1, FMT_U16: 2, FMT_F32: 4}[params.fmt]
+    keep = []
+
+    def layer(x):
+        if x is None:
+            return None, 0
+        a = np.frombuffer(x[0], np.uint8)
+        keep.append(a)
+        return _p(a), x[1]
+
+    src = np.frombuffer(input_clip, np.uint8)
+    ls, lp, ln = layer(source), layer(prev), layer(nxt)
+    out = np.zeros(fb * count, np.uint8)
+    err = ctypes.create_string_buffer(512)
+    st = lib().cref_vectors_in_frames(ctypes.byref(params), _p(src), n_input, ls[0], ls[1], lp[0], lp[1], ln[0], ln[1],
+                                      int(stages[0]), int(stages[1]), int(stages[2]), first, count, _p(out), err, 512)
+    if st:
+        raise RuntimeError(f"status {st}: {err.value.decode()}")
+    return out.tobytes()
+
+
 def retime_frames(params: RetimeParams, clip: bytes, n_in: int, first: int, count: int) -> bytes:
     """Output frames [first, first + count) of `vfx apply retime`; raises on a non-zero status."""
     fb = params.width * params.height * 4 * {FMT_U8: 1, FMT_U16: 2, FMT_F32: 4}[params.fmt]

@@ -263,6 +263,63 @@ def test_gouraud_variant_interpolates_the_vertex_colours():
     assert np.array_equal(out, src.reshape(-1))
 
 
+def _affine_image(w, h):
+    """An opaque float image whose colours are affine in the pixel position: channel k is
+    a_k + b_k x + c_k y (alpha 1)."""
+    coef = np.array([[0.20, 0.011, 0.023], [0.70, -0.017, 0.009], [0.35, 0.006, -0.013]])
+    yy, xx = np.mgrid[0:h, 0:w].astype(np.float64)
+    img = np.ones((h, w, 4))
+    for k in range(3):
+        img[..., k + 1] = coef[k, 0] + coef[k, 1] * xx + coef[k, 2] * yy
+    return img.astype(np.float32), coef
+
+
+def _affine_at(coef, x, y):
+    return np.stack([coef[k, 0] + coef[k, 1] * x + coef[k, 2] * y for k in range(3)], -1)
+
+
+@pytest.mark.parametrize("kind,p", [("shift", (2.0, 1.0)), ("shift", (1.5, -0.75)), ("shift", (0.3, 2.6)),
+                                    ("shift", (-3.25, 0.5)), ("affine", (0.06, 1.04)), ("affine", (-0.1, 0.93))])
+def test_gouraud_reproduces_an_affine_image_under_an_affine_motion(kind, p):
+    """Closed form of the Gouraud variant: the vertex colours of an affine image interpolated over
+    the displaced mesh of an affine motion give, at every covered pixel q, the colour of the source
+    point that lands on q -- for a translation t the colour at q - t, for a rotation by r and a
+    scale s about the centre the colour at the inverse image of q. Exact but for float rounding
+    and the mesh's tie epsilon of 1e-4 px (forward/mesh.rs:33-52), so the bound is 3e-4 px of
+    colour slope plus rounding. A slip in the interpolation (weights of the wrong vertices or
+    edge, 1 - t for t, a dropped term) moves the colours by a fraction of a pixel's slope and
+    fails it; so does a running mean over the hits of a pixel that mixes another triangle's colour."""
+    w, h = 26, 19
+    src, coef = _affine_image(w, h)
+    yy, xx = np.mgrid[0:h, 0:w].astype(np.float64)
+    cx, cy = (w - 1) / 2.0, (h - 1) / 2.0
+    if kind == "shift":
+        flow = const_flow(w, h, p)
+        inv_x, inv_y = xx - p[0], yy - p[1]
+    else:
+        r, s = p
+        ca, sa = np.cos(r), np.sin(r)
+        dx, dy = xx - cx, yy - cy
+        fx = cx + s * (ca * dx - sa * dy) - xx
+        fy = cy + s * (sa * dx + ca * dy) - yy
+        flow = np.stack([fx, fy], -1).astype(np.float32)
+        # the inverse motion of an output pixel
+        ox, oy = (xx - cx) / s, (yy - cy) / s
+        inv_x, inv_y = cx + ca * ox + sa * oy, cy - sa * ox + ca * oy
+    st, out, cov = C.forward_warp(src.reshape(-1), w, h, 2, flow, w, h, 1.0, pyramid=False)
+    assert st == 0
+    out = out.reshape(h, w, 4)
+    covered = cov.reshape(h, w) == 255
+    inside = (inv_x >= 0.5) & (inv_x <= w - 1.5) & (inv_y >= 0.5) & (inv_y <= h - 1.5)
+    assert covered[inside].all()  # every pixel whose source point lies well inside is covered
+    want = _affine_at(coef, inv_x, inv_y)
+    err = np.abs(out[..., 1:].astype(np.float64) - want)[covered]
+    slope = np.abs(coef[:, 1:]).sum(1)
+    assert (err <= 3e-4 * slope + 2e-6).all(), (kind, p, float((err - 3e-4 * slope).max()))
+    assert (out[..., 0][covered] == 1.0).all()
+    assert covered.sum() >= 0.6 * w * h
+
+
 def test_shape_errors_and_hostile_flows():
     src = textured(6, 6, 0, 1)
     st, _, _ = C.forward_warp(textured(1, 6, 0, 1), 1, 6, 0, const_flow(6, 6, (1, 0)), 6, 6, 1.0)

@@ -152,3 +152,101 @@ class Retime:
 
     def frame_bytes(self) -> int:
         return self.width * self.height * 4 * (self.depth // 8)
+
+
+class VinParams(ctypes.Structure):
+    """twx_vin_params of include/twx.h."""
+    _fields_ = [
+        ("width", ctypes.c_int32),
+        ("height", ctypes.c_int32),
+        ("fmt", ctypes.c_int32),
+        ("time_step", ctypes.c_int32),
+        ("fps", ctypes.c_double),
+        ("frame_interp", ctypes.c_int32),
+        ("warping", ctypes.c_int32),
+        ("display", ctypes.c_int32),
+        ("mark_segments", ctypes.c_int32),
+        ("time_remap_mode", ctypes.c_int32),
+        ("speed", ctypes.c_double),
+        ("n_speed_keys", ctypes.c_int32),
+        ("speed_keys", ctypes.POINTER(Key)),
+        ("frame_num", ctypes.c_double),
+        ("in_fps_is_out_fps", ctypes.c_int32),
+        ("input_frame_rate", ctypes.c_double),
+        ("max_displace", ctypes.c_double),
+        ("vec_scale_x", ctypes.c_double),
+        ("vec_scale_y", ctypes.c_double),
+        ("blur", ctypes.c_double),
+        ("quality_high", ctypes.c_int32),
+    ]
+
+
+# parameter id -> default of the vectors-in effect (plugins/vectors-in/src/lib.rs:82-130); the
+# FG blocks must stay at their defaults (their matte layers are not served)
+_VIN_DEFAULTS = {
+    "display": 1, "src_mark_segments": 1, "vectors_max_displace": 2048.0, "vec_scale_x": 1.0, "vec_scale_y": 1.0,
+    "in_fps_is_out_fps": 1, "input_frame_rate": float(__import__("numpy").float32(29.97)), "time_remap_mode": 1,
+    "speed_percent": 100.0, "frame_num": 0.0, "frame_interp": 2, "warping": 2, "blur_compensation": 0.0,
+}
+_VIN_FG = {"fg1_matte_chan": 1.0, "fg2_matte_chan": 1.0, "fg3_matte_chan": 1.0, "fg1_inv_matte_shrink": 0.0,
+           "fg2_inv_matte_shrink": 0.0, "fg3_inv_matte_shrink": 0.0}
+
+
+@dataclass
+class VectorsIn:
+    """One `vfx apply vectors-in` configuration. `params` are the `--param id=value` strings;
+    keyframes are taken for Speed % only (ValueError for other keyed parameters)."""
+    width: int
+    height: int
+    depth: int
+    fps: str = "30"
+    params: list = field(default_factory=list)
+    quality: str = "high"
+
+    def __post_init__(self):
+        self.time_scale, self.time_step, self.fps_f64 = time_base(self.fps)
+        vals = dict(_VIN_DEFAULTS)
+        self.keys = []
+        for p in self.params:
+            k, v = p.split("=", 1)
+            if k == "speed_percent" and "@" in v:
+                self.keys = _parse_keys(v, self.time_step)
+            elif "@" in v:
+                raise ValueError(f"keyframes of {k} are not taken")
+            elif k == "in_fps_is_out_fps":
+                vals[k] = {"1": 1, "true": 1, "on": 1, "0": 0, "false": 0, "off": 0}[v]
+            elif k in _VIN_DEFAULTS:
+                vals[k] = float(v)
+            elif k in _VIN_FG:
+                if float(v) != _VIN_FG[k]:
+                    raise ValueError(f"parameter {k}={v} needs the FG matte layers, which are not served")
+            else:
+                raise ValueError(f"unknown parameter {k}")
+        for k in ("display", "src_mark_segments", "time_remap_mode", "frame_interp", "warping"):
+            if vals[k] != int(vals[k]):
+                raise ValueError(f"popup {k}={vals[k]}")
+        self.vals = vals
+
+    def struct(self) -> VinParams:
+        v = self.vals
+        p = VinParams()
+        p.width, p.height, p.fmt = self.width, self.height, FMT_CODE[self.depth]
+        p.time_step, p.fps = self.time_step, self.fps_f64
+        p.frame_interp, p.warping, p.display = int(v["frame_interp"]), int(v["warping"]), int(v["display"])
+        p.mark_segments, p.time_remap_mode = int(v["src_mark_segments"]), int(v["time_remap_mode"])
+        p.speed, p.frame_num = v["speed_percent"], v["frame_num"]
+        p.in_fps_is_out_fps, p.input_frame_rate = int(v["in_fps_is_out_fps"]), v["input_frame_rate"]
+        p.max_displace, p.vec_scale_x, p.vec_scale_y = v["vectors_max_displace"], v["vec_scale_x"], v["vec_scale_y"]
+        p.blur = v["blur_compensation"]
+        p.quality_high = 1 if self.quality == "high" else 0
+        if self.keys:
+            arr = (Key * len(self.keys))(*[Key(t, x, h, e) for t, x, h, e in self.keys])
+            p.n_speed_keys = len(self.keys)
+            p.speed_keys = ctypes.cast(arr, ctypes.POINTER(Key))
+            p._keys_keepalive = arr  # noqa: SLF001 -- keeps the array alive with the struct
+        else:
+            p.n_speed_keys = 0
+        return p
+
+    def frame_bytes(self) -> int:
+        return self.width * self.height * 4 * (self.depth // 8)
