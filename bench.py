@@ -29,6 +29,52 @@ SPEED = "speed_percent=100@0;25@60;100@120"
 # counter runs (copy-calibrated) recorded in profiles/retime_1080p_hbm_traffic_pmc.md.
 TRAFFIC_RATIO = {"bm_rows": 1.42, "bm_cols": 1.28, "calc_b_gauss": 1.05, "new_disp_gauss": 1.05, "tensor_gauss": 1.02,
                  "frames": 1.03, "bm_transpose": 1.30}
+# The share of the SIMDs' issue slots the family's VALU instructions fill over its time, from the
+# SQ_INSTS_VALU counter run recorded in the same profile.
+VALU_SHARE = {"calc_b_gauss": 0.71, "new_disp_gauss": 0.58, "tensor_gauss": 0.71, "frames": 0.62, "bm_rows": 0.60,
+              "bm_cols": 0.71}
+# The library counts a fused kernel's bytes once (its inputs read once, its outputs written once).
+# The step-boundary model of the CPU path counts every step's inputs and outputs, 4 bytes a float:
+# grad + tensor 24 and the tensor blur 48 B/px (fused: 24), calc_b 32 and the B blur 32 (fused: 32),
+# new_disp 36 and the flow blur 32 (fused: 36); the block-matching transposes are layout work the
+# CPU path does not have.
+STEP_MODEL_FACTOR = {"tensor_gauss": 72.0 / 24.0, "calc_b_gauss": 64.0 / 32.0, "new_disp_gauss": 68.0 / 36.0,
+                     "bm_transpose": 0.0}
+FLOW_FAMILIES = ("pyr_luma_blur5_band_resample", "tensor_gauss", "calc_b_gauss", "new_disp_gauss", "bm_rows", "bm_cols",
+                 "bm_transpose", "range", "upsample", "negate", "solve_init")
+WARP_FAMILIES = ("frames",)
+STAGE_TARGET = 0.60  # of the HBM peak, for the flow and the warp stages
+
+
+def step_model(params, n_out, select, W, H, bpp, w0, h0, levels, sensitivity):
+    """The step-boundary byte model of one render (the CPU path's steps, each step's distinct
+    inputs read once and outputs written once): per solved working pixel, level and direction 588
+    B (grad + tensor 24, tensor blur 48, three iterations of calc_b 32 + B blur 32 + new_disp 36 +
+    flow blur 32, four sweeps 96, range check and damping or upsample 24), per pyramid the frame
+    read plus every level written, per in-between frame both frames read, the output written and
+    both flows read, per exact frame one frame read and one written. Returns (flow bytes, warp
+    bytes, distinct pairs, in-between frames)."""
+    import math
+
+    pairs, vector_frames, exact_frames = set(), 0, 0
+    for n in range(n_out):
+        s = select(params, n)
+        if s.exact:
+            exact_frames += 1
+        else:
+            vector_frames += 1
+            pairs.add((s.a, s.b))
+    sizes, w, h = [], w0, h0
+    for _ in range(levels):
+        sizes.append(w * h)
+        w, h = max(w // 2, 2), max(h // 2, 2)
+    x = math.log(float(sensitivity) ** 2) / math.log(2.0) + levels  # flow.rs:179-197
+    start = min(max(int(math.floor(x)), 0), levels - 1) if x >= 0 else 0
+    solved = sum(sizes[:start + 1])
+    frames_used = {f for p in pairs for f in p}
+    flow = 2 * len(pairs) * solved * 588 + len(frames_used) * (W * H * bpp + 4 * sum(sizes))
+    warp = vector_frames * (3 * W * H * bpp + 2 * w0 * h0 * 8) + exact_frames * 2 * W * H * bpp
+    return flow, warp, len(pairs), vector_frames
 
 
 def main():
@@ -142,11 +188,25 @@ def main():
         achieved = (k["bytes"] / max(k["launches"], 1)) / (avg_ms / 1000.0) / 1e9 if avg_ms > 0 else 0.0
         alg_per_launch = k["bytes"] / max(k["launches"], 1)
         traffic = round(TRAFFIC_RATIO[kern] * alg_per_launch) if kern in TRAFFIC_RATIO else None
+        factor = STEP_MODEL_FACTOR.get(kern, 1.0)
         roofline = {"bound": "hbm", "kernel": kern, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                     "algorithmic_bytes_per_launch": round(alg_per_launch),
                     "avg_launch_ms": round(avg_ms, 4), "launches_per_step": k["launches"] // args.steps,
-                    "share_of_step": round(k["ms"] / args.steps / ms_per_step, 3)}
+                    "share_of_step": round(k["ms"] / args.steps / ms_per_step, 3),
+                    "byte_models": {
+                        "fused": {"frac": round(achieved / HBM_PEAK_GBS, 4),
+                                  "note": "achieved/frac above: the kernel's own inputs read once, outputs written once"},
+                        "step_model": {"achieved": round(achieved * factor, 1),
+                                       "frac": round(achieved * factor / HBM_PEAK_GBS, 4),
+                                       "note": "the CPU path's step boundaries (each step's inputs and outputs)"},
+                        "counter_traffic": None if traffic is None else {
+                            "achieved": round(traffic / (avg_ms / 1000.0) / 1e9, 1),
+                            "frac": round(traffic / (avg_ms / 1000.0) / 1e9 / HBM_PEAK_GBS, 4),
+                            "note": "FETCH_SIZE + WRITE_SIZE bytes (copy-calibrated) per launch"}},
+                    "valu_issue_share": VALU_SHARE.get(kern),
+                    "limiter": "VALU issue (no FMA, no packed FP32 under the bit-exact contract)"
+                               if VALU_SHARE.get(kern, 0.0) > achieved / HBM_PEAK_GBS else "HBM"}
     # whole-path algorithmic bytes per step over the step time
     total_bytes = sum(v["bytes"] for v in prof.values()) / args.steps if prof else 0.0
     path = {"algorithmic_GB_per_step": round(total_bytes / 1e9, 3),
@@ -156,6 +216,22 @@ def main():
                             "launches_per_step": v["launches"] // args.steps,
                             "GBs": round(v["bytes"] / (v["ms"] / 1000.0) / 1e9, 1) if v["ms"] > 0 else 0.0}
                         for n, v in sorted(prof.items(), key=lambda kv: -kv[1]["ms"])}}
+    # the flow and warp stages against the step-boundary byte model of the CPU path
+    from twixtor_amd.backend import flow_geometry, flow_settings, select
+
+    fw0, fh0, levels = flow_geometry(flow_settings(quality=4), spec.width, spec.height)
+    flow_b, warp_b, n_pairs, n_vec = step_model(params, n_out, select, spec.width, spec.height, 16, fw0, fh0, levels,
+                                                0.7)
+    stages = {"target_frac": STAGE_TARGET}
+    for name, fams, model in (("flow", FLOW_FAMILIES, flow_b), ("warp", WARP_FAMILIES, warp_b)):
+        ms = sum(prof[f]["ms"] for f in fams if f in prof) / args.steps if prof else 0.0
+        stages[name] = {"ms_per_step": round(ms, 3), "step_model_GB": round(model / 1e9, 3),
+                        "GBs": round(model / (ms / 1000.0) / 1e9, 1) if ms > 0 else None,
+                        "frac": round(model / (ms / 1000.0) / 1e9 / HBM_PEAK_GBS, 4) if ms > 0 else None,
+                        "families": list(fams)}
+    stages["distinct_pairs"] = n_pairs
+    stages["in_between_frames"] = n_vec
+    path["stages"] = stages
     # the tile shapes the autotuner picked for the fused solve kernels, per level area class
     shapes = {0: "64x32 pu4", 1: "64x32 pu1", 2: "64x16 pu4", 3: "64x16 pu1", 4: "64x64 pu4", 5: "64x64 pu1"}
     path["tile_choices"] = [{"kernel": ("tensor", "calc_b", "new_disp")[k], "log2_area": a, "log2_jobs": j,
@@ -319,19 +395,20 @@ def _families(prof, steps=1):
 
 def vectors_leg(be, frames, steps, warmup):
     """Create Vectors and the Vectors In read-back at 3840x2160 RGBA f32: a clip under a
-    global affine motion with 10 discs; Create Vectors with Normal
-    (half res) vectors, track Previous, Max Displace 2048: output n = the encoded flow of frame n
-    towards frame n - 1, for n = 1 .. frames - 1, one twx_create_vectors_dev call over the clip
-    (device-resident). The read-back (Vectors In at 50 %, warping 2, Blend, one vector layer) of
-    two in-between frames after one untimed frame: the layer's flow (decode, hole fill, Vec
-    Scale), its inversion through the Gouraud forward warp, and the warping code 2 composition;
-    these calls take host buffers, so their wall time includes PCIe, and the kernel time is
-    reported beside it."""
-    import numpy as np
+    global affine motion with 10 discs; Create Vectors with Normal (half res) vectors, track
+    Previous, Max Displace 2048: encoded frame n = the flow of frame n towards frame n - 1, for
+    n = 1 .. frames - 1, one twx_create_vectors_dev call over the clip (device-resident), written
+    as frames 1 .. frames - 1 of a vector layer whose frame 0 is transparent. The read-back is
+    `vfx apply vectors-in` at 50 % (30 fps, warping 2 Forward, Blend, Max Displace 2048) with that
+    layer as Vectors: Prev: output frames 2 .. 2 + 2 * rb_pairs, every odd one an in-between frame
+    (the layer's flow decoded and filled, its inversion through the Gouraud forward warp, the
+    forward compositions with coverage and fill), one twx_vectors_in_frames_dev call, device to
+    device, after one untimed call."""
     import torch
 
     from twixtor_amd import synth
     from twixtor_amd.backend import flow_settings
+    from twixtor_amd.params import VectorsIn
 
     dev = torch.device("cuda:0")
     spec = synth.spec_4k_vectors(frames=frames)
@@ -340,13 +417,13 @@ def vectors_leg(be, frames, steps, warmup):
     clip = torch.cat([gen.wire(t) for t in range(frames)]).contiguous()
     fb = clip.numel() // frames
     xs, ys = list(range(1, frames)), list(range(0, frames - 1))
-    out = torch.empty(fb * len(xs), dtype=torch.uint8, device=dev)
+    layer = torch.zeros(fb * frames, dtype=torch.uint8, device=dev)  # frame 0 transparent
     fs = flow_settings(quality=4)
     md = 2048.0
     torch.cuda.synchronize()
 
     def run():
-        be.create_vectors_dev(fs, W, H, fmt, clip.data_ptr(), frames, xs, ys, md, out.data_ptr())
+        be.create_vectors_dev(fs, W, H, fmt, clip.data_ptr(), frames, xs, ys, md, layer.data_ptr() + fb)
 
     for _ in range(warmup):
         run()
@@ -359,70 +436,88 @@ def vectors_leg(be, frames, steps, warmup):
     torch.cuda.synchronize()
     t_cv = (time.perf_counter() - t0) / steps
     prof_cv = be.profile_read()
+    be.profile(False)
 
-    def engine(k):  # wire (planes G, B, R, A) -> engine A,R,G,B (alpha 1 or the encoded white)
-        p = clip[k * fb:(k + 1) * fb].view(torch.float32).reshape(4, H, W) if k >= 0 else None
-        return torch.stack([p[3], p[2] * p[3], p[0] * p[3], p[1] * p[3]], -1).cpu().numpy().reshape(-1)
+    rb_pairs = 4
+    cfg = VectorsIn(W, H, 32, "30", ["speed_percent=50", "vectors_max_displace=2048"])
+    first, count = 2, 2 * rb_pairs
+    out = torch.empty(fb * count, dtype=torch.uint8, device=dev)
 
-    def read_back(k):
-        enc = out[(k - 1) * fb:k * fb].view(torch.float32).reshape(4, H, W)
-        vec = torch.stack([enc[3], enc[2], enc[0], enc[1]], -1).cpu().numpy().reshape(-1)
-        a, b = engine(k - 1), engine(k)
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        f_ba = be.flow_from_vectors(vec, W, H, fmt, md, (1.0, 1.0), True)
-        f_ab = be.invert_flow(f_ba, W, H, True, True)
-        res = be.compose_vectors(1, 0.5, a, b, W, H, fmt, f_ab, f_ba, W, H, True, warp_mode=2, gouraud=True)
-        return res, time.perf_counter() - t0
+    def read_back():
+        be.vectors_in_frames_dev(cfg, clip.data_ptr(), frames, None, (layer.data_ptr(), frames), None, first, count,
+                                 out.data_ptr())
 
-    read_back(frames - 1)  # warm-up: the first launch of each kernel loads its code
+    read_back()  # warm-up: the first launch of each kernel loads its code
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    read_back()
+    torch.cuda.synchronize()
+    t_rb = time.perf_counter() - t0
+    be.profile(True)
     be.profile_reset()
-    rb_wall, rb = [], None
-    for k in (1, frames // 2):
-        rb, dt = read_back(k)
-        rb_wall.append(dt)
+    read_back()
+    torch.cuda.synchronize()
     prof_rb = be.profile_read()
     be.profile(False)
-    kern_rb = sum(v["ms"] for v in prof_rb.values()) / len(rb_wall)
-    del clip, out
+    kern_rb = sum(v["ms"] for v in prof_rb.values())
+    finite = bool(torch.isfinite(out.view(torch.float32)).all().item())
+    del clip, layer, out
     torch.cuda.empty_cache()
     return {"workload": f"Create Vectors 3840x2160 RGBA f32, {frames} frames (affine background 0.2 deg, x1.002, (3, -1) px per "
                         "frame; 10 discs up to 40 px/frame), Normal vectors (working 1920x1080), track Previous, Max Displace "
-                        f"2048 -> {len(xs)} encoded frames; Vectors In read-back at 50 %, warping 2, Blend",
+                        f"2048 -> {len(xs)} encoded frames; Vectors In read-back of {count} output frames at 50 % "
+                        f"({rb_pairs} in-between), warping 2, Blend, device-resident",
             "create_vectors": {"output_frames": len(xs), "ms": round(t_cv * 1000.0, 2),
                                "frames_per_s": round(len(xs) / t_cv, 1), "kernels": _families(prof_cv, steps)},
-            "read_back": {"frames": len(rb_wall), "wall_ms_per_frame_pcie_inclusive": round(1000.0 * float(np.mean(rb_wall)), 2),
-                          "kernel_ms_per_frame": round(kern_rb, 3), "kernels": _families(prof_rb, len(rb_wall)),
-                          "finite": bool(np.isfinite(rb).all())}}
+            "read_back": {"output_frames": count, "in_between_frames": rb_pairs, "wall_ms": round(1000.0 * t_rb, 2),
+                          "wall_ms_per_in_between_frame": round(1000.0 * t_rb / rb_pairs, 2),
+                          "kernel_ms_per_in_between_frame": round(kern_rb / rb_pairs, 3),
+                          "kernels": _families(prof_rb, 1), "finite": finite}}
 
 
 def cpu_baseline(cfg, clip, out, n_in, n_out, fb, budget_s):
-    """The oracle on the host cores over a bounded, evenly spread sample of the output frames;
-    the sampled frames' bytes must equal the device output."""
-    import numpy as np
-
+    """The oracle on the host: the CPU path restated in C++ with OpenMP over rows (the reference
+    splits rows over rayon's threads the same way), over a bounded, evenly spread sample of the
+    output frames, on every core this process may use and on one thread; the sampled frames' bytes
+    must equal the device output. Each frame is rendered whole as the CPU path does: an in-between
+    frame solves its pair again (the device solves each distinct pair once per clip)."""
     from oracle import cpuref
 
-    cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
-    cores = max(1, min(cores, 16))
-    cpuref.set_threads(cores)
+    nproc = os.cpu_count()
+    affinity = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else nproc
+    omp = os.environ.get("OMP_NUM_THREADS")
+    threads = min(affinity, int(omp)) if omp and omp.isdigit() and int(omp) > 0 else affinity
     host_clip = clip.cpu().numpy().tobytes()
     host_out = out.cpu().numpy()
     params = cfg.struct()
     order = list(range(1, n_out, 7)) + [0]
-    done, t_sum, same = 0, 0.0, True
-    for n in order:
-        t0 = time.perf_counter()
-        got = cpuref.retime_frames(params, host_clip, n_in, n, 1)
-        t_sum += time.perf_counter() - t0
-        done += 1
-        if got != host_out[n * fb:(n + 1) * fb].tobytes():
-            same = False
-        if t_sum >= budget_s:
-            break
-    return {"value": round(done / t_sum, 4), "unit": "frames/s", "cores": cores, "kind": "port",
-            "sample": f"{done} of {n_out} output frames (every 7th), oracle rendering each frame whole as the CPU path does",
-            "bytes_equal_to_device": same}
+
+    def run(n_threads, budget):
+        cpuref.set_threads(n_threads)
+        done, t_sum, same = 0, 0.0, True
+        for n in order:
+            t0 = time.perf_counter()
+            got = cpuref.retime_frames(params, host_clip, n_in, n, 1)
+            t_sum += time.perf_counter() - t0
+            done += 1
+            same &= got == host_out[n * fb:(n + 1) * fb].tobytes()
+            if t_sum >= budget:
+                break
+        return done, t_sum, same
+
+    done, t_all, same = run(threads, budget_s)
+    done1, t_one, same1 = run(1, budget_s / 2)
+    cpuref.set_threads(threads)
+    vector_frames = sum(1 for n in range(n_out) if not cpuref.select(params, n).exact)
+    return {"value": round(done / t_all, 4), "unit": "frames/s", "cores": threads, "kind": "port",
+            "sample": f"{done} of {n_out} output frames (every 7th from frame 1, then frame 0), the oracle rendering each "
+                      "frame whole",
+            "host": {"nproc": nproc, "affinity_cpus": affinity, "OMP_NUM_THREADS": omp},
+            "one_thread": {"value": round(done1 / t_one, 4), "unit": "frames/s", "sample_frames": done1},
+            "solves_per_clip": {"cpu_path": 2 * vector_frames,
+                                "note": "the CPU path solves both directions of each in-between frame's pair; the device "
+                                        "solves each distinct pair once per clip"},
+            "bytes_equal_to_device": bool(same and same1)}
 
 
 if __name__ == "__main__":

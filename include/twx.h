@@ -232,6 +232,26 @@ int twx_create_vectors_dev(twx_handle* h, const twx_flow_settings* fs, int32_t w
                            int32_t count, const int32_t* frame_x, const int32_t* frame_y,
                            float max_displace, void* out, char* err, size_t errlen);
 
+/* The per-frame render of `vfx apply vectors-in` (plugins/vectors-in/src/glue.rs:380-495): output
+ * frames [first, first + count) of a run over the footage `input` (n_input wire frames), the optional
+ * Color Source layer `source` and the vector layers `prev` (Vectors: Prev) and `next` (Vectors:
+ * Next); a NULL layer is a layer that is not set. Every frame is read upside down and the result
+ * turned back (glue.rs:398-410); the flows come from the vector images (decode, push-pull fill,
+ * Vec Scale; vectorin.rs:200-225), a single layer's flow is inverted through the Gouraud forward
+ * warp where a composition reads the missing one; the compositions are the engine's with the
+ * effect's stages (the Gouraud forward warp, taps above first, glue.rs:253). Without a vector layer,
+ * or with a rate below 0.01, the frames are the size-mismatch error fill. The blur compensation
+ * tail and the FG matte layers are refused (TWX_E_UNSUPPORTED). DEVICE pointers. */
+int twx_vectors_in_frames_dev(twx_handle* h, const twx_vin_params* p, const void* input, int32_t n_input,
+                              const void* source, int32_t n_source, const void* prev, int32_t n_prev,
+                              const void* next, int32_t n_next, int32_t first, int32_t count, void* out,
+                              char* err, size_t errlen);
+/* The same with host pointers (uploads the clips, downloads the frames). */
+int twx_vectors_in_frames(twx_handle* h, const twx_vin_params* p, const void* input, int32_t n_input,
+                          const void* source, int32_t n_source, const void* prev, int32_t n_prev,
+                          const void* next, int32_t n_next, int32_t first, int32_t count, void* out,
+                          char* err, size_t errlen);
+
 /* ---- the per-frame render loop of `vfx apply retime` ------------------------------------- */
 
 /* Frame selection of output frame `n` (compose.rs:76-99 and timeline.rs:447-464). */

@@ -5,7 +5,7 @@ from __future__ import annotations
 import ctypes
 import os
 
-from .params import RetimeParams, Selection
+from .params import RetimeParams, Selection, VinParams
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libtwx.so")
@@ -20,6 +20,7 @@ EXPORTS = [
     "twx_retime_frames",
     "twx_forward_warp", "twx_encode_vectors", "twx_decode_vectors", "twx_flow_from_vectors", "twx_invert_flow",
     "twx_create_vectors_dev", "twx_warp_blend", "twx_warp_blend_backward", "twx_warp_blend_backward_noalloc",
+    "twx_vectors_in_frames", "twx_vectors_in_frames_dev",
     "twx_set_graphs", "twx_set_tile_shape", "twx_tile_choices", "twx_get_stats", "twx_cache_stats", "twx_stream", "twx_profile_enable", "twx_profile_reset", "twx_profile_read",
 ]
 
@@ -104,6 +105,8 @@ def lib():
         L.twx_decode_vectors.argtypes = [P, P, I32, I32, I32, F32, P, P] + ERR
         L.twx_flow_from_vectors.argtypes = [P, P, I32, I32, I32, F32, F32, F32, I32, P] + ERR
         L.twx_invert_flow.argtypes = [P, P, I32, I32, I32, I32, P] + ERR
+        for name in ("twx_vectors_in_frames", "twx_vectors_in_frames_dev"):
+            getattr(L, name).argtypes = [P, ctypes.POINTER(VinParams), P, I32, P, I32, P, I32, P, I32, I32, I32, P] + ERR
         L.twx_create_vectors_dev.argtypes = [P, ctypes.POINTER(FlowSettings), I32, I32, I32, P, I32, I32, PI32, PI32, F32,
                                              P] + ERR
         L.twx_warp_blend.argtypes = [P, I32, I32, I32, P, P, P, P, I32, I32, ctypes.c_float, P] + ERR

@@ -14,7 +14,7 @@ import ctypes
 import numpy as np
 
 from . import _ffi
-from .params import Retime, RetimeParams, Selection
+from .params import Retime, RetimeParams, Selection, VectorsIn
 
 DTYPES = {0: np.uint8, 1: np.uint16, 2: np.float32}
 
@@ -263,6 +263,45 @@ class HipBackend:
         _ffi.check(_ffi.lib().twx_create_vectors_dev(self._h, ctypes.byref(fs), width, height, fmt,
                                                      ctypes.c_void_p(clip_ptr), n_in, n, I(*frame_x), I(*frame_y),
                                                      ctypes.c_float(max_displace), ctypes.c_void_p(out_ptr), err, 512), err)
+
+    # ---- the Vectors In render ----
+    def vectors_in_frames(self, cfg: VectorsIn, input_clip: bytes, n_input: int, source=None, prev=None, nxt=None,
+                          first: int = 0, count: int = 1) -> bytes:
+        """Output frames [first, first + count) of `vfx apply vectors-in` on host bytes: `source`,
+        `prev` and `nxt` are (clip bytes, frame count) or None for a layer that is not set."""
+        fb = cfg.frame_bytes()
+        keep = []
+
+        def clip(x, what):
+            if x is None:
+                return None, 0
+            a = np.frombuffer(x[0], np.uint8)
+            if a.size != fb * x[1]:
+                raise ValueError(f"vectors-in {what}: {a.size} bytes, expected {x[1]} frames of {fb}")
+            keep.append(a)
+            return _p(a), x[1]
+
+        i, _ = clip((input_clip, n_input), "input")
+        s, p, n = clip(source, "source"), clip(prev, "Vectors: Prev"), clip(nxt, "Vectors: Next")
+        out = np.zeros(fb * max(count, 0), np.uint8)
+        err = _ffi.errbuf()
+        _ffi.check(_ffi.lib().twx_vectors_in_frames(self._h, ctypes.byref(cfg.struct()), i, n_input, s[0], s[1], p[0], p[1],
+                                                    n[0], n[1], first, count, _p(out), err, 512), err)
+        return out.tobytes()
+
+    def vectors_in_frames_dev(self, cfg: VectorsIn, input_ptr: int, n_input: int, source=None, prev=None, nxt=None,
+                              first: int = 0, count: int = 1, out_ptr: int = 0) -> None:
+        """Device-resident form: the clips are (device address, frame count) or None; `out_ptr`
+        receives `count` wire frames. Blocks until the frames are written."""
+        def clip(x):
+            return (None, 0) if x is None else (ctypes.c_void_p(x[0]), x[1])
+
+        s, p, n = clip(source), clip(prev), clip(nxt)
+        err = _ffi.errbuf()
+        caller_ready()
+        _ffi.check(_ffi.lib().twx_vectors_in_frames_dev(self._h, ctypes.byref(cfg.struct()), ctypes.c_void_p(input_ptr),
+                                                        n_input, s[0], s[1], p[0], p[1], n[0], n[1], first, count,
+                                                        ctypes.c_void_p(out_ptr), err, 512), err)
 
     # ---- the per-frame render loop, batched ----
     def retime_frames(self, cfg: Retime, clip: bytes, n_in: int, first: int, count: int) -> bytes:

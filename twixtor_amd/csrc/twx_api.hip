@@ -1441,7 +1441,19 @@ static int forward_warp_dev(twx_handle* h, DevPool& pool, const void* src, int W
                             std::string& err) {
     hipStream_t s = h->stream;
     const float a = 0.0f > amount ? -amount : amount;
-    if (1e-05 > (double)a || fw_flow_is_zero(h_flow, (long)fw * fh * 2)) {
+    bool still = 1e-05 > (double)a;
+    if (!still && h_flow) {
+        still = fw_flow_is_zero(h_flow, (long)fw * fh * 2);
+    } else if (!still) {  // no host copy of the flow: the same test on the device
+        int* moves = pool.alloc<int>(1);
+        int* hm = pool.pinned<int>(1);
+        launch_zero(s, moves, 1);
+        launch_fw_flow_moves(s, d_flow, (long)fw * fh * 2, moves);
+        HIPCHECK(hipMemcpyAsync(hm, moves, sizeof(int), hipMemcpyDeviceToHost, s));
+        HIPCHECK(hipStreamSynchronize(s));
+        still = *hm == 0;
+    }
+    if (still) {
         HIPCHECK(hipMemcpyAsync(out, src, image_bytes(W, H, fmt), hipMemcpyDeviceToDevice, s));
         if (cov) HIPCHECK(hipMemsetAsync(cov, 0xff, (size_t)W * H, s));
         return TWX_OK;
@@ -1661,6 +1673,28 @@ static int flow_from_vectors_dev(hipStream_t s, DevPool& pool, const void* d_vec
     return TWX_OK;
 }
 
+// invert_flow_on (compose/vector.rs:370-397) of a device flow into `out` (both width x height): T =
+// (-1, -fx, -fy, -1) pushed along the flow by the Gouraud forward warp with coverage, the read-out,
+// the push-pull fill of the uncovered pixels. `h_flow` is the flow's host copy or null; the forward
+// warp's flags go to `fw_flags` (a device int the caller cleared).
+static int invert_flow_dev(twx_handle* h, DevPool& pool, const float* d_flow, const float* h_flow, int W, int H,
+                           bool above_first, float* out, int* fw_flags, std::string& err) {
+    hipStream_t s = h->stream;
+    const size_t n = (size_t)W * H;
+    float* t = pool.alloc<float>(n * 4);
+    float* pushed = pool.alloc<float>(n * 4);
+    uint8_t* cov = pool.alloc<uint8_t>(n);
+    uint8_t* holes = pool.alloc<uint8_t>(n);
+    int* gate = pool.alloc<int>(1);
+    launch_zero(s, gate, 1);
+    launch_invert_prep(s, d_flow, t, (long)n);
+    int st = forward_warp_dev(h, pool, t, W, H, TWX_FMT_F32, d_flow, h_flow, W, H, 1.0f, pushed, cov, fw_flags, err);
+    if (st) return st;
+    launch_invert_post(s, pushed, cov, out, holes, gate, (long)n);
+    std::vector<FillItem> items{FillItem{W, H, out, FILL_FLOW2, 2, holes, gate, above_first ? 1 : 0}};
+    return fill_items(s, pool, items, err);
+}
+
 int twx_flow_from_vectors(twx_handle* h, const void* vec, int32_t width, int32_t height, int32_t fmt, float max_displace,
                           float scale_x, float scale_y, int32_t taps_above_first, float* flow_out, char* err,
                           size_t errlen) {
@@ -1701,21 +1735,12 @@ int twx_invert_flow(twx_handle* h, const float* flow, int32_t width, int32_t hei
         DevPool pool(h);
         const size_t n = (size_t)width * height;
         float* fl = pool.alloc<float>(n * 2);
-        float* t = pool.alloc<float>(n * 4);
-        float* pushed = pool.alloc<float>(n * 4);
-        uint8_t* cov = pool.alloc<uint8_t>(n);
-        uint8_t* holes = pool.alloc<uint8_t>(n);
         float* res = pool.alloc<float>(n * 2);
-        int* flags = pool.alloc<int>(2);
+        int* flags = pool.alloc<int>(1);
         HIPCHECK(hipMemcpyAsync(fl, flow, n * 8, hipMemcpyHostToDevice, s));
-        HIPCHECK(hipMemsetAsync(flags, 0, 2 * sizeof(int), s));
-        launch_invert_prep(s, fl, t, (long)n);
-        int st = forward_warp_dev(h, pool, t, width, height, TWX_FMT_F32, fl, flow, width, height, 1.0f, pushed, cov,
-                                  flags, e);
+        launch_zero(s, flags, 1);
+        int st = invert_flow_dev(h, pool, fl, flow, width, height, taps_above_first != 0, res, flags, e);
         if (st) return st;
-        launch_invert_post(s, pushed, cov, res, holes, flags + 1, (long)n);
-        std::vector<FillItem> items{FillItem{width, height, res, FILL_FLOW2, 2, holes, flags + 1, taps_above_first ? 1 : 0}};
-        if ((st = fill_items(s, pool, items, e))) return st;
         int* hf = pool.pinned<int>(1);
         HIPCHECK(hipMemcpyAsync(out, res, n * 8, hipMemcpyDeviceToHost, s));
         HIPCHECK(hipMemcpyAsync(hf, flags, sizeof(int), hipMemcpyDeviceToHost, s));
@@ -1809,6 +1834,350 @@ int twx_create_vectors_dev(twx_handle* h, const twx_flow_settings* fs, int32_t w
         HIPCHECK(hipGetLastError());
         h->stats.device_ms += now_ms() - t0;
         return TWX_OK;
+    });
+}
+
+// ---- the Vectors In render (plugins/vectors-in/src/glue.rs:380-495, vectorin.rs, timemap.rs) ----
+
+// One layer of a Vectors In render: wire frames on the device, frame i showing at ticks
+// [i * time_step, (i + 1) * time_step); a null `wire` is a layer that is not set.
+struct VinClip {
+    const uint8_t* wire;
+    int32_t n;
+};
+
+// Output frames [first, first + count) into `out` (wire frames on the device). Every frame the
+// render reads is ingested upside down (glue.rs:398-410), once per (layer, clip frame); every
+// vector frame's flow is decoded, filled and scaled once (vectorin.rs:112-121); a single vector
+// layer's flow is inverted once where a composition reads the missing one (compose/vector.rs:
+// 430-456); the compositions are the engine's with the effect's stages, the Gouraud forward warp
+// and the taps above first (glue.rs:253), batched over the output frames. The caller
+// synchronises the stream and checks `bad` ([0] a NaN warp coordinate, [1 + i] output frame i's
+// has-a-hole flag, [1 + count] the forward warps' flags).
+static int vectors_in_enqueue(twx_handle* h, DevPool& pool, const twx_vin_params& p, const VinClip& input,
+                              const VinClip& source, const VinClip& prev, const VinClip& next, int first, int count,
+                              uint8_t* out, int* bad, std::string& err) {
+    hipStream_t s = h->stream;
+    const int W = p.width, H = p.height, fmt = p.fmt;
+    const size_t fb = image_bytes(W, H, fmt), npx = (size_t)W * H;
+    timeline::Rates r;
+    if (timeline::vin_rates(p, r) || (!prev.wire && !next.wire)) {
+        // the rate error or no vector layer: the size mismatch fill (image.rs:391-425), red at half
+        // alpha in the wire's R, G, B, A order, without the finish
+        std::vector<uint8_t> fill(fb);
+        for (size_t i = 0; i < npx; i++) {
+            if (fmt == TWX_FMT_U8) {
+                const uint8_t v[4] = {0xff, 0, 0, 0x80};
+                memcpy(&fill[i * 4], v, 4);
+            } else if (fmt == TWX_FMT_U16) {
+                auto q = [](uint32_t v) { return (uint16_t)(((v < 32768u ? v : 32768u) * 65535u + 16384u) / 32768u); };
+                const uint16_t v[4] = {q(0x8000), 0, 0, q(0x4000)};
+                memcpy(&fill[i * 8], v, 8);
+            } else {
+                float* pl = (float*)fill.data();  // planes G, B, R, A
+                pl[i] = 0.0f;
+                pl[npx + i] = 0.0f;
+                pl[2 * npx + i] = 1.0f;
+                pl[3 * npx + i] = 0.5f;
+            }
+        }
+        uint8_t* d_fill = pool.upload(fill);
+        for (int i = 0; i < count; i++)
+            HIPCHECK(hipMemcpyAsync(out + (size_t)i * fb, d_fill, fb, hipMemcpyDeviceToDevice, s));
+        return TWX_OK;
+    }
+    const VinClip& layer = source.wire ? source : input;  // the retimed layer (glue.rs:86-89)
+    const twx_retime_params rp = timeline::vin_speed_params(p);
+    timeline::SpeedMap map(rp, r);
+    // the frames each output frame reads: (layer 0 retimed / 1 Vectors: Prev / 2 Vectors: Next, clip
+    // frame or -1 outside) -> an engine image slot
+    std::map<std::pair<int, int64_t>, int> slot_of;
+    std::vector<IngestOp> ops;
+    const VinClip* clips[3] = {&layer, &prev, &next};
+    auto slot = [&](int lid, int64_t time) {
+        const int64_t idx = timeline::vin_clip_index(time, p.time_step, clips[lid]->n);
+        auto it = slot_of.find({lid, idx});
+        if (it != slot_of.end()) return it->second;
+        IngestOp op{};
+        op.wire = idx >= 0 ? clips[lid]->wire + (size_t)idx * fb : nullptr;
+        op.engine = pool.alloc<uint8_t>(fb);
+        op.W = W;
+        op.H = H;
+        op.fmt = fmt;
+        op.zero = idx < 0 ? 1 : 0;
+        op.reverse = 1;
+        ops.push_back(op);
+        const int k = (int)ops.size() - 1;
+        slot_of[{lid, idx}] = k;
+        return k;
+    };
+    struct VPlan {
+        int kind = 0;  // 0 the source display, 1 the fast path, 2 the vector path
+        twx_selection sel{};
+        int sa = -1, sb = -1, vab = -1, vba = -1;  // slots of A, B, the Next image at A, the Prev image at B
+        bool need_ab = false, need_ba = false;
+    };
+    std::vector<VPlan> plans(count);
+    const int warp_mode = p.warping == 2 ? 2 : 0;  // glue.rs:96-103
+    for (int i = 0; i < count; i++) {
+        VPlan& pl = plans[i];
+        const int32_t n = first + i;
+        if (p.display == 2) {  // compose.rs:62-71: the retimed layer at current_time
+            pl.kind = 0;
+            pl.sa = slot(0, (int64_t)n * p.time_step);
+            continue;
+        }
+        int st = timeline::vin_select(p, map, r, n, pl.sel, err);
+        if (st) return st;
+        if (pl.sel.interp < 0 || pl.sel.interp > 2) { err = "frame_interp outside 1..3"; return TWX_E_MISMATCH; }
+        const int64_t ta = timeline::vin_time(r, (float)pl.sel.a), tb = timeline::vin_time(r, (float)pl.sel.b);
+        pl.sa = slot(0, ta);
+        pl.sb = slot(0, tb);
+        if (pl.sel.vectors_quality != 0 && !pl.sel.exact) {
+            pl.kind = 2;
+            if (next.wire) pl.vab = slot(2, ta);
+            if (prev.wire) pl.vba = slot(1, tb);
+            pl.need_ab = pl.need_ba = true;
+            if (pl.sel.interp == 0) {  // nearest_uses_f_ab (compose/vector.rs:61-64)
+                const bool ab = (pl.sel.wa < 0.5f && warp_mode < 2) || (0.5f <= pl.sel.wa && warp_mode > 1);
+                pl.need_ab = ab;
+                pl.need_ba = !ab;
+            }
+        } else {
+            pl.kind = 1;
+        }
+    }
+    if (!ops.empty()) launch_ingest_batch(s, pool.upload(ops), ops);
+    // the layers' flows (vectorin.rs:200-225): Max Displace read once, Vec Scale constant
+    const float md = (1.0f / 1.0f) * timeline::read_fixed(timeline::to_fixed(p.max_displace), 1.0f);
+    const float vsx = timeline::read_fixed(timeline::to_fixed(p.vec_scale_x), 1.0f);
+    const float vsy = timeline::read_fixed(timeline::to_fixed(p.vec_scale_y), 1.0f);
+    std::map<int, float*> flow_of;  // vector slot -> its flow
+    auto flow = [&](int vslot) -> float* {
+        auto it = flow_of.find(vslot);
+        if (it != flow_of.end()) return it->second;
+        float* f = pool.alloc<float>(npx * 2);
+        int st = flow_from_vectors_dev(s, pool, ops[vslot].engine, W, H, fmt, md, vsx, vsy, true, f, err);
+        if (st) throw st;
+        flow_of[vslot] = f;
+        return f;
+    };
+    std::map<int, float*> inverse_of;  // vector slot -> the inversion of its flow
+    int* fw_flags = bad + 1 + count;
+    auto inverse = [&](int vslot) -> float* {
+        auto it = inverse_of.find(vslot);
+        if (it != inverse_of.end()) return it->second;
+        float* f = pool.alloc<float>(npx * 2);
+        int st = invert_flow_dev(h, pool, flow(vslot), nullptr, W, H, true, f, fw_flags, err);
+        if (st) throw st;
+        inverse_of[vslot] = f;
+        return f;
+    };
+    std::vector<FrameJob> jobs;
+    try {
+        for (int i = 0; i < count; i++) {
+            const VPlan& pl = plans[i];
+            FrameJob F;
+            memset(&F, 0, sizeof(F));
+            F.a = ops[pl.sa].engine;
+            F.b = pl.sb >= 0 ? ops[pl.sb].engine : F.a;
+            F.out = out + (size_t)i * fb;
+            F.bad = bad;
+            F.fmt = fmt;
+            F.W = W;
+            F.H = H;
+            F.finish = 1;
+            if (pl.kind == 0) {  // the frame through the prepare and the finish
+                F.kind = 0;
+                F.blend_mode = 0;
+                h->stats.fast_frames++;
+                jobs.push_back(F);
+                continue;
+            }
+            F.wa = pl.sel.wa;
+            F.frac = 1.0f - pl.sel.wa;
+            const int interp = pl.sel.interp;
+            if (pl.kind == 1) {  // compose.rs:214-246
+                F.kind = 0;
+                h->stats.fast_frames++;
+                if (interp == 0) {
+                    F.blend_mode = pl.sel.wa >= 0.5f ? 0 : 1;
+                } else if (interp == 1) {
+                    blend_branch((float)(1.0 - (double)pl.sel.wa), F);
+                } else {
+                    blend_branch(1.0f - pl.sel.wa, F);
+                    F.plus_zero = 1;
+                }
+                jobs.push_back(F);
+                continue;
+            }
+            if (W < 2 || H < 2) { err = "the warps need images of at least 2x2"; return TWX_E_MISMATCH; }
+            h->stats.composes++;
+            // the flows: as loaded, the missing one inverted where this composition reads it
+            const float* f_ab = pl.vab >= 0 ? flow(pl.vab) : nullptr;
+            const float* f_ba = pl.vba >= 0 ? flow(pl.vba) : nullptr;
+            if (!f_ab && pl.need_ab) f_ab = inverse(pl.vba);
+            if (!f_ba && pl.need_ba) f_ba = inverse(pl.vab);
+            F.f_ab = f_ab;
+            F.f_ba = f_ba;
+            F.fw = W;
+            F.fh = H;
+            F.warp_a_copy = warp_is_copy(F.frac, W, W) ? 1 : 0;
+            F.warp_b_copy = warp_is_copy(F.wa, W, W) ? 1 : 0;
+            if (warp_mode == 2) {
+                // forward.rs:391-572 with coverage and fill: A pushed with F_AB at 1 - wa, B with F_BA
+                // at wa; Nearest the one frame nearest_uses_f_ab names
+                int st;
+                if (interp == 0) {
+                    const bool from_a = !(0.5f > pl.sel.wa);
+                    void* dst = pool.alloc<uint8_t>(fb);
+                    uint8_t* cov = pool.alloc<uint8_t>(npx);
+                    st = forward_warp_dev(h, pool, from_a ? F.a : F.b, W, H, fmt, from_a ? f_ab : f_ba, nullptr, W, H,
+                                          from_a ? F.frac : F.wa, dst, cov, fw_flags, err);
+                    F.pre_a = dst;
+                    F.pre_cov_a = cov;
+                } else {
+                    void* ta = pool.alloc<uint8_t>(fb);
+                    void* tb = pool.alloc<uint8_t>(fb);
+                    uint8_t* c1 = pool.alloc<uint8_t>(npx);
+                    uint8_t* c2 = pool.alloc<uint8_t>(npx);
+                    st = forward_warp_dev(h, pool, F.a, W, H, fmt, f_ab, nullptr, W, H, F.frac, ta, c1, fw_flags, err);
+                    if (!st) st = forward_warp_dev(h, pool, F.b, W, H, fmt, f_ba, nullptr, W, H, F.wa, tb, c2, fw_flags, err);
+                    F.pre_a = ta;
+                    F.pre_b = tb;
+                    F.pre_cov_a = c1;
+                    F.pre_cov_b = c2;
+                }
+                if (st) return st;
+                setup_smart(pool, F, interp, true, bad + 1 + i);
+            } else if (interp == 0) {
+                F.kind = 1;
+                F.nearest_from_a = 0.5f <= pl.sel.wa ? 1 : 0;
+            } else if (interp == 1) {
+                F.kind = 2;
+                blend_branch(F.frac, F);
+            } else {
+                setup_weighted(pool, F, true);
+            }
+            jobs.push_back(F);
+        }
+    } catch (int st) {
+        return st;
+    }
+    if (!jobs.empty()) {
+        for (auto& F : jobs) frame_job_ratios(F);
+        FrameJob* dj = pool.upload(jobs);
+        launch_frames(s, dj, jobs);
+        launch_weighted(s, dj, jobs);
+        int st = run_fills(s, pool, jobs, err);
+        if (st) return st;
+        launch_finish(s, dj, jobs);
+    }
+    return TWX_OK;
+}
+
+// Vectors In over output frames [first, first + count): checks, then chunks of output frames that
+// fit the memory budget, each enqueued, synchronised and checked.
+static int vectors_in_run(twx_handle* h, const twx_vin_params& p, const VinClip& input, const VinClip& source,
+                          const VinClip& prev, const VinClip& next, int first, int count, uint8_t* out, std::string& err) {
+    if (p.width <= 0 || p.height <= 0 || p.fmt < 0 || p.fmt > 2 || p.time_step <= 0 || count < 0) {
+        err = "vectors-in: bad frame size, format, time step or frame count";
+        return TWX_E_MISMATCH;
+    }
+    if (int st = check_frame_pixels(p.width, p.height, err)) return st;
+    if (p.frame_interp < 1 || p.frame_interp > 3 || p.display < 1 || p.display > 6) {
+        err = "vectors-in: popup out of range";
+        return TWX_E_MISMATCH;
+    }
+    if (p.blur != 0.0) {
+        err = "vectors-in: the blur compensation tail is not served by this backend";
+        return TWX_E_UNSUPPORTED;
+    }
+    if (!input.wire || input.n < 0 || (source.wire && source.n < 0) || (prev.wire && prev.n < 0) || (next.wire && next.n < 0)) {
+        err = "vectors-in: missing input clip";
+        return TWX_E_MISMATCH;
+    }
+    const double t0 = now_ms();
+    const size_t fb = image_bytes(p.width, p.height, p.fmt), npx = (size_t)p.width * p.height;
+    // a frame reads at most four images and holds two flows, an inversion, two pushed frames and
+    // their coverages, the fill's pyramids: about 10 frames and 64 bytes a pixel
+    const size_t per_frame = 10 * fb + 64 * npx;
+    const int chunk = (int)std::max<size_t>(1, std::min<size_t>(64, (h->budget / 4) / std::max<size_t>(per_frame, 1)));
+    for (int c0 = 0; c0 < count; c0 += chunk) {
+        const int n = std::min(chunk, count - c0);
+        DevPool pool(h);
+        int* bad = pool.alloc<int>(2 + (size_t)n);
+        launch_zero(h->stream, bad, 2 + (size_t)n);
+        int st = vectors_in_enqueue(h, pool, p, input, source, prev, next, first + c0, n, out + (size_t)c0 * fb, bad, err);
+        if (st) return st;
+        int* hb = pool.pinned<int>(2 + (size_t)n);
+        HIPCHECK(hipMemcpyAsync(hb, bad, (2 + (size_t)n) * sizeof(int), hipMemcpyDeviceToHost, h->stream));
+        HIPCHECK(hipStreamSynchronize(h->stream));
+        if (hb[0]) {
+            err = "inverse warp: a sample coordinate is NaN or never converges, which is not supported";
+            return TWX_E_UNSUPPORTED;
+        }
+        if ((st = check_fw_flags(hb[1 + n], err))) return st;
+    }
+    h->stats.device_ms += now_ms() - t0;
+    return TWX_OK;
+}
+
+int twx_vectors_in_frames_dev(twx_handle* h, const twx_vin_params* p, const void* input, int32_t n_input,
+                              const void* source, int32_t n_source, const void* prev, int32_t n_prev, const void* next,
+                              int32_t n_next, int32_t first, int32_t count, void* out, char* err, size_t errlen) {
+    return entry(h, err, errlen, [&](std::string& e) -> int {
+        if (!p) { e = "vectors-in: no parameters"; return TWX_E_MISMATCH; }
+        after_caller(h);
+        return vectors_in_run(h, *p, VinClip{(const uint8_t*)input, n_input}, VinClip{(const uint8_t*)source, n_source},
+                              VinClip{(const uint8_t*)prev, n_prev}, VinClip{(const uint8_t*)next, n_next}, first, count,
+                              (uint8_t*)out, e);
+    });
+}
+
+int twx_vectors_in_frames(twx_handle* h, const twx_vin_params* p, const void* input, int32_t n_input, const void* source,
+                          int32_t n_source, const void* prev, int32_t n_prev, const void* next, int32_t n_next,
+                          int32_t first, int32_t count, void* out, char* err, size_t errlen) {
+    return entry(h, err, errlen, [&](std::string& e) -> int {
+        if (!p || p->fmt < 0 || p->fmt > 2 || p->width <= 0 || p->height <= 0 || count < 0) {
+            e = "vectors-in: bad parameters";
+            return TWX_E_MISMATCH;
+        }
+        if (int st = check_frame_pixels(p->width, p->height, e)) return st;
+        hipStream_t s = h->stream;
+        const size_t fb = image_bytes(p->width, p->height, p->fmt);
+        // the clips in memory of their own (the render's chunks reset the handle's arena)
+        std::vector<void*> owned;
+        auto up = [&](const void* src, int32_t n) -> const uint8_t* {
+            if (!src || n <= 0) return (const uint8_t*)src;
+            void* d = nullptr;
+            HIPCHECK(hipMalloc(&d, fb * n));
+            owned.push_back(d);
+            HIPCHECK(hipMemcpyAsync(d, src, fb * n, hipMemcpyHostToDevice, s));
+            return (const uint8_t*)d;
+        };
+        int st = TWX_OK;
+        void* dout = nullptr;
+        try {
+            const uint8_t *di = up(input, n_input), *ds = up(source, n_source), *dp = up(prev, n_prev), *dn = up(next, n_next);
+            if (count > 0) {
+                HIPCHECK(hipMalloc(&dout, fb * count));
+                owned.push_back(dout);
+            }
+            st = vectors_in_run(h, *p, VinClip{di, n_input}, VinClip{ds, n_source}, VinClip{dp, n_prev}, VinClip{dn, n_next},
+                                first, count, (uint8_t*)dout, e);
+            if (!st && count > 0) {
+                HIPCHECK(hipMemcpyAsync(out, dout, fb * count, hipMemcpyDeviceToHost, s));
+                HIPCHECK(hipStreamSynchronize(s));
+            }
+        } catch (...) {
+            (void)hipStreamSynchronize(s);
+            for (void* d : owned) (void)hipFree(d);
+            throw;
+        }
+        (void)hipStreamSynchronize(s);
+        for (void* d : owned) (void)hipFree(d);
+        return st;
     });
 }
 

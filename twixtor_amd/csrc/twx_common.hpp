@@ -269,6 +269,8 @@ struct FwArgs {
 };
 size_t fw_scan_blocks(long n);
 bool fw_flow_is_zero(const float* flow, long n);  // forward.rs:183-195 on a host flow
+// the same test on a device flow: sets *flag (cleared by the caller) when some component moves
+void launch_fw_flow_moves(hipStream_t s, const float* flow, long n, int* flag);
 void fw_grid(FwArgs& A, float amount);             // mesh.rs:11-31
 void launch_fw_count(hipStream_t s, const FwArgs& A);
 void launch_fw_emit_fold(hipStream_t s, const FwArgs& A, uint64_t records);

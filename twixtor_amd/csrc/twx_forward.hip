@@ -18,6 +18,7 @@
 // precision epsilon, the span and last-row thresholds and the running mean's reciprocal in double).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <string>
@@ -566,6 +567,28 @@ __global__ void k_fw_scan_add(uint64_t* __restrict__ offsets, const uint64_t* __
 }
 
 }  // namespace
+
+namespace {
+
+// forward.rs:183-195 on a device flow: sets *flag when a component moves by more than 1e-4 (a
+// NaN does not: it fails the comparison, as on the CPU)
+__global__ void k_fw_flow_moves(const float* __restrict__ f, long n, int* flag) {
+    bool moves = false;
+    const long stride = (long)gridDim.x * blockDim.x;
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const float v = f[i];
+        const float a = v < 0.0f ? -v : v;
+        moves |= 0.0001 < (double)a;
+    }
+    if (__any(moves) && (threadIdx.x & 63) == 0) atomicOr(flag, 1);
+}
+
+}  // namespace
+
+void launch_fw_flow_moves(hipStream_t s, const float* flow, long n, int* flag) {
+    const unsigned blocks = (unsigned)std::min<long>((n + 255) / 256, 2048);
+    TWX_LAUNCH(s, F_FW_MESH, 4.0 * (double)n, k_fw_flow_moves<<<blocks, 256, 0, s>>>(flow, n, flag));
+}
 
 size_t fw_scan_blocks(long n) { return (size_t)((n + FW_SCAN_ITEMS - 1) / FW_SCAN_ITEMS); }
 

@@ -179,6 +179,81 @@ inline int select(const twx_retime_params& p, SpeedMap& map, int32_t n, twx_sele
     return TWX_OK;
 }
 
+// ---- the Vectors In effect's time model (plugins/vectors-in/src/timemap.rs, glue.rs:190-253) ----
+
+// Its rates: the input is 30 fps with "In FPS is Out FPS" checked, the slider otherwise; the
+// output is always the composition's (Rates::new without its equal-rates switch). Returns true
+// for the rate error (out < 0.01 or in <= 0.01), which renders the error fill.
+inline bool vin_rates(const twx_vin_params& p, Rates& r) {
+    double in = p.in_fps_is_out_fps ? 30.0 : (double)((float)p.input_frame_rate * 1.0f);
+    double out = p.fps;
+    in = snap_rate(in);
+    out = snap_rate(out);
+    if (0.01 > out || 0.01 >= in) return true;
+    r.in_fps = in;
+    r.out_fps = out;
+    r.lts = p.time_step;
+    r.dur = (float)(((double)p.time_step * out) / in);
+    return false;
+}
+
+// The Speed parameters of a Vectors In run in the form SpeedMap reads.
+inline twx_retime_params vin_speed_params(const twx_vin_params& p) {
+    twx_retime_params rp{};
+    rp.width = p.width;
+    rp.height = p.height;
+    rp.fmt = p.fmt;
+    rp.time_step = p.time_step;
+    rp.fps = p.fps;
+    rp.speed = p.speed;
+    rp.n_speed_keys = p.n_speed_keys;
+    rp.speed_keys = p.speed_keys;
+    return rp;
+}
+
+// T(k) = trunc((f64)dur * (f64)k + 0.5) (timemap.rs:56-59): the layer time of source frame k.
+inline int32_t vin_time(const Rates& r, float k) { return trunc_i32_f64((double)r.dur * (double)k + 0.5); }
+
+// The layer clip index shown at layer time `t` (TimeBase::frame_of), -1 outside the clip.
+inline int64_t vin_clip_index(int64_t t, int32_t lts, int64_t n_frames) {
+    int64_t q = t / lts, m = t % lts;
+    if (m < 0) q -= 1;
+    return (q >= 0 && q < n_frames) ? q : -1;
+}
+
+// The frame selection of output frame n (timeline.rs:447-464 with the Vectors In time map and a
+// constant Mark Segments, whose two markers are equal): the constant vectors quality 4
+// (prerender.rs:50), 0 for a frame with a side of 1.
+inline int vin_select(const twx_vin_params& p, SpeedMap& map, const Rates& r, int32_t n, twx_selection& sel,
+                      std::string& err) {
+    const int32_t cur = n * p.time_step;
+    const float curf = (float)cur, ltsf = (float)p.time_step;
+    const float t0 = curf / ltsf, t1 = ((float)p.time_step + curf) / ltsf;
+    float s0, s1;
+    if (p.time_remap_mode == 2) {  // Frame Number (timeline.rs:274-279), a constant Frame Num
+        s0 = s1 = (float)((r.in_fps / r.out_fps) * (double)(float)p.frame_num);
+    } else {
+        int st = map.src_pos(t0, s0, err);
+        if (!st) st = map.src_pos(t1, s1, err);
+        if (st) return st;
+    }
+    int vq = (p.width >= 2 && p.height >= 2) ? 4 : 0;
+    int ip = p.frame_interp - 1;
+    const int mark = (p.mark_segments >= 1 && p.mark_segments <= 5) ? p.mark_segments - 1 : 0;
+    if (mark == 3) { vq = 0; ip = 0; }
+    if (mark == 4) { vq = 0; ip = 1; }
+    const int32_t a = trunc_i32_f32(s0);
+    sel.s0 = s0;
+    sel.s1 = s1;
+    sel.a = a;
+    sel.b = (int32_t)((uint32_t)a + 1u);
+    sel.wa = 1.0f - (s0 - (float)a);
+    sel.exact = is_exact(s0, s1, (float)p.blur) ? 1 : 0;
+    sel.vectors_quality = vq;
+    sel.interp = ip;
+    return TWX_OK;
+}
+
 inline int check_params(const twx_retime_params& p, std::string& err) {
     if (p.width <= 0 || p.height <= 0 || p.fmt < 0 || p.fmt > 2 || p.time_step <= 0) {
         err = "retime parameters: bad frame size, format or time step";
