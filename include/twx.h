@@ -22,8 +22,9 @@
  *    the legacy default stream before the call; work the caller queued on any other stream that
  *    writes the inputs or touches the outputs must be complete before the call. When the call
  *    returns, its outputs are complete.
- *  - A frame of 2^28 pixels or more is refused with TWX_E_UNSUPPORTED (the kernels address
- *    frames by 32-bit byte offsets, 16 bytes a pixel at most).
+ *  - A frame of 2^28 pixels or more, or with a side of 2^24 pixels or more, is refused with
+ *    TWX_E_UNSUPPORTED (the kernels address frames by 32-bit byte offsets, 16 bytes a pixel at
+ *    most, and block matching forms texel offsets with 24-bit multiplies).
  *  - Status codes map onto StageError (device/image.rs:14-23): any non-zero status makes the
  *    engine rerun the stage on the CPU. `err`/`errlen` receive a message (may be NULL/0).
  */

@@ -134,6 +134,10 @@ def test_frames_of_2_28_pixels_are_refused_on_the_host():
     with pytest.raises(_ffi.StageError) as e:
         twx_select(cfg.struct(), 1)
     assert e.value.status == _ffi.TWX_E_UNSUPPORTED
+    # a side of 2^24 pixels (block matching's 24-bit texel offsets), under 2^28 pixels
+    with pytest.raises(_ffi.StageError) as e:
+        flow_geometry(flow_settings(quality=4), 1 << 24, 4)
+    assert e.value.status == _ffi.TWX_E_UNSUPPORTED
 
 
 def test_host_buffers_are_checked_before_the_library_is_called():

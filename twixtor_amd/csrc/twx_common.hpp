@@ -18,11 +18,18 @@ constexpr int MAX_TAPS = 8;  // half kernels used: r=7 -> 8, r=5 -> 6, binomial 
 // The frame kernels address a pixel of a frame by a 32-bit byte offset from the frame's base, and
 // the widest pixel is 16 bytes (f32 A,R,G,B engine images, 4-channel float images): a frame must
 // hold fewer than 2^28 pixels. Larger frames are refused (TWX_E_UNSUPPORTED, the CPU's cue), never
-// rendered with wrapped offsets.
+// rendered with wrapped offsets. Block matching forms texel offsets with 24-bit multiplies, so a
+// side of 2^24 pixels or more is refused the same way.
 constexpr uint64_t MAX_FRAME_PIXELS = 1ull << 28;
+constexpr long long MAX_FRAME_SIDE = 1ll << 24;
 inline int check_frame_pixels(long long w, long long h, std::string& err) {
     if (w > 0 && h > 0 && (uint64_t)w * (uint64_t)h >= MAX_FRAME_PIXELS) {
         err = "a frame of 2^28 pixels or more (" + std::to_string(w) + "x" + std::to_string(h) +
+              ") is not served by this backend";
+        return TWX_E_UNSUPPORTED;
+    }
+    if (w >= MAX_FRAME_SIDE || h >= MAX_FRAME_SIDE) {
+        err = "a frame side of 2^24 pixels or more (" + std::to_string(w) + "x" + std::to_string(h) +
               ") is not served by this backend";
         return TWX_E_UNSUPPORTED;
     }

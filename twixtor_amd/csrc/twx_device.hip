@@ -1006,48 +1006,58 @@ __global__ void k_negate(const SolveJob* __restrict__ jobs) {
 // -------------------------------------------------------------------------------------------
 // Block matching (flow.rs:446-622)
 
-// Plane element (x, y) lives at x * SX + y * SY: (1, w) for the plain layout, (h, 1) for the
-// transposed copies the row sweeps use (so that the lanes of a wave, one row each, read
-// consecutive addresses). The arithmetic does not depend on the layout.
+// A sweep's coordinates: `a` along its chains, `c` across them (rows: a = x, c = y; columns:
+// a = y, c = x). In walk layout element (a, c) of a plane sits at a * m + c (m: the extent across),
+// so the lanes of a wave, one chain each, read consecutive addresses. The column sweeps run on the
+// plain planes, which are in walk layout for them. The row sweeps read transposed copies of X and
+// Y; the plain flow is in cross layout for them (c * n + a, n: the extent along): sweep 1 reads it
+// there and writes its result in walk layout into the job's scratch, and sweep 2 reads that and
+// writes its result back into the plain flow, so the flow needs no transposing pass.
 
-// A plane is read in runs: `run` r of a plane starts at element r * RS, and its elements are
-// consecutive (RS = h with element (x, y) at x * h + y for the transposed copies: runs are columns;
-// RS = w for the plain layout: runs are rows). Element e of run r, as a byte address whose
-// consecutive elements the compiler can fetch in one wide load.
-__device__ __forceinline__ gchar* run_at(const gfloat* base, uint32_t RS, int r, int e) {
-    return (gchar*)base + ((uint32_t)r * RS + (uint32_t)e) * 4u;
-}
+// The B plane of a sweep (walk layout) behind a buffer resource: the four runs of a cost's 4 x 4
+// texels are one per-lane offset plus four wave-uniform ones.
+struct BmPlane {
+    __amdgpu_buffer_rsrc_t rsrc;
+    int w, h;          // the level's extent
+    uint32_t rs;       // elements per run (m)
+    uint32_t off_max;  // the largest first-texel offset whose 4 x 4 block stays in the plane
+    float wm2, hm2;
+    __device__ BmPlane(const float* base, int w_, int h_, int n_al, int m)
+        : rsrc(__builtin_amdgcn_make_buffer_rsrc((void*)base, 0, w_ * h_ * 4, 0x00020000)), w(w_), h(h_),
+          rs((uint32_t)m), off_max((uint32_t)((n_al - 4) * m + m - 4)), wm2((float)(w_ - 2)), hm2((float)(h_ - 2)) {}
+    __device__ float word(uint32_t i) const {
+        return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rsrc, (int)(i * 4u), 0, 0));
+    }
+};
 
 // Block::cost (flow.rs:472-524) without branches: `ok` false where the reference returns None (the
-// value is then meaningless). The operations and their order are those of the reference; the 4 x 4
-// texels the nine bilinear samples use are read once, as four runs of four.
+// value is then meaningless). The operations and their order are those of the reference.
 template <bool T, typename Between>
-__device__ __forceinline__ float bm_cost(const gfloat* __restrict__ b, int w, int h, const float p[9], float x2, float y2,
-                                         bool& ok, Between&& between) {
-    const float wm2 = (float)(w - 2), hm2 = (float)(h - 2);
-    ok = !(x2 > wm2 || y2 > hm2);
-    // cvtt(wm2 - 1) is w - 3 (w >= 3); elsewhere the hardware truncation (NaN gives 0, out of
-    // range saturates) equals cvtt wherever the cost exists, and gives ix < 1 or x2 > wm2 wherever
-    // cvtt does, so `ok` comes out the same
-    const int32_t tx2 = trunc_hw(x2), ty2 = trunc_hw(y2);
-    const int32_t ix = x2 == wm2 ? w - 3 : tx2, iy = y2 == hm2 ? h - 3 : ty2;
-    const float fx = x2 == wm2 ? 1.0f : x2 - (float)tx2, fy = y2 == hm2 ? 1.0f : y2 - (float)ty2;
+__device__ __forceinline__ float bm_cost(const BmPlane& B, const float p[9], float x2, float y2, bool& ok,
+                                         Between&& between) {
+    ok = !(x2 > B.wm2 || y2 > B.hm2);
+    // cvtt(x2) for x2 < w - 2: the hardware truncation equals cvtt wherever the cost exists and
+    // gives ix < 1 wherever cvtt does (NaN gives 0, out of range saturates); at x2 == w - 2 the
+    // reference takes ix = w - 3 and fx = 1, which the clamp gives (x2 - (w - 3) is exactly 1)
+    const int32_t ix = min(trunc_hw(x2), B.w - 3), iy = min(trunc_hw(y2), B.h - 3);
     ok = ok && !(ix < 1 || iy < 1);
+    const float fx = x2 - (float)ix, fy = y2 - (float)iy;
     const float w11 = fy * fx;
     const float w10 = fx - w11;
     const float w01 = fy - w11;
     const float w00 = ((1.0f - fx) - fy) + w11;
-    // texels (ix - 1 .. ix + 2, iy - 1 .. iy + 2), inside the plane when ok (ix <= w - 3 and
-    // iy <= h - 3 then); otherwise the first four elements, read four times, and not used
-    uint32_t RS = T ? h : w;
-    int r0 = T ? ix - 1 : iy - 1, e0 = T ? iy - 1 : ix - 1;
-    if (!ok) { RS = 0; r0 = 0; e0 = 0; }
+    // texels (ix - 1 .. ix + 2, iy - 1 .. iy + 2) as four runs of four, the first at element
+    // (r - 1) * rs + (e - 1) (r, e: run and element of texel (ix, iy)). Sides are below 2^24, so
+    // the 24-bit product is exact wherever the cost exists; any other offset is clamped into the
+    // plane, and its texels are not used.
+    const uint32_t r = (uint32_t)(T ? ix : iy), e = (uint32_t)(T ? iy : ix);
+    const uint32_t off = min(__umul24(r, B.rs) + e - (B.rs + 1u), B.off_max) * 4u;
     float t[4][4];
 #pragma unroll
-    for (int r = 0; r < 4; r++) {
-        gchar* q = run_at(b, RS, r0 + r, e0);
+    for (int k = 0; k < 4; k++) {
+        const auto v = __builtin_amdgcn_raw_buffer_load_b128(B.rsrc, (int)off, (int)((uint32_t)k * B.rs * 4u), 0);
 #pragma unroll
-        for (int i = 0; i < 4; i++) t[r][i] = *(const gfloat*)(q + 4 * i);
+        for (int i = 0; i < 4; i++) t[k][i] = __uint_as_float(v[i]);
     }
     between();  // the caller's work between the texel loads and their use
     // texel (ix - 1 + dx, iy - 1 + dy)
@@ -1068,52 +1078,54 @@ __device__ __forceinline__ float bm_cost(const gfloat* __restrict__ b, int w, in
     return sq[8] + s;
 }
 
-// The buffers of a block-matching sweep in its walk coordinates: element (a, c) (a along the
-// chains, c across them) sits at a * m + c. T = true: the row sweeps on the transposed copies
-// (flow in tmp, X and Y in tb); T = false: the column sweeps on the plain planes.
-template <bool T>
-struct BmView {
-    int w, h, n_al, m;
-    gfloat_rw* fl;
-    const gfloat* A;
-    const gfloat* B;
-    __device__ BmView(const SolveJob& J, int level, int buf) {
-        w = J.w[level];
-        h = J.h[level];
-        n_al = T ? w : h;
-        m = T ? h : w;
-        fl = (gfloat_rw*)(T ? J.tmp : J.flow[buf]);
-        A = (const gfloat*)(T ? J.tb : J.X[level]);
-        B = (const gfloat*)(T ? J.tb + (long)w * h : J.Y[level]);
+// A flow plane as a sweep reads or writes it: pixel (a, c) at byte base + a * step + lane, where
+// `lane` is the chain's own part (c * 8 in walk layout, c * n * 8 in cross layout) and a * step
+// is wave-uniform.
+struct BmFlow {
+    gchar* base;
+    uint32_t step, lane;
+    __device__ BmFlow(const float* p, bool walk, int n_al, int m, int c)
+        : base((gchar*)p), step(walk ? 8u * (uint32_t)m : 8u), lane(walk ? 8u * (uint32_t)c : 8u * (uint32_t)c * (uint32_t)n_al) {}
+    __device__ float2 at(int a) const {
+        const gfloat* q = (const gfloat*)(base + ((uint32_t)a * step + lane));
+        return make_float2(q[0], q[1]);
+    }
+    __device__ void put(int a, float u, float v) const {
+        gfloat_rw* q = (gfloat_rw*)(base + ((uint32_t)a * step + lane));
+        q[0] = u;
+        q[1] = v;
     }
 };
 
-// Transposed copies for the row sweeps: the flow (buffer `buf`) into tmp, X and Y into tb's two
-// planes; OUT copies tmp back into the flow. With them the lanes of a wave (one row each) read
-// consecutive addresses, as the column sweeps do on the plain layout.
-template <bool OUT>
+// Transposed copies for the row sweeps, through 32 x 32 tiles in LDS: X and Y (level `level`)
+// into tb's two planes and, with FLOW, the flow (buffer `buf`) into the scratch; OUT copies the
+// scratch back into the flow. A flow pixel (u, v) moves as one 8-byte access.
+template <bool FLOW, bool OUT>
 __global__ void k_bm_transpose(const SolveJob* __restrict__ jobs, int level, int buf) {
     const SolveJob& J = jobs[blockIdx.z];
     if (level > J.start) return;
     const int w = J.w[level], h = J.h[level];
-    if (w < 3 || h < 3) return;
-    __shared__ float t[4][32][33];
+    if (w < 4 || h < 4) return;
+    constexpr int NP = OUT ? 2 : (FLOW ? 4 : 2);
+    __shared__ float t[NP][32][33];
     const int bx = blockIdx.x * 32, by = blockIdx.y * 32;
     if (bx >= w || by >= h) return;
     const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 256 threads: 32 x 8
     const long px = (long)w * h;
-    float2* flow = reinterpret_cast<float2*>(J.flow[buf]);  // a flow pixel (u, v) as one access
+    float2* flow = reinterpret_cast<float2*>(J.flow[buf]);
     float2* ft = reinterpret_cast<float2*>(J.tmp);
     if (!OUT) {
         for (int r = ty; r < 32; r += 8) {
             int x = bx + tx, y = by + r;
             if (x < w && y < h) {
                 long i = (long)y * w + x;
-                const float2 f = flow[i];
-                t[0][r][tx] = f.x;
-                t[1][r][tx] = f.y;
-                t[2][r][tx] = J.X[level][i];
-                t[3][r][tx] = J.Y[level][i];
+                t[0][r][tx] = J.X[level][i];
+                t[1][r][tx] = J.Y[level][i];
+                if (FLOW) {
+                    const float2 f = flow[i];
+                    t[NP - 2][r][tx] = f.x;
+                    t[NP - 1][r][tx] = f.y;
+                }
             }
         }
         __syncthreads();
@@ -1121,17 +1133,16 @@ __global__ void k_bm_transpose(const SolveJob* __restrict__ jobs, int level, int
             int x = bx + r, y = by + tx;
             if (x < w && y < h) {
                 long o = (long)x * h + y;
-                ft[o] = make_float2(t[0][tx][r], t[1][tx][r]);
-                J.tb[o] = t[2][tx][r];
-                J.tb[px + o] = t[3][tx][r];
+                J.tb[o] = t[0][tx][r];
+                J.tb[px + o] = t[1][tx][r];
+                if (FLOW) ft[o] = make_float2(t[NP - 2][tx][r], t[NP - 1][tx][r]);
             }
         }
     } else {
         for (int r = ty; r < 32; r += 8) {
             int x = bx + r, y = by + tx;
             if (x < w && y < h) {
-                long o = (long)x * h + y;
-                const float2 f = ft[o];
+                const float2 f = ft[(long)x * h + y];
                 t[0][tx][r] = f.x;
                 t[1][tx][r] = f.y;
             }
@@ -1139,10 +1150,7 @@ __global__ void k_bm_transpose(const SolveJob* __restrict__ jobs, int level, int
         __syncthreads();
         for (int r = ty; r < 32; r += 8) {
             int x = bx + tx, y = by + r;
-            if (x < w && y < h) {
-                long i = (long)y * w + x;
-                flow[i] = make_float2(t[0][r][tx], t[1][r][tx]);
-            }
+            if (x < w && y < h) flow[(long)y * w + x] = make_float2(t[0][r][tx], t[1][r][tx]);
         }
     }
 }
@@ -1153,22 +1161,41 @@ constexpr int BM_CHUNK = 6;
 // (measured: the row sweeps gain from prefetching four steps ahead, the column sweeps lose at
 // every distance tried and do not prefetch)
 constexpr int BM_PREFETCH_ROWS = 4, BM_PREFETCH_COLS = 0;
+// The row sweeps read and write the plain flow directly (cross layout: one line per chain) on
+// levels of up to this many pixels over the launch's jobs, and run on a transposed copy above it
+// (measured on 148 solves of 960 x 540: cross layout at the 960 x 540 level costs 1.5 ms more
+// than the copies, at 480 x 270 and below it saves their time)
+constexpr double BM_CROSS_MAX_PIXELS = 32e6;
 
 // The state of one lane of a block-matching chain and its steps (Block::step, flow.rs:528-544),
 // run CH at a time with the loop unrolled: the patch runs sit in ring slots that rotate with the
 // step (slot of patch run r at step s: (s + r) mod 3 for DIR = +1, (r - s) mod 3 for DIR = -1;
 // CH is a multiple of 3, so every chunk starts from the same assignment), and the decision
 // updates the incoming vector by selects, so values move between registers only at chunk ends.
-template <bool T, int DIR, int PD>
+//
+// The decision needs no range test of its own: the reference's test that the pixel's own target
+// lies in [0, w - 1] x [0, h - 1] holds wherever the own cost exists (that cost needs the target
+// in [1, w - 2] x [1, h - 2]). A cost that does not exist is replaced by -inf for the own vector
+// and +inf for the incoming one, so "both exist and incoming < own" is one comparison.
+template <bool T, int DIR, int PD, bool CROSS>
 struct BmChain {
-    const BmView<T>& V;
+    static constexpr bool ALWAYS = T && CROSS;  // the sweep writes every pixel (it moves the flow between layouts)
+    const BmPlane& B;
+    gchar* A;        // X in walk layout
+    uint32_t a_lane;  // byte offset of the chain's patch run, across (ce - 1) * 4
+    uint32_t a_step;  // bytes per run (m * 4)
+    const BmFlow& fr;
+    const BmFlow& fw;
     float pf[3];  // the prefetched words of the last step
     int m, n_al, n, ce;
-    bool live;
     int role;
+    float sent;   // the cost where it does not exist: -inf for role 0, +inf for role 1
+    float cf;     // (float)ce
     int a;
-    uint32_t e;
     float cu, cv, nu, nv;
+    bool wr;      // the lane stores (role 0 of a live chain)
+
+    __device__ gchar* arun(int r) const { return A + ((uint32_t)r * a_step + a_lane); }
 
     template <int CH, bool CHECK>
     __device__ __forceinline__ void chunk(int k0, float (&ring)[3][3]) {
@@ -1178,16 +1205,13 @@ struct BmChain {
             const int k = k0 + st;
             if (CHECK && k >= n) break;
             const int an = k + 1 < n ? a + DIR : a;
-            const uint32_t en = (uint32_t)an * m + ce;
-            const float cu_n = ldg(V.fl, 2 * en), cv_n = ldg(V.fl, 2 * en + 1);
+            const float2 cn = fr.at(an);
             // the run entering the patch at the next step (a + 2 DIR), clamped at the chain's end
             const int ar = min(max(an + DIR, 0), n_al - 1);
             float nr[3];
-            {
-                gchar* q = run_at(V.A, (uint32_t)m, ar, ce - 1);
+            gchar* q = arun(ar);
 #pragma unroll
-                for (int i = 0; i < 3; i++) nr[i] = *(const gfloat*)(q + 4 * i);
-            }
+            for (int i = 0; i < 3; i++) nr[i] = *(const gfloat*)(q + 4 * i);
             const int s0 = DIR > 0 ? st % 3 : (3 - st % 3) % 3;        // slot of patch run 0
             const int s1 = DIR > 0 ? (st + 1) % 3 : (4 - st % 3) % 3;  // run 1
             const int s2 = DIR > 0 ? (st + 2) % 3 : (5 - st % 3) % 3;  // run 2
@@ -1197,9 +1221,10 @@ struct BmChain {
             for (int bb = 0; bb < 3; bb++)
 #pragma unroll
                 for (int aa = 0; aa < 3; aa++) p[bb * 3 + aa] = T ? ring[sl[aa]][bb] : ring[sl[bb]][aa];
-            const int x = T ? a : ce, y = T ? ce : a;
-            const float tx = (float)x - cu, ty = (float)y - cv;
-            const bool in_range = 0.0f <= tx && tx <= (float)(V.w - 1) && 0.0f <= ty && ty <= (float)(V.h - 1);
+            const float af = (float)a;
+            const float xf = T ? af : cf, yf = T ? cf : af;
+            // role 0 costs the pixel's own vector, role 1 the incoming one
+            const float vu = role ? nu : cu, vv = role ? nv : cv;
             // Every step touches new lines of A, B and the flow; a load that misses L2 costs the
             // step its full latency, since the step waits on its dependent B texels and the memory
             // counter retires loads in order. So each step asks, right after its own B loads, for
@@ -1211,59 +1236,54 @@ struct BmChain {
                 if (PD == 0) return;
                 asm volatile("" ::"v"(pf[0]), "v"(pf[1]), "v"(pf[2]));
                 const int ap = a + PD * DIR;
-                const float pu = role ? nu : cu, pv = role ? nv : cv;
-                const float al = (float)ap - (T ? pu : pv), ac = (float)ce - (T ? pv : pu);
+                const float al = (float)ap - (T ? vu : vv), ac = cf - (T ? vv : vu);
                 const int rb = min(max(trunc_hw(fminf(fmaxf(al, -8.0f), (float)n_al + 8.0f)) + (DIR > 0 ? 2 : -1), 0), n_al - 1);
                 const int eb = min(max(trunc_hw(fminf(fmaxf(ac, -8.0f), (float)m + 8.0f)) - 1, 0), m - 1);
                 const int ra = min(max(ap + 2 * DIR, 0), n_al - 1);
                 const int rf = min(max(ap + DIR, 0), n_al - 1);
-                pf[0] = *(const gfloat*)run_at(V.B, (uint32_t)m, rb, eb);
-                pf[1] = *(const gfloat*)run_at(V.A, (uint32_t)m, ra, ce - 1);
-                pf[2] = ldg(V.fl, 2 * ((uint32_t)rf * m + ce));
+                pf[0] = B.word(__umul24((uint32_t)rb, (uint32_t)m) + (uint32_t)eb);
+                pf[1] = *(const gfloat*)arun(ra);
+                pf[2] = fr.at(rf).x;
             };
             bool ok;
-            const float c = bm_cost<T>(V.B, V.w, V.h, p, role ? (float)x - nu : tx, role ? (float)y - nv : ty, ok, prefetch);
-            // the partner lane's values by a quad permutation [1, 0, 3, 2] (every lane active)
-            const float c_o = __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(c), 0xB1, 0xF, 0xF, false));
-            const int ok_o = __builtin_amdgcn_mov_dpp(ok ? 1 : 0, 0xB1, 0xF, 0xF, false);
-            const float cost0 = role ? c_o : c, cost1 = role ? c : c_o;
-            const bool ok0 = role ? ok_o != 0 : ok, ok1 = role ? ok : ok_o != 0;
-            const bool take = in_range && ok0 && ok1 && cost1 < cost0;
-            if (take && live && role == 0) {
-                *(gfloat_rw*)((gchar*)V.fl + 8u * e) = nu;
-                *(gfloat_rw*)((gchar*)V.fl + 8u * e + 4u) = nv;
-            }
+            const float c = bm_cost<T>(B, p, xf - vu, yf - vv, ok, prefetch);
+            const float cs = ok ? c : sent;
+            // the partner lane's cost by a quad permutation [1, 0, 3, 2] (every lane active)
+            const float co = __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(cs), 0xB1, 0xF, 0xF, false));
+            // incoming < own: role 0 holds own, role 1 incoming
+            const float inc = role ? cs : co, own = role ? co : cs;
+            const bool take = inc < own;
             nu = take ? nu : cu;
             nv = take ? nv : cv;
+            if (wr && (ALWAYS || take)) fw.put(a, nu, nv);
             // the run leaving the patch (run 0 for DIR = +1, run 2 for DIR = -1) makes room for the
             // entering one
 #pragma unroll
             for (int i = 0; i < 3; i++) ring[DIR > 0 ? s0 : s2][i] = nr[i];
-            e = en;
             a += DIR;
-            cu = cu_n;
-            cv = cv_n;
+            cu = cn.x;
+            cv = cn.y;
         }
     }
 };
 
 // The four sweeps of block matching (flow.rs:551-614) as one kernel: T = true runs sweeps 1
 // (DIR = +1, left to right) and 2 (DIR = -1) with one serial chain per interior row, T = false
-// sweeps 3 (top to bottom) and 4 with one chain per interior column (BmView: adjacent chains are
-// adjacent lanes, so every load is coalesced). Whether a pixel keeps its own vector decides the
-// next pixel's input, and in practice more than half the pixels take the incoming vector, in long
-// runs: a chain does not split usefully. A lane pair runs each chain: role 0 computes the cost of
-// the pixel's own vector, role 1 the incoming vector's, in one instruction stream, and both take
-// the decision (Block::step, flow.rs:528-544). The A patch slides along the chain, one new run
-// per step; the loop runs in unrolled chunks of CH steps so that the values loaded one step ahead
-// need fewer register copies.
-template <bool T, int DIR, int CH>
+// sweeps 3 (top to bottom) and 4 with one chain per interior column (adjacent chains are adjacent
+// lanes). Whether a pixel keeps its own vector decides the next pixel's input, and in practice
+// more than half the pixels take the incoming vector, in long runs: a chain does not split
+// usefully. A lane pair runs each chain: role 0 computes the cost of the pixel's own vector, role
+// 1 the incoming vector's, in one instruction stream, and both take the decision (Block::step,
+// flow.rs:528-544). The A patch slides along the chain, one new run per step; the loop runs in
+// unrolled chunks of CH steps so that the values loaded one step ahead need fewer register copies.
+template <bool T, int DIR, int CH, bool CROSS>
 __global__ __launch_bounds__(64) void k_bm_sweep(const SolveJob* __restrict__ jobs, int level, int buf) {
     const SolveJob& J = jobs[blockIdx.z];
     if (level > J.start) return;
-    if (J.w[level] < 3 || J.h[level] < 3) return;
-    const BmView<T> V(J, level, buf);
-    const int n_al = V.n_al, m = V.m;
+    const int w = J.w[level], h = J.h[level];
+    // with a side below 4 no cost exists (it needs 1 <= ix <= w - 3), so no pixel changes
+    if (w < 4 || h < 4) return;
+    const int n_al = T ? w : h, m = T ? h : w;
     const int c0 = 1 + blockIdx.x * 32;
     if (c0 > m - 2) return;  // the whole workgroup is past the chains (uniform)
     const int n = n_al - 2;
@@ -1273,24 +1293,33 @@ __global__ __launch_bounds__(64) void k_bm_sweep(const SolveJob* __restrict__ jo
     const int cc = c0 + (lane >> 1);
     const bool live = cc <= m - 2;
     const int ce = live ? cc : m - 2;
-    const uint32_t RS = m;
-    int a = a0;
-    uint32_t e = (uint32_t)a * m + ce;
-    const uint32_t ni = (uint32_t)(a - DIR) * m + ce;
-    float nu = ldg(V.fl, 2 * ni), nv = ldg(V.fl, 2 * ni + 1);
-    float cu = ldg(V.fl, 2 * e), cv = ldg(V.fl, 2 * e + 1);
+    const BmPlane B(T ? J.tb + (long)w * h : J.Y[level], w, h, n_al, m);
+    gchar* A = (gchar*)(T ? J.tb : J.X[level]);
+    // flow planes: the plain flow (walk layout for the columns, cross for the rows) and the rows'
+    // scratch (walk layout). CROSS rows: sweep 1 reads plain and writes scratch, sweep 2 the other
+    // way round; otherwise the rows run on a transposed copy of the flow in the scratch.
+    const BmFlow plain(J.flow[buf], !T, n_al, m, ce), scratch(J.tmp, true, n_al, m, ce);
+    const BmFlow& fr = T && (DIR < 0 || !CROSS) ? scratch : plain;
+    const BmFlow& fw = T && (DIR > 0 || !CROSS) ? scratch : plain;
+    const BmFlow& fi = T && !CROSS ? scratch : plain;
+    // the incoming vector of the first pixel: the chain's end pixel, which no sweep changes
+    const float2 f0 = fi.at(a0 - DIR), f1 = fr.at(a0);
+    float nu = f0.x, nv = f0.y, cu = f1.x, cv = f1.y;
     // the A patch (Block::patch, flow.rs:459-470) as three runs of three: run r is A at along
     // a - 1 + r, across ce - 1 .. ce + 1; the runs live in a ring whose slots rotate with the step
+    const uint32_t a_lane = 4u * (uint32_t)(ce - 1), a_step = 4u * (uint32_t)m;
     float ring[3][3];
 #pragma unroll
     for (int r = 0; r < 3; r++) {
-        gchar* q = run_at(V.A, RS, a - 1 + r, ce - 1);
+        gchar* q = A + ((uint32_t)(a0 - 1 + r) * a_step + a_lane);
 #pragma unroll
         for (int i = 0; i < 3; i++) ring[r][i] = *(const gfloat*)(q + 4 * i);
     }
     asm volatile("" ::"v"(nu), "v"(nv), "v"(cu), "v"(cv), "v"(ring[0][0]), "v"(ring[0][1]), "v"(ring[0][2]),
                  "v"(ring[1][0]), "v"(ring[1][1]), "v"(ring[1][2]), "v"(ring[2][0]), "v"(ring[2][1]), "v"(ring[2][2]));
-    BmChain<T, DIR, T ? BM_PREFETCH_ROWS : BM_PREFETCH_COLS> ch{V, {0.0f, 0.0f, 0.0f}, m, n_al, n, ce, live, role, a, e, cu, cv, nu, nv};
+    BmChain<T, DIR, T ? BM_PREFETCH_ROWS : BM_PREFETCH_COLS, CROSS> ch{
+        B, A, a_lane, a_step, fr, fw, {0.0f, 0.0f, 0.0f}, m, n_al, n, ce, role,
+        role ? INFINITY : -INFINITY, (float)ce, a0, cu, cv, nu, nv, live && role == 0};
     const int nfull = n / CH;
     for (int j = 0; j < nfull; j++) ch.template chunk<CH, false>(j * CH, ring);
     if (nfull * CH < n) ch.template chunk<CH, true>(nfull * CH, ring);
@@ -1445,13 +1474,21 @@ static void launch_solve_range(hipStream_t s, const SolveJob* d_jobs, const std:
         dim3 tb(256), tg((mw + 31) / 32, (mh + 31) / 32, nj);
         // a lane pair per chain: 32 chains per 64-lane block
         dim3 cb(64), rg((mh + 31) / 32, 1, nj), cg((mw + 31) / 32, 1, nj);
-        // transposes 24 + 16 B/px, sweeps: A 4 + B 4 + flow read and write 16 B/px each
-        TWX_LAUNCH(s, F_BM_TRANSPOSE, 24.0 * P, k_bm_transpose<false><<<tg, tb, 0, s>>>(d_jobs, l, buf));
-        TWX_LAUNCH(s, F_BM_ROWS, 24.0 * P, (k_bm_sweep<true, 1, BM_CHUNK><<<rg, cb, 0, s>>>(d_jobs, l, buf)));
-        TWX_LAUNCH(s, F_BM_ROWS, 24.0 * P, (k_bm_sweep<true, -1, BM_CHUNK><<<rg, cb, 0, s>>>(d_jobs, l, buf)));
-        TWX_LAUNCH(s, F_BM_TRANSPOSE, 16.0 * P, k_bm_transpose<true><<<tg, tb, 0, s>>>(d_jobs, l, buf));
-        TWX_LAUNCH(s, F_BM_COLS, 24.0 * P, (k_bm_sweep<false, 1, BM_CHUNK><<<cg, cb, 0, s>>>(d_jobs, l, buf)));
-        TWX_LAUNCH(s, F_BM_COLS, 24.0 * P, (k_bm_sweep<false, -1, BM_CHUNK><<<cg, cb, 0, s>>>(d_jobs, l, buf)));
+        // sweeps: A 4 + B 4 + flow read and write 16 B/px each
+        if (P <= BM_CROSS_MAX_PIXELS) {
+            // transposed X and Y 8 + 8 B/px; the row sweeps move the flow between layouts
+            TWX_LAUNCH(s, F_BM_TRANSPOSE, 16.0 * P, (k_bm_transpose<false, false><<<tg, tb, 0, s>>>(d_jobs, l, buf)));
+            TWX_LAUNCH(s, F_BM_ROWS, 24.0 * P, (k_bm_sweep<true, 1, BM_CHUNK, true><<<rg, cb, 0, s>>>(d_jobs, l, buf)));
+            TWX_LAUNCH(s, F_BM_ROWS, 24.0 * P, (k_bm_sweep<true, -1, BM_CHUNK, true><<<rg, cb, 0, s>>>(d_jobs, l, buf)));
+        } else {
+            // transposed X, Y and flow 16 + 16 B/px, and the flow back 8 + 8 B/px
+            TWX_LAUNCH(s, F_BM_TRANSPOSE, 32.0 * P, (k_bm_transpose<true, false><<<tg, tb, 0, s>>>(d_jobs, l, buf)));
+            TWX_LAUNCH(s, F_BM_ROWS, 24.0 * P, (k_bm_sweep<true, 1, BM_CHUNK, false><<<rg, cb, 0, s>>>(d_jobs, l, buf)));
+            TWX_LAUNCH(s, F_BM_ROWS, 24.0 * P, (k_bm_sweep<true, -1, BM_CHUNK, false><<<rg, cb, 0, s>>>(d_jobs, l, buf)));
+            TWX_LAUNCH(s, F_BM_TRANSPOSE, 16.0 * P, (k_bm_transpose<false, true><<<tg, tb, 0, s>>>(d_jobs, l, buf)));
+        }
+        TWX_LAUNCH(s, F_BM_COLS, 24.0 * P, (k_bm_sweep<false, 1, BM_CHUNK, false><<<cg, cb, 0, s>>>(d_jobs, l, buf)));
+        TWX_LAUNCH(s, F_BM_COLS, 24.0 * P, (k_bm_sweep<false, -1, BM_CHUNK, false><<<cg, cb, 0, s>>>(d_jobs, l, buf)));
     };
     for (int l = max_start; l >= 0; l--) {
         int mw = 0, mh = 0;
