@@ -151,6 +151,25 @@ int twx_solve_pair(twx_handle* h, const void* a, const void* b, int32_t width, i
                    int32_t fmt, const twx_flow_settings* fs, float* f_ab_out, float* f_ba_out,
                    char* err, size_t errlen);
 
+/* Many solve_pair calls (Backend::solve_pair, device/exec/backend.rs:191-208) in one batched
+ * launch sequence, for a caller that wants flows rather than frames (the batched solve of a
+ * clip's pairs): pair k writes F_AB = solve(A, B) into f_ab and F_BA = solve(B, A) into f_ba;
+ * either may be NULL to skip that direction. `a` and `b` are DEVICE engine images (rows as given,
+ * like twx_solve's host images), all of one size and format; f_ab and f_ba are DEVICE buffers of
+ * flow_w * flow_h * 2 floats (twx_flow_geometry), distinct from each other and from the images.
+ * Each distinct image pointer has its pyramid built once per call. The flows are those of n
+ * twx_solve_pair calls, byte for byte; a NaN in any flow gives TWX_E_NAN_OUTPUT (the outputs
+ * are then unspecified). */
+typedef struct {
+    const void* a;
+    const void* b;
+    float* f_ab;
+    float* f_ba;
+} twx_pair;
+int twx_solve_pairs_batched(twx_handle* h, int32_t n, const twx_pair* pairs, int32_t width,
+                            int32_t height, int32_t fmt, const twx_flow_settings* fs, char* err,
+                            size_t errlen);
+
 /* ---- Backend::compose_vectors (compose/vector.rs:209-284) -------------------------------- */
 
 /* The composition of one output frame from its two frames and their flows, mode 1, no 360.

@@ -16,13 +16,18 @@ TWX_FMT_U8, TWX_FMT_U16, TWX_FMT_F32 = 0, 1, 2
 # every entry point of include/twx.h
 EXPORTS = [
     "twx_open", "twx_close", "twx_selfcheck", "twx_memory_budget", "twx_flow_geometry", "twx_solve",
-    "twx_solve_pair", "twx_compose_vectors", "twx_select", "twx_retime_frames_dev", "twx_retime_batch",
+    "twx_solve_pair", "twx_solve_pairs_batched", "twx_compose_vectors", "twx_select", "twx_retime_frames_dev", "twx_retime_batch",
     "twx_retime_frames",
     "twx_forward_warp", "twx_encode_vectors", "twx_decode_vectors", "twx_flow_from_vectors", "twx_invert_flow",
     "twx_create_vectors_dev", "twx_warp_blend", "twx_warp_blend_backward", "twx_warp_blend_backward_noalloc",
     "twx_vectors_in_frames", "twx_vectors_in_frames_dev",
     "twx_set_graphs", "twx_set_tile_shape", "twx_tile_choices", "twx_get_stats", "twx_cache_stats", "twx_stream", "twx_profile_enable", "twx_profile_reset", "twx_profile_read",
 ]
+
+
+class Pair(ctypes.Structure):
+    """twx_pair: device engine images A, B and device flow outputs (either may be NULL)."""
+    _fields_ = [("a", ctypes.c_void_p), ("b", ctypes.c_void_p), ("f_ab", ctypes.c_void_p), ("f_ba", ctypes.c_void_p)]
 
 
 class FlowSettings(ctypes.Structure):
@@ -92,6 +97,7 @@ def lib():
                                         ctypes.POINTER(I32)]
         L.twx_solve.argtypes = [P, P, P, I32, I32, I32, ctypes.POINTER(FlowSettings), P] + ERR
         L.twx_solve_pair.argtypes = [P, P, P, I32, I32, I32, ctypes.POINTER(FlowSettings), P, P] + ERR
+        L.twx_solve_pairs_batched.argtypes = [P, I32, ctypes.POINTER(Pair), I32, I32, I32, ctypes.POINTER(FlowSettings)] + ERR
         L.twx_compose_vectors.argtypes = [P, I32, ctypes.c_float, P, P, I32, I32, I32, P, P, I32, I32, I32, I32, I32, P] + ERR
         L.twx_forward_warp.argtypes = [P, P, I32, I32, I32, P, I32, I32, ctypes.c_float, I32, P, P] + ERR
         L.twx_select.argtypes = [ctypes.POINTER(RetimeParams), I32, ctypes.POINTER(Selection)] + ERR

@@ -175,6 +175,18 @@ class HipBackend:
                                              err, 512), err)
         return f_ab, f_ba
 
+    def solve_pairs_batched(self, pairs, width, height, fmt, fs):
+        """Many solve_pair calls in one batch on device-resident images: `pairs` holds
+        (a_ptr, b_ptr, f_ab_ptr, f_ba_ptr) device pointers (a flow pointer may be None to skip
+        that direction); each flow buffer holds flow_w * flow_h * 2 floats."""
+        arr = (_ffi.Pair * max(len(pairs), 1))()
+        for k, (a, b, fab, fba) in enumerate(pairs):
+            arr[k] = _ffi.Pair(a, b, fab, fba)
+        err = _ffi.errbuf()
+        caller_ready()
+        _ffi.check(_ffi.lib().twx_solve_pairs_batched(self._h, len(pairs), arr, width, height, fmt, ctypes.byref(fs),
+                                                      err, 512), err)
+
     def cache_stats(self) -> tuple:
         """Backend::cache_stats (device/exec/backend.rs:42-45): (hits, misses, evictions, bytes
         held) of the resident pyramid cache behind solve and solve_pair."""

@@ -1837,6 +1837,62 @@ int twx_create_vectors_dev(twx_handle* h, const twx_flow_settings* fs, int32_t w
     });
 }
 
+// ---- many solve_pair calls in one batch (device/exec/backend.rs:191-208 per pair) ----
+
+int twx_solve_pairs_batched(twx_handle* h, int32_t n, const twx_pair* pairs, int32_t width, int32_t height, int32_t fmt,
+                            const twx_flow_settings* fs, char* err, size_t errlen) {
+    return entry(h, err, errlen, [&](std::string& e) -> int {
+        if (n < 0 || (n > 0 && !pairs) || !fs || fmt < 0 || fmt > 2) {
+            e = "solve pairs: bad arguments";
+            return TWX_E_MISMATCH;
+        }
+        for (int k = 0; k < n; k++)
+            if (!pairs[k].a || !pairs[k].b) {
+                e = "solve pairs: pair " + std::to_string(k) + " has a null image";
+                return TWX_E_MISMATCH;
+            }
+        Geo g;
+        int st = geometry(*fs, width, height, g, e);
+        if (st) return st;
+        after_caller(h);
+        hipStream_t s = h->stream;
+        double t0 = now_ms();
+        DevPool pool(h);
+        // one pyramid per distinct image of the call
+        std::map<const void*, int> slot_of;
+        std::vector<PyrIn> pyrs;
+        auto levels = [&](const void* img) -> const float* {
+            auto it = slot_of.find(img);
+            if (it != slot_of.end()) {
+                h->stats.pyramid_hits++;
+                return pyrs[it->second].levels;
+            }
+            pyrs.push_back(PyrIn{img, width, height, fmt, g, pool.alloc<float>(pyramid_floats(g))});
+            slot_of[img] = (int)pyrs.size() - 1;
+            return pyrs.back().levels;
+        };
+        std::vector<SolveIn> solves;
+        for (int k = 0; k < n; k++) {
+            const twx_pair& P = pairs[k];
+            if (!P.f_ab && !P.f_ba) continue;
+            const float* la = levels(P.a);
+            const float* lb = levels(P.b);
+            // solve_pair order: F_BA (X = B), then F_AB (X = A); the flows end in the caller's buffers
+            if (P.f_ba) solves.push_back(SolveIn{lb, la, g, P.f_ba});
+            if (P.f_ab) solves.push_back(SolveIn{la, lb, g, P.f_ab});
+        }
+        build_pyramids(s, pool, pyrs);
+        h->stats.pyramid_builds += pyrs.size();
+        h->stats.pair_solves += solves.size();
+        h->stats.solve_calls++;
+        if ((st = run_solves(s, pool, solves, *fs, h->budget / 2, e))) return st;
+        HIPCHECK(hipStreamSynchronize(s));
+        HIPCHECK(hipGetLastError());
+        h->stats.device_ms += now_ms() - t0;
+        return TWX_OK;
+    });
+}
+
 // ---- the Vectors In render (plugins/vectors-in/src/glue.rs:380-495, vectorin.rs, timemap.rs) ----
 
 // One layer of a Vectors In render: wire frames on the device, frame i showing at ticks
