@@ -313,8 +313,10 @@ int twx_warp_blend(twx_handle* h, int32_t channels, int32_t width, int32_t heigh
 /* The backward pass of twx_warp_blend for the upstream gradient grad_out (same shape as out):
  * dL/dA, dL/dB (images) and dL/dF_AB, dL/dF_BA (flows), each written when its pointer is not
  * NULL. Sample points are recomputed, not stored. The flow gradients are summed per flow texel
- * in a fixed order (deterministic); the image gradients are scattered with atomic adds. DEVICE
- * pointers. */
+ * in a fixed order; the image gradients are scattered to the sampled texels and summed in 64-bit
+ * fixed point (each term rounded at 2^-34 of the largest |grad_out|, power-of-two scaled), so
+ * every gradient is the same bits on every run. An upstream gradient holding an infinity or a NaN
+ * is refused with TWX_E_UNSUPPORTED when an image gradient is asked for. DEVICE pointers. */
 int twx_warp_blend_backward(twx_handle* h, int32_t channels, int32_t width, int32_t height,
                             const float* a, const float* b, const float* f_ab, const float* f_ba,
                             int32_t flow_w, int32_t flow_h, float wa, const float* grad_out,
@@ -335,6 +337,20 @@ int twx_warp_blend_backward_noalloc(twx_handle* h, int32_t channels, int32_t wid
                                     const float* f_ba, int32_t flow_w, int32_t flow_h, float wa,
                                     const float* grad_out, float* grad_a, float* grad_b,
                                     float* grad_f_ab, float* grad_f_ba, char* err, size_t errlen);
+
+/* Photometric refinement of a flow pair on the device, in one call: `steps` Adam steps (betas 0.9
+ * and 0.999, eps 1e-8, torch.optim.Adam's default step order) of F_AB and F_BA, updated in place,
+ * on the mean squared error of twx_warp_blend(a, b, F_AB, F_BA, wa) against `truth` (H x W x C).
+ * Each step is the fused forward, loss and flow-value gradient (2 (out - truth) / N through both
+ * warps), the fixed-order flow gathers and the Adam update; nothing returns to the host between
+ * steps, and every sum has a fixed order, so the result is the same bits on every run.
+ * losses (HOST, steps + 1 floats) receives the loss before each step and after the last. No
+ * counterpart in the reference (the differentiable extension of the warp+blend). DEVICE
+ * pointers for the images and flows. */
+int twx_refine_flows(twx_handle* h, int32_t channels, int32_t width, int32_t height, const float* a,
+                     const float* b, const float* truth, float* f_ab, float* f_ba, int32_t flow_w,
+                     int32_t flow_h, float wa, int32_t steps, float lr, float* losses, char* err,
+                     size_t errlen);
 
 /* ---- counters (Exec::summary, device/exec/mod.rs:292-322) -------------------------------- */
 

@@ -4,13 +4,12 @@ The allocation-free entry point runs the same kernels on the library's fixed per
 Checked on consecutive calls (a refused call that sets the NaN flag, a 1280x720 call, then a
 smaller one, so stale flags or gradients left in the shared scratch would show):
 - dL/dF_AB and dL/dF_BA: bit for bit (both are gathered per flow texel in a fixed order);
-- dL/dA and dL/dB: both entry points scatter them with float atomic adds, whose order of
-  summation is not fixed, so two calls agree up to rounding: |x - y| <= 2e-6 * max|y| elementwise.
+- dL/dA and dL/dB: bit for bit (both are summed in 64-bit fixed point, whose sums do not depend on
+  the order of the atomic adds).
 Sizes over TWX_NOALLOC_MAX_PIXELS are refused with status 3 before any launch.
 """
 import ctypes
 
-import numpy as np
 import pytest
 import torch
 
@@ -65,13 +64,9 @@ def test_noalloc_backward_equals_the_allocating_one():
         st_new, msg_new, new = _backward(h, L.twx_warp_blend_backward_noalloc, a, b, fab, fba, wa, go)
         torch.cuda.synchronize()
         assert st_ref == 0 and st_new == 0, (msg_ref, msg_new)
-        for k, name in ((2, "dL/dF_AB"), (3, "dL/dF_BA")):
+        for k, name in ((0, "dL/dA"), (1, "dL/dB"), (2, "dL/dF_AB"), (3, "dL/dF_BA")):
             assert torch.equal(new[k].view(torch.int32), ref[k].view(torch.int32)), f"{name} {H}x{W}"
-        for k, name in ((0, "dL/dA"), (1, "dL/dB")):
-            x, y = new[k].cpu().double().numpy(), ref[k].cpu().double().numpy()
-            scale = float(np.abs(y).max())
-            assert scale > 0
-            np.testing.assert_array_less(np.abs(x - y), 2e-6 * scale + 1e-30, err_msg=f"{name} {H}x{W}")
+            assert float(ref[k].abs().max()) > 0, f"{name} {H}x{W}"
 
 
 def test_noalloc_backward_refuses_sizes_over_its_scratch():

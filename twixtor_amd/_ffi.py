@@ -19,7 +19,7 @@ EXPORTS = [
     "twx_solve_pair", "twx_solve_pairs_batched", "twx_compose_vectors", "twx_select", "twx_retime_frames_dev", "twx_retime_batch",
     "twx_retime_frames",
     "twx_forward_warp", "twx_encode_vectors", "twx_decode_vectors", "twx_flow_from_vectors", "twx_invert_flow",
-    "twx_create_vectors_dev", "twx_warp_blend", "twx_warp_blend_backward", "twx_warp_blend_backward_noalloc",
+    "twx_create_vectors_dev", "twx_warp_blend", "twx_warp_blend_backward", "twx_warp_blend_backward_noalloc", "twx_refine_flows",
     "twx_vectors_in_frames", "twx_vectors_in_frames_dev",
     "twx_set_graphs", "twx_set_tile_shape", "twx_tile_choices", "twx_get_stats", "twx_cache_stats", "twx_stream", "twx_profile_enable", "twx_profile_reset", "twx_profile_read",
 ]
@@ -118,6 +118,7 @@ def lib():
         L.twx_warp_blend.argtypes = [P, I32, I32, I32, P, P, P, P, I32, I32, ctypes.c_float, P] + ERR
         L.twx_warp_blend_backward.argtypes = [P, I32, I32, I32, P, P, P, P, I32, I32, ctypes.c_float, P, P, P, P, P] + ERR
         L.twx_warp_blend_backward_noalloc.argtypes = [P, I32, I32, I32, P, P, P, P, I32, I32, ctypes.c_float, P, P, P, P, P] + ERR
+        L.twx_refine_flows.argtypes = [P, I32, I32, I32, P, P, P, P, P, I32, I32, ctypes.c_float, I32, ctypes.c_float, P] + ERR
         L.twx_set_graphs.argtypes = [P, I32]
         L.twx_set_tile_shape.argtypes = [I32]
         L.twx_tile_choices.argtypes = [PI32, I32]

@@ -2481,21 +2481,16 @@ static int warp_blend_entry(twx_handle* h, bool backward, int32_t channels, int3
             hipStream_t s = h->stream;
             DevPool pool(h);
             const size_t px = (size_t)std::max(width, 0) * std::max(height, 0);
-            const size_t img = px * std::max(channels, 0) * sizeof(float);
-            int* bad = pool.alloc<int>(1);
-            HIPCHECK(hipMemsetAsync(bad, 0, sizeof(int), s));
+            int* flags = pool.alloc<int>(2);
+            HIPCHECK(hipMemsetAsync(flags, 0, 2 * sizeof(int), s));
             float* gp = backward ? pool.alloc<float>(px * 4) : nullptr;
-            if (backward) {
-                if (grad_a) HIPCHECK(hipMemsetAsync(grad_a, 0, img, s));
-                if (grad_b) HIPCHECK(hipMemsetAsync(grad_b, 0, img, s));
-            }
             st = warp_blend_run(s, backward, channels, width, height, a, b, f_ab, f_ba, flow_w, flow_h, wa, out, grad_out,
-                                grad_a, grad_b, grad_f_ab, grad_f_ba, gp, bad, e);
+                                grad_a, grad_b, grad_f_ab, grad_f_ba, gp, flags, e);
             if (!st) {
-                int* hb = pool.pinned<int>(1);
-                HIPCHECK(hipMemcpyAsync(hb, bad, sizeof(int), hipMemcpyDeviceToHost, s));
+                int* hb = pool.pinned<int>(2);
+                HIPCHECK(hipMemcpyAsync(hb, flags, 2 * sizeof(int), hipMemcpyDeviceToHost, s));
                 HIPCHECK(hipStreamSynchronize(s));
-                if (*hb) { e = "warp+blend: a sample point is NaN or never converges"; st = TWX_E_UNSUPPORTED; }
+                st = diff_flags_status(hb, e);
             }
         }
     } catch (const HipError& x) {
@@ -2534,6 +2529,57 @@ int twx_warp_blend_backward_noalloc(twx_handle* h, int32_t channels, int32_t wid
             after_caller(h);
             st = warp_blend_backward_noalloc(h->stream, h->device, channels, width, height, a, b, f_ab, f_ba, flow_w,
                                              flow_h, wa, grad_out, grad_a, grad_b, grad_f_ab, grad_f_ba, e);
+        }
+    } catch (const HipError& x) {
+        e = x.msg;
+        st = TWX_E_DEVICE;
+    }
+    if (st) set_err(err, errlen, e);
+    return st;
+}
+
+int twx_refine_flows(twx_handle* h, int32_t channels, int32_t width, int32_t height, const float* a, const float* b,
+                     const float* truth, float* f_ab, float* f_ba, int32_t flow_w, int32_t flow_h, float wa, int32_t steps,
+                     float lr, float* losses, char* err, size_t errlen) {
+    std::string e;
+    int st = TWX_OK;
+    try {
+        Guard g(h, st, e);
+        if (g.ok()) {
+            if (steps < 0 || !a || !b || !truth || !f_ab || !f_ba || !losses || !(lr >= 0.0f)) {
+                e = "refine flows: bad arguments";
+                st = TWX_E_MISMATCH;
+            } else {
+                after_caller(h);
+                hipStream_t s = h->stream;
+                double t0 = now_ms();
+                DevPool pool(h);
+                const size_t px = (size_t)std::max(width, 0) * std::max(height, 0);
+                const size_t fn = (size_t)std::max(flow_w, 0) * std::max(flow_h, 0) * 2;
+                RefineBufs B{};
+                B.gp = pool.alloc<float>(std::max<size_t>(px * 4, 1));
+                B.g_ab = pool.alloc<float>(std::max<size_t>(fn, 1));
+                B.g_ba = pool.alloc<float>(std::max<size_t>(fn, 1));
+                float* mv = pool.alloc<float>(std::max<size_t>(fn * 4, 1));
+                HIPCHECK(hipMemsetAsync(mv, 0, std::max<size_t>(fn * 4, 1) * sizeof(float), s));
+                B.m_ab = mv;
+                B.v_ab = mv + fn;
+                B.m_ba = mv + 2 * fn;
+                B.v_ba = mv + 3 * fn;
+                B.loss_part = pool.alloc<float>((size_t)std::max(refine_loss_blocks(std::max(width, 1), std::max(height, 1)), 1));
+                B.losses = pool.alloc<float>((size_t)steps + 1);
+                B.flags = pool.alloc<int>(2);
+                HIPCHECK(hipMemsetAsync(B.flags, 0, 2 * sizeof(int), s));
+                st = refine_flows_run(s, channels, width, height, a, b, truth, f_ab, f_ba, flow_w, flow_h, wa, steps, lr, B, e);
+                if (!st) {
+                    int* hb = pool.pinned<int>(2);
+                    HIPCHECK(hipMemcpyAsync(hb, B.flags, 2 * sizeof(int), hipMemcpyDeviceToHost, s));
+                    HIPCHECK(hipMemcpyAsync(losses, B.losses, ((size_t)steps + 1) * sizeof(float), hipMemcpyDeviceToHost, s));
+                    HIPCHECK(hipStreamSynchronize(s));
+                    st = diff_flags_status(hb, e);
+                }
+                h->stats.device_ms += now_ms() - t0;
+            }
         }
     } catch (const HipError& x) {
         e = x.msg;

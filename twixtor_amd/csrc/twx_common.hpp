@@ -167,6 +167,7 @@ enum ProfFam {
     F_INGEST, F_PYR_BLUR, F_GRAD_TENSOR, F_BM_TRANSPOSE,
     F_SOLVE_INIT, F_CALC_B, F_NEW_DISP, F_BM_ROWS, F_BM_COLS, F_RANGE, F_UPSAMPLE, F_NEGATE, F_FRAMES,
     F_WB_MASK, F_WB_SHARPEN, F_FILL_PULL, F_FILL_PUSH, F_FILL_IO, F_DIFF_FWD, F_DIFF_BWD, F_DIFF_GATHER,
+    F_DIFF_IMG, F_DIFF_STEP, F_DIFF_ADAM,
     F_FW_MESH, F_FW_RASTER, F_FW_SCAN, F_FW_FOLD, F_VEC, F_NFAM
 };
 extern const char* const PROF_NAMES[F_NFAM];
@@ -240,10 +241,32 @@ int tile_force(int v);
 int tile_tuning_report(int* out, int cap);
 int run_selfcheck(hipStream_t s, std::string& err);
 // The differentiable warp+blend (twx_diff.hip): the forward into `out`, or the backward into the
-// gradients that are not null; gp_scratch holds 4 floats per pixel, `bad` is set for a NaN point.
+// gradients that are not null; gp_scratch holds 4 floats per pixel; flags (2 ints, zeroed by the
+// caller) receive the NaN-point flag and the largest upstream magnitude, read back into
+// diff_flags_status after the stream's work.
 int warp_blend_run(hipStream_t s, bool backward, int C, int W, int H, const float* a, const float* b, const float* f_ab,
                    const float* f_ba, int fw, int fh, float wa, float* out, const float* grad_out, float* grad_a,
-                   float* grad_b, float* grad_f_ab, float* grad_f_ba, float* gp_scratch, int* bad, std::string& err);
+                   float* grad_b, float* grad_f_ab, float* grad_f_ba, float* gp_scratch, int* flags, std::string& err);
+int diff_flags_status(const int flags[2], std::string& err);
+// Photometric refinement on the device: `steps` Adam steps of the flows (in place) on the mean
+// squared error of the warp+blend against `truth`; losses[t] (device) the loss before step t,
+// losses[steps] after the last. The buffers: gp 4 floats per pixel, g/m/v fw * fh * 2 floats each
+// (m and v zeroed), loss_part refine_loss_blocks(W, H) floats, losses steps + 1, flags 2 ints zeroed.
+struct RefineBufs {
+    float* gp;
+    float* g_ab;
+    float* g_ba;
+    float* m_ab;
+    float* v_ab;
+    float* m_ba;
+    float* v_ba;
+    float* loss_part;
+    float* losses;
+    int* flags;
+};
+int refine_loss_blocks(int W, int H);
+int refine_flows_run(hipStream_t s, int C, int W, int H, const float* a, const float* b, const float* truth, float* f_ab,
+                     float* f_ba, int fw, int fh, float wa, int steps, float lr, const RefineBufs& B, std::string& err);
 // The backward of the warp+blend on the library's fixed per-device scratch (no allocation): the
 // scratch address of `device` is taken once by noalloc_scratch_setup with that device current.
 int noalloc_scratch_setup(int device, std::string& err);

@@ -222,6 +222,7 @@ const char* const PROF_NAMES[F_NFAM] = {"ingest", "pyr_luma_blur5_band_resample"
                                         "bm_rows", "bm_cols", "range", "upsample", "negate", "frames",
                                         "wb_mask", "wb_sharpen_blend", "fill_pull", "fill_push", "fill_io",
                                         "warp_blend_fwd", "warp_blend_bwd", "warp_blend_flow_gather",
+                                        "warp_blend_image_grad", "refine_step", "refine_adam",
                                         "fw_mesh", "fw_raster", "fw_scan", "fw_fold", "vector_coding"};
 thread_local Prof* g_prof = nullptr;
 

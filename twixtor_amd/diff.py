@@ -130,6 +130,29 @@ def refine_flows(a, b, truth, f_ab, f_ba, wa: float = 0.5, steps: int = 50, lr: 
     return fab.detach(), fba.detach(), losses
 
 
+def refine_flows_device(a, b, truth, f_ab, f_ba, wa: float = 0.5, steps: int = 50, lr: float = 0.05):
+    """refine_flows in one library call (twx_refine_flows): every step's forward, L2 gradient,
+    backward and Adam update run on the device with no host round trip, and every sum has a fixed
+    order (the same bits on every run). Returns the refined flows and the losses (before each step
+    and after the last)."""
+    _check_args(a, b, f_ab, f_ba)
+    if truth.shape != a.shape or truth.dtype != torch.float32 or truth.device != a.device:
+        raise ValueError("the truth image must match the images' shape, dtype and device")
+    a, b, truth = (t.contiguous() for t in (a, b, truth))
+    fab = f_ab.detach().contiguous().clone()
+    fba = f_ba.detach().contiguous().clone()
+    H, W, C = a.shape
+    fh, fw, _ = fab.shape
+    losses = np.zeros(steps + 1, np.float32)
+    h = handle(a.device)
+    torch.cuda.current_stream(a.device).synchronize()
+    err = _ffi.errbuf()
+    _ffi.check(_ffi.lib().twx_refine_flows(h._h, C, W, H, _ptr(a), _ptr(b), _ptr(truth), _ptr(fab), _ptr(fba), fw, fh,
+                                           ctypes.c_float(wa), steps, ctypes.c_float(lr),
+                                           losses.ctypes.data_as(ctypes.c_void_p), err, 512), err)
+    return fab, fba, [float(x) for x in losses]
+
+
 def photometric_case(clip, k: int, fs=None, flow_noise: float = 0.5, seed: int = 0):
     """The refinement workload of pair k of a clip rendered at twice the pair rate: A = frame 2k,
     B = frame 2k + 2, the truth frame 2k + 1 (RGB of the straight-alpha frames, alpha 1), and
