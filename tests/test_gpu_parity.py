@@ -314,3 +314,17 @@ def test_special_value_cases_give_the_reference_bytes_or_refuse(backend):
     assert not other, f"{counts}: {other}"
     # the oracle's count: 34 plugin hashes, 2 Weighted Blend cases over non-finite samples refused
     assert counts == {"reference bytes": 34, "refused": 2, "refused on the host": 8}, counts
+
+
+def test_open_time_gate_passes_and_is_quick(backend):
+    """twx_selfcheck (run by twx_open, and on demand): the raw float operations, then the frame
+    kernels' own pixel functions (blend, the extrapolating lerp, the wire finish, a tap lerp with
+    the store rule) on every u8 pair, the u16 range edges plus a seeded sample, f32 special values
+    with NaN payloads compared bit for bit and blend weights where a fused multiply-add would
+    round differently (searched on the host with fmaf), the casts at +-2^31 and +-2^63, and a
+    dependent loop of 64 steps: every result equals the host's."""
+    import time
+
+    t0 = time.perf_counter()
+    backend.selfcheck()
+    assert time.perf_counter() - t0 < 5.0

@@ -13,6 +13,7 @@
 #include <vector>
 
 #include "twx_common.hpp"
+#include "twx_pixel.hpp"
 
 namespace twx {
 
@@ -69,7 +70,262 @@ float from_bits(uint32_t b) {
     return f;
 }
 
+// ---- the frame kernels' own pixel functions (twx_pixel.hpp), device against host ----
+
+// A pixel record: two engine pixels, the blend weight w (per-sample mix) and the lerp weight lw
+// (the extrapolating branch).
+struct PixRec {
+    float p[4], q[4];
+    float w, lw;
+};
+constexpr int PIX_OUT = 13;  // blend 4, lerp 4, finish 4, store rule 1
+
+template <int FMT>
+__host__ __device__ inline void pix_eval(const PixRec& r, uint32_t* o) {
+    float b[4], l[4] = {r.p[0], r.p[1], r.p[2], r.p[3]};
+    uint32_t f[4];
+    blend_pixel<FMT>(r.p, r.q, 3, r.w, 0.0f, b);
+    lerp_pixel<FMT>(l, r.q, r.lw);
+    finish_values<FMT>(r.p, f);
+    for (int k = 0; k < 4; k++) {
+        o[k] = f2u(b[k]);
+        o[4 + k] = f2u(l[k]);
+        o[8 + k] = f[k];
+    }
+    // a bilinear tap's lerp (base first, the NaN rule where it gives a NaN, warp.rs:334-343) and
+    // the warp's store rule (warp.rs:129-146)
+    float t = r.p[1] + (r.q[1] - r.p[1]) * r.w;
+    if (FMT == 2 && is_nan(t)) t = nr_lerp_bf(r.p[1], r.q[1], r.w);
+    o[12] = f2u(store_rule<FMT>(t));
+}
+
+template <int FMT>
+__global__ void k_selfcheck_pix(const PixRec* __restrict__ in, uint32_t* __restrict__ out, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) pix_eval<FMT>(in[i], out + (size_t)i * PIX_OUT);
+}
+
+// Casts at their bounds: cvtt, the 64-bit truncations of a float and of a double, q(), the
+// saturating store rules, and the NaN rule's operations (exact bits, payloads included).
+constexpr int CAST_OUT = 11;
+__host__ __device__ inline void cast_eval(float a, float b, double d, uint32_t* o) {
+    o[0] = (uint32_t)cvtt(a);
+    const int64_t t = cvtt64f(a), u = cvtt64d(d), q = qround(a);
+    o[1] = (uint32_t)t;
+    o[2] = (uint32_t)(t >> 32);
+    o[3] = (uint32_t)u;
+    o[4] = (uint32_t)(u >> 32);
+    o[5] = (uint32_t)q;
+    o[6] = f2u(sat_u8(a));
+    o[7] = f2u(sat_u16(a));
+    o[8] = f2u(nr_add(a, b));
+    o[9] = f2u(nr_div(nr_mul(a, b), nr_sub(b, a)));
+    o[10] = f2u(nr_lerp(a, b, 0.375f));
+}
+
+__global__ void k_selfcheck_cast(const float* __restrict__ fa, const double* __restrict__ dd, uint32_t* __restrict__ out, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) cast_eval(fa[2 * i], fa[2 * i + 1], dd[i], out + (size_t)i * CAST_OUT);
+}
+
+// A dependent loop of 64 steps (any step's rounding difference carries into the rest): the
+// sampler's lerp chain, a division and a product into the denormal range, from each seed pair.
+constexpr int LOOP_STEPS = 64;
+__host__ __device__ inline void loop_eval(float x, float y, uint32_t* o) {
+    for (int k = 0; k < LOOP_STEPS; k++) {
+        const float w = 0.3125f + 0.0078125f * (float)(k & 7);
+        const float t = (y - x) * w + x;
+        y = (x * 1.0009766f + 0.25f) / (t * t + 1.5f);
+        x = t * 0.99951172f + y * 1e-30f * 1e-8f;
+    }
+    o[0] = f2u(x);
+    o[1] = f2u(y);
+}
+
+__global__ void k_selfcheck_loop(const float* __restrict__ in, uint32_t* __restrict__ out, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) loop_eval(in[2 * i], in[2 * i + 1], out + (size_t)i * 2);
+}
+
+// Runs `launch` on a device copy of `in` (records of In) into n * k outputs and compares them
+// with `host` per record; the first difference names the group, the record and the output.
+template <typename In, typename Launch, typename Host>
+int check_group(hipStream_t s, const char* name, const std::vector<In>& in, int k, Launch&& launch, Host&& host,
+                std::string& err) {
+    const int n = (int)in.size();
+    if (n == 0) return TWX_OK;
+    In* d_in = nullptr;
+    uint32_t* d_out = nullptr;
+    std::vector<uint32_t> got((size_t)n * k);
+    if (hipMalloc(&d_in, in.size() * sizeof(In)) != hipSuccess) { err = "hipMalloc failed"; return TWX_E_DEVICE; }
+    if (hipMalloc(&d_out, got.size() * 4) != hipSuccess) { (void)hipFree(d_in); err = "hipMalloc failed"; return TWX_E_DEVICE; }
+    hipError_t e = hipMemcpyAsync(d_in, in.data(), in.size() * sizeof(In), hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) {
+        launch(d_in, d_out, n);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(got.data(), d_out, got.size() * 4, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    (void)hipFree(d_in);
+    (void)hipFree(d_out);
+    if (e != hipSuccess) { err = std::string("self-check launch: ") + hipGetErrorString(e); return TWX_E_DEVICE; }
+    std::vector<uint32_t> want(k);
+    for (int i = 0; i < n; i++) {
+        host(in[i], want.data());
+        for (int j = 0; j < k; j++)
+            if (want[j] != got[(size_t)i * k + j]) {
+                char buf[200];
+                snprintf(buf, sizeof buf, "self-check %s: record %d output %d: device %08x, host %08x", name, i, j,
+                         got[(size_t)i * k + j], want[j]);
+                err = buf;
+                return TWX_E_UNSUPPORTED;
+            }
+    }
+    return TWX_OK;
+}
+
+// The pixel records of one depth: channel values drawn from `vals` pairs (every pair for u8),
+// spread over alpha and colours, at five blend weights and five lerp weights.
+std::vector<PixRec> pix_corpus(int fmt, std::mt19937& rng) {
+    std::vector<std::pair<float, float>> pairs;
+    if (fmt == 0) {
+        for (int x = 0; x < 256; x++)
+            for (int y = 0; y < 256; y++) pairs.push_back({(float)x, (float)y});
+    } else if (fmt == 1) {
+        const int edges[] = {0, 1, 2, 127, 128, 255, 256, 32767, 32768, 32769, 65279, 65534, 65535};
+        for (int x : edges)
+            for (int y : edges) pairs.push_back({(float)x, (float)y});
+        std::uniform_int_distribution<int> u(0, 65535);
+        for (int i = 0; i < 32768; i++) pairs.push_back({(float)u(rng), (float)u(rng)});
+    } else {
+        const uint32_t sp[] = {0x00000000u, 0x80000000u, 0x00000001u, 0x807fffffu, 0x00800000u, 0x3f800000u, 0x3f7fffffu,
+                               0x3f800001u, 0xbf800000u, 0x7f7fffffu, 0xff7fffffu, 0x7f800000u, 0xff800000u, 0x7fc00000u,
+                               0xffc00000u, 0x7fa00001u, 0xffc12345u, 0x7f812345u, 0x3dcccccdu, 0x2edbe6ffu, 0x3effffffu};
+        for (uint32_t x : sp)
+            for (uint32_t y : sp) pairs.push_back({from_bits(x), from_bits(y)});
+        std::uniform_real_distribution<float> u(-0.25f, 1.5f);
+        for (int i = 0; i < 16384; i++) pairs.push_back({u(rng), u(rng)});
+    }
+    const float ws[] = {0.5f, 0.33333334f, 0.75f, 0.0099999998f, 0.99000001f};
+    const float lws[] = {1.25f, -0.25f, 2.0f, 0.5f, -1.0f};
+    const float top = fmt == 0 ? 255.0f : (fmt == 1 ? 65535.0f : 1.0f);
+    std::vector<PixRec> out;
+    out.reserve(pairs.size() * 5);
+    for (size_t i = 0; i < pairs.size(); i++) {
+        const float x = pairs[i].first, y = pairs[i].second;
+        const float xr = fmt == 2 ? 1.0f - x : top - x;
+        const int k = (int)(i % 5);
+        PixRec r;
+        // alpha from the pair, colours from the pair, its mirror and a fixed mid value
+        r.p[0] = x; r.p[1] = y; r.p[2] = xr; r.p[3] = fmt == 2 ? 0.5f : (float)((int)top / 2);
+        r.q[0] = y; r.q[1] = x; r.q[2] = r.p[3]; r.q[3] = xr;
+        r.w = ws[k];
+        r.lw = lws[k];
+        out.push_back(r);
+        r.w = ws[(k + 2) % 5];
+        r.lw = lws[(k + 3) % 5];
+        std::swap(r.p[0], r.p[1]);
+        out.push_back(r);
+    }
+    return out;
+}
+
 }  // namespace
+
+// The corpus of the kernels' own pixel functions, casts at their bounds, products where a fused
+// multiply-add would round differently (found on the host with fmaf), and a dependent loop.
+static int run_selfcheck_kernels(hipStream_t s, std::string& err) {
+    std::mt19937 rng(4321);
+    int st;
+    {
+        auto in = pix_corpus(0, rng);
+        st = check_group(s, "u8 pixels", in, PIX_OUT,
+                         [&](const PixRec* d, uint32_t* o, int n) { k_selfcheck_pix<0><<<(n + 255) / 256, 256, 0, s>>>(d, o, n); },
+                         [](const PixRec& r, uint32_t* o) { pix_eval<0>(r, o); }, err);
+        if (st) return st;
+    }
+    {
+        auto in = pix_corpus(1, rng);
+        st = check_group(s, "u16 pixels", in, PIX_OUT,
+                         [&](const PixRec* d, uint32_t* o, int n) { k_selfcheck_pix<1><<<(n + 255) / 256, 256, 0, s>>>(d, o, n); },
+                         [](const PixRec& r, uint32_t* o) { pix_eval<1>(r, o); }, err);
+        if (st) return st;
+    }
+    {
+        auto in = pix_corpus(2, rng);
+        // products where fmaf rounds differently from a multiply then an add, as blend weights
+        std::uniform_real_distribution<float> u(-2.0f, 2.0f);
+        int found = 0;
+        for (int i = 0; i < 200000 && found < 1024; i++) {
+            const float p = u(rng), q = u(rng), w = std::fabs(u(rng)) * 0.5f;
+            const volatile float d = q - p;
+            const volatile float m = d * w;
+            if (std::fmaf(d, w, p) == m + p) continue;
+            PixRec r{{p, q, p, q}, {q, p, q, p}, w, w + 1.0f};
+            in.push_back(r);
+            found++;
+        }
+        st = check_group(s, "f32 pixels", in, PIX_OUT,
+                         [&](const PixRec* d, uint32_t* o, int n) { k_selfcheck_pix<2><<<(n + 255) / 256, 256, 0, s>>>(d, o, n); },
+                         [](const PixRec& r, uint32_t* o) { pix_eval<2>(r, o); }, err);
+        if (st) return st;
+    }
+    {
+        struct CastRec {
+            float a, b;
+            double d;
+        };
+        std::vector<CastRec> in;
+        auto near = [&](float x) {
+            in.push_back({std::nextafterf(x, -INFINITY), x, (double)x});
+            in.push_back({x, std::nextafterf(x, INFINITY), std::nextafter((double)x, INFINITY)});
+            in.push_back({std::nextafterf(x, INFINITY), -x, std::nextafter((double)x, -INFINITY)});
+        };
+        const float bounds[] = {2147483648.0f, -2147483648.0f, 9223372036854775808.0f, -9223372036854775808.0f, 0.5f,
+                                -0.5f, 254.5f, 255.5f, 65534.5f, 65535.5f, 127.5f, -128.5f, 32767.5f, 1.0f, 0.0f};
+        for (float x : bounds) near(x);
+        const uint32_t sp[] = {0x7f800000u, 0xff800000u, 0x7fc00000u, 0xffc00000u, 0x7fa00001u, 0xff812345u, 0x00000001u,
+                               0x80000000u, 0x7f7fffffu};
+        for (uint32_t x : sp)
+            for (uint32_t y : sp) in.push_back({from_bits(x), from_bits(y), (double)from_bits(x)});
+        std::uniform_int_distribution<uint32_t> bits(0, 0xffffffffu);
+        for (int i = 0; i < 4096; i++) in.push_back({from_bits(bits(rng)), from_bits(bits(rng)), (double)from_bits(bits(rng))});
+        std::vector<float> fa;
+        std::vector<double> dd;
+        for (auto& c : in) { fa.push_back(c.a); fa.push_back(c.b); dd.push_back(c.d); }
+        // one record per cast input, as two arrays on the device
+        std::vector<uint32_t> idx(in.size());
+        for (size_t i = 0; i < in.size(); i++) idx[i] = (uint32_t)i;
+        float* d_fa = nullptr;
+        double* d_dd = nullptr;
+        if (hipMalloc(&d_fa, fa.size() * 4) != hipSuccess) { err = "hipMalloc failed"; return TWX_E_DEVICE; }
+        if (hipMalloc(&d_dd, dd.size() * 8) != hipSuccess) { (void)hipFree(d_fa); err = "hipMalloc failed"; return TWX_E_DEVICE; }
+        hipError_t e = hipMemcpyAsync(d_fa, fa.data(), fa.size() * 4, hipMemcpyHostToDevice, s);
+        if (e == hipSuccess) e = hipMemcpyAsync(d_dd, dd.data(), dd.size() * 8, hipMemcpyHostToDevice, s);
+        if (e != hipSuccess) { (void)hipFree(d_fa); (void)hipFree(d_dd); err = "self-check upload failed"; return TWX_E_DEVICE; }
+        st = check_group(s, "casts", idx, CAST_OUT,
+                         [&](const uint32_t*, uint32_t* o, int n) { k_selfcheck_cast<<<(n + 255) / 256, 256, 0, s>>>(d_fa, d_dd, o, n); },
+                         [&](const uint32_t& i, uint32_t* o) { cast_eval(in[i].a, in[i].b, in[i].d, o); }, err);
+        (void)hipFree(d_fa);
+        (void)hipFree(d_dd);
+        if (st) return st;
+    }
+    {
+        struct Seed {
+            float x, y;
+        };
+        std::vector<Seed> in;
+        std::uniform_real_distribution<float> u(-4.0f, 4.0f);
+        for (int i = 0; i < 1024; i++) in.push_back({u(rng), u(rng)});
+        st = check_group(s, "dependent loop", in, 2,
+                         [&](const Seed* d, uint32_t* o, int n) {
+                             k_selfcheck_loop<<<(n + 255) / 256, 256, 0, s>>>(reinterpret_cast<const float*>(d), o, n);
+                         },
+                         [](const Seed& r, uint32_t* o) { loop_eval(r.x, r.y, o); }, err);
+        if (st) return st;
+    }
+    return TWX_OK;
+}
 
 int run_selfcheck(hipStream_t s, std::string& err) {
     std::vector<float> in;
@@ -128,7 +384,7 @@ int run_selfcheck(hipStream_t s, std::string& err) {
             }
         }
     }
-    return TWX_OK;
+    return run_selfcheck_kernels(s, err);
 }
 
 }  // namespace twx
