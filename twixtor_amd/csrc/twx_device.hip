@@ -14,6 +14,7 @@
 #include <map>
 #include <mutex>
 #include <tuple>
+#include <type_traits>
 
 #include <climits>
 #include <cmath>
@@ -333,6 +334,11 @@ __device__ __forceinline__ void cell(float x, int len, int& io, float& fo) {
     float f = x - (float)i;
     int32_t last = len - 1;
     if (i >= last) { i = last - 1; f = 1.0f; }
+    if (FINITE) {  // i is in 0 .. len - 2 already: the clamps below leave it
+        io = i;
+        fo = f;
+        return;
+    }
     int ii = i < 0 ? 0 : i;
     int cap = len >= 2 ? len - 2 : 0;
     io = min(ii, cap);
@@ -410,7 +416,7 @@ enum Produce { P_TENSOR = 0, P_CALCB = 1, P_NEWDISP = 2 };
 // The pixels' loads are issued together, phase by phase (calc_b's samples of Y depend on its flow
 // loads), through global-address-space pointers and 32-bit offsets; every load is unconditional
 // (in-bounds addresses, results selected). The arithmetic of each pixel is the reference's.
-template <int MODE, int C, int PU>
+template <int MODE, int C, int PU, bool INNER>
 __device__ __forceinline__ void produce(const SolveJob& J, int level, int src, int w, int h, const int px[PU],
                                         const int py[PU], const bool own[PU], float v[PU][C]) {
     uint32_t e[PU];
@@ -423,19 +429,19 @@ __device__ __forceinline__ void produce(const SolveJob& J, int level, int src, i
         for (int u = 0; u < PU; u++) {
             const int x = px[u], y = py[u];
             c[u] = ldg(img, e[u]);
-            vl[u] = ldg(img, e[u] - (x >= 1 ? 1 : 0));
-            vu[u] = ldg(img, e[u] - (y >= 1 ? w : 0));
-            vr[u] = ldg(img, e[u] + (x + 1 <= w - 1 ? 1 : 0));
-            vd[u] = ldg(img, e[u] + (y + 1 <= h - 1 ? w : 0));
+            vl[u] = ldg(img, e[u] - (INNER || x >= 1 ? 1 : 0));
+            vu[u] = ldg(img, e[u] - (INNER || y >= 1 ? w : 0));
+            vr[u] = ldg(img, e[u] + (INNER || x + 1 <= w - 1 ? 1 : 0));
+            vd[u] = ldg(img, e[u] + (INNER || y + 1 <= h - 1 ? w : 0));
         }
 #pragma unroll
         for (int u = 0; u < PU; u++) {
             const int x = px[u], y = py[u];
             float nx = 0.0f, dx = 0.0f, ny = 0.0f, dy = 0.0f;
-            if (x >= 1) { nx = (c[u] - vl[u]) + 0.0f; dx = 1.0f; }
-            if (y >= 1) { ny = (c[u] - vu[u]) + 0.0f; dy = 1.0f; }
-            if (x + 1 <= w - 1) { nx = nx + (vr[u] - c[u]); dx = dx + 1.0f; }
-            if (y + 1 <= h - 1) { ny = ny + (vd[u] - c[u]); dy = dy + 1.0f; }
+            if (INNER || x >= 1) { nx = (c[u] - vl[u]) + 0.0f; dx = 1.0f; }
+            if (INNER || y >= 1) { ny = (c[u] - vu[u]) + 0.0f; dy = 1.0f; }
+            if (INNER || x + 1 <= w - 1) { nx = nx + (vr[u] - c[u]); dx = dx + 1.0f; }
+            if (INNER || y + 1 <= h - 1) { ny = ny + (vd[u] - c[u]); dy = dy + 1.0f; }
             const float gx = nx / dx, gy = ny / dy;
             if (own[u]) {
                 *(gfloat_rw*)((gchar*)J.grad + 8u * e[u]) = gx;
@@ -576,33 +582,41 @@ __global__ __launch_bounds__(TILE_X * TY / TILE_RUN) void k_tile_gauss(const Sol
     // the region, PU pixels per lane at a time with their loads in flight together; a pixel
     // outside the image is computed at a clamped position and not stored
     constexpr int NPROD = (RH * RW + NT - 1) / NT, PU = PUV;
-    // region element k = tid + j * NT of the lane's j-th pixel, as (ry, rx), stepped without division
-    int ry = tid / RW, rx = tid - (tid / RW) * RW;
+    // region element k = tid + j * NT of the lane's j-th pixel, as (ry, rx), stepped without division.
+    // INNER: the region and a one-pixel margin lie inside the image (most tiles), so no pixel needs a
+    // clamp or a border test; the lanes past the region's end compute at a safe inner pixel and
+    // store nothing.
+    auto region = [&](auto inner) {
+        constexpr bool INNER = decltype(inner)::value;
+        int ry = tid / RW, rx = tid - (tid / RW) * RW;
 #pragma unroll 1
-    for (int k0 = 0; k0 < NPROD; k0 += PU) {
-        float v[PU][C];
-        int slot[PU], px[PU], py[PU];
-        bool own[PU];
+        for (int k0 = 0; k0 < NPROD; k0 += PU) {
+            float v[PU][C];
+            int slot[PU], px[PU], py[PU];
+            bool own[PU];
 #pragma unroll
-        for (int u = 0; u < PU; u++) {
-            const int x = x0 - R + rx, y = y0 - R + ry;
-            const bool keep = ry < RH && x >= 0 && x < w && y >= 0 && y < h;
-            own[u] = keep && rx >= R && rx < R + TILE_X && ry >= R && ry < R + TY;
-            px[u] = min(max(x, 0), w - 1);
-            py[u] = min(max(y, 0), h - 1);
-            slot[u] = keep ? ry * RW + rx : -1;
-            rx += NT % RW;
-            ry += NT / RW;
-            if (rx >= RW) { rx -= RW; ry++; }
-        }
-        produce<MODE, C, PU>(J, level, src, w, h, px, py, own, v);
-#pragma unroll
-        for (int u = 0; u < PU; u++)
-            if (slot[u] >= 0) {
-#pragma unroll
-                for (int c = 0; c < C; c++) s_in[slot[u] * C + c] = v[u][c];
+            for (int u = 0; u < PU; u++) {
+                const int x = x0 - R + rx, y = y0 - R + ry;
+                const bool keep = INNER ? ry < RH : ry < RH && x >= 0 && x < w && y >= 0 && y < h;
+                own[u] = keep && rx >= R && rx < R + TILE_X && ry >= R && ry < R + TY;
+                px[u] = INNER ? x : min(max(x, 0), w - 1);
+                py[u] = INNER ? (keep ? y : y0) : min(max(y, 0), h - 1);
+                slot[u] = keep ? ry * RW + rx : -1;
+                rx += NT % RW;
+                ry += NT / RW;
+                if (rx >= RW) { rx -= RW; ry++; }
             }
-    }
+            produce<MODE, C, PU, INNER>(J, level, src, w, h, px, py, own, v);
+#pragma unroll
+            for (int u = 0; u < PU; u++)
+                if (slot[u] >= 0) {
+#pragma unroll
+                    for (int c = 0; c < C; c++) s_in[slot[u] * C + c] = v[u][c];
+                }
+        }
+    };
+    if (x0 - R >= 1 && x0 + TILE_X + R <= w - 1 && y0 - R >= 1 && y0 + TY + R <= h - 1) region(std::true_type{});
+    else region(std::false_type{});
     __syncthreads();
     // horizontal pass: item = (region row, run of TILE_RUN columns); results held until every
     // lane has read its window
