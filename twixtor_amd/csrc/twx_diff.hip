@@ -259,8 +259,12 @@ __global__ void k_wb_backward(DiffArgs P) {
 }
 
 // dL/dflow per flow texel (i, j): the sum over the output pixels whose flow sample has (i, j) as
-// a corner of the corner's bilinear weight times the pixel's flow-value gradient, in a fixed order.
-__global__ void k_wb_flow_gather(DiffArgs P, int which) {
+// a corner of the corner's bilinear weight times the pixel's flow-value gradient, in a fixed order
+// (rows, then columns). blockIdx.z: 0 the flow of the warp of A (F_BA), 1 of B (F_AB); a null
+// output is skipped. The candidate window is one pixel wider on each side than the cells' exact
+// reach (the float product x * rx may round across a cell boundary); each pixel is then tested.
+__global__ void k_wb_flow_gather(DiffArgs P) {
+    const int which = blockIdx.z;
     const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
     const int fw = P.fw, fh = P.fh, W = P.W, H = P.H;
     if (i >= fw || j >= fh) return;
@@ -269,8 +273,8 @@ __global__ void k_wb_flow_gather(DiffArgs P, int which) {
     if (!out) return;
     const float rx = (float)(fw - 1) / (float)(W - 1), ry = (float)(fh - 1) / (float)(H - 1);
     // output columns whose cell index is i - 1 or i (with the last-cell clamp), and rows likewise
-    const int x0 = max(0, (int)floorf((float)(i - 1) / rx) - 2), x1 = min(W - 1, (int)ceilf((float)(i + 1) / rx) + 2);
-    const int y0 = max(0, (int)floorf((float)(j - 1) / ry) - 2), y1 = min(H - 1, (int)ceilf((float)(j + 1) / ry) + 2);
+    const int x0 = max(0, (int)floorf((float)(i - 1) / rx) - 1), x1 = min(W - 1, (int)ceilf((float)(i + 1) / rx) + 1);
+    const int y0 = max(0, (int)floorf((float)(j - 1) / ry) - 1), y1 = min(H - 1, (int)ceilf((float)(j + 1) / ry) + 1);
     float s0 = 0.0f, s1 = 0.0f;
     for (int y = y0; y <= y1; y++) {
         const float qy = (float)y * ry;
@@ -535,9 +539,10 @@ int warp_blend_run(hipStream_t s, bool backward, int C, int W, int H, const floa
     // reads A, B, both flows and the upstream gradient, writes the two per-pixel flow-value
     // gradients
     TWX_LAUNCH(s, F_DIFF_BWD, 3.0 * img + 2.0 * fl + 16.0 * (double)W * H, k_wb_backward<<<grd, blk, 0, s>>>(P));
-    dim3 fgrd((fw + 255) / 256, fh);
-    if (grad_f_ba) TWX_LAUNCH(s, F_DIFF_GATHER, 8.0 * (double)W * H + fl, k_wb_flow_gather<<<fgrd, blk, 0, s>>>(P, 0));
-    if (grad_f_ab) TWX_LAUNCH(s, F_DIFF_GATHER, 8.0 * (double)W * H + fl, k_wb_flow_gather<<<fgrd, blk, 0, s>>>(P, 1));
+    dim3 fgrd((fw + 255) / 256, fh, 2);
+    if (grad_f_ba || grad_f_ab)
+        TWX_LAUNCH(s, F_DIFF_GATHER, (grad_f_ba && grad_f_ab ? 2.0 : 1.0) * (8.0 * (double)W * H + fl),
+                   k_wb_flow_gather<<<fgrd, blk, 0, s>>>(P));
     if (!grad_a && !grad_b) return TWX_OK;
     // the image gradients, two channels a pass, in fixed point in the scratch the flow-value
     // gradients held (16 bytes a pixel: two 64-bit sums)
@@ -571,7 +576,7 @@ int refine_flows_run(hipStream_t s, int C, int W, int H, const float* a, const f
     P.gp_ba = B.gp;
     P.gp_ab = B.gp + (size_t)W * H * 2;
     P.bad = B.flags;
-    dim3 blk(256), grd = wb_grid(W, H), fgrd((fw + 255) / 256, fh);
+    dim3 blk(256), grd = wb_grid(W, H), fgrd((fw + 255) / 256, fh, 2);
     const double N = (double)C * W * H, img = 4.0 * N, fl = 8.0 * (double)fw * fh;
     const float inv_n = (float)(1.0 / N);
     const int nb = (int)grd.x;
@@ -597,8 +602,7 @@ int refine_flows_run(hipStream_t s, int C, int W, int H, const float* a, const f
                    (k_rf_step<<<grd, blk, 0, s>>>(P, truth, inv_n, B.loss_part, grad)));
         TWX_LAUNCH(s, F_DIFF_STEP, 4.0 * nb, (k_rf_loss<<<1, blk, 0, s>>>(B.loss_part, nb, 1.0 / N, B.losses + t)));
         if (!grad) break;
-        TWX_LAUNCH(s, F_DIFF_GATHER, 8.0 * (double)W * H + fl, k_wb_flow_gather<<<fgrd, blk, 0, s>>>(P, 0));
-        TWX_LAUNCH(s, F_DIFF_GATHER, 8.0 * (double)W * H + fl, k_wb_flow_gather<<<fgrd, blk, 0, s>>>(P, 1));
+        TWX_LAUNCH(s, F_DIFF_GATHER, 2.0 * (8.0 * (double)W * H + fl), k_wb_flow_gather<<<fgrd, blk, 0, s>>>(P));
         const double bc1 = 1.0 - std::pow(b1, (double)(t + 1)), bc2 = 1.0 - std::pow(b2, (double)(t + 1));
         A.bc2_sqrt = (float)std::sqrt(bc2);
         A.neg_step = (float)((lr / bc1) * -1.0);
