@@ -543,6 +543,59 @@ __device__ __forceinline__ void run_interior(const float* win, const Taps& t, fl
     }
 }
 
+// The border elements of a tile's horizontal pass: the tile's columns within nh = min(R + 1, w) of
+// either end of the line, whose taps leave it and which take the renormalised formula. They form at
+// most two runs per row, [x0, min(x0 + TILE_X, nh, w)) and [max(x0, nh + hint), min(x0 + TILE_X, w)),
+// so at most 2 (R + 1) columns; the lanes of the block share them out element by element, while
+// every run of the tile takes the interior formula (whose results the write-back drops for these
+// columns). A run holding one border element otherwise sent its whole wave through the border
+// formula for all TILE_RUN elements of every lane.
+template <int R, int C, int RH, int RW, int NT>
+struct HBorder {
+    static constexpr int PER = (RH * 2 * (R + 1) + NT - 1) / NT;
+    float v[PER][C];
+    int x0, bl, brs, nbc;
+    __device__ __forceinline__ HBorder(int tx0, int w) : x0(tx0) {
+        const int nh = min(R + 1, w);
+        const int hint = w > 2 * (R + 1) ? w - 2 * (R + 1) : 0;
+        const int xe = min(x0 + TILE_X, w);
+        bl = max(0, min(xe, nh) - x0);
+        brs = max(x0, nh + hint);
+        nbc = bl + max(0, xe - brs);
+    }
+    // element e of the list: region row e / nbc, the k-th border column of the tile
+    __device__ __forceinline__ bool at(int e, int y0, int h, int& ry, int& x) const {
+        if (e >= RH * nbc) return false;
+        ry = e / nbc;
+        const int k = e - ry * nbc;
+        x = k < bl ? x0 + k : brs + (k - bl);
+        const int y = y0 - R + ry;
+        return y >= 0 && y < h;
+    }
+    template <bool MULTI>
+    __device__ __forceinline__ void compute(const float* s_in, int y0, int w, int h, const Taps& t, int tid) {
+        if (nbc == 0) return;
+#pragma unroll
+        for (int b = 0; b < PER; b++) {
+            int ry, x;
+            if (!at(tid + b * NT, y0, h, ry, x)) continue;
+            const float* line = s_in + ry * RW * C;
+#pragma unroll
+            for (int c = 0; c < C; c++) v[b][c] = line_elem<R, false, MULTI>(line + c, C, x0 - R, w, x, t);
+        }
+    }
+    __device__ __forceinline__ void store(float* s_in, int y0, int h, int tid) const {
+        if (nbc == 0) return;
+#pragma unroll
+        for (int b = 0; b < PER; b++) {
+            int ry, x;
+            if (!at(tid + b * NT, y0, h, ry, x)) continue;
+#pragma unroll
+            for (int c = 0; c < C; c++) s_in[(ry * RW + R + x - x0) * C + c] = v[b][c];
+        }
+    }
+};
+
 // Tiles of a level in an order for the 8 XCDs: the hardware deals workgroup b to XCD b % 8, so
 // tile (b % 8) * ceil(n / 8) + b / 8 gives each XCD a contiguous band of tiles (row-major within a
 // job), whose halos the XCD's own L2 then serves.
@@ -618,8 +671,8 @@ __global__ __launch_bounds__(TILE_X * TY / TILE_RUN) void k_tile_gauss(const Sol
     if (x0 - R >= 1 && x0 + TILE_X + R <= w - 1 && y0 - R >= 1 && y0 + TY + R <= h - 1) region(std::true_type{});
     else region(std::false_type{});
     __syncthreads();
-    // horizontal pass: item = (region row, run of TILE_RUN columns); results held until every
-    // lane has read its window
+    // horizontal pass: item = (region row, run of TILE_RUN columns) with the interior formula, the
+    // border columns element by element (HBorder); results held until every lane has read its window
     const int nh = min(R + 1, w);
     const int hint = w > 2 * (R + 1) ? w - 2 * (R + 1) : 0;
     float hres[HPER][TILE_RUN][C];
@@ -631,26 +684,19 @@ __global__ __launch_bounds__(TILE_X * TY / TILE_RUN) void k_tile_gauss(const Sol
         const int y = y0 - R + ry, xs = x0 + rx0;
         if (y < 0 || y >= h || xs >= w) continue;
         const float* line = s_in + ry * RW * C;
-        if (xs >= nh && xs + TILE_RUN - 1 < nh + hint) {
 #pragma unroll
-            for (int c = 0; c < C; c++) {
-                float win[TILE_RUN + 2 * R];
+        for (int c = 0; c < C; c++) {
+            float win[TILE_RUN + 2 * R];
 #pragma unroll
-                for (int d = 0; d < TILE_RUN + 2 * R; d++) win[d] = line[(rx0 + d) * C + c];
-                float o[TILE_RUN];
-                run_interior<R>(win, t, o);
+            for (int d = 0; d < TILE_RUN + 2 * R; d++) win[d] = line[(rx0 + d) * C + c];
+            float o[TILE_RUN];
+            run_interior<R>(win, t, o);
 #pragma unroll
-                for (int m = 0; m < TILE_RUN; m++) hres[q][m][c] = o[m];
-            }
-        } else {
-#pragma unroll
-            for (int m = 0; m < TILE_RUN; m++) {
-                if (xs + m >= w) continue;
-#pragma unroll
-                for (int c = 0; c < C; c++) hres[q][m][c] = line_elem<R, false, MULTI>(line + c, C, x0 - R, w, xs + m, t);
-            }
+            for (int m = 0; m < TILE_RUN; m++) hres[q][m][c] = o[m];
         }
     }
+    HBorder<R, C, RH, RW, NT> hb(x0, w);
+    hb.template compute<MULTI>(s_in, y0, w, h, t, tid);
     __syncthreads();
 #pragma unroll
     for (int q = 0; q < HPER; q++) {
@@ -661,11 +707,12 @@ __global__ __launch_bounds__(TILE_X * TY / TILE_RUN) void k_tile_gauss(const Sol
         if (y < 0 || y >= h || xs >= w) continue;
 #pragma unroll
         for (int m = 0; m < TILE_RUN; m++) {
-            if (xs + m >= w) continue;
+            if (xs + m < nh || xs + m >= nh + hint) continue;  // outside the line's interior (HBorder's)
 #pragma unroll
             for (int c = 0; c < C; c++) s_in[(ry * RW + R + rx0 + m) * C + c] = hres[q][m][c];
         }
     }
+    hb.store(s_in, y0, h, tid);
     __syncthreads();
     // vertical pass: item = (column, run of TILE_RUN rows), one per lane
     float* out = MODE == P_TENSOR ? J.ta : (MODE == P_CALCB ? J.tb : J.flow[dst]);
@@ -733,17 +780,13 @@ __device__ __forceinline__ bool sep_tile(const Load& load, int w, int h, int x0,
         const int yy = y0 - R + ry, xs = x0 + rx0;
         if (yy < 0 || yy >= h || xs >= w) continue;
         const float* line = s_in + ry * RW;
-        if (xs >= nh && xs + TILE_RUN - 1 < nh + hint) {
-            float win[TILE_RUN + 2 * R];
+        float win[TILE_RUN + 2 * R];
 #pragma unroll
-            for (int d = 0; d < TILE_RUN + 2 * R; d++) win[d] = line[rx0 + d];
-            run_interior<R>(win, t, hres[q]);
-        } else {
-#pragma unroll
-            for (int m = 0; m < TILE_RUN; m++)
-                if (xs + m < w) hres[q][m] = line_elem<R, false, false>(line, 1, x0 - R, w, xs + m, t);
-        }
+        for (int d = 0; d < TILE_RUN + 2 * R; d++) win[d] = line[rx0 + d];
+        run_interior<R>(win, t, hres[q]);
     }
+    HBorder<R, 1, RH, RW, TILE_THREADS> hb(x0, w);
+    hb.template compute<false>(s_in, y0, w, h, t, tid);
     __syncthreads();
 #pragma unroll
     for (int q = 0; q < HPER; q++) {
@@ -754,8 +797,9 @@ __device__ __forceinline__ bool sep_tile(const Load& load, int w, int h, int x0,
         if (yy < 0 || yy >= h || xs >= w) continue;
 #pragma unroll
         for (int m = 0; m < TILE_RUN; m++)
-            if (xs + m < w) s_in[ry * RW + R + rx0 + m] = hres[q][m];
+            if (xs + m >= nh && xs + m < nh + hint) s_in[ry * RW + R + rx0 + m] = hres[q][m];
     }
+    hb.store(s_in, y0, h, tid);
     __syncthreads();
     const int hx = tid % TILE_X, ry0 = (tid / TILE_X) * TILE_RUN;
     x = x0 + hx;
