@@ -953,37 +953,50 @@ __global__ void k_range_update(const SolveJob* __restrict__ jobs, int njobs, int
 }
 
 // upsample_flow (flow.rs:376-391) from level l (buffer 1) into level l-1 (buffer 0)
+// UP_ROWS destination rows per thread: the cell column and its weight are computed once, and a
+// launch has a quarter of the waves (one row per 256-lane block ran at 2.5 TB/s, bound by wave
+// launches rather than by memory)
+constexpr int UP_ROWS = 4;
 __global__ void k_upsample(const SolveJob* __restrict__ jobs, int level) {
     const SolveJob& J = jobs[blockIdx.z];
     if (level > J.start || level == 0) return;
     const int sw = J.w[level], sh = J.h[level];
     const int dw = J.w[level - 1], dh = J.h[level - 1];
     const int x = blockIdx.x * blockDim.x + threadIdx.x;
-    const int y = blockIdx.y;
-    if (x >= dw || y >= dh) return;
+    const int y0 = blockIdx.y * UP_ROWS;
+    if (x >= dw || y0 >= dh) return;
     const gfloat* src = (const gfloat*)J.flow[1];
     float* dst = J.flow[0];
     const float sx = J.up[level][0], sy = J.up[level][1], rx = J.up[level][2], ry = J.up[level][3];
-    float fx = (float)x * rx, fy = (float)y * ry;
-    int ix, iy;
-    float ax, ay;
-    cell(fx, sw, ix, ax);
-    cell(fy, sh, iy, ay);
+    int ix;
+    float ax;
+    cell((float)x * rx, sw, ix, ax);
     // the cell's two flow pixels of its top row, then of its bottom row (a level's flow is under
     // 2^32 bytes: working sizes are at most 16384 x 16384)
-    const uint32_t q = (uint32_t)iy * (uint32_t)sw + (uint32_t)ix;
-    float t0[4], t1[4];
-    ldrun<4>(src, 2 * q, t0);
-    ldrun<4>(src, 2 * (q + (uint32_t)sw), t1);
-    float out[2];
+    float t0[UP_ROWS][4], t1[UP_ROWS][4], ay[UP_ROWS];
 #pragma unroll
-    for (int c = 0; c < 2; c++) {
-        float p00 = t0[c], p10 = t0[2 + c], p01 = t1[c], p11 = t1[2 + c];
-        float top = (p10 - p00) * ax + p00;
-        float bot = (p11 - p01) * ax + p01;
-        out[c] = (bot - top) * ay + top;
+    for (int r = 0; r < UP_ROWS; r++) {
+        const int y = min(y0 + r, dh - 1);
+        int iy;
+        cell((float)y * ry, sh, iy, ay[r]);
+        const uint32_t q = (uint32_t)iy * (uint32_t)sw + (uint32_t)ix;
+        ldrun<4>(src, 2 * q, t0[r]);
+        ldrun<4>(src, 2 * (q + (uint32_t)sw), t1[r]);
     }
-    reinterpret_cast<float2*>(dst)[(long)y * dw + x] = make_float2(out[0] * sx, out[1] * sy);
+#pragma unroll
+    for (int r = 0; r < UP_ROWS; r++) {
+        const int y = y0 + r;
+        if (y >= dh) break;
+        float out[2];
+#pragma unroll
+        for (int c = 0; c < 2; c++) {
+            float p00 = t0[r][c], p10 = t0[r][2 + c], p01 = t1[r][c], p11 = t1[r][2 + c];
+            float top = (p10 - p00) * ax + p00;
+            float bot = (p11 - p01) * ax + p01;
+            out[c] = (bot - top) * ay[r] + top;
+        }
+        reinterpret_cast<float2*>(dst)[(long)y * dw + x] = make_float2(out[0] * sx, out[1] * sy);
+    }
 }
 
 void solve_job_ratios(SolveJob& J) {
@@ -999,19 +1012,35 @@ void solve_job_ratios(SolveJob& J) {
 }
 
 // final negation (flow.rs:1305) of level 0's flow (buffer 1, the output) and the NaN scan the
-// reference device path applies to the downloaded flow (device/solver/mod.rs:575-582)
+// reference device path applies to the downloaded flow (device/solver/mod.rs:575-582); a
+// grid-stride loop over the job's floats, four at a time where the buffer is 16-byte aligned
 __global__ void k_negate(const SolveJob* __restrict__ jobs) {
     const SolveJob& J = jobs[blockIdx.z];
-    const int w = J.w[0], h = J.h[0];
-    const int x = blockIdx.x * blockDim.x + threadIdx.x;
-    const int y = blockIdx.y;
-    if (x >= w || y >= h) return;
+    const long n = (long)J.w[0] * J.h[0] * 2;
     float* flow = J.flow[1];
-    long i = ((long)y * w + x) * 2;
-    float u = -1.0f * flow[i], v = -1.0f * flow[i + 1];
-    flow[i] = u;
-    flow[i + 1] = v;
-    if (isnan(u) || isnan(v)) atomicOr(&J.state[2], 1);
+    const long stride = (long)gridDim.x * blockDim.x;
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    bool bad = false;
+    long done = 0;
+    if (((uintptr_t)flow & 15) == 0) {
+        float4* f4 = reinterpret_cast<float4*>(flow);
+        done = n / 4 * 4;
+        for (long i = t; i < n / 4; i += stride) {
+            float4 v = f4[i];
+            v.x = -1.0f * v.x;
+            v.y = -1.0f * v.y;
+            v.z = -1.0f * v.z;
+            v.w = -1.0f * v.w;
+            bad |= isnan(v.x) || isnan(v.y) || isnan(v.z) || isnan(v.w);
+            f4[i] = v;
+        }
+    }
+    for (long i = done + t; i < n; i += stride) {
+        const float v = -1.0f * flow[i];
+        bad |= isnan(v);
+        flow[i] = v;
+    }
+    if (bad) atomicOr(&J.state[2], 1);
 }
 
 // -------------------------------------------------------------------------------------------
@@ -1535,13 +1564,16 @@ static void launch_solve_range(hipStream_t s, const SolveJob* d_jobs, const std:
             int uw = 0, uh = 0;
             for (auto& j : jobs)
                 if (l <= j.start) { uw = std::max(uw, j.w[l - 1]); uh = std::max(uh, j.h[l - 1]); }
-            TWX_LAUNCH(s, F_UPSAMPLE, 8.0 * P + 8.0 * Pn, k_upsample<<<dim3((uw + 255) / 256, uh, nj), blk, 0, s>>>(d_jobs, l));
+            TWX_LAUNCH(s, F_UPSAMPLE, 8.0 * P + 8.0 * Pn,
+                       k_upsample<<<dim3((uw + 255) / 256, (uh + UP_ROWS - 1) / UP_ROWS, nj), blk, 0, s>>>(d_jobs, l));
         }
     }
     int w0 = 0, h0 = 0;
     double P0 = 0;
     for (auto& j : jobs) { w0 = std::max(w0, j.w[0]); h0 = std::max(h0, j.h[0]); P0 += (double)j.w[0] * j.h[0]; }
-    TWX_LAUNCH(s, F_NEGATE, 16.0 * P0, k_negate<<<dim3((w0 + 255) / 256, h0, nj), dim3(256), 0, s>>>(d_jobs));
+    // about 8 float4 per lane over the largest job
+    const long nb = std::max<long>(1, ((long)w0 * h0 / 2 + 256 * 8 - 1) / (256 * 8));
+    TWX_LAUNCH(s, F_NEGATE, 16.0 * P0, k_negate<<<dim3((unsigned)nb, 1, nj), dim3(256), 0, s>>>(d_jobs));
 }
 
 void launch_solve(hipStream_t s, const SolveJob* d_jobs, const std::vector<SolveJob>& jobs, bool skip_refine) {
