@@ -48,12 +48,14 @@ STAGE_TARGET = 0.60  # of the HBM peak, for the flow and the warp stages
 
 def step_model(params, n_out, select, W, H, bpp, w0, h0, levels, sensitivity):
     """The step-boundary byte model of one render (the CPU path's steps, each step's distinct
-    inputs read once and outputs written once): per solved working pixel, level and direction 588
-    B (grad + tensor 24, tensor blur 48, three iterations of calc_b 32 + B blur 32 + new_disp 36 +
-    flow blur 32, four sweeps 96, range check and damping or upsample 24), per pyramid the frame
-    read plus every level written, per in-between frame both frames read, the output written and
-    both flows read, per exact frame one frame read and one written. Returns (flow bytes, warp
-    bytes, distinct pairs, in-between frames)."""
+    inputs read once and outputs written once): per solved working pixel, level and direction 516
+    B (three iterations of calc_b 32 + B blur 32 + new_disp 36 + flow blur 32, four sweeps 96,
+    range check and damping or upsample 24), per solved working pixel, level and distinct X frame
+    72 B (grad + tensor 24, tensor blur 48: they depend on X alone, and the device computes them
+    once per frame where the CPU path repeats them per solve), per pyramid the frame read plus
+    every level written, per in-between frame both frames read, the output written and both flows
+    read, per exact frame one frame read and one written. Returns (flow bytes, warp bytes,
+    distinct pairs, in-between frames)."""
     import math
 
     pairs, vector_frames, exact_frames = set(), 0, 0
@@ -72,7 +74,9 @@ def step_model(params, n_out, select, W, H, bpp, w0, h0, levels, sensitivity):
     start = min(max(int(math.floor(x)), 0), levels - 1) if x >= 0 else 0
     solved = sum(sizes[:start + 1])
     frames_used = {f for p in pairs for f in p}
-    flow = 2 * len(pairs) * solved * 588 + len(frames_used) * (W * H * bpp + 4 * sum(sizes))
+    # both directions of every pair are solved, so every frame of a pair is the X of a solve
+    flow = (2 * len(pairs) * solved * 516 + len(frames_used) * solved * 72
+            + len(frames_used) * (W * H * bpp + 4 * sum(sizes)))
     warp = vector_frames * (3 * W * H * bpp + 2 * w0 * h0 * 8) + exact_frames * 2 * W * H * bpp
     return flow, warp, len(pairs), vector_frames
 

@@ -491,11 +491,40 @@ int enqueue_solves(hipStream_t s, DevPool& pool, const std::vector<SolveIn>& in,
     float* scratch = pool.alloc<float>(max_px * 12);
     for (auto& ch : chunks) {
         int n = (int)(ch.second - ch.first);
+        // Job order: area descending; within each size bucket of launch_solve, first one job per
+        // distinct X (it computes X's gradient and blurred tensor), then the jobs whose X an earlier
+        // job of the bucket has: they read that job's grad and ta planes, since both depend on X
+        // alone (flow.rs:1274-1277). Reordering inside a bucket keeps its largest job first, so
+        // launch_solve finds the same buckets.
+        std::vector<int> ord(n);
+        for (int k = 0; k < n; k++) ord[k] = k;
+        auto area = [&](int k) { return (long)in[ch.first + k].geo.w0 * in[ch.first + k].geo.h0; };
+        std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return area(a) > area(b); });
+        std::vector<double> areas;
+        for (int k : ord) areas.push_back((double)area(k));
+        std::vector<int> order, owner(n, -1);  // owner[k]: position in `order` of job k's tensor job
+        for (auto [b0, b1] : size_buckets(areas)) {
+            std::map<const float*, int> first;
+            std::vector<int> shared;
+            for (int i = b0; i < b1; i++) {
+                const int k = ord[i];
+                auto it = first.find(in[ch.first + k].X);
+                if (it == first.end()) {
+                    first[in[ch.first + k].X] = (int)order.size();
+                    order.push_back(k);
+                } else {
+                    owner[k] = it->second;
+                    shared.push_back(k);
+                }
+            }
+            order.insert(order.end(), shared.begin(), shared.end());
+        }
         std::vector<SolveJob> jobs(n);
         float* sp = scratch;
-        for (int k = 0; k < n; k++) {
+        for (int q = 0; q < n; q++) {
+            const int k = order[q];
             const SolveIn& si = in[ch.first + k];
-            SolveJob& J = jobs[k];
+            SolveJob& J = jobs[q];
             memset(&J, 0, sizeof(J));
             const float* xl = si.X;
             const float* yl = si.Y;
@@ -517,6 +546,11 @@ int enqueue_solves(hipStream_t s, DevPool& pool, const std::vector<SolveIn>& in,
             J.tmp = sp + px * 4;  // 8-byte aligned (flow pixels are moved as float2 through it)
             J.tb = sp + px * 7;
             J.ta = sp + px * 9;
+            if (owner[k] >= 0) {
+                J.grad = jobs[owner[k]].grad;
+                J.ta = jobs[owner[k]].ta;
+                J.tensor_shared = 1;
+            }
             sp += px * 12;
             J.state = states + (ch.first + k) * 4;
             J.max_w = si.geo.w0;

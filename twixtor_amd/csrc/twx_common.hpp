@@ -68,6 +68,9 @@ struct SolveJob {
     float* tb;             // right-hand side B, 2 channels
     float* tmp;            // block-matching scratch: the transposed flow (2 channels)
     int* state;            // [0] range check still active, [1] any pixel in range, [2] NaN seen
+    // 1: grad and ta are those of an earlier job of the same launch with the same X, which
+    // computes them (they depend on X alone, flow.rs:1274-1277); the tensor kernel skips this job
+    int tensor_shared;
     int max_w, max_h;
     // upsample_flow from level l to l - 1 (flow.rs:376-391), the same for every pixel: the scales
     // sx = (dw - 1) / (sw - 1), sy and their reciprocals (through double), at index l. Filled by

@@ -621,7 +621,7 @@ __global__ __launch_bounds__(TILE_X * TY / TILE_RUN) void k_tile_gauss(const Sol
     const int job = lt / per_job, rem = lt - job * per_job;
     const int tyi = rem / tiles_x, txi = rem - tyi * tiles_x;
     const SolveJob& J = jobs[job];
-    if (level > J.start) return;
+    if (level > J.start || (MODE == P_TENSOR && J.tensor_shared)) return;
     const int w = J.w[level], h = J.h[level];
     const int x0 = txi * TILE_X, y0 = tyi * TY;
     constexpr int NT = TILE_X * TY / TILE_RUN;  // one vertical run of TILE_RUN rows per lane
@@ -1543,8 +1543,19 @@ static void launch_solve_range(hipStream_t s, const SolveJob* d_jobs, const std:
                 if (l > 0) Pn += (double)j.w[l - 1] * j.h[l - 1];
             }
         dim3 blk(256), grd((mw + 255) / 256, mh, nj);
-        // grad + tensor, blurred r = 7: X 4 B in, grad 8 + tensor 12 B out
-        TWX_LAUNCH(s, F_GRAD_TENSOR, 24.0 * P, tile_launch<3, 7, P_TENSOR>(s, d_jobs, l, 0, 0, g7, mw, mh, nj));
+        // grad + tensor, blurred r = 7, once per distinct X: X 4 B in, grad 8 + tensor 12 B out. The
+        // jobs that compute them lead the range; a job sharing another's planes is skipped
+        int nt = 0, tw = 0, th = 0;
+        double Pt = 0;
+        for (int k = 0; k < nj; k++)
+            if (!jobs[k].tensor_shared && l <= jobs[k].start) {
+                nt = k + 1;
+                tw = std::max(tw, jobs[k].w[l]);
+                th = std::max(th, jobs[k].h[l]);
+                Pt += (double)jobs[k].w[l] * jobs[k].h[l];
+            }
+        if (nt > 0)
+            TWX_LAUNCH(s, F_GRAD_TENSOR, 24.0 * Pt, tile_launch<3, 7, P_TENSOR>(s, d_jobs, l, 0, 0, g7, tw, th, nt));
         for (int it = 0; it < 3; it++) {
             const int src = it & 1, dst = src ^ 1;
             // calc_b + B blur: flow 8 + X 4 + Y 4 + grad 8 in, B 8 out
