@@ -1696,32 +1696,45 @@ __device__ __forceinline__ void warp_point(const float* __restrict__ flow, int f
     iy = d[1] * sy + yf;
 }
 
-// bilerp4 (warp.rs:306-331) at a point inside [0, W-1] x [0, H-1], before the store rule
+// bilerp4 (warp.rs:306-331) at a point inside [0, W-1] x [0, H-1], before the store rule, in two
+// parts: the cell's four texels (t[0] = (kx, ky), t[1] = (kx + 1, ky), t[2] = (kx, ky + 1),
+// t[3] = (kx + 1, ky + 1)) and the fractions, then the lerps. A caller with several samples
+// fetches them all before mixing any, so their loads are in flight together.
 template <int FMT>
-__device__ __forceinline__ void bilerp_src(const void* src, int W, int H, float ix, float iy, float o[4]) {
+__device__ __forceinline__ void bilerp_fetch(const void* src, int W, int H, float ix, float iy, float t[4][4], float& ax,
+                                             float& ay) {
     int w1i = W - 1, h1i = H - 1;
     int32_t kx = (int32_t)ix, ky = (int32_t)iy;  // the point is inside the image: cvtt's values
-    float ax = ix - (float)kx, ay = iy - (float)ky;
+    ax = ix - (float)kx;
+    ay = iy - (float)ky;
     if (kx >= w1i) { kx = w1i - 1; ax = 1.0f; }
     if (ky >= h1i) { ky = h1i - 1; ay = 1.0f; }
     uint32_t i00 = (uint32_t)ky * W + kx, i01 = i00 + W;
-    float p00[4], p10[4], p01[4], p11[4];
-    load4<FMT>(src, i00, p00);
-    load4<FMT>(src, i00 + 1, p10);
-    load4<FMT>(src, i01, p01);
-    load4<FMT>(src, i01 + 1, p11);
+    load4<FMT>(src, i00, t[0]);
+    load4<FMT>(src, i00 + 1, t[1]);
+    load4<FMT>(src, i01, t[2]);
+    load4<FMT>(src, i01 + 1, t[3]);
+}
+template <int FMT>
+__device__ __forceinline__ void bilerp_mix(const float t[4][4], float ax, float ay, float o[4]) {
 #pragma unroll
     for (int k = 0; k < 4; k++) {
-        float top = p00[k] + (p10[k] - p00[k]) * ax;
-        float bot = p01[k] + (p11[k] - p01[k]) * ax;
+        float top = t[0][k] + (t[1][k] - t[0][k]) * ax;
+        float bot = t[2][k] + (t[3][k] - t[2][k]) * ax;
         o[k] = top + (bot - top) * ay;
     }
     if (FMT == 2 && (isnan(o[0]) || isnan(o[1]) || isnan(o[2]) || isnan(o[3]))) {
         // warp.rs:334-343: the same lerps again with the NaN rule
 #pragma unroll
         for (int k = 0; k < 4; k++)
-            o[k] = nr_lerp_bf(nr_lerp_bf(p00[k], p10[k], ax), nr_lerp_bf(p01[k], p11[k], ax), ay);
+            o[k] = nr_lerp_bf(nr_lerp_bf(t[0][k], t[1][k], ax), nr_lerp_bf(t[2][k], t[3][k], ax), ay);
     }
+}
+template <int FMT>
+__device__ __forceinline__ void bilerp_src(const void* src, int W, int H, float ix, float iy, float o[4]) {
+    float t[4][4], ax, ay;
+    bilerp_fetch<FMT>(src, W, H, ix, iy, t, ax, ay);
+    bilerp_mix<FMT>(t, ax, ay, o);
 }
 
 // One warped pixel W(src, flow, amount) at output (x, y) with hold edges (warp.rs:500-512), as the
@@ -1741,6 +1754,37 @@ __device__ __forceinline__ bool warp_pixel(const void* src, int W, int H, const 
 #pragma unroll
     for (int k = 0; k < 4; k++) o[k] = store_rule<FMT>(s[k]);
     return true;
+}
+
+// Both warped pixels of a blend, W(A, F_BA, frac) into pa and W(B, F_AB, wa) into pb, with hold
+// edges: the values of two warp_pixel calls, with the two flow samples' loads issued together and
+// then the eight texels' loads together (a point that clip_to_rect refuses or that is NaN samples
+// the image's first cell and gives zeros), so a pixel waits for two memory round trips, not four.
+// Returns false when either point is refused.
+template <int FMT>
+__device__ __forceinline__ bool warp_pixel_pair(const FrameJob& F, int x, int y, float pa[4], float pb[4]) {
+    float ixa, iya, ixb, iyb;
+    warp_point(F.f_ba, F.fw, F.fh, F.rx, F.ry, F.sx[0], F.sy[0], x, y, ixa, iya);
+    warp_point(F.f_ab, F.fw, F.fh, F.rx, F.ry, F.sx[1], F.sy[1], x, y, ixb, iyb);
+    const float w1 = (float)(F.W - 1), h1 = (float)(F.H - 1);
+    const bool oka = clip_to_rect((float)x, (float)y, ixa, iya, w1, h1) && !isnan(ixa) && !isnan(iya);
+    const bool okb = clip_to_rect((float)x, (float)y, ixb, iyb, w1, h1) && !isnan(ixb) && !isnan(iyb);
+    ixa = oka ? ixa : 0.0f;
+    iya = oka ? iya : 0.0f;
+    ixb = okb ? ixb : 0.0f;
+    iyb = okb ? iyb : 0.0f;
+    float ta[4][4], tb[4][4], axa, aya, axb, ayb;
+    bilerp_fetch<FMT>(F.a, F.W, F.H, ixa, iya, ta, axa, aya);
+    bilerp_fetch<FMT>(F.b, F.W, F.H, ixb, iyb, tb, axb, ayb);
+    float sa[4], sb[4];
+    bilerp_mix<FMT>(ta, axa, aya, sa);
+    bilerp_mix<FMT>(tb, axb, ayb, sb);
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        pa[k] = oka ? store_rule<FMT>(sa[k]) : 0.0f;
+        pb[k] = okb ? store_rule<FMT>(sb[k]) : 0.0f;
+    }
+    return oka && okb;
 }
 
 // The store of a faded edge sample (warp.rs:237-295): the stored sample times f,
@@ -1965,6 +2009,8 @@ __global__ void k_frames(const FrameJob* __restrict__ jobs) {
         if (F.pre_a) {
             load4<FMT>(F.pre_a, mi, pa);
             load4<FMT>(F.pre_b, mi, pb);
+        } else if (!F.warp_a_copy && !F.warp_b_copy) {
+            ok = warp_pixel_pair<FMT>(F, x, y, pa, pb);
         } else {
             if (F.warp_a_copy) load4<FMT>(F.a, mi, pa);
             else ok = warp_pixel<FMT>(F.a, F.W, F.H, F.f_ba, F.fw, F.fh, F, 0, x, y, pa);
