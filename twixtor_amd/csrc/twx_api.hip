@@ -16,6 +16,7 @@
 #include <map>
 #include <memory>
 #include <mutex>
+#include <set>
 #include <thread>
 #include <tuple>
 #include <vector>
@@ -491,11 +492,11 @@ int enqueue_solves(hipStream_t s, DevPool& pool, const std::vector<SolveIn>& in,
     float* scratch = pool.alloc<float>(max_px * 12);
     for (auto& ch : chunks) {
         int n = (int)(ch.second - ch.first);
-        // Job order: area descending; within each size bucket of launch_solve, first one job per
-        // distinct X (it computes X's gradient and blurred tensor), then the jobs whose X an earlier
-        // job of the bucket has: they read that job's grad and ta planes, since both depend on X
-        // alone (flow.rs:1274-1277). Reordering inside a bucket keeps its largest job first, so
-        // launch_solve finds the same buckets.
+        // Job order: area descending, in size buckets (one launch sequence each); within a bucket
+        // first one job per distinct X (it computes X's gradient and blurred tensor), then the jobs
+        // whose X an earlier job of the bucket has: they read that job's grad and ta planes, since
+        // both depend on X alone (flow.rs:1274-1277). Each bucket also lists its distinct frames,
+        // whose transposed levels the row sweeps of all its jobs read.
         std::vector<int> ord(n);
         for (int k = 0; k < n; k++) ord[k] = k;
         auto area = [&](int k) { return (long)in[ch.first + k].geo.w0 * in[ch.first + k].geo.h0; };
@@ -503,21 +504,54 @@ int enqueue_solves(hipStream_t s, DevPool& pool, const std::vector<SolveIn>& in,
         std::vector<double> areas;
         for (int k : ord) areas.push_back((double)area(k));
         std::vector<int> order, owner(n, -1);  // owner[k]: position in `order` of job k's tensor job
+        std::vector<SolveBucket> buckets;
+        std::vector<const float*> frames;      // the bucket's distinct frames, bucket after bucket
+        std::vector<int> frame_geo;            // a job (index into `in` of the chunk) with the frame
         for (auto [b0, b1] : size_buckets(areas)) {
+            SolveBucket B{(int)order.size(), 0, (int)frames.size(), 0};
             std::map<const float*, int> first;
+            std::set<const float*> seen;
             std::vector<int> shared;
             for (int i = b0; i < b1; i++) {
                 const int k = ord[i];
-                auto it = first.find(in[ch.first + k].X);
+                const SolveIn& si = in[ch.first + k];
+                auto it = first.find(si.X);
                 if (it == first.end()) {
-                    first[in[ch.first + k].X] = (int)order.size();
+                    first[si.X] = (int)order.size();
                     order.push_back(k);
                 } else {
                     owner[k] = it->second;
                     shared.push_back(k);
                 }
+                for (const float* f : {si.X, si.Y})
+                    if (seen.insert(f).second) {
+                        frames.push_back(f);
+                        frame_geo.push_back(k);
+                    }
             }
             order.insert(order.end(), shared.begin(), shared.end());
+            B.j1 = (int)order.size();
+            B.f1 = (int)frames.size();
+            buckets.push_back(B);
+        }
+        // the frames' transposed planes, one level at a time (the largest, level 0, sized)
+        std::vector<PlaneT> planes(frames.size());
+        std::map<const float*, const float*> tplane;
+        for (size_t f = 0; f < frames.size(); f++) {
+            const SolveIn& si = in[ch.first + frame_geo[f]];
+            PlaneT& P = planes[f];
+            memset(&P, 0, sizeof(P));
+            const float* lv = frames[f];
+            for (int l = 0; l < si.geo.levels; l++) {
+                P.lv[l] = lv;
+                P.w[l] = si.geo.w[l];
+                P.h[l] = si.geo.h[l];
+                lv += (size_t)si.geo.w[l] * si.geo.h[l];
+            }
+            float damp;
+            start_level(fs.sensitivity, si.geo.levels, P.start, damp);
+            P.dst = pool.alloc<float>((size_t)si.geo.w0 * si.geo.h0);
+            tplane[frames[f]] = P.dst;
         }
         std::vector<SolveJob> jobs(n);
         float* sp = scratch;
@@ -552,13 +586,16 @@ int enqueue_solves(hipStream_t s, DevPool& pool, const std::vector<SolveIn>& in,
                 J.tensor_shared = 1;
             }
             sp += px * 12;
+            J.xt = tplane[si.X];
+            J.yt = tplane[si.Y];
             J.state = states + (ch.first + k) * 4;
             J.max_w = si.geo.w0;
             J.max_h = si.geo.h0;
             solve_job_ratios(J);
         }
         SolveJob* d = pool.upload(jobs);
-        launch_solve(s, d, jobs, fs.skip_refine != 0);  // starts with the per-job init kernel
+        PlaneT* dp = pool.upload(planes);
+        launch_solve(s, d, jobs, dp, planes, buckets, fs.skip_refine != 0);  // starts with the per-job init kernel
     }
     (void)err;
     return TWX_OK;

@@ -71,6 +71,10 @@ struct SolveJob {
     // 1: grad and ta are those of an earlier job of the same launch with the same X, which
     // computes them (they depend on X alone, flow.rs:1274-1277); the tensor kernel skips this job
     int tensor_shared;
+    // the transposed X and Y of the level whose rows are being swept: one plane per distinct frame
+    // of the launch, shared by its jobs (written by the plane transposes, PlaneT)
+    const float* xt;
+    const float* yt;
     int max_w, max_h;
     // upsample_flow from level l to l - 1 (flow.rs:376-391), the same for every pixel: the scales
     // sx = (dw - 1) / (sw - 1), sy and their reciprocals (through double), at index l. Filled by
@@ -224,7 +228,21 @@ void launch_ingest(hipStream_t s, const void* wire, void* engine, int W, int H, 
 void launch_ingest_batch(hipStream_t s, const IngestOp* d_ops, const std::vector<IngestOp>& ops);
 void launch_blur5_resample(hipStream_t s, const PlaneOp* d_ops, const std::vector<PlaneOp>& ops, int maxw, int maxh,
                            const Taps& t);
-void launch_solve(hipStream_t s, const SolveJob* d_jobs, const std::vector<SolveJob>& jobs, bool skip_refine);
+// A distinct frame of a solve launch: its band-pass levels and the plane that receives the
+// transposed level for the row sweeps of its jobs (levels up to start).
+struct PlaneT {
+    const float* lv[MAXL];
+    float* dst;
+    int w[MAXL], h[MAXL];
+    int start;
+};
+// A size bucket of a solve launch: jobs [j0, j1) and the frames [f0, f1) they read. The jobs that
+// compute a gradient and tensor (tensor_shared 0) lead the bucket's jobs.
+struct SolveBucket {
+    int j0, j1, f0, f1;
+};
+void launch_solve(hipStream_t s, const SolveJob* d_jobs, const std::vector<SolveJob>& jobs, const PlaneT* d_planes,
+                  const std::vector<PlaneT>& planes, const std::vector<SolveBucket>& buckets, bool skip_refine);
 void launch_frames(hipStream_t s, const FrameJob* d_jobs, const std::vector<FrameJob>& jobs);
 // The Weighted Blend jobs (kinds 3 and 6) after launch_frames: mask clean-up, sharpening, then
 // blend and finish (kind 3) or the Smart Blend composite into the frame's image (kind 6).

@@ -1140,47 +1140,62 @@ struct BmFlow {
 // Transposed copies for the row sweeps, through 32 x 32 tiles in LDS: X and Y (level `level`)
 // into tb's two planes and, with FLOW, the flow (buffer `buf`) into the scratch; OUT copies the
 // scratch back into the flow. A flow pixel (u, v) moves as one 8-byte access.
-template <bool FLOW, bool OUT>
+// The transposed copy of a frame's level for the row sweeps (element (x, y) at x * h + y): 32 x 32
+// tiles through LDS. One plane per distinct frame of the launch (blockIdx.z), which every job with
+// the frame as X or Y reads.
+__global__ void k_plane_transpose(const PlaneT* __restrict__ ops, int level) {
+    const PlaneT& P = ops[blockIdx.z];
+    if (level > P.start) return;
+    const int w = P.w[level], h = P.h[level];
+    if (w < 4 || h < 4) return;  // no sweep runs on such a level
+    __shared__ float t[32][33];
+    const int bx = blockIdx.x * 32, by = blockIdx.y * 32;
+    if (bx >= w || by >= h) return;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 256 threads: 32 x 8
+    const float* src = P.lv[level];
+    for (int r = ty; r < 32; r += 8) {
+        const int x = bx + tx, y = by + r;
+        if (x < w && y < h) t[r][tx] = src[(long)y * w + x];
+    }
+    __syncthreads();
+    for (int r = ty; r < 32; r += 8) {
+        const int x = bx + r, y = by + tx;
+        if (x < w && y < h) P.dst[(long)x * h + y] = t[tx][r];
+    }
+}
+
+// The flow into its transposed scratch copy for the row sweeps (OUT false) and back (OUT true),
+// for levels whose row sweeps run on the transposed flow; a flow pixel (u, v) moves as one 8-byte
+// access (the scratch slice starts 8-byte aligned).
+template <bool OUT>
 __global__ void k_bm_transpose(const SolveJob* __restrict__ jobs, int level, int buf) {
     const SolveJob& J = jobs[blockIdx.z];
     if (level > J.start) return;
     const int w = J.w[level], h = J.h[level];
     if (w < 4 || h < 4) return;
-    constexpr int NP = OUT ? 2 : (FLOW ? 4 : 2);
-    __shared__ float t[NP][32][33];
+    __shared__ float t[2][32][33];
     const int bx = blockIdx.x * 32, by = blockIdx.y * 32;
     if (bx >= w || by >= h) return;
     const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 256 threads: 32 x 8
-    const long px = (long)w * h;
     float2* flow = reinterpret_cast<float2*>(J.flow[buf]);
     float2* ft = reinterpret_cast<float2*>(J.tmp);
     if (!OUT) {
         for (int r = ty; r < 32; r += 8) {
-            int x = bx + tx, y = by + r;
+            const int x = bx + tx, y = by + r;
             if (x < w && y < h) {
-                long i = (long)y * w + x;
-                t[0][r][tx] = J.X[level][i];
-                t[1][r][tx] = J.Y[level][i];
-                if (FLOW) {
-                    const float2 f = flow[i];
-                    t[NP - 2][r][tx] = f.x;
-                    t[NP - 1][r][tx] = f.y;
-                }
+                const float2 f = flow[(long)y * w + x];
+                t[0][r][tx] = f.x;
+                t[1][r][tx] = f.y;
             }
         }
         __syncthreads();
         for (int r = ty; r < 32; r += 8) {
-            int x = bx + r, y = by + tx;
-            if (x < w && y < h) {
-                long o = (long)x * h + y;
-                J.tb[o] = t[0][tx][r];
-                J.tb[px + o] = t[1][tx][r];
-                if (FLOW) ft[o] = make_float2(t[NP - 2][tx][r], t[NP - 1][tx][r]);
-            }
+            const int x = bx + r, y = by + tx;
+            if (x < w && y < h) ft[(long)x * h + y] = make_float2(t[0][tx][r], t[1][tx][r]);
         }
     } else {
         for (int r = ty; r < 32; r += 8) {
-            int x = bx + r, y = by + tx;
+            const int x = bx + r, y = by + tx;
             if (x < w && y < h) {
                 const float2 f = ft[(long)x * h + y];
                 t[0][tx][r] = f.x;
@@ -1189,7 +1204,7 @@ __global__ void k_bm_transpose(const SolveJob* __restrict__ jobs, int level, int
         }
         __syncthreads();
         for (int r = ty; r < 32; r += 8) {
-            int x = bx + tx, y = by + r;
+            const int x = bx + tx, y = by + r;
             if (x < w && y < h) flow[(long)y * w + x] = make_float2(t[0][r][tx], t[1][r][tx]);
         }
     }
@@ -1333,8 +1348,8 @@ __global__ __launch_bounds__(64) void k_bm_sweep(const SolveJob* __restrict__ jo
     const int cc = c0 + (lane >> 1);
     const bool live = cc <= m - 2;
     const int ce = live ? cc : m - 2;
-    const BmPlane B(T ? J.tb + (long)w * h : J.Y[level], w, h, n_al, m);
-    gchar* A = (gchar*)(T ? J.tb : J.X[level]);
+    const BmPlane B(T ? J.yt : J.Y[level], w, h, n_al, m);
+    gchar* A = (gchar*)(T ? J.xt : J.X[level]);
     // flow planes: the plain flow (walk layout for the columns, cross for the rows) and the rows'
     // scratch (walk layout). CROSS rows: sweep 1 reads plain and writes scratch, sweep 2 the other
     // way round; otherwise the rows run on a transposed copy of the flow in the scratch.
@@ -1498,7 +1513,8 @@ int tile_tuning_report(int* out, int cap) {
 
 
 
-static void launch_solve_range(hipStream_t s, const SolveJob* d_jobs, const std::vector<SolveJob>& jobs, bool skip_refine) {
+static void launch_solve_range(hipStream_t s, const SolveJob* d_jobs, const std::vector<SolveJob>& jobs,
+                               const PlaneT* d_planes, const std::vector<PlaneT>& planes, bool skip_refine) {
     int nj = (int)jobs.size();
     if (nj == 0) return;
     int max_start = 0;
@@ -1514,18 +1530,30 @@ static void launch_solve_range(hipStream_t s, const SolveJob* d_jobs, const std:
         dim3 tb(256), tg((mw + 31) / 32, (mh + 31) / 32, nj);
         // a lane pair per chain: 32 chains per 64-lane block
         dim3 cb(64), rg((mh + 31) / 32, 1, nj), cg((mw + 31) / 32, 1, nj);
+        // the transposed X and Y of the level, once per distinct frame: 4 + 4 B per frame pixel
+        int fw = 0, fh = 0;
+        double Pf = 0;
+        for (auto& f : planes)
+            if (l <= f.start) {
+                fw = std::max(fw, f.w[l]);
+                fh = std::max(fh, f.h[l]);
+                Pf += (double)f.w[l] * f.h[l];
+            }
+        if (Pf > 0)
+            TWX_LAUNCH(s, F_BM_TRANSPOSE, 8.0 * Pf,
+                       (k_plane_transpose<<<dim3((fw + 31) / 32, (fh + 31) / 32, (unsigned)planes.size()), tb, 0, s>>>(
+                           d_planes, l)));
         // sweeps: A 4 + B 4 + flow read and write 16 B/px each
         if (P <= BM_CROSS_MAX_PIXELS) {
-            // transposed X and Y 8 + 8 B/px; the row sweeps move the flow between layouts
-            TWX_LAUNCH(s, F_BM_TRANSPOSE, 16.0 * P, (k_bm_transpose<false, false><<<tg, tb, 0, s>>>(d_jobs, l, buf)));
+            // the row sweeps move the flow between layouts
             TWX_LAUNCH(s, F_BM_ROWS, 24.0 * P, (k_bm_sweep<true, 1, BM_CHUNK, true><<<rg, cb, 0, s>>>(d_jobs, l, buf)));
             TWX_LAUNCH(s, F_BM_ROWS, 24.0 * P, (k_bm_sweep<true, -1, BM_CHUNK, true><<<rg, cb, 0, s>>>(d_jobs, l, buf)));
         } else {
-            // transposed X, Y and flow 16 + 16 B/px, and the flow back 8 + 8 B/px
-            TWX_LAUNCH(s, F_BM_TRANSPOSE, 32.0 * P, (k_bm_transpose<true, false><<<tg, tb, 0, s>>>(d_jobs, l, buf)));
+            // the transposed flow 8 + 8 B/px, and back 8 + 8 B/px
+            TWX_LAUNCH(s, F_BM_TRANSPOSE, 16.0 * P, (k_bm_transpose<false><<<tg, tb, 0, s>>>(d_jobs, l, buf)));
             TWX_LAUNCH(s, F_BM_ROWS, 24.0 * P, (k_bm_sweep<true, 1, BM_CHUNK, false><<<rg, cb, 0, s>>>(d_jobs, l, buf)));
             TWX_LAUNCH(s, F_BM_ROWS, 24.0 * P, (k_bm_sweep<true, -1, BM_CHUNK, false><<<rg, cb, 0, s>>>(d_jobs, l, buf)));
-            TWX_LAUNCH(s, F_BM_TRANSPOSE, 16.0 * P, (k_bm_transpose<false, true><<<tg, tb, 0, s>>>(d_jobs, l, buf)));
+            TWX_LAUNCH(s, F_BM_TRANSPOSE, 16.0 * P, (k_bm_transpose<true><<<tg, tb, 0, s>>>(d_jobs, l, buf)));
         }
         TWX_LAUNCH(s, F_BM_COLS, 24.0 * P, (k_bm_sweep<false, 1, BM_CHUNK, false><<<cg, cb, 0, s>>>(d_jobs, l, buf)));
         TWX_LAUNCH(s, F_BM_COLS, 24.0 * P, (k_bm_sweep<false, -1, BM_CHUNK, false><<<cg, cb, 0, s>>>(d_jobs, l, buf)));
@@ -1587,11 +1615,11 @@ static void launch_solve_range(hipStream_t s, const SolveJob* d_jobs, const std:
     TWX_LAUNCH(s, F_NEGATE, 16.0 * P0, k_negate<<<dim3((unsigned)nb, 1, nj), dim3(256), 0, s>>>(d_jobs));
 }
 
-void launch_solve(hipStream_t s, const SolveJob* d_jobs, const std::vector<SolveJob>& jobs, bool skip_refine) {
-    std::vector<double> area;
-    for (auto& j : jobs) area.push_back((double)j.w[0] * j.h[0]);
-    for (auto [b0, b1] : size_buckets(area))
-        launch_solve_range(s, d_jobs + b0, std::vector<SolveJob>(jobs.begin() + b0, jobs.begin() + b1), skip_refine);
+void launch_solve(hipStream_t s, const SolveJob* d_jobs, const std::vector<SolveJob>& jobs, const PlaneT* d_planes,
+                  const std::vector<PlaneT>& planes, const std::vector<SolveBucket>& buckets, bool skip_refine) {
+    for (const SolveBucket& b : buckets)
+        launch_solve_range(s, d_jobs + b.j0, std::vector<SolveJob>(jobs.begin() + b.j0, jobs.begin() + b.j1),
+                           d_planes + b.f0, std::vector<PlaneT>(planes.begin() + b.f0, planes.begin() + b.f1), skip_refine);
 }
 
 __global__ void k_nan_scan(const float* __restrict__ d, size_t n, int* flag) {
