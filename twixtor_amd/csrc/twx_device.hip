@@ -668,8 +668,39 @@ __global__ __launch_bounds__(TILE_X * TY / TILE_RUN) void k_tile_gauss(const Sol
                 }
         }
     };
+    // a tile at the image's edge: only the part of its region inside the image, [xa, xb) x [ya, yb)
+    // in region coordinates, is produced (at a small level a 64-wide region is mostly outside)
+    auto region_clipped = [&]() {
+        const int xa = max(0, R - x0), xb = min(RW, w - x0 + R);
+        const int ya = max(0, R - y0), yb = min(RH, h - y0 + R);
+        const int bw = xb - xa, cnt = bw * (yb - ya);
+#pragma unroll 1
+        for (int k0 = 0; k0 < cnt; k0 += PU * NT) {
+            float v[PU][C];
+            int slot[PU], px[PU], py[PU];
+            bool own[PU];
+#pragma unroll
+            for (int u = 0; u < PU; u++) {
+                const int k = k0 + u * NT + tid;
+                const bool keep = k < cnt;
+                const int kk = keep ? k : 0;
+                const int ry = ya + kk / bw, rx = xa + kk - (kk / bw) * bw;
+                px[u] = x0 - R + rx;
+                py[u] = y0 - R + ry;
+                own[u] = keep && rx >= R && rx < R + TILE_X && ry >= R && ry < R + TY;
+                slot[u] = keep ? ry * RW + rx : -1;
+            }
+            produce<MODE, C, PU, false>(J, level, src, w, h, px, py, own, v);
+#pragma unroll
+            for (int u = 0; u < PU; u++)
+                if (slot[u] >= 0) {
+#pragma unroll
+                    for (int c = 0; c < C; c++) s_in[slot[u] * C + c] = v[u][c];
+                }
+        }
+    };
     if (x0 - R >= 1 && x0 + TILE_X + R <= w - 1 && y0 - R >= 1 && y0 + TY + R <= h - 1) region(std::true_type{});
-    else region(std::false_type{});
+    else region_clipped();
     __syncthreads();
     // horizontal pass: item = (region row, run of TILE_RUN columns) with the interior formula, the
     // border columns element by element (HBorder); results held until every lane has read its window
