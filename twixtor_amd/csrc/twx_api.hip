@@ -471,7 +471,7 @@ struct SolveIn {
 // Enqueues every solve, in chunks that fit the scratch budget, with no host synchronisation:
 // solve k keeps its state in states[4k .. 4k + 3] (device; [2] is set for a NaN in its flow).
 int enqueue_solves(hipStream_t s, DevPool& pool, const std::vector<SolveIn>& in, const twx_flow_settings& fs,
-                   uint64_t scratch_budget, int* states, std::string& err) {
+                   uint64_t scratch_budget, int* states, std::string& err, bool raw) {
     if (in.empty()) return TWX_OK;
     // chunks by the scratch bytes of each job: flow[1] 2 + grad 2 + ta 3 + tb 2 + tmp 3 floats/pixel
     std::vector<std::pair<size_t, size_t>> chunks;
@@ -595,7 +595,7 @@ int enqueue_solves(hipStream_t s, DevPool& pool, const std::vector<SolveIn>& in,
         }
         SolveJob* d = pool.upload(jobs);
         PlaneT* dp = pool.upload(planes);
-        launch_solve(s, d, jobs, dp, planes, buckets, fs.skip_refine != 0);  // starts with the per-job init kernel
+        launch_solve(s, d, jobs, dp, planes, buckets, fs.skip_refine != 0, raw);  // starts with the per-job init kernel
     }
     (void)err;
     return TWX_OK;
@@ -617,7 +617,7 @@ int check_solve_states(hipStream_t s, DevPool& pool, const int* states, size_t n
 int run_solves(hipStream_t s, DevPool& pool, const std::vector<SolveIn>& in, const twx_flow_settings& fs,
                uint64_t scratch_budget, std::string& err) {
     int* states = pool.alloc<int>(std::max<size_t>(in.size(), 1) * 4);
-    int st = enqueue_solves(s, pool, in, fs, scratch_budget, states, err);
+    int st = enqueue_solves(s, pool, in, fs, scratch_budget, states, err, false);
     if (st) return st;
     return check_solve_states(s, pool, states, in.size(), err);
 }
@@ -1008,7 +1008,9 @@ int retime_batch_enqueue(twx_handle* h, DevPool& pool, const std::vector<ClipRun
             std::stable_sort(kv.second.begin(), kv.second.end(), [](const SolveIn& a, const SolveIn& b) {
                 return (long)a.geo.w0 * a.geo.h0 > (long)b.geo.w0 * b.geo.h0;
             });
-            if ((st = enqueue_solves(s, pool, kv.second, fs, h->budget / 2, flags.states + state_off * 4, err))) return st;
+            // the flows' only readers are this render's inverse warps (flow_raw below)
+            if ((st = enqueue_solves(s, pool, kv.second, fs, h->budget / 2, flags.states + state_off * 4, err, true)))
+                return st;
             state_off += kv.second.size();
         }
     }
@@ -1057,6 +1059,7 @@ int retime_batch_enqueue(twx_handle* h, DevPool& pool, const std::vector<ClipRun
                 F.fh = g.h0;
                 if (fsv[r][i].f_ab >= 0) F.f_ab = solves[fsv[r][i].f_ab].out;
                 if (fsv[r][i].f_ba >= 0) F.f_ba = solves[fsv[r][i].f_ba].out;
+                F.flow_raw = 1;
                 F.warp_a_copy = warp_is_copy(F.frac, g.w0, W) ? 1 : 0;
                 F.warp_b_copy = warp_is_copy(F.wa, g.w0, W) ? 1 : 0;
                 if (R.p.smart_blend) {

@@ -106,7 +106,9 @@ struct FrameJob {
     int nearest_from_a;    // Nearest: warp A with F_BA (wa >= 0.5), else warp B with F_AB
     int plus_zero;         // weighted blend without flows on float images: inputs + 0.0 first
     int finish;            // 1: unpremultiply into the wire frame, rows turned back; 0: engine pixel as is
-    int pad;
+    // 1: f_ab and f_ba are the solver's flows before its final negation (flow.rs:1305), which the
+    // warps take into account by subtracting the scaled flow sample from the pixel position
+    int flow_raw;
     // Weighted Blend on the vector path (kind 3, compose/vector.rs:268-273, filters.rs:649-656):
     // the warped frames, two buffers per change mask (ping-pong of the clean-up passes), the
     // difference-mask limit, the sharpening weights of A and B and the blur's vertical tap order.
@@ -241,8 +243,11 @@ struct PlaneT {
 struct SolveBucket {
     int j0, j1, f0, f1;
 };
+// raw: the flows stay as the last level leaves them, without the final negation (their only
+// readers are inverse warps that know it); the NaN scan then runs in the last Gaussian pass.
 void launch_solve(hipStream_t s, const SolveJob* d_jobs, const std::vector<SolveJob>& jobs, const PlaneT* d_planes,
-                  const std::vector<PlaneT>& planes, const std::vector<SolveBucket>& buckets, bool skip_refine);
+                  const std::vector<PlaneT>& planes, const std::vector<SolveBucket>& buckets, bool skip_refine,
+                  bool raw);
 void launch_frames(hipStream_t s, const FrameJob* d_jobs, const std::vector<FrameJob>& jobs);
 // The Weighted Blend jobs (kinds 3 and 6) after launch_frames: mask clean-up, sharpening, then
 // blend and finish (kind 3) or the Smart Blend composite into the frame's image (kind 6).

@@ -614,7 +614,8 @@ __device__ __forceinline__ int xcd_tile(int n) {
 // the separate passes, so the bits are the same.
 template <int C, int R, int MODE, int TY, int PUV>
 __global__ __launch_bounds__(TILE_X * TY / TILE_RUN) void k_tile_gauss(const SolveJob* __restrict__ jobs, int level, int src,
-                                                             int dst, Taps t, int tiles_x, int tiles_y, int njobs) {
+                                                             int dst, Taps t, int tiles_x, int tiles_y, int njobs,
+                                                             int scan) {
     const int per_job = tiles_x * tiles_y;
     const int lt = xcd_tile(per_job * njobs);
     if (lt >= per_job * njobs) return;
@@ -762,21 +763,33 @@ __global__ __launch_bounds__(TILE_X * TY / TILE_RUN) void k_tile_gauss(const Sol
             for (int d = 0; d < TILE_RUN + 2 * R; d++) win[d] = col[(ry0 + d) * RW * C + c];
             run_interior<R>(win, t, o[c]);
         }
+        bool nan = false;
 #pragma unroll
         for (int m = 0; m < TILE_RUN; m++) {
             float* p = out + ((long)(ys + m) * w + x) * C;
 #pragma unroll
-            for (int c = 0; c < C; c++) p[c] = o[c][m];
+            for (int c = 0; c < C; c++) {
+                p[c] = o[c][m];
+                nan = nan || isnan(o[c][m]);
+            }
         }
+        // scan: the last pass of a solve whose flow is not negated afterwards also takes over the
+        // negation pass's NaN scan (device/solver/mod.rs:575-582)
+        if (MODE == P_NEWDISP && scan && nan) atomicOr(&J.state[2], 1);
         return;
     }
+    bool nan = false;
 #pragma unroll
     for (int m = 0; m < TILE_RUN; m++) {
         if (ys + m >= h) break;
 #pragma unroll
-        for (int c = 0; c < C; c++)
-            out[((long)(ys + m) * w + x) * C + c] = line_elem<R, true, MULTI>(col + c, RW * C, y0 - R, h, ys + m, t);
+        for (int c = 0; c < C; c++) {
+            const float v = line_elem<R, true, MULTI>(col + c, RW * C, y0 - R, h, ys + m, t);
+            out[((long)(ys + m) * w + x) * C + c] = v;
+            nan = nan || isnan(v);
+        }
     }
+    if (MODE == P_NEWDISP && scan && nan) atomicOr(&J.state[2], 1);
 }
 
 // The separable pass of one TILE_X x TILE_Y tile of a one-channel plane (conv.rs:144-199,
@@ -963,6 +976,8 @@ __global__ void k_range_apply(const SolveJob* __restrict__ jobs, int level, int 
     const bool zero = J.state[0] != 0 && J.state[1] == 0;
     const bool damp = level == J.start && J.scale_damp;
     if (!zero && !damp) return;  // the flow stays as it is
+    // a zeroed level 0 holds no NaN: clear what a scan in its last pass found
+    if (zero && level == 0 && blockIdx.x == 0 && threadIdx.x == 0) J.state[2] = 0;
     const int w = J.w[level], h = J.h[level];
     float* flow = J.flow[buf];
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < w * h; i += gridDim.x * blockDim.x) {
@@ -1456,23 +1471,24 @@ int tile_force(int v) {
 }
 
 template <int C, int R, int MODE, int TY, int PU>
-static void tile_launch_v(hipStream_t s, const SolveJob* d_jobs, int l, int src, int dst, const Taps& t, int mw, int mh, int nj) {
+static void tile_launch_v(hipStream_t s, const SolveJob* d_jobs, int l, int src, int dst, const Taps& t, int mw, int mh, int nj,
+                          int scan) {
     const int tx = (mw + TILE_X - 1) / TILE_X, ty = (mh + TY - 1) / TY;
     const long ntile = (long)tx * ty * nj;
     k_tile_gauss<C, R, MODE, TY, PU><<<dim3((unsigned)(8 * ((ntile + 7) / 8))), dim3(TILE_X * TY / TILE_RUN), 0, s>>>(
-        d_jobs, l, src, dst, t, tx, ty, nj);
+        d_jobs, l, src, dst, t, tx, ty, nj, scan);
 }
 
 template <int C, int R, int MODE>
 static void tile_launch_idx(int v, hipStream_t s, const SolveJob* d_jobs, int l, int src, int dst, const Taps& t, int mw,
-                            int mh, int nj) {
+                            int mh, int nj, int scan) {
     switch (v) {
-        case 1: tile_launch_v<C, R, MODE, 32, 1>(s, d_jobs, l, src, dst, t, mw, mh, nj); break;
-        case 2: tile_launch_v<C, R, MODE, 16, 4>(s, d_jobs, l, src, dst, t, mw, mh, nj); break;
-        case 3: tile_launch_v<C, R, MODE, 16, 1>(s, d_jobs, l, src, dst, t, mw, mh, nj); break;
-        case 4: tile_launch_v<C, R, MODE, 64, 4>(s, d_jobs, l, src, dst, t, mw, mh, nj); break;
-        case 5: tile_launch_v<C, R, MODE, 64, 1>(s, d_jobs, l, src, dst, t, mw, mh, nj); break;
-        default: tile_launch_v<C, R, MODE, 32, 4>(s, d_jobs, l, src, dst, t, mw, mh, nj); break;
+        case 1: tile_launch_v<C, R, MODE, 32, 1>(s, d_jobs, l, src, dst, t, mw, mh, nj, scan); break;
+        case 2: tile_launch_v<C, R, MODE, 16, 4>(s, d_jobs, l, src, dst, t, mw, mh, nj, scan); break;
+        case 3: tile_launch_v<C, R, MODE, 16, 1>(s, d_jobs, l, src, dst, t, mw, mh, nj, scan); break;
+        case 4: tile_launch_v<C, R, MODE, 64, 4>(s, d_jobs, l, src, dst, t, mw, mh, nj, scan); break;
+        case 5: tile_launch_v<C, R, MODE, 64, 1>(s, d_jobs, l, src, dst, t, mw, mh, nj, scan); break;
+        default: tile_launch_v<C, R, MODE, 32, 4>(s, d_jobs, l, src, dst, t, mw, mh, nj, scan); break;
     }
 }
 
@@ -1484,10 +1500,10 @@ static int ceil_log2(long v) {
 
 template <int C, int R, int MODE>
 static void tile_launch(hipStream_t s, const SolveJob* d_jobs, int l, int src, int dst, const Taps& t, int mw, int mh,
-                        int nj) {
+                        int nj, int scan = 0) {
     const int forced = g_tile_force.load();
     if (forced >= 0) {
-        tile_launch_idx<C, R, MODE>(forced, s, d_jobs, l, src, dst, t, mw, mh, nj);
+        tile_launch_idx<C, R, MODE>(forced, s, d_jobs, l, src, dst, t, mw, mh, nj, scan);
         return;
     }
     const auto key = std::make_tuple(MODE, ceil_log2((long)mw * mh), ceil_log2(nj));
@@ -1506,7 +1522,7 @@ static void tile_launch(hipStream_t s, const SolveJob* d_jobs, int l, int src, i
         for (auto& e : ev) (void)hipEventCreate(&e);
         (void)hipEventRecord(ev[0], s);
         for (int i = 0; i < NL; i++) {
-            tile_launch_idx<C, R, MODE>(i % TILE_NVAR, s, d_jobs, l, src, dst, t, mw, mh, nj);
+            tile_launch_idx<C, R, MODE>(i % TILE_NVAR, s, d_jobs, l, src, dst, t, mw, mh, nj, scan);
             (void)hipEventRecord(ev[i + 1], s);
         }
         (void)hipEventSynchronize(ev[NL]);
@@ -1524,7 +1540,7 @@ static void tile_launch(hipStream_t s, const SolveJob* d_jobs, int l, int src, i
     }
     // with MODE new_disp the default keeps one pixel's loads in flight (its producer is heavier)
     if (choice < 0) choice = MODE == P_NEWDISP ? 1 : 0;
-    tile_launch_idx<C, R, MODE>(choice, s, d_jobs, l, src, dst, t, mw, mh, nj);
+    tile_launch_idx<C, R, MODE>(choice, s, d_jobs, l, src, dst, t, mw, mh, nj, scan);
 }
 
 int tile_tuning_report(int* out, int cap) {
@@ -1545,7 +1561,7 @@ int tile_tuning_report(int* out, int cap) {
 
 
 static void launch_solve_range(hipStream_t s, const SolveJob* d_jobs, const std::vector<SolveJob>& jobs,
-                               const PlaneT* d_planes, const std::vector<PlaneT>& planes, bool skip_refine) {
+                               const PlaneT* d_planes, const std::vector<PlaneT>& planes, bool skip_refine, bool raw) {
     int nj = (int)jobs.size();
     if (nj == 0) return;
     int max_start = 0;
@@ -1619,8 +1635,10 @@ static void launch_solve_range(hipStream_t s, const SolveJob* d_jobs, const std:
             const int src = it & 1, dst = src ^ 1;
             // calc_b + B blur: flow 8 + X 4 + Y 4 + grad 8 in, B 8 out
             TWX_LAUNCH(s, F_CALC_B, 32.0 * P, tile_launch<2, 7, P_CALCB>(s, d_jobs, l, src, 0, g7, mw, mh, nj));
-            // new_disp + flow blur: tensor 12 + B 8 + flow 8 in, flow 8 out
-            TWX_LAUNCH(s, F_NEW_DISP, 36.0 * P, tile_launch<2, 5, P_NEWDISP>(s, d_jobs, l, src, dst, g5, mw, mh, nj));
+            // new_disp + flow blur: tensor 12 + B 8 + flow 8 in, flow 8 out; a raw solve's last pass scans
+            // for NaNs
+            const int scan = raw && l == 0 && it == 2 ? 1 : 0;
+            TWX_LAUNCH(s, F_NEW_DISP, 36.0 * P, tile_launch<2, 5, P_NEWDISP>(s, d_jobs, l, src, dst, g5, mw, mh, nj, scan));
             if (it == 1 && l == 0 && !skip_refine) bm(l, mw, mh, P, 0);
         }
         if (l != 0 && !skip_refine) bm(l, mw, mh, P, 1);
@@ -1641,16 +1659,19 @@ static void launch_solve_range(hipStream_t s, const SolveJob* d_jobs, const std:
     int w0 = 0, h0 = 0;
     double P0 = 0;
     for (auto& j : jobs) { w0 = std::max(w0, j.w[0]); h0 = std::max(h0, j.h[0]); P0 += (double)j.w[0] * j.h[0]; }
+    if (raw) return;
     // about 8 float4 per lane over the largest job
     const long nb = std::max<long>(1, ((long)w0 * h0 / 2 + 256 * 8 - 1) / (256 * 8));
     TWX_LAUNCH(s, F_NEGATE, 16.0 * P0, k_negate<<<dim3((unsigned)nb, 1, nj), dim3(256), 0, s>>>(d_jobs));
 }
 
 void launch_solve(hipStream_t s, const SolveJob* d_jobs, const std::vector<SolveJob>& jobs, const PlaneT* d_planes,
-                  const std::vector<PlaneT>& planes, const std::vector<SolveBucket>& buckets, bool skip_refine) {
+                  const std::vector<PlaneT>& planes, const std::vector<SolveBucket>& buckets, bool skip_refine,
+                  bool raw) {
     for (const SolveBucket& b : buckets)
         launch_solve_range(s, d_jobs + b.j0, std::vector<SolveJob>(jobs.begin() + b.j0, jobs.begin() + b.j1),
-                           d_planes + b.f0, std::vector<PlaneT>(planes.begin() + b.f0, planes.begin() + b.f1), skip_refine);
+                           d_planes + b.f0, std::vector<PlaneT>(planes.begin() + b.f0, planes.begin() + b.f1), skip_refine,
+                           raw);
 }
 
 __global__ void k_nan_scan(const float* __restrict__ d, size_t n, int* flag) {
@@ -1728,7 +1749,7 @@ __device__ __forceinline__ void load4(const void* img, uint32_t i, float o[4]) {
 // rx = (fw - 1) / (W - 1), ry = (fh - 1) / (H - 1) and sx = amount / rx, sy = amount / ry are the
 // same for every pixel of a frame: the host computes them once (frame_job_ratios).
 __device__ __forceinline__ void warp_point(const float* __restrict__ flow, int fw, int fh, float rx, float ry, float sx,
-                                           float sy, int x, int y, float& ix, float& iy) {
+                                           float sy, int x, int y, float& ix, float& iy, bool raw) {
     float xf = (float)x, yf = (float)y;
     float qx = xf * rx, qy = yf * ry;
     int fw1 = fw - 1, fh1 = fh - 1;
@@ -1751,8 +1772,11 @@ __device__ __forceinline__ void warp_point(const float* __restrict__ flow, int f
         float bot = (p11 - p01) * gx + p01;
         d[k] = (bot - top) * gy + top;
     }
-    ix = d[0] * sx + xf;
-    iy = d[1] * sy + yf;
+    // raw: the flow holds v where the reference reads -1 * v. Negation is exact, so the reference's
+    // sample of the negated flow is -d except that an exactly zero result may carry the other sign,
+    // which vanishes in the sum with the pixel position; xf - d * sx is xf + (-d) * sx bit for bit.
+    ix = raw ? xf - d[0] * sx : d[0] * sx + xf;
+    iy = raw ? yf - d[1] * sy : d[1] * sy + yf;
 }
 
 // bilerp4 (warp.rs:306-331) at a point inside [0, W-1] x [0, H-1], before the store rule, in two
@@ -1803,7 +1827,7 @@ template <int FMT>
 __device__ __forceinline__ bool warp_pixel(const void* src, int W, int H, const float* __restrict__ flow, int fw, int fh,
                                            const FrameJob& F, int wi, int x, int y, float o[4]) {
     float ix, iy;
-    warp_point(flow, fw, fh, F.rx, F.ry, F.sx[wi], F.sy[wi], x, y, ix, iy);
+    warp_point(flow, fw, fh, F.rx, F.ry, F.sx[wi], F.sy[wi], x, y, ix, iy, F.flow_raw != 0);
     if (!clip_to_rect((float)x, (float)y, ix, iy, (float)(W - 1), (float)(H - 1)) || isnan(ix) || isnan(iy)) {
         o[0] = o[1] = o[2] = o[3] = 0.0f;
         return false;
@@ -1823,8 +1847,8 @@ __device__ __forceinline__ bool warp_pixel(const void* src, int W, int H, const 
 template <int FMT>
 __device__ __forceinline__ bool warp_pixel_pair(const FrameJob& F, int x, int y, float pa[4], float pb[4]) {
     float ixa, iya, ixb, iyb;
-    warp_point(F.f_ba, F.fw, F.fh, F.rx, F.ry, F.sx[0], F.sy[0], x, y, ixa, iya);
-    warp_point(F.f_ab, F.fw, F.fh, F.rx, F.ry, F.sx[1], F.sy[1], x, y, ixb, iyb);
+    warp_point(F.f_ba, F.fw, F.fh, F.rx, F.ry, F.sx[0], F.sy[0], x, y, ixa, iya, F.flow_raw != 0);
+    warp_point(F.f_ab, F.fw, F.fh, F.rx, F.ry, F.sx[1], F.sy[1], x, y, ixb, iyb, F.flow_raw != 0);
     const float w1 = (float)(F.W - 1), h1 = (float)(F.H - 1);
     const bool oka = clip_to_rect((float)x, (float)y, ixa, iya, w1, h1) && !isnan(ixa) && !isnan(iya);
     const bool okb = clip_to_rect((float)x, (float)y, ixb, iyb, w1, h1) && !isnan(ixb) && !isnan(iyb);
@@ -1863,7 +1887,7 @@ template <int FMT>
 __device__ __forceinline__ bool warp_fade_pixel(const void* src, int W, int H, const float* __restrict__ flow, int fw,
                                                 int fh, const FrameJob& F, int wi, int x, int y, float o[4], int& cov) {
     float ix, iy;
-    warp_point(flow, fw, fh, F.rx, F.ry, F.sx[wi], F.sy[wi], x, y, ix, iy);
+    warp_point(flow, fw, fh, F.rx, F.ry, F.sx[wi], F.sy[wi], x, y, ix, iy, F.flow_raw != 0);
     const float wf = (float)W, hf = (float)H, w1 = (float)(W - 1), h1 = (float)(H - 1);
     o[0] = o[1] = o[2] = o[3] = 0.0f;
     if (!(ix > -1.0f && ix < wf && iy > -1.0f && iy < hf)) {
