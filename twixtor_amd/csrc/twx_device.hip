@@ -953,49 +953,46 @@ void launch_blur5_resample(hipStream_t s, const PlaneOp* d_ops, const std::vecto
     if (x >= w || y >= h) return;
 
 // all_flow_out_of_range (flow.rs:431-442): any pixel with 0 <= x+u <= w-1 and 0 <= y+v <= h-1
-__global__ void k_range_any(const SolveJob* __restrict__ jobs, int level, int buf) {
-    const SolveJob& J = jobs[blockIdx.z];
-    if (level > J.start || J.state[0] == 0) return;  // the check ran out at an earlier level
-    const int w = J.w[level], h = J.h[level];
-    const float* flow = J.flow[buf];
-    // a grid-stride pass: few blocks, since after the first level that finds a pixel in range the
-    // launch does nothing
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < w * h; i += gridDim.x * blockDim.x) {
-        const int y = i / w, x = i - y * w;
-        float px = (float)x + flow[(long)i * 2];
-        float py = (float)y + flow[(long)i * 2 + 1];
-        bool in = 0.0f <= px && px <= (float)(w - 1) && 0.0f <= py && py <= (float)(h - 1);
-        if (in) atomicOr(&J.state[1], 1);
-    }
-}
-
-// the range check result, then the damping at the start level (flow.rs:1291-1300)
-__global__ void k_range_apply(const SolveJob* __restrict__ jobs, int level, int buf) {
-    const SolveJob& J = jobs[blockIdx.z];
+// The range check of a level (all_flow_out_of_range while no coarser level found a pixel in
+// range, flow.rs:1291-1297) and the damping at the start level (flow.rs:1298-1300) in one launch:
+// one 1024-lane block per job, which reduces "any pixel in range" over the level itself, zeroes
+// the flow when there is none (or ends the check when there is one), then damps. A job with the
+// check over and no damping due returns at once, so a launch past the start level costs one wave
+// per job.
+__global__ __launch_bounds__(1024) void k_range(const SolveJob* __restrict__ jobs, int level, int buf) {
+    const SolveJob& J = jobs[blockIdx.x];
     if (level > J.start) return;
-    const bool zero = J.state[0] != 0 && J.state[1] == 0;
+    const bool active = J.state[0] != 0;
     const bool damp = level == J.start && J.scale_damp;
-    if (!zero && !damp) return;  // the flow stays as it is
-    // a zeroed level 0 holds no NaN: clear what a scan in its last pass found
-    if (zero && level == 0 && blockIdx.x == 0 && threadIdx.x == 0) J.state[2] = 0;
+    if (!active && !damp) return;  // the flow stays as it is (uniform over the block)
     const int w = J.w[level], h = J.h[level];
     float* flow = J.flow[buf];
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < w * h; i += gridDim.x * blockDim.x) {
+    bool zero = false;
+    if (active) {
+        int any = 0;
+        for (int i = threadIdx.x; i < w * h && !any; i += blockDim.x) {
+            const int y = i / w, x = i - y * w;
+            const float px = (float)x + flow[(long)i * 2];
+            const float py = (float)y + flow[(long)i * 2 + 1];
+            any = 0.0f <= px && px <= (float)(w - 1) && 0.0f <= py && py <= (float)(h - 1);
+        }
+        any = __syncthreads_or(any);
+        zero = !any;
+        if (threadIdx.x == 0) {
+            if (any) J.state[0] = 0;
+            J.state[1] = 0;
+            // a zeroed level 0 holds no NaN: clear what a scan in its last pass found
+            if (zero && level == 0) J.state[2] = 0;
+        }
+    }
+    if (!zero && !damp) return;
+    for (int i = threadIdx.x; i < w * h; i += blockDim.x) {
         float u = flow[(long)i * 2], v = flow[(long)i * 2 + 1];
         if (zero) { u = 0.0f; v = 0.0f; }
         if (damp) { u = J.damp * u; v = J.damp * v; }
         flow[(long)i * 2] = u;
         flow[(long)i * 2 + 1] = v;
     }
-}
-
-__global__ void k_range_update(const SolveJob* __restrict__ jobs, int njobs, int level) {
-    int j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= njobs) return;
-    const SolveJob& J = jobs[j];
-    if (level > J.start) return;
-    if (J.state[0] != 0 && J.state[1] != 0) J.state[0] = 0;
-    J.state[1] = 0;
 }
 
 // upsample_flow (flow.rs:376-391) from level l (buffer 1) into level l-1 (buffer 0)
@@ -1644,10 +1641,7 @@ static void launch_solve_range(hipStream_t s, const SolveJob* d_jobs, const std:
         if (l != 0 && !skip_refine) bm(l, mw, mh, P, 1);
         // the check reads the flow while no level has found a pixel in range, and the flow is only
         // rewritten when it is zeroed or damped: in practice both at the start level alone
-        const dim3 rgrd((unsigned)std::min<long>(64, ((long)mw * mh + 255) / 256), 1, nj);
-        TWX_LAUNCH(s, F_RANGE, 8.0 * Ps, k_range_any<<<rgrd, blk, 0, s>>>(d_jobs, l, 1));
-        TWX_LAUNCH(s, F_RANGE, 16.0 * Ps, k_range_apply<<<rgrd, blk, 0, s>>>(d_jobs, l, 1));
-        TWX_LAUNCH(s, F_RANGE, 0.0, k_range_update<<<dim3((nj + 63) / 64), dim3(64), 0, s>>>(d_jobs, nj, l));
+        TWX_LAUNCH(s, F_RANGE, 24.0 * Ps, k_range<<<dim3((unsigned)nj), dim3(1024), 0, s>>>(d_jobs, l, 1));
         if (l > 0) {
             int uw = 0, uh = 0;
             for (auto& j : jobs)
