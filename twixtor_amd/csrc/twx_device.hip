@@ -1180,9 +1180,6 @@ struct BmFlow {
     }
 };
 
-// Transposed copies for the row sweeps, through 32 x 32 tiles in LDS: X and Y (level `level`)
-// into tb's two planes and, with FLOW, the flow (buffer `buf`) into the scratch; OUT copies the
-// scratch back into the flow. A flow pixel (u, v) moves as one 8-byte access.
 // The transposed copy of a frame's level for the row sweeps (element (x, y) at x * h + y): 32 x 32
 // tiles through LDS. One plane per distinct frame of the launch (blockIdx.z), which every job with
 // the frame as X or Y reads.
