@@ -41,7 +41,7 @@ VALU_SHARE = {"calc_b_gauss": 0.71, "new_disp_gauss": 0.58, "tensor_gauss": 0.71
 STEP_MODEL_FACTOR = {"tensor_gauss": 72.0 / 24.0, "calc_b_gauss": 64.0 / 32.0, "new_disp_gauss": 68.0 / 36.0,
                      "bm_transpose": 0.0}
 FLOW_FAMILIES = ("pyr_luma_blur5_band_resample", "tensor_gauss", "calc_b_gauss", "new_disp_gauss", "bm_rows", "bm_cols",
-                 "bm_transpose", "range", "upsample", "negate", "solve_init")
+                 "bm_transpose", "range", "upsample", "negate", "solve_init", "coarse_solve")
 WARP_FAMILIES = ("frames",)
 STAGE_TARGET = 0.60  # of the HBM peak, for the flow and the warp stages
 

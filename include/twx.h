@@ -391,6 +391,15 @@ int twx_set_graphs(twx_handle* h, int32_t on);
 int twx_set_tile_shape(int32_t variant);
 int twx_tile_choices(int32_t* out, int32_t cap);
 
+/* The coarse solver (no counterpart in the reference, whose solve runs level by level on the CPU):
+ * the levels of a solve from its start level down while a level holds at most 2560 pixels run in
+ * one persistent kernel per launch, whose workgroups take the jobs from a device work queue and
+ * keep each job's planes in LDS, instead of one launch per step and level. On by default;
+ * twx_set_coarse_solver(0) runs every level through the per-level kernels, with the same bits.
+ * Process-wide; applies to solves enqueued afterwards (a captured graph replays the launches it
+ * captured). */
+int twx_set_coarse_solver(int32_t on);
+
 /* Device pointer of the handle's HIP stream (hipStream_t), so a caller can order its own work. */
 void* twx_stream(twx_handle* h);
 

@@ -21,7 +21,7 @@ EXPORTS = [
     "twx_forward_warp", "twx_encode_vectors", "twx_decode_vectors", "twx_flow_from_vectors", "twx_invert_flow",
     "twx_create_vectors_dev", "twx_warp_blend", "twx_warp_blend_backward", "twx_warp_blend_backward_noalloc", "twx_refine_flows",
     "twx_vectors_in_frames", "twx_vectors_in_frames_dev",
-    "twx_set_graphs", "twx_set_tile_shape", "twx_tile_choices", "twx_get_stats", "twx_cache_stats", "twx_stream", "twx_profile_enable", "twx_profile_reset", "twx_profile_read",
+    "twx_set_graphs", "twx_set_tile_shape", "twx_tile_choices", "twx_set_coarse_solver", "twx_get_stats", "twx_cache_stats", "twx_stream", "twx_profile_enable", "twx_profile_reset", "twx_profile_read",
 ]
 
 
@@ -122,6 +122,7 @@ def lib():
         L.twx_set_graphs.argtypes = [P, I32]
         L.twx_set_tile_shape.argtypes = [I32]
         L.twx_tile_choices.argtypes = [PI32, I32]
+        L.twx_set_coarse_solver.argtypes = [I32]
         L.twx_get_stats.argtypes = [P, ctypes.POINTER(Stats)]
         PU64 = ctypes.POINTER(ctypes.c_uint64)
         L.twx_cache_stats.argtypes = [P, PU64, PU64, PU64, PU64]

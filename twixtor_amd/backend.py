@@ -123,6 +123,11 @@ class HipBackend:
             raise _ffi.StageError(st, f"tile variant {variant} outside -1..5")
 
     @staticmethod
+    def coarse_solver(on: bool = True):
+        """The persistent coarse-level solver on or off for the process (twx_set_coarse_solver)."""
+        _ffi.lib().twx_set_coarse_solver(1 if on else 0)
+
+    @staticmethod
     def tile_choices() -> list:
         """The autotuner's choices: (kernel, log2 level area, log2 job count, variant) per class."""
         n = _ffi.lib().twx_tile_choices(None, 0)

@@ -468,6 +468,15 @@ struct SolveIn {
     float* out;      // w0 * h0 * 2 floats, the flow X(p) ~ Y(p + out(p))
 };
 
+// The coarsest level the per-level kernels run for a solve of these level sizes: the levels above
+// it, from the start level down while a level fits in LDS, run in the coarse solver kernel.
+static int coarse_top(const int* w, const int* h, int start) {
+    if (!coarse_solver_enabled()) return start;
+    int l = start;
+    while (l >= 0 && (long)w[l] * h[l] <= COARSE_MAX_PIXELS) l--;
+    return l;
+}
+
 // Enqueues every solve, in chunks that fit the scratch budget, with no host synchronisation:
 // solve k keeps its state in states[4k .. 4k + 3] (device; [2] is set for a NaN in its flow).
 int enqueue_solves(hipStream_t s, DevPool& pool, const std::vector<SolveIn>& in, const twx_flow_settings& fs,
@@ -508,7 +517,7 @@ int enqueue_solves(hipStream_t s, DevPool& pool, const std::vector<SolveIn>& in,
         std::vector<const float*> frames;      // the bucket's distinct frames, bucket after bucket
         std::vector<int> frame_geo;            // a job (index into `in` of the chunk) with the frame
         for (auto [b0, b1] : size_buckets(areas)) {
-            SolveBucket B{(int)order.size(), 0, (int)frames.size(), 0};
+            SolveBucket B{(int)order.size(), 0, (int)frames.size(), 0, nullptr};
             std::map<const float*, int> first;
             std::set<const float*> seen;
             std::vector<int> shared;
@@ -549,7 +558,9 @@ int enqueue_solves(hipStream_t s, DevPool& pool, const std::vector<SolveIn>& in,
                 lv += (size_t)si.geo.w[l] * si.geo.h[l];
             }
             float damp;
-            start_level(fs.sensitivity, si.geo.levels, P.start, damp);
+            int start;
+            start_level(fs.sensitivity, si.geo.levels, start, damp);
+            P.top = coarse_top(P.w, P.h, start);
             P.dst = pool.alloc<float>((size_t)si.geo.w0 * si.geo.h0);
             tplane[frames[f]] = P.dst;
         }
@@ -572,6 +583,7 @@ int enqueue_solves(hipStream_t s, DevPool& pool, const std::vector<SolveIn>& in,
             }
             J.levels = si.geo.levels;
             start_level(fs.sensitivity, si.geo.levels, J.start, J.damp);
+            J.top = coarse_top(J.w, J.h, J.start);
             J.scale_damp = (std::fabs((double)J.damp - 1.0) >= 1e-6 && J.damp != 1.0f) ? 1 : 0;
             size_t px = (size_t)si.geo.w0 * si.geo.h0;
             J.flow[0] = sp;      // where each level starts (the upsample target)
@@ -593,6 +605,8 @@ int enqueue_solves(hipStream_t s, DevPool& pool, const std::vector<SolveIn>& in,
             J.max_h = si.geo.h0;
             solve_job_ratios(J);
         }
+        int* queues = pool.alloc<int>(buckets.size());  // zeroed by each bucket's init kernel
+        for (size_t b = 0; b < buckets.size(); b++) buckets[b].queue = queues + b;
         SolveJob* d = pool.upload(jobs);
         PlaneT* dp = pool.upload(planes);
         launch_solve(s, d, jobs, dp, planes, buckets, fs.skip_refine != 0, raw);  // starts with the per-job init kernel
@@ -2666,6 +2680,11 @@ int twx_refine_flows(twx_handle* h, int32_t channels, int32_t width, int32_t hei
 int twx_set_tile_shape(int32_t variant) { return tile_force(variant) == 0 ? TWX_OK : TWX_E_MISMATCH; }
 
 int twx_tile_choices(int32_t* out, int32_t cap) { return tile_tuning_report(out, out ? cap : 0); }
+
+int twx_set_coarse_solver(int32_t on) {
+    coarse_solver_enable(on != 0);
+    return TWX_OK;
+}
 
 int twx_set_graphs(twx_handle* h, int32_t on) {
     std::string e;

@@ -60,6 +60,10 @@ struct SolveJob {
     int w[MAXL], h[MAXL];
     int levels;
     int start;             // start level (flow.rs:179-197)
+    // the coarsest level the per-level kernels run; the levels above it (top + 1 .. start, whose
+    // planes fit in LDS) run in the coarse solver kernel, which hands level top its flow in flow[0].
+    // top = start: no coarse part; top = -1: the whole solve runs there
+    int top;
     float damp;
     int scale_damp;        // |damp - 1| >= 1e-6 and damp != 1
     float* flow[2];        // flow, double buffered across the upsample: level l lives in flow[l & 1]
@@ -177,7 +181,7 @@ enum ProfFam {
     F_SOLVE_INIT, F_CALC_B, F_NEW_DISP, F_BM_ROWS, F_BM_COLS, F_RANGE, F_UPSAMPLE, F_NEGATE, F_FRAMES,
     F_WB_MASK, F_WB_SHARPEN, F_FILL_PULL, F_FILL_PUSH, F_FILL_IO, F_DIFF_FWD, F_DIFF_BWD, F_DIFF_GATHER,
     F_DIFF_IMG, F_DIFF_STEP, F_DIFF_ADAM,
-    F_FW_MESH, F_FW_RASTER, F_FW_SCAN, F_FW_FOLD, F_VEC, F_NFAM
+    F_FW_MESH, F_FW_RASTER, F_FW_SCAN, F_FW_FOLD, F_VEC, F_COARSE, F_NFAM
 };
 extern const char* const PROF_NAMES[F_NFAM];
 
@@ -236,13 +240,20 @@ struct PlaneT {
     const float* lv[MAXL];
     float* dst;
     int w[MAXL], h[MAXL];
-    int start;
+    int top;  // the coarsest level its jobs' per-level kernels run (SolveJob::top)
 };
 // A size bucket of a solve launch: jobs [j0, j1) and the frames [f0, f1) they read. The jobs that
-// compute a gradient and tensor (tensor_shared 0) lead the bucket's jobs.
+// compute a gradient and tensor (tensor_shared 0) lead the bucket's jobs. queue: a device counter
+// for the coarse solver's work queue.
 struct SolveBucket {
     int j0, j1, f0, f1;
+    int* queue;
 };
+// The largest level (in pixels) the coarse solver kernel holds in LDS: 15 floats per pixel.
+constexpr int COARSE_MAX_PIXELS = 2560;
+// the coarse solver on or off for the solves enqueued from now on (on by default)
+void coarse_solver_enable(bool on);
+bool coarse_solver_enabled();
 // raw: the flows stay as the last level leaves them, without the final negation (their only
 // readers are inverse warps that know it); the NaN scan then runs in the last Gaussian pass.
 void launch_solve(hipStream_t s, const SolveJob* d_jobs, const std::vector<SolveJob>& jobs, const PlaneT* d_planes,

@@ -176,7 +176,7 @@ const char* const PROF_NAMES[F_NFAM] = {"ingest", "pyr_luma_blur5_band_resample"
                                         "wb_mask", "wb_sharpen_blend", "fill_pull", "fill_push", "fill_io",
                                         "warp_blend_fwd", "warp_blend_bwd", "warp_blend_flow_gather",
                                         "warp_blend_image_grad", "refine_step", "refine_adam",
-                                        "fw_mesh", "fw_raster", "fw_scan", "fw_fold", "vector_coding"};
+                                        "fw_mesh", "fw_raster", "fw_scan", "fw_fold", "vector_coding", "coarse_solve"};
 thread_local Prof* g_prof = nullptr;
 
 hipEvent_t Prof::take() {
@@ -622,7 +622,7 @@ __global__ __launch_bounds__(TILE_X * TY / TILE_RUN) void k_tile_gauss(const Sol
     const int job = lt / per_job, rem = lt - job * per_job;
     const int tyi = rem / tiles_x, txi = rem - tyi * tiles_x;
     const SolveJob& J = jobs[job];
-    if (level > J.start || (MODE == P_TENSOR && J.tensor_shared)) return;
+    if (level > J.top || (MODE == P_TENSOR && J.tensor_shared)) return;
     const int w = J.w[level], h = J.h[level];
     const int x0 = txi * TILE_X, y0 = tyi * TY;
     constexpr int NT = TILE_X * TY / TILE_RUN;  // one vertical run of TILE_RUN rows per lane
@@ -944,15 +944,6 @@ void launch_blur5_resample(hipStream_t s, const PlaneOp* d_ops, const std::vecto
                                                                                                    (int)ops.size()));
 }
 
-#define JOB_PROLOGUE()                                                    \
-    const SolveJob& J = jobs[blockIdx.z];                                 \
-    if (level > J.start) return;                                          \
-    const int w = J.w[level], h = J.h[level];                             \
-    const int x = blockIdx.x * blockDim.x + threadIdx.x;                  \
-    const int y = blockIdx.y;                                             \
-    if (x >= w || y >= h) return;
-
-// all_flow_out_of_range (flow.rs:431-442): any pixel with 0 <= x+u <= w-1 and 0 <= y+v <= h-1
 // The range check of a level (all_flow_out_of_range while no coarser level found a pixel in
 // range, flow.rs:1291-1297) and the damping at the start level (flow.rs:1298-1300) in one launch:
 // one 1024-lane block per job, which reduces "any pixel in range" over the level itself, zeroes
@@ -961,7 +952,7 @@ void launch_blur5_resample(hipStream_t s, const PlaneOp* d_ops, const std::vecto
 // per job.
 __global__ __launch_bounds__(1024) void k_range(const SolveJob* __restrict__ jobs, int level, int buf) {
     const SolveJob& J = jobs[blockIdx.x];
-    if (level > J.start) return;
+    if (level > J.top) return;
     const bool active = J.state[0] != 0;
     const bool damp = level == J.start && J.scale_damp;
     if (!active && !damp) return;  // the flow stays as it is (uniform over the block)
@@ -1002,7 +993,7 @@ __global__ __launch_bounds__(1024) void k_range(const SolveJob* __restrict__ job
 constexpr int UP_ROWS = 4;
 __global__ void k_upsample(const SolveJob* __restrict__ jobs, int level) {
     const SolveJob& J = jobs[blockIdx.z];
-    if (level > J.start || level == 0) return;
+    if (level > J.top || level == 0) return;
     const int sw = J.w[level], sh = J.h[level];
     const int dw = J.w[level - 1], dh = J.h[level - 1];
     const int x = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1185,7 +1176,7 @@ struct BmFlow {
 // the frame as X or Y reads.
 __global__ void k_plane_transpose(const PlaneT* __restrict__ ops, int level) {
     const PlaneT& P = ops[blockIdx.z];
-    if (level > P.start) return;
+    if (level > P.top) return;
     const int w = P.w[level], h = P.h[level];
     if (w < 4 || h < 4) return;  // no sweep runs on such a level
     __shared__ float t[32][33];
@@ -1210,7 +1201,7 @@ __global__ void k_plane_transpose(const PlaneT* __restrict__ ops, int level) {
 template <bool OUT>
 __global__ void k_bm_transpose(const SolveJob* __restrict__ jobs, int level, int buf) {
     const SolveJob& J = jobs[blockIdx.z];
-    if (level > J.start) return;
+    if (level > J.top) return;
     const int w = J.w[level], h = J.h[level];
     if (w < 4 || h < 4) return;
     __shared__ float t[2][32][33];
@@ -1374,7 +1365,7 @@ struct BmChain {
 template <bool T, int DIR, int CH, bool CROSS>
 __global__ __launch_bounds__(64) void k_bm_sweep(const SolveJob* __restrict__ jobs, int level, int buf) {
     const SolveJob& J = jobs[blockIdx.z];
-    if (level > J.start) return;
+    if (level > J.top) return;
     const int w = J.w[level], h = J.h[level];
     // with a side below 4 no cost exists (it needs 1 <= ix <= w - 3), so no pixel changes
     if (w < 4 || h < 4) return;
@@ -1422,12 +1413,295 @@ __global__ __launch_bounds__(64) void k_bm_sweep(const SolveJob* __restrict__ jo
 }
 
 // -------------------------------------------------------------------------------------------
+// The coarse levels of a solve in one persistent kernel (flow.rs:1258-1304 from the start level
+// down to the last level whose planes fit in LDS): a fixed grid of workgroups takes jobs from a
+// device work queue, and each runs its job's coarse levels entirely in LDS (X, Y, the flow, the
+// gradient, the blurred tensor and two three-channel work planes: 15 floats per pixel), with
+// block barriers between the steps. It hands the first level left to the per-level kernels its
+// upsampled flow in flow[0] (or, when every level fits, the finished level 0 in flow[1]). Every
+// element is computed with the operations of the per-level kernels, so the bits do not depend on
+// which path ran a level. One launch replaces about fifteen per level, whose time at these sizes
+// is latency, not work.
+
+constexpr int COARSE_THREADS = 512;
+
+static std::atomic<bool> g_coarse_on{true};
+void coarse_solver_enable(bool on) { g_coarse_on.store(on); }
+bool coarse_solver_enabled() { return g_coarse_on.load(); }
+
+// A C-channel plane's Gaussian (conv.rs:144-199, 272-311) in LDS: the row pass (border columns
+// right first) src -> tmp, then the column pass (border rows above first) tmp -> dst.
+template <int R, int C>
+__device__ __forceinline__ void lds_gauss(const float* src, float* tmp, float* dst, int w, int h, const Taps& t) {
+    constexpr bool MULTI = C > 1;
+    const int n = w * h * C;
+    for (int e = threadIdx.x; e < n; e += COARSE_THREADS) {
+        const int i = e / C, c = e - i * C;
+        const int y = i / w, x = i - y * w;
+        tmp[e] = line_elem<R, false, MULTI>(src + (y * w) * C + c, C, 0, w, x, t);
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < n; e += COARSE_THREADS) {
+        const int i = e / C, c = e - i * C;
+        const int y = i / w, x = i - y * w;
+        dst[e] = line_elem<R, true, MULTI>(tmp + x * C + c, w * C, 0, h, y, t);
+    }
+    __syncthreads();
+}
+
+// Block::cost (flow.rs:472-524) on a plane in LDS; false where the reference returns None.
+__device__ __forceinline__ bool lds_cost(const float* B, int w, int h, const float p[9], float x2, float y2, float& cost) {
+    const float wm2 = (float)(w - 2), hm2 = (float)(h - 2);
+    if (x2 > wm2 || y2 > hm2) return false;
+    int32_t ix, iy;
+    float fx, fy;
+    if (x2 == wm2) { ix = cvtt(wm2 - 1.0f); fx = 1.0f; } else { ix = cvtt(x2); fx = x2 - (float)ix; }
+    if (y2 == hm2) { iy = cvtt(hm2 - 1.0f); fy = 1.0f; } else { iy = cvtt(y2); fy = y2 - (float)iy; }
+    if (ix < 1 || iy < 1) return false;
+    const float w11 = fy * fx;
+    const float w10 = fx - w11;
+    const float w01 = fy - w11;
+    const float w00 = ((1.0f - fx) - fy) + w11;
+    const float* t = B + ((iy - 1) * w + ix - 1);
+    float sq[9];
+#pragma unroll
+    for (int b = 0; b < 3; b++)
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+            const float q = ((w00 * t[b * w + a] + w10 * t[b * w + a + 1]) + w01 * t[(b + 1) * w + a]) +
+                            w11 * t[(b + 1) * w + a + 1];
+            const float d = p[b * 3 + a] - q;
+            sq[b * 3 + a] = d * d;
+        }
+    float s2 = sq[0] + sq[1];
+#pragma unroll
+    for (int k = 2; k < 8; k++) s2 = s2 + sq[k];
+    cost = sq[8] + s2;
+    return true;
+}
+
+// One sweep of block matching (flow.rs:551-614) on the flow F in LDS: the own vectors' costs of
+// every interior pixel first (a pixel holds its own vector until its chain reaches it; -inf where
+// the cost does not exist) into OC, then one lane per chain costing the incoming vector (+inf
+// where it does not exist), which replaces the pixel's vector when it is lower (Block::step,
+// flow.rs:528-544). ROWS: chains are rows (sweeps 1, 2); DIR: the direction along them.
+template <bool ROWS, int DIR>
+__device__ void lds_sweep(const float* X, const float* Y, float* F, float* OC, int w, int h) {
+    auto patch = [&](int x, int y, float p[9]) {
+#pragma unroll
+        for (int b = 0; b < 3; b++)
+#pragma unroll
+            for (int a = 0; a < 3; a++) p[b * 3 + a] = X[(y - 1 + b) * w + x - 1 + a];
+    };
+    for (int e = threadIdx.x; e < w * h; e += COARSE_THREADS) {
+        const int y = e / w, x = e - y * w;
+        if (x < 1 || x > w - 2 || y < 1 || y > h - 2) continue;
+        float p[9], c;
+        patch(x, y, p);
+        OC[e] = lds_cost(Y, w, h, p, (float)x - F[2 * e], (float)y - F[2 * e + 1], c) ? c : -INFINITY;
+    }
+    __syncthreads();
+    const int chains = ROWS ? h - 2 : w - 2, len = ROWS ? w - 2 : h - 2;
+    for (int k = threadIdx.x; k < chains; k += COARSE_THREADS) {
+        const int cc = 1 + k;
+        const int a0 = DIR > 0 ? 1 : len;
+        int e = ROWS ? cc * w + (a0 - DIR) : (a0 - DIR) * w + cc;
+        const int step = ROWS ? DIR : DIR * w;
+        float nu = F[2 * e], nv = F[2 * e + 1];
+        for (int j = 0; j < len; j++) {
+            e += step;
+            const int y = e / w, x = e - y * w;
+            float p[9], c;
+            patch(x, y, p);
+            const float inc = lds_cost(Y, w, h, p, (float)x - nu, (float)y - nv, c) ? c : INFINITY;
+            if (inc < OC[e]) {
+                F[2 * e] = nu;
+                F[2 * e + 1] = nv;
+            } else {
+                nu = F[2 * e];
+                nv = F[2 * e + 1];
+            }
+        }
+    }
+    __syncthreads();
+}
+
+__global__ __launch_bounds__(COARSE_THREADS) void k_coarse_solve(const SolveJob* __restrict__ jobs, int njobs,
+                                                                  int* __restrict__ queue, Taps g7, Taps g5,
+                                                                  int skip_refine) {
+    __shared__ float smem[15 * COARSE_MAX_PIXELS];
+    __shared__ int s_job;
+    for (;;) {
+        if (threadIdx.x == 0) s_job = atomicAdd(queue, 1);
+        __syncthreads();
+        const int job = s_job;
+        __syncthreads();  // every lane has read s_job before lane 0 may take the next one
+        if (job >= njobs) return;
+        const SolveJob& J = jobs[job];
+        if (J.top >= J.start) continue;  // no coarse part
+        float* X = smem;
+        float* Y = X + COARSE_MAX_PIXELS;
+        float* F = Y + COARSE_MAX_PIXELS;
+        float* G = F + 2 * COARSE_MAX_PIXELS;
+        float* TA = G + 2 * COARSE_MAX_PIXELS;
+        float* S1 = TA + 3 * COARSE_MAX_PIXELS;
+        float* S2 = S1 + 3 * COARSE_MAX_PIXELS;
+        bool active = true;  // the range check (k_solve_init started it)
+        {
+            const int n = J.w[J.start] * J.h[J.start] * 2;
+            for (int i = threadIdx.x; i < n; i += COARSE_THREADS) F[i] = 0.0f;
+        }
+        for (int l = J.start; l > J.top; l--) {
+            const int w = J.w[l], h = J.h[l], n = w * h;
+            for (int i = threadIdx.x; i < n; i += COARSE_THREADS) {
+                X[i] = J.X[l][i];
+                Y[i] = J.Y[l][i];
+            }
+            __syncthreads();
+            // calc_grad + calc_a (flow.rs:283-328), the tensor blurred r = 7
+            for (int i = threadIdx.x; i < n; i += COARSE_THREADS) {
+                const int y = i / w, x = i - y * w;
+                const float c = X[i];
+                float nx = 0.0f, dx = 0.0f, ny = 0.0f, dy = 0.0f;
+                if (x >= 1) { nx = (c - X[i - 1]) + 0.0f; dx = 1.0f; }
+                if (y >= 1) { ny = (c - X[i - w]) + 0.0f; dy = 1.0f; }
+                if (x + 1 <= w - 1) { nx = nx + (X[i + 1] - c); dx = dx + 1.0f; }
+                if (y + 1 <= h - 1) { ny = ny + (X[i + w] - c); dy = dy + 1.0f; }
+                const float gx = nx / dx, gy = ny / dy;
+                G[2 * i] = gx;
+                G[2 * i + 1] = gy;
+                S1[3 * i] = gx * gx;
+                S1[3 * i + 1] = gx * gy;
+                S1[3 * i + 2] = gy * gy;
+            }
+            __syncthreads();
+            lds_gauss<7, 3>(S1, S2, TA, w, h, g7);
+            for (int it = 0; it < 3; it++) {
+                // calc_b (flow.rs:334-357), blurred r = 7
+                for (int i = threadIdx.x; i < n; i += COARSE_THREADS) {
+                    const int y = i / w, x = i - y * w;
+                    const float sx = (float)x - F[2 * i], sy = (float)y - F[2 * i + 1];
+                    float bx = 0.0f, by = 0.0f;
+                    if (!(sx < 0.0f || (float)w - 1.0f < sx || sy < 0.0f || (float)h - 1.0f < sy)) {
+                        const float r = bilinear1(Y, w, h, sx, sy) - X[i];
+                        bx = r * G[2 * i];
+                        by = r * G[2 * i + 1];
+                    }
+                    S1[2 * i] = bx;
+                    S1[2 * i + 1] = by;
+                }
+                __syncthreads();
+                lds_gauss<7, 2>(S1, S2, S1, w, h, g7);
+                // new_disp (flow.rs:361-371), the flow blurred r = 5
+                for (int i = threadIdx.x; i < n; i += COARSE_THREADS) {
+                    const float aa = TA[3 * i], ab = TA[3 * i + 1], ad = TA[3 * i + 2];
+                    const float bx = S1[2 * i], by = S1[2 * i + 1];
+                    float u = F[2 * i], v = F[2 * i + 1];
+                    const float t = ab * ab - ad * aa;
+                    if (fabsf(t) > 40.0f) {
+                        u = (ab * by - ad * bx) / t + u;
+                        v = (ab * bx - aa * by) / t + v;
+                    }
+                    S2[2 * i] = u;
+                    S2[2 * i + 1] = v;
+                }
+                __syncthreads();
+                lds_gauss<5, 2>(S2, S1, F, w, h, g5);
+                if (it == 1 && l == 0 && !skip_refine && w >= 4 && h >= 4) {
+                    lds_sweep<true, 1>(X, Y, F, S1, w, h);
+                    lds_sweep<true, -1>(X, Y, F, S1, w, h);
+                    lds_sweep<false, 1>(X, Y, F, S1, w, h);
+                    lds_sweep<false, -1>(X, Y, F, S1, w, h);
+                }
+            }
+            if (l != 0 && !skip_refine && w >= 4 && h >= 4) {
+                lds_sweep<true, 1>(X, Y, F, S1, w, h);
+                lds_sweep<true, -1>(X, Y, F, S1, w, h);
+                lds_sweep<false, 1>(X, Y, F, S1, w, h);
+                lds_sweep<false, -1>(X, Y, F, S1, w, h);
+            }
+            // the range check while no level has found a pixel in range (flow.rs:1291-1297), the
+            // damping at the start level (flow.rs:1298-1300)
+            bool zero = false;
+            if (active) {
+                int any = 0;
+                for (int i = threadIdx.x; i < n && !any; i += COARSE_THREADS) {
+                    const int y = i / w, x = i - y * w;
+                    const float px = (float)x + F[2 * i], py = (float)y + F[2 * i + 1];
+                    any = 0.0f <= px && px <= (float)(w - 1) && 0.0f <= py && py <= (float)(h - 1);
+                }
+                any = __syncthreads_or(any);
+                zero = !any;
+                active = !any;
+            }
+            const bool damp = l == J.start && J.scale_damp;
+            if (zero || damp) {
+                for (int i = threadIdx.x; i < 2 * n; i += COARSE_THREADS) {
+                    float u = F[i];
+                    if (zero) u = 0.0f;
+                    if (damp) u = J.damp * u;
+                    F[i] = u;
+                }
+                __syncthreads();
+            }
+            if (l > 0) {
+                // upsample_flow (flow.rs:376-391) into the next level: S1, or flow[0] at the handoff
+                const int dw = J.w[l - 1], dh = J.h[l - 1];
+                const float sx = J.up[l][0], sy = J.up[l][1], rx = J.up[l][2], ry = J.up[l][3];
+                float* dst = l - 1 > J.top ? S1 : J.flow[0];
+                for (int i = threadIdx.x; i < dw * dh; i += COARSE_THREADS) {
+                    const int y = i / dw, x = i - y * dw;
+                    int ix, iy;
+                    float ax, ay;
+                    cell((float)x * rx, w, ix, ax);
+                    cell((float)y * ry, h, iy, ay);
+                    const float* t0 = F + 2 * (iy * w + ix);
+                    const float* t1 = t0 + 2 * w;
+                    float out[2];
+#pragma unroll
+                    for (int c = 0; c < 2; c++) {
+                        const float p00 = t0[c], p10 = t0[2 + c], p01 = t1[c], p11 = t1[2 + c];
+                        const float top = (p10 - p00) * ax + p00;
+                        const float bot = (p11 - p01) * ax + p01;
+                        out[c] = (bot - top) * ay + top;
+                    }
+                    dst[2 * i] = out[0] * sx;
+                    dst[2 * i + 1] = out[1] * sy;
+                }
+                __syncthreads();
+                if (l - 1 > J.top) {
+                    for (int i = threadIdx.x; i < 2 * dw * dh; i += COARSE_THREADS) F[i] = S1[i];
+                    __syncthreads();
+                }
+            } else {
+                // the whole solve ran here: level 0 into the output, scanned for NaNs (the negation
+                // pass scans it again when it runs)
+                int nan = 0;
+                for (int i = threadIdx.x; i < 2 * n; i += COARSE_THREADS) {
+                    const float u = F[i];
+                    J.flow[1][i] = u;
+                    nan |= isnan(u) ? 1 : 0;
+                }
+                nan = __syncthreads_or(nan);
+                if (threadIdx.x == 0 && nan) J.state[2] = 1;
+            }
+        }
+        if (threadIdx.x == 0) {
+            J.state[0] = active ? 1 : 0;
+            J.state[1] = 0;
+        }
+        __syncthreads();
+    }
+}
+
+// -------------------------------------------------------------------------------------------
 // The solve driver: per level, the kernel sequence of flow.rs:1271-1304 over every job.
 
 // the flow starts at zero at the start level, in buffer 0; the range check starts active
 // (flow.rs:1268-1270)
-__global__ void k_solve_init(const SolveJob* __restrict__ jobs) {
+__global__ void k_solve_init(const SolveJob* __restrict__ jobs, int* __restrict__ queue) {
     const SolveJob& J = jobs[blockIdx.z];
+    if (queue && blockIdx.x == 0 && blockIdx.z == 0 && threadIdx.x == 0) *queue = 0;  // the coarse solver's work queue
     long n = (long)J.w[J.start] * J.h[J.start] * 2;
     long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) J.flow[0][i] = 0.0f;
@@ -1555,17 +1829,30 @@ int tile_tuning_report(int* out, int cap) {
 
 
 static void launch_solve_range(hipStream_t s, const SolveJob* d_jobs, const std::vector<SolveJob>& jobs,
-                               const PlaneT* d_planes, const std::vector<PlaneT>& planes, bool skip_refine, bool raw) {
+                               const PlaneT* d_planes, const std::vector<PlaneT>& planes, int* queue, bool skip_refine,
+                               bool raw) {
     int nj = (int)jobs.size();
     if (nj == 0) return;
-    int max_start = 0;
+    int max_top = -1, coarse = 0;
     long max_init = 0;
+    double Pc = 0;  // pixels the coarse solver reads and writes
     for (auto& j : jobs) {
-        max_start = std::max(max_start, j.start);
+        max_top = std::max(max_top, j.top);
         max_init = std::max(max_init, (long)j.w[j.start] * j.h[j.start] * 2);
+        if (j.top < j.start) {
+            coarse++;
+            for (int l = j.start; l > j.top; l--) Pc += (double)j.w[l] * j.h[l];
+        }
     }
-    TWX_LAUNCH(s, F_SOLVE_INIT, 0.0, k_solve_init<<<dim3((unsigned)((max_init + 255) / 256), 1, nj), dim3(256), 0, s>>>(d_jobs));
+    TWX_LAUNCH(s, F_SOLVE_INIT, 0.0,
+               k_solve_init<<<dim3((unsigned)((max_init + 255) / 256), 1, nj), dim3(256), 0, s>>>(d_jobs, coarse ? queue : nullptr));
     const Taps g7 = gaussian_taps(7.0f), g5 = gaussian_taps(5.0f);
+    if (coarse) {
+        // X and Y 4 + 4 in per pixel of each coarse level, the flow handed over 8 out
+        TWX_LAUNCH(s, F_COARSE, 16.0 * Pc,
+                   k_coarse_solve<<<dim3((unsigned)std::min(coarse, 256)), dim3(COARSE_THREADS), 0, s>>>(
+                       d_jobs, nj, queue, g7, g5, skip_refine ? 1 : 0));
+    }
     // block matching on the flow in buffer `buf`
     auto bm = [&](int l, int mw, int mh, double P, int buf) {
         dim3 tb(256), tg((mw + 31) / 32, (mh + 31) / 32, nj);
@@ -1575,7 +1862,7 @@ static void launch_solve_range(hipStream_t s, const SolveJob* d_jobs, const std:
         int fw = 0, fh = 0;
         double Pf = 0;
         for (auto& f : planes)
-            if (l <= f.start) {
+            if (l <= f.top) {
                 fw = std::max(fw, f.w[l]);
                 fh = std::max(fh, f.h[l]);
                 Pf += (double)f.w[l] * f.h[l];
@@ -1599,12 +1886,12 @@ static void launch_solve_range(hipStream_t s, const SolveJob* d_jobs, const std:
         TWX_LAUNCH(s, F_BM_COLS, 24.0 * P, (k_bm_sweep<false, 1, BM_CHUNK, false><<<cg, cb, 0, s>>>(d_jobs, l, buf)));
         TWX_LAUNCH(s, F_BM_COLS, 24.0 * P, (k_bm_sweep<false, -1, BM_CHUNK, false><<<cg, cb, 0, s>>>(d_jobs, l, buf)));
     };
-    for (int l = max_start; l >= 0; l--) {
+    for (int l = max_top; l >= 0; l--) {
         int mw = 0, mh = 0;
         double P = 0, Pn = 0;  // pixels of this level and of the next finer one, over the jobs
         double Ps = 0;         // pixels of the jobs that start at this level
         for (auto& j : jobs)
-            if (l <= j.start) {
+            if (l <= j.top) {
                 mw = std::max(mw, j.w[l]);
                 mh = std::max(mh, j.h[l]);
                 P += (double)j.w[l] * j.h[l];
@@ -1617,7 +1904,7 @@ static void launch_solve_range(hipStream_t s, const SolveJob* d_jobs, const std:
         int nt = 0, tw = 0, th = 0;
         double Pt = 0;
         for (int k = 0; k < nj; k++)
-            if (!jobs[k].tensor_shared && l <= jobs[k].start) {
+            if (!jobs[k].tensor_shared && l <= jobs[k].top) {
                 nt = k + 1;
                 tw = std::max(tw, jobs[k].w[l]);
                 th = std::max(th, jobs[k].h[l]);
@@ -1642,7 +1929,7 @@ static void launch_solve_range(hipStream_t s, const SolveJob* d_jobs, const std:
         if (l > 0) {
             int uw = 0, uh = 0;
             for (auto& j : jobs)
-                if (l <= j.start) { uw = std::max(uw, j.w[l - 1]); uh = std::max(uh, j.h[l - 1]); }
+                if (l <= j.top) { uw = std::max(uw, j.w[l - 1]); uh = std::max(uh, j.h[l - 1]); }
             TWX_LAUNCH(s, F_UPSAMPLE, 8.0 * P + 8.0 * Pn,
                        k_upsample<<<dim3((uw + 255) / 256, (uh + UP_ROWS - 1) / UP_ROWS, nj), blk, 0, s>>>(d_jobs, l));
         }
@@ -1661,8 +1948,8 @@ void launch_solve(hipStream_t s, const SolveJob* d_jobs, const std::vector<Solve
                   bool raw) {
     for (const SolveBucket& b : buckets)
         launch_solve_range(s, d_jobs + b.j0, std::vector<SolveJob>(jobs.begin() + b.j0, jobs.begin() + b.j1),
-                           d_planes + b.f0, std::vector<PlaneT>(planes.begin() + b.f0, planes.begin() + b.f1), skip_refine,
-                           raw);
+                           d_planes + b.f0, std::vector<PlaneT>(planes.begin() + b.f0, planes.begin() + b.f1), b.queue,
+                           skip_refine, raw);
 }
 
 __global__ void k_nan_scan(const float* __restrict__ d, size_t n, int* flag) {
