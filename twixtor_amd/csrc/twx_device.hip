@@ -1429,21 +1429,32 @@ static std::atomic<bool> g_coarse_on{true};
 void coarse_solver_enable(bool on) { g_coarse_on.store(on); }
 bool coarse_solver_enabled() { return g_coarse_on.load(); }
 
+// i / w for the pixel indices of a coarse level (i < 2^16, w < 2^16): the high word of i times
+// ceil(2^32 / w), exact in that range (its error, below i / 2^32 <= 2^-16 <= 1 / w, never carries
+// past the next multiple of 1 / w); one multiply instead of a division.
+struct RowDiv {
+    uint32_t m;
+    int w;
+    __device__ explicit RowDiv(int w_) : m(w_ > 1 ? (uint32_t)(0xFFFFFFFFu / (uint32_t)w_) + 1u : 0u), w(w_) {}
+    __device__ int operator()(int i) const { return w > 1 ? (int)__umulhi((uint32_t)i, m) : i; }
+};
+
 // A C-channel plane's Gaussian (conv.rs:144-199, 272-311) in LDS: the row pass (border columns
 // right first) src -> tmp, then the column pass (border rows above first) tmp -> dst.
 template <int R, int C>
-__device__ __forceinline__ void lds_gauss(const float* src, float* tmp, float* dst, int w, int h, const Taps& t) {
+__device__ __forceinline__ void lds_gauss(const float* src, float* tmp, float* dst, int w, int h, const Taps& t,
+                                          const RowDiv& row) {
     constexpr bool MULTI = C > 1;
     const int n = w * h * C;
     for (int e = threadIdx.x; e < n; e += COARSE_THREADS) {
         const int i = e / C, c = e - i * C;
-        const int y = i / w, x = i - y * w;
+        const int y = row(i), x = i - y * w;
         tmp[e] = line_elem<R, false, MULTI>(src + (y * w) * C + c, C, 0, w, x, t);
     }
     __syncthreads();
     for (int e = threadIdx.x; e < n; e += COARSE_THREADS) {
         const int i = e / C, c = e - i * C;
-        const int y = i / w, x = i - y * w;
+        const int y = row(i), x = i - y * w;
         dst[e] = line_elem<R, true, MULTI>(tmp + x * C + c, w * C, 0, h, y, t);
     }
     __syncthreads();
@@ -1486,40 +1497,64 @@ __device__ __forceinline__ bool lds_cost(const float* B, int w, int h, const flo
 // where it does not exist), which replaces the pixel's vector when it is lower (Block::step,
 // flow.rs:528-544). ROWS: chains are rows (sweeps 1, 2); DIR: the direction along them.
 template <bool ROWS, int DIR>
-__device__ void lds_sweep(const float* X, const float* Y, float* F, float* OC, int w, int h) {
-    auto patch = [&](int x, int y, float p[9]) {
+__device__ void lds_sweep(const float* X, const float* Y, float* F, float* OC, int w, int h, const RowDiv& row) {
+    for (int e = threadIdx.x; e < w * h; e += COARSE_THREADS) {
+        const int y = row(e), x = e - y * w;
+        if (x < 1 || x > w - 2 || y < 1 || y > h - 2) continue;
+        float p[9], c;
 #pragma unroll
         for (int b = 0; b < 3; b++)
 #pragma unroll
             for (int a = 0; a < 3; a++) p[b * 3 + a] = X[(y - 1 + b) * w + x - 1 + a];
-    };
-    for (int e = threadIdx.x; e < w * h; e += COARSE_THREADS) {
-        const int y = e / w, x = e - y * w;
-        if (x < 1 || x > w - 2 || y < 1 || y > h - 2) continue;
-        float p[9], c;
-        patch(x, y, p);
         OC[e] = lds_cost(Y, w, h, p, (float)x - F[2 * e], (float)y - F[2 * e + 1], c) ? c : -INFINITY;
     }
     __syncthreads();
     const int chains = ROWS ? h - 2 : w - 2, len = ROWS ? w - 2 : h - 2;
     for (int k = threadIdx.x; k < chains; k += COARSE_THREADS) {
-        const int cc = 1 + k;
-        const int a0 = DIR > 0 ? 1 : len;
-        int e = ROWS ? cc * w + (a0 - DIR) : (a0 - DIR) * w + cc;
+        // the chain's end pixel (which no sweep changes) gives the first incoming vector
+        int x = ROWS ? (DIR > 0 ? 0 : w - 1) : 1 + k;
+        int y = ROWS ? 1 + k : (DIR > 0 ? 0 : h - 1);
+        int e = y * w + x;
         const int step = ROWS ? DIR : DIR * w;
         float nu = F[2 * e], nv = F[2 * e + 1];
+        // the patch around the first pixel, then one new run of three per step
+        float p[9];
+#pragma unroll
+        for (int b = 0; b < 3; b++)
+#pragma unroll
+            for (int a = 0; a < 3; a++) {
+                const int px = ROWS ? x + DIR * a : x - 1 + a, py = ROWS ? y - 1 + b : y + DIR * b;
+                const int pa = ROWS ? (DIR > 0 ? a : 2 - a) : a, pb = ROWS ? b : (DIR > 0 ? b : 2 - b);
+                p[pb * 3 + pa] = X[py * w + px];
+            }
         for (int j = 0; j < len; j++) {
+            x += ROWS ? DIR : 0;
+            y += ROWS ? 0 : DIR;
             e += step;
-            const int y = e / w, x = e - y * w;
-            float p[9], c;
-            patch(x, y, p);
+            if (j > 0) {
+                // slide the patch one pixel along the chain
+#pragma unroll
+                for (int r = 0; r < 3; r++) {
+                    if (ROWS) {
+                        const float v = X[(y - 1 + r) * w + x + DIR];
+                        if (DIR > 0) { p[r * 3] = p[r * 3 + 1]; p[r * 3 + 1] = p[r * 3 + 2]; p[r * 3 + 2] = v; }
+                        else { p[r * 3 + 2] = p[r * 3 + 1]; p[r * 3 + 1] = p[r * 3]; p[r * 3] = v; }
+                    } else {
+                        const float v = X[(y + DIR) * w + x - 1 + r];
+                        if (DIR > 0) { p[r] = p[3 + r]; p[3 + r] = p[6 + r]; p[6 + r] = v; }
+                        else { p[6 + r] = p[3 + r]; p[3 + r] = p[r]; p[r] = v; }
+                    }
+                }
+            }
+            const float cu = F[2 * e], cv = F[2 * e + 1], own = OC[e];
+            float c;
             const float inc = lds_cost(Y, w, h, p, (float)x - nu, (float)y - nv, c) ? c : INFINITY;
-            if (inc < OC[e]) {
+            if (inc < own) {
                 F[2 * e] = nu;
                 F[2 * e + 1] = nv;
             } else {
-                nu = F[2 * e];
-                nv = F[2 * e + 1];
+                nu = cu;
+                nv = cv;
             }
         }
     }
@@ -1553,6 +1588,7 @@ __global__ __launch_bounds__(COARSE_THREADS) void k_coarse_solve(const SolveJob*
         }
         for (int l = J.start; l > J.top; l--) {
             const int w = J.w[l], h = J.h[l], n = w * h;
+            const RowDiv row(w);
             for (int i = threadIdx.x; i < n; i += COARSE_THREADS) {
                 X[i] = J.X[l][i];
                 Y[i] = J.Y[l][i];
@@ -1560,7 +1596,7 @@ __global__ __launch_bounds__(COARSE_THREADS) void k_coarse_solve(const SolveJob*
             __syncthreads();
             // calc_grad + calc_a (flow.rs:283-328), the tensor blurred r = 7
             for (int i = threadIdx.x; i < n; i += COARSE_THREADS) {
-                const int y = i / w, x = i - y * w;
+                const int y = row(i), x = i - y * w;
                 const float c = X[i];
                 float nx = 0.0f, dx = 0.0f, ny = 0.0f, dy = 0.0f;
                 if (x >= 1) { nx = (c - X[i - 1]) + 0.0f; dx = 1.0f; }
@@ -1575,11 +1611,11 @@ __global__ __launch_bounds__(COARSE_THREADS) void k_coarse_solve(const SolveJob*
                 S1[3 * i + 2] = gy * gy;
             }
             __syncthreads();
-            lds_gauss<7, 3>(S1, S2, TA, w, h, g7);
+            lds_gauss<7, 3>(S1, S2, TA, w, h, g7, row);
             for (int it = 0; it < 3; it++) {
                 // calc_b (flow.rs:334-357), blurred r = 7
                 for (int i = threadIdx.x; i < n; i += COARSE_THREADS) {
-                    const int y = i / w, x = i - y * w;
+                    const int y = row(i), x = i - y * w;
                     const float sx = (float)x - F[2 * i], sy = (float)y - F[2 * i + 1];
                     float bx = 0.0f, by = 0.0f;
                     if (!(sx < 0.0f || (float)w - 1.0f < sx || sy < 0.0f || (float)h - 1.0f < sy)) {
@@ -1591,7 +1627,7 @@ __global__ __launch_bounds__(COARSE_THREADS) void k_coarse_solve(const SolveJob*
                     S1[2 * i + 1] = by;
                 }
                 __syncthreads();
-                lds_gauss<7, 2>(S1, S2, S1, w, h, g7);
+                lds_gauss<7, 2>(S1, S2, S1, w, h, g7, row);
                 // new_disp (flow.rs:361-371), the flow blurred r = 5
                 for (int i = threadIdx.x; i < n; i += COARSE_THREADS) {
                     const float aa = TA[3 * i], ab = TA[3 * i + 1], ad = TA[3 * i + 2];
@@ -1606,19 +1642,19 @@ __global__ __launch_bounds__(COARSE_THREADS) void k_coarse_solve(const SolveJob*
                     S2[2 * i + 1] = v;
                 }
                 __syncthreads();
-                lds_gauss<5, 2>(S2, S1, F, w, h, g5);
+                lds_gauss<5, 2>(S2, S1, F, w, h, g5, row);
                 if (it == 1 && l == 0 && !skip_refine && w >= 4 && h >= 4) {
-                    lds_sweep<true, 1>(X, Y, F, S1, w, h);
-                    lds_sweep<true, -1>(X, Y, F, S1, w, h);
-                    lds_sweep<false, 1>(X, Y, F, S1, w, h);
-                    lds_sweep<false, -1>(X, Y, F, S1, w, h);
+                    lds_sweep<true, 1>(X, Y, F, S1, w, h, row);
+                    lds_sweep<true, -1>(X, Y, F, S1, w, h, row);
+                    lds_sweep<false, 1>(X, Y, F, S1, w, h, row);
+                    lds_sweep<false, -1>(X, Y, F, S1, w, h, row);
                 }
             }
             if (l != 0 && !skip_refine && w >= 4 && h >= 4) {
-                lds_sweep<true, 1>(X, Y, F, S1, w, h);
-                lds_sweep<true, -1>(X, Y, F, S1, w, h);
-                lds_sweep<false, 1>(X, Y, F, S1, w, h);
-                lds_sweep<false, -1>(X, Y, F, S1, w, h);
+                lds_sweep<true, 1>(X, Y, F, S1, w, h, row);
+                lds_sweep<true, -1>(X, Y, F, S1, w, h, row);
+                lds_sweep<false, 1>(X, Y, F, S1, w, h, row);
+                lds_sweep<false, -1>(X, Y, F, S1, w, h, row);
             }
             // the range check while no level has found a pixel in range (flow.rs:1291-1297), the
             // damping at the start level (flow.rs:1298-1300)
@@ -1626,7 +1662,7 @@ __global__ __launch_bounds__(COARSE_THREADS) void k_coarse_solve(const SolveJob*
             if (active) {
                 int any = 0;
                 for (int i = threadIdx.x; i < n && !any; i += COARSE_THREADS) {
-                    const int y = i / w, x = i - y * w;
+                    const int y = row(i), x = i - y * w;
                     const float px = (float)x + F[2 * i], py = (float)y + F[2 * i + 1];
                     any = 0.0f <= px && px <= (float)(w - 1) && 0.0f <= py && py <= (float)(h - 1);
                 }
@@ -1649,8 +1685,9 @@ __global__ __launch_bounds__(COARSE_THREADS) void k_coarse_solve(const SolveJob*
                 const int dw = J.w[l - 1], dh = J.h[l - 1];
                 const float sx = J.up[l][0], sy = J.up[l][1], rx = J.up[l][2], ry = J.up[l][3];
                 float* dst = l - 1 > J.top ? S1 : J.flow[0];
+                const RowDiv drow(dw);
                 for (int i = threadIdx.x; i < dw * dh; i += COARSE_THREADS) {
-                    const int y = i / dw, x = i - y * dw;
+                    const int y = drow(i), x = i - y * dw;
                     int ix, iy;
                     float ax, ay;
                     cell((float)x * rx, w, ix, ax);
