@@ -1095,8 +1095,11 @@ int retime_batch_enqueue(twx_handle* h, DevPool& pool, const std::vector<ClipRun
             jobs.push_back(F);
         }
     }
-    std::stable_sort(jobs.begin(), jobs.end(),
-                     [](const FrameJob& a, const FrameJob& b) { return (long)a.W * a.H > (long)b.W * b.H; });
+    // largest first; within a size, the blend pairs together (they run in a kernel of their own)
+    std::stable_sort(jobs.begin(), jobs.end(), [](const FrameJob& a, const FrameJob& b) {
+        const long aa = (long)a.W * a.H, ab = (long)b.W * b.H;
+        return aa != ab ? aa > ab : frame_is_blend_pair(a) > frame_is_blend_pair(b);
+    });
     if (!jobs.empty()) {
         for (auto& F : jobs) frame_job_ratios(F);
         FrameJob* dj = pool.upload(jobs);

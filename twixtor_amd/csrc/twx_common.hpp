@@ -146,6 +146,11 @@ struct FrameJob {
 
 // Fills the per-frame warp ratios of a job (IEEE division on the host, as on the device).
 void frame_job_ratios(FrameJob& F);
+// A frame launch_frames sends through its kernel for the common vector-path frame (kind 2, both
+// warps computed, no pushed frames).
+__host__ __device__ inline bool frame_is_blend_pair(const FrameJob& F) {
+    return F.kind == 2 && !F.pre_a && !F.warp_a_copy && !F.warp_b_copy;
+}
 
 constexpr int MAXFL = 24;  // levels of the hole-fill pyramid (holefill.rs:145-176)
 constexpr int FILL_FLOW2 = 3;  // FillPlane.fmt of a two-channel float flow (holefill.rs:506-521)

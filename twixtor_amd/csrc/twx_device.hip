@@ -2445,6 +2445,30 @@ __global__ void k_frames(const FrameJob* __restrict__ jobs) {
     finish_pixel<FMT>(F.out, n, oi, e);
 }
 
+// The common vector-path frame (kind 2: Blend of two inverse warps, neither an early-out copy,
+// no pushed frames) on its own: the same functions as k_frames' branch for it, in a kernel a
+// fraction of k_frames' size (whose other kinds and their NaN-rule paths make a large body).
+
+template <int FMT>
+__global__ void k_frames_blend(const FrameJob* __restrict__ jobs) {
+    const FrameJob& F = jobs[blockIdx.z];
+    if (F.fmt != FMT) return;
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = blockIdx.y;  // motion row
+    if (x >= F.W || y >= F.H) return;
+    const long n = (long)F.W * F.H;
+    float pa[4], pb[4], e[4];
+    const bool ok = warp_pixel_pair<FMT>(F, x, y, pa, pb);
+    blend_pixel<FMT>(pa, pb, F.blend_mode, F.weight, F.lerp_w, e);
+    if (!ok) atomicOr(F.bad, 1);
+    if (!F.finish) {
+        typename Px<FMT>::T* o = reinterpret_cast<typename Px<FMT>::T*>(F.out) + ((long)y * F.W + x) * 4;
+        for (int k = 0; k < 4; k++) o[k] = (typename Px<FMT>::T)e[k];
+        return;
+    }
+    finish_pixel<FMT>(F.out, n, (long)(F.H - 1 - y) * F.W + x, e);
+}
+
 void frame_job_ratios(FrameJob& F) {
     F.rx = (float)(F.fw - 1) / (float)(F.W - 1);
     F.ry = (float)(F.fh - 1) / (float)(F.H - 1);
@@ -2467,7 +2491,20 @@ void launch_frames(hipStream_t s, const FrameJob* d_jobs, const std::vector<Fram
         launch_frames_range(s, d_jobs + b0, std::vector<FrameJob>(jobs.begin() + b0, jobs.begin() + b1));
 }
 
+static void launch_frames_run(hipStream_t s, const FrameJob* d_jobs, const std::vector<FrameJob>& jobs, bool pair);
+
 static void launch_frames_range(hipStream_t s, const FrameJob* d_jobs, const std::vector<FrameJob>& jobs) {
+    // runs of consecutive jobs: the blend pairs through k_frames_blend, the others through k_frames
+    for (size_t r0 = 0; r0 < jobs.size();) {
+        const bool pair = frame_is_blend_pair(jobs[r0]);
+        size_t r1 = r0 + 1;
+        while (r1 < jobs.size() && frame_is_blend_pair(jobs[r1]) == pair) r1++;
+        launch_frames_run(s, d_jobs + r0, std::vector<FrameJob>(jobs.begin() + r0, jobs.begin() + r1), pair);
+        r0 = r1;
+    }
+}
+
+static void launch_frames_run(hipStream_t s, const FrameJob* d_jobs, const std::vector<FrameJob>& jobs, bool pair) {
     if (jobs.empty()) return;
     int mw = 0, mh = 0;
     bool has[3] = {false, false, false};
@@ -2485,6 +2522,12 @@ static void launch_frames_range(hipStream_t s, const FrameJob* d_jobs, const std
         if (f.kind == 3 || f.kind == 6) { imgs = 4.0; masks = 2.0; }
         if (f.kind >= 4) masks += 1.0;
         bytes += imgs * px * bpp + masks * px + flows;
+    }
+    if (pair) {
+        if (has[0]) TWX_LAUNCH(s, F_FRAMES, bytes, k_frames_blend<0><<<grd, blk, 0, s>>>(d_jobs));
+        if (has[1]) TWX_LAUNCH(s, F_FRAMES, bytes, k_frames_blend<1><<<grd, blk, 0, s>>>(d_jobs));
+        if (has[2]) TWX_LAUNCH(s, F_FRAMES, bytes, k_frames_blend<2><<<grd, blk, 0, s>>>(d_jobs));
+        return;
     }
     if (has[0]) TWX_LAUNCH(s, F_FRAMES, bytes, k_frames<0><<<grd, blk, 0, s>>>(d_jobs));
     if (has[1]) TWX_LAUNCH(s, F_FRAMES, bytes, k_frames<1><<<grd, blk, 0, s>>>(d_jobs));
