@@ -27,12 +27,14 @@ SPEED = "speed_percent=100@0;25@60;100@120"
 
 # Measured HBM bytes over algorithmic bytes per kernel family, from the FETCH_SIZE / WRITE_SIZE
 # counter runs (copy-calibrated) recorded in profiles/retime_1080p_hbm_traffic_pmc.md.
-TRAFFIC_RATIO = {"bm_rows": 1.42, "bm_cols": 1.28, "calc_b_gauss": 1.05, "new_disp_gauss": 1.05, "tensor_gauss": 1.02,
-                 "frames": 1.03, "bm_transpose": 1.30}
+TRAFFIC_RATIO = {"frames": 1.03, "new_disp_gauss": 1.00, "calc_b_gauss": 1.03, "bm_cols": 1.26, "bm_rows": 1.22,
+                 "ingest": 1.00, "pyr_luma_blur5_band_resample": 1.08, "bm_transpose": 1.09, "upsample": 1.27,
+                 "tensor_gauss": 1.00}
 # The share of the SIMDs' issue slots the family's VALU instructions fill over its time, from the
 # SQ_INSTS_VALU counter run recorded in the same profile.
-VALU_SHARE = {"calc_b_gauss": 0.71, "new_disp_gauss": 0.58, "tensor_gauss": 0.71, "frames": 0.62, "bm_rows": 0.60,
-              "bm_cols": 0.71}
+VALU_SHARE = {"calc_b_gauss": 0.75, "frames": 0.67, "bm_rows": 0.58, "bm_cols": 0.61, "new_disp_gauss": 0.58,
+              "pyr_luma_blur5_band_resample": 0.53, "tensor_gauss": 0.65, "bm_transpose": 0.23, "upsample": 0.38,
+              "ingest": 0.10}
 # The library counts a fused kernel's bytes once (its inputs read once, its outputs written once).
 # The step-boundary model of the CPU path counts every step's inputs and outputs, 4 bytes a float:
 # grad + tensor 24 and the tensor blur 48 B/px (fused: 24), calc_b 32 and the B blur 32 (fused: 32),
