@@ -186,8 +186,8 @@ int twx_solve_pairs_batched(twx_handle* h, int32_t n, const twx_pair* pairs, int
  * stages.rs:60-62): 1 for the retime and vectors-in front-ends (frontend.rs:238-241,
  * vectors-in/src/glue.rs:242), 0 for the plain CPU stages. `forward_gouraud` is the forward warp
  * variant of the stage set: 1 the Gouraud variant (stages::GouraudForward, stages.rs:101-130,
- * which vectors-in and retime-fgbg install), 0 the textured variant (refused with
- * TWX_E_UNSUPPORTED for warping codes 2 and 3). Host engine images. */
+ * which vectors-in and retime-fgbg install), 0 the textured variant (the engine's plain stage
+ * set: pyramid-sampled vertices, forward.rs:229-320). Host engine images. */
 int twx_compose_vectors(twx_handle* h, int32_t interp, float wa, const void* a, const void* b,
                         int32_t width, int32_t height, int32_t fmt, const float* f_ab,
                         const float* f_ba, int32_t flow_w, int32_t flow_h, int32_t warp_mode,
@@ -199,11 +199,13 @@ int twx_compose_vectors(twx_handle* h, int32_t interp, float wa, const void* a, 
 /* The forward warp of `src` by `flow` times `amount`: the source pixels as the vertices of a
  * displaced mesh, its triangles scan converted in cell order, the running mean of the colours per
  * covered pixel in triangle order. `pyramid` is ForwardOpts::pyramid: 0 the Gouraud variant
- * (vertex colours), 1 the textured variant (refused with TWX_E_UNSUPPORTED). `coverage` (width *
- * height bytes: 255 covered, 0 hole) may be NULL. The early outs copy `src` with full coverage.
- * TWX_E_UNSUPPORTED as the CPU path for the degenerate last row pass that would write outside the
- * image, and for what this device does not serve: a triangle over 65536 covered pixels, NaN
- * colours. Host buffers. */
+ * (vertex colours), 1 the textured variant (each vertex carries its texture coordinates and mip
+ * level, mesh.rs:54-88, and each covered pixel samples the source's mip pyramid there,
+ * pyramid.rs). `coverage` (width * height bytes: 255 covered, 0 hole) may be NULL. The early outs
+ * copy `src` with full coverage. TWX_E_UNSUPPORTED as the CPU path for the degenerate last row
+ * pass that would write outside the image, and for what this device does not serve: a triangle
+ * over 65536 covered pixels, NaN colours (a NaN vertex colour of the Gouraud variant, a NaN sample
+ * of the textured one), a textured tap below the image. Host buffers. */
 int twx_forward_warp(twx_handle* h, const void* src, int32_t width, int32_t height, int32_t fmt,
                      const float* flow, int32_t flow_w, int32_t flow_h, float amount,
                      int32_t pyramid, void* out, uint8_t* coverage, char* err, size_t errlen);
@@ -235,8 +237,8 @@ int twx_flow_from_vectors(twx_handle* h, const void* vec, int32_t width, int32_t
 
 /* invert_flow_on (compose/vector.rs:370-397): T = (-1, -fx, -fy, -1) pushed along the flow by
  * the forward warp with coverage (amount 1), the uncovered pixels filled. `forward_gouraud` is
- * the stage set's forward variant (1 for Vectors In; 0, the textured one, is refused with
- * TWX_E_UNSUPPORTED). Host buffers. */
+ * the stage set's forward variant (1 for Vectors In; 0 the textured one of the plain stage set).
+ * Host buffers. */
 int twx_invert_flow(twx_handle* h, const float* flow, int32_t width, int32_t height,
                     int32_t forward_gouraud, int32_t taps_above_first, float* out, char* err,
                     size_t errlen);

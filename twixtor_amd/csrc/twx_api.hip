@@ -1524,12 +1524,33 @@ int twx_solve_pair(twx_handle* h, const void* a, const void* b, int32_t width, i
     return st;
 }
 
-// The Gouraud forward warp of a device image into `out` (and `cov`, may be null); the flow on the
-// device, with its host copy for the early-out test (forward.rs:245-256). `dflags` is a device int
-// the caller cleared; its bits 8 and 16 are read by the caller after its final synchronisation.
+// The mip levels of a textured forward warp's vertices (mesh.rs:54-88) from their arguments m:
+// -(logf(m) * (1 / logf(2))) + 0, in place, with the host C runtime's logf (the reference's device
+// path takes the logarithm on the host too; a device logarithm would differ in the last bits).
+static void fw_mip_levels_host(float* m, size_t n) {
+    const float inv_ln2 = 1.0f / logf(2.0f);
+    auto run = [=](size_t a, size_t b) {
+        for (size_t i = a; i < b; i++) m[i] = -(logf(m[i]) * inv_ln2) + 0.0f;
+    };
+    const size_t nt = n < (1u << 18) ? 1 : std::min<size_t>(16, std::max(1u, std::thread::hardware_concurrency()));
+    if (nt == 1) {
+        run(0, n);
+        return;
+    }
+    std::vector<std::thread> th;
+    const size_t step = (n + nt - 1) / nt;
+    for (size_t a = 0; a < n; a += step) th.emplace_back(run, a, std::min(n, a + step));
+    for (auto& t : th) t.join();
+}
+
+// The forward warp of a device image into `out` (and `cov`, may be null): the Gouraud variant
+// (vertex colours), or with `textured` the pyramid-sampled variant (ForwardOpts::pyramid,
+// forward.rs:229-320); the flow on the device, with its host copy for the early-out test
+// (forward.rs:245-256). `dflags` is a device int the caller cleared; its bits 8, 16, 32 and 64 are
+// read by the caller after its final synchronisation (check_fw_flags).
 static int forward_warp_dev(twx_handle* h, DevPool& pool, const void* src, int W, int H, int fmt, const float* d_flow,
-                            const float* h_flow, int fw, int fh, float amount, void* out, uint8_t* cov, int* dflags,
-                            std::string& err) {
+                            const float* h_flow, int fw, int fh, float amount, bool textured, void* out, uint8_t* cov,
+                            int* dflags, std::string& err) {
     hipStream_t s = h->stream;
     const float a = 0.0f > amount ? -amount : amount;
     bool still = 1e-05 > (double)a;
@@ -1564,8 +1585,40 @@ static int forward_warp_dev(twx_handle* h, DevPool& pool, const void* src, int W
     A.fh = fh;
     A.fmt = fmt;
     A.flags = dflags;
+    A.textured = textured ? 1 : 0;
     fw_grid(A, amount);
     const long n = (long)W * H;
+    if (textured) {
+        // pyramid.rs:20-63: the levels (halved, at least 2, from max(w, 2) x max(h, 2)) below the
+        // source, and the mip level of every vertex when there is more than one level
+        A.nlev = fw_level_count(W, H);
+        if (A.nlev > FW_MAX_LEVELS) {
+            err = "forward warp: a texture pyramid deeper than this backend's level table";
+            return TWX_E_UNSUPPORTED;
+        }
+        A.lev[0] = src;
+        A.lw[0] = W;
+        A.lh[0] = H;
+        int cw = std::max(W, 2), ch = std::max(H, 2);
+        for (int i = 1; i < A.nlev; i++) {
+            cw = std::max(cw / 2, 2);
+            ch = std::max(ch / 2, 2);
+            A.lw[i] = cw;
+            A.lh[i] = ch;
+            A.lev[i] = pool.alloc<uint8_t>(image_bytes(cw, ch, fmt));
+        }
+        if (A.nlev > 1) {
+            launch_fw_pyramid(s, A, pool.alloc<float>((size_t)n * 8));
+            float* m = pool.alloc<float>(n);
+            float* hm = pool.pinned<float>(n);
+            launch_fw_mip_arg(s, A, m);
+            HIPCHECK(hipMemcpyAsync(hm, m, n * sizeof(float), hipMemcpyDeviceToHost, s));
+            HIPCHECK(hipStreamSynchronize(s));
+            fw_mip_levels_host(hm, n);
+            HIPCHECK(hipMemcpyAsync(m, hm, n * sizeof(float), hipMemcpyHostToDevice, s));
+            A.mips = m;
+        }
+    }
     A.pos = pool.alloc<float2>(n);
     A.counts = pool.alloc<uint32_t>(n);
     A.fill = pool.alloc<uint32_t>(n);
@@ -1609,6 +1662,14 @@ static int check_fw_flags(int f, std::string& err) {
         err = "forward warp: the device hit lists were inconsistent";
         return TWX_E_DEVICE;
     }
+    if (f & 32) {
+        err = "forward warp: a textured sample below the image (a pyramid tap the CPU path would index out of bounds)";
+        return TWX_E_UNSUPPORTED;
+    }
+    if (f & 64) {
+        err = "forward warp: a NaN colour (the NaN rule of the colour arithmetic has no device kernel)";
+        return TWX_E_UNSUPPORTED;
+    }
     return TWX_OK;
 }
 
@@ -1624,10 +1685,6 @@ int twx_forward_warp(twx_handle* h, const void* src, int32_t width, int32_t heig
                 e = "forward warp needs images and a flow of at least 2x2";
                 st = TWX_E_MISMATCH;
             } else if ((st = check_frame_pixels(width, height, e))) {
-            } else if (pyramid) {
-                e = "forward warp: the textured variant (ForwardOpts::pyramid) has no device kernel here; the Gouraud "
-                    "variant has";
-                st = TWX_E_UNSUPPORTED;
             } else {
                 hipStream_t s = h->stream;
                 DevPool pool(h);
@@ -1639,8 +1696,8 @@ int twx_forward_warp(twx_handle* h, const void* src, int32_t width, int32_t heig
                 HIPCHECK(hipMemcpyAsync(img, src, fb, hipMemcpyHostToDevice, s));
                 HIPCHECK(hipMemcpyAsync(fl, flow, fn * 4, hipMemcpyHostToDevice, s));
                 HIPCHECK(hipMemsetAsync(flags, 0, sizeof(int), s));
-                st = forward_warp_dev(h, pool, img, width, height, fmt, fl, flow, flow_w, flow_h, amount, img + fb, cov,
-                                      flags, e);
+                st = forward_warp_dev(h, pool, img, width, height, fmt, fl, flow, flow_w, flow_h, amount, pyramid != 0,
+                                      img + fb, cov, flags, e);
                 if (!st) {
                     int* hf = pool.pinned<int>(1);
                     HIPCHECK(hipMemcpyAsync(out, img + fb, fb, hipMemcpyDeviceToHost, s));
@@ -1765,11 +1822,12 @@ static int flow_from_vectors_dev(hipStream_t s, DevPool& pool, const void* d_vec
 }
 
 // invert_flow_on (compose/vector.rs:370-397) of a device flow into `out` (both width x height): T =
-// (-1, -fx, -fy, -1) pushed along the flow by the Gouraud forward warp with coverage, the read-out,
-// the push-pull fill of the uncovered pixels. `h_flow` is the flow's host copy or null; the forward
-// warp's flags go to `fw_flags` (a device int the caller cleared).
+// (-1, -fx, -fy, -1) pushed along the flow by the forward warp with coverage (Gouraud, or the
+// textured variant), the read-out, the push-pull fill of the uncovered pixels. `h_flow` is the
+// flow's host copy or null; the forward warp's flags go to `fw_flags` (a device int the caller
+// cleared).
 static int invert_flow_dev(twx_handle* h, DevPool& pool, const float* d_flow, const float* h_flow, int W, int H,
-                           bool above_first, float* out, int* fw_flags, std::string& err) {
+                           bool above_first, bool textured, float* out, int* fw_flags, std::string& err) {
     hipStream_t s = h->stream;
     const size_t n = (size_t)W * H;
     float* t = pool.alloc<float>(n * 4);
@@ -1779,7 +1837,8 @@ static int invert_flow_dev(twx_handle* h, DevPool& pool, const float* d_flow, co
     int* gate = pool.alloc<int>(1);
     launch_zero(s, gate, 1);
     launch_invert_prep(s, d_flow, t, (long)n);
-    int st = forward_warp_dev(h, pool, t, W, H, TWX_FMT_F32, d_flow, h_flow, W, H, 1.0f, pushed, cov, fw_flags, err);
+    int st = forward_warp_dev(h, pool, t, W, H, TWX_FMT_F32, d_flow, h_flow, W, H, 1.0f, textured, pushed, cov, fw_flags,
+                              err);
     if (st) return st;
     launch_invert_post(s, pushed, cov, out, holes, gate, (long)n);
     std::vector<FillItem> items{FillItem{W, H, out, FILL_FLOW2, 2, holes, gate, above_first ? 1 : 0}};
@@ -1818,10 +1877,6 @@ int twx_invert_flow(twx_handle* h, const float* flow, int32_t width, int32_t hei
             return TWX_E_UNSUPPORTED;
         }
         if (int st = check_frame_pixels(width, height, e)) return st;
-        if (!forward_gouraud) {
-            e = "the flow inversion through the textured forward warp has no device kernel here";
-            return TWX_E_UNSUPPORTED;
-        }
         hipStream_t s = h->stream;
         DevPool pool(h);
         const size_t n = (size_t)width * height;
@@ -1830,7 +1885,8 @@ int twx_invert_flow(twx_handle* h, const float* flow, int32_t width, int32_t hei
         int* flags = pool.alloc<int>(1);
         HIPCHECK(hipMemcpyAsync(fl, flow, n * 8, hipMemcpyHostToDevice, s));
         launch_zero(s, flags, 1);
-        int st = invert_flow_dev(h, pool, fl, flow, width, height, taps_above_first != 0, res, flags, e);
+        int st = invert_flow_dev(h, pool, fl, flow, width, height, taps_above_first != 0, forward_gouraud == 0, res, flags,
+                                 e);
         if (st) return st;
         int* hf = pool.pinned<int>(1);
         HIPCHECK(hipMemcpyAsync(out, res, n * 8, hipMemcpyDeviceToHost, s));
@@ -2115,7 +2171,7 @@ static int vectors_in_enqueue(twx_handle* h, DevPool& pool, const twx_vin_params
         auto it = inverse_of.find(vslot);
         if (it != inverse_of.end()) return it->second;
         float* f = pool.alloc<float>(npx * 2);
-        int st = invert_flow_dev(h, pool, flow(vslot), nullptr, W, H, true, f, fw_flags, err);
+        int st = invert_flow_dev(h, pool, flow(vslot), nullptr, W, H, true, false, f, fw_flags, err);
         if (st) throw st;
         inverse_of[vslot] = f;
         return f;
@@ -2180,7 +2236,7 @@ static int vectors_in_enqueue(twx_handle* h, DevPool& pool, const twx_vin_params
                     void* dst = pool.alloc<uint8_t>(fb);
                     uint8_t* cov = pool.alloc<uint8_t>(npx);
                     st = forward_warp_dev(h, pool, from_a ? F.a : F.b, W, H, fmt, from_a ? f_ab : f_ba, nullptr, W, H,
-                                          from_a ? F.frac : F.wa, dst, cov, fw_flags, err);
+                                          from_a ? F.frac : F.wa, false, dst, cov, fw_flags, err);
                     F.pre_a = dst;
                     F.pre_cov_a = cov;
                 } else {
@@ -2188,8 +2244,11 @@ static int vectors_in_enqueue(twx_handle* h, DevPool& pool, const twx_vin_params
                     void* tb = pool.alloc<uint8_t>(fb);
                     uint8_t* c1 = pool.alloc<uint8_t>(npx);
                     uint8_t* c2 = pool.alloc<uint8_t>(npx);
-                    st = forward_warp_dev(h, pool, F.a, W, H, fmt, f_ab, nullptr, W, H, F.frac, ta, c1, fw_flags, err);
-                    if (!st) st = forward_warp_dev(h, pool, F.b, W, H, fmt, f_ba, nullptr, W, H, F.wa, tb, c2, fw_flags, err);
+                    st = forward_warp_dev(h, pool, F.a, W, H, fmt, f_ab, nullptr, W, H, F.frac, false, ta, c1, fw_flags,
+                                          err);
+                    if (!st)
+                        st = forward_warp_dev(h, pool, F.b, W, H, fmt, f_ba, nullptr, W, H, F.wa, false, tb, c2, fw_flags,
+                                              err);
                     F.pre_a = ta;
                     F.pre_b = tb;
                     F.pre_cov_a = c1;
@@ -2345,12 +2404,7 @@ int twx_compose_vectors(twx_handle* h, int32_t interp, float wa, const void* a, 
             set_err(err, errlen, "warping code outside 0..3");
             return TWX_E_MISMATCH;
         }
-        const bool forward = warp_mode >= 2;
-        if (forward && !forward_gouraud) {
-            set_err(err, errlen, "the textured forward warp (warping codes 2 and 3 without stages::GouraudForward) has no "
-                                 "device kernel here");
-            return TWX_E_UNSUPPORTED;
-        }
+        const bool forward = warp_mode >= 2, textured = forward_gouraud == 0;
         if (width < 2 || height < 2 || flow_w < 2 || flow_h < 2 || fmt < 0 || fmt > 2) {
             set_err(err, errlen, "the warps need images and a flow of at least 2x2");
             return TWX_E_MISMATCH;
@@ -2407,7 +2461,8 @@ int twx_compose_vectors(twx_handle* h, int32_t interp, float wa, const void* a, 
                 void* dst = fill ? (void*)pool.alloc<uint8_t>(fb) : F.out;
                 uint8_t* cov = fill ? pool.alloc<uint8_t>(np) : nullptr;
                 st = forward_warp_dev(h, pool, from_a ? F.a : F.b, width, height, fmt, from_a ? F.f_ab : F.f_ba,
-                                      from_a ? f_ab : f_ba, flow_w, flow_h, from_a ? 1.0f - wa : wa, dst, cov, bad + 2, e);
+                                      from_a ? f_ab : f_ba, flow_w, flow_h, from_a ? 1.0f - wa : wa, textured, dst, cov,
+                                      bad + 2, e);
                 F.pre_a = dst;
                 F.pre_cov_a = cov;
                 direct = !fill;
@@ -2416,11 +2471,11 @@ int twx_compose_vectors(twx_handle* h, int32_t interp, float wa, const void* a, 
                 void* tb = pool.alloc<uint8_t>(fb);
                 uint8_t* c1 = fill ? pool.alloc<uint8_t>(np) : nullptr;
                 uint8_t* c2 = fill ? pool.alloc<uint8_t>(np) : nullptr;
-                st = forward_warp_dev(h, pool, F.a, width, height, fmt, F.f_ab, f_ab, flow_w, flow_h, F.frac, ta, c1,
-                                      bad + 2, e);
+                st = forward_warp_dev(h, pool, F.a, width, height, fmt, F.f_ab, f_ab, flow_w, flow_h, F.frac, textured,
+                                      ta, c1, bad + 2, e);
                 if (!st)
-                    st = forward_warp_dev(h, pool, F.b, width, height, fmt, F.f_ba, f_ba, flow_w, flow_h, wa, tb, c2,
-                                          bad + 2, e);
+                    st = forward_warp_dev(h, pool, F.b, width, height, fmt, F.f_ba, f_ba, flow_w, flow_h, wa, textured, tb,
+                                          c2, bad + 2, e);
                 F.pre_a = ta;
                 F.pre_b = tb;
                 F.pre_cov_a = c1;

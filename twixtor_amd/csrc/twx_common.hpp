@@ -316,11 +316,13 @@ int warp_blend_backward_noalloc(hipStream_t s, int device, int C, int W, int H, 
                                 const float* f_ab, const float* f_ba, int fw, int fh, float wa, const float* grad_out,
                                 float* grad_a, float* grad_b, float* grad_f_ab, float* grad_f_ba, std::string& err);
 
-// The Gouraud forward warp (twx_forward.hip): device buffers of one warp. The caller allocates
-// pos (W*H), counts and fill (W*H, zeroed), offsets (W*H), block_totals (fw_scan_blocks(W*H)),
-// total (1) and flags (1, zeroed); after launch_fw_count and a read of *total, rec_t and rec_c
-// (total records each), then launch_fw_emit_fold. flags: 1 the last row pass would write outside
-// the image, 2 a triangle over the per-triangle hit cap, 4 a NaN colour, 8 a repeated hit.
+// The forward warp (twx_forward.hip): device buffers of one warp. The caller allocates pos (W*H),
+// counts and fill (W*H, zeroed), offsets (W*H), block_totals (fw_scan_blocks(W*H)), total (1) and
+// flags (1, zeroed); after launch_fw_count and a read of *total, rec_t and rec_c (total records
+// each), then launch_fw_emit_fold. flags: 1 the last row pass would write outside the image, 2 a
+// triangle over the per-triangle hit cap, 4 a NaN vertex colour (Gouraud), 8 and 16 a hit list
+// inconsistent with the count pass, 32 a textured tap below the image, 64 a NaN textured sample.
+constexpr int FW_MAX_LEVELS = 16;  // mip levels of the textured forward warp (a side of 65536 gives 16)
 struct FwArgs {
     const void* src;
     void* out;
@@ -338,6 +340,14 @@ struct FwArgs {
     int W, H, fw, fh, fmt;
     float rx, ry, sx, sy;
     uint64_t ntri;
+    // the textured variant (forward.rs:229-320): the vertices carry (u, v, d) and every hit samples
+    // the source's mip pyramid (forward/pyramid.rs) at them; mips: the per-vertex mip level d (null
+    // when the pyramid has one level); lev[0] is the source, lev[1..nlev) its blurred levels
+    int textured;
+    int nlev;
+    const float* mips;
+    const void* lev[FW_MAX_LEVELS];
+    int lw[FW_MAX_LEVELS], lh[FW_MAX_LEVELS];
 };
 size_t fw_scan_blocks(long n);
 bool fw_flow_is_zero(const float* flow, long n);  // forward.rs:183-195 on a host flow
@@ -346,6 +356,12 @@ void launch_fw_flow_moves(hipStream_t s, const float* flow, long n, int* flag);
 void fw_grid(FwArgs& A, float amount);             // mesh.rs:11-31
 void launch_fw_count(hipStream_t s, const FwArgs& A);
 void launch_fw_emit_fold(hipStream_t s, const FwArgs& A, uint64_t records);
+// The textured variant: the mip pyramid's level count (pyramid.rs:20-35), the argument m of each
+// vertex's mip level d = -(logf(m) / ln 2) (mesh.rs:54-88; the logarithm is the host C runtime's,
+// as in the reference), and the pyramid levels 1.. of the source into lev (pyramid.rs:37-63).
+int fw_level_count(int w, int h);
+void launch_fw_mip_arg(hipStream_t s, const FwArgs& A, float* m);
+void launch_fw_pyramid(hipStream_t s, FwArgs& A, float* scratch);
 
 // Vector image coding (vectors.rs:57-130): one output image per job (blockIdx.z). The flow at
 // the working size is sampled at every output pixel; `finish` 1 writes the wire frame with the

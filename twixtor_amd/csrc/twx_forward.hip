@@ -24,6 +24,7 @@
 #include <string>
 
 #include "twx_common.hpp"
+#include "twx_pixel.hpp"
 
 namespace twx {
 
@@ -319,19 +320,13 @@ __device__ __forceinline__ void fw_vertices(const FwArgs& A, uint32_t t, FwV tri
     }
 }
 
-// mesh.rs:33-52: x' = fx * sx + c, y' = fy * sy + r, the flow sampled bilinearly at (c * rx,
-// r * ry) (Flow::bilerp, warp.rs:415-442); rows 1.. and columns 1.. add 0.0001 in double
-__global__ void k_fw_mesh(FwArgs A) {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x, r = blockIdx.y;
-    if (c >= A.W || r >= A.H) return;
-    const float cf = (float)c, rf = (float)r;
-    const float qx = cf * A.rx, qy = rf * A.ry;
+// Flow::bilerp (warp.rs:415-442) of the warp's flow at (qx, qy) inside [0, fw - 1] x [0, fh - 1]
+__device__ __forceinline__ void fw_flow_at(const FwArgs& A, float qx, float qy, float d[2]) {
     int32_t jx = fw_cvtt(qx), jy = fw_cvtt(qy);
     float gx = qx - (float)jx, gy = qy - (float)jy;
     if (jx >= A.fw - 1) { jx = A.fw - 2; gx = 1.0f; }
     if (jy >= A.fh - 1) { jy = A.fh - 2; gy = 1.0f; }
     const long f00 = ((long)jy * A.fw + jx) * 2, f01 = f00 + (long)A.fw * 2;
-    float d[2];
 #pragma unroll
     for (int k = 0; k < 2; k++) {
         const float p00 = A.flow[f00 + k], p10 = A.flow[f00 + 2 + k], p01 = A.flow[f01 + k], p11 = A.flow[f01 + 2 + k];
@@ -339,6 +334,16 @@ __global__ void k_fw_mesh(FwArgs A) {
         const float bot = (p11 - p01) * gx + p01;
         d[k] = (bot - top) * gy + top;
     }
+}
+
+// mesh.rs:33-52: x' = fx * sx + c, y' = fy * sy + r, the flow sampled bilinearly at (c * rx,
+// r * ry); rows 1.. and columns 1.. add 0.0001 in double
+__global__ void k_fw_mesh(FwArgs A) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x, r = blockIdx.y;
+    if (c >= A.W || r >= A.H) return;
+    const float cf = (float)c, rf = (float)r;
+    float d[2];
+    fw_flow_at(A, cf * A.rx, rf * A.ry, d);
     float px = d[0] * A.sx + cf, py = d[1] * A.sy + rf;
     if (r != 0 && c >= 1) {
         px = (float)((double)px + 0.0001);
@@ -346,12 +351,211 @@ __global__ void k_fw_mesh(FwArgs A) {
     }
     const long i = (long)r * A.W + c;
     A.pos[i] = make_float2(px, py);
-    if (A.fmt == TWX_FMT_F32) {
-        // a NaN colour at any vertex (every pixel is a vertex of some triangle when W, H >= 2)
+    if (A.fmt == TWX_FMT_F32 && !A.textured) {
+        // a NaN colour at any vertex (every pixel is a vertex of some triangle when W, H >= 2); the
+        // textured variant checks the colours it samples instead
         float col[4];
         fw_texel<2>(A.src, i, col);
         if (isnan(col[0]) || isnan(col[1]) || isnan(col[2]) || isnan(col[3])) atomicOr(A.flags, 4);
     }
+}
+
+// mesh.rs:54-88 up to the logarithm: the argument m of vertex (c, r)'s mip level, from the
+// distances of its 8 displaced neighbours (indices mirrored at the borders) to it: the diagonals
+// summed in single precision, widened and scaled by 1/sqrt(2), the axis neighbours added in double,
+// narrowed, times 0.125. The host takes d = -(logf(m) * (1 / logf(2))) + 0 with its C runtime's
+// logf, whose last bits a device logarithm would not reproduce.
+__global__ void k_fw_mip_arg(FwArgs A, float* __restrict__ m) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x, r = blockIdx.y;
+    if (c >= A.W || r >= A.H) return;
+    const int32_t w = A.W, h = A.H;
+    const int32_t up = r - 1 < 0 ? 1 : r - 1, down = h - 1 < r + 1 ? h - 2 : r + 1;
+    const int32_t left = c - 1 < 0 ? 1 : c - 1, right = w - 1 < c + 1 ? w - 2 : c + 1;
+    float f[2];
+    fw_flow_at(A, (float)c * A.rx, (float)r * A.ry, f);
+    const float xc = f[0] * A.sx + (float)c, yc = f[1] * A.sy + (float)r;
+    auto len = [&](int32_t ci, int32_t ri) {
+        float n[2];
+        fw_flow_at(A, (float)ci * A.rx, (float)ri * A.ry, n);
+        const float dx = (n[0] * A.sx + (float)ci) - xc, dy = (n[1] * A.sy + (float)ri) - yc;
+        return sqrtf(dy * dy + dx * dx);
+    };
+    const float ru = len(right, up), lu = len(left, up), ld = len(left, down), rd = len(right, down);
+    const float u = len(c, up), dn = len(c, down), l = len(left, r), rr = len(right, r);
+    const float diag = ((ru + lu) + ld) + rd;
+    double t = (double)diag * 0.7071067811865476;
+    t += (double)u;
+    t += (double)dn;
+    t += (double)l;
+    t += (double)rr;
+    m[(long)r * w + c] = (float)((double)(float)t * 0.125);
+}
+
+// Pixel i of a four-channel image of format FMT as floats.
+template <int FMT>
+__device__ __forceinline__ float4 fw_ld4(const void* img, long i) {
+    float o[4];
+    fw_texel<FMT>(img, i, o);
+    return make_float4(o[0], o[1], o[2], o[3]);
+}
+
+// The store of the pyramid's blur passes (filters.rs:112-119): trunc(v + 0.5), clamped, trunc
+// again for the integer formats; floats as they are.
+template <int FMT>
+__device__ __forceinline__ float fwp_store(float v) {
+    if (FMT == 2) return v;
+    const float mx = FMT == 0 ? 255.0f : 65535.0f;
+    float f = (float)fw_cvtt(v + 0.5f);
+    f = f > 0.0f ? f : 0.0f;
+    f = f < mx ? f : mx;
+    return (float)fw_cvtt(f + 0.5f);
+}
+
+// One output pixel of the pyramid's blur (pyramid.rs:126-177, the 1D filter of filters.rs with the
+// 5-tap binomial half kernel [0.375, 0.25, 0.0625]): `q[j]` for j = 0..4 is the line's pixel
+// i - 2 + j (only those inside the line are read); edge pixels renormalise over the taps inside
+// the line, a lane equal to the sentinel -1.0 carrying no weight.
+__device__ __forceinline__ float4 fwp_conv(const float4 q[5], int i, int n) {
+    const float S = -1.0f, k[3] = {0.375f, 0.25f, 0.0625f};
+    const int left = min(n, 3), mid = n > 6 ? n - 6 : 0;
+    const float4 c = q[2];
+    float acc[4] = {c.x * k[0], c.y * k[0], c.z * k[0], c.w * k[0]};
+    if (i >= left && i < left + mid) {
+#pragma unroll
+        for (int j = 2; j >= 1; j--) {
+            const float4 r = q[2 + j], l = q[2 - j];
+            acc[0] = acc[0] + (r.x + l.x) * k[j];
+            acc[1] = acc[1] + (r.y + l.y) * k[j];
+            acc[2] = acc[2] + (r.z + l.z) * k[j];
+            acc[3] = acc[3] + (r.w + l.w) * k[j];
+        }
+        return make_float4(acc[0], acc[1], acc[2], acc[3]);
+    }
+    float wsum[4] = {k[0], k[0], k[0], k[0]};
+#pragma unroll
+    for (int j = 2; j >= 1; j--) {
+        const float4 r4 = i + j < n ? q[2 + j] : make_float4(S, S, S, S);
+        const float4 l4 = i >= j ? q[2 - j] : make_float4(S, S, S, S);
+        const float r[4] = {r4.x, r4.y, r4.z, r4.w}, l[4] = {l4.x, l4.y, l4.z, l4.w};
+#pragma unroll
+        for (int a = 0; a < 4; a++) {
+            const float wr = (r[a] != S ? 1.0f : 0.0f) * k[j];
+            wsum[a] = wsum[a] + wr;
+            const float rt = wr * r[a];
+            const float wl = (l[a] != S ? 1.0f : 0.0f) * k[j];
+            const float lt = l[a] * wl;
+            wsum[a] = wsum[a] + wl;
+            acc[a] = (acc[a] + rt) + lt;
+        }
+    }
+    return make_float4(acc[0] / wsum[0], acc[1] / wsum[1], acc[2] / wsum[2], acc[3] / wsum[3]);
+}
+
+// The row pass of a level's blur: the previous level (format FMT) into `out` (the stored values
+// as floats).
+template <int FMT>
+__global__ void k_fwp_rows(const void* __restrict__ src, float4* __restrict__ out, int w, int h) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
+    if (x >= w || y >= h) return;
+    float4 q[5];
+#pragma unroll
+    for (int j = 0; j < 5; j++) {
+        const int xx = min(max(x - 2 + j, 0), w - 1);  // outside the line: never read by fwp_conv
+        q[j] = fw_ld4<FMT>(src, (long)y * w + xx);
+    }
+    const float4 v = fwp_conv(q, x, w);
+    out[(long)y * w + x] = make_float4(fwp_store<FMT>(v.x), fwp_store<FMT>(v.y), fwp_store<FMT>(v.z), fwp_store<FMT>(v.w));
+}
+
+// The column pass, on the row pass's result as a whole (pyramid.rs:185-194).
+template <int FMT>
+__global__ void k_fwp_cols(const float4* __restrict__ src, float4* __restrict__ out, int w, int h) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
+    if (x >= w || y >= h) return;
+    float4 q[5];
+#pragma unroll
+    for (int j = 0; j < 5; j++) {
+        const int yy = min(max(y - 2 + j, 0), h - 1);
+        q[j] = src[(long)yy * w + x];
+    }
+    const float4 v = fwp_conv(q, y, h);
+    out[(long)y * w + x] = make_float4(fwp_store<FMT>(v.x), fwp_store<FMT>(v.y), fwp_store<FMT>(v.z), fwp_store<FMT>(v.w));
+}
+
+// bilerp4 (warp.rs:306-331) of four floats per pixel: truncating conversion, the upper clamp
+// (ix >= w1 -> w1 - 1, fx = 1), the three lerps, and with a NaN among the results the same lerps
+// by the NaN rule. `bad` is set where the CPU would index below the image (a coordinate below 0
+// that does not truncate to 0, or NaN); the result is then zeros.
+template <class Load>
+__device__ __forceinline__ void fw_bilerp4(const Load& ld, int w, int w1, int h1, float x, float y, float o[4], bool& bad) {
+    int32_t ix = fw_cvtt(x), iy = fw_cvtt(y);
+    float fx = x - (float)ix, fy = y - (float)iy;
+    if (ix >= w1) { ix = w1 - 1; fx = 1.0f; }
+    if (iy >= h1) { iy = h1 - 1; fy = 1.0f; }
+    if (ix < 0 || iy < 0) {
+        bad = true;
+        o[0] = o[1] = o[2] = o[3] = 0.0f;
+        return;
+    }
+    const long i00 = (long)iy * w + ix, i01 = i00 + w;
+    const float4 a = ld(i00), b = ld(i00 + 1), c = ld(i01), d = ld(i01 + 1);
+    const float pa[4] = {a.x, a.y, a.z, a.w}, pb[4] = {b.x, b.y, b.z, b.w};
+    const float pc[4] = {c.x, c.y, c.z, c.w}, pd[4] = {d.x, d.y, d.z, d.w};
+    bool nan = false;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const float top = (pb[k] - pa[k]) * fx + pa[k];
+        const float bot = (pd[k] - pc[k]) * fx + pc[k];
+        o[k] = (bot - top) * fy + top;
+        nan = nan || isnan(o[k]);
+    }
+    if (nan) {
+#pragma unroll
+        for (int k = 0; k < 4; k++) o[k] = nr_lerp_bf(nr_lerp_bf(pa[k], pb[k], fx), nr_lerp_bf(pc[k], pd[k], fx), fy);
+    }
+}
+
+// pyramid.rs resample_plane: level (dw, dh) from the blurred previous level, corner aligned,
+// through bilerp4 and the warp's store (Pix::store)
+template <int FMT>
+__global__ void k_fwp_resample(const float4* __restrict__ src, int sw, int sh, void* __restrict__ dst, int dw, int dh,
+                               float sx, float sy) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
+    if (x >= dw || y >= dh) return;
+    float v[4];
+    bool bad = false;
+    fw_bilerp4([&](long i) { return src[i]; }, sw, sw - 1, sh - 1, (float)x * sx, (float)y * sy, v, bad);
+    typename FwPx<FMT>::T* o = reinterpret_cast<typename FwPx<FMT>::T*>(dst) + ((long)y * dw + x) * 4;
+#pragma unroll
+    for (int k = 0; k < 4; k++) o[k] = (typename FwPx<FMT>::T)store_rule<FMT>(v[k]);
+}
+
+// pyramid.rs:97-110: one bilinear tap of level i at (w1 * u, h1 * v)
+template <int FMT>
+__device__ __forceinline__ void fw_tap(const FwArgs& A, int i, float u, float v, float o[4], bool& bad) {
+    const int w1 = A.lw[i] - 1, h1 = A.lh[i] - 1;
+    const void* img = A.lev[i];
+    fw_bilerp4([&](long j) { return fw_ld4<FMT>(img, j); }, A.lw[i], w1, h1, (float)w1 * u, (float)h1 * v, o, bad);
+}
+
+// pyramid.rs:68-92 and forward.rs:130-141: the colour of a textured hit at (u, v) of mip level d
+template <int FMT>
+__device__ __forceinline__ void fw_sample(const FwArgs& A, float u, float v, float d, float o[4], bool& bad) {
+    if (0.0f < d) {
+        if (d < (float)(A.nlev - 1)) {
+            const int32_t i = fw_cvtt(d);
+            const float frac = d - (float)i;
+            float a[4], b[4];
+            fw_tap<FMT>(A, i, u, v, a, bad);
+            fw_tap<FMT>(A, i + 1, u, v, b, bad);
+#pragma unroll
+            for (int k = 0; k < 4; k++) o[k] = (b[k] - a[k]) * frac + a[k];
+            return;
+        }
+        fw_tap<FMT>(A, A.nlev - 1, u, v, o, bad);
+        return;
+    }
+    fw_tap<FMT>(A, 0, u, v, o, bad);
 }
 
 // Per-triangle work cap: a triangle that would cover more pixels stops and the warp is refused.
@@ -377,13 +581,28 @@ __global__ void k_fw_count(FwArgs A) {
     if (st == 2) atomicOr(A.flags, 2);
 }
 
-template <int FMT>
+// TEX: the textured variant, whose vertices carry (u, v, d) = (c / (w - 1), r / (h - 1), the
+// vertex's mip level) (forward.rs:271-289) and whose hits sample the pyramid there; else the
+// Gouraud variant, whose vertices carry their source pixel's colour.
+template <int FMT, bool TEX>
 __global__ void k_fw_emit(FwArgs A) {
     const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= A.ntri) return;
     FwV tri[3];
     fw_vertices(A, (uint32_t)t, tri);
-    auto tex = [&](uint32_t i, float o[4]) { fw_texel<FMT>(A.src, i, o); };
+    const float w1f = (float)(A.W - 1), h1f = (float)(A.H - 1);
+    auto tex = [&](uint32_t i, float o[4]) {
+        if (TEX) {
+            const uint32_t r = i / (uint32_t)A.W, c = i - r * (uint32_t)A.W;
+            o[0] = (float)(int32_t)c / w1f;
+            o[1] = (float)(int32_t)r / h1f;
+            o[2] = A.mips ? A.mips[i] : 0.0f;
+            o[3] = 0.0f;
+        } else {
+            fw_texel<FMT>(A.src, i, o);
+        }
+    };
+    bool bad = false, nanc = false;
     auto hit = [&](int32_t x, int32_t y, const float* p) -> bool {
         const long i = (long)y * A.W + x;
         const uint32_t k = atomicAdd(&A.fill[i], 1u);
@@ -393,10 +612,18 @@ __global__ void k_fw_emit(FwArgs A) {
         }
         const uint64_t slot = A.offsets[i] + k;
         A.rec_t[slot] = (uint32_t)t;
-        A.rec_c[slot] = make_float4(p[0], p[1], p[2], p[3]);
+        if (TEX) {
+            float c[4];
+            fw_sample<FMT>(A, p[0], p[1], p[2], c, bad);
+            nanc = nanc || isnan(c[0]) || isnan(c[1]) || isnan(c[2]) || isnan(c[3]);
+            A.rec_c[slot] = make_float4(c[0], c[1], c[2], c[3]);
+        } else {
+            A.rec_c[slot] = make_float4(p[0], p[1], p[2], p[3]);
+        }
         return true;
     };
     fw_triangle<true>(tri, A.W - 1, A.H - 1, hit, tex);
+    if (TEX && (bad || nanc)) atomicOr(A.flags, (bad ? 32 : 0) | (nanc ? 64 : 0));
 }
 
 // forward.rs:104-128
@@ -631,15 +858,64 @@ void launch_fw_emit_fold(hipStream_t s, const FwArgs& A, uint64_t records) {
     // back, the output and coverage written
     const double emit_bytes = 8.0 * px + 4.0 * ss * px + 20.0 * (double)records;
     const double fold_bytes = 12.0 * px + 20.0 * (double)records + 4.0 * ss * px + (A.cov ? px : 0.0);
+    const bool tx = A.textured != 0;
     if (A.fmt == TWX_FMT_U8) {
-        TWX_LAUNCH(s, F_FW_RASTER, emit_bytes, k_fw_emit<0><<<tg, blk, 0, s>>>(A));
+        if (tx) TWX_LAUNCH(s, F_FW_RASTER, emit_bytes, (k_fw_emit<0, true><<<tg, blk, 0, s>>>(A)));
+        else TWX_LAUNCH(s, F_FW_RASTER, emit_bytes, (k_fw_emit<0, false><<<tg, blk, 0, s>>>(A)));
         TWX_LAUNCH(s, F_FW_FOLD, fold_bytes, k_fw_fold<0><<<dim3((A.W + 255) / 256, A.H), blk, 0, s>>>(A));
     } else if (A.fmt == TWX_FMT_U16) {
-        TWX_LAUNCH(s, F_FW_RASTER, emit_bytes, k_fw_emit<1><<<tg, blk, 0, s>>>(A));
+        if (tx) TWX_LAUNCH(s, F_FW_RASTER, emit_bytes, (k_fw_emit<1, true><<<tg, blk, 0, s>>>(A)));
+        else TWX_LAUNCH(s, F_FW_RASTER, emit_bytes, (k_fw_emit<1, false><<<tg, blk, 0, s>>>(A)));
         TWX_LAUNCH(s, F_FW_FOLD, fold_bytes, k_fw_fold<1><<<dim3((A.W + 255) / 256, A.H), blk, 0, s>>>(A));
     } else {
-        TWX_LAUNCH(s, F_FW_RASTER, emit_bytes, k_fw_emit<2><<<tg, blk, 0, s>>>(A));
+        if (tx) TWX_LAUNCH(s, F_FW_RASTER, emit_bytes, (k_fw_emit<2, true><<<tg, blk, 0, s>>>(A)));
+        else TWX_LAUNCH(s, F_FW_RASTER, emit_bytes, (k_fw_emit<2, false><<<tg, blk, 0, s>>>(A)));
         TWX_LAUNCH(s, F_FW_FOLD, fold_bytes, k_fw_fold<2><<<dim3((A.W + 255) / 256, A.H), blk, 0, s>>>(A));
+    }
+}
+
+int fw_level_count(int w, int h) {
+    int n = 1;
+    int32_t a = std::max(w, 2), b = std::max(h, 2);
+    for (;;) {
+        a /= 2;
+        if (a < 2) break;
+        b /= 2;
+        if (b < 2) break;
+        n++;
+    }
+    return n;
+}
+
+void launch_fw_mip_arg(hipStream_t s, const FwArgs& A, float* m) {
+    const double px = (double)A.W * A.H;
+    TWX_LAUNCH(s, F_FW_MESH, 8.0 * A.fw * A.fh + 4.0 * px, k_fw_mip_arg<<<dim3((A.W + 255) / 256, A.H), dim3(256), 0, s>>>(A, m));
+}
+
+// pyramid.rs:37-63: level i from level i - 1, blurred (rows, then columns on the row result) and
+// resampled to the level's size; `scratch` holds two float4 planes of the source's size
+void launch_fw_pyramid(hipStream_t s, FwArgs& A, float* scratch) {
+    float4* t0 = reinterpret_cast<float4*>(scratch);
+    float4* t1 = t0 + (size_t)A.lw[0] * A.lh[0];
+    const dim3 blk(256);
+    for (int i = 1; i < A.nlev; i++) {
+        const int pw = A.lw[i - 1], ph = A.lh[i - 1], dw = A.lw[i], dh = A.lh[i];
+        const dim3 g((pw + 255) / 256, ph), gd((dw + 255) / 256, dh);
+        const float sx = ((float)pw - 1.0f) / (float)(dw - 1), sy = ((float)ph - 1.0f) / (float)(dh - 1);
+        const double bytes = (double)pw * ph * (4.0 * sample_size(A.fmt) + 16.0 * 3.0) + (double)dw * dh * 4.0 * sample_size(A.fmt);
+        if (A.fmt == TWX_FMT_U8) {
+            TWX_LAUNCH(s, F_FW_MESH, bytes, (k_fwp_rows<0><<<g, blk, 0, s>>>(A.lev[i - 1], t0, pw, ph)));
+            TWX_LAUNCH(s, F_FW_MESH, 0.0, (k_fwp_cols<0><<<g, blk, 0, s>>>(t0, t1, pw, ph)));
+            TWX_LAUNCH(s, F_FW_MESH, 0.0, (k_fwp_resample<0><<<gd, blk, 0, s>>>(t1, pw, ph, (void*)A.lev[i], dw, dh, sx, sy)));
+        } else if (A.fmt == TWX_FMT_U16) {
+            TWX_LAUNCH(s, F_FW_MESH, bytes, (k_fwp_rows<1><<<g, blk, 0, s>>>(A.lev[i - 1], t0, pw, ph)));
+            TWX_LAUNCH(s, F_FW_MESH, 0.0, (k_fwp_cols<1><<<g, blk, 0, s>>>(t0, t1, pw, ph)));
+            TWX_LAUNCH(s, F_FW_MESH, 0.0, (k_fwp_resample<1><<<gd, blk, 0, s>>>(t1, pw, ph, (void*)A.lev[i], dw, dh, sx, sy)));
+        } else {
+            TWX_LAUNCH(s, F_FW_MESH, bytes, (k_fwp_rows<2><<<g, blk, 0, s>>>(A.lev[i - 1], t0, pw, ph)));
+            TWX_LAUNCH(s, F_FW_MESH, 0.0, (k_fwp_cols<2><<<g, blk, 0, s>>>(t0, t1, pw, ph)));
+            TWX_LAUNCH(s, F_FW_MESH, 0.0, (k_fwp_resample<2><<<gd, blk, 0, s>>>(t1, pw, ph, (void*)A.lev[i], dw, dh, sx, sy)));
+        }
     }
 }
 
