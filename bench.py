@@ -48,6 +48,26 @@ WARP_FAMILIES = ("frames",)
 STAGE_TARGET = 0.60  # of the HBM peak, for the flow and the warp stages
 
 
+def copy_rate_gbs(torch, mib=1024, reps=10):
+    """The practical HBM ceiling of this device: a device-to-device copy of `mib` MiB (torch's copy
+    kernel), timed with HIP events over `reps` copies after a warm-up; read plus written bytes over
+    the time. Larger than the last-level cache (256 MB), so every byte crosses HBM."""
+    src = torch.ones(mib << 18, dtype=torch.float32, device="cuda:0")
+    dst = torch.empty_like(src)
+    for _ in range(2):
+        dst.copy_(src)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record()
+    for _ in range(reps):
+        dst.copy_(src)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    del src, dst
+    return 2.0 * (mib << 20) / (ms / 1000.0) / 1e9
+
+
 def step_model(params, n_out, select, W, H, bpp, w0, h0, levels, sensitivity):
     """The step-boundary byte model of one render (the CPU path's steps, each step's distinct
     inputs read once and outputs written once): per solved working pixel, level and direction 516
@@ -185,6 +205,7 @@ def main():
         prof = be.profile_read()
         be.profile(False)
 
+    copy_gbs = copy_rate_gbs(torch)
     # dominant kernel by HIP-event time over the timed steps
     kern = max(prof, key=lambda k: prof[k]["ms"]) if prof else None
     roofline = None
@@ -210,6 +231,10 @@ def main():
                             "achieved": round(traffic / (avg_ms / 1000.0) / 1e9, 1),
                             "frac": round(traffic / (avg_ms / 1000.0) / 1e9 / HBM_PEAK_GBS, 4),
                             "note": "FETCH_SIZE + WRITE_SIZE bytes (copy-calibrated) per launch"}},
+                    "copy_reference": {"GBs": round(copy_gbs, 1), "frac_of_peak": round(copy_gbs / HBM_PEAK_GBS, 4),
+                                       "achieved_over_copy": round(achieved / copy_gbs, 4),
+                                       "note": "a 1 GiB device-to-device copy on the same device, HIP events: the "
+                                               "bandwidth a plain streaming kernel reaches here"},
                     "valu_issue_share": VALU_SHARE.get(kern),
                     "limiter": "VALU issue (no FMA, no packed FP32 under the bit-exact contract)"
                                if VALU_SHARE.get(kern, 0.0) > achieved / HBM_PEAK_GBS else "HBM"}
@@ -228,12 +253,13 @@ def main():
     fw0, fh0, levels = flow_geometry(flow_settings(quality=4), spec.width, spec.height)
     flow_b, warp_b, n_pairs, n_vec = step_model(params, n_out, select, spec.width, spec.height, 16, fw0, fh0, levels,
                                                 0.7)
-    stages = {"target_frac": STAGE_TARGET}
+    stages = {"target_frac": STAGE_TARGET, "copy_GBs": round(copy_gbs, 1)}
     for name, fams, model in (("flow", FLOW_FAMILIES, flow_b), ("warp", WARP_FAMILIES, warp_b)):
         ms = sum(prof[f]["ms"] for f in fams if f in prof) / args.steps if prof else 0.0
         stages[name] = {"ms_per_step": round(ms, 3), "step_model_GB": round(model / 1e9, 3),
                         "GBs": round(model / (ms / 1000.0) / 1e9, 1) if ms > 0 else None,
                         "frac": round(model / (ms / 1000.0) / 1e9 / HBM_PEAK_GBS, 4) if ms > 0 else None,
+                        "over_copy": round(model / (ms / 1000.0) / 1e9 / copy_gbs, 4) if ms > 0 else None,
                         "families": list(fams)}
     stages["distinct_pairs"] = n_pairs
     stages["in_between_frames"] = n_vec

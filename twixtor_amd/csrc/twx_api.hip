@@ -482,14 +482,15 @@ static int coarse_top(const int* w, const int* h, int start) {
 int enqueue_solves(hipStream_t s, DevPool& pool, const std::vector<SolveIn>& in, const twx_flow_settings& fs,
                    uint64_t scratch_budget, int* states, std::string& err, bool raw) {
     if (in.empty()) return TWX_OK;
-    // chunks by the scratch bytes of each job: flow[1] 2 + grad 2 + ta 3 + tb 2 + tmp 3 floats/pixel
+    // chunks by the scratch bytes of each job: flow[0] 2 + grad 2 + tmp 3 + tb 2 + ta 3 + block-matching
+    // costs 2 floats/pixel
     std::vector<std::pair<size_t, size_t>> chunks;
     size_t max_px = 0;
     for (size_t i0 = 0; i0 < in.size();) {
         size_t i1 = i0, px_sum = 0;
         while (i1 < in.size()) {
             size_t px = (size_t)in[i1].geo.w0 * in[i1].geo.h0;
-            if (i1 > i0 && (px_sum + px) * 12 * sizeof(float) > scratch_budget) break;
+            if (i1 > i0 && (px_sum + px) * 14 * sizeof(float) > scratch_budget) break;
             px_sum += px;
             i1++;
         }
@@ -498,7 +499,7 @@ int enqueue_solves(hipStream_t s, DevPool& pool, const std::vector<SolveIn>& in,
         i0 = i1;
     }
     // one scratch set, reused by every chunk in stream order
-    float* scratch = pool.alloc<float>(max_px * 12);
+    float* scratch = pool.alloc<float>(max_px * 14);
     for (auto& ch : chunks) {
         int n = (int)(ch.second - ch.first);
         // Job order: area descending, in size buckets (one launch sequence each); within a bucket
@@ -592,12 +593,14 @@ int enqueue_solves(hipStream_t s, DevPool& pool, const std::vector<SolveIn>& in,
             J.tmp = sp + px * 4;  // 8-byte aligned (flow pixels are moved as float2 through it)
             J.tb = sp + px * 7;
             J.ta = sp + px * 9;
+            J.cost[0] = sp + px * 12;
+            J.cost[1] = sp + px * 13;
             if (owner[k] >= 0) {
                 J.grad = jobs[owner[k]].grad;
                 J.ta = jobs[owner[k]].ta;
                 J.tensor_shared = 1;
             }
-            sp += px * 12;
+            sp += px * 14;
             J.xt = tplane[si.X];
             J.yt = tplane[si.Y];
             J.state = states + (ch.first + k) * 4;

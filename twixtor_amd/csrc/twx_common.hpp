@@ -71,6 +71,9 @@ struct SolveJob {
     float* ta;             // structure tensor, 3 channels
     float* tb;             // right-hand side B, 2 channels
     float* tmp;            // block-matching scratch: the transposed flow (2 channels)
+    // block-matching costs carried between sweeps (one float per pixel each): the cost of every
+    // interior pixel's vector after a sweep, [0] in the row sweeps' walk layout, [1] plain
+    float* cost[2];
     int* state;            // [0] range check still active, [1] any pixel in range, [2] NaN seen
     // 1: grad and ta are those of an earlier job of the same launch with the same X, which
     // computes them (they depend on X alone, flow.rs:1274-1277); the tensor kernel skips this job

@@ -1171,6 +1171,16 @@ struct BmFlow {
     }
 };
 
+// A cost plane as a sweep reads or writes it: the same addressing as BmFlow with one float per pixel.
+struct BmCost {
+    gchar* base;
+    uint32_t step, lane;
+    __device__ BmCost(float* p, bool walk, int n_al, int m, int c)
+        : base((gchar*)p), step(walk ? 4u * (uint32_t)m : 4u), lane(walk ? 4u * (uint32_t)c : 4u * (uint32_t)c * (uint32_t)n_al) {}
+    __device__ float at(int a) const { return *(const gfloat*)(base + ((uint32_t)a * step + lane)); }
+    __device__ void put(int a, float v) const { *(gfloat_rw*)(base + ((uint32_t)a * step + lane)) = v; }
+};
+
 // The transposed copy of a frame's level for the row sweeps (element (x, y) at x * h + y): 32 x 32
 // tiles through LDS. One plane per distinct frame of the launch (blockIdx.z), which every job with
 // the frame as X or Y reads.
@@ -1198,13 +1208,14 @@ __global__ void k_plane_transpose(const PlaneT* __restrict__ ops, int level) {
 // The flow into its transposed scratch copy for the row sweeps (OUT false) and back (OUT true),
 // for levels whose row sweeps run on the transposed flow; a flow pixel (u, v) moves as one 8-byte
 // access (the scratch slice starts 8-byte aligned).
+// Back out, the carried costs move from the walk layout into the plain one as well.
 template <bool OUT>
 __global__ void k_bm_transpose(const SolveJob* __restrict__ jobs, int level, int buf) {
     const SolveJob& J = jobs[blockIdx.z];
     if (level > J.top) return;
     const int w = J.w[level], h = J.h[level];
     if (w < 4 || h < 4) return;
-    __shared__ float t[2][32][33];
+    __shared__ float t[OUT ? 3 : 2][32][33];
     const int bx = blockIdx.x * 32, by = blockIdx.y * 32;
     if (bx >= w || by >= h) return;
     const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 256 threads: 32 x 8
@@ -1231,12 +1242,16 @@ __global__ void k_bm_transpose(const SolveJob* __restrict__ jobs, int level, int
                 const float2 f = ft[(long)x * h + y];
                 t[0][tx][r] = f.x;
                 t[1][tx][r] = f.y;
+                t[OUT ? 2 : 0][tx][r] = J.cost[0][(long)x * h + y];
             }
         }
         __syncthreads();
         for (int r = ty; r < 32; r += 8) {
             const int x = bx + tx, y = by + r;
-            if (x < w && y < h) flow[(long)y * w + x] = make_float2(t[0][r][tx], t[1][r][tx]);
+            if (x < w && y < h) {
+                flow[(long)y * w + x] = make_float2(t[0][r][tx], t[1][r][tx]);
+                J.cost[1][(long)y * w + x] = t[OUT ? 2 : 0][r][tx];
+            }
         }
     }
 }
@@ -1263,15 +1278,28 @@ constexpr double BM_CROSS_MAX_PIXELS = 32e6;
 // lies in [0, w - 1] x [0, h - 1] holds wherever the own cost exists (that cost needs the target
 // in [1, w - 2] x [1, h - 2]). A cost that does not exist is replaced by -inf for the own vector
 // and +inf for the incoming one, so "both exist and incoming < own" is one comparison.
-template <bool T, int DIR, int PD, bool CROSS>
+//
+// Every sweep visits the same pixels (the interior), and a pixel's vector after a sweep is the one
+// whose cost that sweep computed last for it: the incoming vector's when the pixel took it, its own
+// otherwise. The cost is a function of the pixel and the vector alone, so that value is the own
+// cost the next sweep would compute. The second row sweep (a lane pair: role 0 the own vector,
+// role 1 the incoming one) stores it for every pixel (COST_OUT), and the column sweeps (CARRIED:
+// one lane per chain) read it instead of computing it, cost only the incoming vector, and the
+// first of them stores the new value where the pixel changes. (The row sweeps keep the lane pair:
+// their chains are fewer and longer, and with half the waves a carried row sweep ran slower.)
+template <bool T, int DIR, int PD, bool CROSS, bool CARRIED, bool COST_OUT>
 struct BmChain {
     static constexpr bool ALWAYS = T && CROSS;  // the sweep writes every pixel (it moves the flow between layouts)
+    // the costs are stored for every pixel: the first sweep, and a sweep that moves them between layouts
+    static constexpr bool COST_ALL = COST_OUT && (!CARRIED || ALWAYS);
     const BmPlane& B;
     gchar* A;        // X in walk layout
     uint32_t a_lane;  // byte offset of the chain's patch run, across (ce - 1) * 4
     uint32_t a_step;  // bytes per run (m * 4)
     const BmFlow& fr;
     const BmFlow& fw;
+    const BmCost& ccr;  // the carried costs (CARRIED)
+    const BmCost& ccw;  // where the costs go (COST_OUT)
     float pf[3];  // the prefetched words of the last step
     int m, n_al, n, ce;
     int role;
@@ -1279,19 +1307,29 @@ struct BmChain {
     float cf;     // (float)ce
     int a;
     float cu, cv, nu, nv;
+    float oc;     // CARRIED: the own cost of the current pixel (-inf where it does not exist)
     bool wr;      // the lane stores (role 0 of a live chain)
+    // COST_ALL: the costs of the last chunk, stored during the next chunk's first step after its
+    // loads (so that the in-order memory counter makes none of them wait for the stores) or after
+    // the chain's last chunk. (Deferring the flow's stores the same way made the sweeps slower.)
+    float pc[BM_CHUNK];
+    int pa = 0, pn = 0;
 
     __device__ gchar* arun(int r) const { return A + ((uint32_t)r * a_step + a_lane); }
 
     template <int CH, bool CHECK>
     __device__ __forceinline__ void chunk(int k0, float (&ring)[3][3]) {
         static_assert(CH % 3 == 0, "the ring's rotation repeats every three steps");
+        static_assert(CH == BM_CHUNK, "the pending costs hold one chunk");
+        const int a_first = a;
+        float cb[CH];
 #pragma unroll
         for (int st = 0; st < CH; st++) {
             const int k = k0 + st;
             if (CHECK && k >= n) break;
             const int an = k + 1 < n ? a + DIR : a;
             const float2 cn = fr.at(an);
+            const float ocn = CARRIED ? ccr.at(an) : 0.0f;
             // the run entering the patch at the next step (a + 2 DIR), clamped at the chain's end
             const int ar = min(max(an + DIR, 0), n_al - 1);
             float nr[3];
@@ -1309,8 +1347,9 @@ struct BmChain {
                 for (int aa = 0; aa < 3; aa++) p[bb * 3 + aa] = T ? ring[sl[aa]][bb] : ring[sl[bb]][aa];
             const float af = (float)a;
             const float xf = T ? af : cf, yf = T ? cf : af;
-            // role 0 costs the pixel's own vector, role 1 the incoming one
-            const float vu = role ? nu : cu, vv = role ? nv : cv;
+            // role 0 costs the pixel's own vector, role 1 (and every lane of a CARRIED sweep) the
+            // incoming one
+            const float vu = CARRIED || role ? nu : cu, vv = CARRIED || role ? nv : cv;
             // Every step touches new lines of A, B and the flow; a load that misses L2 costs the
             // step its full latency, since the step waits on its dependent B texels and the memory
             // counter retires loads in order. So each step asks, right after its own B loads, for
@@ -1319,6 +1358,7 @@ struct BmChain {
             // after that step's own wait has covered them, so they cost the chain no wait of their
             // own, and by their step the lines sit in L2.
             auto prefetch = [&]() {
+                if (COST_ALL && st == 0) flush();
                 if (PD == 0) return;
                 asm volatile("" ::"v"(pf[0]), "v"(pf[1]), "v"(pf[2]));
                 const int ap = a + PD * DIR;
@@ -1334,14 +1374,23 @@ struct BmChain {
             bool ok;
             const float c = bm_cost<T>(B, p, xf - vu, yf - vv, ok, prefetch);
             const float cs = ok ? c : sent;
-            // the partner lane's cost by a quad permutation [1, 0, 3, 2] (every lane active)
-            const float co = __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(cs), 0xB1, 0xF, 0xF, false));
-            // incoming < own: role 0 holds own, role 1 incoming
-            const float inc = role ? cs : co, own = role ? co : cs;
+            float inc, own;
+            if (CARRIED) {
+                inc = cs;
+                own = oc;
+            } else {
+                // the partner lane's cost by a quad permutation [1, 0, 3, 2] (every lane active)
+                const float co = __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(cs), 0xB1, 0xF, 0xF, false));
+                // incoming < own: role 0 holds own, role 1 incoming
+                inc = role ? cs : co;
+                own = role ? co : cs;
+            }
             const bool take = inc < own;
             nu = take ? nu : cu;
             nv = take ? nv : cv;
             if (wr && (ALWAYS || take)) fw.put(a, nu, nv);
+            if (COST_ALL) cb[st] = take ? inc : own;
+            else if (COST_OUT && wr && take) ccw.put(a, inc);
             // the run leaving the patch (run 0 for DIR = +1, run 2 for DIR = -1) makes room for the
             // entering one
 #pragma unroll
@@ -1349,7 +1398,22 @@ struct BmChain {
             a += DIR;
             cu = cn.x;
             cv = cn.y;
+            if (CARRIED) oc = ocn;
         }
+        if (COST_ALL) {
+#pragma unroll
+            for (int st = 0; st < CH; st++) pc[st] = cb[st];
+            pa = a_first;
+            pn = CHECK ? n - k0 : CH;
+        }
+    }
+    // the pending costs out (after the chain's last chunk: called once more)
+    __device__ __forceinline__ void flush() {
+        if (!COST_ALL || !wr) return;
+#pragma unroll
+        for (int i = 0; i < BM_CHUNK; i++)
+            if (i < pn) ccw.put(pa + i * DIR, pc[i]);
+        pn = 0;
     }
 };
 
@@ -1362,7 +1426,9 @@ struct BmChain {
 // 1 the incoming vector's, in one instruction stream, and both take the decision (Block::step,
 // flow.rs:528-544). The A patch slides along the chain, one new run per step; the loop runs in
 // unrolled chunks of CH steps so that the values loaded one step ahead need fewer register copies.
-template <bool T, int DIR, int CH, bool CROSS>
+// CARRIED (sweeps 3 and 4): one lane per chain, the own vectors' costs read from the previous
+// sweep (BmChain); COST_OUT: the sweep stores the costs of its pixels' vectors for the next one.
+template <bool T, int DIR, int CH, bool CROSS, bool CARRIED, bool COST_OUT>
 __global__ __launch_bounds__(64) void k_bm_sweep(const SolveJob* __restrict__ jobs, int level, int buf) {
     const SolveJob& J = jobs[blockIdx.z];
     if (level > J.top) return;
@@ -1370,13 +1436,14 @@ __global__ __launch_bounds__(64) void k_bm_sweep(const SolveJob* __restrict__ jo
     // with a side below 4 no cost exists (it needs 1 <= ix <= w - 3), so no pixel changes
     if (w < 4 || h < 4) return;
     const int n_al = T ? w : h, m = T ? h : w;
-    const int c0 = 1 + blockIdx.x * 32;
+    constexpr int CPB = CARRIED ? 64 : 32;  // chains per 64-lane block
+    const int c0 = 1 + blockIdx.x * CPB;
     if (c0 > m - 2) return;  // the whole workgroup is past the chains (uniform)
     const int n = n_al - 2;
     const int a0 = DIR > 0 ? 1 : n_al - 2;
     const int lane = threadIdx.x;
-    const int role = lane & 1;
-    const int cc = c0 + (lane >> 1);
+    const int role = CARRIED ? 1 : lane & 1;
+    const int cc = c0 + (CARRIED ? lane : lane >> 1);
     const bool live = cc <= m - 2;
     const int ce = live ? cc : m - 2;
     const BmPlane B(T ? J.yt : J.Y[level], w, h, n_al, m);
@@ -1388,6 +1455,12 @@ __global__ __launch_bounds__(64) void k_bm_sweep(const SolveJob* __restrict__ jo
     const BmFlow& fr = T && (DIR < 0 || !CROSS) ? scratch : plain;
     const BmFlow& fw = T && (DIR > 0 || !CROSS) ? scratch : plain;
     const BmFlow& fi = T && !CROSS ? scratch : plain;
+    // cost planes: [0] the rows' walk layout, [1] plain (walk layout for the columns). On levels
+    // whose rows run on the transposed flow, sweep 2 writes [0], the transposes back move it into
+    // [1], and sweeps 3 and 4 read [1] (3 updates it in place).
+    const BmCost cwalk(J.cost[0], true, n_al, m, ce), cplain(J.cost[1], !T, n_al, m, ce);
+    const BmCost& ccr = cplain;
+    const BmCost& ccw = T && !CROSS ? cwalk : cplain;
     // the incoming vector of the first pixel: the chain's end pixel, which no sweep changes
     const float2 f0 = fi.at(a0 - DIR), f1 = fr.at(a0);
     float nu = f0.x, nv = f0.y, cu = f1.x, cv = f1.y;
@@ -1403,12 +1476,14 @@ __global__ __launch_bounds__(64) void k_bm_sweep(const SolveJob* __restrict__ jo
     }
     asm volatile("" ::"v"(nu), "v"(nv), "v"(cu), "v"(cv), "v"(ring[0][0]), "v"(ring[0][1]), "v"(ring[0][2]),
                  "v"(ring[1][0]), "v"(ring[1][1]), "v"(ring[1][2]), "v"(ring[2][0]), "v"(ring[2][1]), "v"(ring[2][2]));
-    BmChain<T, DIR, T ? BM_PREFETCH_ROWS : BM_PREFETCH_COLS, CROSS> ch{
-        B, A, a_lane, a_step, fr, fw, {0.0f, 0.0f, 0.0f}, m, n_al, n, ce, role,
-        role ? INFINITY : -INFINITY, (float)ce, a0, cu, cv, nu, nv, live && role == 0};
+    const float oc = CARRIED ? ccr.at(a0) : 0.0f;
+    BmChain<T, DIR, T ? BM_PREFETCH_ROWS : BM_PREFETCH_COLS, CROSS, CARRIED, COST_OUT> ch{
+        B, A, a_lane, a_step, fr, fw, ccr, ccw, {0.0f, 0.0f, 0.0f}, m, n_al, n, ce, role,
+        role ? INFINITY : -INFINITY, (float)ce, a0, cu, cv, nu, nv, oc, live && (CARRIED || role == 0)};
     const int nfull = n / CH;
     for (int j = 0; j < nfull; j++) ch.template chunk<CH, false>(j * CH, ring);
     if (nfull * CH < n) ch.template chunk<CH, true>(nfull * CH, ring);
+    ch.flush();
     asm volatile("" ::"v"(ch.pf[0]), "v"(ch.pf[1]), "v"(ch.pf[2]));
 }
 
@@ -1893,8 +1968,9 @@ static void launch_solve_range(hipStream_t s, const SolveJob* d_jobs, const std:
     // block matching on the flow in buffer `buf`
     auto bm = [&](int l, int mw, int mh, double P, int buf) {
         dim3 tb(256), tg((mw + 31) / 32, (mh + 31) / 32, nj);
-        // a lane pair per chain: 32 chains per 64-lane block
-        dim3 cb(64), rg((mh + 31) / 32, 1, nj), cg((mw + 31) / 32, 1, nj);
+        // a lane pair per chain, 32 chains per 64-lane block; column sweeps with carried own costs:
+        // a lane per chain, 64 per block
+        dim3 cb(64), rg((mh + 31) / 32, 1, nj), cg((mw + 31) / 32, 1, nj), cg1((mw + 63) / 64, 1, nj);
         // the transposed X and Y of the level, once per distinct frame: 4 + 4 B per frame pixel
         int fw = 0, fh = 0;
         double Pf = 0;
@@ -1908,20 +1984,33 @@ static void launch_solve_range(hipStream_t s, const SolveJob* d_jobs, const std:
             TWX_LAUNCH(s, F_BM_TRANSPOSE, 8.0 * Pf,
                        (k_plane_transpose<<<dim3((fw + 31) / 32, (fh + 31) / 32, (unsigned)planes.size()), tb, 0, s>>>(
                            d_planes, l)));
-        // sweeps: A 4 + B 4 + flow read and write 16 B/px each
+        // sweeps: A 4 + B 4 + flow read and write 16 B/px each (the carried costs, 4 B/px written
+        // by sweeps 2 and 3 and read by sweeps 3 and 4, are layout work the CPU path does not have)
         if (P <= BM_CROSS_MAX_PIXELS) {
-            // the row sweeps move the flow between layouts
-            TWX_LAUNCH(s, F_BM_ROWS, 24.0 * P, (k_bm_sweep<true, 1, BM_CHUNK, true><<<rg, cb, 0, s>>>(d_jobs, l, buf)));
-            TWX_LAUNCH(s, F_BM_ROWS, 24.0 * P, (k_bm_sweep<true, -1, BM_CHUNK, true><<<rg, cb, 0, s>>>(d_jobs, l, buf)));
+            // the row sweeps move the flow between layouts; no costs are carried (storing them in
+            // the plain layout from a row sweep cost more than the column sweeps saved)
+            TWX_LAUNCH(s, F_BM_ROWS, 24.0 * P,
+                       (k_bm_sweep<true, 1, BM_CHUNK, true, false, false><<<rg, cb, 0, s>>>(d_jobs, l, buf)));
+            TWX_LAUNCH(s, F_BM_ROWS, 24.0 * P,
+                       (k_bm_sweep<true, -1, BM_CHUNK, true, false, false><<<rg, cb, 0, s>>>(d_jobs, l, buf)));
+            TWX_LAUNCH(s, F_BM_COLS, 24.0 * P,
+                       (k_bm_sweep<false, 1, BM_CHUNK, false, false, false><<<cg, cb, 0, s>>>(d_jobs, l, buf)));
+            TWX_LAUNCH(s, F_BM_COLS, 24.0 * P,
+                       (k_bm_sweep<false, -1, BM_CHUNK, false, false, false><<<cg, cb, 0, s>>>(d_jobs, l, buf)));
         } else {
-            // the transposed flow 8 + 8 B/px, and back 8 + 8 B/px
+            // the rows on the transposed flow (8 + 8 B/px there, and back 8 + 8 B/px with the costs
+            // 4 + 4); sweep 2 stores the costs, which the column sweeps carry
             TWX_LAUNCH(s, F_BM_TRANSPOSE, 16.0 * P, (k_bm_transpose<false><<<tg, tb, 0, s>>>(d_jobs, l, buf)));
-            TWX_LAUNCH(s, F_BM_ROWS, 24.0 * P, (k_bm_sweep<true, 1, BM_CHUNK, false><<<rg, cb, 0, s>>>(d_jobs, l, buf)));
-            TWX_LAUNCH(s, F_BM_ROWS, 24.0 * P, (k_bm_sweep<true, -1, BM_CHUNK, false><<<rg, cb, 0, s>>>(d_jobs, l, buf)));
+            TWX_LAUNCH(s, F_BM_ROWS, 24.0 * P,
+                       (k_bm_sweep<true, 1, BM_CHUNK, false, false, false><<<rg, cb, 0, s>>>(d_jobs, l, buf)));
+            TWX_LAUNCH(s, F_BM_ROWS, 24.0 * P,
+                       (k_bm_sweep<true, -1, BM_CHUNK, false, false, true><<<rg, cb, 0, s>>>(d_jobs, l, buf)));
             TWX_LAUNCH(s, F_BM_TRANSPOSE, 16.0 * P, (k_bm_transpose<true><<<tg, tb, 0, s>>>(d_jobs, l, buf)));
+            TWX_LAUNCH(s, F_BM_COLS, 24.0 * P,
+                       (k_bm_sweep<false, 1, BM_CHUNK, false, true, true><<<cg1, cb, 0, s>>>(d_jobs, l, buf)));
+            TWX_LAUNCH(s, F_BM_COLS, 24.0 * P,
+                       (k_bm_sweep<false, -1, BM_CHUNK, false, true, false><<<cg1, cb, 0, s>>>(d_jobs, l, buf)));
         }
-        TWX_LAUNCH(s, F_BM_COLS, 24.0 * P, (k_bm_sweep<false, 1, BM_CHUNK, false><<<cg, cb, 0, s>>>(d_jobs, l, buf)));
-        TWX_LAUNCH(s, F_BM_COLS, 24.0 * P, (k_bm_sweep<false, -1, BM_CHUNK, false><<<cg, cb, 0, s>>>(d_jobs, l, buf)));
     };
     for (int l = max_top; l >= 0; l--) {
         int mw = 0, mh = 0;
