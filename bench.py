@@ -42,7 +42,7 @@ VALU_SHARE = {"calc_b_gauss": 0.75, "frames": 0.67, "bm_rows": 0.58, "bm_cols": 
 # CPU path does not have.
 STEP_MODEL_FACTOR = {"tensor_gauss": 72.0 / 24.0, "calc_b_gauss": 64.0 / 32.0, "new_disp_gauss": 68.0 / 36.0,
                      "bm_transpose": 0.0}
-FLOW_FAMILIES = ("pyr_luma_blur5_band_resample", "tensor_gauss", "calc_b_gauss", "new_disp_gauss", "bm_rows", "bm_cols",
+FLOW_FAMILIES = ("ingest", "pyr_luma_blur5_band_resample", "tensor_gauss", "calc_b_gauss", "new_disp_gauss", "bm_rows", "bm_cols",
                  "bm_transpose", "range", "upsample", "negate", "solve_init", "coarse_solve")
 WARP_FAMILIES = ("frames",)
 STAGE_TARGET = 0.60  # of the HBM peak, for the flow and the warp stages
@@ -75,14 +75,16 @@ def step_model(params, n_out, select, W, H, bpp, w0, h0, levels, sensitivity):
     range check and damping or upsample 24), per solved working pixel, level and distinct X frame
     72 B (grad + tensor 24, tensor blur 48: they depend on X alone, and the device computes them
     once per frame where the CPU path repeats them per solve), per pyramid the frame read plus
-    every level written, per in-between frame both frames read, the output written and both flows
-    read, per exact frame one frame read and one written. Returns (flow bytes, warp bytes,
-    distinct pairs, in-between frames)."""
+    every level written, per source frame its ingest (the wire frame read, the engine image
+    written), per in-between frame both frames read, the output written and both flows read, per
+    exact frame one frame read and one written. Returns (flow bytes, warp bytes, distinct pairs,
+    in-between frames)."""
     import math
 
-    pairs, vector_frames, exact_frames = set(), 0, 0
+    pairs, vector_frames, exact_frames, sources = set(), 0, 0, set()
     for n in range(n_out):
         s = select(params, n)
+        sources.update((s.a,) if s.exact else (s.a, s.b))
         if s.exact:
             exact_frames += 1
         else:
@@ -98,7 +100,7 @@ def step_model(params, n_out, select, W, H, bpp, w0, h0, levels, sensitivity):
     frames_used = {f for p in pairs for f in p}
     # both directions of every pair are solved, so every frame of a pair is the X of a solve
     flow = (2 * len(pairs) * solved * 516 + len(frames_used) * solved * 72
-            + len(frames_used) * (W * H * bpp + 4 * sum(sizes)))
+            + len(frames_used) * (W * H * bpp + 4 * sum(sizes)) + len(sources) * 2 * W * H * bpp)
     warp = vector_frames * (3 * W * H * bpp + 2 * w0 * h0 * 8) + exact_frames * 2 * W * H * bpp
     return flow, warp, len(pairs), vector_frames
 

@@ -348,6 +348,11 @@ struct PyrIn {
     int W, H, fmt;
     Geo geo;
     float* levels;    // pyramid_floats(geo) floats
+    // or the frame's wire bytes (rows reversed into the engine image's order with `reverse`),
+    // read by the first step in place of img; `eng` (may be null) then receives the engine image
+    const uint8_t* wire = nullptr;
+    uint8_t* eng = nullptr;
+    int reverse = 0;
 };
 
 void build_pyramids(hipStream_t s, DevPool& pool, const std::vector<PyrIn>& in) {
@@ -394,12 +399,17 @@ void build_pyramids(hipStream_t s, DevPool& pool, const std::vector<PyrIn>& in) 
         // the buffer holding the plane of step k (reductions, then levels): A, B, A, ...
         auto cur = [&](int i, int k) { return (k % 2 == 0) ? bufA[i] : bufB[i]; };
         const PlaneOp empty{};
-        // a frame's first step reads its engine image and takes the luma there (prep.rs:168-235)
+        // a frame's first step reads its engine image, or its wire frame, and takes the luma there
+        // (prep.rs:168-235)
         auto source = [&](int i, int k, PlaneOp& op) {
             if (k == 0) {
-                op.src = in[c0 + i].img;
-                op.fmt = in[c0 + i].fmt;
+                const PyrIn& p = in[c0 + i];
+                op.src = p.img;
+                op.fmt = p.fmt;
                 op.luma = 1;
+                op.wire = p.wire;
+                op.eng = p.eng;
+                op.reverse = p.reverse;
             }
         };
         // reduction: R x (blur5, resample to the halved size)
@@ -436,7 +446,7 @@ void build_pyramids(hipStream_t s, DevPool& pool, const std::vector<PyrIn>& in) 
         // the fused kernel's tiling relies on
         for (size_t k = 0; k < steps.size(); k++)
             for (auto& op : steps[k])
-                if (op.src && op.dst && kinds[k] == 2) {
+                if ((op.src || op.wire) && op.dst && kinds[k] == 2) {
                     op.rx = ((float)op.sw - 1.0f) / (float)(op.dw - 1);
                     op.ry = ((float)op.sh - 1.0f) / (float)(op.dh - 1);
                     if (!(op.rx >= 2.0f && op.ry >= 2.0f)) throw HipError{"pyramid: a level that does not halve"};
@@ -910,28 +920,6 @@ int retime_batch_enqueue(twx_handle* h, DevPool& pool, const std::vector<ClipRun
         total_frames += R.count;
     }
 
-    // every source frame into its engine image (rows reversed for the motion path), largest first
-    {
-        std::vector<IngestOp> ops;
-        for (auto& sl : slots) {
-            const ClipRun& R = runs[sl.run];
-            const size_t fb = image_bytes(R.p.width, R.p.height, R.p.fmt);
-            IngestOp op{};
-            op.wire = sl.idx >= 0 ? R.clip + (size_t)sl.idx * fb : nullptr;
-            op.engine = sl.eng;
-            op.W = R.p.width;
-            op.H = R.p.height;
-            op.fmt = R.p.fmt;
-            op.zero = sl.idx < 0 ? 1 : 0;
-            op.reverse = 1;
-            ops.push_back(op);
-        }
-        std::stable_sort(ops.begin(), ops.end(), [](const IngestOp& a, const IngestOp& b) {
-            return (long)a.W * a.H > (long)b.W * b.H;
-        });
-        if (!ops.empty()) launch_ingest_batch(s, pool.upload(ops), ops);
-    }
-
     // pyramids of the frames the vector path solves on, and the unique solves; flow settings come
     // from the run (quality flag, sensitivity) and the frame's vectors quality
     std::map<std::pair<int, int>, int> pyr_of;  // (slot, vq) -> pyramid
@@ -1001,6 +989,42 @@ int retime_batch_enqueue(twx_handle* h, DevPool& pool, const std::vector<ClipRun
                 fsv[r][i].f_ab = solve(pl.sa, pl.sb, vq);
             }
         }
+    }
+    // A frame with a pyramid is read from the clip by its pyramids' first step, and the first of
+    // them writes its engine image (the ingest's bytes) on the way; every other source frame goes
+    // through the ingest (rows reversed for the motion path), largest first.
+    std::vector<char> fused(slots.size(), 0);
+    for (auto& [key, k] : pyr_of) {
+        const int sl = key.first;
+        const Slot& S = slots[sl];
+        if (S.idx < 0) continue;  // a transparent frame: the ingest writes its zeros
+        const ClipRun& R = runs[S.run];
+        pyrs[k].wire = R.clip + (size_t)S.idx * image_bytes(R.p.width, R.p.height, R.p.fmt);
+        pyrs[k].reverse = 1;
+        if (!fused[sl]) pyrs[k].eng = S.eng;
+        fused[sl] = 1;
+    }
+    {
+        std::vector<IngestOp> ops;
+        for (size_t k = 0; k < slots.size(); k++) {
+            const Slot& sl = slots[k];
+            if (fused[k]) continue;
+            const ClipRun& R = runs[sl.run];
+            const size_t fb = image_bytes(R.p.width, R.p.height, R.p.fmt);
+            IngestOp op{};
+            op.wire = sl.idx >= 0 ? R.clip + (size_t)sl.idx * fb : nullptr;
+            op.engine = sl.eng;
+            op.W = R.p.width;
+            op.H = R.p.height;
+            op.fmt = R.p.fmt;
+            op.zero = sl.idx < 0 ? 1 : 0;
+            op.reverse = 1;
+            ops.push_back(op);
+        }
+        std::stable_sort(ops.begin(), ops.end(), [](const IngestOp& a, const IngestOp& b) {
+            return (long)a.W * a.H > (long)b.W * b.H;
+        });
+        if (!ops.empty()) launch_ingest_batch(s, pool.upload(ops), ops);
     }
     {
         std::vector<PyrIn> sorted = pyrs;
@@ -1924,22 +1948,17 @@ int twx_create_vectors_dev(twx_handle* h, const twx_flow_settings* fs, int32_t w
         double t0 = now_ms();
         DevPool pool(h);
         const size_t fb = image_bytes(width, height, fmt);
-        // the engine images (flipped rows) and pyramids of the frames used, once each
+        // the pyramids of the frames used, once each, read from the clip (engine rows flipped, as
+        // the ingest would store them; nothing else needs the engine images)
         std::map<int, int> slot_of;
-        std::vector<IngestOp> ops;
         std::vector<PyrIn> pyrs;
         auto slot = [&](int idx) {
             auto it = slot_of.find(idx);
             if (it != slot_of.end()) return it->second;
-            IngestOp op{};
-            op.wire = (const uint8_t*)clip + (size_t)idx * fb;
-            op.engine = pool.alloc<uint8_t>(fb);
-            op.W = width;
-            op.H = height;
-            op.fmt = fmt;
-            op.reverse = 1;
-            ops.push_back(op);
-            pyrs.push_back(PyrIn{op.engine, width, height, fmt, g, pool.alloc<float>(pyramid_floats(g))});
+            PyrIn pi{nullptr, width, height, fmt, g, pool.alloc<float>(pyramid_floats(g))};
+            pi.wire = (const uint8_t*)clip + (size_t)idx * fb;
+            pi.reverse = 1;
+            pyrs.push_back(pi);
             int k = (int)pyrs.size() - 1;
             slot_of[idx] = k;
             return k;
@@ -1960,7 +1979,6 @@ int twx_create_vectors_dev(twx_handle* h, const twx_flow_settings* fs, int32_t w
             solve_for[k] = (int)solves.size() - 1;
             solve_of[key] = solve_for[k];
         }
-        if (!ops.empty()) launch_ingest_batch(s, pool.upload(ops), ops);
         build_pyramids(s, pool, pyrs);
         h->stats.pyramid_builds += pyrs.size();
         h->stats.pair_solves += solves.size();

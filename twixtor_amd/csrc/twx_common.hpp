@@ -50,6 +50,12 @@ struct PlaneOp {
     int luma;  // k_blur5_resample: src is an engine image whose luma is the plane
     float* aux;  // k_blur5_resample: where the band src - blur goes (null: none)
     float rx, ry;  // k_blur5_resample: (sw - 1) / (dw - 1) and (sh - 1) / (dh - 1), computed on the host
+    // k_blur5_resample, a frame's first step: the plane is the luma of the engine image of this
+    // wire frame (format fmt, sw x sh, rows reversed with `reverse`) instead of src; eng (may be
+    // null) receives that engine image, written by the tiles as the ingest would write it
+    const uint8_t* wire;
+    uint8_t* eng;
+    int reverse;
 };
 
 // One flow solve X -> Y over a pyramid pair (flow.rs:1258-1307). Device resident table entry;

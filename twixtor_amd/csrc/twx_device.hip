@@ -62,6 +62,38 @@ __device__ __forceinline__ void ldgn(const gfloat* base, uint32_t i, float out[N
 // -------------------------------------------------------------------------------------------
 // Wire -> engine image (image.rs:173-203, frame.rs:79-111, image.rs:264-298)
 
+// The engine pixel of wire pixel si (n: the frame's pixels, for the f32 planes).
+__device__ __forceinline__ uchar4 engine_px_u8(const uint8_t* __restrict__ wire, long si) {
+    uchar4 c = reinterpret_cast<const uchar4*>(wire)[si];
+    float a = (float)c.w * bits_f(0x3b808081u);
+    uchar4 o;
+    o.x = c.w;
+    o.y = (uint8_t)qround((float)c.x * a);
+    o.z = (uint8_t)qround((float)c.y * a);
+    o.w = (uint8_t)qround((float)c.z * a);
+    return o;
+}
+__device__ __forceinline__ ushort4 engine_px_u16(const uint8_t* __restrict__ wire, long si) {
+    ushort4 c = reinterpret_cast<const ushort4*>(wire)[si];
+    auto q15 = [](uint32_t v) -> uint32_t { return (v * 32768u + 32767u) / 65535u; };
+    uint32_t r = q15(c.x), g = q15(c.y), b = q15(c.z), al = q15(c.w);
+    float a = (float)al * bits_f(0x387fff00u);
+    ushort4 o;
+    o.x = (uint16_t)qround((float)al * bits_f(0x3fffff00u));
+    o.y = (uint16_t)qround((float)r * a);
+    o.z = (uint16_t)qround((float)g * a);
+    o.w = (uint16_t)qround((float)b * a);
+    return o;
+}
+__device__ __forceinline__ float4 engine_px_f32(const uint8_t* __restrict__ wire, long si, long n) {
+    const float* pl = reinterpret_cast<const float*>(wire);  // planes G, B, R, A
+    float g = pl[si], b = pl[n + si], r = pl[2 * n + si], a = pl[3 * n + si];
+    float4 o = make_float4(a, a * r, a * g, a * b);
+    if (isnan(o.x) || isnan(o.y) || isnan(o.z) || isnan(o.w))  // image.rs:291-300, the NaN rule
+        o = make_float4(a, nr_mul(a, r), nr_mul(a, g), nr_mul(a, b));
+    return o;
+}
+
 template <int FMT>
 __device__ __forceinline__ void ingest_pixel(const uint8_t* __restrict__ wire, uint8_t* __restrict__ engine, int W, int H,
                                              int x, int y, int reverse) {
@@ -69,34 +101,9 @@ __device__ __forceinline__ void ingest_pixel(const uint8_t* __restrict__ wire, u
     long si = (long)y * W + x;
     int yo = reverse ? H - 1 - y : y;
     long di = (long)yo * W + x;
-    if (FMT == TWX_FMT_U8) {
-        uchar4 c = reinterpret_cast<const uchar4*>(wire)[si];
-        float a = (float)c.w * bits_f(0x3b808081u);
-        uchar4 o;
-        o.x = c.w;
-        o.y = (uint8_t)qround((float)c.x * a);
-        o.z = (uint8_t)qround((float)c.y * a);
-        o.w = (uint8_t)qround((float)c.z * a);
-        reinterpret_cast<uchar4*>(engine)[di] = o;
-    } else if (FMT == TWX_FMT_U16) {
-        ushort4 c = reinterpret_cast<const ushort4*>(wire)[si];
-        auto q15 = [](uint32_t v) -> uint32_t { return (v * 32768u + 32767u) / 65535u; };
-        uint32_t r = q15(c.x), g = q15(c.y), b = q15(c.z), al = q15(c.w);
-        float a = (float)al * bits_f(0x387fff00u);
-        ushort4 o;
-        o.x = (uint16_t)qround((float)al * bits_f(0x3fffff00u));
-        o.y = (uint16_t)qround((float)r * a);
-        o.z = (uint16_t)qround((float)g * a);
-        o.w = (uint16_t)qround((float)b * a);
-        reinterpret_cast<ushort4*>(engine)[di] = o;
-    } else {
-        const float* pl = reinterpret_cast<const float*>(wire);  // planes G, B, R, A
-        float g = pl[si], b = pl[n + si], r = pl[2 * n + si], a = pl[3 * n + si];
-        float4 o = make_float4(a, a * r, a * g, a * b);
-        if (isnan(o.x) || isnan(o.y) || isnan(o.z) || isnan(o.w))  // image.rs:291-300, the NaN rule
-            o = make_float4(a, nr_mul(a, r), nr_mul(a, g), nr_mul(a, b));
-        reinterpret_cast<float4*>(engine)[di] = o;
-    }
+    if (FMT == TWX_FMT_U8) reinterpret_cast<uchar4*>(engine)[di] = engine_px_u8(wire, si);
+    else if (FMT == TWX_FMT_U16) reinterpret_cast<ushort4*>(engine)[di] = engine_px_u16(wire, si);
+    else reinterpret_cast<float4*>(engine)[di] = engine_px_f32(wire, si, n);
 }
 
 template <int FMT>
@@ -230,20 +237,43 @@ void prof_end(hipStream_t s, int fam, double bytes, hipEvent_t a) {
 // -------------------------------------------------------------------------------------------
 // Luma (prep.rs:168-235, retime path: no alpha, no video range)
 
-__device__ __forceinline__ float luma_at(const void* img, int fmt, long i) {
-    if (fmt == TWX_FMT_U8) {
-        uchar4 p = reinterpret_cast<const uchar4*>(img)[i];
-        float r = (float)p.y, g = (float)p.z, b = (float)p.w;
-        return (r * 0.3f + g * 0.59f) + b * 0.11f;
-    }
-    if (fmt == TWX_FMT_U16) {
-        ushort4 p = reinterpret_cast<const ushort4*>(img)[i];
-        double r = p.y, g = p.z, b = p.w;
-        return (float)(((g * 0.59 + r * 0.3) + b * 0.11) / 257.0);
-    }
-    float4 p = reinterpret_cast<const float4*>(img)[i];
+__device__ __forceinline__ float luma_px(uchar4 p) {
+    float r = (float)p.y, g = (float)p.z, b = (float)p.w;
+    return (r * 0.3f + g * 0.59f) + b * 0.11f;
+}
+__device__ __forceinline__ float luma_px(ushort4 p) {
+    double r = p.y, g = p.z, b = p.w;
+    return (float)(((g * 0.59 + r * 0.3) + b * 0.11) / 257.0);
+}
+__device__ __forceinline__ float luma_px(float4 p) {
     double r = p.y, g = p.z, b = p.w;
     return (float)(((g * 0.59 + r * 0.3) + b * 0.11) * 255.0);
+}
+__device__ __forceinline__ float luma_at(const void* img, int fmt, long i) {
+    if (fmt == TWX_FMT_U8) return luma_px(reinterpret_cast<const uchar4*>(img)[i]);
+    if (fmt == TWX_FMT_U16) return luma_px(reinterpret_cast<const ushort4*>(img)[i]);
+    return luma_px(reinterpret_cast<const float4*>(img)[i]);
+}
+
+// The luma of engine pixel (x, y) of a frame read from its wire bytes (the engine pixel as the
+// ingest makes it, rows reversed with `reverse`), stored into the engine image `eng` when that is
+// not null.
+__device__ __forceinline__ float wire_luma(const uint8_t* __restrict__ wire, uint8_t* __restrict__ eng, int fmt, int W,
+                                           int H, int reverse, int x, int y) {
+    const long si = (long)(reverse ? H - 1 - y : y) * W + x, di = (long)y * W + x;
+    if (fmt == TWX_FMT_U8) {
+        const uchar4 p = engine_px_u8(wire, si);
+        if (eng) reinterpret_cast<uchar4*>(eng)[di] = p;
+        return luma_px(p);
+    }
+    if (fmt == TWX_FMT_U16) {
+        const ushort4 p = engine_px_u16(wire, si);
+        if (eng) reinterpret_cast<ushort4*>(eng)[di] = p;
+        return luma_px(p);
+    }
+    const float4 p = engine_px_f32(wire, si, (long)W * H);
+    if (eng) reinterpret_cast<float4*>(eng)[di] = p;
+    return luma_px(p);
 }
 
 // -------------------------------------------------------------------------------------------
@@ -810,7 +840,7 @@ __device__ __forceinline__ bool sep_tile(const Load& load, int w, int h, int x0,
         const int ry = k / RW, rx = k - ry * RW;
         const int xx = x0 - R + rx, yy = y0 - R + ry;
         if (xx < 0 || xx >= w || yy < 0 || yy >= h) continue;
-        s_in[k] = load((long)yy * w + xx);
+        s_in[k] = load(xx, yy);
     }
     __syncthreads();
     const int nh = min(R + 1, w);
@@ -891,13 +921,20 @@ __global__ __launch_bounds__(TILE_THREADS) void k_blur5_resample(const PlaneOp* 
     if (x0 >= w || y0 >= h) return;
     __shared__ float s_in[(TILE_Y + 2 * R) * (TILE_X + 2 * R)];
     __shared__ float s_blur[TILE_Y * TILE_X];
-    // the plane: a float plane, or (op.luma) the luma of an engine image of format op.fmt
+    // the plane: a float plane, or (op.luma) the luma of an engine image of format op.fmt, or
+    // (op.wire) of the engine image of a wire frame, whose pixels in the tile's pitch the tile then
+    // stores into op.eng (when not null) as the ingest would
     const float* __restrict__ src = (const float*)op.src;
-    auto plain = [&](long i) { return src[i]; };
-    auto luma = [&](long i) { return luma_at(op.src, op.fmt, i); };
+    auto plain = [&](int xx, int yy) { return src[(long)yy * w + xx]; };
+    auto luma = [&](int xx, int yy) { return luma_at(op.src, op.fmt, (long)yy * w + xx); };
+    auto wire = [&](int xx, int yy) {
+        const bool own = xx >= x0 && xx < x0 + PYR_PX && yy >= y0 && yy < y0 + PYR_PY;
+        return wire_luma(op.wire, own ? op.eng : nullptr, op.fmt, w, h, op.reverse, xx, yy);
+    };
     float o[TILE_RUN];
     int x, ys;
-    const bool any = op.luma ? sep_tile<R, true>(luma, w, h, x0, y0, t, s_in, o, x, ys)
+    const bool any = op.wire ? sep_tile<R, true>(wire, w, h, x0, y0, t, s_in, o, x, ys)
+                   : op.luma ? sep_tile<R, true>(luma, w, h, x0, y0, t, s_in, o, x, ys)
                              : sep_tile<R, true>(plain, w, h, x0, y0, t, s_in, o, x, ys);
     if (any) {
 #pragma unroll
@@ -906,7 +943,9 @@ __global__ __launch_bounds__(TILE_THREADS) void k_blur5_resample(const PlaneOp* 
             s_blur[(ys + m - y0) * TILE_X + (x - x0)] = o[m];
             if (op.aux && x < x0 + PYR_PX && ys + m < y0 + PYR_PY) {
                 const long i = (long)(ys + m) * w + x;
-                op.aux[i] = (op.luma ? luma(i) : src[i]) - o[m];
+                const float v = op.wire ? wire_luma(op.wire, nullptr, op.fmt, w, h, op.reverse, x, ys + m)
+                                        : (op.luma ? luma(x, ys + m) : src[i]);
+                op.aux[i] = v - o[m];
             }
         }
     }
@@ -937,7 +976,8 @@ void launch_blur5_resample(hipStream_t s, const PlaneOp* d_ops, const std::vecto
     const long n = (long)tx * ty * (long)ops.size();
     double bytes = 0;
     for (auto& o : ops)
-        bytes += (double)o.sw * o.sh * ((o.luma ? 4.0 * sample_size(o.fmt) : 4.0) + (o.aux ? 4.0 : 0.0)) +
+        bytes += (double)o.sw * o.sh * ((o.luma || o.wire ? 4.0 * sample_size(o.fmt) : 4.0) + (o.aux ? 4.0 : 0.0) +
+                                        (o.eng ? 4.0 * sample_size(o.fmt) : 0.0)) +
                  (o.dst ? 4.0 * o.dw * o.dh : 0.0);
     TWX_LAUNCH(s, F_PYR_BLUR, bytes,
                k_blur5_resample<<<dim3((unsigned)(8 * ((n + 7) / 8))), dim3(TILE_THREADS), 0, s>>>(d_ops, t, tx, ty,
@@ -2997,7 +3037,7 @@ __global__ __launch_bounds__(TILE_THREADS) void k_fill_tile(const FillPlane* __r
     float* dst = push ? P.t1 : P.t2;
     float o[TILE_RUN];
     int x, ys;
-    auto load = [&](long i) { return src[i]; };
+    auto load = [&](int xx, int yy) { return src[(long)yy * w + xx]; };
     const bool any = P.above_first ? sep_tile<R, true>(load, w, h, x0, y0, t, s_in, o, x, ys)
                                    : sep_tile<R, false>(load, w, h, x0, y0, t, s_in, o, x, ys);
     if (!any) return;
