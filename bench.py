@@ -27,14 +27,18 @@ SPEED = "speed_percent=100@0;25@60;100@120"
 
 # Measured HBM bytes over algorithmic bytes per kernel family, from the FETCH_SIZE / WRITE_SIZE
 # counter runs (copy-calibrated) recorded in profiles/retime_1080p_hbm_traffic_pmc.md.
-TRAFFIC_RATIO = {"frames": 1.03, "new_disp_gauss": 1.00, "calc_b_gauss": 1.03, "bm_cols": 1.26, "bm_rows": 1.22,
-                 "ingest": 1.00, "pyr_luma_blur5_band_resample": 1.08, "bm_transpose": 1.09, "upsample": 1.27,
+TRAFFIC_RATIO = {"frames": 1.03, "new_disp_gauss": 1.02, "calc_b_gauss": 1.03, "bm_cols": 1.23, "bm_rows": 1.30,
+                 "ingest": 1.00, "pyr_luma_blur5_band_resample": 1.08, "bm_transpose": 1.39, "upsample": 1.27,
                  "tensor_gauss": 1.00}
 # The share of the SIMDs' issue slots the family's VALU instructions fill over its time, from the
 # SQ_INSTS_VALU counter run recorded in the same profile.
-VALU_SHARE = {"calc_b_gauss": 0.75, "frames": 0.67, "bm_rows": 0.58, "bm_cols": 0.61, "new_disp_gauss": 0.58,
-              "pyr_luma_blur5_band_resample": 0.53, "tensor_gauss": 0.65, "bm_transpose": 0.23, "upsample": 0.38,
-              "ingest": 0.10}
+VALU_SHARE = {"calc_b_gauss": 0.74, "frames": 0.67, "bm_rows": 0.55, "bm_cols": 0.49, "new_disp_gauss": 0.60,
+              "pyr_luma_blur5_band_resample": 0.43, "tensor_gauss": 0.65, "bm_transpose": 0.19, "upsample": 0.37,
+              "ingest": 0.06}
+# What bounds a family where neither the HBM fraction nor the VALU share says it: the block-matching
+# sweeps are serial chains, each step's decision feeding the next step's texel gathers.
+LIMITER = {"bm_rows": "serial chain latency (each step's decision feeds the next step's texel gathers)",
+           "bm_cols": "serial chain latency (each step's decision feeds the next step's texel gathers)"}
 # The library counts a fused kernel's bytes once (its inputs read once, its outputs written once).
 # The step-boundary model of the CPU path counts every step's inputs and outputs, 4 bytes a float:
 # grad + tensor 24 and the tensor blur 48 B/px (fused: 24), calc_b 32 and the B blur 32 (fused: 32),
@@ -238,8 +242,8 @@ def main():
                                        "note": "a 1 GiB device-to-device copy on the same device, HIP events: the "
                                                "bandwidth a plain streaming kernel reaches here"},
                     "valu_issue_share": VALU_SHARE.get(kern),
-                    "limiter": "VALU issue (no FMA, no packed FP32 under the bit-exact contract)"
-                               if VALU_SHARE.get(kern, 0.0) > achieved / HBM_PEAK_GBS else "HBM"}
+                    "limiter": LIMITER.get(kern) or ("VALU issue (no FMA, no packed FP32 under the bit-exact contract)"
+                                                     if VALU_SHARE.get(kern, 0.0) > achieved / HBM_PEAK_GBS else "HBM")}
     # whole-path algorithmic bytes per step over the step time
     total_bytes = sum(v["bytes"] for v in prof.values()) / args.steps if prof else 0.0
     path = {"algorithmic_GB_per_step": round(total_bytes / 1e9, 3),
