@@ -108,6 +108,12 @@ typedef struct twx_vin_params {
     double vec_scale_x, vec_scale_y;
     double blur;                 /* "Motion Blur Compensation"; non-zero is refused */
     int32_t quality_high;
+    /* The engine stage set of the render: 0 the effect's own (the Gouraud forward warp, hole-fill
+     * taps above first, frames turned upside down: plugins/vectors-in/src/glue.rs:57-75, :253,
+     * :398-410); 1 the engine's plain set on rows as given (the textured forward warp, taps below
+     * first: stages.rs), the configuration the reference's recorded Vectors In grid outputs
+     * (parity/vectors-in results/base.grid.tsv) were made with. */
+    int32_t plain_stages;
 } twx_vin_params;
 
 /* The frame selection of one output frame (plugins/retime-engine/src/timeline.rs:359-380). */

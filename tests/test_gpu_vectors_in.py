@@ -1,5 +1,6 @@
 """The Vectors In render on the device (twx_vectors_in_frames, twx_vectors_in_frames_dev) against
-the oracle's, byte for byte, over the reference's Vectors In parity grid.
+the oracle's, byte for byte, over the reference's Vectors In parity grid, and with the engine's
+plain stage set against the grid's recorded output hashes directly.
 
 Every graded case the render serves (no FG matte layers, no blur compensation, no keyframes but
 Speed %'s: sizes 2x2 to 64x48 at three depths, Speed and Frame Number, input rates, segment marks,
@@ -53,6 +54,47 @@ def test_every_served_grid_case_equals_the_oracle(backend):
     # refusals only where the samples are not finite
     assert all(".sp" in r or "spsrc" in r or "spvec" in r for r in refused), refused
     assert same >= 350, (same, len(refused))
+
+
+def test_plain_stages_give_the_recorded_hashes(backend):
+    """With the engine's plain stage set (the textured forward warp, hole-fill taps below first,
+    rows as given) the device renders the graded cases to the reference's own recorded outputs:
+    each case's SHA-256 against results/base.grid.tsv, the device checked against the reference's
+    bytes with no oracle in between (the six frames past the end of short vector layers, whose
+    handling changed after the recording, against the oracle). A case may instead be refused with
+    status 3 only where its samples are not finite (the forward warp's NaN colour rule is not
+    restated on the device)."""
+    from test_oracle_vectors_in import PAST_LAYER_END, vin_cases, vin_clip, vin_frames_of, vin_layers, vin_served
+
+    from twixtor_amd._ffi import TWX_E_UNSUPPORTED, StageError
+
+    hashed, refused, other = 0, [], []
+    for c in vin_cases():
+        cfg = vin_served(c)
+        if cfg is None:
+            continue
+        cfg.stages = "plain"
+        src, prev, nxt = vin_layers(c)
+        clip, n_in = vin_clip(c["input"]), vin_frames_of(c["input"])
+        try:
+            got = backend.vectors_in_frames(cfg, clip, n_in, src, prev, nxt, 0, c["frames"])
+        except StageError as e:
+            assert e.status == TWX_E_UNSUPPORTED, (c["id"], e)
+            refused.append(c["id"])
+            continue
+        if c["id"] in PAST_LAYER_END:
+            want = cpuref.vectors_in_frames(cfg.struct(), clip, n_in, src, prev, nxt, 0, c["frames"],
+                                            stages=cpuref.VIN_PLAIN)
+            if got != want:
+                other.append((c["id"], "oracle bytes"))
+        elif hashlib.sha256(got).hexdigest() == c["sha256"]:
+            hashed += 1
+        else:
+            other.append((c["id"], "recorded hash"))
+    print(f"plain stages: {hashed} recorded hashes, {len(refused)} refused", flush=True)
+    assert not other, other[:10]
+    assert all(".sp" in r or "spsrc" in r or "spvec" in r for r in refused), refused
+    assert hashed + len(refused) + len(PAST_LAYER_END) >= 392 and hashed >= 350, (hashed, len(refused))
 
 
 def test_device_pointers_equal_host_pointers_and_frame_ranges(backend):

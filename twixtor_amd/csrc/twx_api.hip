@@ -2111,6 +2111,10 @@ static int vectors_in_enqueue(twx_handle* h, DevPool& pool, const twx_vin_params
         return TWX_OK;
     }
     const VinClip& layer = source.wire ? source : input;  // the retimed layer (glue.rs:86-89)
+    // the stage set: the effect's (glue.rs:57-75: Gouraud forward warp, taps above first, every
+    // frame turned upside down and back) or the engine's plain set on rows as given
+    const bool plain = p.plain_stages != 0;
+    const bool gouraud = !plain, above = !plain, flip = !plain;
     const twx_retime_params rp = timeline::vin_speed_params(p);
     timeline::SpeedMap map(rp, r);
     // the frames each output frame reads: (layer 0 retimed / 1 Vectors: Prev / 2 Vectors: Next, clip
@@ -2129,7 +2133,7 @@ static int vectors_in_enqueue(twx_handle* h, DevPool& pool, const twx_vin_params
         op.H = H;
         op.fmt = fmt;
         op.zero = idx < 0 ? 1 : 0;
-        op.reverse = 1;
+        op.reverse = flip ? 1 : 0;
         ops.push_back(op);
         const int k = (int)ops.size() - 1;
         slot_of[{lid, idx}] = k;
@@ -2181,7 +2185,7 @@ static int vectors_in_enqueue(twx_handle* h, DevPool& pool, const twx_vin_params
         auto it = flow_of.find(vslot);
         if (it != flow_of.end()) return it->second;
         float* f = pool.alloc<float>(npx * 2);
-        int st = flow_from_vectors_dev(s, pool, ops[vslot].engine, W, H, fmt, md, vsx, vsy, true, f, err);
+        int st = flow_from_vectors_dev(s, pool, ops[vslot].engine, W, H, fmt, md, vsx, vsy, above, f, err);
         if (st) throw st;
         flow_of[vslot] = f;
         return f;
@@ -2192,7 +2196,7 @@ static int vectors_in_enqueue(twx_handle* h, DevPool& pool, const twx_vin_params
         auto it = inverse_of.find(vslot);
         if (it != inverse_of.end()) return it->second;
         float* f = pool.alloc<float>(npx * 2);
-        int st = invert_flow_dev(h, pool, flow(vslot), nullptr, W, H, true, false, f, fw_flags, err);
+        int st = invert_flow_dev(h, pool, flow(vslot), nullptr, W, H, above, !gouraud, f, fw_flags, err);
         if (st) throw st;
         inverse_of[vslot] = f;
         return f;
@@ -2210,7 +2214,7 @@ static int vectors_in_enqueue(twx_handle* h, DevPool& pool, const twx_vin_params
             F.fmt = fmt;
             F.W = W;
             F.H = H;
-            F.finish = 1;
+            F.finish = flip ? 1 : 2;
             if (pl.kind == 0) {  // the frame through the prepare and the finish
                 F.kind = 0;
                 F.blend_mode = 0;
@@ -2257,7 +2261,7 @@ static int vectors_in_enqueue(twx_handle* h, DevPool& pool, const twx_vin_params
                     void* dst = pool.alloc<uint8_t>(fb);
                     uint8_t* cov = pool.alloc<uint8_t>(npx);
                     st = forward_warp_dev(h, pool, from_a ? F.a : F.b, W, H, fmt, from_a ? f_ab : f_ba, nullptr, W, H,
-                                          from_a ? F.frac : F.wa, false, dst, cov, fw_flags, err);
+                                          from_a ? F.frac : F.wa, !gouraud, dst, cov, fw_flags, err);
                     F.pre_a = dst;
                     F.pre_cov_a = cov;
                 } else {
@@ -2265,18 +2269,18 @@ static int vectors_in_enqueue(twx_handle* h, DevPool& pool, const twx_vin_params
                     void* tb = pool.alloc<uint8_t>(fb);
                     uint8_t* c1 = pool.alloc<uint8_t>(npx);
                     uint8_t* c2 = pool.alloc<uint8_t>(npx);
-                    st = forward_warp_dev(h, pool, F.a, W, H, fmt, f_ab, nullptr, W, H, F.frac, false, ta, c1, fw_flags,
-                                          err);
+                    st = forward_warp_dev(h, pool, F.a, W, H, fmt, f_ab, nullptr, W, H, F.frac, !gouraud, ta, c1,
+                                          fw_flags, err);
                     if (!st)
-                        st = forward_warp_dev(h, pool, F.b, W, H, fmt, f_ba, nullptr, W, H, F.wa, false, tb, c2, fw_flags,
-                                              err);
+                        st = forward_warp_dev(h, pool, F.b, W, H, fmt, f_ba, nullptr, W, H, F.wa, !gouraud, tb, c2,
+                                              fw_flags, err);
                     F.pre_a = ta;
                     F.pre_b = tb;
                     F.pre_cov_a = c1;
                     F.pre_cov_b = c2;
                 }
                 if (st) return st;
-                setup_smart(pool, F, interp, true, bad + 1 + i);
+                setup_smart(pool, F, interp, above, bad + 1 + i);
             } else if (interp == 0) {
                 F.kind = 1;
                 F.nearest_from_a = 0.5f <= pl.sel.wa ? 1 : 0;
@@ -2284,7 +2288,7 @@ static int vectors_in_enqueue(twx_handle* h, DevPool& pool, const twx_vin_params
                 F.kind = 2;
                 blend_branch(F.frac, F);
             } else {
-                setup_weighted(pool, F, true);
+                setup_weighted(pool, F, above);
             }
             jobs.push_back(F);
         }

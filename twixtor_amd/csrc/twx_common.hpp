@@ -118,7 +118,7 @@ struct FrameJob {
     int warp_b_copy;
     int nearest_from_a;    // Nearest: warp A with F_BA (wa >= 0.5), else warp B with F_AB
     int plus_zero;         // weighted blend without flows on float images: inputs + 0.0 first
-    int finish;            // 1: unpremultiply into the wire frame, rows turned back; 0: engine pixel as is
+    int finish;            // 1: unpremultiply into the wire frame, rows turned back; 2: the same on rows as given; 0: engine pixel as is
     // 1: f_ab and f_ba are the solver's flows before its final negation (flow.rs:1305), which the
     // warps take into account by subtracting the scaled flow sample from the pixel position
     int flow_raw;

@@ -2569,8 +2569,8 @@ __global__ void k_frames(const FrameJob* __restrict__ jobs) {
         for (int k = 0; k < 4; k++) o[k] = (typename Px<FMT>::T)e[k];
         return;
     }
-    // the composed image is in motion rows: row y is the output's row H-1-y
-    long oi = (long)(F.H - 1 - y) * F.W + x;
+    // the composed image is in motion rows: row y is the output's row H-1-y (finish 2: rows as given)
+    long oi = (long)(F.finish == 2 ? y : F.H - 1 - y) * F.W + x;
     finish_pixel<FMT>(F.out, n, oi, e);
 }
 
@@ -2595,7 +2595,7 @@ __global__ void k_frames_blend(const FrameJob* __restrict__ jobs) {
         for (int k = 0; k < 4; k++) o[k] = (typename Px<FMT>::T)e[k];
         return;
     }
-    finish_pixel<FMT>(F.out, n, (long)(F.H - 1 - y) * F.W + x, e);
+    finish_pixel<FMT>(F.out, n, (long)(F.finish == 2 ? y : F.H - 1 - y) * F.W + x, e);
 }
 
 void frame_job_ratios(FrameJob& F) {
@@ -2913,7 +2913,7 @@ __global__ __launch_bounds__(256) void k_weighted(const FrameJob* __restrict__ j
             typename Px<FMT>::T* o = reinterpret_cast<typename Px<FMT>::T*>(F.out) + mi * 4;
             for (int k = 0; k < 4; k++) o[k] = (typename Px<FMT>::T)e[k];
         } else {
-            finish_pixel<FMT>(F.out, n, (long)(H - 1 - gy) * W + gx, e);
+            finish_pixel<FMT>(F.out, n, (long)(F.finish == 2 ? gy : H - 1 - gy) * W + gx, e);
         }
     }
 }
@@ -3285,7 +3285,7 @@ __global__ void k_finish(const FrameJob* __restrict__ jobs) {
     const long mi = (long)y * F.W + x;
     float e[4];
     load4<FMT>(F.comp, mi, e);
-    finish_pixel<FMT>(F.out, (long)F.W * F.H, (long)(F.H - 1 - y) * F.W + x, e);
+    finish_pixel<FMT>(F.out, (long)F.W * F.H, (long)(F.finish == 2 ? y : F.H - 1 - y) * F.W + x, e);
 }
 
 static void launch_finish_range(hipStream_t s, const FrameJob* d_jobs, const std::vector<FrameJob>& jobs);

@@ -178,6 +178,7 @@ class VinParams(ctypes.Structure):
         ("vec_scale_y", ctypes.c_double),
         ("blur", ctypes.c_double),
         ("quality_high", ctypes.c_int32),
+        ("plain_stages", ctypes.c_int32),
     ]
 
 
@@ -202,6 +203,9 @@ class VectorsIn:
     fps: str = "30"
     params: list = field(default_factory=list)
     quality: str = "high"
+    # "effect": the effect's own stage set (Gouraud forward warp, taps above first, rows flipped);
+    # "plain": the engine's plain stage set on rows as given (textured forward warp, taps below first)
+    stages: str = "effect"
 
     def __post_init__(self):
         self.time_scale, self.time_step, self.fps_f64 = time_base(self.fps)
@@ -239,6 +243,7 @@ class VectorsIn:
         p.max_displace, p.vec_scale_x, p.vec_scale_y = v["vectors_max_displace"], v["vec_scale_x"], v["vec_scale_y"]
         p.blur = v["blur_compensation"]
         p.quality_high = 1 if self.quality == "high" else 0
+        p.plain_stages = {"effect": 0, "plain": 1}[self.stages]
         if self.keys:
             arr = (Key * len(self.keys))(*[Key(t, x, h, e) for t, x, h, e in self.keys])
             p.n_speed_keys = len(self.keys)
