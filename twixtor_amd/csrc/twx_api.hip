@@ -1581,22 +1581,24 @@ static int forward_warp_dev(twx_handle* h, DevPool& pool, const void* src, int W
     hipStream_t s = h->stream;
     const float a = 0.0f > amount ? -amount : amount;
     bool still = 1e-05 > (double)a;
-    if (!still && h_flow) {
-        still = fw_flow_is_zero(h_flow, (long)fw * fh * 2);
-    } else if (!still) {  // no host copy of the flow: the same test on the device
-        int* moves = pool.alloc<int>(1);
-        int* hm = pool.pinned<int>(1);
-        launch_zero(s, moves, 1);
-        launch_fw_flow_moves(s, d_flow, (long)fw * fh * 2, moves);
-        HIPCHECK(hipMemcpyAsync(hm, moves, sizeof(int), hipMemcpyDeviceToHost, s));
-        HIPCHECK(hipStreamSynchronize(s));
-        still = *hm == 0;
-    }
-    if (still) {
+    auto copy_still = [&]() {
         HIPCHECK(hipMemcpyAsync(out, src, image_bytes(W, H, fmt), hipMemcpyDeviceToDevice, s));
         if (cov) HIPCHECK(hipMemsetAsync(cov, 0xff, (size_t)W * H, s));
         return TWX_OK;
+    };
+    // no host copy of the flow: the same test on the device, read at the warp's first synchronisation
+    // (the passes enqueued before it write only this warp's own buffers)
+    int* hm = nullptr;
+    if (!still && h_flow) {
+        still = fw_flow_is_zero(h_flow, (long)fw * fh * 2);
+    } else if (!still) {
+        int* moves = pool.alloc<int>(1);
+        hm = pool.pinned<int>(1);
+        launch_zero(s, moves, 1);
+        launch_fw_flow_moves(s, d_flow, (long)fw * fh * 2, moves);
+        HIPCHECK(hipMemcpyAsync(hm, moves, sizeof(int), hipMemcpyDeviceToHost, s));
     }
+    if (still) return copy_still();
     if ((long)W * H > (long)UINT32_MAX) {  // the rasteriser's vertices carry 32-bit pixel indices
         err = "forward warp: an image of more than 2^32 - 1 pixels";
         return TWX_E_UNSUPPORTED;
@@ -1637,12 +1639,13 @@ static int forward_warp_dev(twx_handle* h, DevPool& pool, const void* src, int W
         if (A.nlev > 1) {
             launch_fw_pyramid(s, A, pool.alloc<float>((size_t)n * 8));
             float* m = pool.alloc<float>(n);
-            float* hm = pool.pinned<float>(n);
+            float* hmip = pool.pinned<float>(n);
             launch_fw_mip_arg(s, A, m);
-            HIPCHECK(hipMemcpyAsync(hm, m, n * sizeof(float), hipMemcpyDeviceToHost, s));
+            HIPCHECK(hipMemcpyAsync(hmip, m, n * sizeof(float), hipMemcpyDeviceToHost, s));
             HIPCHECK(hipStreamSynchronize(s));
-            fw_mip_levels_host(hm, n);
-            HIPCHECK(hipMemcpyAsync(m, hm, n * sizeof(float), hipMemcpyHostToDevice, s));
+            if (hm && *hm == 0) return copy_still();
+            fw_mip_levels_host(hmip, n);
+            HIPCHECK(hipMemcpyAsync(m, hmip, n * sizeof(float), hipMemcpyHostToDevice, s));
             A.mips = m;
         }
     }
@@ -1654,12 +1657,19 @@ static int forward_warp_dev(twx_handle* h, DevPool& pool, const void* src, int W
     A.total = pool.alloc<uint64_t>(1);
     HIPCHECK(hipMemsetAsync(A.counts, 0, n * sizeof(uint32_t), s));
     HIPCHECK(hipMemsetAsync(A.fill, 0, n * sizeof(uint32_t), s));
+    // the count pass's flags (1, 2, 4) in a word of this warp's own, so that a warp that turns out
+    // still leaves nothing in the caller's flags; the emit and fold passes report into dflags
+    int* cflags = pool.alloc<int>(1);
+    launch_zero(s, cflags, 1);
+    A.flags = cflags;
     launch_fw_count(s, A);
     uint64_t* ht = pool.pinned<uint64_t>(1);
     int* hf = pool.pinned<int>(1);
     HIPCHECK(hipMemcpyAsync(ht, A.total, sizeof(uint64_t), hipMemcpyDeviceToHost, s));
-    HIPCHECK(hipMemcpyAsync(hf, dflags, sizeof(int), hipMemcpyDeviceToHost, s));
+    HIPCHECK(hipMemcpyAsync(hf, cflags, sizeof(int), hipMemcpyDeviceToHost, s));
     HIPCHECK(hipStreamSynchronize(s));
+    if (hm && *hm == 0) return copy_still();
+    A.flags = dflags;
     if (*hf & 1) {
         err = "forward warp: the degenerate last row pass of the rasteriser would write outside the image, which is not "
               "supported";

@@ -750,18 +750,37 @@ __global__ void __launch_bounds__(FW_SCAN_THREADS) k_fw_scan_blocks(const uint32
                                                                     uint64_t* __restrict__ block_totals, long n) {
     __shared__ uint64_t sh[FW_SCAN_THREADS];
     const long base = (long)blockIdx.x * FW_SCAN_ITEMS + (long)threadIdx.x * FW_SCAN_PER_THREAD;
+    // a lane's eight counts as two 16-byte loads and its eight offsets as four 16-byte stores where
+    // the run lies inside the array (both arrays start 256-byte aligned), element by element at the end
+    const bool whole = base + FW_SCAN_PER_THREAD <= n;
     uint64_t v[FW_SCAN_PER_THREAD], s = 0;
+    if (whole) {
+        const uint4 c0 = *reinterpret_cast<const uint4*>(counts + base);
+        const uint4 c1 = *reinterpret_cast<const uint4*>(counts + base + 4);
+        v[0] = c0.x; v[1] = c0.y; v[2] = c0.z; v[3] = c0.w;
+        v[4] = c1.x; v[5] = c1.y; v[6] = c1.z; v[7] = c1.w;
+    } else {
 #pragma unroll
-    for (int k = 0; k < FW_SCAN_PER_THREAD; k++) {
-        v[k] = base + k < n ? counts[base + k] : 0;
-        s += v[k];
+        for (int k = 0; k < FW_SCAN_PER_THREAD; k++) v[k] = base + k < n ? counts[base + k] : 0;
     }
+#pragma unroll
+    for (int k = 0; k < FW_SCAN_PER_THREAD; k++) s += v[k];
     uint64_t total;
     uint64_t run = fw_block_exclusive(s, sh, total);
+    uint64_t o[FW_SCAN_PER_THREAD];
 #pragma unroll
     for (int k = 0; k < FW_SCAN_PER_THREAD; k++) {
-        if (base + k < n) offsets[base + k] = run;
+        o[k] = run;
         run += v[k];
+    }
+    if (whole) {
+#pragma unroll
+        for (int k = 0; k < FW_SCAN_PER_THREAD; k += 2)
+            *reinterpret_cast<ulonglong2*>(offsets + base + k) = make_ulonglong2(o[k], o[k + 1]);
+    } else {
+#pragma unroll
+        for (int k = 0; k < FW_SCAN_PER_THREAD; k++)
+            if (base + k < n) offsets[base + k] = o[k];
     }
     if (threadIdx.x == 0) block_totals[blockIdx.x] = total;
 }
@@ -799,22 +818,34 @@ namespace {
 
 // forward.rs:183-195 on a device flow: sets *flag when a component moves by more than 1e-4 (a
 // NaN does not: it fails the comparison, as on the CPU)
+// Four 16-byte loads in flight per lane, the last n % 4 components one by one; one atomic per
+// block that saw a moving component (a few hundred blocks: every wave's atomic on the one word cost
+// about 12 ns each, serialised).
+__device__ __forceinline__ bool fw_moves(float v) {
+    const float a = v < 0.0f ? -v : v;
+    return 0.0001 < (double)a;
+}
 __global__ void k_fw_flow_moves(const float* __restrict__ f, long n, int* flag) {
     bool moves = false;
-    const long stride = (long)gridDim.x * blockDim.x;
-    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-        const float v = f[i];
-        const float a = v < 0.0f ? -v : v;
-        moves |= 0.0001 < (double)a;
+    // a flow that is not 16-byte aligned takes the element loop below for all of it
+    const long n4 = (reinterpret_cast<uintptr_t>(f) & 15) ? 0 : n / 4, stride = (long)gridDim.x * blockDim.x;
+    const float4* f4 = reinterpret_cast<const float4*>(f);
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n4; i += 4 * stride) {
+        float4 v[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) v[u] = i + u * stride < n4 ? f4[i + u * stride] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+#pragma unroll
+        for (int u = 0; u < 4; u++) moves |= fw_moves(v[u].x) || fw_moves(v[u].y) || fw_moves(v[u].z) || fw_moves(v[u].w);
     }
-    if (__any(moves) && (threadIdx.x & 63) == 0) atomicOr(flag, 1);
+    for (long j = n4 * 4 + (long)blockIdx.x * blockDim.x + threadIdx.x; j < n; j += stride) moves |= fw_moves(f[j]);
+    if (__syncthreads_or(moves) && threadIdx.x == 0) atomicOr(flag, 1);
 }
 
 }  // namespace
 
 void launch_fw_flow_moves(hipStream_t s, const float* flow, long n, int* flag) {
-    const unsigned blocks = (unsigned)std::min<long>((n + 255) / 256, 2048);
-    TWX_LAUNCH(s, F_FW_MESH, 4.0 * (double)n, k_fw_flow_moves<<<blocks, 256, 0, s>>>(flow, n, flag));
+    const unsigned blocks = (unsigned)std::max<long>(1, std::min<long>((n / 4 + 4095) / 4096, 256));
+    TWX_LAUNCH(s, F_FW_MESH, 4.0 * (double)n, k_fw_flow_moves<<<blocks, 1024, 0, s>>>(flow, n, flag));
 }
 
 size_t fw_scan_blocks(long n) { return (size_t)((n + FW_SCAN_ITEMS - 1) / FW_SCAN_ITEMS); }
