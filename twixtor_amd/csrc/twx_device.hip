@@ -2192,8 +2192,8 @@ __device__ __forceinline__ void load4(const void* img, uint32_t i, float o[4]) {
 // sampled bilinearly at (x * rx, y * ry) (Flow::bilerp, warp.rs:415-442), scaled by amount / r.
 // rx = (fw - 1) / (W - 1), ry = (fh - 1) / (H - 1) and sx = amount / rx, sy = amount / ry are the
 // same for every pixel of a frame: the host computes them once (frame_job_ratios).
-__device__ __forceinline__ void warp_point(const float* __restrict__ flow, int fw, int fh, float rx, float ry, float sx,
-                                           float sy, int x, int y, float& ix, float& iy, bool raw) {
+__device__ __forceinline__ void flow_sample(const float* __restrict__ flow, int fw, int fh, float rx, float ry, int x, int y,
+                                            float d[2]) {
     float xf = (float)x, yf = (float)y;
     float qx = xf * rx, qy = yf * ry;
     int fw1 = fw - 1, fh1 = fh - 1;
@@ -2208,7 +2208,6 @@ __device__ __forceinline__ void warp_point(const float* __restrict__ flow, int f
     float t0[4], t1[4];  // the two flow pixels of the cell's top row, then of its bottom row
     ldrun<4>(gf, 2 * q00, t0);
     ldrun<4>(gf, 2 * (q00 + fw), t1);
-    float d[2];
 #pragma unroll
     for (int k = 0; k < 2; k++) {
         float p00 = t0[k], p10 = t0[2 + k], p01 = t1[k], p11 = t1[2 + k];
@@ -2216,11 +2215,22 @@ __device__ __forceinline__ void warp_point(const float* __restrict__ flow, int f
         float bot = (p11 - p01) * gx + p01;
         d[k] = (bot - top) * gy + top;
     }
-    // raw: the flow holds v where the reference reads -1 * v. Negation is exact, so the reference's
-    // sample of the negated flow is -d except that an exactly zero result may carry the other sign,
-    // which vanishes in the sum with the pixel position; xf - d * sx is xf + (-d) * sx bit for bit.
+}
+// The sample point of output (x, y) from the flow sample d, scaled by (sx, sy). raw: the flow
+// holds v where the reference reads -1 * v. Negation is exact, so the reference's sample of the
+// negated flow is -d except that an exactly zero result may carry the other sign, which vanishes in
+// the sum with the pixel position; xf - d * sx is xf + (-d) * sx bit for bit.
+__device__ __forceinline__ void warp_point_at(const float d[2], float sx, float sy, int x, int y, float& ix, float& iy,
+                                              bool raw) {
+    const float xf = (float)x, yf = (float)y;
     ix = raw ? xf - d[0] * sx : d[0] * sx + xf;
     iy = raw ? yf - d[1] * sy : d[1] * sy + yf;
+}
+__device__ __forceinline__ void warp_point(const float* __restrict__ flow, int fw, int fh, float rx, float ry, float sx,
+                                           float sy, int x, int y, float& ix, float& iy, bool raw) {
+    float d[2];
+    flow_sample(flow, fw, fh, rx, ry, x, y, d);
+    warp_point_at(d, sx, sy, x, y, ix, iy, raw);
 }
 
 // bilerp4 (warp.rs:306-331) at a point inside [0, W-1] x [0, H-1], before the store rule, in two
@@ -2289,10 +2299,11 @@ __device__ __forceinline__ bool warp_pixel(const void* src, int W, int H, const 
 // the image's first cell and gives zeros), so a pixel waits for two memory round trips, not four.
 // Returns false when either point is refused.
 template <int FMT>
-__device__ __forceinline__ bool warp_pixel_pair(const FrameJob& F, int x, int y, float pa[4], float pb[4]) {
+__device__ __forceinline__ bool warp_pixel_pair_at(const FrameJob& F, const float dba[2], const float dab[2], int x, int y,
+                                                   float pa[4], float pb[4]) {
     float ixa, iya, ixb, iyb;
-    warp_point(F.f_ba, F.fw, F.fh, F.rx, F.ry, F.sx[0], F.sy[0], x, y, ixa, iya, F.flow_raw != 0);
-    warp_point(F.f_ab, F.fw, F.fh, F.rx, F.ry, F.sx[1], F.sy[1], x, y, ixb, iyb, F.flow_raw != 0);
+    warp_point_at(dba, F.sx[0], F.sy[0], x, y, ixa, iya, F.flow_raw != 0);
+    warp_point_at(dab, F.sx[1], F.sy[1], x, y, ixb, iyb, F.flow_raw != 0);
     const float w1 = (float)(F.W - 1), h1 = (float)(F.H - 1);
     const bool oka = clip_to_rect((float)x, (float)y, ixa, iya, w1, h1) && !isnan(ixa) && !isnan(iya);
     const bool okb = clip_to_rect((float)x, (float)y, ixb, iyb, w1, h1) && !isnan(ixb) && !isnan(iyb);
@@ -2312,6 +2323,13 @@ __device__ __forceinline__ bool warp_pixel_pair(const FrameJob& F, int x, int y,
         pb[k] = okb ? store_rule<FMT>(sb[k]) : 0.0f;
     }
     return oka && okb;
+}
+template <int FMT>
+__device__ __forceinline__ bool warp_pixel_pair(const FrameJob& F, int x, int y, float pa[4], float pb[4]) {
+    float dba[2], dab[2];
+    flow_sample(F.f_ba, F.fw, F.fh, F.rx, F.ry, x, y, dba);
+    flow_sample(F.f_ab, F.fw, F.fh, F.rx, F.ry, x, y, dab);
+    return warp_pixel_pair_at<FMT>(F, dba, dab, x, y, pa, pb);
 }
 
 // The store of a faded edge sample (warp.rs:237-295): the stored sample times f,
@@ -2575,27 +2593,52 @@ __global__ void k_frames(const FrameJob* __restrict__ jobs) {
 }
 
 // The common vector-path frame (kind 2: Blend of two inverse warps, neither an early-out copy,
-// no pushed frames) on its own: the same functions as k_frames' branch for it, in a kernel a
-// fraction of k_frames' size (whose other kinds and their NaN-rule paths make a large body).
+// no pushed frames) in a kernel of its own, a fraction of k_frames' size (whose other kinds and
+// their NaN-rule paths make a large body), with the same functions as k_frames' branch for it.
+// The frames of one source pair go together (same_pair: the same frames, flows, size and format;
+// the weights differ): a workgroup takes the pixel run of every frame of its group, the flows
+// sampled once for all of them, and the later frames' texels come from the cache lines the first
+// one brought in.
+constexpr int FG_MAX = 384;
+struct FrameGroups {
+    int n;
+    int first[FG_MAX];
+    int count[FG_MAX];
+};
 
 template <int FMT>
-__global__ void k_frames_blend(const FrameJob* __restrict__ jobs) {
-    const FrameJob& F = jobs[blockIdx.z];
-    if (F.fmt != FMT) return;
+__global__ void k_frames_blend_group(const FrameJob* __restrict__ jobs, FrameGroups G) {
+    const int j0 = G.first[blockIdx.z], nj = G.count[blockIdx.z];
+    const FrameJob& F0 = jobs[j0];
+    if (F0.fmt != FMT) return;
     const int x = blockIdx.x * blockDim.x + threadIdx.x;
     const int y = blockIdx.y;  // motion row
-    if (x >= F.W || y >= F.H) return;
-    const long n = (long)F.W * F.H;
-    float pa[4], pb[4], e[4];
-    const bool ok = warp_pixel_pair<FMT>(F, x, y, pa, pb);
-    blend_pixel<FMT>(pa, pb, F.blend_mode, F.weight, F.lerp_w, e);
-    if (!ok) atomicOr(F.bad, 1);
-    if (!F.finish) {
-        typename Px<FMT>::T* o = reinterpret_cast<typename Px<FMT>::T*>(F.out) + ((long)y * F.W + x) * 4;
-        for (int k = 0; k < 4; k++) o[k] = (typename Px<FMT>::T)e[k];
-        return;
+    if (x >= F0.W || y >= F0.H) return;
+    const long n = (long)F0.W * F0.H;
+    float dba[2], dab[2];
+    flow_sample(F0.f_ba, F0.fw, F0.fh, F0.rx, F0.ry, x, y, dba);
+    flow_sample(F0.f_ab, F0.fw, F0.fh, F0.rx, F0.ry, x, y, dab);
+    for (int j = 0; j < nj; j++) {
+        const FrameJob& F = jobs[j0 + j];
+        float pa[4], pb[4], e[4];
+        const bool ok = warp_pixel_pair_at<FMT>(F, dba, dab, x, y, pa, pb);
+        blend_pixel<FMT>(pa, pb, F.blend_mode, F.weight, F.lerp_w, e);
+        if (!ok) atomicOr(F.bad, 1);
+        if (!F.finish) {
+            typename Px<FMT>::T* o = reinterpret_cast<typename Px<FMT>::T*>(F.out) + ((long)y * F.W + x) * 4;
+            for (int k = 0; k < 4; k++) o[k] = (typename Px<FMT>::T)e[k];
+        } else {
+            finish_pixel<FMT>(F.out, n, (long)(F.finish == 2 ? y : F.H - 1 - y) * F.W + x, e);
+        }
     }
-    finish_pixel<FMT>(F.out, n, (long)(F.finish == 2 ? y : F.H - 1 - y) * F.W + x, e);
+}
+
+// Jobs k and k + 1 blend the same pair the same way but for their weights: the same source frames,
+// flows, sizes, format, flow convention and finish.
+static bool same_pair(const FrameJob& a, const FrameJob& b) {
+    return a.a == b.a && a.b == b.b && a.f_ab == b.f_ab && a.f_ba == b.f_ba && a.W == b.W && a.H == b.H &&
+           a.fw == b.fw && a.fh == b.fh && a.fmt == b.fmt && a.flow_raw == b.flow_raw && a.finish == b.finish &&
+           a.rx == b.rx && a.ry == b.ry;
 }
 
 void frame_job_ratios(FrameJob& F) {
@@ -2623,7 +2666,7 @@ void launch_frames(hipStream_t s, const FrameJob* d_jobs, const std::vector<Fram
 static void launch_frames_run(hipStream_t s, const FrameJob* d_jobs, const std::vector<FrameJob>& jobs, bool pair);
 
 static void launch_frames_range(hipStream_t s, const FrameJob* d_jobs, const std::vector<FrameJob>& jobs) {
-    // runs of consecutive jobs: the blend pairs through k_frames_blend, the others through k_frames
+    // runs of consecutive jobs: the blend pairs through k_frames_blend_group, the others through k_frames
     for (size_t r0 = 0; r0 < jobs.size();) {
         const bool pair = frame_is_blend_pair(jobs[r0]);
         size_t r1 = r0 + 1;
@@ -2653,9 +2696,32 @@ static void launch_frames_run(hipStream_t s, const FrameJob* d_jobs, const std::
         bytes += imgs * px * bpp + masks * px + flows;
     }
     if (pair) {
-        if (has[0]) TWX_LAUNCH(s, F_FRAMES, bytes, k_frames_blend<0><<<grd, blk, 0, s>>>(d_jobs));
-        if (has[1]) TWX_LAUNCH(s, F_FRAMES, bytes, k_frames_blend<1><<<grd, blk, 0, s>>>(d_jobs));
-        if (has[2]) TWX_LAUNCH(s, F_FRAMES, bytes, k_frames_blend<2><<<grd, blk, 0, s>>>(d_jobs));
+        // consecutive jobs of one pair as a group, up to FG_MAX groups a launch (the table travels in
+        // the kernel's arguments)
+        FrameGroups G{};
+        double gbytes = 0.0;
+        auto flush = [&]() {
+            if (G.n == 0) return;
+            const dim3 gg((mw + 255) / 256, mh, (unsigned)G.n);
+            if (has[0]) TWX_LAUNCH(s, F_FRAMES, gbytes, k_frames_blend_group<0><<<gg, blk, 0, s>>>(d_jobs, G));
+            if (has[1]) TWX_LAUNCH(s, F_FRAMES, gbytes, k_frames_blend_group<1><<<gg, blk, 0, s>>>(d_jobs, G));
+            if (has[2]) TWX_LAUNCH(s, F_FRAMES, gbytes, k_frames_blend_group<2><<<gg, blk, 0, s>>>(d_jobs, G));
+            G.n = 0;
+            gbytes = 0.0;
+        };
+        for (size_t k = 0; k < jobs.size();) {
+            size_t e = k + 1;
+            while (e < jobs.size() && same_pair(jobs[k], jobs[e])) e++;
+            if (G.n == FG_MAX) flush();
+            G.first[G.n] = (int)k;
+            G.count[G.n] = (int)(e - k);
+            G.n++;
+            for (size_t i = k; i < e; i++)
+                gbytes += 3.0 * (double)jobs[i].W * jobs[i].H * 4.0 * sample_size(jobs[i].fmt) + 16.0 * jobs[i].fw * jobs[i].fh;
+            k = e;
+        }
+        flush();
+        (void)bytes;
         return;
     }
     if (has[0]) TWX_LAUNCH(s, F_FRAMES, bytes, k_frames<0><<<grd, blk, 0, s>>>(d_jobs));
