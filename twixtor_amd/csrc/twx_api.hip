@@ -358,7 +358,7 @@ struct PyrIn {
 void build_pyramids(hipStream_t s, DevPool& pool, const std::vector<PyrIn>& in) {
     if (in.empty()) return;
     const Taps bin = binomial_taps();
-    const size_t chunk = 64;
+    const size_t chunk = 128;  // pyramids per launch sequence (the small levels' launches are latency)
     // per frame scratch: the planes of steps 1, 3, ... in B (half size) and of steps 2, 4, ... in A
     // (quarter size); one set for the largest chunk, reused by every chunk in stream order (step 0
     // reads the engine image, and the blurred planes stay in LDS)
