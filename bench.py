@@ -27,13 +27,13 @@ SPEED = "speed_percent=100@0;25@60;100@120"
 
 # Measured HBM bytes over algorithmic bytes per kernel family, from the FETCH_SIZE / WRITE_SIZE
 # counter runs (copy-calibrated) recorded in profiles/retime_1080p_hbm_traffic_pmc.md.
-TRAFFIC_RATIO = {"frames": 0.77, "new_disp_gauss": 1.02, "calc_b_gauss": 1.04, "bm_cols": 1.23, "bm_rows": 1.31,
+TRAFFIC_RATIO = {"frames": 0.77, "new_disp_gauss": 1.04, "calc_b_gauss": 1.03, "bm_cols": 1.23, "bm_rows": 1.30,
                  "ingest": 1.00, "pyr_luma_blur5_band_resample": 1.08, "bm_transpose": 1.39, "upsample": 1.27,
                  "tensor_gauss": 1.00}
 # The share of the SIMDs' issue slots the family's VALU instructions fill over its time, from the
 # SQ_INSTS_VALU counter run recorded in the same profile.
-VALU_SHARE = {"calc_b_gauss": 0.72, "frames": 0.61, "bm_rows": 0.55, "bm_cols": 0.50, "new_disp_gauss": 0.59,
-              "pyr_luma_blur5_band_resample": 0.43, "tensor_gauss": 0.65, "bm_transpose": 0.19, "upsample": 0.37,
+VALU_SHARE = {"calc_b_gauss": 0.73, "frames": 0.61, "bm_rows": 0.55, "bm_cols": 0.50, "new_disp_gauss": 0.58,
+              "pyr_luma_blur5_band_resample": 0.46, "tensor_gauss": 0.67, "bm_transpose": 0.19, "upsample": 0.37,
               "ingest": 0.06}
 # What bounds a family where neither the HBM fraction nor the VALU share says it: the block-matching
 # sweeps are serial chains, each step's decision feeding the next step's texel gathers.
